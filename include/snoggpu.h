@@ -1,0 +1,292 @@
+/*
+ * snoggpu.h -- C ABI of the MI355X (gfx950) path-tracing core.
+ *
+ * Plain C types only: pointers, sizes, fixed-width integers and floats.
+ * Every call returns an int status (SGR_OK == 0); on failure the message
+ * is available from sgr_last_error() (thread-local, valid until the next
+ * failing call on the same thread).
+ *
+ * The entry points replace, one for one, the reference's CPU render path:
+ *
+ *   sgr_scene_create      Scene::Scene + SpaceBuilderFactory::make_space
+ *                         (render/scene.cc:26-44, space/space-builder.h:86-93,
+ *                          space/octree-builder.cc:174-677) and the light
+ *                          sampler set-up (render/scene.cc:34-44)
+ *   sgr_trace_closest     Space::intersect (space/space.cc:55-66)
+ *   sgr_trace_occluded    Space::occludes (space/space.cc:182-194)
+ *   sgr_render            RenderMgr::render (render-mgr/render-mgr.cc:49-74)
+ *                         driving Renderer::render_packet
+ *                         (render-mgr/renderer.cc:41-100), PathInteg::Li
+ *                         (render/path-integ.cc:133-407) and the
+ *                         ImageSampledOutput filtered splat
+ *                         (image/image-filter-conv.h:109-186)
+ *   sgr_finalize_image    ImageSampledOutput::set_raw_min_y
+ *                         (image/image-sampled-output.cc:63-126)
+ *   sgr_scene_octree_stats Octree::stats (space/octree.cc:308-367)
+ *
+ * Threading: one sgr_ctx per host thread; calls on one context are
+ * serialised by the caller.  Scenes are immutable after creation and may
+ * be shared by renders on the same context.
+ */
+#ifndef SNOGGPU_H
+#define SNOGGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGR_OK            0
+#define SGR_ERR_INVALID   1   /* bad argument or unsupported scene feature */
+#define SGR_ERR_HIP       2   /* HIP runtime error */
+#define SGR_ERR_NOMEM     3   /* device allocation failed / over budget */
+#define SGR_ERR_LIMIT     4   /* a documented size limit was exceeded */
+#define SGR_ERR_NOTREADY  5   /* job not finished yet (sgr_job_poll) */
+
+/* Material kinds (material/lambert.cc, material/cook-torrance.cc,
+ * material/glass.cc, material/mirror.cc, material/glow.cc). */
+#define SGR_MAT_LAMBERT        1
+#define SGR_MAT_COOK_TORRANCE  2
+#define SGR_MAT_GLASS          3
+#define SGR_MAT_MIRROR         4
+#define SGR_MAT_GLOW           5
+
+/* Acceleration structure (render/global-render-state.cc:63-74 "accel"). */
+#define SGR_ACCEL_OCTREE  0
+#define SGR_ACCEL_TRIV    1   /* brute force, TrivSpace (space/triv-space.h) */
+
+typedef struct sgr_ctx sgr_ctx;
+typedef struct sgr_scene sgr_scene;
+
+/* One material.  Field meaning per kind:
+ *   LAMBERT        color = reflectance
+ *   COOK_TORRANCE  color = diffuse colour, gloss_color = specular colour,
+ *                  m = Ward roughness, ior_n/ior_k = complex IOR
+ *   GLASS          ior_n = IOR of the enclosed medium (no absorption)
+ *   MIRROR         gloss_color = reflectance, ior_n/ior_k = IOR,
+ *                  color = underlying Lambert colour (all zero: none)
+ *   GLOW           color = emitted radiance; underlying = index of the
+ *                  material supplying the BSDF, or -1 for none           */
+typedef struct sgr_material {
+    int32_t type;
+    float   color[3];
+    float   gloss_color[3];
+    float   m;
+    float   ior_n;
+    float   ior_k;
+    int32_t underlying;
+} sgr_material;
+
+/* A triangle mesh (surface/mesh.h).  Vertices are float xyz; triangles
+ * are vertex-index triples.  tri_materials may be NULL, then every
+ * triangle uses `material`.  axis is the mesh axis used for tangent
+ * frames (surface/mesh.cc:563-600); left_handed selects the normal
+ * convention (surface/mesh.h:267-271). */
+typedef struct sgr_mesh {
+    const float    *vertices;
+    uint32_t        num_vertices;
+    const uint32_t *triangles;
+    uint32_t        num_triangles;
+    const uint32_t *tri_materials;
+    uint32_t        material;
+    float           axis[3];
+    int32_t         left_handed;
+} sgr_mesh;
+
+/* Triangle or parallelogram primitive (surface/tripar.h). */
+typedef struct sgr_tripar {
+    float    v0[3];
+    float    e1[3];
+    float    e2[3];
+    int32_t  parallelogram;
+    uint32_t material;
+} sgr_tripar;
+
+/* Sphere primitive (surface/sphere.h). */
+typedef struct sgr_sphere {
+    float    center[3];
+    float    radius;
+    uint32_t material;
+} sgr_sphere;
+
+/* Lat-long environment map light (light/envmap-light.cc,
+ * texture/spheremap.h).  texels: RGB float, row-major, row 0 = top
+ * (north).  frame: 3x3 row-major, rows are the light frame's x, y, z
+ * axes in world space (identity = map z axis is world z). */
+typedef struct sgr_envmap {
+    const float *texels;
+    uint32_t     width;
+    uint32_t     height;
+    float        frame[9];
+} sgr_envmap;
+
+/* Whole scene.  Surfaces are inserted into the acceleration structure in
+ * this order: meshes (triangle order), then tripars, then spheres.  The
+ * global primitive id of a hit is its index in that order.  Light
+ * samplers are created for GLOW tripars and spheres (insertion order),
+ * then for each envmap. */
+typedef struct sgr_scene_desc {
+    const sgr_mesh     *meshes;
+    uint32_t            num_meshes;
+    const sgr_tripar   *tripars;
+    uint32_t            num_tripars;
+    const sgr_sphere   *spheres;
+    uint32_t            num_spheres;
+    const sgr_material *materials;
+    uint32_t            num_materials;
+    const sgr_envmap   *envmaps;
+    uint32_t            num_envmaps;
+    float               min_trace;   /* render/render-params.h:32-36 */
+    int32_t             accel;       /* SGR_ACCEL_* */
+} sgr_scene_desc;
+
+/* Rays and hits, structure-of-arrays (geometry/ray.h:84-89).  Pointers
+ * are host memory unless the call says otherwise. */
+typedef struct sgr_rays {
+    const float *ox, *oy, *oz;
+    const float *dx, *dy, *dz;
+    const float *t0, *t1;
+} sgr_rays;
+
+typedef struct sgr_hits {
+    uint32_t *prim;   /* 0xffffffff = no hit */
+    float    *t;
+    float    *u;
+    float    *v;
+} sgr_hits;
+
+/* Pinhole camera (camera/camera.h:322-389): eye_vec = forward
+ * + right*(2x-1)*tan_half_fov_x + up*(2y-1)*tan_half_fov_y, not
+ * normalised. */
+typedef struct sgr_camera {
+    float pos[3];
+    float forward[3];
+    float up[3];
+    float right[3];
+    float tan_half_fov_x;
+    float tan_half_fov_y;
+} sgr_camera;
+
+/* Render parameters (render/path-integ.cc:61-91, image/image-filter.cc,
+ * image/image-mitchell-filt.h:31-33, image/image-filter-conv.h:35). */
+typedef struct sgr_render_params {
+    uint32_t width, height;        /* full image size */
+    uint32_t spp;                  /* "samples" */
+    uint32_t min_path_len;         /* default 3 */
+    uint32_t max_path_len;         /* default 25 */
+    uint32_t direct_samples;       /* default 1; 0 disables direct light */
+    uint64_t seed;                 /* job seed for the counter RNG */
+    float    filter_b, filter_c;   /* Mitchell B, C (1/3, 1/3) */
+    float    filter_radius;        /* 2 */
+    float    neg_clamp;            /* -0.1 */
+    float    bg_alpha;             /* "background_alpha", 1 */
+    /* Rendered window [x0, x0+w) x [y0, y0+h) of the image; the filter
+     * margin (ceil(radius - 0.5001) pixels, snogray.lua:546-550) is added
+     * around it by the renderer. w == 0 means the whole image. */
+    uint32_t x0, y0, w, h;
+    /* Paths kept in flight per wavefront step (0: choose). */
+    uint32_t paths_in_flight;
+} sgr_render_params;
+
+/* Counters named after RenderStats::IsecStats (render/render-stats.h:30-52)
+ * plus device timings. */
+typedef struct sgr_stats {
+    uint64_t eye_rays;
+    uint64_t scene_intersect_calls;       /* closest-hit rays */
+    uint64_t shadow_tests;                /* any-hit rays */
+    uint64_t node_visits_closest;         /* octree nodes entered */
+    uint64_t surf_tests_closest;          /* primitive tests */
+    uint64_t node_visits_shadow;
+    uint64_t surf_tests_shadow;
+    uint64_t illum_calls;
+    uint64_t samples;                     /* nominal W*H*spp */
+    uint64_t paths_traced;                /* incl. filter margin */
+    double   ms_render;                   /* device time of the render */
+    double   ms_extend;                   /* closest-hit kernels */
+    double   ms_shade;
+    double   ms_shadow;                   /* any-hit kernels */
+    double   ms_splat;
+    double   ms_other;
+    uint32_t extend_launches;
+    uint32_t shadow_launches;
+} sgr_stats;
+
+/* Octree statistics (space/octree.h:53-67). */
+typedef struct sgr_octree_stats {
+    uint64_t num_nodes;
+    uint64_t num_leaf_nodes;
+    uint64_t num_surfaces;
+    uint64_t num_dup_surfaces;
+    uint32_t max_depth;
+    float    avg_depth;
+} sgr_octree_stats;
+
+/* ---- context ---------------------------------------------------------- */
+const char *sgr_last_error(void);
+int sgr_ctx_create(int device, sgr_ctx **out);
+int sgr_ctx_destroy(sgr_ctx *ctx);
+/* Device memory currently held by the context (scenes + buffers). */
+int sgr_ctx_memory(sgr_ctx *ctx, uint64_t *bytes);
+
+/* ---- scenes ----------------------------------------------------------- */
+int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *desc,
+                     sgr_scene **out);
+int sgr_scene_destroy(sgr_scene *scene);
+int sgr_scene_octree_stats(const sgr_scene *scene, sgr_octree_stats *out);
+/* Flattened octree, for parity tests: nodes are 4 x uint32 (first child,
+ * child mask, first surface-list entry, surface count) in breadth-first
+ * order; surf_ids are global primitive ids.  Pass NULL buffers to query
+ * the sizes. */
+int sgr_octree_export(const sgr_scene *scene, uint32_t *nodes,
+                      uint64_t *num_nodes, uint32_t *surf_ids,
+                      uint64_t *num_surf_ids, float *origin_size4);
+/* Host-only octree build of a descriptor (no device needed); same
+ * output as sgr_octree_export plus Octree::stats.  Any pointer may be
+ * NULL.  Replaces Octree::Builder::add + make_space
+ * (space/octree-builder.cc:174-677) for inspection. */
+int sgr_octree_build(const sgr_scene_desc *desc, uint32_t *nodes,
+                     uint64_t *num_nodes, uint32_t *surf_ids,
+                     uint64_t *num_surf_ids, float *origin_size4,
+                     sgr_octree_stats *stats);
+/* Horizon (scene bbox diameter, render/scene.cc:28) and scene bbox. */
+int sgr_scene_bounds(const sgr_scene *scene, float *bbox_min3,
+                     float *bbox_max3, float *horizon);
+/* Device memory footprint of the scene in bytes. */
+int sgr_scene_footprint(const sgr_scene *scene, uint64_t *bytes);
+
+/* ---- ray batches (parity entry points) -------------------------------- */
+int sgr_trace_closest(sgr_scene *scene, const sgr_rays *rays, size_t n,
+                      sgr_hits *hits, sgr_stats *stats);
+/* occluded[i] = 1 if some surface blocks ray i within (t0, t1). */
+int sgr_trace_occluded(sgr_scene *scene, const sgr_rays *rays, size_t n,
+                       uint8_t *occluded, sgr_stats *stats);
+
+/* ---- rendering -------------------------------------------------------- */
+/* Render into caller-owned accumulators: accum has 5 floats per image
+ * pixel (premultiplied r, g, b, alpha, then filter weight), row-major,
+ * width*height*5.  accum_on_device != 0: accum is a device pointer of the
+ * context's device; else host memory.  Accumulators are ADDED to (clear
+ * them first for a fresh image). */
+int sgr_render(sgr_scene *scene, const sgr_camera *cam,
+               const sgr_render_params *params, float *accum,
+               int accum_on_device, sgr_stats *stats);
+/* Per-sample output for parity tests: for every rendered path (window
+ * plus margin, row-major over pixels, then sample index) writes
+ * sx, sy, r, g, b, alpha (6 floats).  out must hold
+ * (w+2m)*(h+2m)*spp*6 floats (host memory). */
+int sgr_render_samples(sgr_scene *scene, const sgr_camera *cam,
+                       const sgr_render_params *params, float *out,
+                       sgr_stats *stats);
+/* Normalise accumulators into an RGBA image (4 floats per pixel), as
+ * ImageSampledOutput::set_raw_min_y does: divide by weight, clamp alpha,
+ * clamp colour at 0, un-premultiply, exposure, contrast. */
+int sgr_finalize_image(const float *accum, uint32_t width, uint32_t height,
+                       float exposure, float contrast, float *rgba);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SNOGGPU_H */

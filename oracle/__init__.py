@@ -1,0 +1,190 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes front end of liboracle.so, the CPU restatement of the reference's
+path-tracing render path.  Only tests/, the smoke check and the CPU
+baseline leg of bench.py may import this package; the product never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from snogray_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_scene_create.restype = C.c_void_p
+        L.orc_scene_create.argtypes = [C.POINTER(abi.SceneDescC), C.c_int,
+                                       C.c_char_p, C.c_int]
+        L.orc_scene_destroy.argtypes = [C.c_void_p]
+        L.orc_octree_export.argtypes = [C.c_void_p, abi.u32p,
+                                        C.POINTER(C.c_uint64), abi.u32p,
+                                        C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_double)]
+        L.orc_octree_stats.argtypes = [C.c_void_p, C.POINTER(abi.OctreeStats)]
+        L.orc_scene_bounds.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
+        L.orc_trace.argtypes = [C.c_void_p, C.POINTER(abi.Rays), C.c_size_t,
+                                C.POINTER(abi.Hits), C.POINTER(C.c_uint8),
+                                C.c_int, C.POINTER(C.c_uint64)]
+        L.orc_render.argtypes = [C.c_void_p, C.POINTER(abi.Camera),
+                                 C.POINTER(abi.RenderParams),
+                                 C.POINTER(C.c_double), abi.f32p, C.c_int,
+                                 C.POINTER(abi.Stats), C.c_char_p, C.c_int]
+        L.orc_bsdf_probe.argtypes = [C.POINTER(abi.Material)] + \
+            [C.POINTER(C.c_double)] * 2 + [C.c_int, C.c_double, C.c_double,
+                                           C.POINTER(C.c_double), C.c_int,
+                                           C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double), C.c_int,
+                                           C.POINTER(C.c_double), C.c_int,
+                                           C.c_int]
+        L.orc_fresnel.restype = C.c_double
+        L.orc_fresnel.argtypes = [C.c_double] * 4
+        L.orc_mitchell.restype = C.c_double
+        L.orc_mitchell.argtypes = [C.c_double] * 4
+        L.orc_envmap_probe.argtypes = [abi.f32p, C.c_uint32, C.c_uint32,
+                                       C.POINTER(C.c_double), C.c_int,
+                                       C.POINTER(C.c_double)]
+        L.orc_stream_probe.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_uint32,
+                                       C.c_uint32, abi.f32p, C.c_uint32,
+                                       C.c_uint32, abi.f32p]
+        L.orc_permute.restype = C.c_uint32
+        L.orc_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
+        _lib = L
+    return _lib
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleScene:
+    """CPU-oracle scene (fp32 reproduces the reference arithmetic, fp64 is
+    the high-precision oracle)."""
+
+    def __init__(self, desc, fp64=False):
+        err = C.create_string_buffer(512)
+        self.desc = desc
+        self.h = lib().orc_scene_create(C.byref(desc.c()), 1 if fp64 else 0,
+                                        err, 512)
+        if not self.h:
+            raise RuntimeError("oracle scene: " + err.value.decode())
+        self.fp64 = fp64
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def octree(self):
+        nn, ns = C.c_uint64(), C.c_uint64()
+        lib().orc_octree_export(self.h, None, C.byref(nn), None, C.byref(ns), None)
+        nodes = np.zeros((nn.value, 4), np.uint32)
+        surfs = np.zeros(ns.value, np.uint32)
+        osz = np.zeros(4, np.float64)
+        lib().orc_octree_export(self.h, abi.uptr(nodes), C.byref(nn),
+                                abi.uptr(surfs), C.byref(ns), dptr(osz))
+        return nodes, surfs, osz
+
+    def octree_stats(self):
+        st = abi.OctreeStats()
+        lib().orc_octree_stats(self.h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def bounds(self):
+        a, b, h = np.zeros(3), np.zeros(3), C.c_double()
+        lib().orc_scene_bounds(self.h, dptr(a), dptr(b), C.byref(h))
+        return a, b, h.value
+
+    def trace_closest(self, batch, nthreads=8):
+        hb = abi.HitBuffer(batch.n)
+        cnt = (C.c_uint64 * 2)()
+        lib().orc_trace(self.h, C.byref(batch.c), batch.n, C.byref(hb.c), None,
+                        nthreads, cnt)
+        return hb, (cnt[0], cnt[1])
+
+    def trace_occluded(self, batch, nthreads=8):
+        occ = np.zeros(batch.n, np.uint8)
+        cnt = (C.c_uint64 * 2)()
+        lib().orc_trace(self.h, C.byref(batch.c), batch.n, None,
+                        occ.ctypes.data_as(C.POINTER(C.c_uint8)), nthreads, cnt)
+        return occ, (cnt[0], cnt[1])
+
+    def render(self, cam, params, nthreads=8, samples=False):
+        acc = np.zeros((params.height, params.width, 5), np.float64)
+        x0, y0, w, h, m = abi.render_window(params)
+        out = None
+        if samples:
+            out = np.zeros(((h + 2 * m) * (w + 2 * m) * params.spp, 6), np.float32)
+        st = abi.Stats()
+        err = C.create_string_buffer(512)
+        rc = lib().orc_render(self.h, C.byref(cam), C.byref(params), dptr(acc),
+                              abi.fptr(out) if out is not None else None,
+                              nthreads, C.byref(st), err, 512)
+        if rc:
+            raise RuntimeError("oracle render: " + err.value.decode())
+        return acc, out, st.as_dict()
+
+
+def bsdf_probe(mat, v, gn, params, dirs, back=False, media_ior=1.0,
+               enclosing_ior=1.0, flags_s=0x37, flags_e=0x37):
+    m = abi.Material()
+    m.type = mat["type"]
+    m.color[:] = list(mat.get("color", (0, 0, 0)))
+    m.gloss_color[:] = list(mat.get("gloss_color", (0, 0, 0)))
+    m.m, m.ior_n, m.ior_k = mat.get("m", 0), mat.get("ior_n", 1.5), mat.get("ior_k", 0)
+    m.underlying = -1
+    v = np.ascontiguousarray(v, np.float64)
+    gn = np.ascontiguousarray(gn, np.float64)
+    params = np.ascontiguousarray(params, np.float64).reshape(-1, 2)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    out_s = np.zeros((len(params), 8))
+    out_e = np.zeros((len(dirs), 4))
+    lib().orc_bsdf_probe(C.byref(m), dptr(v), dptr(gn), 1 if back else 0,
+                         media_ior, enclosing_ior, dptr(params), len(params),
+                         dptr(out_s), dptr(dirs), len(dirs), dptr(out_e),
+                         flags_s, flags_e)
+    return out_s, out_e
+
+
+def fresnel(n1, n2, k2, cos_angle):
+    return lib().orc_fresnel(n1, n2, k2, cos_angle)
+
+
+def mitchell(x, radius=2.0, b=1.0 / 3, c=1.0 / 3):
+    return lib().orc_mitchell(x, radius, b, c)
+
+
+def envmap_probe(texels, params):
+    t = np.ascontiguousarray(texels, np.float32)
+    p = np.ascontiguousarray(params, np.float64).reshape(-1, 2)
+    out = np.zeros((len(p), 10))
+    lib().orc_envmap_probe(abi.fptr(t), t.shape[1], t.shape[0], dptr(p), len(p),
+                           dptr(out))
+    return out
+
+
+def stream_probe(seed, px, py, chan, spp, s=0, nr=0):
+    uv = np.zeros((spp, 2), np.float32)
+    r = np.zeros(max(nr, 1), np.float32)
+    lib().orc_stream_probe(seed, px, py, chan, spp, abi.fptr(uv), s, nr,
+                           abi.fptr(r))
+    return uv, r[:nr]
+
+
+def permute(i, n, key):
+    return lib().orc_permute(i, n, key)

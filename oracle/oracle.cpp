@@ -1,0 +1,604 @@
+// ORACLE -- test infrastructure only.  Only tests/, the smoke check and the
+// CPU-baseline leg of bench.py load liboracle.so; the product never does.
+//
+// CPU restatement of the reference's plain path-tracing render path
+// (render/path-integ.cc:133-407, render/direct-illum.cc:82-255,
+// render-mgr/renderer.cc:41-100, image/image-filter-conv.h:109-186) and
+// its ray queries (space/space.cc:36-194, space/octree.cc:101-300).
+// C entry points for ctypes; every call catches exceptions and returns a
+// status, the message is copied into the caller's buffer.
+#include "orc_render.h"
+#include <atomic>
+#include <thread>
+#include <memory>
+#include <cstdio>
+#include <functional>
+
+using namespace orc;
+
+namespace {
+
+struct Counters {
+  uint64_t eye = 0, isect = 0, shadow = 0, nc_c = 0, st_c = 0, nc_s = 0,
+           st_s = 0, illum = 0, paths = 0;
+  void add(const Counters &o) {
+    eye += o.eye; isect += o.isect; shadow += o.shadow; nc_c += o.nc_c;
+    st_c += o.st_c; nc_s += o.nc_s; st_s += o.st_s; illum += o.illum;
+    paths += o.paths;
+  }
+};
+
+// render/mis-sample-weight.h:45-77 (power heuristic, one sample each).
+template<class R> static inline R mis_weight(R pdf, R other_pdf) {
+  R term = R(1) * pdf;
+  R other = R(1) * other_pdf;
+  if (term > R(1e10f)) return 1;
+  if (other > R(1e10f)) return 0;
+  R t2 = term * term, o2 = other * other;
+  return (term * pdf) / (t2 + o2);
+}
+
+// image/image-mitchell-filt.h:45-65
+static inline float mitchell1(float x, float inv_radius, float b, float c) {
+  x = std::fabs(2.f * x * inv_radius);
+  if (x > 2.f) return 0;
+  if (x > 1.f)
+    return ((-b - 6 * c) * x * x * x + (6 * b + 30 * c) * x * x +
+            (-12 * b - 48 * c) * x + (8 * b + 24 * c)) * (1.f / 6.f);
+  return ((12 - 9 * b - 6 * c) * x * x * x + (-18 + 12 * b + 6 * c) * x * x +
+          (6 - 2 * b)) * (1.f / 6.f);
+}
+
+// Sample-stream layout (see orc_rng.h).
+struct Stream {
+  uint32_t spp, minp, L, ds;
+  GridShape g_cam, g_dir, g_cont, g_dir_rand;
+  uint32_t nsub_dir, nsub_dir_rand;
+  void init(uint32_t spp_, uint32_t minp_, uint32_t L_, uint32_t ds_) {
+    spp = spp_; minp = minp_; L = L_; ds = ds_;
+    g_cam = grid_shape(spp);
+    g_cont = grid_shape(spp);
+    g_dir = grid_shape(spp * (ds ? ds : 1));
+    nsub_dir = g_dir.total / spp;           // render/sample-set.h:330-345
+    g_dir_rand = grid_shape(ds ? ds : 1);   // one-sample random set
+    nsub_dir_rand = g_dir_rand.total;
+  }
+  uint32_t chan_light(uint32_t b, uint32_t l) const { return 1 + b * (2 * L + 1) + 2 * l; }
+  uint32_t chan_cont(uint32_t b) const { return 1 + b * (2 * L + 1) + 2 * L; }
+  static void strat(uint64_t pk, uint32_t chan, uint32_t entry, const GridShape &g,
+                    float &u, float &v) {
+    uint64_t ck = chan_key(pk, chan);
+    uint32_t cell = permute(entry, g.total, ck);
+    uint32_t j = cell % g.u_steps, i = cell / g.u_steps;
+    float us = 1.0f / (float)g.u_steps, vs = 1.0f / (float)g.v_steps;
+    u = clamp01((float)j * us + draw(ck, 2 * entry) * us);
+    v = clamp01((float)i * vs + draw(ck, 2 * entry + 1) * vs);
+  }
+  static uint32_t dim(uint32_t b, uint32_t slot) { return (b << 20) | slot; }
+};
+
+template<class R> struct Tracer {
+  const Scene<R> &sc;
+  const sgr_render_params &P;
+  Stream st;
+  Counters cnt;
+
+  Tracer(const Scene<R> &s, const sgr_render_params &p) : sc(s), P(p) {
+    st.init(p.spp, p.min_path_len, (uint32_t)s.lights.size(), p.direct_samples);
+  }
+
+  bool occluded(const Ray<R> &r) {
+    cnt.shadow++;
+    return sc.accel.occluded(r, &cnt.nc_s, &cnt.st_s);
+  }
+
+  // DirectIllum::sample_light (render/direct-illum.cc:126-255)
+  C3<R> sample_light(const Isec<R> &is, const Bsdf<R> &bsdf, const Light<R> &L,
+                     R lu, R lv, R bu, R bv, R extra, unsigned flags) {
+    C3<R> rad(0);
+    LightSample<R> ls = sc.light_sample(L, is, lu, lv, extra);
+    if (ls.pdf > 0 && ls.val.gt0()) {
+      BsdfValue<R> bval = bsdf.eval(ls.dir, flags);
+      if (bval.val.gt0()) {
+        Ray<R> ray = sc.recursive_ray(is, ls.dir, ls.dist, true);
+        if (!occluded(ray)) {
+          C3<R> r = ls.val;
+          r *= mis_weight(ls.pdf, bval.pdf);
+          r *= bval.val;
+          r *= std::fabs(is.cos_n(ls.dir));
+          r /= ls.pdf;
+          rad += r;
+        }
+      }
+    }
+    BsdfSample<R> bs = bsdf.sample(bu, bv, flags);
+    if (bs.pdf > 0 && bs.val.gt0()) {
+      LightSample<R> lval = sc.light_eval(L, is, bs.dir);
+      if (lval.pdf > 0 && lval.val.gt0()) {
+        Ray<R> ray = sc.recursive_ray(is, bs.dir, lval.dist, true);
+        if (!occluded(ray)) {
+          C3<R> r = lval.val;
+          r *= mis_weight(bs.pdf, lval.pdf);
+          r *= bs.val;
+          r *= std::fabs(is.cos_n(bs.dir));
+          r /= bs.pdf;
+          rad += r;
+        }
+      }
+    }
+    return rad;
+  }
+
+  // DirectIllum::sample_all_lights (render/direct-illum.cc:82-115)
+  C3<R> direct(const Isec<R> &is, const Bsdf<R> &bsdf, uint32_t b, uint64_t pk,
+               uint64_t qk, uint32_t s, unsigned flags) {
+    cnt.illum++;
+    C3<R> rad(0);
+    bool strat = b < st.minp;
+    uint32_t nsub = strat ? st.nsub_dir : st.nsub_dir_rand;
+    for (uint32_t l = 0; l < st.L; l++) {
+      C3<R> lrad(0);
+      for (uint32_t j = 0; j < nsub; j++) {
+        float lu, lv, bu, bv;
+        uint32_t base = 16 + (l * nsub + j) * 8;
+        if (strat) {
+          Stream::strat(pk, st.chan_light(b, l), s * nsub + j, st.g_dir, lu, lv);
+          Stream::strat(pk, st.chan_light(b, l) + 1, s * nsub + j, st.g_dir, bu, bv);
+        } else {
+          lu = draw(qk, Stream::dim(b, base + 0));
+          lv = draw(qk, Stream::dim(b, base + 1));
+          bu = draw(qk, Stream::dim(b, base + 2));
+          bv = draw(qk, Stream::dim(b, base + 3));
+        }
+        float extra = draw(qk, Stream::dim(b, base + 4));
+        lrad += sample_light(is, bsdf, sc.lights[l], R(lu), R(lv), R(bu), R(bv),
+                             R(extra), flags);
+      }
+      rad += lrad / R(float(nsub));
+    }
+    return rad;
+  }
+
+  // PathInteg::Li (render/path-integ.cc:133-407).  Returns radiance and
+  // sets alpha.
+  C3<R> Li(Ray<R> ray, uint64_t pk, uint64_t qk, uint32_t s, R &alpha) {
+    R media[64];
+    int depth = 1;
+    media[0] = 1;  // RenderContext::default_medium (ior 1)
+    C3<R> radiance(0), T(1);
+    R rr_ratio = 1;
+    bool after_spec = false;
+    alpha = 1;
+    uint32_t path_len = 0;
+    for (;;) {
+      R hu, hv;
+      cnt.isect++;
+      unsigned prim = sc.accel.closest(ray, hu, hv, &cnt.nc_c, &cnt.st_c);
+      bool include = path_len == 0 || after_spec || P.direct_samples == 0;
+      if (prim == 0xffffffffu) {
+        if (include) radiance += sc.background(ray.dir) * T;
+        if (path_len == 0 && radiance.is_zero()) alpha = R(P.bg_alpha);
+        break;
+      }
+      Isec<R> is;
+      R enclosing = depth >= 2 ? media[depth - 2] : R(1);
+      sc.make_isec(prim, ray, hu, hv, media[depth - 1], enclosing, is);
+      const Material<R> &hm = sc.mats[is.mat];
+      if (include && hm.type == SGR_MAT_GLOW && !is.back)  // material/glow.cc:36-40
+        radiance += hm.color * T;
+      if (is.bsdf_mat < 0) break;
+      const Material<R> &bm = sc.mats[is.bsdf_mat];
+      Bsdf<R> bsdf;
+      bsdf.init(bm, is, bm.ior_n);
+      if (P.direct_samples != 0)
+        radiance += direct(is, bsdf, path_len, pk, qk, s, ALL & ~SPECULAR) * T;
+      float cu, cv;
+      if (path_len < st.minp) {
+        Stream::strat(pk, st.chan_cont(path_len), s, st.g_cont, cu, cv);
+      } else {
+        cu = draw(qk, Stream::dim(path_len, 1));
+        cv = draw(qk, Stream::dim(path_len, 2));
+      }
+      BsdfSample<R> bs = bsdf.sample(R(cu), R(cv), ALL);
+      if (bs.pdf == 0 || bs.val.is_zero()) break;
+      C3<R> f = bs.val * std::fabs(is.cos_n(bs.dir));
+      f = f / bs.pdf;
+      T *= f;
+      if (path_len > st.minp) {
+        R p = mn(T.intensity() * rr_ratio, R(1));
+        R rr = R(draw(qk, Stream::dim(path_len, 0)));
+        if (rr > p) break;
+        T /= p;
+        rr_ratio *= p;
+      }
+      if (path_len == P.max_path_len) break;
+      ray = sc.recursive_ray(is, bs.dir, 0, false);
+      after_spec = (bs.flags & SPECULAR) != 0;
+      if (bs.flags & TRANSMISSIVE) {  // material/media.cc:75-118
+        bool has;
+        R mior = sc.medium_ior(is.mat, has);
+        if (has) {
+          if (is.back) {
+            if (depth >= 2) depth--;
+          } else if (depth < 64) {
+            media[depth++] = mior;
+          }
+        }
+      }
+      path_len++;
+    }
+    return radiance;
+  }
+
+  // One path: camera ray (render-mgr/renderer.cc:71-85,
+  // camera/camera.h:322-389) and PathInteg::Li.  Output: sx, sy, Tint.
+  void path(const sgr_camera &cam, R max_trace, int px, int py, uint32_t s,
+            float out[6]) {
+    uint64_t pk = pixel_key(P.seed, px, py);
+    uint64_t qk = path_key(pk, s);
+    float cu, cv;
+    Stream::strat(pk, 0, s, st.g_cam, cu, cv);
+    float cx = float(px) + cu, cy = float(py) + cv;
+    float fu = cx / float(P.width), fv = (float(P.height) - cy) / float(P.height);
+    R x = R(2) * R(fu) - R(1), y = R(2) * R(fv) - R(1);
+    V3<R> fwd(cam.forward[0], cam.forward[1], cam.forward[2]);
+    V3<R> right(cam.right[0], cam.right[1], cam.right[2]);
+    V3<R> up(cam.up[0], cam.up[1], cam.up[2]);
+    Ray<R> ray;
+    ray.origin = V3<R>(cam.pos[0], cam.pos[1], cam.pos[2]);
+    ray.dir = fwd + right * x * R(cam.tan_half_fov_x) + up * y * R(cam.tan_half_fov_y);
+    ray.t0 = 0;
+    ray.t1 = max_trace;
+    cnt.eye++;
+    cnt.paths++;
+    R alpha;
+    C3<R> L = Li(ray, pk, qk, s, alpha);
+    out[0] = cx;
+    out[1] = cy;
+    out[2] = float(L.r * alpha);  // Tint(radiance, alpha) premultiplies
+    out[3] = float(L.g * alpha);
+    out[4] = float(L.b * alpha);
+    out[5] = float(alpha);
+  }
+};
+
+struct OrcScene {
+  int fp64;
+  std::unique_ptr<Scene<float>> sf;
+  std::unique_ptr<Scene<double>> sd;
+};
+
+static void set_err(char *err, int n, const char *msg) {
+  if (err && n > 0) {
+    std::snprintf(err, (size_t)n, "%s", msg);
+  }
+}
+
+static void parallel_for(size_t n, int nthreads,
+                         const std::function<void(size_t, int)> &fn) {
+  if (nthreads <= 1) {
+    for (size_t i = 0; i < n; i++) fn(i, 0);
+    return;
+  }
+  std::atomic<size_t> next(0);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; t++)
+    th.emplace_back([&, t]() {
+      for (;;) {
+        size_t i = next.fetch_add(1);
+        if (i >= n) break;
+        fn(i, t);
+      }
+    });
+  for (auto &x : th) x.join();
+}
+
+template<class R>
+static void trace_impl(const Scene<R> &sc, const sgr_rays *rays, size_t n,
+                       sgr_hits *hits, uint8_t *occ, int nthreads,
+                       uint64_t *counters) {
+  std::vector<Counters> cs((size_t)(nthreads > 0 ? nthreads : 1));
+  size_t chunk = 4096;
+  size_t nchunks = (n + chunk - 1) / chunk;
+  parallel_for(nchunks, nthreads, [&](size_t c, int t) {
+    size_t b = c * chunk, e = std::min(n, b + chunk);
+    for (size_t i = b; i < e; i++) {
+      Ray<R> r;
+      r.origin = V3<R>(rays->ox[i], rays->oy[i], rays->oz[i]);
+      r.dir = V3<R>(rays->dx[i], rays->dy[i], rays->dz[i]);
+      r.t0 = rays->t0[i];
+      r.t1 = rays->t1[i];
+      if (hits) {
+        R u = 0, v = 0;
+        unsigned p = sc.accel.closest(r, u, v, &cs[t].nc_c, &cs[t].st_c);
+        hits->prim[i] = p;
+        hits->t[i] = p == 0xffffffffu ? 0.f : float(r.t1);
+        hits->u[i] = float(u);
+        hits->v[i] = float(v);
+      }
+      if (occ) occ[i] = sc.accel.occluded(r, &cs[t].nc_s, &cs[t].st_s) ? 1 : 0;
+    }
+  });
+  if (counters) {
+    Counters tot;
+    for (auto &c : cs) tot.add(c);
+    counters[0] = hits ? tot.nc_c : tot.nc_s;
+    counters[1] = hits ? tot.st_c : tot.st_s;
+  }
+}
+
+template<class R>
+static void render_impl(const Scene<R> &sc, const sgr_camera &cam,
+                        const sgr_render_params &P0, double *accum,
+                        float *samples_out, int nthreads, sgr_stats *stats) {
+  sgr_render_params P = P0;
+  if (P.w == 0) { P.x0 = 0; P.y0 = 0; P.w = P.width; P.h = P.height; }
+  int margin = (int)std::ceil(P.filter_radius - 0.5001f);  // image-filter-conv.h:39-40
+  int X0 = (int)P.x0 - margin, Y0 = (int)P.y0 - margin;
+  int PW = (int)P.w + 2 * margin, PH = (int)P.h + 2 * margin;
+  // Renderer::render_packet: max_trace = (scene bbox + camera pos).diameter()
+  BBox<R> bb = sc.bbox;
+  bb.add(V3<R>(cam.pos[0], cam.pos[1], cam.pos[2]));
+  R max_trace = bb.diameter();
+  int nt = nthreads > 0 ? nthreads : 1;
+  std::vector<Tracer<R>> tracers;
+  for (int t = 0; t < nt; t++) tracers.emplace_back(sc, P);
+  std::vector<R> acc((size_t)P.width * P.height * 5, R(0));
+  float inv_r = 1.f / P.filter_radius;
+  int fr = margin;
+  const int band = 16;
+  std::vector<float> buf;
+  for (int yb = 0; yb < PH; yb += band) {
+    int ye = std::min(PH, yb + band);
+    size_t rows = (size_t)(ye - yb);
+    buf.assign(rows * PW * P.spp * 6, 0.f);
+    parallel_for(rows * PW, nt, [&](size_t i, int t) {
+      int py = Y0 + yb + (int)(i / PW), px = X0 + (int)(i % PW);
+      for (uint32_t s = 0; s < P.spp; s++)
+        tracers[t].path(cam, max_trace, px, py, s, &buf[(i * P.spp + s) * 6]);
+    });
+    if (samples_out)
+      std::memcpy(samples_out + (size_t)yb * PW * P.spp * 6, buf.data(),
+                  buf.size() * sizeof(float));
+    // ImageFilterConv::add_sample (image/image-filter-conv.h:109-186)
+    for (size_t k = 0; k < rows * PW * P.spp; k++) {
+      const float *o = &buf[k * 6];
+      float sx = o[0], sy = o[1];
+      int x = int(sx), y = int(sy);
+      for (int fy = -fr; fy <= fr; fy++) {
+        int py = y + fy;
+        if (py < 0 || py >= (int)P.height) continue;
+        for (int fx = -fr; fx <= fr; fx++) {
+          int px = x + fx;
+          if (px < 0 || px >= (int)P.width) continue;
+          float w = mitchell1(sx - (float(px) + 0.5f), inv_r, P.filter_b, P.filter_c) *
+                    mitchell1(sy - (float(py) + 0.5f), inv_r, P.filter_b, P.filter_c);
+          R ws[4] = {R(o[2]) * R(w), R(o[3]) * R(w), R(o[4]) * R(w), R(o[5]) * R(w)};
+          R wr = R(w);
+          if (w < 0) {
+            R minc = ws[0];
+            minc = mn(ws[1], minc);
+            minc = mn(ws[2], minc);
+            if (minc < R(P.neg_clamp)) {
+              R cs = R(P.neg_clamp) / minc;
+              wr *= cs;
+              for (int q = 0; q < 4; q++) ws[q] *= cs;
+            }
+          }
+          R *a = &acc[((size_t)py * P.width + px) * 5];
+          for (int q = 0; q < 4; q++) a[q] += ws[q];
+          a[4] += wr;
+        }
+      }
+    }
+  }
+  if (accum)
+    for (size_t i = 0; i < acc.size(); i++) accum[i] += double(acc[i]);
+  if (stats) {
+    Counters tot;
+    for (auto &t : tracers) tot.add(t.cnt);
+    std::memset(stats, 0, sizeof(*stats));
+    stats->eye_rays = tot.eye;
+    stats->scene_intersect_calls = tot.isect;
+    stats->shadow_tests = tot.shadow;
+    stats->node_visits_closest = tot.nc_c;
+    stats->surf_tests_closest = tot.st_c;
+    stats->node_visits_shadow = tot.nc_s;
+    stats->surf_tests_shadow = tot.st_s;
+    stats->illum_calls = tot.illum;
+    stats->samples = (uint64_t)P.w * P.h * P.spp;
+    stats->paths_traced = tot.paths;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void *orc_scene_create(const sgr_scene_desc *d, int fp64, char *err, int errlen) {
+  try {
+    OrcScene *s = new OrcScene;
+    s->fp64 = fp64;
+    if (fp64) { s->sd.reset(new Scene<double>); s->sd->build(*d); }
+    else { s->sf.reset(new Scene<float>); s->sf->build(*d); }
+    return s;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return nullptr;
+  }
+}
+
+void orc_scene_destroy(void *s) { delete (OrcScene *)s; }
+
+int orc_octree_export(void *h, uint32_t *nodes, uint64_t *nn, uint32_t *surfs,
+                      uint64_t *ns, double *origin_size) {
+  OrcScene *s = (OrcScene *)h;
+  if (s->fp64) return 1;
+  const Accel<float> &a = s->sf->accel;
+  *nn = a.nodes.size();
+  *ns = a.surfs.size();
+  if (nodes)
+    for (size_t i = 0; i < a.nodes.size(); i++) {
+      nodes[4 * i] = a.nodes[i].first_child;
+      nodes[4 * i + 1] = a.nodes[i].mask;
+      nodes[4 * i + 2] = a.nodes[i].surf_begin;
+      nodes[4 * i + 3] = a.nodes[i].surf_count;
+    }
+  if (surfs) std::memcpy(surfs, a.surfs.data(), a.surfs.size() * 4);
+  if (origin_size) {
+    origin_size[0] = a.origin.x; origin_size[1] = a.origin.y;
+    origin_size[2] = a.origin.z; origin_size[3] = a.size;
+  }
+  return 0;
+}
+
+int orc_octree_stats(void *h, sgr_octree_stats *out) {
+  OrcScene *s = (OrcScene *)h;
+  if (s->fp64) return 1;
+  OctStats st;
+  if (!s->sf->accel.nodes.empty()) oct_upd_stats(s->sf->accel.nodes, 0, st);
+  out->num_nodes = st.num_nodes;
+  out->num_leaf_nodes = st.num_leaf_nodes;
+  out->num_surfaces = st.num_surfaces;
+  out->num_dup_surfaces = st.num_surfaces - s->sf->num_real;
+  out->max_depth = st.max_depth;
+  out->avg_depth = st.avg_depth;
+  return 0;
+}
+
+int orc_scene_bounds(void *h, double *bmin, double *bmax, double *horizon) {
+  OrcScene *s = (OrcScene *)h;
+  if (s->fp64) {
+    bmin[0] = s->sd->bbox.min.x; bmin[1] = s->sd->bbox.min.y; bmin[2] = s->sd->bbox.min.z;
+    bmax[0] = s->sd->bbox.max.x; bmax[1] = s->sd->bbox.max.y; bmax[2] = s->sd->bbox.max.z;
+    *horizon = s->sd->horizon;
+  } else {
+    bmin[0] = s->sf->bbox.min.x; bmin[1] = s->sf->bbox.min.y; bmin[2] = s->sf->bbox.min.z;
+    bmax[0] = s->sf->bbox.max.x; bmax[1] = s->sf->bbox.max.y; bmax[2] = s->sf->bbox.max.z;
+    *horizon = s->sf->horizon;
+  }
+  return 0;
+}
+
+int orc_trace(void *h, const sgr_rays *rays, size_t n, sgr_hits *hits,
+              uint8_t *occ, int nthreads, uint64_t *counters) {
+  OrcScene *s = (OrcScene *)h;
+  if (s->fp64) trace_impl<double>(*s->sd, rays, n, hits, occ, nthreads, counters);
+  else trace_impl<float>(*s->sf, rays, n, hits, occ, nthreads, counters);
+  return 0;
+}
+
+int orc_render(void *h, const sgr_camera *cam, const sgr_render_params *p,
+               double *accum, float *samples_out, int nthreads, sgr_stats *stats,
+               char *err, int errlen) {
+  try {
+    OrcScene *s = (OrcScene *)h;
+    if (p->spp == 0) throw std::runtime_error("spp must be > 0");
+    if (s->fp64) render_impl<double>(*s->sd, *cam, *p, accum, samples_out, nthreads, stats);
+    else render_impl<float>(*s->sf, *cam, *p, accum, samples_out, nthreads, stats);
+    return 0;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+// --- unit probes ----------------------------------------------------------
+
+// BSDF probe: builds an intersection whose normal frame is the identity
+// (so directions are given in the normal frame), view vector v (unit,
+// v.z > 0 after the back-face flip), geometric normal gn, back flag and
+// media IORs, then samples at n parameter pairs and evaluates n
+// directions.  out_s: val rgb, pdf, dir xyz, flags (8 per sample);
+// out_e: val rgb, pdf (4 per direction).
+int orc_bsdf_probe(const sgr_material *mat, const double *v, const double *gn,
+                   int back, double media_ior, double enclosing_ior,
+                   const double *params, int n, double *out_s,
+                   const double *dirs, int nd, double *out_e, int flags_s,
+                   int flags_e) {
+  Isec<float> is;
+  is.v = V3<float>((float)v[0], (float)v[1], (float)v[2]);
+  is.geom_n = V3<float>((float)gn[0], (float)gn[1], (float)gn[2]);
+  is.back = back != 0;
+  is.media_ior = (float)media_ior;
+  is.enclosing_ior = (float)enclosing_ior;
+  is.mat = 0;
+  is.bsdf_mat = 0;
+  Material<float> m{mat->type, C3<float>(mat->color[0], mat->color[1], mat->color[2]),
+                    C3<float>(mat->gloss_color[0], mat->gloss_color[1], mat->gloss_color[2]),
+                    mat->m, mat->ior_n, mat->ior_k, mat->underlying};
+  Bsdf<float> b;
+  b.init(m, is, m.ior_n);
+  for (int i = 0; i < n; i++) {
+    BsdfSample<float> s = b.sample((float)params[2 * i], (float)params[2 * i + 1],
+                                   (unsigned)flags_s);
+    double *o = &out_s[8 * i];
+    o[0] = s.val.r; o[1] = s.val.g; o[2] = s.val.b; o[3] = s.pdf;
+    o[4] = s.dir.x; o[5] = s.dir.y; o[6] = s.dir.z; o[7] = s.flags;
+  }
+  for (int i = 0; i < nd; i++) {
+    V3<float> d((float)dirs[3 * i], (float)dirs[3 * i + 1], (float)dirs[3 * i + 2]);
+    BsdfValue<float> e = b.eval(d, (unsigned)flags_e);
+    double *o = &out_e[4 * i];
+    o[0] = e.val.r; o[1] = e.val.g; o[2] = e.val.b; o[3] = e.pdf;
+  }
+  return 0;
+}
+
+double orc_fresnel(double n1, double n2, double k2, double cos_angle) {
+  return Fresnel<float>((float)n1, (float)n2, (float)k2).reflectance((float)cos_angle);
+}
+
+double orc_mitchell(double x, double radius, double b, double c) {
+  return mitchell1((float)x, 1.f / (float)radius, (float)b, (float)c);
+}
+
+// Envmap probe: for each param pair, sample the light histogram
+// (u, v, pdf); then pdf_at(u, v) of the returned position.  Also returns
+// the world direction and the map lookup.  out: u, v, pdf, pdf_at, dx,
+// dy, dz, r, g, b (10 per sample).
+int orc_envmap_probe(const float *texels, uint32_t w, uint32_t h,
+                     const double *params, int n, double *out) {
+  Envmap<double> E;
+  E.w = w;
+  E.h = h;
+  E.tex.assign(texels, texels + (size_t)3 * w * h);
+  E.build();
+  for (int i = 0; i < n; i++) {
+    float u, v, pdf;
+    E.hist_sample((float)params[2 * i], (float)params[2 * i + 1], u, v, pdf);
+    double *o = &out[10 * i];
+    o[0] = u; o[1] = v; o[2] = pdf; o[3] = E.hist_pdf(u, v);
+    V3<double> d = Envmap<double>::latlong_to_vec(u, v);
+    o[4] = d.x; o[5] = d.y; o[6] = d.z;
+    C3<double> c = E.map(d);
+    o[7] = c.r; o[8] = c.g; o[9] = c.b;
+  }
+  return 0;
+}
+
+// Sample-stream probe: stratified UV values of channel CHAN for all SPP
+// samples of pixel (px, py), and NR plain draws of path S.
+int orc_stream_probe(uint64_t seed, int px, int py, uint32_t chan, uint32_t spp,
+                     float *uv_out, uint32_t s, uint32_t nr, float *r_out) {
+  uint64_t pk = pixel_key(seed, px, py);
+  GridShape g = grid_shape(spp);
+  for (uint32_t i = 0; i < spp; i++) {
+    float u, v;
+    uint64_t ck = chan_key(pk, chan);
+    uint32_t cell = permute(i, g.total, ck);
+    uint32_t j = cell % g.u_steps, k = cell / g.u_steps;
+    float us = 1.0f / (float)g.u_steps, vs = 1.0f / (float)g.v_steps;
+    u = clamp01((float)j * us + draw(ck, 2 * i) * us);
+    v = clamp01((float)k * vs + draw(ck, 2 * i + 1) * vs);
+    uv_out[2 * i] = u;
+    uv_out[2 * i + 1] = v;
+  }
+  uint64_t qk = path_key(pk, s);
+  for (uint32_t d = 0; d < nr; d++) r_out[d] = draw(qk, d);
+  return 0;
+}
+
+uint32_t orc_permute(uint32_t i, uint32_t n, uint64_t key) { return permute(i, n, key); }
+
+}  // extern "C"

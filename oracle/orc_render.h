@@ -1,0 +1,912 @@
+// ORACLE -- test infrastructure only.  Nothing in the product links this.
+//
+// CPU restatement of the reference's render path: scene set-up, intersect
+// frames, BSDFs, light samplers, the PathInteg bounce loop with
+// DirectIllum MIS, eye rays and the Mitchell splat.  Templated on R
+// (float = the reference's fp32 arithmetic, double = FP64 oracle).
+#pragma once
+#include "orc_geom.h"
+#include "orc_rng.h"
+#include "../include/snoggpu.h"
+#include <stdexcept>
+#include <string>
+
+namespace orc {
+
+// color/color.h:35-279.  Comparisons use component sums (:241-264).
+template<class R> struct C3 {
+  R r, g, b;
+  C3() : r(0), g(0), b(0) {}
+  C3(R v) : r(v), g(v), b(v) {}
+  C3(R a, R c, R d) : r(a), g(c), b(d) {}
+  C3 operator+(const C3 &o) const { return C3(r + o.r, g + o.g, b + o.b); }
+  C3 operator-(const C3 &o) const { return C3(r - o.r, g - o.g, b - o.b); }
+  C3 operator*(const C3 &o) const { return C3(r * o.r, g * o.g, b * o.b); }
+  C3 operator*(R s) const { return C3(r * s, g * s, b * s); }
+  // color/color.h:151-158: division multiplies by the reciprocal.
+  C3 operator/(R d) const { return operator*(R(1) / d); }
+  C3 &operator+=(const C3 &o) { r += o.r; g += o.g; b += o.b; return *this; }
+  C3 &operator*=(const C3 &o) { r *= o.r; g *= o.g; b *= o.b; return *this; }
+  C3 &operator*=(R s) { r *= s; g *= s; b *= s; return *this; }
+  C3 &operator/=(R d) { return operator*=(R(1) / d); }
+  R sum() const { R s = r; s += g; s += b; return s; }
+  R intensity() const { return sum() / R(3); }        // :160-166
+  bool gt0() const { return sum() > 0; }              // Color > 0
+  bool is_zero() const { return r == 0 && g == 0 && b == 0; }  // == 0
+  R min_component() const {  // :191-197
+    R m = r;
+    m = mn(g, m);
+    m = mn(b, m);
+    return m;
+  }
+};
+
+enum {  // material/bsdf.h:46-70
+  REFLECTIVE = 0x10, TRANSMISSIVE = 0x20, ALL_DIRECTIONS = 0x30,
+  SPECULAR = 0x01, GLOSSY = 0x02, DIFFUSE = 0x04, ALL_LAYERS = 0x07,
+  ALL = 0x37
+};
+
+template<class R> struct BsdfSample {
+  C3<R> val; R pdf = 0; V3<R> dir; unsigned flags = 0;
+};
+template<class R> struct BsdfValue { C3<R> val; R pdf = 0; };
+
+template<class R> struct Material {
+  int type;
+  C3<R> color, gloss;
+  R m, ior_n, ior_k;
+  int underlying;
+};
+
+// Intersection state (render/intersect.h:30-160).
+template<class R> struct Isec {
+  Frame<R> nf, gf;
+  V3<R> v, geom_n;
+  bool back;
+  int mat;        // material providing Le (GLOW) or the BSDF
+  int bsdf_mat;   // material providing the BSDF, -1 = none
+  R media_ior;    // innermost medium (isec.media.medium.ior)
+  R enclosing_ior;
+  R cos_n(const V3<R> &d) const { return d.z; }
+  R cos_geom_n(const V3<R> &d) const { return dot(d, geom_n); }
+  R cos_v(const V3<R> &d) const { return mn(dot(v, d), R(1)); }  // :61-64
+};
+
+// material/fresnel.h:42-177
+template<class R> struct Fresnel {
+  R n, k, n2k2, n2_m_k2;
+  Fresnel(R n1, R n2) : n(n2 / n1), k(0), n2k2(0), n2_m_k2(n * n) {}
+  Fresnel(R n1, R ior2n, R ior2k)
+      : n(ior2n / n1), k(ior2k / n1), n2k2(n * n * k * k),
+        n2_m_k2(n * n - k * k) {}
+  R reflectance(R cos_refl_angle) const {
+    cos_refl_angle = std::fabs(clampv(cos_refl_angle, R(-1), R(1)));
+    R refl_angle = std::acos(cos_refl_angle);
+    R Rp, Rs;
+    if (k == 0) {
+      R sin_trans = clampv(std::sin(refl_angle) / n, R(-1), R(1));
+      R cos_trans = std::sqrt(R(1) - sin_trans * sin_trans);
+      R nc1 = n * cos_refl_angle;
+      R nc2 = n * cos_trans;
+      R Fs = (nc1 - cos_trans) / (nc1 + cos_trans);
+      R Fp = (cos_refl_angle - nc2) / (cos_refl_angle + nc2);
+      Rs = Fs * Fs;
+      Rp = Fp * Fp;
+    } else {
+      R sin_refl = std::sin(refl_angle);
+      R a0 = n2_m_k2 - sin_refl * sin_refl;
+      R sin_tan = sin_refl * std::tan(refl_angle);
+      R common = std::sqrt(a0 * a0 + R(4) * n2k2);
+      R a2 = (common + a0) / R(2);
+      R b2 = (common - a0) / R(2);
+      R a2pb2 = a2 + b2;
+      R a = std::sqrt(a2);
+      R rs1 = a2pb2 + cos_refl_angle * cos_refl_angle;
+      R rs2 = R(2) * a * cos_refl_angle;
+      Rs = (rs1 - rs2) / (rs1 + rs2);
+      R rp1 = a2pb2 + sin_tan * sin_tan;
+      R rp2 = R(2) * a * sin_tan;
+      Rp = Rs * ((rp1 - rp2) / (rp1 + rp2));
+    }
+    return clamp01((Rs + Rp) / R(2));
+  }
+};
+
+// geometry/spherical-coords.h:155-168 / material/dist.h:28-31
+template<class R>
+static inline V3<R> z_normal_symm_vec(R cos_theta, R v) {
+  R lng = v * R(2) * R(PIf);
+  R sin_lat = cos_theta;
+  R cos_lat = std::sqrt(R(1) - sin_lat * sin_lat);
+  R cos_lng = std::cos(lng);
+  R sin_lng = std::sqrt(R(1) - cos_lng * cos_lng);
+  if (lng < 0 || lng > R(PI_D)) sin_lng = -sin_lng;
+  return V3<R>(cos_lng * cos_lat, -sin_lng * cos_lat, sin_lat);
+}
+
+// material/ward-dist.h:22-81
+template<class R> struct WardDist {
+  R m, inv_m2;
+  explicit WardDist(R m_) : m(m_), inv_m2(m_ == 0 ? R(0) : R(1) / (m_ * m_)) {}
+  V3<R> sample(R u, R v) const {
+    R cos_theta = u >= 1 ? R(0) : R(1) / std::sqrt(m * m * -std::log(R(1) - u) + R(1));
+    return z_normal_symm_vec(cos_theta, v);
+  }
+  R pdf(R cos_theta) const {
+    if (cos_theta == 0) return 0;
+    R ic = R(1) / cos_theta;
+    R ic2 = ic * ic;
+    R ic3 = ic2 * ic;
+    R tan2 = ic2 - R(1);
+    return inv_m2 * R(INV_PIf) * ic3 * std::exp(-tan2 * inv_m2);
+  }
+};
+
+// One BSDF instance at an intersection.
+template<class R> struct Bsdf {
+  int type = 0;  // 0 = none
+  const Isec<R> *isec = nullptr;
+  C3<R> color, gloss;
+  // Cook-Torrance (material/cook-torrance.cc:38-70)
+  R m = 0, diff_weight = 0, inv_diff_weight = 0, inv_gloss_weight = 0;
+  R nv = 0, inv_4_nv = 0;
+  unsigned gloss_layer = 0, have_layers = 0;
+  R fr_n1 = 1, fr_n = 1, fr_k = 0;  // Fresnel(n1, Ior(n, k))
+  // Glass (material/glass.cc:32-42)
+  R old_ior = 1, new_ior = 1;
+  bool has_under = false;  // Mirror's underlying Lambert
+
+  R fresnel_ct(R c) const { return Fresnel<R>(fr_n1, fr_n, fr_k).reflectance(c); }
+
+  void init(const Material<R> &mt, const Isec<R> &is, R glass_medium_ior) {
+    isec = &is;
+    type = mt.type;
+    if (type == SGR_MAT_LAMBERT) {
+      color = mt.color;
+    } else if (type == SGR_MAT_COOK_TORRANCE) {
+      m = mt.m;
+      color = mt.color;
+      gloss = mt.gloss;
+      R di = color.intensity(), gi = gloss.intensity();
+      diff_weight = (di + gi) == 0 ? R(0) : di / (di + gi);
+      inv_diff_weight = diff_weight == 0 ? R(0) : R(1) / diff_weight;
+      inv_gloss_weight = diff_weight == 1 ? R(0) : R(1) / (R(1) - diff_weight);
+      fr_n1 = is.media_ior;
+      fr_n = mt.ior_n;
+      fr_k = mt.ior_k;
+      nv = is.cos_n(is.v);
+      inv_4_nv = (nv != 0) ? R(1) / (R(4) * nv) : R(0);
+      gloss_layer = m < R(0.5) ? GLOSSY : DIFFUSE;
+      have_layers = (diff_weight > 0 ? DIFFUSE : 0) | (diff_weight < 1 ? gloss_layer : 0);
+    } else if (type == SGR_MAT_GLASS) {
+      bool entering = !is.back;
+      old_ior = entering ? is.media_ior : glass_medium_ior;
+      new_ior = entering ? glass_medium_ior : is.enclosing_ior;
+    } else if (type == SGR_MAT_MIRROR) {
+      gloss = mt.gloss;  // reflectance
+      color = mt.color;  // underlying Lambert colour
+      fr_n1 = is.media_ior;
+      fr_n = mt.ior_n;
+      fr_k = mt.ior_k;
+      // material/mirror.cc:28-33: no underlying material if the colour is
+      // below Eps (Color < Color(Eps) compares component sums).
+      has_under = !(color.sum() < R(3e-15));
+    }
+  }
+
+  // --- Lambert (material/lambert.cc:38-62)
+  BsdfSample<R> lambert_sample(R u, R v, unsigned flags, const C3<R> &col) const {
+    BsdfSample<R> s;
+    if ((flags & (REFLECTIVE | DIFFUSE)) == (REFLECTIVE | DIFFUSE)) {
+      R cos_theta = std::sqrt(u);
+      R pdf = cos_theta * R(INV_PIf);
+      V3<R> dir = z_normal_symm_vec(cos_theta, v);
+      if (isec->cos_n(dir) > 0 && isec->cos_geom_n(dir) > 0) {
+        s.val = col * R(INV_PIf);
+        s.pdf = pdf;
+        s.dir = dir;
+        s.flags = REFLECTIVE | DIFFUSE;
+      }
+    }
+    return s;
+  }
+  BsdfValue<R> lambert_eval(const V3<R> &dir, unsigned flags) const {
+    BsdfValue<R> r;
+    R cn = isec->cos_n(dir);
+    if ((flags & (DIFFUSE | REFLECTIVE)) == (DIFFUSE | REFLECTIVE) && cn > 0) {
+      r.val = color * R(INV_PIf);
+      r.pdf = cn * R(INV_PIf);
+    }
+    return r;
+  }
+
+  // --- Cook-Torrance (material/cook-torrance.cc:236-276)
+  C3<R> ct_val(const V3<R> &l, const V3<R> &h, unsigned flags, R ddw, R &pdf) const {
+    R nl = isec->cos_n(l);
+    C3<R> col(0);
+    pdf = 0;
+    if (flags & DIFFUSE) {
+      R diff = R(INV_PIf);
+      R diff_pdf = nl * R(INV_PIf);
+      pdf += diff_pdf * ddw;
+      col += color * diff;
+    }
+    if (flags & gloss_layer) {
+      WardDist<R> wd(m);
+      R nh = isec->cos_n(h);
+      R inv_nl = (nl != 0) ? (R(1) / nl) : R(0);
+      R vh = isec->cos_v(h);
+      R G = mn(R(2) * nh * ((nv > nl) ? nl : nv) / vh, R(1));
+      R gl = fresnel_ct(vh) * wd.pdf(nh) * G * inv_4_nv * inv_nl;
+      R gl_pdf = wd.pdf(nh) / (R(4) * vh);
+      pdf += gl_pdf * (R(1) - ddw);
+      col += gloss * gl;
+    }
+    return col;
+  }
+  BsdfSample<R> ct_sample(R u, R v, unsigned desired) const {  // :76-160
+    BsdfSample<R> s;
+    V3<R> l, h;
+    unsigned flags = REFLECTIVE;
+    if (!(desired & REFLECTIVE)) return s;
+    desired &= have_layers;
+    if (!desired) return s;
+    if (isec->v.z < 0) return s;
+    R ddw, iddw, idgw;
+    if (desired == (DIFFUSE | gloss_layer)) {
+      ddw = diff_weight; iddw = inv_diff_weight; idgw = inv_gloss_weight;
+    } else if (desired == DIFFUSE) {
+      ddw = 1; iddw = 1; idgw = 0;
+    } else {
+      ddw = 0; iddw = 0; idgw = 1;
+    }
+    if (u < ddw) {
+      if (desired != DIFFUSE) u = u * iddw;
+      l = z_normal_symm_vec(std::sqrt(u), v);
+      h = (isec->v + l).unit();
+      flags |= DIFFUSE;
+    } else {
+      u = (u - ddw) * idgw;
+      h = WardDist<R>(m).sample(u, v);
+      if (isec->cos_v(h) < 0) h = -h;
+      l = mirror(isec->v, h);
+      flags |= gloss_layer;
+    }
+    if (isec->cos_n(l) > R(float(1e-15)) && isec->cos_geom_n(l) > R(float(1e-15))) {
+      R pdf;
+      s.val = ct_val(l, h, desired, ddw, pdf);
+      s.pdf = pdf;
+      s.dir = l;
+      s.flags = flags;
+    }
+    return s;
+  }
+  BsdfValue<R> ct_eval(const V3<R> &dir, unsigned flags) const {  // :165-190
+    BsdfValue<R> r;
+    R cn = isec->cos_n(dir);
+    if ((flags & REFLECTIVE) && cn > 0) {
+      flags &= have_layers;
+      R ddw = (flags == (DIFFUSE | gloss_layer)) ? diff_weight
+              : (flags == DIFFUSE) ? R(1) : R(0);
+      V3<R> h = (isec->v + dir).unit();
+      R pdf;
+      r.val = ct_val(dir, h, flags, ddw, pdf);
+      r.pdf = pdf;
+    }
+    return r;
+  }
+
+  // --- Glass (material/glass.cc:48-117)
+  BsdfSample<R> glass_sample(R u, R, unsigned flags) const {
+    BsdfSample<R> s;
+    if (flags & SPECULAR) {
+      flags &= ALL_DIRECTIONS;
+      V3<R> xmit_dir = refraction(-isec->v, V3<R>(0, 0, 1), old_ior, new_ior);
+      R cos_xmit = -isec->cos_n(xmit_dir);
+      R cos_refl = std::fabs(isec->cos_n(isec->v));
+      R xmit = (cos_xmit == 0) ? R(0)
+               : R(1) - Fresnel<R>(new_ior, old_ior).reflectance(cos_xmit);
+      R refl = (cos_refl == 0) ? R(0) : Fresnel<R>(old_ior, new_ior).reflectance(cos_refl);
+      if (xmit + refl != 0) {
+        R xp = (flags == TRANSMISSIVE) ? R(1) : (flags == REFLECTIVE) ? R(0)
+               : xmit / (xmit + refl);
+        if (u < xp) {
+          if (cos_xmit != 0) xmit /= cos_xmit;
+          s.val = C3<R>(xmit);
+          s.pdf = xp;
+          s.dir = xmit_dir;
+          s.flags = SPECULAR | TRANSMISSIVE;
+        } else {
+          if (cos_refl != 0) refl /= cos_refl;
+          s.val = C3<R>(refl);
+          s.pdf = R(1) - xp;
+          s.dir = mirror(isec->v, V3<R>(0, 0, 1));
+          s.flags = SPECULAR | REFLECTIVE;
+        }
+      }
+    }
+    return s;
+  }
+
+  // --- Mirror (material/mirror.cc:54-120)
+  BsdfSample<R> mirror_sample(R u, R v, unsigned flags) const {
+    BsdfSample<R> s;
+    if ((flags & (SPECULAR | REFLECTIVE)) == (SPECULAR | REFLECTIVE)) {
+      R cos_refl = isec->cos_n(isec->v);
+      if (cos_refl != 0) {
+        Fresnel<R> fr(fr_n1, fr_n, fr_k);
+        C3<R> refl = gloss * fr.reflectance(cos_refl) / cos_refl;
+        if (refl.sum() > R(3e-15) && isec->cos_geom_n(isec->v) > 0) {
+          s.val = refl;
+          s.pdf = 1;
+          s.dir = mirror(isec->v, V3<R>(0, 0, 1));
+          s.flags = SPECULAR | REFLECTIVE;
+        } else if (has_under) {
+          s = lambert_sample(u, v, flags, color);
+          R fres_refl = fr.reflectance(isec->cos_n(s.dir));
+          C3<R> rr = gloss * fres_refl;
+          s.val *= (C3<R>(1) - rr);
+        }
+      }
+    }
+    return s;
+  }
+
+  BsdfSample<R> sample(R u, R v, unsigned flags) const {
+    switch (type) {
+      case SGR_MAT_LAMBERT: return lambert_sample(u, v, flags, color);
+      case SGR_MAT_COOK_TORRANCE: return ct_sample(u, v, flags);
+      case SGR_MAT_GLASS: return glass_sample(u, v, flags);
+      case SGR_MAT_MIRROR: return mirror_sample(u, v, flags);
+    }
+    return BsdfSample<R>();
+  }
+  BsdfValue<R> eval(const V3<R> &dir, unsigned flags) const {
+    switch (type) {
+      case SGR_MAT_LAMBERT: return lambert_eval(dir, flags);
+      case SGR_MAT_COOK_TORRANCE: return ct_eval(dir, flags);
+    }
+    return BsdfValue<R>();  // glass and mirror: specular only
+  }
+};
+
+// Envmap light data (light/envmap-light.cc, geometry/hist-2d-dist.cc,
+// texture/spheremap.{h,cc}, texture/matrix-linterp.h).
+template<class R> struct Envmap {
+  unsigned w = 0, h = 0;
+  std::vector<float> tex;  // rgb, row 0 = top
+  Frame<R> frame;
+  unsigned hw = 0, hh = 0;
+  std::vector<float> row_cum, col_cum;
+
+  C3<R> texel(unsigned x, unsigned y) const {
+    const float *p = &tex[3 * ((size_t)y * w + x)];
+    return C3<R>(p[0], p[1], p[2]);
+  }
+  // texture/matrix-linterp.h:30-70 + texture/matrix-tex.tcc:51-67
+  C3<R> lookup(float uu, float vv) const {
+    float u = uu - std::floor(uu), v = vv - std::floor(vv);
+    float x = u * float(w) - 0.5f, y = v * float(h) - 0.5f;
+    float x_lo = std::floor(x), y_lo = std::floor(y);
+    float x_hi_fr = x - x_lo, y_hi_fr = y - y_lo;
+    float x_lo_fr = 1 - x_hi_fr, y_lo_fr = 1 - y_hi_fr;
+    int xi_lo_u = int(x_lo), yi_lo_u = int(y_lo);
+    unsigned xi_hi = (unsigned)(xi_lo_u + 1), yi_hi = (unsigned)(yi_lo_u + 1);
+    if (xi_lo_u < 0) xi_lo_u += (int)w;
+    if (yi_lo_u < 0) yi_lo_u += (int)h;
+    if (xi_hi >= w) xi_hi -= w;
+    if (yi_hi >= h) yi_hi -= h;
+    unsigned xi_lo = (unsigned)xi_lo_u, yi_lo = (unsigned)yi_lo_u;
+    yi_lo = h - yi_lo - 1;
+    yi_hi = h - yi_hi - 1;
+    return texel(xi_lo, yi_lo) * R(x_lo_fr * y_lo_fr) +
+           texel(xi_lo, yi_hi) * R(x_lo_fr * y_hi_fr) +
+           texel(xi_hi, yi_lo) * R(x_hi_fr * y_lo_fr) +
+           texel(xi_hi, yi_hi) * R(x_hi_fr * y_hi_fr);
+  }
+  // geometry/spherical-coords.h:60-142 (z_axis_latlong)
+  static void latlong(const V3<R> &d, float &u, float &v) {
+    float lng = float(std::atan2(-d.y, d.x));
+    float lat = float(std::asin(clampv(d.z / std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z),
+                                       R(-1), R(1))));
+    u = clamp01(lng * INV_PIf * 0.5f + 0.5f);
+    v = clamp01(lat * INV_PIf + 0.5f);
+  }
+  // geometry/spherical-coords.h:175-205 (z_axis_latlong_to_vec)
+  static V3<R> latlong_to_vec(float u, float v) {
+    float lng = (u - 0.5f) * PIf * 2;
+    float lat = (v - 0.5f) * PIf;
+    float colat = float(PI_D / 2) - lat;
+    return z_normal_symm_vec_lng(R(std::cos(colat)), R(lng));
+  }
+  static V3<R> z_normal_symm_vec_lng(R cos_colat, R lng) {
+    R sin_lat = cos_colat;
+    R cos_lat = std::sqrt(R(1) - sin_lat * sin_lat);
+    R cos_lng = std::cos(lng);
+    R sin_lng = std::sqrt(R(1) - cos_lng * cos_lng);
+    if (lng < 0 || lng > R(PI_D)) sin_lng = -sin_lng;
+    return V3<R>(cos_lng * cos_lat, -sin_lng * cos_lat, sin_lat);
+  }
+  C3<R> map(const V3<R> &d) const {
+    float u, v;
+    latlong(d, u, v);
+    return lookup(u, v);
+  }
+
+  void build() {
+    // texture/spheremap.cc:35-71 light map (block average, y flipped)
+    unsigned div = 1;
+    while (w / div > 1024 || h / div > 1024) div++;
+    hw = w / div;
+    hh = h / div;
+    float avg_scale = 1.f / float(div * div);
+    std::vector<float> lmap((size_t)hw * hh * 3, 0.f);
+    for (unsigned y = 0; y < h; y++)
+      for (unsigned x = 0; x < w; x++) {
+        unsigned lx = x / div;
+        unsigned ly = hh - y / div - 1;
+        if (lx >= hw || ly >= hh) continue;
+        float *p = &lmap[3 * ((size_t)ly * hw + lx)];
+        const float *t = &tex[3 * ((size_t)y * w + x)];
+        p[0] = p[0] + t[0] * avg_scale;
+        p[1] = p[1] + t[1] * avg_scale;
+        p[2] = p[2] + t[2] * avg_scale;
+      }
+    // light/envmap-light.cc:70-95 histogram with cos(latitude) weight
+    std::vector<float> bins((size_t)hw * hh, 0.f);
+    double inc = PI_D / hh;
+    double lat = -PI_D / 2 + inc / 2;
+    for (unsigned r = 0; r < hh; r++) {
+      double sc = std::cos(lat);
+      for (unsigned c = 0; c < hw; c++) {
+        const float *p = &lmap[3 * ((size_t)r * hw + c)];
+        float s = p[0]; s += p[1]; s += p[2];
+        float inten = s / 3.f;
+        double val = double(inten) * sc;
+        bins[(size_t)r * hw + c] += float(val);
+      }
+      lat += inc;
+    }
+    // geometry/hist-2d-dist.cc:52-110
+    row_cum.assign(hh, 0.f);
+    col_cum.assign((size_t)hw * hh, 0.f);
+    double total = 0;
+    for (size_t i = 0; i < bins.size(); i++) total += double(bins[i]);
+    double inv_total = total == 0 ? 0 : 1 / total;
+    double ns = 0;
+    for (unsigned r = 0; r < hh; r++) {
+      for (unsigned c = 0; c < hw; c++) ns += double(bins[(size_t)r * hw + c]) * inv_total;
+      row_cum[r] = float(ns);
+    }
+    for (unsigned r = 0; r < hh; r++) {
+      double rs = 0;
+      for (unsigned c = 0; c < hw; c++) rs += double(bins[(size_t)r * hw + c]);
+      double irs = rs == 0 ? 0 : 1 / rs;
+      double nrs = 0;
+      for (unsigned c = 0; c < hw; c++) {
+        nrs += double(bins[(size_t)r * hw + c]) * irs;
+        col_cum[(size_t)r * hw + c] = float(nrs);
+      }
+    }
+  }
+
+  float bin_pdf(unsigned col, unsigned row) const {  // hist-2d-dist.cc:245-262
+    float rp = row_cum[row];
+    if (row != 0) rp -= row_cum[row - 1];
+    const float *cc = &col_cum[(size_t)row * hw];
+    float cp = cc[col];
+    if (col != 0) cp -= cc[col - 1];
+    return rp * cp * float(hw * hh);
+  }
+  // hist-2d-dist.cc:180-233
+  bool hist_sample(float pu, float pv, float &ou, float &ov, float &pdf) const {
+    float u = mn(pu, 1.f), v = mn(pv, 1.f);
+    unsigned row = (unsigned)(std::lower_bound(row_cum.begin(), row_cum.end(), v) - row_cum.begin());
+    if (row == hh) { pdf = 0; ou = ov = 0; return false; }
+    const float *cc = &col_cum[(size_t)row * hw];
+    unsigned col = (unsigned)(std::lower_bound(cc, cc + hw, u) - cc);
+    pdf = bin_pdf(col, row);
+    float cw = 1.f / float(hw), rh = 1.f / float(hh);
+    ou = float(col) * cw + std::fmod(pu, cw);
+    ov = float(row) * rh + std::fmod(pv, rh);
+    return true;
+  }
+  float hist_pdf(float u, float v) const {  // hist-2d-dist.cc:265-273
+    int col = clampv(int(u * float(hw)), 0, int(hw) - 1);
+    int row = clampv(int(v * float(hh)), 0, int(hh) - 1);
+    return bin_pdf((unsigned)col, (unsigned)row);
+  }
+};
+
+template<class R> struct LightSample { C3<R> val; R pdf = 0; V3<R> dir; R dist = 0; };
+
+template<class R> struct Light {
+  int kind;  // 0 = tripar area light, 1 = sphere light, 2 = envmap
+  unsigned prim;
+  int env;
+  C3<R> intensity;
+};
+
+template<class R> struct Scene {
+  std::vector<Prim<R>> prims;
+  std::vector<int> prim_mat;
+  std::vector<int> prim_mesh;
+  struct MeshInfo { V3<R> axis, center; bool left_handed; };
+  std::vector<MeshInfo> meshes;
+  std::vector<Material<R>> mats;
+  std::vector<Light<R>> lights;
+  std::vector<Envmap<R>> envs;
+  Accel<R> accel;
+  OctreeBuilder<R> builder;
+  BBox<R> bbox;
+  R horizon = 0;
+  R min_trace = R(1e-3);
+  unsigned long num_real = 0;
+
+  int bsdf_material(int m) const {
+    int guard = 0;
+    while (m >= 0 && mats[m].type == SGR_MAT_GLOW && guard++ < 8) m = mats[m].underlying;
+    return m;
+  }
+  // material/material.h medium(): the glass medium, through GLOW wrappers.
+  R medium_ior(int m, bool &has) const {
+    int b = bsdf_material(m);
+    has = b >= 0 && mats[b].type == SGR_MAT_GLASS;
+    return has ? mats[b].ior_n : R(1);
+  }
+
+  void build(const sgr_scene_desc &d) {
+    for (unsigned i = 0; i < d.num_materials; i++) {
+      const sgr_material &m = d.materials[i];
+      if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_GLOW)
+        throw std::runtime_error("bad material type");
+      mats.push_back(Material<R>{m.type, C3<R>(m.color[0], m.color[1], m.color[2]),
+                                 C3<R>(m.gloss_color[0], m.gloss_color[1], m.gloss_color[2]),
+                                 R(m.m), R(m.ior_n), R(m.ior_k), m.underlying});
+    }
+    std::vector<BBox<R>> pb;
+    for (unsigned mi = 0; mi < d.num_meshes; mi++) {
+      const sgr_mesh &m = d.meshes[mi];
+      BBox<R> mb;
+      for (unsigned k = 0; k < m.num_vertices; k++)
+        mb.add(V3<R>(m.vertices[3 * k], m.vertices[3 * k + 1], m.vertices[3 * k + 2]));
+      // geometry/pos.h:123-126 midpoint
+      V3<R> cen((mb.min.x + mb.max.x) / R(2), (mb.min.y + mb.max.y) / R(2),
+                (mb.min.z + mb.max.z) / R(2));
+      meshes.push_back(MeshInfo{V3<R>(m.axis[0], m.axis[1], m.axis[2]), cen,
+                                m.left_handed != 0});
+      bbox.add(mb);
+      for (unsigned t = 0; t < m.num_triangles; t++) {
+        const uint32_t *ix = &m.triangles[3 * t];
+        V3<R> a(m.vertices[3 * ix[0]], m.vertices[3 * ix[0] + 1], m.vertices[3 * ix[0] + 2]);
+        V3<R> b(m.vertices[3 * ix[1]], m.vertices[3 * ix[1] + 1], m.vertices[3 * ix[1] + 2]);
+        V3<R> c(m.vertices[3 * ix[2]], m.vertices[3 * ix[2] + 1], m.vertices[3 * ix[2] + 2]);
+        Prim<R> p;
+        p.kind = 0;
+        p.parallelogram = false;
+        p.v0 = a;
+        p.e1 = b - a;  // surface/mesh.cc:735-736
+        p.e2 = c - a;
+        p.radius = 0;
+        prims.push_back(p);
+        prim_mat.push_back(m.tri_materials ? (int)m.tri_materials[t] : (int)m.material);
+        prim_mesh.push_back((int)mi);
+        BBox<R> tb(a);
+        tb.add(b);
+        tb.add(c);
+        pb.push_back(tb);
+      }
+    }
+    for (unsigned i = 0; i < d.num_tripars; i++) {
+      const sgr_tripar &t = d.tripars[i];
+      Prim<R> p;
+      p.kind = 1;
+      p.parallelogram = t.parallelogram != 0;
+      p.v0 = V3<R>(t.v0[0], t.v0[1], t.v0[2]);
+      p.e1 = V3<R>(t.e1[0], t.e1[1], t.e1[2]);
+      p.e2 = V3<R>(t.e2[0], t.e2[1], t.e2[2]);
+      p.radius = 0;
+      prims.push_back(p);
+      prim_mat.push_back((int)t.material);
+      prim_mesh.push_back(-1);
+      BBox<R> tb(p.v0);  // surface/tripar.cc:140-148
+      tb.add(p.v0 + p.e1);
+      tb.add(p.v0 + p.e2);
+      if (p.parallelogram) tb.add(p.v0 + p.e1 + p.e2);
+      pb.push_back(tb);
+      bbox.add(tb);
+    }
+    for (unsigned i = 0; i < d.num_spheres; i++) {
+      const sgr_sphere &s = d.spheres[i];
+      Prim<R> p;
+      p.kind = 2;
+      p.parallelogram = false;
+      p.center = V3<R>(s.center[0], s.center[1], s.center[2]);
+      p.radius = R(s.radius);
+      prims.push_back(p);
+      prim_mat.push_back((int)s.material);
+      prim_mesh.push_back(-1);
+      BBox<R> sb;  // surface/sphere.cc bbox
+      sb.min = V3<R>(p.center.x - p.radius, p.center.y - p.radius, p.center.z - p.radius);
+      sb.max = V3<R>(p.center.x + p.radius, p.center.y + p.radius, p.center.z + p.radius);
+      pb.push_back(sb);
+      bbox.add(sb);
+    }
+    // Acceleration structure; degenerate mesh triangles are not inserted
+    // (surface/mesh.cc:832-846).
+    accel.prims = &prims;
+    accel.use_triv = d.accel == SGR_ACCEL_TRIV;
+    for (unsigned p = 0; p < prims.size(); p++) {
+      if (prims[p].kind == 0) {
+        V3<R> nu = cross(prims[p].e2, prims[p].e1);
+        if (!(nu.length_squared() > 0)) continue;
+      }
+      if (accel.use_triv) accel.triv_order.push_back(p);
+      else builder.add(p, pb[p]);
+      num_real++;
+    }
+    if (!accel.use_triv) {
+      builder.flatten(accel.nodes, accel.surfs);
+      accel.origin = builder.origin;
+      accel.size = builder.size;
+    }
+    horizon = bbox.diameter();  // render/scene.cc:28
+    min_trace = R(d.min_trace);
+    // Light samplers (render/scene.cc:34; surface/primitive.cc:48-54;
+    // surface/sphere.cc add_light_samplers; light/envmap-light.cc:66-68)
+    for (unsigned p = 0; p < prims.size(); p++) {
+      int m = prim_mat[p];
+      if (prims[p].kind != 0 && m >= 0 && mats[m].type == SGR_MAT_GLOW) {
+        Light<R> L;
+        L.kind = prims[p].kind == 1 ? 0 : 1;
+        L.prim = p;
+        L.env = -1;
+        L.intensity = mats[m].color;
+        lights.push_back(L);
+      }
+    }
+    for (unsigned e = 0; e < d.num_envmaps; e++) {
+      const sgr_envmap &em = d.envmaps[e];
+      Envmap<R> E;
+      E.w = em.width;
+      E.h = em.height;
+      E.tex.assign(em.texels, em.texels + (size_t)3 * em.width * em.height);
+      E.frame = Frame<R>(V3<R>(), V3<R>(em.frame[0], em.frame[1], em.frame[2]),
+                         V3<R>(em.frame[3], em.frame[4], em.frame[5]),
+                         V3<R>(em.frame[6], em.frame[7], em.frame[8]));
+      E.build();
+      envs.push_back(E);
+      Light<R> L;
+      L.kind = 2;
+      L.prim = 0;
+      L.env = (int)e;
+      lights.push_back(L);
+    }
+  }
+
+  // Scene::background (render/scene.cc:62-73)
+  C3<R> background(const V3<R> &dir) const {
+    C3<R> r(0);
+    for (const Light<R> &L : lights)
+      if (L.kind == 2) {
+        const Envmap<R> &E = envs[L.env];
+        r += E.map(E.frame.to(dir));
+      }
+    return r;
+  }
+
+  // Frame construction for a hit (surface/mesh.cc:563-709,
+  // surface/tripar.cc:45-66, surface/sphere.cc IsecInfo::make_intersect)
+  // followed by Intersect::finish_init (render/intersect.cc:86-136).
+  void make_isec(unsigned p, const Ray<R> &ray, R hu, R hv, R media_ior,
+                 R enclosing_ior, Isec<R> &is) const {
+    (void)hu;
+    (void)hv;
+    const Prim<R> &pr = prims[p];
+    V3<R> point = ray.origin + ray.dir * ray.t1;
+    Frame<R> nf, gf;
+    if (pr.kind == 0) {
+      const MeshInfo &mi = meshes[prim_mesh[p]];
+      V3<R> raw = mi.left_handed ? cross(pr.e2, pr.e1) : cross(pr.e1, pr.e2);
+      V3<R> norm = raw.unit();
+      V3<R> s = cross(norm, mi.axis);
+      if (s.length_squared() < R(1e-15)) {
+        V3<R> cent = mi.center - point;
+        s = cross(norm, cent);
+        if (s.length_squared() < R(1e-15)) s = norm.perpendicular();
+      }
+      s = s.unit();
+      V3<R> t = cross(s, norm);
+      gf = Frame<R>(point, s, t, norm);
+      nf = gf;
+    } else if (pr.kind == 1) {
+      R e1_len = pr.e1.length(), inv_e1_len = R(1) / e1_len;
+      V3<R> norm = cross(pr.e2, pr.e1).unit();
+      V3<R> s = pr.e1 * inv_e1_len;
+      V3<R> t = cross(s, norm);
+      nf = Frame<R>(point, s, t, norm);
+      gf = nf;
+    } else {
+      V3<R> norm = (point - pr.center).unit();
+      V3<R> s = cross(norm, V3<R>(0, 0, 1));
+      if (s.length_squared() < R(1e-15)) s = norm.perpendicular();
+      s = s.unit();
+      V3<R> t = cross(s, norm);
+      nf = Frame<R>(point, s, t, norm);
+      gf = nf;
+    }
+    V3<R> wv = -ray.dir.unit();
+    is.v = nf.to(wv);
+    V3<R> gv = gf.to(wv);
+    is.back = gv.z < 0;
+    if (is.back) {
+      is.v.z = -is.v.z;
+      nf.z = -nf.z;
+    }
+    is.geom_n = nf.to(gf.z);
+    is.geom_n.z = std::fabs(is.geom_n.z);
+    is.nf = nf;
+    is.gf = gf;
+    is.mat = prim_mat[p];
+    is.bsdf_mat = bsdf_material(is.mat);
+    is.media_ior = media_ior;
+    is.enclosing_ior = enclosing_ior;
+  }
+
+  // Intersect::recursive_ray (render/intersect.cc:195-218)
+  Ray<R> recursive_ray(const Isec<R> &is, const V3<R> &dir, R max_dist, bool shadow) const {
+    Ray<R> r;
+    r.origin = is.nf.origin;
+    r.dir = is.nf.from(dir);
+    r.t0 = min_trace;
+    if (shadow) {
+      if (max_dist == 0) max_dist = horizon;
+      r.t1 = max_dist - min_trace;
+    } else {
+      r.t1 = horizon;
+    }
+    return r;
+  }
+
+  // Light::Sampler::sample (light/surface-light-sampler.cc:47-64,
+  // light/sphere-light-sampler.cc:53-92, light/envmap-light.cc:100-125)
+  LightSample<R> light_sample(const Light<R> &L, const Isec<R> &is, R u, R v,
+                              R extra) const {
+    LightSample<R> out;
+    if (L.kind == 0) {
+      const Prim<R> &pr = prims[L.prim];
+      // surface/tripar.cc:210-233 area sample
+      V3<R> norm = cross(pr.e2, pr.e1);
+      R area = norm.length();
+      norm = norm / area;
+      if (!pr.parallelogram && u + v > 1) { u = R(1) - u; v = R(1) - v; }
+      if (!pr.parallelogram) area *= R(0.5f);
+      V3<R> pos = pr.v0 + pr.e1 * u + pr.e2 * v;
+      R apdf = R(1) / area;
+      R dist, pdf;
+      V3<R> dir;
+      if (angular(pos, norm, apdf, is.nf.origin, dist, dir, pdf)) {
+        V3<R> d = is.nf.to(dir);
+        if (d.z > 0) {
+          out.val = L.intensity;
+          out.pdf = pdf;
+          out.dir = d;
+          out.dist = dist;
+        }
+      }
+    } else if (L.kind == 1) {
+      const Prim<R> &pr = prims[L.prim];
+      V3<R> lc = is.nf.to(pr.center - is.nf.origin);
+      R radius = pr.radius;
+      if (lc.z >= -radius) {
+        R pdf = R(1) / sphere_solid_angle(lc, radius);
+        R r_sqrt = radius * std::sqrt(u);
+        R ang = v * R(2) * R(PIf);
+        R x = r_sqrt * std::cos(ang);
+        R y = r_sqrt * std::sin(ang);
+        R z = std::sqrt(std::fabs(radius * radius - x * x - y * y)) *
+              std::sin(R(PIf) * (extra - R(0.5f)));
+        V3<R> sv = lc + V3<R>(x, y, z);
+        if (is.cos_n(sv) > 0 && is.cos_geom_n(sv) > 0) {
+          V3<R> sd = sv.unit();
+          R dist;
+          if (sphere_intersects(V3<R>(0, 0, 0), radius, -lc, sd, R(0), dist)) {
+            out.val = L.intensity;
+            out.pdf = pdf;
+            out.dir = sd;
+            out.dist = dist;
+          }
+        }
+      }
+    } else {
+      const Envmap<R> &E = envs[L.env];
+      float mu, mv, pdf;
+      E.hist_sample(float(u), float(v), mu, mv, pdf);
+      V3<R> ld = Envmap<R>::latlong_to_vec(mu, mv);
+      V3<R> wd = E.frame.from(ld);
+      V3<R> d = is.nf.to(wd);
+      if (is.cos_n(d) < 0 || is.cos_geom_n(d) < 0) return out;
+      out.val = E.map(ld);
+      out.pdf = R(pdf) * R(0.25f * INV_PIf);
+      out.dir = d;
+      out.dist = 0;
+    }
+    return out;
+  }
+
+  // surface/surface-sampler.cc:26-46 AngularSample
+  static bool angular(const V3<R> &pos, const V3<R> &normal, R apdf, const V3<R> &vp,
+                      R &dist, V3<R> &dir, R &pdf) {
+    V3<R> vv = pos - vp;
+    dist = vv.length();
+    pdf = 0;
+    if (dist > 0) {
+      R inv = R(1) / dist;
+      dir = vv * inv;
+      R adj = dot(-normal, dir);
+      if (adj > 0) {
+        R dw_dA = adj * (inv * inv);
+        pdf = apdf / dw_dA;
+      }
+    }
+    return pdf > 0;
+  }
+
+  // light/sphere-light-sampler.cc:40-48
+  static R sphere_solid_angle(const V3<R> &lc, R radius) {
+    R dist = lc.length();
+    if (dist < radius) return R(4) * R(PIf);
+    return R(2) * R(PIf) * (R(1) - std::cos(std::asin(radius / dist)));
+  }
+
+  // Light::Sampler::eval (light/surface-light-sampler.cc:86-100,
+  // light/sphere-light-sampler.cc:130-152, light/envmap-light.cc:150-170)
+  LightSample<R> light_eval(const Light<R> &L, const Isec<R> &is, const V3<R> &dir) const {
+    LightSample<R> out;
+    if (L.kind == 0) {
+      const Prim<R> &pr = prims[L.prim];
+      V3<R> wdir = is.nf.from(dir);
+      R t, uu, vv;
+      if (tripar_intersects(pr.v0, pr.e1, pr.e2, pr.parallelogram, is.nf.origin, wdir,
+                            R(0), t, uu, vv)) {
+        V3<R> norm = cross(pr.e2, pr.e1);
+        R area = norm.length();
+        norm = norm / area;
+        if (!pr.parallelogram && uu + vv > 1) { uu = R(1) - uu; vv = R(1) - vv; }
+        if (!pr.parallelogram) area *= R(0.5f);
+        V3<R> pos = pr.v0 + pr.e1 * uu + pr.e2 * vv;
+        R dist, pdf;
+        V3<R> sd;
+        if (angular(pos, norm, R(1) / area, is.nf.origin, dist, sd, pdf)) {
+          out.val = L.intensity;
+          out.pdf = pdf;
+          out.dist = dist;
+        }
+      }
+    } else if (L.kind == 1) {
+      const Prim<R> &pr = prims[L.prim];
+      V3<R> lc = is.nf.to(pr.center - is.nf.origin);
+      if (lc.z >= -pr.radius) {
+        R dist;
+        if (sphere_intersects(V3<R>(0, 0, 0), pr.radius, -lc, dir, R(0), dist)) {
+          out.val = L.intensity;
+          out.pdf = R(1) / sphere_solid_angle(lc, pr.radius);
+          out.dist = dist;
+        }
+      }
+    } else {
+      const Envmap<R> &E = envs[L.env];
+      V3<R> wd = is.nf.from(dir);
+      V3<R> ld = E.frame.to(wd);
+      float mu, mv;
+      Envmap<R>::latlong(ld, mu, mv);
+      out.val = E.lookup(mu, mv);
+      out.pdf = R(E.hist_pdf(mu, mv)) * R(0.25f * INV_PIf);
+      out.dist = 0;
+    }
+    return out;
+  }
+};
+
+}  // namespace orc

@@ -1,0 +1,185 @@
+// Device-side data layout and small math for the gfx950 kernels.
+//
+// Arithmetic follows the reference's fp32 operation order where results
+// must match it bit for bit (ray/triangle tests and octree traversal);
+// the library is built with -ffp-contract=off, IEEE division and square
+// root, and denormals kept, so each written operation is one IEEE
+// operation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sgr {
+
+#define SGR_MAX_DEPTH 64  // octree levels a traversal can hold (checked at build)
+
+// ---------------------------------------------------------------------------
+// Scene view passed to kernels by value.
+// ---------------------------------------------------------------------------
+struct DevMaterial {
+  int type;
+  float color[3];
+  float gloss[3];
+  float m, ior_n, ior_k;
+  int bsdf;      // material index providing the BSDF (-1 = none)
+  int medium;    // 1 if the BSDF material is glass (has a medium)
+  // Cook-Torrance constants (material/cook-torrance.cc:38-70)
+  float diff_weight, inv_diff_weight, inv_gloss_weight;
+  unsigned gloss_layer, have_layers;
+  int mirror_under;  // Mirror: underlying Lambert present
+};
+
+struct DevLight {
+  int kind;        // 0 = tripar area light, 1 = sphere light, 2 = envmap
+  uint32_t prim;
+  float intensity[3];
+};
+
+struct DevMesh {
+  float axis[3];
+  float center[3];
+  int left_handed;
+};
+
+struct SceneView {
+  const uint4 *nodes;        // first_child, mask, surf_begin, surf_count
+  const uint32_t *surfs;     // primitive ids
+  const float4 *tri;         // 3 per triangle-like prim: v0|flags, e1, e2
+  const float4 *sph;         // centre, radius
+  const uint32_t *prim_mat;  // material index per prim
+  const uint32_t *prim_mesh; // mesh index per triangle (0xffffffff others)
+  const DevMesh *meshes;
+  const DevMaterial *mats;
+  const DevLight *lights;
+  const float4 *env_tex;     // rgb_ per texel, row 0 = top
+  const float *env_row_cum;
+  const float *env_col_cum;
+  uint32_t n_tri_like;       // triangles + tripars
+  uint32_t n_tri;            // mesh triangles
+  uint32_t n_prims;
+  uint32_t n_nodes;
+  uint32_t n_surfs;
+  uint32_t n_lights;
+  uint32_t use_triv;
+  uint32_t env_w, env_h, env_hw, env_hh;
+  int has_env;
+  float env_frame[9];
+  float origin[3];
+  float size;
+  float min_trace;
+  float horizon;
+  float bbox_lo[3], bbox_hi[3];
+};
+
+// ---------------------------------------------------------------------------
+// Vector helpers (geometry/vec.h semantics: division multiplies by the
+// reciprocal, unit() likewise, ternary min/max).
+// ---------------------------------------------------------------------------
+struct V3 {
+  float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float a, float b, float c) { return V3{a, b, c}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+  return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float len2(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+__device__ __forceinline__ float vlen(V3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ V3 unit(V3 a) { float l = vlen(a); return a * (1.0f / l); }
+__device__ __forceinline__ V3 vdiv(V3 a, float d) { return a * (1.0f / d); }
+__device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float clamp01f(float v) {
+  v = v < 1.0f ? v : 1.0f;
+  return v > 0.0f ? v : 0.0f;
+}
+__device__ __forceinline__ V3 ld3(const float4 &f) { return V3{f.x, f.y, f.z}; }
+
+struct Frame {
+  V3 o, x, y, z;
+  __device__ __forceinline__ V3 to(V3 v) const { return V3{dot(v, x), dot(v, y), dot(v, z)}; }
+  __device__ __forceinline__ V3 from(V3 v) const {
+    return V3{x.x * v.x + y.x * v.y + z.x * v.z, x.y * v.x + y.y * v.y + z.y * v.z,
+              x.z * v.x + y.z * v.y + z.z * v.z};
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Counter-based sample stream (same definition as the CPU oracle's;
+// splitmix64 finaliser, cycle-walking permutation).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+#define SGR_GOLD 0x9e3779b97f4a7c15ULL
+__device__ __forceinline__ uint64_t pixel_key(uint64_t seed, int px, int py) {
+  uint64_t p = ((uint64_t)(uint32_t)px << 32) | (uint64_t)(uint32_t)py;
+  return mix64(seed ^ mix64(p + SGR_GOLD));
+}
+__device__ __forceinline__ uint64_t path_key(uint64_t pk, uint32_t s) {
+  return mix64(pk ^ (((uint64_t)s + 1) * 0xd1b54a32d192ed03ULL));
+}
+__device__ __forceinline__ uint64_t chan_key(uint64_t pk, uint32_t chan) {
+  return mix64(pk + ((uint64_t)chan + 1) * 0xa0761d6478bd642fULL);
+}
+__device__ __forceinline__ float draw(uint64_t key, uint32_t d) {
+  uint32_t r = (uint32_t)(mix64(key + ((uint64_t)d + 1) * SGR_GOLD) >> 32);
+  return (float)(r >> 8) * (1.0f / 16777216.0f);
+}
+__device__ __forceinline__ uint32_t permute(uint32_t i, uint32_t n, uint64_t key) {
+  if (n <= 1) return 0;
+  uint32_t w = n - 1;
+  w |= w >> 1; w |= w >> 2; w |= w >> 4; w |= w >> 8; w |= w >> 16;
+  uint32_t mask = w;
+  unsigned bits = 32 - __clz(w);
+  unsigned hs = (bits + 1) / 2;
+  uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+  do {
+    i ^= k0 & mask;
+    i = (i * 0x9e3779b1u) & mask;
+    i ^= i >> hs;
+    i = (i * (k1 | 1u)) & mask;
+    i ^= i >> hs;
+    i = (i + (k1 >> 16)) & mask;
+  } while (i >= n);
+  return i;
+}
+struct Grid {
+  uint32_t u, v, total;
+};
+// Stratified UV of channel CHAN, entry E (render/grid.cc:23-60 jittered
+// grid, shuffled per pixel and channel).
+__device__ __forceinline__ void strat_uv(uint64_t pk, uint32_t chan, uint32_t e, Grid g,
+                                         float &u, float &v) {
+  uint64_t ck = chan_key(pk, chan);
+  uint32_t cell = permute(e, g.total, ck);
+  uint32_t j = cell % g.u, i = cell / g.u;
+  float us = 1.0f / (float)g.u, vs = 1.0f / (float)g.v;
+  u = clamp01f((float)j * us + draw(ck, 2 * e) * us);
+  v = clamp01f((float)i * vs + draw(ck, 2 * e + 1) * vs);
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level queue append: ballot + lane prefix + one atomic per wave.
+// Must be called by all active lanes of the wave.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return 0xffffffffu;
+  uint32_t lane = __lane_id();
+  uint32_t prefix = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+  int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  return pred ? base + prefix : 0xffffffffu;
+}
+
+}  // namespace sgr

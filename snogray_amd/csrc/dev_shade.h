@@ -1,0 +1,672 @@
+// Device shading: intersection frames, BSDFs, light samplers and the
+// envmap, in the reference's fp32 arithmetic.
+#pragma once
+#include "dev_trace.h"
+
+namespace sgr {
+
+#define SGR_PIf 3.14159265358979323846f
+#define SGR_INV_PIf 0.31830988618379067154f
+#define SGR_PI_D 3.14159265358979323846
+
+enum : unsigned {  // material/bsdf.h:46-70
+  F_REFLECTIVE = 0x10, F_TRANSMISSIVE = 0x20, F_ALL_DIRECTIONS = 0x30,
+  F_SPECULAR = 0x01, F_GLOSSY = 0x02, F_DIFFUSE = 0x04, F_ALL = 0x37
+};
+
+struct C3 {
+  float r, g, b;
+};
+__device__ __forceinline__ C3 c3(float v) { return C3{v, v, v}; }
+__device__ __forceinline__ C3 c3(float a, float b, float c) { return C3{a, b, c}; }
+__device__ __forceinline__ C3 operator+(C3 a, C3 b) { return C3{a.r + b.r, a.g + b.g, a.b + b.b}; }
+__device__ __forceinline__ C3 operator*(C3 a, C3 b) { return C3{a.r * b.r, a.g * b.g, a.b * b.b}; }
+__device__ __forceinline__ C3 operator*(C3 a, float s) { return C3{a.r * s, a.g * s, a.b * s}; }
+__device__ __forceinline__ float csum(C3 a) { float s = a.r; s += a.g; s += a.b; return s; }
+__device__ __forceinline__ bool cgt0(C3 a) { return csum(a) > 0.0f; }
+__device__ __forceinline__ bool czero(C3 a) { return a.r == 0.0f && a.g == 0.0f && a.b == 0.0f; }
+__device__ __forceinline__ C3 cload(const float *p) { return C3{p[0], p[1], p[2]}; }
+
+// Intersection state (render/intersect.cc:86-136).
+struct Isec {
+  Frame nf, gf;
+  V3 v, gn;
+  bool back;
+  int mat, bmat;
+  float media_ior, encl_ior;
+};
+
+// Hit frames: surface/mesh.cc:563-709, surface/tripar.cc:45-66,
+// surface/sphere.cc make_intersect; then Intersect::finish_init.
+__device__ __forceinline__ void make_isec(const SceneView &S, uint32_t p, const RayF &ray,
+                                          float media_ior, float encl_ior, Isec &is) {
+  V3 point = ray.o + ray.d * ray.t1;
+  Frame nf, gf;
+  if (p < S.n_tri_like) {
+    V3 e1 = ld3(S.tri[3 * p + 1]), e2 = ld3(S.tri[3 * p + 2]);
+    if (p < S.n_tri) {
+      const DevMesh &M = S.meshes[S.prim_mesh[p]];
+      V3 raw = M.left_handed ? cross(e2, e1) : cross(e1, e2);
+      V3 norm = unit(raw);
+      V3 axis = v3(M.axis[0], M.axis[1], M.axis[2]);
+      V3 s = cross(norm, axis);
+      if (len2(s) < 1e-15f) {
+        V3 cent = v3(M.center[0], M.center[1], M.center[2]) - point;
+        s = cross(norm, cent);
+        if (len2(s) < 1e-15f) {  // geometry/vec.h:119-123 perpendicular
+          float nz = (norm.x == norm.z) ? norm.z : -norm.z;
+          s = v3(norm.y * norm.y + norm.x * nz, -nz * nz - norm.x * norm.y,
+                 norm.x * norm.x - norm.y * nz);
+        }
+      }
+      s = unit(s);
+      V3 t = cross(s, norm);
+      gf = Frame{point, s, t, norm};
+      nf = gf;
+    } else {
+      float e1_len = vlen(e1), inv = 1.0f / e1_len;
+      V3 norm = unit(cross(e2, e1));
+      V3 s = e1 * inv;
+      V3 t = cross(s, norm);
+      nf = Frame{point, s, t, norm};
+      gf = nf;
+    }
+  } else {
+    float4 sp = S.sph[p - S.n_tri_like];
+    V3 norm = unit(point - ld3(sp));
+    V3 s = cross(norm, v3(0.0f, 0.0f, 1.0f));
+    if (len2(s) < 1e-15f) {
+      float nz = (norm.x == norm.z) ? norm.z : -norm.z;
+      s = v3(norm.y * norm.y + norm.x * nz, -nz * nz - norm.x * norm.y,
+             norm.x * norm.x - norm.y * nz);
+    }
+    s = unit(s);
+    V3 t = cross(s, norm);
+    nf = Frame{point, s, t, norm};
+    gf = nf;
+  }
+  V3 wv = -unit(ray.d);
+  is.v = nf.to(wv);
+  V3 gv = gf.to(wv);
+  is.back = gv.z < 0.0f;
+  if (is.back) {
+    is.v.z = -is.v.z;
+    nf.z = -nf.z;
+  }
+  is.gn = nf.to(gf.z);
+  is.gn.z = fabsf(is.gn.z);
+  is.nf = nf;
+  is.gf = gf;
+  is.mat = (int)S.prim_mat[p];
+  is.bmat = S.mats[is.mat].bsdf;
+  is.media_ior = media_ior;
+  is.encl_ior = encl_ior;
+}
+
+__device__ __forceinline__ float cos_geom(const Isec &is, V3 d) { return dot(d, is.gn); }
+__device__ __forceinline__ float cos_v(const Isec &is, V3 d) { return fmn(dot(is.v, d), 1.0f); }
+
+// material/fresnel.h:62-130
+__device__ __forceinline__ float fresnel_refl(float n, float k, float cosr) {
+  float c = cosr;
+  c = c < 1.0f ? c : 1.0f;
+  c = c > -1.0f ? c : -1.0f;
+  c = fabsf(c);
+  float ang = acosf(c);
+  float Rs, Rp;
+  if (k == 0.0f) {
+    float st = sinf(ang) / n;
+    st = st < 1.0f ? st : 1.0f;
+    st = st > -1.0f ? st : -1.0f;
+    float ct = sqrtf(1.0f - st * st);
+    float nc1 = n * c, nc2 = n * ct;
+    float Fs = (nc1 - ct) / (nc1 + ct);
+    float Fp = (c - nc2) / (c + nc2);
+    Rs = Fs * Fs;
+    Rp = Fp * Fp;
+  } else {
+    float n2k2 = n * n * k * k, n2mk2 = n * n - k * k;
+    float sr = sinf(ang);
+    float a0 = n2mk2 - sr * sr;
+    float stn = sr * tanf(ang);
+    float common = sqrtf(a0 * a0 + 4.0f * n2k2);
+    float a2 = (common + a0) / 2.0f, b2 = (common - a0) / 2.0f;
+    float apb = a2 + b2, a = sqrtf(a2);
+    float rs1 = apb + c * c, rs2 = 2.0f * a * c;
+    Rs = (rs1 - rs2) / (rs1 + rs2);
+    float rp1 = apb + stn * stn, rp2 = 2.0f * a * stn;
+    Rp = Rs * ((rp1 - rp2) / (rp1 + rp2));
+  }
+  return clamp01f((Rs + Rp) / 2.0f);
+}
+// Fresnel(n1, Ior(n2, k2)).reflectance
+__device__ __forceinline__ float fresnel(float n1, float n2, float k2, float c) {
+  return fresnel_refl(n2 / n1, k2 / n1, c);
+}
+
+// geometry/spherical-coords.h:155-168 via material/dist.h:28-31
+__device__ __forceinline__ V3 z_symm_vec(float cos_theta, float lng) {
+  float sin_lat = cos_theta;
+  float cos_lat = sqrtf(1.0f - sin_lat * sin_lat);
+  float cos_lng = cosf(lng);
+  float sin_lng = sqrtf(1.0f - cos_lng * cos_lng);
+  if (lng < 0.0f || lng > (float)SGR_PI_D) sin_lng = -sin_lng;
+  return v3(cos_lng * cos_lat, -sin_lng * cos_lat, sin_lat);
+}
+__device__ __forceinline__ V3 z_normal_symm_vec(float cos_theta, float v) {
+  return z_symm_vec(cos_theta, v * 2.0f * SGR_PIf);
+}
+
+// material/ward-dist.h:43-76
+__device__ __forceinline__ float ward_pdf(float m, float cos_theta) {
+  if (cos_theta == 0.0f) return 0.0f;
+  float inv_m2 = m == 0.0f ? 0.0f : 1.0f / (m * m);
+  float ic = 1.0f / cos_theta, ic2 = ic * ic, ic3 = ic2 * ic;
+  float tan2 = ic2 - 1.0f;
+  return inv_m2 * SGR_INV_PIf * ic3 * expf(-tan2 * inv_m2);
+}
+
+struct BSample {
+  C3 val;
+  float pdf;
+  V3 dir;
+  unsigned flags;
+};
+struct BValue {
+  C3 val;
+  float pdf;
+};
+
+__device__ __forceinline__ BSample bsample_none() {
+  BSample s;
+  s.val = c3(0.0f);
+  s.pdf = 0.0f;
+  s.dir = v3(0.0f, 0.0f, 0.0f);
+  s.flags = 0;
+  return s;
+}
+
+// material/lambert.cc:38-62
+__device__ __forceinline__ BSample lambert_sample(const Isec &is, C3 col, float u, float v,
+                                                  unsigned flags) {
+  BSample s = bsample_none();
+  if ((flags & (F_REFLECTIVE | F_DIFFUSE)) == (F_REFLECTIVE | F_DIFFUSE)) {
+    float ct = sqrtf(u);
+    float pdf = ct * SGR_INV_PIf;
+    V3 d = z_normal_symm_vec(ct, v);
+    if (d.z > 0.0f && cos_geom(is, d) > 0.0f) {
+      s.val = col * SGR_INV_PIf;
+      s.pdf = pdf;
+      s.dir = d;
+      s.flags = F_REFLECTIVE | F_DIFFUSE;
+    }
+  }
+  return s;
+}
+
+// material/cook-torrance.cc:236-276
+__device__ __forceinline__ C3 ct_val(const Isec &is, const DevMaterial &M, V3 l, V3 h,
+                                     unsigned flags, float ddw, float nv, float inv_4_nv,
+                                     float &pdf) {
+  float nl = l.z;
+  C3 col = c3(0.0f);
+  pdf = 0.0f;
+  if (flags & F_DIFFUSE) {
+    float diff_pdf = nl * SGR_INV_PIf;
+    pdf = pdf + diff_pdf * ddw;
+    col = col + cload(M.color) * SGR_INV_PIf;
+  }
+  if (flags & M.gloss_layer) {
+    float nh = h.z;
+    float inv_nl = (nl != 0.0f) ? (1.0f / nl) : 0.0f;
+    float vh = cos_v(is, h);
+    float G = fmn(2.0f * nh * ((nv > nl) ? nl : nv) / vh, 1.0f);
+    float D = ward_pdf(M.m, nh);
+    float gl = fresnel(is.media_ior, M.ior_n, M.ior_k, vh) * D * G * inv_4_nv * inv_nl;
+    float gl_pdf = D / (4.0f * vh);
+    pdf = pdf + gl_pdf * (1.0f - ddw);
+    col = col + cload(M.gloss) * gl;
+  }
+  return col;
+}
+
+__device__ __forceinline__ BSample ct_sample(const Isec &is, const DevMaterial &M, float u,
+                                             float v, unsigned desired) {
+  BSample s = bsample_none();
+  if (!(desired & F_REFLECTIVE)) return s;
+  desired &= M.have_layers;
+  if (!desired) return s;
+  if (is.v.z < 0.0f) return s;
+  float nv = is.v.z, inv_4_nv = (nv != 0.0f) ? 1.0f / (4.0f * nv) : 0.0f;
+  float ddw, iddw, idgw;
+  if (desired == (F_DIFFUSE | M.gloss_layer)) {
+    ddw = M.diff_weight; iddw = M.inv_diff_weight; idgw = M.inv_gloss_weight;
+  } else if (desired == F_DIFFUSE) {
+    ddw = 1.0f; iddw = 1.0f; idgw = 0.0f;
+  } else {
+    ddw = 0.0f; iddw = 0.0f; idgw = 1.0f;
+  }
+  V3 l, h;
+  unsigned flags = F_REFLECTIVE;
+  if (u < ddw) {
+    if (desired != F_DIFFUSE) u = u * iddw;
+    l = z_normal_symm_vec(sqrtf(u), v);
+    h = unit(is.v + l);
+    flags |= F_DIFFUSE;
+  } else {
+    u = (u - ddw) * idgw;
+    float ct = u >= 1.0f ? 0.0f : 1.0f / sqrtf(M.m * M.m * -logf(1.0f - u) + 1.0f);
+    h = z_normal_symm_vec(ct, v);
+    if (cos_v(is, h) < 0.0f) h = -h;
+    l = h * dot(is.v, h) * 2.0f - is.v;
+    flags |= M.gloss_layer;
+  }
+  if (l.z > 1e-15f && cos_geom(is, l) > 1e-15f) {
+    float pdf;
+    s.val = ct_val(is, M, l, h, desired, ddw, nv, inv_4_nv, pdf);
+    s.pdf = pdf;
+    s.dir = l;
+    s.flags = flags;
+  }
+  return s;
+}
+
+__device__ __forceinline__ BValue ct_eval(const Isec &is, const DevMaterial &M, V3 d,
+                                          unsigned flags) {
+  BValue r;
+  r.val = c3(0.0f);
+  r.pdf = 0.0f;
+  if ((flags & F_REFLECTIVE) && d.z > 0.0f) {
+    flags &= M.have_layers;
+    float ddw = (flags == (F_DIFFUSE | M.gloss_layer)) ? M.diff_weight
+                : (flags == F_DIFFUSE) ? 1.0f : 0.0f;
+    V3 h = unit(is.v + d);
+    float nv = is.v.z, inv_4_nv = (nv != 0.0f) ? 1.0f / (4.0f * nv) : 0.0f;
+    float pdf;
+    r.val = ct_val(is, M, d, h, flags, ddw, nv, inv_4_nv, pdf);
+    r.pdf = pdf;
+  }
+  return r;
+}
+
+// geometry/vec.h:166-186
+__device__ __forceinline__ V3 refraction(V3 v, V3 n, float ior_in, float ior_out) {
+  float ratio = ior_in / ior_out;
+  float c1 = -dot(v, n);
+  float c2 = 1.0f - (ratio * ratio) * (1.0f - c1 * c1);
+  if (c2 < -1e-15f) return v3(0.0f, 0.0f, 0.0f);
+  if (c2 < 0.0f) c2 = 0.0f;
+  return (v * ratio) + (n * (ratio * c1 - sqrtf(c2)));
+}
+
+// material/glass.cc:48-117 (media: innermost ior / enclosing ior)
+__device__ __forceinline__ BSample glass_sample(const Isec &is, const DevMaterial &M,
+                                                float u, unsigned flags) {
+  BSample s = bsample_none();
+  if (flags & F_SPECULAR) {
+    flags &= F_ALL_DIRECTIONS;
+    bool entering = !is.back;
+    float old_ior = entering ? is.media_ior : M.ior_n;
+    float new_ior = entering ? M.ior_n : is.encl_ior;
+    V3 xd = refraction(-is.v, v3(0.0f, 0.0f, 1.0f), old_ior, new_ior);
+    float cx = -xd.z;
+    float cr = fabsf(is.v.z);
+    float xmit = (cx == 0.0f) ? 0.0f : 1.0f - fresnel_refl(old_ior / new_ior, 0.0f, cx);
+    float refl = (cr == 0.0f) ? 0.0f : fresnel_refl(new_ior / old_ior, 0.0f, cr);
+    if (xmit + refl != 0.0f) {
+      float xp = (flags == F_TRANSMISSIVE) ? 1.0f : (flags == F_REFLECTIVE) ? 0.0f
+                 : xmit / (xmit + refl);
+      if (u < xp) {
+        if (cx != 0.0f) xmit = xmit / cx;
+        s.val = c3(xmit);
+        s.pdf = xp;
+        s.dir = xd;
+        s.flags = F_SPECULAR | F_TRANSMISSIVE;
+      } else {
+        if (cr != 0.0f) refl = refl / cr;
+        s.val = c3(refl);
+        s.pdf = 1.0f - xp;
+        s.dir = v3(0.0f, 0.0f, 1.0f) * (is.v.z * 2.0f) - is.v;
+        s.flags = F_SPECULAR | F_REFLECTIVE;
+      }
+    }
+  }
+  return s;
+}
+
+// material/mirror.cc:54-120
+__device__ __forceinline__ BSample mirror_sample(const Isec &is, const DevMaterial &M,
+                                                 float u, float v, unsigned flags) {
+  BSample s = bsample_none();
+  if ((flags & (F_SPECULAR | F_REFLECTIVE)) == (F_SPECULAR | F_REFLECTIVE)) {
+    float cr = is.v.z;
+    if (cr != 0.0f) {
+      float fr = fresnel(is.media_ior, M.ior_n, M.ior_k, cr);
+      C3 refl = cload(M.gloss) * fr;
+      refl = refl * (1.0f / cr);
+      if (csum(refl) > 3e-15f && cos_geom(is, is.v) > 0.0f) {
+        s.val = refl;
+        s.pdf = 1.0f;
+        s.dir = v3(0.0f, 0.0f, 1.0f) * (is.v.z * 2.0f) - is.v;
+        s.flags = F_SPECULAR | F_REFLECTIVE;
+      } else if (M.mirror_under) {
+        s = lambert_sample(is, cload(M.color), u, v, flags);
+        float f2 = fresnel(is.media_ior, M.ior_n, M.ior_k, s.dir.z);
+        C3 rr = cload(M.gloss) * f2;
+        s.val = s.val * C3{1.0f - rr.r, 1.0f - rr.g, 1.0f - rr.b};
+      }
+    }
+  }
+  return s;
+}
+
+__device__ __forceinline__ BSample bsdf_sample(const Isec &is, const DevMaterial &M, float u,
+                                               float v, unsigned flags) {
+  switch (M.type) {
+    case 1: return lambert_sample(is, cload(M.color), u, v, flags);
+    case 2: return ct_sample(is, M, u, v, flags);
+    case 3: return glass_sample(is, M, u, flags);
+    case 4: return mirror_sample(is, M, u, v, flags);
+  }
+  return bsample_none();
+}
+
+__device__ __forceinline__ BValue bsdf_eval(const Isec &is, const DevMaterial &M, V3 d,
+                                            unsigned flags) {
+  BValue r;
+  r.val = c3(0.0f);
+  r.pdf = 0.0f;
+  if (M.type == 1) {
+    if ((flags & (F_DIFFUSE | F_REFLECTIVE)) == (F_DIFFUSE | F_REFLECTIVE) && d.z > 0.0f) {
+      r.val = cload(M.color) * SGR_INV_PIf;
+      r.pdf = d.z * SGR_INV_PIf;
+    }
+  } else if (M.type == 2) {
+    r = ct_eval(is, M, d, flags);
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Envmap (texture/spheremap.h, texture/matrix-linterp.h,
+// geometry/hist-2d-dist.cc, light/envmap-light.cc)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ C3 env_texel(const SceneView &S, uint32_t x, uint32_t y) {
+  float4 t = S.env_tex[(size_t)y * S.env_w + x];
+  return C3{t.x, t.y, t.z};
+}
+__device__ __forceinline__ C3 env_lookup(const SceneView &S, float uu, float vv) {
+  float u = uu - floorf(uu), v = vv - floorf(vv);
+  float x = u * (float)S.env_w - 0.5f, y = v * (float)S.env_h - 0.5f;
+  float xl = floorf(x), yl = floorf(y);
+  float xhf = x - xl, yhf = y - yl;
+  float xlf = 1.0f - xhf, ylf = 1.0f - yhf;
+  int xi = (int)xl, yi = (int)yl;
+  uint32_t xh = (uint32_t)(xi + 1), yh = (uint32_t)(yi + 1);
+  if (xi < 0) xi += (int)S.env_w;
+  if (yi < 0) yi += (int)S.env_h;
+  if (xh >= S.env_w) xh -= S.env_w;
+  if (yh >= S.env_h) yh -= S.env_h;
+  uint32_t xlo = (uint32_t)xi, ylo = (uint32_t)yi;
+  ylo = S.env_h - ylo - 1;
+  yh = S.env_h - yh - 1;
+  C3 a = env_texel(S, xlo, ylo) * (xlf * ylf);
+  C3 b = env_texel(S, xlo, yh) * (xlf * yhf);
+  C3 c = env_texel(S, xh, ylo) * (xhf * ylf);
+  C3 d = env_texel(S, xh, yh) * (xhf * yhf);
+  return ((a + b) + c) + d;
+}
+__device__ __forceinline__ void latlong(V3 d, float &u, float &v) {
+  float lng = atan2f(-d.y, d.x);
+  float z = d.z / sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+  z = z < 1.0f ? z : 1.0f;
+  z = z > -1.0f ? z : -1.0f;
+  float lat = asinf(z);
+  u = clamp01f(lng * SGR_INV_PIf * 0.5f + 0.5f);
+  v = clamp01f(lat * SGR_INV_PIf + 0.5f);
+}
+__device__ __forceinline__ V3 latlong_to_vec(float u, float v) {
+  float lng = (u - 0.5f) * SGR_PIf * 2.0f;
+  float lat = (v - 0.5f) * SGR_PIf;
+  float colat = (float)(SGR_PI_D / 2) - lat;
+  return z_symm_vec(cosf(colat), lng);
+}
+__device__ __forceinline__ V3 env_to(const SceneView &S, V3 w) {
+  return v3(w.x * S.env_frame[0] + w.y * S.env_frame[1] + w.z * S.env_frame[2],
+            w.x * S.env_frame[3] + w.y * S.env_frame[4] + w.z * S.env_frame[5],
+            w.x * S.env_frame[6] + w.y * S.env_frame[7] + w.z * S.env_frame[8]);
+}
+__device__ __forceinline__ V3 env_from(const SceneView &S, V3 l) {
+  return v3(S.env_frame[0] * l.x + S.env_frame[3] * l.y + S.env_frame[6] * l.z,
+            S.env_frame[1] * l.x + S.env_frame[4] * l.y + S.env_frame[7] * l.z,
+            S.env_frame[2] * l.x + S.env_frame[5] * l.y + S.env_frame[8] * l.z);
+}
+__device__ __forceinline__ C3 env_map(const SceneView &S, V3 ld) {
+  float u, v;
+  latlong(ld, u, v);
+  return env_lookup(S, u, v);
+}
+__device__ __forceinline__ float env_bin_pdf(const SceneView &S, uint32_t col, uint32_t row) {
+  float rp = S.env_row_cum[row];
+  if (row != 0) rp -= S.env_row_cum[row - 1];
+  const float *cc = S.env_col_cum + (size_t)row * S.env_hw;
+  float cp = cc[col];
+  if (col != 0) cp -= cc[col - 1];
+  return rp * cp * (float)(S.env_hw * S.env_hh);
+}
+// std::lower_bound: first index with a[i] >= val.
+__device__ __forceinline__ uint32_t lower_bound_f(const float *a, uint32_t n, float val) {
+  uint32_t lo = 0, cnt = n;
+  while (cnt > 0) {
+    uint32_t step = cnt / 2, mid = lo + step;
+    if (a[mid] < val) {
+      lo = mid + 1;
+      cnt -= step + 1;
+    } else {
+      cnt = step;
+    }
+  }
+  return lo;
+}
+__device__ __forceinline__ bool env_hist_sample(const SceneView &S, float pu, float pv,
+                                                float &ou, float &ov, float &pdf) {
+  float u = fmn(pu, 1.0f), v = fmn(pv, 1.0f);
+  uint32_t row = lower_bound_f(S.env_row_cum, S.env_hh, v);
+  if (row == S.env_hh) {
+    pdf = 0.0f;
+    ou = ov = 0.0f;
+    return false;
+  }
+  uint32_t col = lower_bound_f(S.env_col_cum + (size_t)row * S.env_hw, S.env_hw, u);
+  pdf = env_bin_pdf(S, col, row);
+  float cw = 1.0f / (float)S.env_hw, rh = 1.0f / (float)S.env_hh;
+  ou = (float)col * cw + fmodf(pu, cw);
+  ov = (float)row * rh + fmodf(pv, rh);
+  return true;
+}
+__device__ __forceinline__ float env_hist_pdf(const SceneView &S, float u, float v) {
+  int col = (int)(u * (float)S.env_hw);
+  int row = (int)(v * (float)S.env_hh);
+  int cmax = (int)S.env_hw - 1, rmax = (int)S.env_hh - 1;
+  col = col < cmax ? col : cmax;
+  col = col > 0 ? col : 0;
+  row = row < rmax ? row : rmax;
+  row = row > 0 ? row : 0;
+  return env_bin_pdf(S, (uint32_t)col, (uint32_t)row);
+}
+
+// Scene::background (render/scene.cc:62-73)
+__device__ __forceinline__ C3 background(const SceneView &S, V3 dir) {
+  C3 r = c3(0.0f);
+  if (S.has_env) r = r + env_map(S, env_to(S, dir));
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Light samplers
+// ---------------------------------------------------------------------------
+struct LSample {
+  C3 val;
+  float pdf;
+  V3 dir;
+  float dist;
+};
+
+// surface/surface-sampler.cc:26-46
+__device__ __forceinline__ bool angular(V3 pos, V3 normal, float apdf, V3 vp, float &dist,
+                                        V3 &dir, float &pdf) {
+  V3 vv = pos - vp;
+  dist = vlen(vv);
+  pdf = 0.0f;
+  if (dist > 0.0f) {
+    float inv = 1.0f / dist;
+    dir = vv * inv;
+    float adj = dot(-normal, dir);
+    if (adj > 0.0f) {
+      float dwda = adj * (inv * inv);
+      pdf = apdf / dwda;
+    }
+  }
+  return pdf > 0.0f;
+}
+
+// light/sphere-light-sampler.cc:40-48
+__device__ __forceinline__ float sphere_solid_angle(V3 lc, float radius) {
+  float dist = vlen(lc);
+  if (dist < radius) return 4.0f * SGR_PIf;
+  return 2.0f * SGR_PIf * (1.0f - cosf(asinf(radius / dist)));
+}
+
+// Tripar area sample at (u, v) (surface/tripar.cc:210-233).
+__device__ __forceinline__ void tripar_area(V3 v0, V3 e1, V3 e2, bool par, float u, float v,
+                                            V3 &pos, V3 &norm, float &apdf) {
+  norm = cross(e2, e1);
+  float area = vlen(norm);
+  norm = norm * (1.0f / area);
+  if (!par && u + v > 1.0f) {
+    u = 1.0f - u;
+    v = 1.0f - v;
+  }
+  if (!par) area = area * 0.5f;
+  pos = (v0 + e1 * u) + e2 * v;
+  apdf = 1.0f / area;
+}
+
+__device__ __forceinline__ LSample light_sample(const SceneView &S, const DevLight &L,
+                                                const Isec &is, float u, float v,
+                                                float extra) {
+  LSample o;
+  o.val = c3(0.0f);
+  o.pdf = 0.0f;
+  o.dir = v3(0.0f, 0.0f, 0.0f);
+  o.dist = 0.0f;
+  if (L.kind == 0) {  // light/surface-light-sampler.cc:47-64
+    float4 a = S.tri[3 * L.prim], b = S.tri[3 * L.prim + 1], c = S.tri[3 * L.prim + 2];
+    V3 pos, norm;
+    float apdf, dist, pdf;
+    V3 dir;
+    tripar_area(ld3(a), ld3(b), ld3(c), a.w != 0.0f, u, v, pos, norm, apdf);
+    if (angular(pos, norm, apdf, is.nf.o, dist, dir, pdf)) {
+      V3 d = is.nf.to(dir);
+      if (d.z > 0.0f) {
+        o.val = cload(L.intensity);
+        o.pdf = pdf;
+        o.dir = d;
+        o.dist = dist;
+      }
+    }
+  } else if (L.kind == 1) {  // light/sphere-light-sampler.cc:53-92
+    float4 sp = S.sph[L.prim - S.n_tri_like];
+    float radius = sp.w;
+    V3 lc = is.nf.to(ld3(sp) - is.nf.o);
+    if (lc.z >= -radius) {
+      float pdf = 1.0f / sphere_solid_angle(lc, radius);
+      float rs = radius * sqrtf(u);
+      float ang = v * 2.0f * SGR_PIf;
+      float x = rs * cosf(ang), y = rs * sinf(ang);
+      float z = sqrtf(fabsf(radius * radius - x * x - y * y)) * sinf(SGR_PIf * (extra - 0.5f));
+      V3 sv = lc + v3(x, y, z);
+      if (sv.z > 0.0f && cos_geom(is, sv) > 0.0f) {
+        V3 sd = unit(sv);
+        float dist;
+        if (sphere_hit(v3(0.0f, 0.0f, 0.0f), radius, -lc, sd, 0.0f, dist)) {
+          o.val = cload(L.intensity);
+          o.pdf = pdf;
+          o.dir = sd;
+          o.dist = dist;
+        }
+      }
+    }
+  } else {  // light/envmap-light.cc:100-125
+    float mu, mv, pdf;
+    env_hist_sample(S, u, v, mu, mv, pdf);
+    V3 ldir = latlong_to_vec(mu, mv);
+    V3 wd = env_from(S, ldir);
+    V3 d = is.nf.to(wd);
+    if (d.z < 0.0f || cos_geom(is, d) < 0.0f) return o;
+    o.val = env_map(S, ldir);
+    o.pdf = pdf * (0.25f * SGR_INV_PIf);
+    o.dir = d;
+    o.dist = 0.0f;
+  }
+  return o;
+}
+
+__device__ __forceinline__ LSample light_eval(const SceneView &S, const DevLight &L,
+                                              const Isec &is, V3 dir) {
+  LSample o;
+  o.val = c3(0.0f);
+  o.pdf = 0.0f;
+  o.dir = dir;
+  o.dist = 0.0f;
+  if (L.kind == 0) {  // light/surface-light-sampler.cc:86-100
+    float4 a = S.tri[3 * L.prim], b = S.tri[3 * L.prim + 1], c = S.tri[3 * L.prim + 2];
+    bool par = a.w != 0.0f;
+    V3 wdir = is.nf.from(dir);
+    RayF r;
+    r.o = is.nf.o;
+    r.d = wdir;
+    r.t0 = 0.0f;
+    r.t1 = __builtin_inff();
+    float t, uu, vv;
+    if (tripar_hit(ld3(a), ld3(b), ld3(c), par, r, t, uu, vv)) {
+      V3 pos, norm, sd;
+      float apdf, dist, pdf;
+      tripar_area(ld3(a), ld3(b), ld3(c), par, uu, vv, pos, norm, apdf);
+      if (angular(pos, norm, apdf, is.nf.o, dist, sd, pdf)) {
+        o.val = cload(L.intensity);
+        o.pdf = pdf;
+        o.dist = dist;
+      }
+    }
+  } else if (L.kind == 1) {  // light/sphere-light-sampler.cc:130-152
+    float4 sp = S.sph[L.prim - S.n_tri_like];
+    V3 lc = is.nf.to(ld3(sp) - is.nf.o);
+    if (lc.z >= -sp.w) {
+      float dist;
+      if (sphere_hit(v3(0.0f, 0.0f, 0.0f), sp.w, -lc, dir, 0.0f, dist)) {
+        o.val = cload(L.intensity);
+        o.pdf = 1.0f / sphere_solid_angle(lc, sp.w);
+        o.dist = dist;
+      }
+    }
+  } else {  // light/envmap-light.cc:150-170
+    V3 wd = is.nf.from(dir);
+    V3 ld = env_to(S, wd);
+    float mu, mv;
+    latlong(ld, mu, mv);
+    o.val = env_lookup(S, mu, mv);
+    o.pdf = env_hist_pdf(S, mu, mv) * (0.25f * SGR_INV_PIf);
+  }
+  return o;
+}
+
+// render/mis-sample-weight.h:45-77
+__device__ __forceinline__ float mis_weight(float pdf, float other) {
+  float term = 1.0f * pdf, ot = 1.0f * other;
+  if (term > 1e10f) return 1.0f;
+  if (ot > 1e10f) return 0.0f;
+  return (term * pdf) / (term * term + ot * ot);
+}
+
+}  // namespace sgr

@@ -1,0 +1,86 @@
+// Render-time buffers and kernel launchers shared by the host library and
+// the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "dev_common.h"
+
+namespace sgr {
+
+#define SGR_MEDIA_MAX 8       // media stack levels per path
+#define SGR_MAX_SLOTS 64      // shadow-ray slots per path (2 per light sample)
+
+// Path state word.
+#define SGR_ST_LEN_MASK   0xffu
+#define SGR_ST_SPEC_SHIFT 8
+#define SGR_ST_DEPTH_SHIFT 12
+#define SGR_ST_PENDING    (1u << 16)
+#define SGR_ST_DONE       (1u << 17)
+#define SGR_ST_STRAT      (1u << 18)
+
+// Counter block: ray-queue lengths per bounce, then shadow-queue lengths.
+#define SGR_MAX_BOUNCES   64
+#define SGR_CNT_Q         0
+#define SGR_CNT_S         (SGR_MAX_BOUNCES + 2)
+#define SGR_NUM_COUNTERS  (2 * SGR_MAX_BOUNCES + 4)
+
+// Device statistics slots (unsigned long long).
+#define SGR_STAT_NODES_C 0
+#define SGR_STAT_TESTS_C 1
+#define SGR_STAT_NODES_S 2
+#define SGR_STAT_TESTS_S 3
+#define SGR_STAT_ILLUM   4
+#define SGR_NUM_STATS    8
+
+struct RayBatchDev {
+  const float *ox, *oy, *oz, *dx, *dy, *dz, *t0, *t1;
+  uint32_t *prim;
+  float *t, *u, *v;
+  uint8_t *occ;
+};
+
+struct RenderView {
+  uint32_t width, height, spp, minp, maxp, ds, L;
+  uint32_t nsub_strat, nsub_rand, K;
+  Grid g_cam, g_dir, g_cont;
+  uint64_t seed;
+  float bg_alpha, fb, fc, inv_radius, neg_clamp;
+  int fr;
+  int X0, Y0, PW, PH;     // pattern window including the filter margin
+  uint32_t s0, sb;        // this chunk renders samples [s0, s0 + sb)
+  uint32_t npaths;        // PW * PH * sb
+  float cam_pos[3], fwd[3], up[3], right[3];
+  float tanx, tany, max_trace;
+};
+
+struct PathBufs {
+  float *ray[8];      // ox oy oz dx dy dz t0 t1
+  uint32_t *hprim;
+  float *ht, *hu, *hv;
+  float *T[3], *rad[3], *Tp[3];
+  float *rr, *alpha;
+  uint32_t *state;
+  float *media;       // SGR_MEDIA_MAX x npaths
+  float *sray[8];     // shadow rays, K per path
+  float *sc[3];       // shadow contributions
+  uint8_t *svis;
+  uint32_t *q[2];     // ray queues (ping-pong)
+  uint32_t *sq;       // shadow queue
+  uint32_t *counts;   // SGR_NUM_COUNTERS
+  float *out;         // 6 floats per path: sx sy r g b a
+};
+
+void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool any,
+                  unsigned long long *stats, hipStream_t st);
+void launch_raygen(const RenderView &R, const PathBufs &P, hipStream_t st);
+void launch_extend(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
+                   unsigned long long *stats, unsigned grid, hipStream_t st);
+void launch_shade(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
+                  unsigned long long *stats, unsigned grid, hipStream_t st);
+void launch_occlude(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
+                    unsigned long long *stats, unsigned grid, hipStream_t st);
+void launch_gather(const RenderView &R, const PathBufs &P, uint32_t b, unsigned grid,
+                   hipStream_t st);
+void launch_splat(const RenderView &R, const float *out, float *accum, hipStream_t st);
+
+}  // namespace sgr

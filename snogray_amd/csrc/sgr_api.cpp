@@ -1,0 +1,983 @@
+// C ABI of the path-tracing core (include/snoggpu.h): contexts, scene
+// upload with the host octree build, ray batches and the wavefront render
+// driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/snoggpu.h"
+#include "kernels.h"
+#include "octree_build.h"
+
+using namespace sgr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(SGR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct sgr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t bytes = 0;          // device bytes held (scenes + work buffers)
+  // render work buffers, grown on demand
+  void *work = nullptr;
+  size_t work_bytes = 0;
+  unsigned long long *stats = nullptr;
+  int num_cu = 256;
+  bool kernel_timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+struct sgr_scene {
+  sgr_ctx *ctx = nullptr;
+  SceneView view{};
+  std::vector<void *> allocs;
+  uint64_t bytes = 0;
+  FlatOctree oct;
+  float bbox_lo[3], bbox_hi[3];
+  uint32_t n_lights = 0;
+};
+
+namespace {
+
+int dev_alloc(sgr_ctx *ctx, sgr_scene *sc, size_t bytes, void **out) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(out, bytes);
+  if (e != hipSuccess)
+    return fail(SGR_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  if (sc) {
+    sc->allocs.push_back(*out);
+    sc->bytes += bytes;
+  }
+  ctx->bytes += bytes;
+  return SGR_OK;
+}
+
+template<class T>
+int upload(sgr_ctx *ctx, sgr_scene *sc, const std::vector<T> &v, const T **out) {
+  void *p = nullptr;
+  int rc = dev_alloc(ctx, sc, v.size() * sizeof(T), &p);
+  if (rc) return rc;
+  if (!v.empty())
+    HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  *out = (const T *)p;
+  return SGR_OK;
+}
+
+// Envmap light distribution: light map (texture/spheremap.cc:35-71),
+// cos(latitude)-weighted histogram (light/envmap-light.cc:70-95) and its
+// row / column cumulative sums, built in double and stored in float
+// (geometry/hist-2d-dist.cc:52-110).
+void build_env_cdf(const sgr_envmap &em, std::vector<float> &row_cum,
+                   std::vector<float> &col_cum, uint32_t &hw, uint32_t &hh) {
+  uint32_t w = em.width, h = em.height, div = 1;
+  while (w / div > 1024 || h / div > 1024) div++;
+  hw = w / div;
+  hh = h / div;
+  const float scale = 1.f / float(div * div);
+  std::vector<float> lmap((size_t)hw * hh * 3, 0.f);
+  for (uint32_t y = 0; y < h; y++) {
+    for (uint32_t x = 0; x < w; x++) {
+      uint32_t lx = x / div, ly = hh - y / div - 1;
+      if (lx >= hw || ly >= hh) continue;
+      const float *t = em.texels + 3 * ((size_t)y * w + x);
+      float *d = &lmap[3 * ((size_t)ly * hw + lx)];
+      for (int c = 0; c < 3; c++) d[c] = d[c] + t[c] * scale;
+    }
+  }
+  std::vector<float> bins((size_t)hw * hh, 0.f);
+  const double step = 3.14159265358979323846 / hh;
+  double lat = -3.14159265358979323846 / 2 + step / 2;
+  for (uint32_t r = 0; r < hh; r++, lat += step) {
+    double c = std::cos(lat);
+    for (uint32_t k = 0; k < hw; k++) {
+      const float *px = &lmap[3 * ((size_t)r * hw + k)];
+      float s = px[0];
+      s += px[1];
+      s += px[2];
+      bins[(size_t)r * hw + k] += float(double(s / 3.f) * c);
+    }
+  }
+  double total = 0;
+  for (float b : bins) total += double(b);
+  const double inv_total = total == 0 ? 0 : 1 / total;
+  row_cum.assign(hh, 0.f);
+  col_cum.assign((size_t)hw * hh, 0.f);
+  double run = 0;
+  for (uint32_t r = 0; r < hh; r++) {
+    for (uint32_t k = 0; k < hw; k++) run += double(bins[(size_t)r * hw + k]) * inv_total;
+    row_cum[r] = float(run);
+  }
+  for (uint32_t r = 0; r < hh; r++) {
+    double rs = 0;
+    for (uint32_t k = 0; k < hw; k++) rs += double(bins[(size_t)r * hw + k]);
+    const double inv = rs == 0 ? 0 : 1 / rs;
+    double acc = 0;
+    for (uint32_t k = 0; k < hw; k++) {
+      acc += double(bins[(size_t)r * hw + k]) * inv;
+      col_cum[(size_t)r * hw + k] = float(acc);
+    }
+  }
+}
+
+inline float fmin_t(float a, float b) { return a < b ? a : b; }
+inline float fmax_t(float a, float b) { return a > b ? a : b; }
+
+// Grid shape of a channel with n samples (render/grid.cc:26-38).
+Grid grid_shape(uint32_t n) {
+  uint32_t up = 1;
+  while (up * up < n) up++;
+  uint32_t down = up;
+  while (down > 1 && (2 * down - 1) * (2 * down - 1) > 4 * n) down--;
+  return Grid{up, down, up * down};
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sgr_last_error(void) { return g_err.c_str(); }
+
+int sgr_ctx_create(int device, sgr_ctx **out) {
+  if (!out) return fail(SGR_ERR_INVALID, "sgr_ctx_create: null out");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return fail(SGR_ERR_INVALID, "sgr_ctx_create: no such device " + std::to_string(device));
+  HIPCHK(hipSetDevice(device));
+  sgr_ctx *c = new sgr_ctx;
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(SGR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  for (auto &ev : c->ev) (void)hipEventCreate(&ev);
+  if (hipMalloc((void **)&c->stats, SGR_NUM_STATS * sizeof(unsigned long long)) != hipSuccess) {
+    delete c;
+    return fail(SGR_ERR_NOMEM, "sgr_ctx_create: stats buffer");
+  }
+  const char *kt = std::getenv("SGR_KERNEL_TIMING");
+  c->kernel_timing = kt && kt[0] == '1';
+  *out = c;
+  return SGR_OK;
+}
+
+int sgr_ctx_destroy(sgr_ctx *ctx) {
+  if (!ctx) return SGR_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->work) (void)hipFree(ctx->work);
+  if (ctx->stats) (void)hipFree(ctx->stats);
+  for (auto &ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SGR_OK;
+}
+
+int sgr_ctx_memory(sgr_ctx *ctx, uint64_t *bytes) {
+  if (!ctx || !bytes) return fail(SGR_ERR_INVALID, "sgr_ctx_memory: null argument");
+  *bytes = ctx->bytes;
+  return SGR_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Host-side scene preparation: validation, flattened primitives, the
+// octree, light samplers and the envmap distribution.
+struct HostScene {
+  std::vector<DevMaterial> mats;
+  std::vector<float4> tri, sph, env_tex;
+  std::vector<uint32_t> prim_mat, prim_mesh, order;
+  std::vector<DevMesh> meshes;
+  std::vector<DevLight> lights;
+  std::vector<float> row_cum, col_cum;
+  FlatOctree oct;
+  SceneView V{};
+  float blo[3], bhi[3];
+};
+
+int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
+  if (!d) return fail(SGR_ERR_INVALID, "sgr_scene_create: null descriptor");
+  if (d->num_envmaps > 1)
+    return fail(SGR_ERR_INVALID, "sgr_scene_create: at most one envmap light is supported");
+  if (d->accel != SGR_ACCEL_OCTREE && d->accel != SGR_ACCEL_TRIV)
+    return fail(SGR_ERR_INVALID, "sgr_scene_create: unknown accel");
+  const uint32_t nm = d->num_materials;
+  // ---- materials ----
+  std::vector<DevMaterial> &mats = H.mats;
+  mats.resize(nm);
+  for (uint32_t i = 0; i < nm; i++) {
+    const sgr_material &m = d->materials[i];
+    if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_GLOW)
+      return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": unknown type");
+    if (m.type == SGR_MAT_GLOW && m.underlying >= (int)nm)
+      return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": bad underlying");
+    DevMaterial &o = mats[i];
+    std::memset(&o, 0, sizeof o);
+    o.type = m.type;
+    for (int c = 0; c < 3; c++) {
+      o.color[c] = m.color[c];
+      o.gloss[c] = m.gloss_color[c];
+    }
+    o.m = m.m;
+    o.ior_n = m.ior_n;
+    o.ior_k = m.ior_k;
+    if (m.type == SGR_MAT_GLASS && !(m.ior_n > 0))
+      return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": glass ior must be > 0");
+    if (m.type == SGR_MAT_COOK_TORRANCE) {  // material/cook-torrance.cc:38-70
+      float di = ((m.color[0] + m.color[1]) + m.color[2]) / 3.0f;
+      float gi = ((m.gloss_color[0] + m.gloss_color[1]) + m.gloss_color[2]) / 3.0f;
+      o.diff_weight = (di + gi) == 0 ? 0.0f : di / (di + gi);
+      o.inv_diff_weight = o.diff_weight == 0 ? 0.0f : 1.0f / o.diff_weight;
+      o.inv_gloss_weight = o.diff_weight == 1 ? 0.0f : 1.0f / (1.0f - o.diff_weight);
+      o.gloss_layer = m.m < 0.5f ? 0x02u : 0x04u;
+      o.have_layers = (o.diff_weight > 0 ? 0x04u : 0u) | (o.diff_weight < 1 ? o.gloss_layer : 0u);
+    }
+    if (m.type == SGR_MAT_MIRROR) {  // material/mirror.cc:28-33
+      float s = m.color[0];
+      s += m.color[1];
+      s += m.color[2];
+      o.mirror_under = !(s < 3e-15f);
+    }
+  }
+  for (uint32_t i = 0; i < nm; i++) {  // resolve GLOW -> BSDF material
+    int b = (int)i, guard = 0;
+    while (b >= 0 && mats[b].type == SGR_MAT_GLOW && guard++ < 16) b = d->materials[b].underlying;
+    if (b >= 0 && mats[b].type == SGR_MAT_GLOW) b = -1;
+    mats[i].bsdf = b;
+    mats[i].medium = (b >= 0 && mats[b].type == SGR_MAT_GLASS) ? 1 : 0;
+  }
+  // ---- primitives ----
+  uint64_t n_tri = 0;
+  for (uint32_t i = 0; i < d->num_meshes; i++) n_tri += d->meshes[i].num_triangles;
+  uint64_t n_tl = n_tri + d->num_tripars;
+  uint64_t n_prims = n_tl + d->num_spheres;
+  if (n_prims >= (1ull << 28))
+    return fail(SGR_ERR_LIMIT, "sgr_scene_create: more than 2^28 primitives");
+  std::vector<float4> &tri = H.tri, &sph = H.sph;
+  tri.resize((size_t)n_tl * 3);
+  sph.resize(d->num_spheres);
+  std::vector<uint32_t> &prim_mat = H.prim_mat, &prim_mesh = H.prim_mesh, &order = H.order;
+  prim_mat.assign(n_prims, 0);
+  prim_mesh.assign(n_prims, 0xffffffffu);
+  std::vector<DevMesh> &meshes = H.meshes;
+  meshes.resize(d->num_meshes);
+  std::vector<Box3> boxes(n_prims);
+  order.reserve(n_prims);
+  float *blo = H.blo, *bhi = H.bhi;
+  for (int a = 0; a < 3; a++) {
+    blo[a] = 3.4028235e38f;
+    bhi[a] = -3.4028235e38f;
+  }
+  auto grow = [&](const float *lo, const float *hi) {
+    for (int a = 0; a < 3; a++) {
+      blo[a] = fmin_t(blo[a], lo[a]);
+      bhi[a] = fmax_t(bhi[a], hi[a]);
+    }
+  };
+  uint32_t p = 0;
+  for (uint32_t mi = 0; mi < d->num_meshes; mi++) {
+    const sgr_mesh &m = d->meshes[mi];
+    if (m.num_triangles && (!m.vertices || !m.triangles))
+      return fail(SGR_ERR_INVALID, "mesh " + std::to_string(mi) + ": null arrays");
+    float mlo[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f};
+    float mhi[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
+    for (uint32_t k = 0; k < m.num_vertices; k++)
+      for (int a = 0; a < 3; a++) {
+        mlo[a] = fmin_t(mlo[a], m.vertices[3 * k + a]);
+        mhi[a] = fmax_t(mhi[a], m.vertices[3 * k + a]);
+      }
+    if (m.num_vertices) grow(mlo, mhi);
+    DevMesh &dm = meshes[mi];
+    for (int a = 0; a < 3; a++) {
+      dm.axis[a] = m.axis[a];
+      dm.center[a] = (mlo[a] + mhi[a]) / 2;  // geometry/pos.h:123-126
+    }
+    dm.left_handed = m.left_handed ? 1 : 0;
+    for (uint32_t t = 0; t < m.num_triangles; t++, p++) {
+      uint32_t ix[3] = {m.triangles[3 * t], m.triangles[3 * t + 1], m.triangles[3 * t + 2]};
+      for (int c = 0; c < 3; c++)
+        if (ix[c] >= m.num_vertices)
+          return fail(SGR_ERR_INVALID, "mesh " + std::to_string(mi) + ": vertex index out of range");
+      const float *a = m.vertices + 3 * ix[0], *b = m.vertices + 3 * ix[1], *c = m.vertices + 3 * ix[2];
+      // surface/mesh.cc:735-736: corner v0, edges v1-v0, v2-v0.
+      float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+      float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+      tri[3 * p] = make_float4(a[0], a[1], a[2], 0.0f);
+      tri[3 * p + 1] = make_float4(e1[0], e1[1], e1[2], 0.0f);
+      tri[3 * p + 2] = make_float4(e2[0], e2[1], e2[2], 0.0f);
+      uint32_t mat = m.tri_materials ? m.tri_materials[t] : m.material;
+      if (mat >= nm) return fail(SGR_ERR_INVALID, "mesh " + std::to_string(mi) + ": bad material");
+      prim_mat[p] = mat;
+      prim_mesh[p] = mi;
+      Box3 &bx = boxes[p];
+      for (int k = 0; k < 3; k++) {
+        bx.lo[k] = fmin_t(fmin_t(a[k], b[k]), c[k]);
+        bx.hi[k] = fmax_t(fmax_t(a[k], b[k]), c[k]);
+      }
+      // surface/mesh.cc:832-846: degenerate triangles are not inserted.
+      float n0 = e2[1] * e1[2] - e2[2] * e1[1], n1 = e2[2] * e1[0] - e2[0] * e1[2],
+            n2 = e2[0] * e1[1] - e2[1] * e1[0];
+      if (n0 * n0 + n1 * n1 + n2 * n2 > 0) order.push_back(p);
+    }
+  }
+  for (uint32_t i = 0; i < d->num_tripars; i++, p++) {
+    const sgr_tripar &t = d->tripars[i];
+    if (t.material >= nm) return fail(SGR_ERR_INVALID, "tripar " + std::to_string(i) + ": bad material");
+    tri[3 * p] = make_float4(t.v0[0], t.v0[1], t.v0[2], t.parallelogram ? 1.0f : 0.0f);
+    tri[3 * p + 1] = make_float4(t.e1[0], t.e1[1], t.e1[2], 0.0f);
+    tri[3 * p + 2] = make_float4(t.e2[0], t.e2[1], t.e2[2], 0.0f);
+    prim_mat[p] = t.material;
+    // surface/tripar.cc:140-148
+    float c1[3], c2[3], c3[3];
+    for (int k = 0; k < 3; k++) {
+      c1[k] = t.v0[k] + t.e1[k];
+      c2[k] = t.v0[k] + t.e2[k];
+      c3[k] = (t.v0[k] + t.e1[k]) + t.e2[k];
+    }
+    Box3 &bx = boxes[p];
+    for (int k = 0; k < 3; k++) {
+      bx.lo[k] = fmin_t(fmin_t(t.v0[k], c1[k]), c2[k]);
+      bx.hi[k] = fmax_t(fmax_t(t.v0[k], c1[k]), c2[k]);
+      if (t.parallelogram) {
+        bx.lo[k] = fmin_t(bx.lo[k], c3[k]);
+        bx.hi[k] = fmax_t(bx.hi[k], c3[k]);
+      }
+    }
+    grow(bx.lo, bx.hi);
+    order.push_back(p);
+  }
+  for (uint32_t i = 0; i < d->num_spheres; i++, p++) {
+    const sgr_sphere &s = d->spheres[i];
+    if (s.material >= nm) return fail(SGR_ERR_INVALID, "sphere " + std::to_string(i) + ": bad material");
+    sph[i] = make_float4(s.center[0], s.center[1], s.center[2], s.radius);
+    prim_mat[p] = s.material;
+    Box3 &bx = boxes[p];
+    for (int k = 0; k < 3; k++) {
+      bx.lo[k] = s.center[k] - s.radius;
+      bx.hi[k] = s.center[k] + s.radius;
+    }
+    grow(bx.lo, bx.hi);
+    order.push_back(p);
+  }
+  SceneView &V = H.V;
+  std::memset(&V, 0, sizeof V);
+  // ---- acceleration structure ----
+  if (d->accel == SGR_ACCEL_OCTREE) {
+    build_octree(boxes, order, H.oct);
+    if (H.oct.max_depth > SGR_MAX_DEPTH)
+      return fail(SGR_ERR_LIMIT, "sgr_scene_create: octree depth " +
+                                     std::to_string(H.oct.max_depth) + " exceeds " +
+                                     std::to_string(SGR_MAX_DEPTH));
+    if (H.oct.nodes.size() / 4 >= (1ull << 28))
+      return fail(SGR_ERR_LIMIT, "sgr_scene_create: more than 2^28 octree nodes");
+    V.n_nodes = (uint32_t)(H.oct.nodes.size() / 4);
+    for (int a = 0; a < 3; a++) V.origin[a] = H.oct.origin[a];
+    V.size = H.oct.size;
+    V.n_surfs = (uint32_t)H.oct.surf_ids.size();
+  } else {
+    H.oct.num_real = order.size();
+    V.use_triv = 1;
+    V.n_surfs = (uint32_t)order.size();
+  }
+  V.n_tri = (uint32_t)n_tri;
+  V.n_tri_like = (uint32_t)n_tl;
+  V.n_prims = (uint32_t)n_prims;
+  V.min_trace = d->min_trace;
+  {  // render/scene.cc:28 horizon = bbox diameter
+    float ex = bhi[0] - blo[0], ey = bhi[1] - blo[1], ez = bhi[2] - blo[2];
+    V.horizon = p ? std::sqrt(ex * ex + ey * ey + ez * ez) : 0.0f;
+  }
+  for (int a = 0; a < 3; a++) {
+    V.bbox_lo[a] = blo[a];
+    V.bbox_hi[a] = bhi[a];
+  }
+  // ---- lights (render/scene.cc:34, surface/primitive.cc:48-54) ----
+  for (uint32_t q = (uint32_t)n_tri; q < n_prims; q++) {
+    uint32_t m = prim_mat[q];
+    if (mats[m].type == SGR_MAT_GLOW) {
+      DevLight L;
+      L.kind = q < n_tl ? 0 : 1;
+      L.prim = q;
+      for (int c = 0; c < 3; c++) L.intensity[c] = mats[m].color[c];
+      H.lights.push_back(L);
+    }
+  }
+  if (d->num_envmaps == 1) {
+    const sgr_envmap &em = d->envmaps[0];
+    if (!em.texels || em.width == 0 || em.height == 0)
+      return fail(SGR_ERR_INVALID, "envmap: empty");
+    H.env_tex.resize((size_t)em.width * em.height);
+    for (size_t i = 0; i < H.env_tex.size(); i++)
+      H.env_tex[i] = make_float4(em.texels[3 * i], em.texels[3 * i + 1], em.texels[3 * i + 2], 0.f);
+    build_env_cdf(em, H.row_cum, H.col_cum, V.env_hw, V.env_hh);
+    V.env_w = em.width;
+    V.env_h = em.height;
+    V.has_env = 1;
+    for (int k = 0; k < 9; k++) V.env_frame[k] = em.frame[k];
+    DevLight L;
+    L.kind = 2;
+    L.prim = 0;
+    L.intensity[0] = L.intensity[1] = L.intensity[2] = 0;
+    H.lights.push_back(L);
+  }
+  V.n_lights = (uint32_t)H.lights.size();
+  return SGR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sgr_octree_build(const sgr_scene_desc *d, uint32_t *nodes, uint64_t *num_nodes,
+                     uint32_t *surf_ids, uint64_t *num_surf_ids, float *origin_size4,
+                     sgr_octree_stats *stats) {
+  HostScene H;
+  int rc = prepare_scene(d, H);
+  if (rc) return rc;
+  if (num_nodes) *num_nodes = H.oct.nodes.size() / 4;
+  if (num_surf_ids) *num_surf_ids = H.oct.surf_ids.size();
+  if (nodes) std::memcpy(nodes, H.oct.nodes.data(), H.oct.nodes.size() * 4);
+  if (surf_ids) std::memcpy(surf_ids, H.oct.surf_ids.data(), H.oct.surf_ids.size() * 4);
+  if (origin_size4) {
+    for (int a = 0; a < 3; a++) origin_size4[a] = H.oct.origin[a];
+    origin_size4[3] = H.oct.size;
+  }
+  if (stats) {
+    stats->num_nodes = H.oct.nodes.size() / 4;
+    stats->num_leaf_nodes = H.oct.num_leaf;
+    stats->num_surfaces = H.oct.num_surfaces;
+    stats->num_dup_surfaces = H.oct.num_surfaces - H.oct.num_real;
+    stats->max_depth = H.oct.max_depth;
+    stats->avg_depth = H.oct.avg_depth;
+  }
+  return SGR_OK;
+}
+
+int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
+  if (!ctx || !d || !out) return fail(SGR_ERR_INVALID, "sgr_scene_create: null argument");
+  HostScene H;
+  int rc = prepare_scene(d, H);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  sgr_scene *sc = new sgr_scene;
+  sc->ctx = ctx;
+  for (int a = 0; a < 3; a++) {
+    sc->bbox_lo[a] = H.blo[a];
+    sc->bbox_hi[a] = H.bhi[a];
+  }
+  sc->oct = std::move(H.oct);
+  sc->n_lights = H.V.n_lights;
+  SceneView &V = sc->view;
+  V = H.V;
+  std::vector<uint4> nodes4(V.n_nodes);
+  for (uint32_t i = 0; i < V.n_nodes; i++)
+    nodes4[i] = make_uint4(sc->oct.nodes[4 * i], sc->oct.nodes[4 * i + 1],
+                           sc->oct.nodes[4 * i + 2], sc->oct.nodes[4 * i + 3]);
+  const std::vector<uint32_t> &surfs = V.use_triv ? H.order : sc->oct.surf_ids;
+  if (!rc) rc = upload(ctx, sc, nodes4, &V.nodes);
+  if (!rc) rc = upload(ctx, sc, surfs, &V.surfs);
+  if (!rc) rc = upload(ctx, sc, H.tri, &V.tri);
+  if (!rc) rc = upload(ctx, sc, H.sph, &V.sph);
+  if (!rc) rc = upload(ctx, sc, H.prim_mat, &V.prim_mat);
+  if (!rc) rc = upload(ctx, sc, H.prim_mesh, &V.prim_mesh);
+  if (!rc) rc = upload(ctx, sc, H.meshes, &V.meshes);
+  if (!rc) rc = upload(ctx, sc, H.mats, &V.mats);
+  if (!rc) rc = upload(ctx, sc, H.lights, &V.lights);
+  if (!rc) rc = upload(ctx, sc, H.env_tex, &V.env_tex);
+  if (!rc) rc = upload(ctx, sc, H.row_cum, &V.env_row_cum);
+  if (!rc) rc = upload(ctx, sc, H.col_cum, &V.env_col_cum);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = fail(SGR_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+  }
+  if (rc) {
+    sgr_scene_destroy(sc);
+    return rc;
+  }
+  *out = sc;
+  return SGR_OK;
+}
+
+int sgr_scene_destroy(sgr_scene *sc) {
+  if (!sc) return SGR_OK;
+  if (sc->ctx) {
+    (void)hipSetDevice(sc->ctx->device);
+    (void)hipStreamSynchronize(sc->ctx->stream);
+    sc->ctx->bytes -= sc->bytes;
+  }
+  for (void *p : sc->allocs) (void)hipFree(p);
+  delete sc;
+  return SGR_OK;
+}
+
+int sgr_scene_octree_stats(const sgr_scene *sc, sgr_octree_stats *out) {
+  if (!sc || !out) return fail(SGR_ERR_INVALID, "sgr_scene_octree_stats: null argument");
+  out->num_nodes = sc->oct.nodes.size() / 4;
+  out->num_leaf_nodes = sc->oct.num_leaf;
+  out->num_surfaces = sc->oct.num_surfaces;
+  out->num_dup_surfaces = sc->oct.num_surfaces - sc->oct.num_real;
+  out->max_depth = sc->oct.max_depth;
+  out->avg_depth = sc->oct.avg_depth;
+  return SGR_OK;
+}
+
+int sgr_octree_export(const sgr_scene *sc, uint32_t *nodes, uint64_t *num_nodes,
+                      uint32_t *surf_ids, uint64_t *num_surf_ids, float *origin_size4) {
+  if (!sc || !num_nodes || !num_surf_ids)
+    return fail(SGR_ERR_INVALID, "sgr_octree_export: null argument");
+  *num_nodes = sc->oct.nodes.size() / 4;
+  *num_surf_ids = sc->oct.surf_ids.size();
+  if (nodes) std::memcpy(nodes, sc->oct.nodes.data(), sc->oct.nodes.size() * 4);
+  if (surf_ids) std::memcpy(surf_ids, sc->oct.surf_ids.data(), sc->oct.surf_ids.size() * 4);
+  if (origin_size4) {
+    for (int a = 0; a < 3; a++) origin_size4[a] = sc->oct.origin[a];
+    origin_size4[3] = sc->oct.size;
+  }
+  return SGR_OK;
+}
+
+int sgr_scene_bounds(const sgr_scene *sc, float *lo, float *hi, float *horizon) {
+  if (!sc) return fail(SGR_ERR_INVALID, "sgr_scene_bounds: null scene");
+  for (int a = 0; a < 3; a++) {
+    if (lo) lo[a] = sc->bbox_lo[a];
+    if (hi) hi[a] = sc->bbox_hi[a];
+  }
+  if (horizon) *horizon = sc->view.horizon;
+  return SGR_OK;
+}
+
+int sgr_scene_footprint(const sgr_scene *sc, uint64_t *bytes) {
+  if (!sc || !bytes) return fail(SGR_ERR_INVALID, "sgr_scene_footprint: null argument");
+  *bytes = sc->bytes;
+  return SGR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ray batches
+// ---------------------------------------------------------------------------
+static int trace_common(sgr_scene *sc, const sgr_rays *rays, size_t n, sgr_hits *hits,
+                        uint8_t *occ, sgr_stats *stats) {
+  if (!sc || !rays) return fail(SGR_ERR_INVALID, "sgr_trace: null argument");
+  if (n >= (1ull << 31)) return fail(SGR_ERR_LIMIT, "sgr_trace: batch too large");
+  if (n == 0) return SGR_OK;
+  sgr_ctx *ctx = sc->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  size_t in_bytes = n * 8 * sizeof(float);
+  size_t out_bytes = hits ? n * 16 : n;
+  void *buf = nullptr;
+  HIPCHK(hipMalloc(&buf, in_bytes + out_bytes));
+  float *f = (float *)buf;
+  const float *src[8] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->t0, rays->t1};
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 8 && e == hipSuccess; k++)
+    e = hipMemcpyAsync(f + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(ctx->stats, 0, SGR_NUM_STATS * sizeof(unsigned long long), ctx->stream);
+  RayBatchDev rb;
+  rb.ox = f; rb.oy = f + n; rb.oz = f + 2 * n; rb.dx = f + 3 * n; rb.dy = f + 4 * n;
+  rb.dz = f + 5 * n; rb.t0 = f + 6 * n; rb.t1 = f + 7 * n;
+  uint32_t *ob = (uint32_t *)(f + 8 * n);
+  rb.prim = ob;
+  rb.t = (float *)(ob + n);
+  rb.u = (float *)(ob + 2 * n);
+  rb.v = (float *)(ob + 3 * n);
+  rb.occ = (uint8_t *)ob;
+  if (e == hipSuccess) {
+    hipEventRecord(ctx->ev[0], ctx->stream);
+    launch_trace(sc->view, rb, (uint32_t)n, hits == nullptr, ctx->stats, ctx->stream);
+    e = hipGetLastError();
+    hipEventRecord(ctx->ev[1], ctx->stream);
+  }
+  if (e == hipSuccess) {
+    if (hits) {
+      e = hipMemcpyAsync(hits->prim, rb.prim, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(hits->t, rb.t, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(hits->u, rb.u, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(hits->v, rb.v, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    } else {
+      e = hipMemcpyAsync(occ, rb.occ, n, hipMemcpyDeviceToHost, ctx->stream);
+    }
+  }
+  unsigned long long st[SGR_NUM_STATS] = {0};
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(st, ctx->stats, sizeof st, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && stats) {
+    std::memset(stats, 0, sizeof *stats);
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    stats->ms_render = ms;
+    if (hits) {
+      stats->scene_intersect_calls = n;
+      stats->node_visits_closest = st[0];
+      stats->surf_tests_closest = st[1];
+      stats->ms_extend = ms;
+      stats->extend_launches = 1;
+    } else {
+      stats->shadow_tests = n;
+      stats->node_visits_shadow = st[0];
+      stats->surf_tests_shadow = st[1];
+      stats->ms_shadow = ms;
+      stats->shadow_launches = 1;
+    }
+  }
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(SGR_ERR_HIP, std::string("sgr_trace: ") + hipGetErrorString(e));
+  return SGR_OK;
+}
+
+int sgr_trace_closest(sgr_scene *sc, const sgr_rays *rays, size_t n, sgr_hits *hits,
+                      sgr_stats *stats) {
+  if (!hits) return fail(SGR_ERR_INVALID, "sgr_trace_closest: null hits");
+  return trace_common(sc, rays, n, hits, nullptr, stats);
+}
+
+int sgr_trace_occluded(sgr_scene *sc, const sgr_rays *rays, size_t n, uint8_t *occluded,
+                       sgr_stats *stats) {
+  if (!occluded) return fail(SGR_ERR_INVALID, "sgr_trace_occluded: null output");
+  return trace_common(sc, rays, n, nullptr, occluded, stats);
+}
+
+// ---------------------------------------------------------------------------
+// Rendering
+// ---------------------------------------------------------------------------
+namespace {
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Carve the path buffers for CAP paths with K shadow slots each out of
+// one allocation.
+size_t path_bytes(size_t cap, size_t K, PathBufs *P, char *base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> char * {
+    char *r = base ? base + off : nullptr;
+    off += align_up(bytes);
+    return r;
+  };
+  for (int k = 0; k < 8; k++) {
+    char *r = take(cap * 4);
+    if (P) P->ray[k] = (float *)r;
+  }
+  char *r = take(cap * 4); if (P) P->hprim = (uint32_t *)r;
+  r = take(cap * 4); if (P) P->ht = (float *)r;
+  r = take(cap * 4); if (P) P->hu = (float *)r;
+  r = take(cap * 4); if (P) P->hv = (float *)r;
+  for (int k = 0; k < 3; k++) { r = take(cap * 4); if (P) P->T[k] = (float *)r; }
+  for (int k = 0; k < 3; k++) { r = take(cap * 4); if (P) P->rad[k] = (float *)r; }
+  for (int k = 0; k < 3; k++) { r = take(cap * 4); if (P) P->Tp[k] = (float *)r; }
+  r = take(cap * 4); if (P) P->rr = (float *)r;
+  r = take(cap * 4); if (P) P->alpha = (float *)r;
+  r = take(cap * 4); if (P) P->state = (uint32_t *)r;
+  r = take(cap * 4 * SGR_MEDIA_MAX); if (P) P->media = (float *)r;
+  size_t ns = cap * (K ? K : 1);
+  for (int k = 0; k < 8; k++) { r = take(ns * 4); if (P) P->sray[k] = (float *)r; }
+  for (int k = 0; k < 3; k++) { r = take(ns * 4); if (P) P->sc[k] = (float *)r; }
+  r = take(ns); if (P) P->svis = (uint8_t *)r;
+  r = take(cap * 4); if (P) P->q[0] = (uint32_t *)r;
+  r = take(cap * 4); if (P) P->q[1] = (uint32_t *)r;
+  r = take(ns * 4); if (P) P->sq = (uint32_t *)r;
+  r = take(SGR_NUM_COUNTERS * 4); if (P) P->counts = (uint32_t *)r;
+  r = take(cap * 6 * 4); if (P) P->out = (float *)r;
+  return off;
+}
+
+struct RenderPlan {
+  RenderView R;
+  uint32_t sb_max;
+  uint32_t npix;
+};
+
+int plan_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p, RenderPlan &pl) {
+  if (!sc || !cam || !p) return fail(SGR_ERR_INVALID, "sgr_render: null argument");
+  if (p->width == 0 || p->height == 0 || p->spp == 0)
+    return fail(SGR_ERR_INVALID, "sgr_render: width, height and spp must be > 0");
+  if (p->max_path_len + 2 > SGR_MAX_BOUNCES)
+    return fail(SGR_ERR_LIMIT, "sgr_render: max_path_len above " + std::to_string(SGR_MAX_BOUNCES - 2));
+  if (!(p->filter_radius > 0.5001f))
+    return fail(SGR_ERR_INVALID, "sgr_render: filter_radius must exceed 0.5");
+  RenderView &R = pl.R;
+  std::memset(&R, 0, sizeof R);
+  uint32_t x0 = p->x0, y0 = p->y0, w = p->w, h = p->h;
+  if (w == 0) { x0 = 0; y0 = 0; w = p->width; h = p->height; }
+  if (h == 0 || (uint64_t)x0 + w > p->width || (uint64_t)y0 + h > p->height)
+    return fail(SGR_ERR_INVALID, "sgr_render: window outside the image");
+  int fr = (int)std::ceil(p->filter_radius - 0.5001f);  // image/image-filter-conv.h:39-40
+  R.width = p->width; R.height = p->height; R.spp = p->spp;
+  R.minp = p->min_path_len; R.maxp = p->max_path_len; R.ds = p->direct_samples;
+  R.L = sc->n_lights;
+  uint32_t ds = R.ds ? R.ds : 1;
+  R.g_cam = grid_shape(R.spp);
+  R.g_cont = grid_shape(R.spp);
+  R.g_dir = grid_shape(R.spp * ds);
+  R.nsub_strat = R.g_dir.total / R.spp;      // render/sample-set.h:330-345
+  R.nsub_rand = grid_shape(ds).total;
+  uint32_t nsub_max = std::max(R.nsub_strat, R.nsub_rand);
+  R.K = 2 * R.L * nsub_max;
+  if (R.K > SGR_MAX_SLOTS)
+    return fail(SGR_ERR_LIMIT, "sgr_render: lights x direct samples needs " + std::to_string(R.K) +
+                                   " shadow slots per path (limit " + std::to_string(SGR_MAX_SLOTS) + ")");
+  R.seed = p->seed;
+  R.bg_alpha = p->bg_alpha;
+  R.fb = p->filter_b; R.fc = p->filter_c;
+  R.inv_radius = 1.0f / p->filter_radius;
+  R.neg_clamp = -std::fabs(p->neg_clamp);
+  R.fr = fr;
+  R.X0 = (int)x0 - fr; R.Y0 = (int)y0 - fr;
+  R.PW = (int)w + 2 * fr; R.PH = (int)h + 2 * fr;
+  for (int a = 0; a < 3; a++) {
+    R.cam_pos[a] = cam->pos[a]; R.fwd[a] = cam->forward[a];
+    R.up[a] = cam->up[a]; R.right[a] = cam->right[a];
+  }
+  R.tanx = cam->tan_half_fov_x; R.tany = cam->tan_half_fov_y;
+  // Renderer::render_packet: max_trace = (scene bbox + camera).diameter()
+  float lo[3], hi[3];
+  for (int a = 0; a < 3; a++) {
+    lo[a] = fmin_t(sc->bbox_lo[a], cam->pos[a]);
+    hi[a] = fmax_t(sc->bbox_hi[a], cam->pos[a]);
+  }
+  float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+  R.max_trace = std::sqrt(ex * ex + ey * ey + ez * ez);
+  pl.npix = (uint32_t)(R.PW * R.PH);
+  uint64_t want = p->paths_in_flight ? p->paths_in_flight : (1u << 22);
+  uint64_t sb = want / pl.npix;
+  if (sb < 1) sb = 1;
+  if (sb > R.spp) sb = R.spp;
+  if ((uint64_t)pl.npix * sb >= (1ull << 31))
+    return fail(SGR_ERR_LIMIT, "sgr_render: image too large for one chunk");
+  pl.sb_max = (uint32_t)sb;
+  return SGR_OK;
+}
+
+int ensure_work(sgr_ctx *ctx, size_t bytes) {
+  if (ctx->work_bytes >= bytes) return SGR_OK;
+  if (ctx->work) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->work);
+    ctx->bytes -= ctx->work_bytes;
+    ctx->work = nullptr;
+    ctx->work_bytes = 0;
+  }
+  hipError_t e = hipMalloc(&ctx->work, bytes);
+  if (e != hipSuccess) return fail(SGR_ERR_NOMEM, std::string("render buffers: ") + hipGetErrorString(e));
+  ctx->work_bytes = bytes;
+  ctx->bytes += bytes;
+  return SGR_OK;
+}
+
+// Runs the chunks; per chunk, optional host copy of the per-sample output.
+int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_host,
+               sgr_stats *stats) {
+  sgr_ctx *ctx = sc->ctx;
+  RenderView R = pl.R;
+  size_t cap = (size_t)pl.npix * pl.sb_max;
+  size_t need = path_bytes(cap, R.K, nullptr, nullptr);
+  int rc = ensure_work(ctx, need);
+  if (rc) return rc;
+  PathBufs P;
+  path_bytes(cap, R.K, &P, (char *)ctx->work);
+  HIPCHK(hipMemsetAsync(ctx->stats, 0, SGR_NUM_STATS * sizeof(unsigned long long), ctx->stream));
+  unsigned grid = (unsigned)std::min<size_t>((cap + 255) / 256, (size_t)ctx->num_cu * 16);
+  if (grid == 0) grid = 1;
+  std::vector<float> chunk_out;
+  std::vector<uint32_t> counts(SGR_NUM_COUNTERS);
+  uint64_t isect = 0, shadow = 0;
+  uint32_t n_ext = 0, n_occ = 0;
+  double ms_ext = 0, ms_shade = 0, ms_occ = 0, ms_splat = 0, ms_other = 0;
+  const bool kt = ctx->kernel_timing && stats;
+  auto t_begin = [&]() { if (kt) hipEventRecord(ctx->ev[2], ctx->stream); };
+  auto t_end = [&](double &acc) {
+    if (kt) {
+      hipEventRecord(ctx->ev[3], ctx->stream);
+      hipEventSynchronize(ctx->ev[3]);
+      float ms = 0;
+      hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+      acc += ms;
+    }
+  };
+  HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+  for (uint32_t s0 = 0; s0 < R.spp; s0 += pl.sb_max) {
+    R.s0 = s0;
+    R.sb = std::min(pl.sb_max, R.spp - s0);
+    R.npaths = pl.npix * R.sb;
+    t_begin();
+    launch_raygen(R, P, ctx->stream);
+    t_end(ms_other);
+    for (uint32_t b = 0; b <= R.maxp; b++) {
+      t_begin();
+      launch_extend(sc->view, R, P, b, ctx->stats, grid, ctx->stream);
+      t_end(ms_ext);
+      t_begin();
+      launch_shade(sc->view, R, P, b, ctx->stats, grid, ctx->stream);
+      t_end(ms_shade);
+      t_begin();
+      launch_occlude(sc->view, R, P, b, ctx->stats, grid, ctx->stream);
+      t_end(ms_occ);
+      t_begin();
+      launch_gather(R, P, b, grid, ctx->stream);
+      t_end(ms_other);
+      n_ext++;
+      n_occ++;
+    }
+    if (accum_dev) {
+      t_begin();
+      launch_splat(R, P.out, accum_dev, ctx->stream);
+      t_end(ms_splat);
+    }
+    HIPCHK(hipGetLastError());
+    if (stats) {
+      HIPCHK(hipMemcpyAsync(counts.data(), P.counts, SGR_NUM_COUNTERS * 4, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    }
+    if (samples_host) {
+      chunk_out.resize((size_t)R.npaths * 6);
+      HIPCHK(hipMemcpyAsync(chunk_out.data(), P.out, chunk_out.size() * 4, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    }
+    if (stats || samples_host) HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (stats)
+      for (uint32_t b = 0; b <= R.maxp; b++) {
+        isect += counts[SGR_CNT_Q + b];
+        shadow += counts[SGR_CNT_S + b];
+      }
+    if (samples_host) {
+      for (uint32_t pix = 0; pix < pl.npix; pix++)
+        std::memcpy(samples_host + ((size_t)pix * R.spp + s0) * 6, &chunk_out[(size_t)pix * R.sb * 6],
+                    (size_t)R.sb * 6 * 4);
+    }
+  }
+  HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+  unsigned long long st[SGR_NUM_STATS];
+  HIPCHK(hipMemcpyAsync(st, ctx->stats, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    uint32_t w = pl.R.PW - 2 * pl.R.fr, h = pl.R.PH - 2 * pl.R.fr;
+    stats->eye_rays = (uint64_t)pl.npix * R.spp;
+    stats->paths_traced = (uint64_t)pl.npix * R.spp;
+    stats->samples = (uint64_t)w * h * R.spp;
+    stats->scene_intersect_calls = isect;
+    stats->shadow_tests = shadow;
+    stats->node_visits_closest = st[SGR_STAT_NODES_C];
+    stats->surf_tests_closest = st[SGR_STAT_TESTS_C];
+    stats->node_visits_shadow = st[SGR_STAT_NODES_S];
+    stats->surf_tests_shadow = st[SGR_STAT_TESTS_S];
+    stats->illum_calls = st[SGR_STAT_ILLUM];
+    stats->ms_render = ms;
+    stats->ms_extend = ms_ext;
+    stats->ms_shade = ms_shade;
+    stats->ms_shadow = ms_occ;
+    stats->ms_splat = ms_splat;
+    stats->ms_other = ms_other;
+    stats->extend_launches = n_ext;
+    stats->shadow_launches = n_occ;
+  }
+  return SGR_OK;
+}
+
+}  // namespace
+
+int sgr_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p, float *accum,
+               int accum_on_device, sgr_stats *stats) {
+  RenderPlan pl;
+  int rc = plan_render(sc, cam, p, pl);
+  if (rc) return rc;
+  if (!accum) return fail(SGR_ERR_INVALID, "sgr_render: null accumulators");
+  sgr_ctx *ctx = sc->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  size_t abytes = (size_t)p->width * p->height * 5 * sizeof(float);
+  float *dacc = accum;
+  if (!accum_on_device) {
+    HIPCHK(hipMalloc((void **)&dacc, abytes));
+    hipError_t e = hipMemcpyAsync(dacc, accum, abytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(dacc);
+      return fail(SGR_ERR_HIP, std::string("sgr_render: ") + hipGetErrorString(e));
+    }
+  }
+  rc = run_render(sc, pl, dacc, nullptr, stats);
+  if (!accum_on_device) {
+    if (!rc) {
+      hipError_t e = hipMemcpy(accum, dacc, abytes, hipMemcpyDeviceToHost);
+      if (e != hipSuccess) rc = fail(SGR_ERR_HIP, std::string("sgr_render: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(dacc);
+  }
+  return rc;
+}
+
+int sgr_render_samples(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p,
+                       float *out, sgr_stats *stats) {
+  RenderPlan pl;
+  int rc = plan_render(sc, cam, p, pl);
+  if (rc) return rc;
+  if (!out) return fail(SGR_ERR_INVALID, "sgr_render_samples: null output");
+  HIPCHK(hipSetDevice(sc->ctx->device));
+  return run_render(sc, pl, nullptr, out, stats);
+}
+
+// ImageSampledOutput::set_raw_min_y (image/image-sampled-output.cc:63-126)
+int sgr_finalize_image(const float *accum, uint32_t width, uint32_t height, float exposure,
+                       float contrast, float *rgba) {
+  if (!accum || !rgba) return fail(SGR_ERR_INVALID, "sgr_finalize_image: null argument");
+  for (size_t i = 0; i < (size_t)width * height; i++) {
+    const float *a = accum + 5 * i;
+    float c[3] = {a[0], a[1], a[2]};
+    float alpha = a[3], w = a[4];
+    if (w > 0 && w != 1) {
+      float iw = 1 / w;
+      for (float &x : c) x *= iw;
+      alpha *= iw;
+    }
+    alpha = alpha < 1.f ? alpha : 1.f;
+    alpha = alpha > 0.f ? alpha : 0.f;
+    for (float &x : c) x = x > 0.f ? x : 0.f;
+    if (alpha != 0 && alpha != 1) {
+      float ia = 1 / alpha;
+      for (float &x : c) x *= ia;
+    }
+    if (exposure != 1)
+      for (float &x : c) x *= exposure;
+    if (contrast != 1)
+      for (float &x : c) x = std::pow(x > 0.f ? x : 0.f, contrast);
+    float *o = rgba + 4 * i;
+    o[0] = c[0]; o[1] = c[1]; o[2] = c[2]; o[3] = alpha;
+  }
+  return SGR_OK;
+}
+
+}  // extern "C"

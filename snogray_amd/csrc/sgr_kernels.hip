@@ -1,0 +1,490 @@
+// gfx950 kernels of the path-tracing core.
+//
+// Ray batches: k_trace_closest / k_trace_any (Space::intersect /
+// Space::occludes, space/space.cc:55-66, :182-194).
+//
+// Wavefront path tracer (PathInteg::Li, render/path-integ.cc:133-407,
+// restructured): one path per (pattern pixel, sample) of a chunk of
+// samples; per bounce
+//   k_extend   closest hit of every ray in the ray queue
+//   k_shade    emission, direct lighting (DirectIllum::sample_light,
+//              render/direct-illum.cc:126-255: one light sample and one
+//              BSDF sample per light, power-heuristic MIS), BSDF
+//              continuation, Russian roulette; appends shadow rays and the
+//              next ray queue with wave ballots + prefix sums
+//   k_occlude  any hit of every queued shadow ray
+//   k_gather   adds the unoccluded direct terms in the reference's order,
+//              writes finished paths
+// then k_splat gathers the chunk's samples into the image accumulators
+// with the Mitchell filter (image/image-filter-conv.h:109-186).
+#include "dev_shade.h"
+#include "kernels.h"
+
+namespace sgr {
+
+// ---------------------------------------------------------------------------
+// Ray batches
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_trace_closest(SceneView S, RayBatchDev rb, uint32_t n,
+                                                       unsigned long long *stats) {
+  TraceCount cnt{0, 0};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    RayF r;
+    r.o = v3(rb.ox[i], rb.oy[i], rb.oz[i]);
+    r.d = v3(rb.dx[i], rb.dy[i], rb.dz[i]);
+    r.t0 = rb.t0[i];
+    r.t1 = rb.t1[i];
+    float u = 0.0f, v = 0.0f;
+    uint32_t p = trace_ray<false>(S, r, u, v, cnt);
+    rb.prim[i] = p;
+    rb.t[i] = p == SGR_MISS ? 0.0f : r.t1;
+    rb.u[i] = u;
+    rb.v[i] = v;
+  }
+  wave_count(&stats[0], cnt.nodes);
+  wave_count(&stats[1], cnt.tests);
+}
+
+__global__ void __launch_bounds__(256) k_trace_any(SceneView S, RayBatchDev rb, uint32_t n,
+                                                   unsigned long long *stats) {
+  TraceCount cnt{0, 0};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    RayF r;
+    r.o = v3(rb.ox[i], rb.oy[i], rb.oz[i]);
+    r.d = v3(rb.dx[i], rb.dy[i], rb.dz[i]);
+    r.t0 = rb.t0[i];
+    r.t1 = rb.t1[i];
+    float u, v;
+    uint32_t p = trace_ray<true>(S, r, u, v, cnt);
+    rb.occ[i] = p == SGR_MISS ? 0 : 1;
+  }
+  wave_count(&stats[0], cnt.nodes);
+  wave_count(&stats[1], cnt.tests);
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront path tracer
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void path_ident(const RenderView &R, uint32_t i, int &px, int &py,
+                                           uint32_t &s) {
+  uint32_t pix = i / R.sb;
+  s = R.s0 + (i - pix * R.sb);
+  px = R.X0 + (int)(pix % (uint32_t)R.PW);
+  py = R.Y0 + (int)(pix / (uint32_t)R.PW);
+}
+
+__device__ __forceinline__ void load_ray(const PathBufs &P, uint32_t p, RayF &r) {
+  r.o = v3(P.ray[0][p], P.ray[1][p], P.ray[2][p]);
+  r.d = v3(P.ray[3][p], P.ray[4][p], P.ray[5][p]);
+  r.t0 = P.ray[6][p];
+  r.t1 = P.ray[7][p];
+}
+__device__ __forceinline__ void store_ray(const PathBufs &P, uint32_t p, const RayF &r) {
+  P.ray[0][p] = r.o.x; P.ray[1][p] = r.o.y; P.ray[2][p] = r.o.z;
+  P.ray[3][p] = r.d.x; P.ray[4][p] = r.d.y; P.ray[5][p] = r.d.z;
+  P.ray[6][p] = r.t0; P.ray[7][p] = r.t1;
+}
+
+// Camera ray (render-mgr/renderer.cc:71-85, camera/camera.h:322-389) and
+// path-state reset.  Thread 0 also resets the queue counters.
+__global__ void __launch_bounds__(256) k_raygen(RenderView R, PathBufs P) {
+  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid < SGR_NUM_COUNTERS) P.counts[gid] = gid == 0 ? R.npaths : 0u;
+  for (uint32_t i = gid; i < R.npaths; i += gridDim.x * blockDim.x) {
+    int px, py;
+    uint32_t s;
+    path_ident(R, i, px, py, s);
+    uint64_t pk = pixel_key(R.seed, px, py);
+    float cu, cv;
+    strat_uv(pk, 0, s, R.g_cam, cu, cv);
+    float cx = (float)px + cu, cy = (float)py + cv;
+    float fu = cx / (float)R.width, fv = ((float)R.height - cy) / (float)R.height;
+    float x = 2.0f * fu - 1.0f, y = 2.0f * fv - 1.0f;
+    V3 fwd = v3(R.fwd[0], R.fwd[1], R.fwd[2]);
+    V3 right = v3(R.right[0], R.right[1], R.right[2]);
+    V3 up = v3(R.up[0], R.up[1], R.up[2]);
+    RayF r;
+    r.o = v3(R.cam_pos[0], R.cam_pos[1], R.cam_pos[2]);
+    r.d = (fwd + (right * x) * R.tanx) + (up * y) * R.tany;
+    r.t0 = 0.0f;
+    r.t1 = R.max_trace;
+    store_ray(P, i, r);
+    P.T[0][i] = 1.0f; P.T[1][i] = 1.0f; P.T[2][i] = 1.0f;
+    P.rad[0][i] = 0.0f; P.rad[1][i] = 0.0f; P.rad[2][i] = 0.0f;
+    P.rr[i] = 1.0f;
+    P.alpha[i] = 1.0f;
+    P.state[i] = 1u << SGR_ST_DEPTH_SHIFT;  // path_len 0, media depth 1
+    P.media[i] = 1.0f;                      // default medium, ior 1
+    P.q[0][i] = i;
+    P.out[6 * (size_t)i + 0] = cx;
+    P.out[6 * (size_t)i + 1] = cy;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_extend(SceneView S, PathBufs P, uint32_t b,
+                                                unsigned long long *stats) {
+  TraceCount cnt{0, 0};
+  const uint32_t n = P.counts[SGR_CNT_Q + b];
+  const uint32_t *q = P.q[b & 1];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    uint32_t p = q[j];
+    RayF r;
+    load_ray(P, p, r);
+    float u = 0.0f, v = 0.0f;
+    uint32_t h = trace_ray<false>(S, r, u, v, cnt);
+    P.hprim[p] = h;
+    P.ht[p] = r.t1;
+    P.hu[p] = u;
+    P.hv[p] = v;
+  }
+  wave_count(&stats[SGR_STAT_NODES_C], cnt.nodes);
+  wave_count(&stats[SGR_STAT_TESTS_C], cnt.tests);
+}
+
+__global__ void __launch_bounds__(256) k_occlude(SceneView S, PathBufs P, uint32_t b,
+                                                 unsigned long long *stats) {
+  TraceCount cnt{0, 0};
+  const uint32_t n = P.counts[SGR_CNT_S + b];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    uint32_t slot = P.sq[j];
+    RayF r;
+    r.o = v3(P.sray[0][slot], P.sray[1][slot], P.sray[2][slot]);
+    r.d = v3(P.sray[3][slot], P.sray[4][slot], P.sray[5][slot]);
+    r.t0 = P.sray[6][slot];
+    r.t1 = P.sray[7][slot];
+    float u, v;
+    uint32_t h = trace_ray<true>(S, r, u, v, cnt);
+    P.svis[slot] = h == SGR_MISS ? 1 : 0;
+  }
+  wave_count(&stats[SGR_STAT_NODES_S], cnt.nodes);
+  wave_count(&stats[SGR_STAT_TESTS_S], cnt.tests);
+}
+
+// One bounce of PathInteg::Li for every path in the ray queue.
+__global__ void __launch_bounds__(256) k_shade(SceneView S, RenderView R, PathBufs P, uint32_t b,
+                                               unsigned long long *stats) {
+  const uint32_t n = P.counts[SGR_CNT_Q + b];
+  const uint32_t *q = P.q[b & 1];
+  uint32_t *qn = P.q[(b + 1) & 1];
+  uint32_t *qn_count = &P.counts[SGR_CNT_Q + b + 1];
+  uint32_t *sq_count = &P.counts[SGR_CNT_S + b];
+  uint32_t illum = 0;
+  const uint32_t N = R.npaths;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    uint32_t p = q[j];
+    uint32_t st = P.state[p];
+    uint32_t path_len = st & SGR_ST_LEN_MASK;
+    bool after_spec = (st >> SGR_ST_SPEC_SHIFT) & 1u;
+    uint32_t depth = (st >> SGR_ST_DEPTH_SHIFT) & 0xfu;
+    bool include = path_len == 0 || after_spec || R.ds == 0;
+    C3 T = c3(P.T[0][p], P.T[1][p], P.T[2][p]);
+    C3 rad = c3(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
+    float alpha = P.alpha[p];
+    RayF ray;
+    load_ray(P, p, ray);
+    uint32_t prim = P.hprim[p];
+    bool done = false, cont = false;
+    uint32_t nst = st & ~(SGR_ST_PENDING | SGR_ST_DONE | SGR_ST_STRAT);
+    if (prim == SGR_MISS) {
+      if (include) rad = rad + background(S, ray.d) * T;
+      if (path_len == 0 && czero(rad)) alpha = R.bg_alpha;
+      done = true;
+    } else {
+      ray.t1 = P.ht[p];
+      Isec is;
+      float mtop = P.media[(size_t)(depth - 1) * N + p];
+      float mencl = depth >= 2 ? P.media[(size_t)(depth - 2) * N + p] : 1.0f;
+      make_isec(S, prim, ray, mtop, mencl, is);
+      const DevMaterial &HM = S.mats[is.mat];
+      if (include && HM.type == 5 && !is.back) rad = rad + cload(HM.color) * T;
+      if (is.bmat < 0) {
+        done = true;
+      } else {
+        const DevMaterial &M = S.mats[is.bmat];
+        int px, py;
+        uint32_t s;
+        path_ident(R, p, px, py, s);
+        uint64_t pk = pixel_key(R.seed, px, py);
+        uint64_t qk = path_key(pk, s);
+        // ---- direct lighting (DirectIllum::sample_all_lights) ----
+        if (R.ds != 0) {
+          illum++;
+          const unsigned dflags = F_ALL & ~F_SPECULAR;
+          bool strat = path_len < R.minp;
+          uint32_t nsub = strat ? R.nsub_strat : R.nsub_rand;
+          for (uint32_t l = 0; l < R.L; l++) {
+            const DevLight &L = S.lights[l];
+            for (uint32_t jj = 0; jj < nsub; jj++) {
+              float lu, lv, bu, bv;
+              uint32_t base = 16 + (l * nsub + jj) * 8;
+              if (strat) {
+                uint32_t chan = 1 + path_len * (2 * R.L + 1) + 2 * l;
+                strat_uv(pk, chan, s * nsub + jj, R.g_dir, lu, lv);
+                strat_uv(pk, chan + 1, s * nsub + jj, R.g_dir, bu, bv);
+              } else {
+                lu = draw(qk, (path_len << 20) | (base + 0));
+                lv = draw(qk, (path_len << 20) | (base + 1));
+                bu = draw(qk, (path_len << 20) | (base + 2));
+                bv = draw(qk, (path_len << 20) | (base + 3));
+              }
+              float extra = draw(qk, (path_len << 20) | (base + 4));
+              size_t slot_a = (size_t)p * R.K + 2 * (l * nsub + jj), slot_b = slot_a + 1;
+              P.svis[slot_a] = 0;
+              P.svis[slot_b] = 0;
+              // light sample (render/direct-illum.cc:147-197)
+              LSample ls = light_sample(S, L, is, lu, lv, extra);
+              if (ls.pdf > 0.0f && cgt0(ls.val)) {
+                BValue bval = bsdf_eval(is, M, ls.dir, dflags);
+                if (cgt0(bval.val)) {
+                  C3 c = ls.val;
+                  c = c * mis_weight(ls.pdf, bval.pdf);
+                  c = c * bval.val;
+                  c = c * fabsf(ls.dir.z);
+                  c = c * (1.0f / ls.pdf);
+                  float md = ls.dist == 0.0f ? S.horizon : ls.dist;
+                  V3 wd = is.nf.from(ls.dir);
+                  P.sray[0][slot_a] = is.nf.o.x; P.sray[1][slot_a] = is.nf.o.y;
+                  P.sray[2][slot_a] = is.nf.o.z; P.sray[3][slot_a] = wd.x;
+                  P.sray[4][slot_a] = wd.y; P.sray[5][slot_a] = wd.z;
+                  P.sray[6][slot_a] = S.min_trace; P.sray[7][slot_a] = md - S.min_trace;
+                  P.sc[0][slot_a] = c.r; P.sc[1][slot_a] = c.g; P.sc[2][slot_a] = c.b;
+                  uint32_t at = wave_append(sq_count, true);
+                  P.sq[at] = (uint32_t)slot_a;
+                }
+              }
+              // BSDF sample (render/direct-illum.cc:199-252)
+              BSample bs = bsdf_sample(is, M, bu, bv, dflags);
+              if (bs.pdf > 0.0f && cgt0(bs.val)) {
+                LSample lval = light_eval(S, L, is, bs.dir);
+                if (lval.pdf > 0.0f && cgt0(lval.val)) {
+                  C3 c = lval.val;
+                  c = c * mis_weight(bs.pdf, lval.pdf);
+                  c = c * bs.val;
+                  c = c * fabsf(bs.dir.z);
+                  c = c * (1.0f / bs.pdf);
+                  float md = lval.dist == 0.0f ? S.horizon : lval.dist;
+                  V3 wd = is.nf.from(bs.dir);
+                  P.sray[0][slot_b] = is.nf.o.x; P.sray[1][slot_b] = is.nf.o.y;
+                  P.sray[2][slot_b] = is.nf.o.z; P.sray[3][slot_b] = wd.x;
+                  P.sray[4][slot_b] = wd.y; P.sray[5][slot_b] = wd.z;
+                  P.sray[6][slot_b] = S.min_trace; P.sray[7][slot_b] = md - S.min_trace;
+                  P.sc[0][slot_b] = c.r; P.sc[1][slot_b] = c.g; P.sc[2][slot_b] = c.b;
+                  uint32_t at = wave_append(sq_count, true);
+                  P.sq[at] = (uint32_t)slot_b;
+                }
+              }
+            }
+          }
+          nst |= SGR_ST_PENDING | (strat ? SGR_ST_STRAT : 0u);
+          P.Tp[0][p] = T.r; P.Tp[1][p] = T.g; P.Tp[2][p] = T.b;
+        }
+        // ---- continuation (render/path-integ.cc:308-401) ----
+        float cu, cv;
+        if (path_len < R.minp) {
+          uint32_t chan = 1 + path_len * (2 * R.L + 1) + 2 * R.L;
+          strat_uv(pk, chan, s, R.g_cont, cu, cv);
+        } else {
+          cu = draw(qk, (path_len << 20) | 1u);
+          cv = draw(qk, (path_len << 20) | 2u);
+        }
+        BSample bs = bsdf_sample(is, M, cu, cv, F_ALL);
+        if (bs.pdf == 0.0f || czero(bs.val)) {
+          done = true;
+        } else {
+          C3 f = bs.val * fabsf(bs.dir.z);
+          f = f * (1.0f / bs.pdf);
+          T = T * f;
+          float rrr = P.rr[p];
+          if (path_len > R.minp) {
+            float pc = fmn(((T.r + T.g) + T.b) / 3.0f * rrr, 1.0f);
+            float rr = draw(qk, (path_len << 20) | 0u);
+            if (rr > pc) {
+              done = true;
+            } else {
+              T = T * (1.0f / pc);
+              rrr = rrr * pc;
+            }
+          }
+          if (!done && path_len == R.maxp) done = true;
+          if (!done) {
+            RayF nr;
+            nr.o = is.nf.o;
+            nr.d = is.nf.from(bs.dir);
+            nr.t0 = S.min_trace;
+            nr.t1 = S.horizon;
+            store_ray(P, p, nr);
+            P.rr[p] = rrr;
+            bool spec = (bs.flags & F_SPECULAR) != 0;
+            if ((bs.flags & F_TRANSMISSIVE) && HM.medium) {
+              // Media::update_stack_for_transmission (material/media.cc:75-118)
+              if (is.back) {
+                if (depth >= 2) depth--;
+              } else if (depth < SGR_MEDIA_MAX) {
+                P.media[(size_t)depth * N + p] = M.ior_n;
+                depth++;
+              }
+            }
+            nst = (nst & (SGR_ST_PENDING | SGR_ST_STRAT)) | (path_len + 1) |
+                  ((spec ? 1u : 0u) << SGR_ST_SPEC_SHIFT) | (depth << SGR_ST_DEPTH_SHIFT);
+            cont = true;
+          }
+        }
+      }
+    }
+    if (done) nst |= SGR_ST_DONE;
+    P.state[p] = nst;
+    P.T[0][p] = T.r; P.T[1][p] = T.g; P.T[2][p] = T.b;
+    P.rad[0][p] = rad.r; P.rad[1][p] = rad.g; P.rad[2][p] = rad.b;
+    P.alpha[p] = alpha;
+    // Next ray queue: wave ballot + prefix sum, one atomic per wave.
+    uint32_t at = wave_append(qn_count, cont);
+    if (cont) qn[at] = p;
+  }
+  wave_count(&stats[SGR_STAT_ILLUM], illum);
+}
+
+// Adds the unoccluded direct terms (render/direct-illum.cc:96-113 sums,
+// render/path-integ.cc:277-306 product with the path transmittance) and
+// writes finished paths: sx, sy, Tint(radiance, alpha).
+__global__ void __launch_bounds__(256) k_gather(RenderView R, PathBufs P, uint32_t b) {
+  const uint32_t n = P.counts[SGR_CNT_Q + b];
+  const uint32_t *q = P.q[b & 1];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    uint32_t p = q[j];
+    uint32_t st = P.state[p];
+    if (!(st & (SGR_ST_PENDING | SGR_ST_DONE))) continue;
+    C3 rad = c3(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
+    if (st & SGR_ST_PENDING) {
+      uint32_t nsub = (st & SGR_ST_STRAT) ? R.nsub_strat : R.nsub_rand;
+      float inv_n = 1.0f / (float)nsub;
+      C3 D = c3(0.0f);
+      for (uint32_t l = 0; l < R.L; l++) {
+        C3 lrad = c3(0.0f);
+        for (uint32_t jj = 0; jj < nsub; jj++) {
+          size_t sa = (size_t)p * R.K + 2 * (l * nsub + jj);
+          C3 r = c3(0.0f);
+          if (P.svis[sa]) r = r + c3(P.sc[0][sa], P.sc[1][sa], P.sc[2][sa]);
+          if (P.svis[sa + 1]) r = r + c3(P.sc[0][sa + 1], P.sc[1][sa + 1], P.sc[2][sa + 1]);
+          lrad = lrad + r;
+        }
+        D = D + lrad * inv_n;
+      }
+      rad = rad + D * c3(P.Tp[0][p], P.Tp[1][p], P.Tp[2][p]);
+      P.rad[0][p] = rad.r; P.rad[1][p] = rad.g; P.rad[2][p] = rad.b;
+      st &= ~(SGR_ST_PENDING | SGR_ST_STRAT);
+      P.state[p] = st;
+    }
+    if (st & SGR_ST_DONE) {
+      float a = P.alpha[p];
+      float *o = P.out + 6 * (size_t)p;
+      o[2] = rad.r * a;
+      o[3] = rad.g * a;
+      o[4] = rad.b * a;
+      o[5] = a;
+    }
+  }
+}
+
+// image/image-mitchell-filt.h:45-65
+__device__ __forceinline__ float mitchell1(float x, float inv_radius, float B, float Cc) {
+  x = fabsf(2.0f * x * inv_radius);
+  if (x > 2.0f) return 0.0f;
+  if (x > 1.0f)
+    return ((-B - 6 * Cc) * x * x * x + (6 * B + 30 * Cc) * x * x + (-12 * B - 48 * Cc) * x +
+            (8 * B + 24 * Cc)) * (1.0f / 6.0f);
+  return ((12 - 9 * B - 6 * Cc) * x * x * x + (-18 + 12 * B + 6 * Cc) * x * x + (6 - 2 * B)) *
+         (1.0f / 6.0f);
+}
+
+// Gather form of ImageFilterConv::add_sample: each image pixel sums the
+// weighted samples of the pattern pixels within the filter radius (a
+// sample of pattern pixel x reaches image pixels int(sx) +- radius; the
+// taps that truncation adds or drops all carry weight 0).  One thread
+// per image pixel, no atomics, fixed summation order.
+__global__ void __launch_bounds__(256) k_splat(RenderView R, const float *out, float *accum) {
+  int ox0 = R.X0 > 0 ? R.X0 : 0;
+  int oy0 = R.Y0 > 0 ? R.Y0 : 0;
+  int ox1 = R.X0 + R.PW < (int)R.width ? R.X0 + R.PW : (int)R.width;
+  int oy1 = R.Y0 + R.PH < (int)R.height ? R.Y0 + R.PH : (int)R.height;
+  int ow = ox1 - ox0, oh = oy1 - oy0;
+  if (ow <= 0 || oh <= 0) return;
+  uint32_t total = (uint32_t)(ow * oh);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    int qx = ox0 + (int)(i % (uint32_t)ow), qy = oy0 + (int)(i / (uint32_t)ow);
+    float acc[5] = {0, 0, 0, 0, 0};
+    int sy0 = qy - R.fr > R.Y0 ? qy - R.fr : R.Y0;
+    int sy1 = qy + R.fr < R.Y0 + R.PH - 1 ? qy + R.fr : R.Y0 + R.PH - 1;
+    int sx0 = qx - R.fr > R.X0 ? qx - R.fr : R.X0;
+    int sx1 = qx + R.fr < R.X0 + R.PW - 1 ? qx + R.fr : R.X0 + R.PW - 1;
+    for (int sy = sy0; sy <= sy1; sy++)
+      for (int sx = sx0; sx <= sx1; sx++) {
+        uint32_t pix = (uint32_t)(sy - R.Y0) * (uint32_t)R.PW + (uint32_t)(sx - R.X0);
+        const float *o = out + 6 * (size_t)pix * R.sb;
+        for (uint32_t k = 0; k < R.sb; k++, o += 6) {
+          float fx = o[0], fy = o[1];
+          int ix = (int)fx, iy = (int)fy;
+          if (ix - qx > R.fr || qx - ix > R.fr || iy - qy > R.fr || qy - iy > R.fr) continue;
+          float w = mitchell1(fx - ((float)qx + 0.5f), R.inv_radius, R.fb, R.fc) *
+                    mitchell1(fy - ((float)qy + 0.5f), R.inv_radius, R.fb, R.fc);
+          float ws0 = o[2] * w, ws1 = o[3] * w, ws2 = o[4] * w, ws3 = o[5] * w, wr = w;
+          if (w < 0.0f) {
+            float mc = ws0;
+            mc = ws1 < mc ? ws1 : mc;
+            mc = ws2 < mc ? ws2 : mc;
+            if (mc < R.neg_clamp) {
+              float cs = R.neg_clamp / mc;
+              wr = wr * cs;
+              ws0 = ws0 * cs; ws1 = ws1 * cs; ws2 = ws2 * cs; ws3 = ws3 * cs;
+            }
+          }
+          acc[0] += ws0; acc[1] += ws1; acc[2] += ws2; acc[3] += ws3; acc[4] += wr;
+        }
+      }
+    float *a = accum + 5 * ((size_t)qy * R.width + (size_t)qx);
+    a[0] += acc[0]; a[1] += acc[1]; a[2] += acc[2]; a[3] += acc[3]; a[4] += acc[4];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-callable launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(uint32_t n, unsigned cap) {
+  unsigned g = (n + 255) / 256;
+  if (g == 0) g = 1;
+  return g < cap ? g : cap;
+}
+
+void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool any,
+                  unsigned long long *stats, hipStream_t st) {
+  unsigned g = grid_for(n, 65535);
+  if (any) hipLaunchKernelGGL(k_trace_any, dim3(g), dim3(256), 0, st, S, rb, n, stats);
+  else hipLaunchKernelGGL(k_trace_closest, dim3(g), dim3(256), 0, st, S, rb, n, stats);
+}
+
+void launch_raygen(const RenderView &R, const PathBufs &P, hipStream_t st) {
+  hipLaunchKernelGGL(k_raygen, dim3(grid_for(R.npaths, 65535)), dim3(256), 0, st, R, P);
+}
+void launch_extend(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
+                   unsigned long long *stats, unsigned grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, st, S, P, b, stats);
+  (void)R;
+}
+void launch_shade(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
+                  unsigned long long *stats, unsigned grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, S, R, P, b, stats);
+}
+void launch_occlude(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
+                    unsigned long long *stats, unsigned grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_occlude, dim3(grid), dim3(256), 0, st, S, P, b, stats);
+  (void)R;
+}
+void launch_gather(const RenderView &R, const PathBufs &P, uint32_t b, unsigned grid,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, st, R, P, b);
+}
+void launch_splat(const RenderView &R, const float *out, float *accum, hipStream_t st) {
+  uint32_t n = (uint32_t)((R.PW) * (R.PH));
+  hipLaunchKernelGGL(k_splat, dim3(grid_for(n, 65535)), dim3(256), 0, st, R, out, accum);
+}
+
+}  // namespace sgr

@@ -1,0 +1,208 @@
+"""ctypes front end of libsnoggpu.so (include/snoggpu.h).
+
+There is no fallback: if the library is missing or fails to load, every
+entry point raises.  Build it with ``python -m snogray_amd.build``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsnoggpu.so")
+
+EXPORTS = [
+    "sgr_last_error", "sgr_ctx_create", "sgr_ctx_destroy", "sgr_ctx_memory",
+    "sgr_scene_create", "sgr_scene_destroy", "sgr_scene_octree_stats",
+    "sgr_octree_export", "sgr_octree_build", "sgr_scene_bounds", "sgr_scene_footprint",
+    "sgr_trace_closest", "sgr_trace_occluded", "sgr_render",
+    "sgr_render_samples", "sgr_finalize_image",
+]
+
+_lib = None
+
+
+class SgrError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libsnoggpu.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SgrError(f"{LIB_PATH} is not built; run python -m snogray_amd.build")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.sgr_last_error.restype = C.c_char_p
+    L.sgr_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.sgr_ctx_destroy.argtypes = [vp]
+    L.sgr_ctx_memory.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.sgr_scene_create.argtypes = [vp, C.POINTER(abi.SceneDescC), C.POINTER(vp)]
+    L.sgr_scene_destroy.argtypes = [vp]
+    L.sgr_scene_octree_stats.argtypes = [vp, C.POINTER(abi.OctreeStats)]
+    L.sgr_octree_export.argtypes = [vp, abi.u32p, C.POINTER(C.c_uint64), abi.u32p,
+                                    C.POINTER(C.c_uint64), abi.f32p]
+    L.sgr_octree_build.argtypes = [C.POINTER(abi.SceneDescC), abi.u32p,
+                                   C.POINTER(C.c_uint64), abi.u32p,
+                                   C.POINTER(C.c_uint64), abi.f32p,
+                                   C.POINTER(abi.OctreeStats)]
+    L.sgr_scene_bounds.argtypes = [vp, abi.f32p, abi.f32p, abi.f32p]
+    L.sgr_scene_footprint.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.sgr_trace_closest.argtypes = [vp, C.POINTER(abi.Rays), C.c_size_t,
+                                    C.POINTER(abi.Hits), C.POINTER(abi.Stats)]
+    L.sgr_trace_occluded.argtypes = [vp, C.POINTER(abi.Rays), C.c_size_t,
+                                     C.POINTER(C.c_uint8), C.POINTER(abi.Stats)]
+    L.sgr_render.argtypes = [vp, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams),
+                             abi.f32p, C.c_int, C.POINTER(abi.Stats)]
+    L.sgr_render_samples.argtypes = [vp, C.POINTER(abi.Camera),
+                                     C.POINTER(abi.RenderParams), abi.f32p,
+                                     C.POINTER(abi.Stats)]
+    L.sgr_finalize_image.argtypes = [abi.f32p, C.c_uint32, C.c_uint32, C.c_float,
+                                     C.c_float, abi.f32p]
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().sgr_last_error()
+        raise SgrError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+class Context:
+    """One device context (HIP stream + work buffers) of libsnoggpu."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().sgr_ctx_create(device, C.byref(h)), "sgr_ctx_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sgr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def memory(self):
+        b = C.c_uint64()
+        check(lib().sgr_ctx_memory(self.h, C.byref(b)), "sgr_ctx_memory")
+        return b.value
+
+
+class Scene:
+    """Device-resident scene built from a SceneDesc."""
+
+    def __init__(self, ctx, desc):
+        self.ctx = ctx
+        self.desc = desc
+        h = C.c_void_p()
+        check(lib().sgr_scene_create(ctx.h, C.byref(desc.c()), C.byref(h)),
+              "sgr_scene_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sgr_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def octree_stats(self):
+        st = abi.OctreeStats()
+        check(lib().sgr_scene_octree_stats(self.h, C.byref(st)), "octree_stats")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def octree(self):
+        nn, ns = C.c_uint64(), C.c_uint64()
+        check(lib().sgr_octree_export(self.h, None, C.byref(nn), None, C.byref(ns), None),
+              "octree_export")
+        nodes = np.zeros((nn.value, 4), np.uint32)
+        surfs = np.zeros(ns.value, np.uint32)
+        osz = np.zeros(4, np.float32)
+        check(lib().sgr_octree_export(self.h, abi.uptr(nodes), C.byref(nn),
+                                      abi.uptr(surfs), C.byref(ns), abi.fptr(osz)),
+              "octree_export")
+        return nodes, surfs, osz
+
+    def bounds(self):
+        lo, hi, hz = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(1, np.float32)
+        check(lib().sgr_scene_bounds(self.h, abi.fptr(lo), abi.fptr(hi), abi.fptr(hz)),
+              "scene_bounds")
+        return lo, hi, float(hz[0])
+
+    def footprint(self):
+        b = C.c_uint64()
+        check(lib().sgr_scene_footprint(self.h, C.byref(b)), "scene_footprint")
+        return b.value
+
+    def trace_closest(self, batch):
+        hb = abi.HitBuffer(batch.n)
+        st = abi.Stats()
+        check(lib().sgr_trace_closest(self.h, C.byref(batch.c), batch.n, C.byref(hb.c),
+                                      C.byref(st)), "sgr_trace_closest")
+        return hb, st.as_dict()
+
+    def trace_occluded(self, batch):
+        occ = np.zeros(batch.n, np.uint8)
+        st = abi.Stats()
+        check(lib().sgr_trace_occluded(self.h, C.byref(batch.c), batch.n,
+                                       occ.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                       C.byref(st)), "sgr_trace_occluded")
+        return occ, st.as_dict()
+
+    def render(self, cam, params, accum=None):
+        """Render into host accumulators (H, W, 5) float32; returns (accum, stats)."""
+        if accum is None:
+            accum = np.zeros((params.height, params.width, 5), np.float32)
+        assert accum.dtype == np.float32 and accum.flags.c_contiguous
+        st = abi.Stats()
+        check(lib().sgr_render(self.h, C.byref(cam), C.byref(params), abi.fptr(accum), 0,
+                               C.byref(st)), "sgr_render")
+        return accum, st.as_dict()
+
+    def render_device(self, cam, params, accum_ptr):
+        """Render into device accumulators at accum_ptr (int address)."""
+        st = abi.Stats()
+        check(lib().sgr_render(self.h, C.byref(cam), C.byref(params),
+                               C.cast(C.c_void_p(accum_ptr), abi.f32p), 1, C.byref(st)),
+              "sgr_render")
+        return st.as_dict()
+
+    def render_samples(self, cam, params):
+        x0, y0, w, h, m = abi.render_window(params)
+        out = np.zeros(((h + 2 * m) * (w + 2 * m) * params.spp, 6), np.float32)
+        st = abi.Stats()
+        check(lib().sgr_render_samples(self.h, C.byref(cam), C.byref(params),
+                                       abi.fptr(out), C.byref(st)), "sgr_render_samples")
+        return out, st.as_dict()
+
+
+def octree_build(desc):
+    """Host-only octree build: (nodes (N,4) u32, surf ids, origin+size, stats)."""
+    nn, ns = C.c_uint64(), C.c_uint64()
+    check(lib().sgr_octree_build(C.byref(desc.c()), None, C.byref(nn), None,
+                                 C.byref(ns), None, None), "sgr_octree_build")
+    nodes = np.zeros((nn.value, 4), np.uint32)
+    surfs = np.zeros(ns.value, np.uint32)
+    osz = np.zeros(4, np.float32)
+    st = abi.OctreeStats()
+    check(lib().sgr_octree_build(C.byref(desc.c()), abi.uptr(nodes), C.byref(nn),
+                                 abi.uptr(surfs), C.byref(ns), abi.fptr(osz),
+                                 C.byref(st)), "sgr_octree_build")
+    return nodes, surfs, osz, {k: getattr(st, k) for k, _ in st._fields_}
+
+
+def finalize(accum, exposure=1.0, contrast=1.0):
+    a = np.ascontiguousarray(accum, np.float32)
+    h, w = a.shape[:2]
+    out = np.zeros((h, w, 4), np.float32)
+    check(lib().sgr_finalize_image(abi.fptr(a), w, h, exposure, contrast, abi.fptr(out)),
+          "sgr_finalize_image")
+    return out
