@@ -188,7 +188,12 @@ typedef struct sgr_render_params {
     uint32_t x0, y0, w, h;
     /* Paths kept in flight per wavefront step (0: choose). */
     uint32_t paths_in_flight;
+    uint32_t flags;                /* SGR_RENDER_* */
 } sgr_render_params;
+
+/* sgr_render_params.flags: time every kernel launch with HIP events on
+ * the render stream (no host waits) into sgr_stats.ms_* */
+#define SGR_RENDER_KERNEL_TIMING 1u
 
 /* Counters named after RenderStats::IsecStats (render/render-stats.h:30-52)
  * plus device timings. */

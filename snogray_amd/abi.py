@@ -17,6 +17,7 @@ SGR_MAT_GLOW = 5
 SGR_ACCEL_OCTREE = 0
 SGR_ACCEL_TRIV = 1
 MISS = 0xFFFFFFFF
+SGR_RENDER_KERNEL_TIMING = 1
 
 f32p = C.POINTER(C.c_float)
 u32p = C.POINTER(C.c_uint32)
@@ -86,7 +87,7 @@ class RenderParams(C.Structure):
                 ("filter_radius", C.c_float), ("neg_clamp", C.c_float),
                 ("bg_alpha", C.c_float), ("x0", C.c_uint32),
                 ("y0", C.c_uint32), ("w", C.c_uint32), ("h", C.c_uint32),
-                ("paths_in_flight", C.c_uint32)]
+                ("paths_in_flight", C.c_uint32), ("flags", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -120,7 +121,7 @@ def uptr(a):
 def make_params(width, height, spp, max_path_len=8, min_path_len=3,
                 direct_samples=1, seed=1, window=None, filter_b=1.0 / 3,
                 filter_c=1.0 / 3, filter_radius=2.0, neg_clamp=-0.1,
-                bg_alpha=1.0, paths_in_flight=0):
+                bg_alpha=1.0, paths_in_flight=0, flags=0):
     p = RenderParams()
     p.width, p.height, p.spp = width, height, spp
     p.min_path_len, p.max_path_len = min_path_len, max_path_len
@@ -130,6 +131,7 @@ def make_params(width, height, spp, max_path_len=8, min_path_len=3,
     if window is not None:
         p.x0, p.y0, p.w, p.h = window
     p.paths_in_flight = paths_in_flight
+    p.flags = flags
     return p
 
 

@@ -30,6 +30,8 @@ namespace sgr {
 #define SGR_STAT_NODES_S 2
 #define SGR_STAT_TESTS_S 3
 #define SGR_STAT_ILLUM   4
+#define SGR_STAT_ISECT   5
+#define SGR_STAT_SHADOW  6
 #define SGR_NUM_STATS    8
 
 struct RayBatchDev {
@@ -82,5 +84,7 @@ void launch_occlude(const SceneView &S, const RenderView &R, const PathBufs &P, 
 void launch_gather(const RenderView &R, const PathBufs &P, uint32_t b, unsigned grid,
                    hipStream_t st);
 void launch_splat(const RenderView &R, const float *out, float *accum, hipStream_t st);
+void launch_count_queues(const RenderView &R, const PathBufs &P, unsigned long long *stats,
+                         hipStream_t st);
 
 }  // namespace sgr

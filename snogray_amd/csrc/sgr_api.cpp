@@ -50,8 +50,9 @@ struct sgr_ctx {
   size_t work_bytes = 0;
   unsigned long long *stats = nullptr;
   int num_cu = 256;
-  bool kernel_timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<hipEvent_t> evpool;  // per-launch timing events
+  uint32_t live_scenes = 0;
 };
 
 struct sgr_scene {
@@ -185,20 +186,22 @@ int sgr_ctx_create(int device, sgr_ctx **out) {
     delete c;
     return fail(SGR_ERR_NOMEM, "sgr_ctx_create: stats buffer");
   }
-  const char *kt = std::getenv("SGR_KERNEL_TIMING");
-  c->kernel_timing = kt && kt[0] == '1';
   *out = c;
   return SGR_OK;
 }
 
 int sgr_ctx_destroy(sgr_ctx *ctx) {
   if (!ctx) return SGR_OK;
+  if (ctx->live_scenes)
+    return fail(SGR_ERR_INVALID, "sgr_ctx_destroy: " + std::to_string(ctx->live_scenes) +
+                                     " scene(s) still alive; destroy them first");
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->work) (void)hipFree(ctx->work);
   if (ctx->stats) (void)hipFree(ctx->stats);
   for (auto &ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
+  for (auto &ev : ctx->evpool) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SGR_OK;
@@ -494,6 +497,7 @@ int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
   HIPCHK(hipSetDevice(ctx->device));
   sgr_scene *sc = new sgr_scene;
   sc->ctx = ctx;
+  ctx->live_scenes++;
   for (int a = 0; a < 3; a++) {
     sc->bbox_lo[a] = H.blo[a];
     sc->bbox_hi[a] = H.bhi[a];
@@ -537,6 +541,7 @@ int sgr_scene_destroy(sgr_scene *sc) {
     (void)hipSetDevice(sc->ctx->device);
     (void)hipStreamSynchronize(sc->ctx->stream);
     sc->ctx->bytes -= sc->bytes;
+    sc->ctx->live_scenes--;
   }
   for (void *p : sc->allocs) (void)hipFree(p);
   delete sc;
@@ -716,6 +721,7 @@ size_t path_bytes(size_t cap, size_t K, PathBufs *P, char *base) {
 
 struct RenderPlan {
   RenderView R;
+  uint32_t flags;
   uint32_t sb_max;
   uint32_t npix;
 };
@@ -778,6 +784,7 @@ int plan_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p
   if ((uint64_t)pl.npix * sb >= (1ull << 31))
     return fail(SGR_ERR_LIMIT, "sgr_render: image too large for one chunk");
   pl.sb_max = (uint32_t)sb;
+  pl.flags = p->flags;
   return SGR_OK;
 }
 
@@ -812,20 +819,32 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   unsigned grid = (unsigned)std::min<size_t>((cap + 255) / 256, (size_t)ctx->num_cu * 16);
   if (grid == 0) grid = 1;
   std::vector<float> chunk_out;
-  std::vector<uint32_t> counts(SGR_NUM_COUNTERS);
-  uint64_t isect = 0, shadow = 0;
   uint32_t n_ext = 0, n_occ = 0;
-  double ms_ext = 0, ms_shade = 0, ms_occ = 0, ms_splat = 0, ms_other = 0;
-  const bool kt = ctx->kernel_timing && stats;
-  auto t_begin = [&]() { if (kt) hipEventRecord(ctx->ev[2], ctx->stream); };
-  auto t_end = [&](double &acc) {
-    if (kt) {
-      hipEventRecord(ctx->ev[3], ctx->stream);
-      hipEventSynchronize(ctx->ev[3]);
-      float ms = 0;
-      hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
-      acc += ms;
+  // Per-launch timing: event pairs recorded on the render stream without
+  // host waits, read after the final synchronisation.
+  enum { K_EXT, K_SHADE, K_OCC, K_SPLAT, K_OTHER, K_N };
+  double kms[K_N] = {0, 0, 0, 0, 0};
+  const bool kt = stats && (pl.flags & SGR_RENDER_KERNEL_TIMING);
+  std::vector<std::pair<int, size_t>> marks;
+  size_t next_ev = 0;
+  auto ev_get = [&]() -> hipEvent_t {
+    if (next_ev == ctx->evpool.size()) {
+      hipEvent_t e;
+      (void)hipEventCreate(&e);
+      ctx->evpool.push_back(e);
     }
+    return ctx->evpool[next_ev++];
+  };
+  size_t mark0 = 0;
+  auto t_begin = [&]() {
+    if (!kt) return;
+    mark0 = next_ev;
+    (void)hipEventRecord(ev_get(), ctx->stream);
+  };
+  auto t_end = [&](int kind) {
+    if (!kt) return;
+    (void)hipEventRecord(ev_get(), ctx->stream);
+    marks.push_back({kind, mark0});
   };
   HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
   for (uint32_t s0 = 0; s0 < R.spp; s0 += pl.sb_max) {
@@ -834,45 +853,37 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     R.npaths = pl.npix * R.sb;
     t_begin();
     launch_raygen(R, P, ctx->stream);
-    t_end(ms_other);
+    t_end(K_OTHER);
     for (uint32_t b = 0; b <= R.maxp; b++) {
       t_begin();
       launch_extend(sc->view, R, P, b, ctx->stats, grid, ctx->stream);
-      t_end(ms_ext);
+      t_end(K_EXT);
       t_begin();
       launch_shade(sc->view, R, P, b, ctx->stats, grid, ctx->stream);
-      t_end(ms_shade);
+      t_end(K_SHADE);
       t_begin();
       launch_occlude(sc->view, R, P, b, ctx->stats, grid, ctx->stream);
-      t_end(ms_occ);
+      t_end(K_OCC);
       t_begin();
       launch_gather(R, P, b, grid, ctx->stream);
-      t_end(ms_other);
+      t_end(K_OTHER);
       n_ext++;
       n_occ++;
     }
     if (accum_dev) {
       t_begin();
       launch_splat(R, P.out, accum_dev, ctx->stream);
-      t_end(ms_splat);
+      t_end(K_SPLAT);
     }
+    launch_count_queues(R, P, ctx->stats, ctx->stream);
     HIPCHK(hipGetLastError());
-    if (stats) {
-      HIPCHK(hipMemcpyAsync(counts.data(), P.counts, SGR_NUM_COUNTERS * 4, hipMemcpyDeviceToHost,
-                            ctx->stream));
-    }
     if (samples_host) {
       chunk_out.resize((size_t)R.npaths * 6);
       HIPCHK(hipMemcpyAsync(chunk_out.data(), P.out, chunk_out.size() * 4, hipMemcpyDeviceToHost,
                             ctx->stream));
     }
-    if (stats || samples_host) HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (stats)
-      for (uint32_t b = 0; b <= R.maxp; b++) {
-        isect += counts[SGR_CNT_Q + b];
-        shadow += counts[SGR_CNT_S + b];
-      }
     if (samples_host) {
+      HIPCHK(hipStreamSynchronize(ctx->stream));
       for (uint32_t pix = 0; pix < pl.npix; pix++)
         std::memcpy(samples_host + ((size_t)pix * R.spp + s0) * 6, &chunk_out[(size_t)pix * R.sb * 6],
                     (size_t)R.sb * 6 * 4);
@@ -890,19 +901,24 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     stats->eye_rays = (uint64_t)pl.npix * R.spp;
     stats->paths_traced = (uint64_t)pl.npix * R.spp;
     stats->samples = (uint64_t)w * h * R.spp;
-    stats->scene_intersect_calls = isect;
-    stats->shadow_tests = shadow;
+    stats->scene_intersect_calls = st[SGR_STAT_ISECT];
+    stats->shadow_tests = st[SGR_STAT_SHADOW];
     stats->node_visits_closest = st[SGR_STAT_NODES_C];
     stats->surf_tests_closest = st[SGR_STAT_TESTS_C];
     stats->node_visits_shadow = st[SGR_STAT_NODES_S];
     stats->surf_tests_shadow = st[SGR_STAT_TESTS_S];
     stats->illum_calls = st[SGR_STAT_ILLUM];
+    for (auto &m : marks) {
+      float e = 0;
+      (void)hipEventElapsedTime(&e, ctx->evpool[m.second], ctx->evpool[m.second + 1]);
+      kms[m.first] += e;
+    }
     stats->ms_render = ms;
-    stats->ms_extend = ms_ext;
-    stats->ms_shade = ms_shade;
-    stats->ms_shadow = ms_occ;
-    stats->ms_splat = ms_splat;
-    stats->ms_other = ms_other;
+    stats->ms_extend = kms[K_EXT];
+    stats->ms_shade = kms[K_SHADE];
+    stats->ms_shadow = kms[K_OCC];
+    stats->ms_splat = kms[K_SPLAT];
+    stats->ms_other = kms[K_OTHER];
     stats->extend_launches = n_ext;
     stats->shadow_launches = n_occ;
   }

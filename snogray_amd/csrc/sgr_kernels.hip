@@ -445,6 +445,17 @@ __global__ void __launch_bounds__(256) k_splat(RenderView R, const float *out, f
   }
 }
 
+// Adds this chunk's queue lengths (closest-hit and any-hit rays) to the
+// device statistics; one wave.
+__global__ void __launch_bounds__(64) k_count_queues(RenderView R, PathBufs P,
+                                                     unsigned long long *stats) {
+  uint32_t b = threadIdx.x;
+  uint32_t q = b <= R.maxp ? P.counts[SGR_CNT_Q + b] : 0u;
+  uint32_t s = b <= R.maxp ? P.counts[SGR_CNT_S + b] : 0u;
+  wave_count(&stats[SGR_STAT_ISECT], q);
+  wave_count(&stats[SGR_STAT_SHADOW], s);
+}
+
 // ---------------------------------------------------------------------------
 // Host-callable launchers
 // ---------------------------------------------------------------------------
@@ -481,6 +492,10 @@ void launch_occlude(const SceneView &S, const RenderView &R, const PathBufs &P, 
 void launch_gather(const RenderView &R, const PathBufs &P, uint32_t b, unsigned grid,
                    hipStream_t st) {
   hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, st, R, P, b);
+}
+void launch_count_queues(const RenderView &R, const PathBufs &P, unsigned long long *stats,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_count_queues, dim3(1), dim3(64), 0, st, R, P, stats);
 }
 void launch_splat(const RenderView &R, const float *out, float *accum, hipStream_t st) {
   uint32_t n = (uint32_t)((R.PW) * (R.PH));

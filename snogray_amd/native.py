@@ -3,8 +3,10 @@
 There is no fallback: if the library is missing or fails to load, every
 entry point raises.  Build it with ``python -m snogray_amd.build``.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -73,17 +75,32 @@ def check(rc, what):
         raise SgrError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
 
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    """Release device objects while the HIP runtime is still up."""
+    for ctx in list(_live_contexts):
+        ctx.close()
+
+
 class Context:
-    """One device context (HIP stream + work buffers) of libsnoggpu."""
+    """One device context (HIP stream + work buffers) of libsnoggpu.
+    Closing it first closes every scene created on it."""
 
     def __init__(self, device=0):
         h = C.c_void_p()
         check(lib().sgr_ctx_create(device, C.byref(h)), "sgr_ctx_create")
         self.h = h
+        self.scenes = weakref.WeakSet()
+        _live_contexts.add(self)
 
     def close(self):
         if getattr(self, "h", None):
-            lib().sgr_ctx_destroy(self.h)
+            for sc in list(self.scenes):
+                sc.close()
+            check(lib().sgr_ctx_destroy(self.h), "sgr_ctx_destroy")
             self.h = None
 
     def __del__(self):
@@ -105,6 +122,7 @@ class Scene:
         check(lib().sgr_scene_create(ctx.h, C.byref(desc.c()), C.byref(h)),
               "sgr_scene_create")
         self.h = h
+        ctx.scenes.add(self)
 
     def close(self):
         if getattr(self, "h", None):
