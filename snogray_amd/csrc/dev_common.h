@@ -43,7 +43,8 @@ struct DevMesh {
 
 struct SceneView {
   const uint4 *nodes;        // first_child, mask, surf_begin, surf_count
-  const uint32_t *surfs;     // primitive ids
+  const uint32_t *surfs;     // primitive ids (host export only)
+  const float4 *refs;        // 3 per surface reference, surface-list order
   const float4 *tri;         // 3 per triangle-like prim: v0|flags, e1, e2
   const float4 *sph;         // centre, radius
   const uint32_t *prim_mat;  // material index per prim

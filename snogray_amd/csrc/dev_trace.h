@@ -69,16 +69,22 @@ __device__ __forceinline__ bool sphere_hit(V3 c, float radius, V3 ro, V3 rd, flo
   return false;
 }
 
-__device__ __forceinline__ bool prim_hit(const SceneView &S, uint32_t p, const RayF &r,
-                                         float &t, float &u, float &v) {
-  if (p < S.n_tri_like) {
-    float4 a = S.tri[3 * p], b = S.tri[3 * p + 1], c = S.tri[3 * p + 2];
-    return tripar_hit(ld3(a), ld3(b), ld3(c), a.w != 0.0f, r, t, u, v);
+// One surface reference: three float4 in surface-list order.
+//   triangle / tripar: (v0, prim id bits) (e1, kind) (e2, 0)
+//   sphere:            (centre, prim id bits) (radius, 0, 0, kind)
+// kind: 0 triangle, 1 parallelogram, 2 sphere.
+__device__ __forceinline__ bool ref_hit(const float4 *refs, uint32_t k, const RayF &r,
+                                        float &t, float &u, float &v, uint32_t &prim) {
+  float4 a = refs[3 * k], b = refs[3 * k + 1];
+  uint32_t kind = __float_as_uint(b.w);
+  prim = __float_as_uint(a.w);
+  if (kind != 2u) {
+    float4 c = refs[3 * k + 2];
+    return tripar_hit(ld3(a), ld3(b), ld3(c), kind == 1u, r, t, u, v);
   }
-  float4 s = S.sph[p - S.n_tri_like];
   u = 0.0f;
   v = 0.0f;
-  if (sphere_hit(ld3(s), s.w, r.o, r.d, r.t0, t)) return t < r.t1;
+  if (sphere_hit(ld3(a), b.x, r.o, r.d, r.t0, t)) return t < r.t1;
   return false;
 }
 
@@ -86,25 +92,45 @@ struct TraceCount {
   uint32_t nodes, tests;
 };
 
-// Octree search (space/octree.cc:101-300).  The recursion is kept as an
-// explicit stack of parent frames (node | next-child cursor, parametric
-// bounds); the current frame lives in registers.  Child visiting order,
-// live ray.t1 pruning and surface order match the recursive search, so
-// the first of several equal-t hits wins exactly as on the CPU.
+// Child occupancy of a node in visiting order: bit j is set when the
+// child visited j-th (parametric order, 3 and 4 swapped, xor the ray
+// octant) exists.
+__device__ __forceinline__ uint32_t visit_mask(uint32_t mask, uint32_t oct) {
+  uint32_t m = mask;
+  if (oct & 4u) m = ((m & 0x0fu) << 4) | ((m >> 4) & 0x0fu);
+  if (oct & 2u) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
+  if (oct & 1u) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
+  uint32_t b3 = (m >> 3) & 1u, b4 = (m >> 4) & 1u;
+  return (m & ~0x18u) | (b3 << 4) | (b4 << 3);
+}
+
+// Octree search (space/octree.cc:101-300) without recursion.
+//
+// The recursive search visits, for each node, its surfaces and then its
+// children in perturbed parametric order, pruning a child by its
+// parametric interval against the live ray extent (ray.t1 shrinks on
+// every closer hit).  Here the current node lives in registers and a
+// parent is pushed (first child | next position, child masks, parametric
+// bounds) only when it still has children left to visit.  A child is
+// tested against the same prune conditions, with the same ray extent,
+// at the same point of the walk, before it is entered, so the surfaces
+// are tested in the reference's order and the first of several equal-t
+// hits wins exactly as on the CPU.  Node-call counts include the pruned
+// calls, as the reference counts them.
 // ANY: stop at the first hit (space/space.cc:103-116 / :160-180).
 template<bool ANY>
 __device__ uint32_t trace_ray(const SceneView &S, RayF &ray, float &hu, float &hv,
                               TraceCount &cnt) {
   uint32_t hit = SGR_MISS;
   float t, u, v;
+  uint32_t prim;
   if (S.use_triv) {  // TrivSpace (space/triv-space.h:48-58)
     for (uint32_t k = 0; k < S.n_surfs; k++) {
-      uint32_t p = S.surfs[k];
       cnt.tests++;
-      if (prim_hit(S, p, ray, t, u, v)) {
-        if (ANY) return p;
+      if (ref_hit(S.refs, k, ray, t, u, v, prim)) {
+        if (ANY) return prim;
         ray.t1 = t;
-        hit = p;
+        hit = prim;
         hu = u;
         hv = v;
       }
@@ -129,96 +155,86 @@ __device__ uint32_t trace_ray(const SceneView &S, RayF &ray, float &hu, float &h
     bz1 = (S.origin[2] - ray.o.z + S.size) * iz;
     if (bz0 > bz1) { float q = bz0; bz0 = bz1; bz1 = q; }
   }
-  uint32_t st_node[SGR_MAX_DEPTH];
-  float st_b[SGR_MAX_DEPTH][6];
+  cnt.nodes++;
+  if (fmx(ray.t0, fmx(bx0, fmx(by0, bz0))) >= fmn(ray.t1, fmn(bx1, fmn(by1, bz1)))) return hit;
+  float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
   int sp = 0;
-  uint32_t cn = 0, cmask = 0, cfirst = 0, ci = 0;
-  float xh = 0, yh = 0, zh = 0, xm = 0, ym = 0, zm = 0;
-  bool enter = true;
+  uint32_t cn = 0;
   for (;;) {
-    bool pop = false;
-    if (enter) {
-      cnt.nodes++;
-      float min_t = fmx(ray.t0, fmx(bx0, fmx(by0, bz0)));
-      float max_t = fmn(ray.t1, fmn(bx1, fmn(by1, bz1)));
-      if (min_t >= max_t) {
-        pop = true;
-      } else {
-        uint4 nd = S.nodes[cn];
-        for (uint32_t k = 0; k < nd.w; k++) {
-          uint32_t p = S.surfs[nd.z + k];
-          cnt.tests++;
-          if (prim_hit(S, p, ray, t, u, v)) {
-            if (ANY) return p;
-            ray.t1 = t;
-            hit = p;
-            hu = u;
-            hv = v;
-          }
-        }
-        cmask = nd.y;
-        cfirst = nd.x;
-        if (!cmask) {
-          pop = true;
-        } else {
-          xh = (bx1 - bx0) / 2.0f;
-          yh = (by1 - by0) / 2.0f;
-          zh = (bz1 - bz0) / 2.0f;
-          xm = bx0 + xh;
-          ym = by0 + yh;
-          zm = bz0 + zh;
-          ci = 0;
-          enter = false;
-        }
-      }
-    }
-    if (!pop) {
-      // Next child in perturbed parametric order (3 and 4 swapped).
-      bool down = false;
-      while (ci < 8) {
-        uint32_t i = ci++;
-        uint32_t child = (i == 3) ? 4u : (i == 4) ? 3u : i;
-        uint32_t real = child ^ oct;
-        if (!(cmask & (1u << real))) continue;
-        float t0 = ray.t0, t1 = ray.t1;
-        if (((child & 4) ? (t1 > xm) : (t0 < xm)) && ((child & 2) ? (t1 > ym) : (t0 < ym)) &&
-            ((child & 1) ? (t1 > zm) : (t0 < zm))) {
-          st_node[sp] = cn | (ci << 28);
-          st_b[sp][0] = bx0; st_b[sp][1] = bx1; st_b[sp][2] = by0;
-          st_b[sp][3] = by1; st_b[sp][4] = bz0; st_b[sp][5] = bz1;
-          sp++;
-          float cx = bx0 + ((child & 4) ? xh : 0.0f);
-          float cy = by0 + ((child & 2) ? yh : 0.0f);
-          float cz = bz0 + ((child & 1) ? zh : 0.0f);
-          bx0 = cx; bx1 = cx + xh;
-          by0 = cy; by1 = cy + yh;
-          bz0 = cz; bz1 = cz + zh;
-          cn = cfirst + __popc(cmask & ((1u << real) - 1u));
-          enter = true;
-          down = true;
-          break;
-        }
-      }
-      if (down) continue;
-    }
-    // Return to the parent frame.
-    if (sp == 0) break;
-    sp--;
-    uint32_t packed = st_node[sp];
-    cn = packed & 0x0fffffffu;
-    ci = packed >> 28;
-    bx0 = st_b[sp][0]; bx1 = st_b[sp][1]; by0 = st_b[sp][2];
-    by1 = st_b[sp][3]; bz0 = st_b[sp][4]; bz1 = st_b[sp][5];
+    // Enter node cn: its surfaces, then set up the child walk.
     uint4 nd = S.nodes[cn];
-    cmask = nd.y;
-    cfirst = nd.x;
-    xh = (bx1 - bx0) / 2.0f;
-    yh = (by1 - by0) / 2.0f;
-    zh = (bz1 - bz0) / 2.0f;
-    xm = bx0 + xh;
-    ym = by0 + yh;
-    zm = bz0 + zh;
-    enter = false;
+    for (uint32_t k = nd.z, e = nd.z + nd.w; k < e; k++) {
+      cnt.tests++;
+      if (ref_hit(S.refs, k, ray, t, u, v, prim)) {
+        if (ANY) return prim;
+        ray.t1 = t;
+        hit = prim;
+        hu = u;
+        hv = v;
+      }
+    }
+    uint32_t mask = nd.y, first = nd.x;
+    uint32_t vis = visit_mask(mask, oct);
+    uint32_t ci = 0;
+    float xh = 0, yh = 0, zh = 0, xm = 0, ym = 0, zm = 0;
+    if (vis) {
+      xh = (bx1 - bx0) / 2.0f;
+      yh = (by1 - by0) / 2.0f;
+      zh = (bz1 - bz0) / 2.0f;
+      xm = bx0 + xh;
+      ym = by0 + yh;
+      zm = bz0 + zh;
+    }
+    bool entered = false;
+    for (;;) {
+      // Next existing child in visiting order.
+      uint32_t rest = vis >> ci;
+      if (rest) {
+        uint32_t i = ci + (uint32_t)__builtin_ctz(rest);
+        ci = i + 1;
+        uint32_t child = (i == 3) ? 4u : (i == 4) ? 3u : i;
+        float t0 = ray.t0, t1 = ray.t1;
+        if (!(((child & 4) ? (t1 > xm) : (t0 < xm)) && ((child & 2) ? (t1 > ym) : (t0 < ym)) &&
+              ((child & 1) ? (t1 > zm) : (t0 < zm))))
+          continue;
+        float cx = bx0 + ((child & 4) ? xh : 0.0f);
+        float cy = by0 + ((child & 2) ? yh : 0.0f);
+        float cz = bz0 + ((child & 1) ? zh : 0.0f);
+        float cx1 = cx + xh, cy1 = cy + yh, cz1 = cz + zh;
+        cnt.nodes++;
+        if (fmx(t0, fmx(cx, fmx(cy, cz))) >= fmn(t1, fmn(cx1, fmn(cy1, cz1)))) continue;
+        uint32_t real = child ^ oct;
+        if (vis >> ci) {  // siblings left: keep this node's walk state
+          st0[sp] = make_float4(__uint_as_float(first | (ci << 28)),
+                                __uint_as_float(mask | (vis << 8)), bx0, bx1);
+          st1[sp] = make_float4(by0, by1, bz0, bz1);
+          sp++;
+        }
+        cn = first + __popc(mask & ((1u << real) - 1u));
+        bx0 = cx; bx1 = cx1;
+        by0 = cy; by1 = cy1;
+        bz0 = cz; bz1 = cz1;
+        entered = true;
+        break;
+      }
+      // This node is done: resume the nearest pending ancestor.
+      if (sp == 0) break;
+      sp--;
+      float4 a = st0[sp], b = st1[sp];
+      uint32_t w0 = __float_as_uint(a.x), w1 = __float_as_uint(a.y);
+      first = w0 & 0x0fffffffu;
+      ci = w0 >> 28;
+      mask = w1 & 0xffu;
+      vis = (w1 >> 8) & 0xffu;
+      bx0 = a.z; bx1 = a.w; by0 = b.x; by1 = b.y; bz0 = b.z; bz1 = b.w;
+      xh = (bx1 - bx0) / 2.0f;
+      yh = (by1 - by0) / 2.0f;
+      zh = (bz1 - bz0) / 2.0f;
+      xm = bx0 + xh;
+      ym = by0 + yh;
+      zm = bz0 + zh;
+    }
+    if (!entered) break;
   }
   return hit;
 }

@@ -1,0 +1,185 @@
+"""Render parity on the GPU against the CPU oracle.
+
+The device path tracer and the oracle draw the same counter-based sample
+stream, so each path can be compared directly: camera jitter must be
+identical and path radiance equal to rounding (paths whose transcendental
+results differ in the last ulp may branch differently, so a small
+fraction is allowed to diverge).  Pixel statistics are checked with the
+3-standard-error test on independent seeds, and the closed-form furnace
+pins the estimator's expectation."""
+import numpy as np
+import pytest
+
+import oracle
+from snogray_amd import abi, native, scenes
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def compare_samples(got, ref, rtol):
+    assert got.shape == ref.shape
+    assert np.array_equal(got[:, :2], ref[:, :2])  # sx, sy
+    rel = np.abs(got[:, 2:] - ref[:, 2:]) / (np.abs(ref[:, 2:]) + 1e-3)
+    return float(np.mean(rel.max(1) <= rtol))
+
+
+def test_cornell_paths_match_oracle(ctx):
+    sd, cam = scenes.cornell_box()
+    sc, osc = native.Scene(ctx, sd), oracle.OracleScene(sd)
+    p = abi.make_params(48, 40, 16, max_path_len=8)
+    got, gst = sc.render_samples(cam, p)
+    _, ref, ost = osc.render(cam, p, samples=True)
+    assert compare_samples(got, ref, 1e-4) > 0.995
+    for k in ("eye_rays", "scene_intersect_calls", "shadow_tests", "illum_calls"):
+        assert abs(gst[k] - ost[k]) <= 0.002 * ost[k], k
+
+
+def test_soup_paths_match_oracle(ctx):
+    """Cook-Torrance, glass (media stack) and the envmap light."""
+    sd, cam = helpers.small_soup(level=4, n_spheres=6, seed=3)
+    sc, osc = native.Scene(ctx, sd), oracle.OracleScene(sd)
+    p = abi.make_params(64, 36, 8, max_path_len=10)
+    got, _ = sc.render_samples(cam, p)
+    _, ref, _ = osc.render(cam, p, samples=True)
+    assert compare_samples(got, ref, 1e-3) > 0.97
+
+
+def test_lights_and_materials_paths_match_oracle(ctx):
+    """Sphere light, mirror with an underlying Lambert, glass sphere,
+    direct_samples = 2."""
+    sd = scenes.SceneDesc()
+    wall = sd.lambert((0.6, 0.6, 0.6))
+    mir = sd.mirror((0.9, 0.9, 0.9), ior=1.5, color=(0.2, 0.1, 0.1))
+    gl = sd.glass(1.5)
+    glow = sd.glow((20.0, 18.0, 15.0))
+    ct = sd.cook_torrance((0.3, 0.2, 0.1), (0.6, 0.6, 0.6), 0.2, 1.6)
+    sd.tripar((-5, 0, -5), (0, 0, 10), (10, 0, 0), wall, parallelogram=True)
+    sd.tripar((-5, 0, 5), (10, 0, 0), (0, 6, 0), ct, parallelogram=True)
+    sd.sphere((-1.5, 1, 1), 1.0, mir)
+    sd.sphere((1.5, 1, 0.5), 1.0, gl)
+    sd.sphere((0, 4.5, 0), 0.5, glow)
+    cam = scenes.look_at((0, 2, -8), (0, 1.5, 0), vfov_deg=45.0)
+    sc, osc = native.Scene(ctx, sd), oracle.OracleScene(sd)
+    p = abi.make_params(40, 40, 8, max_path_len=8, direct_samples=2)
+    got, _ = sc.render_samples(cam, p)
+    _, ref, _ = osc.render(cam, p, samples=True)
+    assert compare_samples(got, ref, 1e-3) > 0.97
+
+
+def test_accumulators_match_oracle(ctx):
+    sd, cam = scenes.cornell_box()
+    sc, osc = native.Scene(ctx, sd), oracle.OracleScene(sd)
+    p = abi.make_params(64, 64, 16, max_path_len=8)
+    acc, _ = sc.render(cam, p)
+    oacc, _, _ = osc.render(cam, p)
+    np.testing.assert_allclose(acc[..., 4], oacc[..., 4], rtol=1e-5, atol=1e-5)
+    img = native.finalize(acc)
+    oimg = acc_finalize_numpy(oacc)
+    close = np.abs(img[..., :3] - oimg[..., :3]) <= 1e-3 * (np.abs(oimg[..., :3]) + 1e-2)
+    assert close.mean() > 0.99
+
+
+def acc_finalize_numpy(acc):
+    """ImageSampledOutput::set_raw_min_y (image/image-sampled-output.cc:63-126)."""
+    c, a, w = acc[..., :3].copy(), acc[..., 3].copy(), acc[..., 4]
+    m = (w > 0) & (w != 1)
+    c[m] /= w[m, None]
+    a[m] /= w[m]
+    a = np.clip(a, 0, 1)
+    c = np.maximum(c, 0)
+    m = (a != 0) & (a != 1)
+    c[m] /= a[m, None]
+    return np.concatenate([c, a[..., None]], -1)
+
+
+def test_finalize_matches_numpy():
+    rng = np.random.default_rng(1)
+    acc = rng.normal(size=(8, 9, 5)).astype(np.float32)
+    acc[..., 4] = np.abs(acc[..., 4]) + 0.1
+    acc[0, 0, 4] = 1.0
+    np.testing.assert_allclose(native.finalize(acc), acc_finalize_numpy(acc), rtol=1e-6)
+
+
+def test_furnace_closed_form(ctx):
+    """Closed box of emitting Lambert walls: E * sum_{j<=M+1} a^j."""
+    a, M = 0.6, 5
+    sd, cam = scenes.furnace_box(albedo=a, emission=1.0)
+    sc = native.Scene(ctx, sd)
+    got, _ = sc.render_samples(cam, abi.make_params(32, 32, 64, max_path_len=M,
+                                                    min_path_len=50, seed=3))
+    v = got[:, 2].reshape(-1, 64).mean(1)
+    expect = sum(a ** j for j in range(M + 2))
+    se = v.std() / np.sqrt(len(v))
+    assert abs(v.mean() - expect) < 4 * se + 1e-4
+
+
+def image_stats(render, seeds):
+    imgs = np.stack([render(s) for s in seeds])
+    return imgs.mean(0), imgs.std(0, ddof=1) / np.sqrt(len(seeds))
+
+
+def test_pixel_means_within_3se_of_oracle(ctx):
+    """Independent seeds on the GPU and in the CPU oracle: per-pixel and
+    channel means agree within 3 combined standard errors."""
+    sd, cam = scenes.cornell_box()
+    sc, osc = native.Scene(ctx, sd), oracle.OracleScene(sd)
+
+    def g(seed):
+        acc, _ = sc.render(cam, abi.make_params(24, 24, 16, max_path_len=8, seed=seed))
+        return native.finalize(acc)[..., :3]
+
+    def o(seed):
+        acc, _, _ = osc.render(cam, abi.make_params(24, 24, 16, max_path_len=8, seed=seed))
+        return acc_finalize_numpy(acc)[..., :3]
+
+    mg, sg = image_stats(g, range(100, 116))
+    mo, so = image_stats(o, range(200, 216))
+    z = np.abs(mg - mo) / np.sqrt(sg ** 2 + so ** 2 + 1e-12)
+    assert np.mean(z <= 3) > 0.97
+
+
+def test_determinism_and_chunking(ctx):
+    """Same seed: identical samples whatever the number of paths in flight;
+    repeated renders give identical accumulators."""
+    sd, cam = scenes.cornell_box()
+    sc = native.Scene(ctx, sd)
+    p1 = abi.make_params(40, 30, 9, max_path_len=8, seed=5)
+    p2 = abi.make_params(40, 30, 9, max_path_len=8, seed=5, paths_in_flight=2000)
+    a, _ = sc.render_samples(cam, p1)
+    b, _ = sc.render_samples(cam, p2)
+    assert np.array_equal(a, b)
+    x, _ = sc.render(cam, p1)
+    y, _ = sc.render(cam, p1)
+    assert np.array_equal(x, y)
+    z, _ = sc.render(cam, p2)
+    np.testing.assert_allclose(x, z, rtol=1e-5, atol=1e-6)
+
+
+def test_window_tiles_equal_full_render(ctx):
+    """Rendering the image as two windows gives the same paths as one
+    full render (render-mgr/render-pattern.h windows plus filter margin)."""
+    sd, cam = scenes.cornell_box()
+    sc = native.Scene(ctx, sd)
+    full = abi.make_params(32, 32, 4, max_path_len=8, seed=9)
+    accf, _ = sc.render(cam, full)
+    acct = np.zeros_like(accf)
+    for win in ((0, 0, 32, 16), (0, 16, 32, 16)):
+        acct, _ = sc.render(cam, abi.make_params(32, 32, 4, max_path_len=8, seed=9,
+                                                 window=win), acct)
+    # Each tile also traces its 2-row filter margin, which splats 2 rows
+    # further: rows 12-19 receive both tiles' margin samples, all other
+    # rows see exactly the full render's samples in the same order.
+    assert np.array_equal(accf[:12], acct[:12])
+    assert np.array_equal(accf[20:], acct[20:])
+
+
+def test_invalid_parameters_raise(ctx):
+    sd, cam = scenes.cornell_box()
+    sc = native.Scene(ctx, sd)
+    with pytest.raises(native.SgrError, match="spp"):
+        sc.render(cam, abi.make_params(8, 8, 0))
+    with pytest.raises(native.SgrError, match="window"):
+        sc.render(cam, abi.make_params(8, 8, 1, window=(4, 4, 8, 8)))
+    with pytest.raises(native.SgrError, match="max_path_len"):
+        sc.render(cam, abi.make_params(8, 8, 1, max_path_len=100))
