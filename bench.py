@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run one plain 256 MiB device copy (reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -126,6 +129,11 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / args.steps
+    if args.calibrate:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = src.clone()
+        torch.cuda.synchronize()
+        log(f"calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} written")
     samples = W * H * SPP
     value = samples * args.steps / wall
     assert torch.isfinite(acc).all().item()

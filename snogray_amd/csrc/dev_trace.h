@@ -204,9 +204,28 @@ __device__ uint32_t trace_ray(const SceneView &S, RayF &ray, float &hu, float &h
         cnt.nodes++;
         if (fmx(t0, fmx(cx, fmx(cy, cz))) >= fmn(t1, fmn(cx1, fmn(cy1, cz1)))) continue;
         uint32_t real = child ^ oct;
-        if (vis >> ci) {  // siblings left: keep this node's walk state
+        // Siblings still to visit: drop those that already fail the prune
+        // test with the current ray extent -- t0 is fixed and t1 only
+        // shrinks, so they would fail when their turn comes too.
+        uint32_t later = 0;
+        for (uint32_t r2 = vis >> ci; r2; r2 &= r2 - 1) {
+          uint32_t j = ci + (uint32_t)__builtin_ctz(r2);
+          uint32_t ch = (j == 3) ? 4u : (j == 4) ? 3u : j;
+          if (!(((ch & 4) ? (t1 > xm) : (t0 < xm)) && ((ch & 2) ? (t1 > ym) : (t0 < ym)) &&
+                ((ch & 1) ? (t1 > zm) : (t0 < zm))))
+            continue;
+          float sx = bx0 + ((ch & 4) ? xh : 0.0f);
+          float sy = by0 + ((ch & 2) ? yh : 0.0f);
+          float sz = bz0 + ((ch & 1) ? zh : 0.0f);
+          if (fmx(t0, fmx(sx, fmx(sy, sz))) >= fmn(t1, fmn(sx + xh, fmn(sy + yh, sz + zh)))) {
+            cnt.nodes++;  // a call the recursive search makes and prunes at once
+            continue;
+          }
+          later |= 1u << j;
+        }
+        if (later) {  // siblings left: keep this node's walk state
           st0[sp] = make_float4(__uint_as_float(first | (ci << 28)),
-                                __uint_as_float(mask | (vis << 8)), bx0, bx1);
+                                __uint_as_float(mask | (later << 8)), bx0, bx1);
           st1[sp] = make_float4(by0, by1, bz0, bz1);
           sp++;
         }

@@ -841,7 +841,7 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   path_bytes(cap, R.K, &P, (char *)ctx->work);
   HIPCHK(hipMemsetAsync(ctx->stats, 0, SGR_NUM_STATS * sizeof(unsigned long long), ctx->stream));
   unsigned grid = (unsigned)std::min<size_t>((cap + 255) / 256, (size_t)ctx->num_cu * 16);
-  if (grid == 0) grid = 1;
+  grid = (grid + 7) & ~7u;  // whole blocks per XCD (see block_range)
   std::vector<float> chunk_out;
   uint32_t n_ext = 0, n_occ = 0;
   // Per-launch timing: event pairs recorded on the render stream without

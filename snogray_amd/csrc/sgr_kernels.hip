@@ -62,6 +62,18 @@ __global__ void __launch_bounds__(256) k_trace_any(SceneView S, RayBatchDev rb, 
   wave_count(&stats[1], cnt.tests);
 }
 
+// Queue range of this block.  Hardware deals workgroups to the 8 XCDs
+// round-robin; remapping so that the blocks of one XCD take one
+// contiguous slice of the queue keeps neighbouring rays (neighbouring
+// pixels) on the same L2.  The grid is a multiple of 8.
+__device__ __forceinline__ void block_range(uint32_t n, uint32_t &beg, uint32_t &end) {
+  const uint32_t G = gridDim.x;
+  const uint32_t lb = (blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
+  const uint32_t per = (n + G - 1) / G;
+  beg = lb * per;
+  end = beg + per < n ? beg + per : n;
+}
+
 // ---------------------------------------------------------------------------
 // Wavefront path tracer
 // ---------------------------------------------------------------------------
@@ -126,7 +138,9 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S, PathBufs P, uint32_
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   const uint32_t *q = P.q[b & 1];
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+  uint32_t jb, je;
+  block_range(n, jb, je);
+  for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
     uint32_t p = q[j];
     RayF r;
     load_ray(P, p, r);
@@ -145,7 +159,9 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S, PathBufs P, uint32
                                                  unsigned long long *stats) {
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_S + b];
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+  uint32_t jb, je;
+  block_range(n, jb, je);
+  for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
     uint32_t slot = P.sq[j];
     RayF r;
     r.o = v3(P.sray[0][slot], P.sray[1][slot], P.sray[2][slot]);
@@ -170,7 +186,9 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S, RenderView R, PathBu
   uint32_t *sq_count = &P.counts[SGR_CNT_S + b];
   uint32_t illum = 0;
   const uint32_t N = R.npaths;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+  uint32_t jb, je;
+  block_range(n, jb, je);
+  for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
     uint32_t p = q[j];
     uint32_t st = P.state[p];
     uint32_t path_len = st & SGR_ST_LEN_MASK;
@@ -349,7 +367,9 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S, RenderView R, PathBu
 __global__ void __launch_bounds__(256) k_gather(RenderView R, PathBufs P, uint32_t b) {
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   const uint32_t *q = P.q[b & 1];
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+  uint32_t jb, je;
+  block_range(n, jb, je);
+  for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
     uint32_t p = q[j];
     uint32_t st = P.state[p];
     if (!(st & (SGR_ST_PENDING | SGR_ST_DONE))) continue;
