@@ -290,6 +290,76 @@ int sgr_render_samples(sgr_scene *scene, const sgr_camera *cam,
 int sgr_finalize_image(const float *accum, uint32_t width, uint32_t height,
                        float exposure, float contrast, float *rgba);
 
+/* ---- job engine: many renders sharing one stream of launches ----------
+ * The reference renders one image per RenderMgr::render call
+ * (render-mgr/render-mgr.cc:35-63); these calls have no counterpart
+ * there.  A job is one sgr_render (same parameters, same result bits:
+ * each job keeps the sample chunking sgr_render would use).  An engine
+ * step takes the next sample chunk of as many admitted jobs as fit the
+ * path capacity and runs them through one raygen/extend/shade/occlude/
+ * gather/splat sequence; a job contributes at most one chunk per step.
+ * Not thread-safe: one host thread drives a context's engine. */
+typedef struct sgr_engine_config {
+    uint64_t path_capacity;   /* paths in flight per step (0: 1<<22) */
+    uint64_t memory_budget;   /* bytes of job accumulators admitted at
+                                 once (0: no limit) */
+    uint32_t grid_blocks;     /* workgroups per launch (0: 16 per CU);
+                                 rounded up to a multiple of 8 */
+    uint32_t use_graph;       /* replay each step from a captured
+                                 hipGraph (one per bounce count) */
+} sgr_engine_config;
+
+enum { SGR_JOB_QUEUED = 0, SGR_JOB_ACTIVE = 1, SGR_JOB_DONE = 2 };
+
+typedef struct sgr_job_status {
+    int32_t  state;           /* SGR_JOB_* */
+    uint32_t chunks_done;
+    uint32_t chunks_total;
+    uint32_t steps;           /* engine steps that carried the job */
+    double   t_submit_ms;     /* host clock, ms since engine creation */
+    double   t_start_ms;      /* first step carrying the job began */
+    double   t_done_ms;       /* its last step finished */
+    uint64_t footprint;       /* bytes counted against memory_budget */
+} sgr_job_status;
+
+typedef struct sgr_step_info {
+    uint32_t jobs;            /* jobs in the step */
+    uint32_t jobs_finished;   /* jobs that completed in it */
+    uint64_t paths;
+    double   ms_device;       /* device time of the step */
+    uint32_t max_path_len;    /* bounces launched */
+    uint32_t graph;           /* 1: replayed from a captured graph */
+} sgr_step_info;
+
+/* (Re)configure the context's engine; only while it holds no jobs. */
+int sgr_engine_configure(sgr_ctx *ctx, const sgr_engine_config *cfg);
+/* Bytes a job with these parameters counts against memory_budget
+ * (engine-owned accumulators; 0 when the caller supplies them). */
+int sgr_job_footprint(const sgr_render_params *params, int caller_accum,
+                      uint64_t *bytes);
+/* Queue a job.  accum_dev: device accumulators (5 floats per pixel,
+ * added to) or NULL for engine-owned zeroed ones.  The scene must stay
+ * alive until the job is released (sgr_scene_destroy refuses before). */
+int sgr_job_submit(sgr_ctx *ctx, sgr_scene *scene, const sgr_camera *cam,
+                   const sgr_render_params *params, float *accum_dev,
+                   uint64_t *job_id);
+/* Admit queued jobs that fit the budget, then run one step.  info may be
+ * NULL; info->jobs == 0 means there was nothing to run. */
+int sgr_engine_step(sgr_ctx *ctx, sgr_step_info *info);
+int sgr_job_poll(sgr_ctx *ctx, uint64_t job_id, sgr_job_status *status);
+/* Copy a finished job's accumulators (width*height*5 floats) to host
+ * (dst_on_device == 0) or device memory. */
+int sgr_job_result(sgr_ctx *ctx, uint64_t job_id, float *dst, int dst_on_device);
+/* Forget a job (any state); frees engine-owned accumulators. */
+int sgr_job_release(sgr_ctx *ctx, uint64_t job_id);
+/* Engine-wide counters since configure (same meaning as sgr_render's). */
+int sgr_engine_stats(sgr_ctx *ctx, sgr_stats *stats);
+/* Submit n jobs, step until all are done, copy results, release them.
+ * accum[i] is device memory (on_device) or host memory; added to. */
+int sgr_render_batch(sgr_ctx *ctx, uint32_t n, sgr_scene *const *scenes,
+                     const sgr_camera *cams, const sgr_render_params *params,
+                     float *const *accum, int on_device, sgr_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

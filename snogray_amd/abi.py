@@ -103,6 +103,33 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class EngineConfig(C.Structure):
+    _fields_ = [("path_capacity", C.c_uint64), ("memory_budget", C.c_uint64),
+                ("grid_blocks", C.c_uint32), ("use_graph", C.c_uint32)]
+
+
+SGR_JOB_QUEUED, SGR_JOB_ACTIVE, SGR_JOB_DONE = 0, 1, 2
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("chunks_done", C.c_uint32),
+                ("chunks_total", C.c_uint32), ("steps", C.c_uint32),
+                ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double),
+                ("t_done_ms", C.c_double), ("footprint", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class StepInfo(C.Structure):
+    _fields_ = [("jobs", C.c_uint32), ("jobs_finished", C.c_uint32),
+                ("paths", C.c_uint64), ("ms_device", C.c_double),
+                ("max_path_len", C.c_uint32), ("graph", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class OctreeStats(C.Structure):
     _fields_ = [("num_nodes", C.c_uint64), ("num_leaf_nodes", C.c_uint64),
                 ("num_surfaces", C.c_uint64),

@@ -53,6 +53,23 @@ struct RenderView {
   uint32_t npaths;        // PW * PH * sb
   float cam_pos[3], fwd[3], up[3], right[3];
   float tanx, tany, max_trace;
+  // Multi-job steps: scene table slot, first path of this segment in the
+  // step, first output pixel of this segment in the step's splat, and the
+  // job's accumulators (device).
+  uint32_t scene, path_base, out_base;
+  float *accum;
+};
+
+// One engine step over several jobs: per-segment render views, per-slot
+// scene views, and prefix offsets of the segments' paths and splat pixels.
+#define SGR_MAX_SEGMENTS 256
+struct StepView {
+  const RenderView *R;
+  const SceneView *S;
+  const uint32_t *seg_base;  // nseg + 1 path offsets
+  const uint32_t *out_base;  // nseg + 1 splat-pixel offsets
+  const uint32_t *hdr;       // nseg, npaths, nout (device: a captured step
+                             // graph replays with new tables)
 };
 
 struct PathBufs {
@@ -70,21 +87,27 @@ struct PathBufs {
   uint32_t *sq;       // shadow queue
   uint32_t *counts;   // SGR_NUM_COUNTERS
   float *out;         // 6 floats per path: sx sy r g b a
+  uint32_t *pseg;     // segment of each path (multi-job steps)
+  uint32_t n;         // path capacity (stride of the media stack)
+  uint32_t K;         // shadow slots per path
 };
 
 void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool any,
                   unsigned long long *stats, hipStream_t st);
-void launch_raygen(const RenderView &R, const PathBufs &P, hipStream_t st);
-void launch_extend(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
-                   unsigned long long *stats, unsigned grid, hipStream_t st);
-void launch_shade(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
-                  unsigned long long *stats, unsigned grid, hipStream_t st);
-void launch_occlude(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
-                    unsigned long long *stats, unsigned grid, hipStream_t st);
-void launch_gather(const RenderView &R, const PathBufs &P, uint32_t b, unsigned grid,
-                   hipStream_t st);
-void launch_splat(const RenderView &R, const float *out, float *accum, hipStream_t st);
-void launch_count_queues(const RenderView &R, const PathBufs &P, unsigned long long *stats,
+void launch_raygen(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
+                   unsigned grid, hipStream_t st);
+void launch_extend(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
+                   uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st);
+void launch_shade(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
+                  const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
+                  hipStream_t st);
+void launch_occlude(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
+                    uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st);
+void launch_gather(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
+                   uint32_t b, unsigned grid, hipStream_t st);
+void launch_splat(bool multi, const RenderView &R, const StepView &J, const float *out,
+                  unsigned grid, hipStream_t st);
+void launch_count_queues(uint32_t maxp, const PathBufs &P, unsigned long long *stats,
                          hipStream_t st);
 
 }  // namespace sgr

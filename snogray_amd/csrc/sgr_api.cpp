@@ -15,10 +15,11 @@
 #include "../../include/snoggpu.h"
 #include "kernels.h"
 #include "octree_build.h"
+#include "sgr_internal.h"
 
 using namespace sgr;
 
-namespace {
+namespace sgr {
 
 thread_local std::string g_err;
 
@@ -27,43 +28,7 @@ int fail(int code, const std::string &msg) {
   return code;
 }
 
-#define HIPCHK(expr)                                                              \
-  do {                                                                            \
-    hipError_t e_ = (expr);                                                       \
-    if (e_ != hipSuccess)                                                         \
-      return fail(SGR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-struct DevBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-};
-
-}  // namespace
-
-struct sgr_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  uint64_t bytes = 0;          // device bytes held (scenes + work buffers)
-  // render work buffers, grown on demand
-  void *work = nullptr;
-  size_t work_bytes = 0;
-  unsigned long long *stats = nullptr;
-  int num_cu = 256;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  std::vector<hipEvent_t> evpool;  // per-launch timing events
-  uint32_t live_scenes = 0;
-};
-
-struct sgr_scene {
-  sgr_ctx *ctx = nullptr;
-  SceneView view{};
-  std::vector<void *> allocs;
-  uint64_t bytes = 0;
-  FlatOctree oct;
-  float bbox_lo[3], bbox_hi[3];
-  uint32_t n_lights = 0;
-};
+}  // namespace sgr
 
 namespace {
 
@@ -163,7 +128,7 @@ Grid grid_shape(uint32_t n) {
 
 extern "C" {
 
-const char *sgr_last_error(void) { return g_err.c_str(); }
+const char *sgr_last_error(void) { return sgr::g_err.c_str(); }
 
 int sgr_ctx_create(int device, sgr_ctx **out) {
   if (!out) return fail(SGR_ERR_INVALID, "sgr_ctx_create: null out");
@@ -193,7 +158,7 @@ int sgr_ctx_create(int device, sgr_ctx **out) {
 int sgr_ctx_destroy(sgr_ctx *ctx) {
   if (!ctx) return SGR_OK;
   if (ctx->live_scenes)
-    return fail(SGR_ERR_INVALID, "sgr_ctx_destroy: " + std::to_string(ctx->live_scenes) +
+    return sgr::fail(SGR_ERR_INVALID, "sgr_ctx_destroy: " + std::to_string(ctx->live_scenes) +
                                      " scene(s) still alive; destroy them first");
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
@@ -202,6 +167,7 @@ int sgr_ctx_destroy(sgr_ctx *ctx) {
   for (auto &ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto &ev : ctx->evpool) (void)hipEventDestroy(ev);
+  sgr::engine_destroy(ctx);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SGR_OK;
@@ -561,6 +527,9 @@ int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
 
 int sgr_scene_destroy(sgr_scene *sc) {
   if (!sc) return SGR_OK;
+  if (sc->jobs_using)
+    return fail(SGR_ERR_INVALID, "sgr_scene_destroy: " + std::to_string(sc->jobs_using) +
+                                     " job(s) still refer to the scene; release them first");
   if (sc->ctx) {
     (void)hipSetDevice(sc->ctx->device);
     (void)hipStreamSynchronize(sc->ctx->stream);
@@ -703,7 +672,9 @@ int sgr_trace_occluded(sgr_scene *sc, const sgr_rays *rays, size_t n, uint8_t *o
 // ---------------------------------------------------------------------------
 // Rendering
 // ---------------------------------------------------------------------------
-namespace {
+}  // extern "C"
+
+namespace sgr {
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -740,15 +711,9 @@ size_t path_bytes(size_t cap, size_t K, PathBufs *P, char *base) {
   r = take(ns * 4); if (P) P->sq = (uint32_t *)r;
   r = take(SGR_NUM_COUNTERS * 4); if (P) P->counts = (uint32_t *)r;
   r = take(cap * 6 * 4); if (P) P->out = (float *)r;
+  r = take(cap * 4); if (P) P->pseg = (uint32_t *)r;
   return off;
 }
-
-struct RenderPlan {
-  RenderView R;
-  uint32_t flags;
-  uint32_t sb_max;
-  uint32_t npix;
-};
 
 int plan_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p, RenderPlan &pl) {
   if (!sc || !cam || !p) return fail(SGR_ERR_INVALID, "sgr_render: null argument");
@@ -828,6 +793,10 @@ int ensure_work(sgr_ctx *ctx, size_t bytes) {
   return SGR_OK;
 }
 
+}  // namespace sgr
+
+namespace {
+
 // Runs the chunks; per chunk, optional host copy of the per-sample output.
 int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_host,
                sgr_stats *stats) {
@@ -871,35 +840,43 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     marks.push_back({kind, mark0});
   };
   HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+  StepView J;
+  std::memset(&J, 0, sizeof J);
+  R.path_base = 0;
+  R.out_base = 0;
+  R.scene = 0;
+  R.accum = accum_dev;
+  P.K = R.K ? R.K : 1;
   for (uint32_t s0 = 0; s0 < R.spp; s0 += pl.sb_max) {
     R.s0 = s0;
     R.sb = std::min(pl.sb_max, R.spp - s0);
     R.npaths = pl.npix * R.sb;
+    P.n = (uint32_t)cap;
     t_begin();
-    launch_raygen(R, P, ctx->stream);
+    launch_raygen(false, R, J, P, grid, ctx->stream);
     t_end(K_OTHER);
     for (uint32_t b = 0; b <= R.maxp; b++) {
       t_begin();
-      launch_extend(sc->view, R, P, b, ctx->stats, grid, ctx->stream);
+      launch_extend(false, sc->view, J, P, b, ctx->stats, grid, ctx->stream);
       t_end(K_EXT);
       t_begin();
-      launch_shade(sc->view, R, P, b, ctx->stats, grid, ctx->stream);
+      launch_shade(false, sc->view, R, J, P, b, ctx->stats, grid, ctx->stream);
       t_end(K_SHADE);
       t_begin();
-      launch_occlude(sc->view, R, P, b, ctx->stats, grid, ctx->stream);
+      launch_occlude(false, sc->view, J, P, b, ctx->stats, grid, ctx->stream);
       t_end(K_OCC);
       t_begin();
-      launch_gather(R, P, b, grid, ctx->stream);
+      launch_gather(false, R, J, P, b, grid, ctx->stream);
       t_end(K_OTHER);
       n_ext++;
       n_occ++;
     }
     if (accum_dev) {
       t_begin();
-      launch_splat(R, P.out, accum_dev, ctx->stream);
+      launch_splat(false, R, J, P.out, grid, ctx->stream);
       t_end(K_SPLAT);
     }
-    launch_count_queues(R, P, ctx->stats, ctx->stream);
+    launch_count_queues(R.maxp, P, ctx->stats, ctx->stream);
     HIPCHK(hipGetLastError());
     if (samples_host) {
       chunk_out.resize((size_t)R.npaths * 6);
@@ -950,6 +927,8 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
 }
 
 }  // namespace
+
+extern "C" {
 
 int sgr_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p, float *accum,
                int accum_on_device, sgr_stats *stats) {

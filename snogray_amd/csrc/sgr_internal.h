@@ -1,0 +1,67 @@
+// Library-internal state shared by the C-ABI sources (not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/snoggpu.h"
+#include "kernels.h"
+#include "octree_build.h"
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return sgr::fail(SGR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace sgr {
+struct Engine;
+}
+
+struct sgr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t bytes = 0;          // device bytes held (scenes + work buffers + jobs)
+  void *work = nullptr;        // path buffers, grown on demand
+  size_t work_bytes = 0;
+  unsigned long long *stats = nullptr;
+  int num_cu = 256;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<hipEvent_t> evpool;  // per-launch timing events
+  uint32_t live_scenes = 0;
+  sgr::Engine *engine = nullptr;   // multi-job engine (sgr_jobs.cpp)
+};
+
+struct sgr_scene {
+  sgr_ctx *ctx = nullptr;
+  sgr::SceneView view{};
+  std::vector<void *> allocs;
+  uint64_t bytes = 0;
+  sgr::FlatOctree oct;
+  float bbox_lo[3], bbox_hi[3];
+  uint32_t n_lights = 0;
+  uint32_t jobs_using = 0;         // jobs of the engine referring to it
+};
+
+namespace sgr {
+
+extern thread_local std::string g_err;
+int fail(int code, const std::string &msg);
+
+// Validated render set-up of one job (window, margin, sample layout).
+struct RenderPlan {
+  RenderView R;
+  uint32_t flags;
+  uint32_t sb_max;
+  uint32_t npix;   // pattern pixels including the filter margin
+};
+int plan_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p, RenderPlan &pl);
+// Path buffers for CAP paths with K shadow slots, carved from BASE
+// (NULL: only the size is computed).
+size_t path_bytes(size_t cap, size_t K, PathBufs *P, char *base);
+int ensure_work(sgr_ctx *ctx, size_t bytes);
+void engine_destroy(sgr_ctx *ctx);
+
+}  // namespace sgr

@@ -77,8 +77,9 @@ __device__ __forceinline__ void block_range(uint32_t n, uint32_t &beg, uint32_t 
 // ---------------------------------------------------------------------------
 // Wavefront path tracer
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void path_ident(const RenderView &R, uint32_t i, int &px, int &py,
+__device__ __forceinline__ void path_ident(const RenderView &R, uint32_t p, int &px, int &py,
                                            uint32_t &s) {
+  uint32_t i = p - R.path_base;
   uint32_t pix = i / R.sb;
   s = R.s0 + (i - pix * R.sb);
   px = R.X0 + (int)(pix % (uint32_t)R.PW);
@@ -97,12 +98,31 @@ __device__ __forceinline__ void store_ray(const PathBufs &P, uint32_t p, const R
   P.ray[6][p] = r.t0; P.ray[7][p] = r.t1;
 }
 
+// Index of the segment holding element i: largest k with base[k] <= i.
+__device__ __forceinline__ uint32_t find_segment(const uint32_t *base, uint32_t nseg, uint32_t i) {
+  uint32_t lo = 0, hi = nseg;  // base[lo] <= i < base[hi]
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) / 2;
+    if (base[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // Camera ray (render-mgr/renderer.cc:71-85, camera/camera.h:322-389) and
 // path-state reset.  Thread 0 also resets the queue counters.
-__global__ void __launch_bounds__(256) k_raygen(RenderView R, PathBufs P) {
+template<bool MULTI>
+__global__ void __launch_bounds__(256) k_raygen(RenderView R1, StepView J, PathBufs P) {
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid < SGR_NUM_COUNTERS) P.counts[gid] = gid == 0 ? R.npaths : 0u;
-  for (uint32_t i = gid; i < R.npaths; i += gridDim.x * blockDim.x) {
+  const uint32_t total = MULTI ? J.hdr[1] : R1.npaths;
+  if (gid < SGR_NUM_COUNTERS) P.counts[gid] = gid == 0 ? total : 0u;
+  for (uint32_t i = gid; i < total; i += gridDim.x * blockDim.x) {
+    uint32_t seg = 0;
+    if (MULTI) {
+      seg = find_segment(J.seg_base, J.hdr[0], i);
+      P.pseg[i] = seg;
+    }
+    const RenderView R = MULTI ? J.R[seg] : R1;
     int px, py;
     uint32_t s;
     path_ident(R, i, px, py, s);
@@ -133,7 +153,8 @@ __global__ void __launch_bounds__(256) k_raygen(RenderView R, PathBufs P) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_extend(SceneView S, PathBufs P, uint32_t b,
+template<bool MULTI>
+__global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                 unsigned long long *stats) {
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_Q + b];
@@ -142,6 +163,7 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S, PathBufs P, uint32_
   block_range(n, jb, je);
   for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
     uint32_t p = q[j];
+    const SceneView S = MULTI ? J.S[J.R[P.pseg[p]].scene] : S1;
     RayF r;
     load_ray(P, p, r);
     float u = 0.0f, v = 0.0f;
@@ -155,7 +177,8 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S, PathBufs P, uint32_
   wave_count(&stats[SGR_STAT_TESTS_C], cnt.tests);
 }
 
-__global__ void __launch_bounds__(256) k_occlude(SceneView S, PathBufs P, uint32_t b,
+template<bool MULTI>
+__global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                  unsigned long long *stats) {
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_S + b];
@@ -163,6 +186,7 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S, PathBufs P, uint32
   block_range(n, jb, je);
   for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
     uint32_t slot = P.sq[j];
+    const SceneView S = MULTI ? J.S[J.R[P.pseg[slot / P.K]].scene] : S1;
     RayF r;
     r.o = v3(P.sray[0][slot], P.sray[1][slot], P.sray[2][slot]);
     r.d = v3(P.sray[3][slot], P.sray[4][slot], P.sray[5][slot]);
@@ -177,19 +201,22 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S, PathBufs P, uint32
 }
 
 // One bounce of PathInteg::Li for every path in the ray queue.
-__global__ void __launch_bounds__(256) k_shade(SceneView S, RenderView R, PathBufs P, uint32_t b,
-                                               unsigned long long *stats) {
+template<bool MULTI>
+__global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, StepView J, PathBufs P,
+                                               uint32_t b, unsigned long long *stats) {
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   const uint32_t *q = P.q[b & 1];
   uint32_t *qn = P.q[(b + 1) & 1];
   uint32_t *qn_count = &P.counts[SGR_CNT_Q + b + 1];
   uint32_t *sq_count = &P.counts[SGR_CNT_S + b];
   uint32_t illum = 0;
-  const uint32_t N = R.npaths;
+  const uint32_t N = P.n;
   uint32_t jb, je;
   block_range(n, jb, je);
   for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
     uint32_t p = q[j];
+    const RenderView R = MULTI ? J.R[P.pseg[p]] : R1;
+    const SceneView S = MULTI ? J.S[R.scene] : S1;
     uint32_t st = P.state[p];
     uint32_t path_len = st & SGR_ST_LEN_MASK;
     bool after_spec = (st >> SGR_ST_SPEC_SHIFT) & 1u;
@@ -246,7 +273,7 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S, RenderView R, PathBu
                 bv = draw(qk, (path_len << 20) | (base + 3));
               }
               float extra = draw(qk, (path_len << 20) | (base + 4));
-              size_t slot_a = (size_t)p * R.K + 2 * (l * nsub + jj), slot_b = slot_a + 1;
+              size_t slot_a = (size_t)p * P.K + 2 * (l * nsub + jj), slot_b = slot_a + 1;
               P.svis[slot_a] = 0;
               P.svis[slot_b] = 0;
               // light sample (render/direct-illum.cc:147-197)
@@ -364,7 +391,8 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S, RenderView R, PathBu
 // Adds the unoccluded direct terms (render/direct-illum.cc:96-113 sums,
 // render/path-integ.cc:277-306 product with the path transmittance) and
 // writes finished paths: sx, sy, Tint(radiance, alpha).
-__global__ void __launch_bounds__(256) k_gather(RenderView R, PathBufs P, uint32_t b) {
+template<bool MULTI>
+__global__ void __launch_bounds__(256) k_gather(RenderView R1, StepView J, PathBufs P, uint32_t b) {
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   const uint32_t *q = P.q[b & 1];
   uint32_t jb, je;
@@ -373,6 +401,7 @@ __global__ void __launch_bounds__(256) k_gather(RenderView R, PathBufs P, uint32
     uint32_t p = q[j];
     uint32_t st = P.state[p];
     if (!(st & (SGR_ST_PENDING | SGR_ST_DONE))) continue;
+    const RenderView R = MULTI ? J.R[P.pseg[p]] : R1;
     C3 rad = c3(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
     if (st & SGR_ST_PENDING) {
       uint32_t nsub = (st & SGR_ST_STRAT) ? R.nsub_strat : R.nsub_rand;
@@ -381,7 +410,7 @@ __global__ void __launch_bounds__(256) k_gather(RenderView R, PathBufs P, uint32
       for (uint32_t l = 0; l < R.L; l++) {
         C3 lrad = c3(0.0f);
         for (uint32_t jj = 0; jj < nsub; jj++) {
-          size_t sa = (size_t)p * R.K + 2 * (l * nsub + jj);
+          size_t sa = (size_t)p * P.K + 2 * (l * nsub + jj);
           C3 r = c3(0.0f);
           if (P.svis[sa]) r = r + c3(P.sc[0][sa], P.sc[1][sa], P.sc[2][sa]);
           if (P.svis[sa + 1]) r = r + c3(P.sc[0][sa + 1], P.sc[1][sa + 1], P.sc[2][sa + 1]);
@@ -420,16 +449,33 @@ __device__ __forceinline__ float mitchell1(float x, float inv_radius, float B, f
 // weighted samples of the pattern pixels within the filter radius (a
 // sample of pattern pixel x reaches image pixels int(sx) +- radius; the
 // taps that truncation adds or drops all carry weight 0).  One thread
-// per image pixel, no atomics, fixed summation order.
-__global__ void __launch_bounds__(256) k_splat(RenderView R, const float *out, float *accum) {
-  int ox0 = R.X0 > 0 ? R.X0 : 0;
-  int oy0 = R.Y0 > 0 ? R.Y0 : 0;
+// per image pixel, no atomics, fixed summation order.  The image pixels
+// of a segment are those its pattern window covers, clipped to the image.
+__device__ __forceinline__ void splat_bounds(const RenderView &R, int &ox0, int &oy0, int &ow,
+                                             int &oh) {
+  ox0 = R.X0 > 0 ? R.X0 : 0;
+  oy0 = R.Y0 > 0 ? R.Y0 : 0;
   int ox1 = R.X0 + R.PW < (int)R.width ? R.X0 + R.PW : (int)R.width;
   int oy1 = R.Y0 + R.PH < (int)R.height ? R.Y0 + R.PH : (int)R.height;
-  int ow = ox1 - ox0, oh = oy1 - oy0;
-  if (ow <= 0 || oh <= 0) return;
-  uint32_t total = (uint32_t)(ow * oh);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+  ow = ox1 - ox0;
+  oh = oy1 - oy0;
+}
+
+template<bool MULTI>
+__global__ void __launch_bounds__(256) k_splat(RenderView R1, StepView J, const float *out) {
+  uint32_t total;
+  if (MULTI) {
+    total = J.hdr[2];
+  } else {
+    int ox0, oy0, ow, oh;
+    splat_bounds(R1, ox0, oy0, ow, oh);
+    total = (ow > 0 && oh > 0) ? (uint32_t)(ow * oh) : 0u;
+  }
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    const RenderView R = MULTI ? J.R[find_segment(J.out_base, J.hdr[0], g)] : R1;
+    int ox0, oy0, ow, oh;
+    splat_bounds(R, ox0, oy0, ow, oh);
+    uint32_t i = g - R.out_base;
     int qx = ox0 + (int)(i % (uint32_t)ow), qy = oy0 + (int)(i / (uint32_t)ow);
     float acc[5] = {0, 0, 0, 0, 0};
     int sy0 = qy - R.fr > R.Y0 ? qy - R.fr : R.Y0;
@@ -439,7 +485,7 @@ __global__ void __launch_bounds__(256) k_splat(RenderView R, const float *out, f
     for (int sy = sy0; sy <= sy1; sy++)
       for (int sx = sx0; sx <= sx1; sx++) {
         uint32_t pix = (uint32_t)(sy - R.Y0) * (uint32_t)R.PW + (uint32_t)(sx - R.X0);
-        const float *o = out + 6 * (size_t)pix * R.sb;
+        const float *o = out + 6 * ((size_t)R.path_base + (size_t)pix * R.sb);
         for (uint32_t k = 0; k < R.sb; k++, o += 6) {
           float fx = o[0], fy = o[1];
           int ix = (int)fx, iy = (int)fy;
@@ -460,18 +506,18 @@ __global__ void __launch_bounds__(256) k_splat(RenderView R, const float *out, f
           acc[0] += ws0; acc[1] += ws1; acc[2] += ws2; acc[3] += ws3; acc[4] += wr;
         }
       }
-    float *a = accum + 5 * ((size_t)qy * R.width + (size_t)qx);
+    float *a = R.accum + 5 * ((size_t)qy * R.width + (size_t)qx);
     a[0] += acc[0]; a[1] += acc[1]; a[2] += acc[2]; a[3] += acc[3]; a[4] += acc[4];
   }
 }
 
-// Adds this chunk's queue lengths (closest-hit and any-hit rays) to the
+// Adds this step's queue lengths (closest-hit and any-hit rays) to the
 // device statistics; one wave.
-__global__ void __launch_bounds__(64) k_count_queues(RenderView R, PathBufs P,
+__global__ void __launch_bounds__(64) k_count_queues(uint32_t maxp, PathBufs P,
                                                      unsigned long long *stats) {
   uint32_t b = threadIdx.x;
-  uint32_t q = b <= R.maxp ? P.counts[SGR_CNT_Q + b] : 0u;
-  uint32_t s = b <= R.maxp ? P.counts[SGR_CNT_S + b] : 0u;
+  uint32_t q = b <= maxp ? P.counts[SGR_CNT_Q + b] : 0u;
+  uint32_t s = b <= maxp ? P.counts[SGR_CNT_S + b] : 0u;
   wave_count(&stats[SGR_STAT_ISECT], q);
   wave_count(&stats[SGR_STAT_SHADOW], s);
 }
@@ -492,34 +538,42 @@ void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool an
   else hipLaunchKernelGGL(k_trace_closest, dim3(g), dim3(256), 0, st, S, rb, n, stats);
 }
 
-void launch_raygen(const RenderView &R, const PathBufs &P, hipStream_t st) {
-  hipLaunchKernelGGL(k_raygen, dim3(grid_for(R.npaths, 65535)), dim3(256), 0, st, R, P);
+void launch_raygen(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
+                   unsigned grid, hipStream_t st) {
+  if (multi) hipLaunchKernelGGL(k_raygen<true>, dim3(grid), dim3(256), 0, st, R, J, P);
+  else hipLaunchKernelGGL(k_raygen<false>, dim3(grid), dim3(256), 0, st, R, J, P);
 }
-void launch_extend(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
-                   unsigned long long *stats, unsigned grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, st, S, P, b, stats);
-  (void)R;
+void launch_extend(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
+                   uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
+  if (multi) hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
 }
-void launch_shade(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
-                  unsigned long long *stats, unsigned grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, S, R, P, b, stats);
+void launch_shade(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
+                  const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
+                  hipStream_t st) {
+  if (multi)
+    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats);
+  else
+    hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats);
 }
-void launch_occlude(const SceneView &S, const RenderView &R, const PathBufs &P, uint32_t b,
-                    unsigned long long *stats, unsigned grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_occlude, dim3(grid), dim3(256), 0, st, S, P, b, stats);
-  (void)R;
+void launch_occlude(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
+                    uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
+  if (multi) hipLaunchKernelGGL(k_occlude<true>, dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else hipLaunchKernelGGL(k_occlude<false>, dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
 }
-void launch_gather(const RenderView &R, const PathBufs &P, uint32_t b, unsigned grid,
-                   hipStream_t st) {
-  hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, st, R, P, b);
+void launch_gather(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
+                   uint32_t b, unsigned grid, hipStream_t st) {
+  if (multi) hipLaunchKernelGGL(k_gather<true>, dim3(grid), dim3(256), 0, st, R, J, P, b);
+  else hipLaunchKernelGGL(k_gather<false>, dim3(grid), dim3(256), 0, st, R, J, P, b);
 }
-void launch_count_queues(const RenderView &R, const PathBufs &P, unsigned long long *stats,
+void launch_splat(bool multi, const RenderView &R, const StepView &J, const float *out,
+                  unsigned grid, hipStream_t st) {
+  if (multi) hipLaunchKernelGGL(k_splat<true>, dim3(grid), dim3(256), 0, st, R, J, out);
+  else hipLaunchKernelGGL(k_splat<false>, dim3(grid), dim3(256), 0, st, R, J, out);
+}
+void launch_count_queues(uint32_t maxp, const PathBufs &P, unsigned long long *stats,
                          hipStream_t st) {
-  hipLaunchKernelGGL(k_count_queues, dim3(1), dim3(64), 0, st, R, P, stats);
-}
-void launch_splat(const RenderView &R, const float *out, float *accum, hipStream_t st) {
-  uint32_t n = (uint32_t)((R.PW) * (R.PH));
-  hipLaunchKernelGGL(k_splat, dim3(grid_for(n, 65535)), dim3(256), 0, st, R, out, accum);
+  hipLaunchKernelGGL(k_count_queues, dim3(1), dim3(64), 0, st, maxp, P, stats);
 }
 
 }  // namespace sgr

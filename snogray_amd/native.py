@@ -21,6 +21,9 @@ EXPORTS = [
     "sgr_octree_export", "sgr_octree_build", "sgr_scene_bounds", "sgr_scene_footprint",
     "sgr_trace_closest", "sgr_trace_occluded", "sgr_render",
     "sgr_render_samples", "sgr_finalize_image",
+    "sgr_engine_configure", "sgr_job_footprint", "sgr_job_submit", "sgr_engine_step",
+    "sgr_job_poll", "sgr_job_result", "sgr_job_release", "sgr_engine_stats",
+    "sgr_render_batch",
 ]
 
 _lib = None
@@ -65,6 +68,19 @@ def lib():
                                      C.POINTER(abi.Stats)]
     L.sgr_finalize_image.argtypes = [abi.f32p, C.c_uint32, C.c_uint32, C.c_float,
                                      C.c_float, abi.f32p]
+    L.sgr_engine_configure.argtypes = [vp, C.POINTER(abi.EngineConfig)]
+    L.sgr_job_footprint.argtypes = [C.POINTER(abi.RenderParams), C.c_int,
+                                    C.POINTER(C.c_uint64)]
+    L.sgr_job_submit.argtypes = [vp, vp, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams),
+                                 abi.f32p, C.POINTER(C.c_uint64)]
+    L.sgr_engine_step.argtypes = [vp, C.POINTER(abi.StepInfo)]
+    L.sgr_job_poll.argtypes = [vp, C.c_uint64, C.POINTER(abi.JobStatus)]
+    L.sgr_job_result.argtypes = [vp, C.c_uint64, abi.f32p, C.c_int]
+    L.sgr_job_release.argtypes = [vp, C.c_uint64]
+    L.sgr_engine_stats.argtypes = [vp, C.POINTER(abi.Stats)]
+    L.sgr_render_batch.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(abi.Camera),
+                                   C.POINTER(abi.RenderParams), C.POINTER(abi.f32p),
+                                   C.c_int, C.POINTER(abi.Stats)]
     _lib = L
     return L
 
@@ -94,10 +110,13 @@ class Context:
         check(lib().sgr_ctx_create(device, C.byref(h)), "sgr_ctx_create")
         self.h = h
         self.scenes = weakref.WeakSet()
+        self.engines = weakref.WeakSet()
         _live_contexts.add(self)
 
     def close(self):
         if getattr(self, "h", None):
+            for eng in list(self.engines):
+                eng.release_all()
             for sc in list(self.scenes):
                 sc.close()
             check(lib().sgr_ctx_destroy(self.h), "sgr_ctx_destroy")
@@ -200,6 +219,96 @@ class Scene:
         check(lib().sgr_render_samples(self.h, C.byref(cam), C.byref(params),
                                        abi.fptr(out), C.byref(st)), "sgr_render_samples")
         return out, st.as_dict()
+
+
+class Engine:
+    """The context's multi-job render engine (sgr_job_* calls).
+
+    Jobs are sgr_render calls queued on one context; ``step`` admits what
+    fits the memory budget and runs the next sample chunk of every active
+    job that fits the path capacity in one launch sequence."""
+
+    def __init__(self, ctx, path_capacity=0, memory_budget=0, grid_blocks=0, use_graph=True):
+        self.ctx = ctx
+        self.jobs = set()
+        ctx.engines.add(self)
+        self.configure(path_capacity, memory_budget, grid_blocks, use_graph)
+
+    def configure(self, path_capacity=0, memory_budget=0, grid_blocks=0, use_graph=True):
+        cfg = abi.EngineConfig(path_capacity, memory_budget, grid_blocks, int(bool(use_graph)))
+        check(lib().sgr_engine_configure(self.ctx.h, C.byref(cfg)), "sgr_engine_configure")
+        self.cfg = cfg
+
+    @staticmethod
+    def footprint(params, caller_accum=False):
+        b = C.c_uint64()
+        check(lib().sgr_job_footprint(C.byref(params), int(caller_accum), C.byref(b)),
+              "sgr_job_footprint")
+        return b.value
+
+    def submit(self, scene, cam, params, accum_ptr=None):
+        jid = C.c_uint64()
+        acc = C.cast(C.c_void_p(accum_ptr), abi.f32p) if accum_ptr else None
+        check(lib().sgr_job_submit(self.ctx.h, scene.h, C.byref(cam), C.byref(params), acc,
+                                   C.byref(jid)), "sgr_job_submit")
+        self.jobs.add(jid.value)
+        return jid.value
+
+    def step(self):
+        si = abi.StepInfo()
+        check(lib().sgr_engine_step(self.ctx.h, C.byref(si)), "sgr_engine_step")
+        return si.as_dict()
+
+    def poll(self, jid):
+        s = abi.JobStatus()
+        check(lib().sgr_job_poll(self.ctx.h, jid, C.byref(s)), "sgr_job_poll")
+        return s.as_dict()
+
+    def result(self, jid, width, height):
+        out = np.zeros((height, width, 5), np.float32)
+        check(lib().sgr_job_result(self.ctx.h, jid, abi.fptr(out), 0), "sgr_job_result")
+        return out
+
+    def result_device(self, jid, dst_ptr):
+        check(lib().sgr_job_result(self.ctx.h, jid, C.cast(C.c_void_p(dst_ptr), abi.f32p), 1),
+              "sgr_job_result")
+
+    def release(self, jid):
+        check(lib().sgr_job_release(self.ctx.h, jid), "sgr_job_release")
+        self.jobs.discard(jid)
+
+    def release_all(self):
+        for jid in list(self.jobs):
+            self.release(jid)
+
+    def stats(self):
+        st = abi.Stats()
+        check(lib().sgr_engine_stats(self.ctx.h, C.byref(st)), "sgr_engine_stats")
+        return st.as_dict()
+
+    def run(self):
+        """Step until no job has work left; returns the step infos."""
+        steps = []
+        while True:
+            si = self.step()
+            if si["jobs"] == 0:
+                return steps
+            steps.append(si)
+
+
+def render_batch(ctx, jobs):
+    """sgr_render_batch over [(scene, cam, params)]: host accumulators
+    (H, W, 5) per job and the batch's counters."""
+    n = len(jobs)
+    scs = (C.c_void_p * n)(*[j[0].h.value for j in jobs])
+    cams = (abi.Camera * n)(*[j[1] for j in jobs])
+    ps = (abi.RenderParams * n)(*[j[2] for j in jobs])
+    outs = [np.zeros((j[2].height, j[2].width, 5), np.float32) for j in jobs]
+    ptrs = (abi.f32p * n)(*[abi.fptr(o) for o in outs])
+    st = abi.Stats()
+    check(lib().sgr_render_batch(ctx.h, n, scs, cams, ps, ptrs, 0, C.byref(st)),
+          "sgr_render_batch")
+    return outs, st.as_dict()
 
 
 def octree_build(desc):
