@@ -1,11 +1,21 @@
-"""Headline benchmark: plain GPU path tracing of the seeded 1.31M-triangle
-icosphere soup at 1920x1080, 64 spp, max path length 10, on one MI355X.
+"""Benchmarks of the GPU path tracer on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--workload render|cornell|batch|serve]
 
-One step is one full render (every pixel, every sample, filter margin
-included) into device-resident accumulators; the scene is built and
-uploaded before the timed region.  Prints one JSON line.
+render (default): plain path tracing of the seeded 1.31M-triangle
+icosphere soup at 1920x1080, 64 spp, max path length 10.  One step is
+one full frame (every pixel and sample, filter margin included) into
+device-resident accumulators; the scene is built and uploaded before the
+timed region.  Beside it: the CPU renderer's samples/s on the host cores,
+and equal-time relMSE of both against a 4096-spp FP64 CPU reference on a
+centred crop.
+cornell: the Cornell box at 256x256, 16 spp, max path length 8.
+batch: 64 seeded jobs of uneven resolution, spp and scene size in the
+multi-job engine, configuration chosen by the autotuner.
+serve: Poisson arrivals of the same job distribution (4, 16, 64 jobs/s),
+continuous batching, p50/p99 latency.
+Prints one JSON line per run (per rate for serve).
 """
 import argparse
 import json
@@ -28,10 +38,19 @@ HBM_PEAK_GBS = 8000.0
 # and the ray record (32 B in; 16 B hit out / 1 B flag out + 4 B queue id).
 NODE_B, TEST_B, LIST_B = 16, 36, 4
 RAY_IN_B, HIT_OUT_B, OCC_OUT_B, QUEUE_B = 32, 16, 1, 4
+CROP_W, CROP_H, REF_SPP = 64, 36, 4096
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
 
 
 def algorithmic_bytes(st):
@@ -44,11 +63,11 @@ def algorithmic_bytes(st):
     return ext, occ
 
 
-def read_traffic(kernel):
+def read_traffic(kernel, name):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration), if a committed
     summary records it."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_icosphere_soup_1080p.txt")
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -57,60 +76,97 @@ def read_traffic(kernel):
     return None
 
 
-def cpu_baseline(sd, cam, seed):
-    """The CPU oracle (the reference's algorithm restated in C++) on a
-    bounded crop of the same workload: the centred 480x270 window of the
-    1920x1080 image at 16 spp, all allowed host cores."""
-    import oracle
-    try:
-        cores = min(len(os.sched_getaffinity(0)), 16)
-    except AttributeError:
-        cores = min(os.cpu_count() or 1, 16)
-    from snogray_amd import abi
-    osc = oracle.OracleScene(sd)
-    ww, hh, spp = 480, 270, 16
-    p = abi.make_params(W, H, spp, max_path_len=MAXP, min_path_len=MINP, seed=seed,
-                        window=((W - ww) // 2, (H - hh) // 2, ww, hh))
-    t = time.perf_counter()
-    _, _, st = osc.render(cam, p, nthreads=cores)
-    dt = time.perf_counter() - t
-    return {"value": ww * hh * spp / dt, "unit": "samples/s", "cores": cores,
-            "kind": "port",
-            "sample": f"centred {ww}x{hh} window of the 1920x1080 image, {spp} spp, "
-                      f"max_path_len {MAXP}, same scene and camera ({dt:.1f} s)"}
+def relmse(x, ref):
+    """mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2)."""
+    x = np.asarray(x, np.float64)
+    r = np.asarray(ref, np.float64)
+    return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--calibrate", action="store_true",
-                    help="also run one plain 256 MiB device copy (reference kernel "
-                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is a design in DESIGN.md, not a measured path.",
-              file=sys.stderr)
-        sys.exit(2)
+def finalize(acc):
+    from snogray_amd.serve import finalize_numpy
+    return finalize_numpy(acc)
 
+
+def crop_window(w, h, cw, ch):
+    return ((w - cw) // 2, (h - ch) // 2, cw, ch)
+
+
+# --------------------------------------------------------------------------
+# CPU baseline leg: the oracle (the reference's algorithm restated in C++)
+# timed and compared on the host cores.  Nothing else here touches it.
+# --------------------------------------------------------------------------
+class CpuBaseline:
+    def __init__(self, sd):
+        import oracle
+        self.oracle = oracle
+        self.sd = sd
+        self.cores = host_cores()
+        self._osc = {}
+
+    def scene(self, fp64=False):
+        if fp64 not in self._osc:
+            self._osc[fp64] = self.oracle.OracleScene(self.sd, fp64=fp64)
+        return self._osc[fp64]
+
+    def throughput(self, cam, width, height, maxp, minp, seed, target_s=15.0):
+        """samples/s of full-frame renders: a 1-spp pass sizes a second
+        pass of about TARGET_S seconds, which is the one reported."""
+        from snogray_amd import abi
+        osc = self.scene()
+        p = abi.make_params(width, height, 1, max_path_len=maxp, min_path_len=minp, seed=seed)
+        t = time.perf_counter()
+        osc.render(cam, p, nthreads=self.cores)
+        dt1 = time.perf_counter() - t
+        spp = int(min(64, max(1, round(target_s / dt1))))
+        p = abi.make_params(width, height, spp, max_path_len=maxp, min_path_len=minp,
+                            seed=seed + 1)
+        t = time.perf_counter()
+        osc.render(cam, p, nthreads=self.cores)
+        dt = time.perf_counter() - t
+        return {"value": width * height * spp / dt, "unit": "samples/s", "cores": self.cores,
+                "kind": "port",
+                "sample": f"full {width}x{height} frame at {spp} spp, max_path_len {maxp}, "
+                          f"same scene and camera, {dt:.1f} s on {self.cores} threads"}
+
+    def crop(self, cam, width, height, spp, maxp, minp, seed, fp64=False):
+        from snogray_amd import abi
+        x0, y0, cw, ch = crop_window(width, height, CROP_W, CROP_H)
+        p = abi.make_params(width, height, spp, max_path_len=maxp, min_path_len=minp, seed=seed,
+                            window=(x0, y0, cw, ch))
+        t = time.perf_counter()
+        acc, _, _ = self.scene(fp64).render(cam, p, nthreads=self.cores)
+        return finalize(acc[y0:y0 + ch, x0:x0 + cw]), time.perf_counter() - t
+
+    def pixel_check(self, job, desc, acc, window=16, seeds=12):
+        """3-standard-error pixel test of a served job's window against
+        independent-seed CPU renders."""
+        from snogray_amd import serve
+        osc = self.oracle.OracleScene(desc)
+        ww, hh = min(window, job.width), min(window, job.height)
+        x0, y0 = (job.width - ww) // 2, (job.height - hh) // 2
+        imgs = []
+        for k in range(seeds):
+            p = job.params(window=(x0, y0, ww, hh))
+            p.seed = 1000 + 7919 * k + job.seed % 1000
+            a, _, _ = osc.render(job.camera(), p, nthreads=self.cores)
+            imgs.append(finalize(a[y0:y0 + hh, x0:x0 + ww]))
+        frac, ok = serve.pixel_test(finalize(acc[y0:y0 + hh, x0:x0 + ww]), imgs)
+        return {"pass_fraction": frac, "passed": bool(ok)}
+
+
+# --------------------------------------------------------------------------
+def frame_bench(args, sd, cam, width, height, spp, maxp, minp, workload, traffic_file):
     import torch
-    from snogray_amd import abi, native, scenes
+    from snogray_amd import abi, native
 
-    log("building scene")
-    t0 = time.perf_counter()
-    sd, cam = scenes.icosphere_soup()
-    t_gen = time.perf_counter() - t0
     ctx = native.Context(0)
     t0 = time.perf_counter()
     sc = native.Scene(ctx, sd)
     t_build = time.perf_counter() - t0
-    acc = torch.zeros((H, W, 5), dtype=torch.float32, device="cuda")
+    acc = torch.zeros((height, width, 5), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
-    p = abi.make_params(W, H, SPP, max_path_len=MAXP, min_path_len=MINP, seed=args.seed,
+    p = abi.make_params(width, height, spp, max_path_len=maxp, min_path_len=minp, seed=args.seed,
                         flags=abi.SGR_RENDER_KERNEL_TIMING)
     for i in range(args.warmup):
         log(f"warmup {i}")
@@ -129,14 +185,18 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / args.steps
+    assert torch.isfinite(acc).all().item()
+    # one more frame alone for the image-quality comparison
+    acc.zero_()
+    sc.render_device(cam, p, acc.data_ptr())
+    gpu_img = acc.cpu().numpy()
     if args.calibrate:
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         dst = src.clone()
         torch.cuda.synchronize()
         log(f"calibration copy: {src.numel() * 4} bytes read, {dst.numel() * 4} written")
-    samples = W * H * SPP
+    samples = width * height * spp
     value = samples * args.steps / wall
-    assert torch.isfinite(acc).all().item()
 
     ext_b, occ_b = algorithmic_bytes(tot)
     kernels = {"k_extend": (tot["ms_extend"], ext_b, tot["extend_launches"]),
@@ -144,33 +204,26 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k][0])
     ms, bytes_, launches = kernels[dom]
     achieved = bytes_ / (ms / 1e3) / 1e9
-    traffic = read_traffic(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic,
+                "traffic": read_traffic(dom, traffic_file),
                 "algorithmic_bytes_per_launch": bytes_ / launches,
                 "avg_launch_ms": ms / launches}
-    cpu = None
-    if not args.no_cpu:
-        log("cpu baseline")
-        cpu = cpu_baseline(sd, cam, args.seed)
     per_step = {k: tot[k] / args.steps for k in tot}
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded icosphere soup and Gaussian-lobe envmap built in code",
-        "config": {"workload": "icosphere soup, 1920x1080, 64 spp, max path length 10",
-                   "triangles": int(sd.num_triangles), "width": W, "height": H,
-                   "spp": SPP, "max_path_len": MAXP, "min_path_len": MINP,
-                   "envmap": "1024x512 fp32", "parallelism": "single GPU"},
+        "data": "synthetic: scene built in code from a fixed seed",
+        "config": {"workload": workload, "triangles": int(sd.num_triangles), "width": width,
+                   "height": height, "spp": spp, "max_path_len": maxp, "min_path_len": minp,
+                   "parallelism": "single GPU"},
         "roofline": roofline,
-        "cpu_baseline": cpu,
-        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "cpu_baseline": None,
         "end_to_end_samples_per_s": round(samples / (ms_step / 1e3 + t_build), 1),
-        "scene_build_s": round(t_build, 3), "scene_gen_s": round(t_gen, 3),
+        "scene_build_s": round(t_build, 3),
         "kernel_ms_per_step": {k: round(per_step[k], 3) for k in
                                ("ms_extend", "ms_shade", "ms_shadow", "ms_splat", "ms_other")},
         "counters_per_step": {k: int(per_step[k]) for k in
@@ -178,9 +231,159 @@ def main():
                                "surf_tests_closest", "node_visits_shadow",
                                "surf_tests_shadow")},
     }
-    print(json.dumps(line), flush=True)
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = CpuBaseline(sd)
+        line["cpu_baseline"] = cpu.throughput(cam, width, height, maxp, minp, args.seed + 10)
+        line["gpu_vs_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+        log("fp64 reference crop")
+        ref, t_ref = cpu.crop(cam, width, height, REF_SPP, maxp, minp, args.seed + 1000,
+                              fp64=True)
+        x0, y0, cw, ch = crop_window(width, height, CROP_W, CROP_H)
+        g = relmse(finalize(gpu_img[y0:y0 + ch, x0:x0 + cw]), ref)
+        cpu_same, _ = cpu.crop(cam, width, height, spp, maxp, minp, args.seed + 2000)
+        # what the CPU renders of the whole frame in the GPU's frame time
+        spp_eq = line["cpu_baseline"]["value"] * (ms_step / 1e3) / (width * height)
+        cpu_eq, _ = cpu.crop(cam, width, height, max(1, int(spp_eq)), maxp, minp,
+                             args.seed + 3000)
+        c_eq = relmse(cpu_eq, ref)
+        line["relmse"] = {
+            "reference": f"FP64 CPU oracle, {REF_SPP} spp, centred {cw}x{ch} crop "
+                         f"({t_ref:.1f} s)",
+            "gpu": round(g, 6), "gpu_spp": spp,
+            "cpu_same_spp": round(relmse(cpu_same, ref), 6),
+            "cpu_equal_time": round(c_eq, 6),
+            "cpu_equal_time_spp": round(spp_eq, 3),
+            "equal_time_gain": round(c_eq / g, 2) if g > 0 else None,
+        }
     sc.close()
     ctx.close()
+    return line
+
+
+def batch_bench(args):
+    from snogray_amd import batch, native, workloads
+    jobs = workloads.draw_jobs(64, args.seed)
+    ctx = native.Context(0)
+    log("building 64 scenes")
+    t0 = time.perf_counter()
+    S, J = {}, []
+    for js in jobs:
+        S[js.scene] = native.Scene(ctx, js.scene.build()[0])
+        J.append((S[js.scene], js.camera(), js.params()))
+    t_build = time.perf_counter() - t0
+    best, table = batch.autotune(ctx, J, log=log)
+    log(f"tuned: {best}")
+    rep = batch.run_batch(ctx, J, **best)
+    # the same jobs one after another through sgr_render, for comparison
+    import torch
+    t0 = time.perf_counter()
+    for sc, cam, p in J:
+        acc = torch.zeros((p.height, p.width, 5), dtype=torch.float32, device="cuda")
+        sc.render_device(cam, p, acc.data_ptr())
+        del acc
+    torch.cuda.synchronize()
+    seq = time.perf_counter() - t0
+    line = {
+        "metric": "batched samples/s (64 uneven jobs); p50/p99 job latency",
+        "value": round(rep["samples_per_s"], 1), "unit": "samples/s", "n_gpus": 1,
+        "steps": rep["steps"], "warmup": 0, "ms_per_step": round(1e3 * rep["wall_s"] /
+                                                                   rep["steps"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: 64 seeded jobs, icosphere soups with own envmaps",
+        "config": {"workload": "64 jobs: resolution log-U[64,1024] per side, spp log-U[1,256], "
+                               "triangles log-U[10k,2M], max path length 10",
+                   "autotuned": best, "parallelism": "single GPU"},
+        "p50_ms": round(rep["p50_ms"], 1), "p99_ms": round(rep["p99_ms"], 1),
+        "wall_s": round(rep["wall_s"], 3), "samples": rep["samples"],
+        "sequential_render_s": round(seq, 3),
+        "batch_vs_sequential": round(seq / rep["wall_s"], 3),
+        "scene_build_s": round(t_build, 1),
+        "autotune_table": [[c["path_capacity"], c["grid_blocks"], round(r * 1e3)]
+                           for c, r in table],
+        "graph_steps": rep["graph_steps"], "max_jobs_per_step": rep["max_jobs_per_step"],
+    }
+    for sc in S.values():
+        sc.close()
+    ctx.close()
+    return line
+
+
+def serve_bench(args):
+    from snogray_amd import native, serve
+    ctx = native.Context(0)
+    lines = []
+    cpu = None
+    for rate in args.rates:
+        checker = None
+        if not args.no_cpu:
+            cpu = CpuBaseline(None)
+            checker = cpu.pixel_check
+        log(f"serving at {rate} jobs/s for {args.duration} s")
+        rep = serve.serve_poisson(ctx, rate, args.duration, seed=args.seed, check_fn=checker,
+                                  log=log)
+        lines.append({
+            "metric": "served samples/s; p50/p99 job latency (Poisson arrivals)",
+            "value": round(rep["samples_per_s"], 1), "unit": "samples/s", "n_gpus": 1,
+            "steps": rep["steps"], "warmup": 0,
+            "ms_per_step": round(rep["device_ms"] / max(1, rep["steps"]), 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic: seeded job distribution over a 48-scene pool",
+            "config": {"workload": f"Poisson {rate} jobs/s for {args.duration} s, <=32 resident "
+                                   "scenes, admission by HBM footprint",
+                       "parallelism": "single GPU"},
+            "p50_ms": round(rep["p50_ms"], 1), "p99_ms": round(rep["p99_ms"], 1),
+            "jobs": rep["jobs"], "wall_s": round(rep["wall_s"], 2),
+            "max_active_jobs": rep["max_active_jobs"],
+            "peak_budget_gb": round(rep["peak_budget_bytes"] / 2 ** 30, 2),
+            "cpu_pixel_checks": rep["checks"],
+        })
+    ctx.close()
+    return lines
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--workload", default="render",
+                    choices=("render", "cornell", "batch", "serve"))
+    ap.add_argument("--rates", type=float, nargs="+", default=[4.0, 16.0, 64.0])
+    ap.add_argument("--duration", type=float, default=120.0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run one plain 256 MiB device copy (reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is a design in DESIGN.md, not a measured path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from snogray_amd import scenes
+    if args.workload == "render":
+        log("building scene")
+        sd, cam = scenes.icosphere_soup()
+        line = frame_bench(args, sd, cam, W, H, SPP, MAXP, MINP,
+                           "icosphere soup, 1920x1080, 64 spp, max path length 10",
+                           "pmc_traffic_icosphere_soup_1080p.txt")
+        line["config"]["envmap"] = "1024x512 fp32"
+        print(json.dumps(line), flush=True)
+    elif args.workload == "cornell":
+        sd, cam = scenes.cornell_box()
+        line = frame_bench(args, sd, cam, 256, 256, 16, 8, 3,
+                           "Cornell box, 256x256, 16 spp, max path length 8",
+                           "pmc_traffic_cornell_256.txt")
+        line["metric"] = "samples/s (Cornell box 256x256)"
+        print(json.dumps(line), flush=True)
+    elif args.workload == "batch":
+        print(json.dumps(batch_bench(args)), flush=True)
+    elif args.workload == "serve":
+        for line in serve_bench(args):
+            print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

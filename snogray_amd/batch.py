@@ -1,0 +1,95 @@
+"""Batched uneven jobs on the multi-job engine, with an autotuner for the
+engine's path capacity (paths in flight per step, i.e. the ray / shade /
+shadow queue length) and its launch width (workgroups per launch).
+
+The tuner times a few engine steps of a probe batch (the real jobs with
+their sample counts cut down) per candidate and keeps the configuration
+with the most paths per device-millisecond.  A job's chunk is
+min(its own chunk, capacity / pattern pixels) samples, so tuner choices
+change a job's result only in the fp32 summation order of its
+accumulators."""
+import itertools
+import time
+
+import numpy as np
+
+from . import abi, native
+
+DEFAULT_CAPACITIES = (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+DEFAULT_GRIDS = (0, 4096 * 2, 4096 * 4)  # 0: 16 workgroups per CU
+
+
+def _probe(ctx, jobs, capacity, grid, steps, spp_cap):
+    eng = native.Engine(ctx, path_capacity=capacity, grid_blocks=grid, use_graph=True)
+    ids = []
+    try:
+        for sc, cam, p in jobs:
+            q = abi.RenderParams.from_buffer_copy(p)
+            q.spp = min(q.spp, spp_cap)
+            ids.append(eng.submit(sc, cam, q))
+        eng.step()  # captures the step graph
+        paths, ms = 0, 0.0
+        for _ in range(steps):
+            si = eng.step()
+            if si["jobs"] == 0:
+                break
+            paths += si["paths"]
+            ms += si["ms_device"]
+    finally:
+        eng.release_all()
+    return paths / ms if ms > 0 else 0.0
+
+
+def autotune(ctx, jobs, capacities=DEFAULT_CAPACITIES, grids=DEFAULT_GRIDS, steps=3,
+             spp_cap=16, probe_jobs=24, log=None):
+    """JOBS: [(native.Scene, camera, params)].  Returns (best config
+    dict, [(config, paths per ms)])."""
+    def npix(p):
+        _, _, w, h, m = abi.render_window(p)
+        return (w + 2 * m) * (h + 2 * m)
+    max_npix = max(npix(p) for _, _, p in jobs)
+    probe = jobs[:probe_jobs]
+    table = []
+    for cap, grid in itertools.product(capacities, grids):
+        if cap < max_npix:
+            continue
+        rate = _probe(ctx, probe, cap, grid, steps, spp_cap)
+        table.append(({"path_capacity": cap, "grid_blocks": grid}, rate))
+        if log:
+            log(f"autotune capacity {cap} grid {grid or 'auto'}: {rate / 1e3:.1f} M paths/s")
+    best = max(table, key=lambda t: t[1])[0]
+    return best, table
+
+
+def run_batch(ctx, jobs, path_capacity=1 << 22, grid_blocks=0, keep_results=False):
+    """Submit every job, step until all are done.  Returns a report with
+    aggregate samples/s and per-job latency (submit to done)."""
+    eng = native.Engine(ctx, path_capacity=path_capacity, grid_blocks=grid_blocks,
+                        use_graph=True)
+    results = {}
+    t0 = time.perf_counter()
+    ids = [eng.submit(sc, cam, p) for sc, cam, p in jobs]
+    steps = eng.run()
+    wall = time.perf_counter() - t0
+    lat, samples = [], 0
+    for i, (jid, (_, _, p)) in enumerate(zip(ids, jobs)):
+        st = eng.poll(jid)
+        lat.append(st["t_done_ms"] - st["t_submit_ms"])
+        x0, y0, w, h, _ = abi.render_window(p)
+        samples += w * h * p.spp
+        if keep_results:
+            results[i] = eng.result(jid, p.width, p.height)
+    stats = eng.stats()
+    eng.release_all()
+    lat = np.array(lat)
+    rep = {"jobs": len(jobs), "wall_s": wall, "samples": int(samples),
+           "samples_per_s": samples / wall,
+           "device_ms": float(sum(s["ms_device"] for s in steps)),
+           "steps": len(steps), "graph_steps": int(sum(s["graph"] for s in steps)),
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "max_jobs_per_step": int(max(s["jobs"] for s in steps)),
+           "scene_intersect_calls": stats["scene_intersect_calls"],
+           "shadow_tests": stats["shadow_tests"]}
+    if keep_results:
+        rep["results"] = results
+    return rep

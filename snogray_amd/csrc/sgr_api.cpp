@@ -46,12 +46,12 @@ int dev_alloc(sgr_ctx *ctx, sgr_scene *sc, size_t bytes, void **out) {
 }
 
 template<class T>
-int upload(sgr_ctx *ctx, sgr_scene *sc, const std::vector<T> &v, const T **out) {
+int upload(sgr_ctx *ctx, sgr_scene *sc, hipStream_t st, const std::vector<T> &v, const T **out) {
   void *p = nullptr;
   int rc = dev_alloc(ctx, sc, v.size() * sizeof(T), &p);
   if (rc) return rc;
   if (!v.empty())
-    HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
   *out = (const T *)p;
   return SGR_OK;
 }
@@ -158,7 +158,7 @@ int sgr_ctx_create(int device, sgr_ctx **out) {
 int sgr_ctx_destroy(sgr_ctx *ctx) {
   if (!ctx) return SGR_OK;
   if (ctx->live_scenes)
-    return sgr::fail(SGR_ERR_INVALID, "sgr_ctx_destroy: " + std::to_string(ctx->live_scenes) +
+    return sgr::fail(SGR_ERR_INVALID, "sgr_ctx_destroy: " + std::to_string(ctx->live_scenes.load()) +
                                      " scene(s) still alive; destroy them first");
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
@@ -501,21 +501,29 @@ int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
       refs[3 * k + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
   }
-  if (!rc) rc = upload(ctx, sc, nodes4, &V.nodes);
-  if (!rc) rc = upload(ctx, sc, refs, &V.refs);
-  if (!rc) rc = upload(ctx, sc, H.tri, &V.tri);
-  if (!rc) rc = upload(ctx, sc, H.sph, &V.sph);
-  if (!rc) rc = upload(ctx, sc, H.prim_mat, &V.prim_mat);
-  if (!rc) rc = upload(ctx, sc, H.prim_mesh, &V.prim_mesh);
-  if (!rc) rc = upload(ctx, sc, H.meshes, &V.meshes);
-  if (!rc) rc = upload(ctx, sc, H.mats, &V.mats);
-  if (!rc) rc = upload(ctx, sc, H.lights, &V.lights);
-  if (!rc) rc = upload(ctx, sc, H.env_tex, &V.env_tex);
-  if (!rc) rc = upload(ctx, sc, H.row_cum, &V.env_row_cum);
-  if (!rc) rc = upload(ctx, sc, H.col_cum, &V.env_col_cum);
-  if (!rc) {
-    hipError_t e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) rc = fail(SGR_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+  // Uploads go through a stream of their own: a scene may be created by
+  // one host thread while another drives renders on the context's stream.
+  hipStream_t up = nullptr;
+  {
+    hipError_t e = hipStreamCreateWithFlags(&up, hipStreamNonBlocking);
+    if (e != hipSuccess) rc = fail(SGR_ERR_HIP, std::string("upload stream: ") + hipGetErrorString(e));
+  }
+  if (!rc) rc = upload(ctx, sc, up, nodes4, &V.nodes);
+  if (!rc) rc = upload(ctx, sc, up, refs, &V.refs);
+  if (!rc) rc = upload(ctx, sc, up, H.tri, &V.tri);
+  if (!rc) rc = upload(ctx, sc, up, H.sph, &V.sph);
+  if (!rc) rc = upload(ctx, sc, up, H.prim_mat, &V.prim_mat);
+  if (!rc) rc = upload(ctx, sc, up, H.prim_mesh, &V.prim_mesh);
+  if (!rc) rc = upload(ctx, sc, up, H.meshes, &V.meshes);
+  if (!rc) rc = upload(ctx, sc, up, H.mats, &V.mats);
+  if (!rc) rc = upload(ctx, sc, up, H.lights, &V.lights);
+  if (!rc) rc = upload(ctx, sc, up, H.env_tex, &V.env_tex);
+  if (!rc) rc = upload(ctx, sc, up, H.row_cum, &V.env_row_cum);
+  if (!rc) rc = upload(ctx, sc, up, H.col_cum, &V.env_col_cum);
+  if (up) {
+    hipError_t e = hipStreamSynchronize(up);
+    if (!rc && e != hipSuccess) rc = fail(SGR_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+    (void)hipStreamDestroy(up);
   }
   if (rc) {
     sgr_scene_destroy(sc);
@@ -532,7 +540,7 @@ int sgr_scene_destroy(sgr_scene *sc) {
                                      " job(s) still refer to the scene; release them first");
   if (sc->ctx) {
     (void)hipSetDevice(sc->ctx->device);
-    (void)hipStreamSynchronize(sc->ctx->stream);
+    (void)hipDeviceSynchronize();  // no render of any thread still reads it
     sc->ctx->bytes -= sc->bytes;
     sc->ctx->live_scenes--;
   }

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -23,14 +24,14 @@ struct Engine;
 struct sgr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  uint64_t bytes = 0;          // device bytes held (scenes + work buffers + jobs)
+  std::atomic<uint64_t> bytes{0};  // device bytes held (scenes + buffers)
   void *work = nullptr;        // path buffers, grown on demand
   size_t work_bytes = 0;
   unsigned long long *stats = nullptr;
   int num_cu = 256;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<hipEvent_t> evpool;  // per-launch timing events
-  uint32_t live_scenes = 0;
+  std::atomic<uint32_t> live_scenes{0};
   sgr::Engine *engine = nullptr;   // multi-job engine (sgr_jobs.cpp)
 };
 

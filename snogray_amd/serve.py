@@ -1,0 +1,322 @@
+"""Serving loop: continuous batching of render requests on one GPU.
+
+Requests arrive over time (a Poisson process in the benchmark).  A
+request is admitted as soon as its scene is resident and its accumulators
+fit the HBM budget; admitted requests join the job engine, whose every
+step carries the next sample chunk of every active job, so requests that
+arrive while others run overlap with them on the device.  Up to
+``max_scenes`` scenes stay resident (least recently used idle scenes are
+evicted when a new one needs the room); scenes are built on a host thread
+while the GPU keeps stepping.  Every ``check_every``-th finished job is
+handed, with its accumulators, to an optional check function (the
+benchmark compares a window of it with the CPU renderer by the
+3-standard-error pixel test) on a host thread.
+
+``ResidencyPlanner`` is the admission bookkeeping alone (no device
+calls), so its invariant -- resident scene bytes + admitted accumulator
+bytes + reserved bytes never exceed the budget -- is testable anywhere.
+"""
+import collections
+import concurrent.futures as cf
+import time
+
+import numpy as np
+
+from . import abi, native
+
+
+class ResidencyPlanner:
+    """HBM-footprint admission: scenes (resident, LRU, at most
+    MAX_SCENES) and job accumulators against BUDGET bytes, of which
+    RESERVED (engine path buffers) are held from the start."""
+
+    def __init__(self, budget, max_scenes=32, reserved=0):
+        if reserved > budget:
+            raise ValueError("reserved bytes exceed the budget")
+        self.budget = int(budget)
+        self.max_scenes = int(max_scenes)
+        self.reserved = int(reserved)
+        self.scenes = collections.OrderedDict()  # key -> [bytes, active jobs]
+        self.job_bytes = 0
+        self.peak = self.used()
+
+    def used(self):
+        return self.reserved + self.job_bytes + sum(v[0] for v in self.scenes.values())
+
+    def resident(self, key):
+        return key in self.scenes
+
+    def _evictable(self, protect=()):
+        return [k for k, v in self.scenes.items() if v[1] == 0 and k not in protect]
+
+    def plan_scene(self, key, nbytes, protect=()):
+        """Scenes to evict so that scene KEY of NBYTES fits (None if it
+        cannot fit now).  Idle scenes are evicted oldest first."""
+        if key in self.scenes:
+            return []
+        free = self.budget - self.used()
+        slots = self.max_scenes - len(self.scenes)
+        evict = []
+        for k in self._evictable(protect):
+            if free >= nbytes and slots >= 1:
+                break
+            evict.append(k)
+            free += self.scenes[k][0]
+            slots += 1
+        if free >= nbytes and slots >= 1:
+            return evict
+        return None
+
+    def make_room(self, nbytes, protect=()):
+        """Idle scenes to evict so that NBYTES more fit (None if they
+        cannot)."""
+        free = self.budget - self.used()
+        evict = []
+        for k in self._evictable(protect):
+            if free >= nbytes:
+                break
+            evict.append(k)
+            free += self.scenes[k][0]
+        return evict if free >= nbytes else None
+
+    def add_scene(self, key, nbytes):
+        assert key not in self.scenes
+        assert len(self.scenes) < self.max_scenes
+        assert self.used() + nbytes <= self.budget
+        self.scenes[key] = [int(nbytes), 0]
+        self._note()
+
+    def resize_scene(self, key, nbytes):
+        """Replace a scene's estimated size by its measured one; returns
+        False (and changes nothing) if that would break the budget."""
+        old = self.scenes[key][0]
+        if self.used() - old + nbytes > self.budget:
+            return False
+        self.scenes[key][0] = int(nbytes)
+        self._note()
+        return True
+
+    def evict(self, key):
+        assert self.scenes[key][1] == 0, "scene still has active jobs"
+        del self.scenes[key]
+
+    def can_admit(self, key, nbytes):
+        return key in self.scenes and self.used() + nbytes <= self.budget
+
+    def admit(self, key, nbytes):
+        assert self.can_admit(key, nbytes)
+        self.scenes[key][1] += 1
+        self.scenes.move_to_end(key)
+        self.job_bytes += int(nbytes)
+        self._note()
+
+    def release(self, key, nbytes):
+        self.scenes[key][1] -= 1
+        assert self.scenes[key][1] >= 0
+        self.job_bytes -= int(nbytes)
+
+    def _note(self):
+        u = self.used()
+        assert u <= self.budget, "budget exceeded"
+        self.peak = max(self.peak, u)
+
+
+def pixel_test(gpu_img, cpu_imgs, min_pass=0.95):
+    """3-standard-error test per pixel and channel: GPU_IMG (one render
+    at the job's spp) against the mean of CPU_IMGS (independent seeds at
+    the same spp).  se_gpu is the spread of one CPU render, se_cpu that
+    spread over sqrt(k).  Returns (fraction passing, passed)."""
+    c = np.stack(cpu_imgs)
+    k = len(c)
+    mean = c.mean(0)
+    sd = c.std(0, ddof=1)
+    tol = 3 * np.sqrt(sd ** 2 + sd ** 2 / k) + 1e-6 * np.abs(mean) + 1e-7
+    frac = float(np.mean(np.abs(gpu_img - mean) <= tol))
+    return frac, frac >= min_pass
+
+
+def finalize_numpy(acc):
+    """ImageSampledOutput::set_raw_min_y normalisation of accumulators."""
+    acc = np.asarray(acc, np.float64)
+    w = acc[..., 4:5]
+    col = np.where(w > 0, acc[..., :3] / np.where(w > 0, w, 1), acc[..., :3])
+    a = np.clip(np.where(w[..., 0] > 0, acc[..., 3] / np.where(w[..., 0] > 0, w[..., 0], 1),
+                         acc[..., 3]), 0, 1)
+    col = np.maximum(col, 0)
+    col = np.where((a > 0)[..., None] & (a < 1)[..., None], col / np.where(a > 0, a, 1)[..., None],
+                   col)
+    return col
+
+
+class Server:
+    """Drives one context's job engine for a stream of (arrival time,
+    JobSpec) requests; see the module docstring."""
+
+    def __init__(self, ctx, budget, max_scenes=32, path_capacity=1 << 22, check_every=50,
+                 check_fn=None, use_graph=True):
+        self.ctx = ctx
+        self.engine = native.Engine(ctx, path_capacity=path_capacity, use_graph=use_graph)
+        # the engine's path buffers are held for the whole run
+        per_path = 340 + 45 * 2
+        self.planner = ResidencyPlanner(budget, max_scenes, reserved=per_path * path_capacity)
+        self.scenes = {}        # SceneSpec -> native.Scene
+        self.building = {}      # SceneSpec -> Future
+        self.builder = cf.ThreadPoolExecutor(1)
+        self.checker = cf.ThreadPoolExecutor(1)
+        self.check_every = check_every if check_fn else 0
+        self.check_fn = check_fn  # (job, scene desc, accumulators) -> dict
+        self.descs = {}
+
+    def _build(self, spec):
+        sd, _ = spec.build()
+        sc = native.Scene(self.ctx, sd)
+        return sd, sc
+
+    def run(self, requests, drain=True, log=None, keep_results=False):
+        """REQUESTS: list of (arrival seconds, JobSpec), sorted by time.
+        Returns a report dict (with every job's accumulators under
+        "results" if KEEP_RESULTS)."""
+        results = {}
+        reqs = collections.deque((a, i, job) for i, (a, job) in enumerate(requests))
+        pending = collections.deque()   # (arrival, index, spec)
+        active = {}                     # job id -> (arrival, index, spec, bytes)
+        lat, done_samples, checks, check_futs = [], 0, [], []
+        n_done = 0
+        t0 = time.perf_counter()
+        steps = 0
+        step_ms = 0.0
+        max_active = 0
+        t_report = 0.0
+        while reqs or pending or active or self.building:
+            now = time.perf_counter() - t0
+            if log and now - t_report >= 10.0:
+                t_report = now
+                log(f"serving t={now:.0f}s: {len(lat)} done, {len(active)} active, "
+                    f"{len(pending)} waiting, {len(self.scenes)} scenes resident")
+            while reqs and reqs[0][0] <= now:
+                pending.append(reqs.popleft())
+            # finished scene builds
+            for spec, fut in list(self.building.items()):
+                if fut.done():
+                    sd, sc = fut.result()
+                    del self.building[spec]
+                    self.descs[spec] = sd
+                    self.scenes[spec] = sc
+                    fp = sc.footprint()
+                    if not self.planner.resize_scene(spec, fp):
+                        # measured above the estimate: evict idle scenes
+                        ev = self.planner.make_room(fp - self.planner.scenes[spec][0],
+                                                    protect=(spec,))
+                        if ev is None:
+                            raise RuntimeError("scene does not fit the HBM budget")
+                        for k in ev:
+                            self._evict(k)
+                        assert self.planner.resize_scene(spec, fp)
+            # admission, in arrival order; a job waiting for its scene does
+            # not block later jobs whose scenes are resident
+            keep = collections.deque()
+            blocked_mem = False
+            while pending:
+                a, idx, job = pending.popleft()
+                spec = job.scene
+                nbytes = native.Engine.footprint(job.params())
+                if spec in self.scenes and not blocked_mem:
+                    if self.planner.can_admit(spec, nbytes):
+                        self.planner.admit(spec, nbytes)
+                        jid = self.engine.submit(self.scenes[spec], job.camera(), job.params())
+                        active[jid] = (a, idx, job, nbytes)
+                        continue
+                    blocked_mem = True
+                elif spec not in self.building and spec not in self.scenes:
+                    if spec.estimated_bytes() + nbytes > self.planner.budget - self.planner.reserved:
+                        raise RuntimeError(f"job {idx}: scene and accumulators exceed the budget")
+                    ev = self.planner.plan_scene(spec, spec.estimated_bytes())
+                    if ev is not None:
+                        for k in ev:
+                            self._evict(k)
+                        self.planner.add_scene(spec, spec.estimated_bytes())
+                        self.building[spec] = self.builder.submit(self._build, spec)
+                keep.append((a, idx, job))
+            pending = keep
+            max_active = max(max_active, len(active))
+            if active:
+                si = self.engine.step()
+                steps += 1
+                step_ms += si["ms_device"]
+                if si["jobs_finished"]:
+                    tnow = time.perf_counter() - t0
+                    for jid in list(active):
+                        st = self.engine.poll(jid)
+                        if st["state"] != abi.SGR_JOB_DONE:
+                            continue
+                        a, idx, job, nbytes = active.pop(jid)
+                        lat.append(tnow - a)
+                        done_samples += job.samples
+                        n_done += 1
+                        acc = None
+                        if keep_results:
+                            acc = results[idx] = self.engine.result(jid, job.width, job.height)
+                        if self.check_every and n_done % self.check_every == 0:
+                            if acc is None:
+                                acc = self.engine.result(jid, job.width, job.height)
+                            check_futs.append((idx, self.checker.submit(
+                                self.check_fn, job, self.descs[job.scene], acc)))
+                        self.engine.release(jid)
+                        self.planner.release(job.scene, nbytes)
+            else:
+                wait = 0.001
+                if reqs and not pending and not self.building:
+                    wait = max(0.0, min(0.05, reqs[0][0] - (time.perf_counter() - t0)))
+                time.sleep(wait)
+            if not drain and not reqs:
+                break
+        wall = time.perf_counter() - t0
+        for idx, f in check_futs:
+            checks.append(dict(f.result(), job=idx))
+        lat_ms = np.array(lat) * 1e3
+        return {
+            "jobs": len(lat), "wall_s": wall,
+            "samples": int(done_samples),
+            "samples_per_s": done_samples / wall if wall > 0 else 0.0,
+            "p50_ms": float(np.percentile(lat_ms, 50)) if len(lat) else None,
+            "p99_ms": float(np.percentile(lat_ms, 99)) if len(lat) else None,
+            "mean_ms": float(lat_ms.mean()) if len(lat) else None,
+            "steps": steps, "device_ms": step_ms,
+            "max_active_jobs": max_active,
+            "peak_budget_bytes": self.planner.peak, "budget_bytes": self.planner.budget,
+            "resident_scenes_max": self.planner.max_scenes,
+            "checks": checks,
+            **({"results": results} if keep_results else {}),
+        }
+
+    def _evict(self, spec):
+        self.planner.evict(spec)
+        sc = self.scenes.pop(spec, None)
+        if sc is not None:
+            sc.close()
+        self.descs.pop(spec, None)
+
+    def close(self):
+        self.builder.shutdown(wait=True)
+        self.checker.shutdown(wait=True)
+        self.engine.release_all()
+        for spec in list(self.scenes):
+            self.scenes.pop(spec).close()
+
+
+def serve_poisson(ctx, rate, duration, n_scene_pool=48, seed=1, budget=None, log=None, **kw):
+    """Poisson arrivals at RATE jobs/s for DURATION s, jobs drawn from the
+    batched-job distribution over a pool of scenes."""
+    from . import workloads
+    pool = workloads.scene_pool(n_scene_pool, seed)
+    times = workloads.poisson_arrivals(rate, duration, seed + 1)
+    jobs = workloads.draw_jobs(len(times), seed + 2, scene_pool=pool)
+    if budget is None:
+        budget = 64 << 30
+    srv = Server(ctx, budget, **kw)
+    try:
+        rep = srv.run(list(zip(times, jobs)), log=log)
+    finally:
+        srv.close()
+    rep.update({"rate_jobs_per_s": rate, "duration_s": duration, "scene_pool": n_scene_pool})
+    return rep

@@ -1,0 +1,79 @@
+"""Host-side admission bookkeeping of the serving loop: under random
+arrivals, completions and scene sizes the resident-scene bytes plus the
+admitted accumulator bytes plus the reserved bytes never exceed the
+budget, at most max_scenes scenes are resident, and only scenes with no
+active job are evicted."""
+import numpy as np
+import pytest
+
+from snogray_amd.serve import ResidencyPlanner, pixel_test
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_budget_never_exceeded(seed):
+    rng = np.random.default_rng(seed)
+    budget = int(rng.integers(2_000, 10_000))
+    reserved = int(rng.integers(0, 1_000))
+    P = ResidencyPlanner(budget, max_scenes=int(rng.integers(1, 6)), reserved=reserved)
+    scene_bytes = {k: int(rng.integers(100, 3_000)) for k in range(12)}
+    active = []            # (scene, bytes)
+    admitted = rejected = evictions = 0
+    for _ in range(3000):
+        if active and rng.random() < 0.45:
+            k, b = active.pop(int(rng.integers(len(active))))
+            P.release(k, b)
+            continue
+        k = int(rng.integers(12))
+        jb = int(rng.integers(10, 600))
+        if not P.resident(k):
+            ev = P.plan_scene(k, scene_bytes[k])
+            if ev is None:
+                rejected += 1
+                continue
+            for e in ev:
+                assert all(s != e for s, _ in active)  # never evict a busy scene
+                P.evict(e)
+                evictions += 1
+            P.add_scene(k, scene_bytes[k])
+            # the measured size may differ from the estimate
+            if rng.random() < 0.3:
+                P.resize_scene(k, int(scene_bytes[k] * rng.uniform(0.8, 1.2)))
+        if P.can_admit(k, jb):
+            P.admit(k, jb)
+            active.append((k, jb))
+            admitted += 1
+        else:
+            rejected += 1
+        used = reserved + sum(b for _, b in active) + sum(v[0] for v in P.scenes.values())
+        assert used == P.used() <= budget
+        assert len(P.scenes) <= P.max_scenes
+    assert P.peak <= budget
+    assert admitted > 100 and evictions > 0
+
+
+def test_planner_edge_cases():
+    P = ResidencyPlanner(1000, max_scenes=2, reserved=100)
+    assert P.plan_scene("a", 901) is None          # larger than what is left
+    assert P.plan_scene("a", 900) == []
+    P.add_scene("a", 900)
+    assert not P.can_admit("a", 1)                 # full
+    assert not P.can_admit("b", 0)                 # not resident
+    assert P.plan_scene("b", 500) == ["a"]         # idle scene makes room
+    P.admit("a", 0)
+    assert P.plan_scene("b", 500) is None          # a is busy now
+    P.release("a", 0)
+    with pytest.raises(ValueError):
+        ResidencyPlanner(10, reserved=11)
+    assert P.make_room(50) == ["a"] and P.make_room(2000) is None
+
+
+def test_pixel_test_accepts_same_distribution_and_rejects_shift():
+    rng = np.random.default_rng(0)
+    truth = rng.uniform(0.1, 2.0, (16, 16, 3))
+    sigma = 0.1 * truth
+    cpu = [truth + sigma * rng.standard_normal(truth.shape) for _ in range(12)]
+    gpu = truth + sigma * rng.standard_normal(truth.shape)
+    frac, ok = pixel_test(gpu, cpu)
+    assert ok and frac > 0.97
+    frac, ok = pixel_test(gpu * 1.5, cpu)
+    assert not ok
