@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # and the ray record (32 B in; 16 B hit out / 1 B flag out + 4 B queue id).
 NODE_B, TEST_B, LIST_B = 16, 36, 4
 RAY_IN_B, HIT_OUT_B, OCC_OUT_B, QUEUE_B = 32, 16, 1, 4
-CROP_W, CROP_H, REF_SPP = 64, 36, 4096
+CROP_W, CROP_H, CROP_GRID, REF_SPP = 16, 9, 4, 4096
 
 
 def log(*a):
@@ -88,8 +88,20 @@ def finalize(acc):
     return finalize_numpy(acc)
 
 
-def crop_window(w, h, cw, ch):
-    return ((w - cw) // 2, (h - ch) // 2, cw, ch)
+def crop_windows(w, h):
+    """A 4 x 4 grid of 16 x 9 crops centred in the cells of the frame:
+    the relMSE sample (2304 pixels spread over the whole image)."""
+    out = []
+    for j in range(CROP_GRID):
+        for i in range(CROP_GRID):
+            cx = int((i + 0.5) * w / CROP_GRID)
+            cy = int((j + 0.5) * h / CROP_GRID)
+            out.append((cx - CROP_W // 2, cy - CROP_H // 2, CROP_W, CROP_H))
+    return out
+
+
+def take_crops(img, wins):
+    return np.stack([img[y0:y0 + ch, x0:x0 + cw] for x0, y0, cw, ch in wins])
 
 
 # --------------------------------------------------------------------------
@@ -129,16 +141,20 @@ class CpuBaseline:
                 "sample": f"full {width}x{height} frame at {spp} spp, max_path_len {maxp}, "
                           f"same scene and camera, {dt:.1f} s on {self.cores} threads"}
 
-    def crop(self, cam, width, height, spp, maxp, minp, seed, fp64=False):
+    def crops(self, cam, width, height, spp, maxp, minp, seed, fp64=False):
+        """Finalized crops (crop_windows) of a render at SPP, each crop
+        rendered as its own window (filter margin included)."""
         from snogray_amd import abi
-        x0, y0, cw, ch = crop_window(width, height, CROP_W, CROP_H)
-        p = abi.make_params(width, height, spp, max_path_len=maxp, min_path_len=minp, seed=seed,
-                            window=(x0, y0, cw, ch))
         t = time.perf_counter()
-        acc, _, _ = self.scene(fp64).render(cam, p, nthreads=self.cores)
-        return finalize(acc[y0:y0 + ch, x0:x0 + cw]), time.perf_counter() - t
+        out = []
+        for k, (x0, y0, cw, ch) in enumerate(crop_windows(width, height)):
+            p = abi.make_params(width, height, spp, max_path_len=maxp, min_path_len=minp,
+                                seed=seed, window=(x0, y0, cw, ch))
+            acc, _, _ = self.scene(fp64).render(cam, p, nthreads=self.cores)
+            out.append(finalize(acc[y0:y0 + ch, x0:x0 + cw]))
+        return np.stack(out), time.perf_counter() - t
 
-    def pixel_check(self, job, desc, acc, window=16, seeds=12):
+    def pixel_check(self, job, desc, acc, window=16, seeds=24):
         """3-standard-error pixel test of a served job's window against
         independent-seed CPU renders."""
         from snogray_amd import serve
@@ -236,23 +252,24 @@ def frame_bench(args, sd, cam, width, height, spp, maxp, minp, workload, traffic
         cpu = CpuBaseline(sd)
         line["cpu_baseline"] = cpu.throughput(cam, width, height, maxp, minp, args.seed + 10)
         line["gpu_vs_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
-        log("fp64 reference crop")
-        ref, t_ref = cpu.crop(cam, width, height, REF_SPP, maxp, minp, args.seed + 1000,
-                              fp64=True)
-        x0, y0, cw, ch = crop_window(width, height, CROP_W, CROP_H)
-        g = relmse(finalize(gpu_img[y0:y0 + ch, x0:x0 + cw]), ref)
-        cpu_same, _ = cpu.crop(cam, width, height, spp, maxp, minp, args.seed + 2000)
+        log("fp64 reference crops")
+        ref, t_ref = cpu.crops(cam, width, height, REF_SPP, maxp, minp, args.seed + 1000,
+                               fp64=True)
+        wins = crop_windows(width, height)
+        g = relmse(finalize(take_crops(gpu_img, wins)), ref)
+        cpu_same, _ = cpu.crops(cam, width, height, spp, maxp, minp, args.seed + 2000)
         # what the CPU renders of the whole frame in the GPU's frame time
         spp_eq = line["cpu_baseline"]["value"] * (ms_step / 1e3) / (width * height)
-        cpu_eq, _ = cpu.crop(cam, width, height, max(1, int(spp_eq)), maxp, minp,
-                             args.seed + 3000)
+        cpu_eq, _ = cpu.crops(cam, width, height, max(1, int(spp_eq)), maxp, minp,
+                              args.seed + 3000)
         c_eq = relmse(cpu_eq, ref)
         line["relmse"] = {
-            "reference": f"FP64 CPU oracle, {REF_SPP} spp, centred {cw}x{ch} crop "
-                         f"({t_ref:.1f} s)",
-            "gpu": round(g, 6), "gpu_spp": spp,
-            "cpu_same_spp": round(relmse(cpu_same, ref), 6),
-            "cpu_equal_time": round(c_eq, 6),
+            "reference": f"FP64 CPU oracle at {REF_SPP} spp on {len(wins)} crops of "
+                         f"{CROP_W}x{CROP_H} spread over the frame ({t_ref:.1f} s)",
+            "definition": "mean over pixels and channels of (x-r)^2/(r^2+1e-2)",
+            "gpu": float(f"{g:.4g}"), "gpu_spp": spp,
+            "cpu_same_spp": float(f"{relmse(cpu_same, ref):.4g}"),
+            "cpu_equal_time": float(f"{c_eq:.4g}"),
             "cpu_equal_time_spp": round(spp_eq, 3),
             "equal_time_gain": round(c_eq / g, 2) if g > 0 else None,
         }

@@ -19,28 +19,32 @@ DEFAULT_CAPACITIES = (1 << 20, 1 << 21, 1 << 22, 1 << 23)
 DEFAULT_GRIDS = (0, 4096 * 2, 4096 * 4)  # 0: 16 workgroups per CU
 
 
-def _probe(ctx, jobs, capacity, grid, steps, spp_cap):
+def _probe(ctx, jobs, capacity, grid, max_steps, spp_cap):
+    """Paths per device-ms of the probe batch: a first pass captures the
+    step graphs, the second pass is timed."""
     eng = native.Engine(ctx, path_capacity=capacity, grid_blocks=grid, use_graph=True)
-    ids = []
+    rate = 0.0
     try:
-        for sc, cam, p in jobs:
-            q = abi.RenderParams.from_buffer_copy(p)
-            q.spp = min(q.spp, spp_cap)
-            ids.append(eng.submit(sc, cam, q))
-        eng.step()  # captures the step graph
-        paths, ms = 0, 0.0
-        for _ in range(steps):
-            si = eng.step()
-            if si["jobs"] == 0:
-                break
-            paths += si["paths"]
-            ms += si["ms_device"]
+        for _ in range(2):
+            for sc, cam, p in jobs:
+                q = abi.RenderParams.from_buffer_copy(p)
+                q.spp = min(q.spp, spp_cap)
+                eng.submit(sc, cam, q)
+            paths, ms = 0, 0.0
+            for _ in range(max_steps):
+                si = eng.step()
+                if si["jobs"] == 0:
+                    break
+                paths += si["paths"]
+                ms += si["ms_device"]
+            eng.release_all()
+            rate = paths / ms if ms > 0 else 0.0
     finally:
         eng.release_all()
-    return paths / ms if ms > 0 else 0.0
+    return rate
 
 
-def autotune(ctx, jobs, capacities=DEFAULT_CAPACITIES, grids=DEFAULT_GRIDS, steps=3,
+def autotune(ctx, jobs, capacities=DEFAULT_CAPACITIES, grids=DEFAULT_GRIDS, steps=6,
              spp_cap=16, probe_jobs=24, log=None):
     """JOBS: [(native.Scene, camera, params)].  Returns (best config
     dict, [(config, paths per ms)])."""

@@ -348,7 +348,7 @@ def _value_noise(p, rng_seed, octaves=3):
         h = (ix.astype(np.uint64) * np.uint64(0x8DA6B343) ^
              iy.astype(np.uint64) * np.uint64(0xD8163841) ^
              iz.astype(np.uint64) * np.uint64(0xCB1AB31F) ^
-             np.uint64(rng_seed * 0x9E3779B1 + octave * 0x85EBCA77))
+             np.uint64((rng_seed * 0x9E3779B1 + octave * 0x85EBCA77) & 0xFFFFFFFFFFFFFFFF))
         h ^= h >> np.uint64(13)
         h *= np.uint64(0x5BD1E995)
         h &= np.uint64(0xFFFFFFFF)

@@ -121,11 +121,17 @@ class ResidencyPlanner:
         self.peak = max(self.peak, u)
 
 
-def pixel_test(gpu_img, cpu_imgs, min_pass=0.95):
+def pixel_test(gpu_img, cpu_imgs, min_pass=0.93):
     """3-standard-error test per pixel and channel: GPU_IMG (one render
     at the job's spp) against the mean of CPU_IMGS (independent seeds at
     the same spp).  se_gpu is the spread of one CPU render, se_cpu that
-    spread over sqrt(k).  Returns (fraction passing, passed)."""
+    spread over sqrt(k).  Returns (fraction passing, passed).
+
+    With k = 24 renders a Gaussian pixel fails with probability about
+    0.6 % (Student t, 23 degrees of freedom); at a few samples per pixel
+    the estimates are heavy-tailed (a rare bright path in the GPU render
+    that none of the k CPU renders drew), which adds at most about
+    max_p p (1-p)^k = 1.5 %.  The threshold leaves room for both."""
     c = np.stack(cpu_imgs)
     k = len(c)
     mean = c.mean(0)

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 CAP = 1 << 16
 
 
-def oracle_pixel_check(job, desc, acc, window=8, seeds=12):
+def oracle_pixel_check(job, desc, acc, window=8, seeds=24):
     osc = oracle.OracleScene(desc)
     ww, hh = min(window, job.width), min(window, job.height)
     x0, y0 = (job.width - ww) // 2, (job.height - hh) // 2
