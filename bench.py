@@ -183,7 +183,7 @@ def frame_bench(args, sd, cam, width, height, spp, maxp, minp, workload, traffic
     acc = torch.zeros((height, width, 5), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
     p = abi.make_params(width, height, spp, max_path_len=maxp, min_path_len=minp, seed=args.seed,
-                        flags=abi.SGR_RENDER_KERNEL_TIMING)
+                        flags=abi.SGR_RENDER_KERNEL_TIMING, paths_in_flight=args.paths_in_flight)
     for i in range(args.warmup):
         log(f"warmup {i}")
         sc.render_device(cam, p, acc.data_ptr())
@@ -235,6 +235,7 @@ def frame_bench(args, sd, cam, width, height, spp, maxp, minp, workload, traffic
         "data": "synthetic: scene built in code from a fixed seed",
         "config": {"workload": workload, "triangles": int(sd.num_triangles), "width": width,
                    "height": height, "spp": spp, "max_path_len": maxp, "min_path_len": minp,
+                   "paths_in_flight": args.paths_in_flight or abi.SGR_DEFAULT_PATHS_IN_FLIGHT,
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": None,
@@ -279,7 +280,9 @@ def frame_bench(args, sd, cam, width, height, spp, maxp, minp, workload, traffic
 
 
 def batch_bench(args):
+    import torch
     from snogray_amd import batch, native, workloads
+    torch.zeros(1, device="cuda")  # torch's device set-up before the library's
     jobs = workloads.draw_jobs(64, args.seed)
     ctx = native.Context(0)
     log("building 64 scenes")
@@ -293,7 +296,6 @@ def batch_bench(args):
     log(f"tuned: {best}")
     rep = batch.run_batch(ctx, J, **best)
     # the same jobs one after another through sgr_render, for comparison
-    import torch
     t0 = time.perf_counter()
     for sc, cam, p in J:
         acc = torch.zeros((p.height, p.width, 5), dtype=torch.float32, device="cuda")
@@ -347,11 +349,13 @@ def serve_bench(args):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
             "data": "synthetic: seeded job distribution over a 48-scene pool",
             "config": {"workload": f"Poisson {rate} jobs/s for {args.duration} s, <=32 resident "
-                                   "scenes, admission by HBM footprint",
+                                   "scenes (prewarmed) of a 48-scene pool, admission by HBM "
+                                   "footprint",
                        "parallelism": "single GPU"},
             "p50_ms": round(rep["p50_ms"], 1), "p99_ms": round(rep["p99_ms"], 1),
             "jobs": rep["jobs"], "wall_s": round(rep["wall_s"], 2),
             "max_active_jobs": rep["max_active_jobs"],
+            "prewarm_s": round(rep["prewarm_s"], 1),
             "peak_budget_gb": round(rep["peak_budget_bytes"] / 2 ** 30, 2),
             "cpu_pixel_checks": rep["checks"],
         })
@@ -369,6 +373,8 @@ def main():
     ap.add_argument("--workload", default="render",
                     choices=("render", "cornell", "batch", "serve"))
     ap.add_argument("--rates", type=float, nargs="+", default=[4.0, 16.0, 64.0])
+    ap.add_argument("--paths-in-flight", type=int, default=0,
+                    help="paths per chunk of a frame (0: the library default)")
     ap.add_argument("--duration", type=float, default=120.0)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one plain 256 MiB device copy (reference kernel "

@@ -179,17 +179,21 @@ typedef struct sgr_render_params {
     uint32_t direct_samples;       /* default 1; 0 disables direct light */
     uint64_t seed;                 /* job seed for the counter RNG */
     float    filter_b, filter_c;   /* Mitchell B, C (1/3, 1/3) */
-    float    filter_radius;        /* 2 */
+    float    filter_radius;        /* 2 (at most 12.5) */
     float    neg_clamp;            /* -0.1 */
     float    bg_alpha;             /* "background_alpha", 1 */
     /* Rendered window [x0, x0+w) x [y0, y0+h) of the image; the filter
      * margin (ceil(radius - 0.5001) pixels, snogray.lua:546-550) is added
      * around it by the renderer. w == 0 means the whole image. */
     uint32_t x0, y0, w, h;
-    /* Paths kept in flight per wavefront step (0: choose). */
+    /* Paths kept in flight per wavefront step (0: the default below);
+     * a frame runs in chunks of whole-image x floor(this / pixels)
+     * samples. */
     uint32_t paths_in_flight;
     uint32_t flags;                /* SGR_RENDER_* */
 } sgr_render_params;
+
+#define SGR_DEFAULT_PATHS_IN_FLIGHT (1u << 26)
 
 /* sgr_render_params.flags: time every kernel launch with HIP events on
  * the render stream (no host waits) into sgr_stats.ms_* */
@@ -300,7 +304,8 @@ int sgr_finalize_image(const float *accum, uint32_t width, uint32_t height,
  * gather/splat sequence; a job contributes at most one chunk per step.
  * Not thread-safe: one host thread drives a context's engine. */
 typedef struct sgr_engine_config {
-    uint64_t path_capacity;   /* paths in flight per step (0: 1<<22) */
+    uint64_t path_capacity;   /* paths in flight per step (0: the
+                                 default SGR_DEFAULT_PATHS_IN_FLIGHT) */
     uint64_t memory_budget;   /* bytes of job accumulators admitted at
                                  once (0: no limit) */
     uint32_t grid_blocks;     /* workgroups per launch (0: 16 per CU);

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import abi, native
 
-DEFAULT_CAPACITIES = (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+DEFAULT_CAPACITIES = (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
 DEFAULT_GRIDS = (0, 4096 * 2, 4096 * 4)  # 0: 16 workgroups per CU
 
 
@@ -65,7 +65,7 @@ def autotune(ctx, jobs, capacities=DEFAULT_CAPACITIES, grids=DEFAULT_GRIDS, step
     return best, table
 
 
-def run_batch(ctx, jobs, path_capacity=1 << 22, grid_blocks=0, keep_results=False):
+def run_batch(ctx, jobs, path_capacity=abi.SGR_DEFAULT_PATHS_IN_FLIGHT, grid_blocks=0, keep_results=False):
     """Submit every job, step until all are done.  Returns a report with
     aggregate samples/s and per-job latency (submit to done)."""
     eng = native.Engine(ctx, path_capacity=path_capacity, grid_blocks=grid_blocks,

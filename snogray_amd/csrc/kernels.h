@@ -54,22 +54,22 @@ struct RenderView {
   float cam_pos[3], fwd[3], up[3], right[3];
   float tanx, tany, max_trace;
   // Multi-job steps: scene table slot, first path of this segment in the
-  // step, first output pixel of this segment in the step's splat, and the
-  // job's accumulators (device).
+  // step, first splat tile of this segment in the step, and the job's
+  // accumulators (device).
   uint32_t scene, path_base, out_base;
   float *accum;
 };
 
 // One engine step over several jobs: per-segment render views, per-slot
-// scene views, and prefix offsets of the segments' paths and splat pixels.
+// scene views, and prefix offsets of the segments' paths and splat tiles.
 #define SGR_MAX_SEGMENTS 256
 struct StepView {
   const RenderView *R;
   const SceneView *S;
   const uint32_t *seg_base;  // nseg + 1 path offsets
-  const uint32_t *out_base;  // nseg + 1 splat-pixel offsets
-  const uint32_t *hdr;       // nseg, npaths, nout (device: a captured step
-                             // graph replays with new tables)
+  const uint32_t *out_base;  // nseg + 1 splat-tile offsets
+  const uint32_t *hdr;       // segments, paths, splat tiles (device: a
+                             // captured step graph replays with new tables)
 };
 
 struct PathBufs {

@@ -738,6 +738,8 @@ int plan_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p
   if (h == 0 || (uint64_t)x0 + w > p->width || (uint64_t)y0 + h > p->height)
     return fail(SGR_ERR_INVALID, "sgr_render: window outside the image");
   int fr = (int)std::ceil(p->filter_radius - 0.5001f);  // image/image-filter-conv.h:39-40
+  if (fr > 12)
+    return fail(SGR_ERR_LIMIT, "sgr_render: filter_radius above 12.5 is not supported");
   R.width = p->width; R.height = p->height; R.spp = p->spp;
   R.minp = p->min_path_len; R.maxp = p->max_path_len; R.ds = p->direct_samples;
   R.L = sc->n_lights;
@@ -774,7 +776,7 @@ int plan_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p
   float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
   R.max_trace = std::sqrt(ex * ex + ey * ey + ez * ez);
   pl.npix = (uint32_t)(R.PW * R.PH);
-  uint64_t want = p->paths_in_flight ? p->paths_in_flight : (1u << 22);
+  uint64_t want = p->paths_in_flight ? p->paths_in_flight : SGR_DEFAULT_PATHS_IN_FLIGHT;
   uint64_t sb = want / pl.npix;
   if (sb < 1) sb = 1;
   if (sb > R.spp) sb = R.spp;

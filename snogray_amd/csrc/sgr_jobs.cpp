@@ -34,7 +34,7 @@ struct StepTables {
   SceneView S[SGR_MAX_SCENE_SLOTS];
   uint32_t seg_base[SGR_MAX_SEGMENTS + 1];
   uint32_t out_base[SGR_MAX_SEGMENTS + 1];
-  uint32_t hdr[4];  // nseg, npaths, nout
+  uint32_t hdr[4];  // segments, paths, splat tiles
 };
 
 struct Job {
@@ -49,7 +49,7 @@ struct Job {
   int state = SGR_JOB_QUEUED;
   double t_submit = 0, t_start = -1, t_done = -1;
   uint64_t footprint = 0;
-  uint32_t npix_out = 0;  // image pixels the splat writes
+  uint32_t tiles_out = 0;  // splat tiles of a chunk
 };
 
 struct Engine {
@@ -101,7 +101,7 @@ int engine_get(sgr_ctx *ctx, Engine *&E) {
     return SGR_OK;
   }
   E = new Engine;
-  E->cfg.path_capacity = 1u << 22;
+  E->cfg.path_capacity = SGR_DEFAULT_PATHS_IN_FLIGHT;
   E->cfg.use_graph = 1;
   hipError_t e = hipHostMalloc((void **)&E->host, sizeof(StepTables), hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc((void **)&E->dev, sizeof(StepTables));
@@ -149,12 +149,14 @@ uint64_t accum_bytes(const sgr_render_params *p) {
   return (uint64_t)p->width * p->height * 5 * sizeof(float);
 }
 
-// Image pixels the splat of a chunk writes (the pattern window clipped to
-// the image), as the splat kernel bounds them.
-uint32_t splat_pixels(const RenderView &R) {
+// 16 x 16 tiles of image pixels the splat of a chunk writes (the pattern
+// window clipped to the image), as the splat kernel enumerates them.
+uint32_t splat_tiles(const RenderView &R) {
   int ox0 = std::max(R.X0, 0), oy0 = std::max(R.Y0, 0);
   int ox1 = std::min(R.X0 + R.PW, (int)R.width), oy1 = std::min(R.Y0 + R.PH, (int)R.height);
-  return (uint32_t)(std::max(ox1 - ox0, 0) * std::max(oy1 - oy0, 0));
+  int ow = ox1 - ox0, oh = oy1 - oy0;
+  if (ow <= 0 || oh <= 0) return 0;
+  return (uint32_t)((ow + 15) / 16) * (uint32_t)((oh + 15) / 16);
 }
 
 void enqueue_step(sgr_ctx *ctx, Engine *E, uint32_t maxp, unsigned grid) {
@@ -255,7 +257,7 @@ int sgr_engine_configure(sgr_ctx *ctx, const sgr_engine_config *cfg) {
   int rc = engine_get(ctx, E);
   if (rc) return rc;
   if (!E->jobs.empty()) return fail(SGR_ERR_INVALID, "sgr_engine_configure: engine holds jobs");
-  uint64_t cap = cfg->path_capacity ? cfg->path_capacity : (1u << 22);
+  uint64_t cap = cfg->path_capacity ? cfg->path_capacity : (SGR_DEFAULT_PATHS_IN_FLIGHT);
   if (cap >= (1ull << 31)) return fail(SGR_ERR_LIMIT, "sgr_engine_configure: path_capacity too large");
   if (cfg->grid_blocks > (1u << 20)) return fail(SGR_ERR_LIMIT, "sgr_engine_configure: grid_blocks too large");
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -314,7 +316,7 @@ int sgr_job_submit(sgr_ctx *ctx, sgr_scene *sc, const sgr_camera *cam,
     j.owns_accum = true;
   }
   j.pl.R.accum = j.accum;
-  j.npix_out = splat_pixels(j.pl.R);
+  j.tiles_out = splat_tiles(j.pl.R);
   j.id = E->next_id++;
   j.sc = sc;
   j.t_submit = E->now_ms();
@@ -378,7 +380,7 @@ int sgr_engine_step(sgr_ctx *ctx, sgr_step_info *info) {
     T.seg_base[i] = pb;
     T.out_base[i] = ob;
     pb += R.npaths;
-    ob += j.npix_out;
+    ob += j.tiles_out;
   }
   T.seg_base[run.size()] = pb;
   T.out_base[run.size()] = ob;

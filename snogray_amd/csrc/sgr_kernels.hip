@@ -448,9 +448,21 @@ __device__ __forceinline__ float mitchell1(float x, float inv_radius, float B, f
 // Gather form of ImageFilterConv::add_sample: each image pixel sums the
 // weighted samples of the pattern pixels within the filter radius (a
 // sample of pattern pixel x reaches image pixels int(sx) +- radius; the
-// taps that truncation adds or drops all carry weight 0).  One thread
-// per image pixel, no atomics, fixed summation order.  The image pixels
-// of a segment are those its pattern window covers, clipped to the image.
+// taps that truncation adds or drops all carry weight 0).  No atomics,
+// fixed summation order.  The image pixels of a segment are those its
+// pattern window covers, clipped to the image.
+//
+// A workgroup takes a 16 x 16 tile of image pixels.  Per pass it stages
+// the records (sx, sy, r, g, b, a) of G consecutive samples of every
+// pattern pixel of the tile plus its filter halo into LDS with
+// contiguous loads (a pixel's samples are adjacent in the output
+// buffer), then every thread accumulates its pixel's taps from LDS:
+// each record is read from memory once per tile instead of once per tap.
+// Per pixel the sum runs over passes, then halo rows and columns, then
+// the samples of the pass.
+#define SPLAT_T 16
+#define SPLAT_LDS_RECS 1600   // records staged per pass (38.4 KB)
+
 __device__ __forceinline__ void splat_bounds(const RenderView &R, int &ox0, int &oy0, int &ow,
                                              int &oh) {
   ox0 = R.X0 > 0 ? R.X0 : 0;
@@ -461,53 +473,85 @@ __device__ __forceinline__ void splat_bounds(const RenderView &R, int &ox0, int 
   oh = oy1 - oy0;
 }
 
+__device__ __forceinline__ uint32_t splat_tiles(const RenderView &R) {
+  int ox0, oy0, ow, oh;
+  splat_bounds(R, ox0, oy0, ow, oh);
+  if (ow <= 0 || oh <= 0) return 0u;
+  return (uint32_t)((ow + SPLAT_T - 1) / SPLAT_T) * (uint32_t)((oh + SPLAT_T - 1) / SPLAT_T);
+}
+
 template<bool MULTI>
 __global__ void __launch_bounds__(256) k_splat(RenderView R1, StepView J, const float *out) {
-  uint32_t total;
-  if (MULTI) {
-    total = J.hdr[2];
-  } else {
-    int ox0, oy0, ow, oh;
-    splat_bounds(R1, ox0, oy0, ow, oh);
-    total = (ow > 0 && oh > 0) ? (uint32_t)(ow * oh) : 0u;
-  }
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
-    const RenderView R = MULTI ? J.R[find_segment(J.out_base, J.hdr[0], g)] : R1;
+  __shared__ float rec[6][SPLAT_LDS_RECS];
+  const uint32_t total = MULTI ? J.hdr[2] : splat_tiles(R1);
+  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const RenderView R = MULTI ? J.R[find_segment(J.out_base, J.hdr[0], t)] : R1;
     int ox0, oy0, ow, oh;
     splat_bounds(R, ox0, oy0, ow, oh);
-    uint32_t i = g - R.out_base;
-    int qx = ox0 + (int)(i % (uint32_t)ow), qy = oy0 + (int)(i / (uint32_t)ow);
+    const uint32_t ti = t - (MULTI ? R.out_base : 0u);
+    const uint32_t ntx = (uint32_t)(ow + SPLAT_T - 1) / SPLAT_T;
+    const int tx0 = ox0 + (int)(ti % ntx) * SPLAT_T, ty0 = oy0 + (int)(ti / ntx) * SPLAT_T;
+    const int fr = R.fr;
+    const int RW = SPLAT_T + 2 * fr;            // staged region, pattern pixels
+    const uint32_t region = (uint32_t)(RW * RW);
+    uint32_t G = SPLAT_LDS_RECS / region;       // samples per pass
+    if (G > R.sb) G = R.sb;
+    const int lx = (int)(threadIdx.x % SPLAT_T), ly = (int)(threadIdx.x / SPLAT_T);
+    const int qx = tx0 + lx, qy = ty0 + ly;
+    const bool live = qx < ox0 + ow && qy < oy0 + oh;
     float acc[5] = {0, 0, 0, 0, 0};
-    int sy0 = qy - R.fr > R.Y0 ? qy - R.fr : R.Y0;
-    int sy1 = qy + R.fr < R.Y0 + R.PH - 1 ? qy + R.fr : R.Y0 + R.PH - 1;
-    int sx0 = qx - R.fr > R.X0 ? qx - R.fr : R.X0;
-    int sx1 = qx + R.fr < R.X0 + R.PW - 1 ? qx + R.fr : R.X0 + R.PW - 1;
-    for (int sy = sy0; sy <= sy1; sy++)
-      for (int sx = sx0; sx <= sx1; sx++) {
-        uint32_t pix = (uint32_t)(sy - R.Y0) * (uint32_t)R.PW + (uint32_t)(sx - R.X0);
-        const float *o = out + 6 * ((size_t)R.path_base + (size_t)pix * R.sb);
-        for (uint32_t k = 0; k < R.sb; k++, o += 6) {
-          float fx = o[0], fy = o[1];
-          int ix = (int)fx, iy = (int)fy;
-          if (ix - qx > R.fr || qx - ix > R.fr || iy - qy > R.fr || qy - iy > R.fr) continue;
-          float w = mitchell1(fx - ((float)qx + 0.5f), R.inv_radius, R.fb, R.fc) *
-                    mitchell1(fy - ((float)qy + 0.5f), R.inv_radius, R.fb, R.fc);
-          float ws0 = o[2] * w, ws1 = o[3] * w, ws2 = o[4] * w, ws3 = o[5] * w, wr = w;
-          if (w < 0.0f) {
-            float mc = ws0;
-            mc = ws1 < mc ? ws1 : mc;
-            mc = ws2 < mc ? ws2 : mc;
-            if (mc < R.neg_clamp) {
-              float cs = R.neg_clamp / mc;
-              wr = wr * cs;
-              ws0 = ws0 * cs; ws1 = ws1 * cs; ws2 = ws2 * cs; ws3 = ws3 * cs;
-            }
-          }
-          acc[0] += ws0; acc[1] += ws1; acc[2] += ws2; acc[3] += ws3; acc[4] += wr;
+    for (uint32_t g0 = 0; g0 < R.sb; g0 += G) {
+      const uint32_t gn = R.sb - g0 < G ? R.sb - g0 : G;
+      const uint32_t nrec = region * gn;
+      __syncthreads();  // previous pass done with the records
+      for (uint32_t e = threadIdx.x; e < nrec; e += blockDim.x) {
+        const uint32_t cell = e / gn, k = e - cell * gn;
+        const int sx = tx0 - fr + (int)(cell % (uint32_t)RW);
+        const int sy = ty0 - fr + (int)(cell / (uint32_t)RW);
+        float v[6] = {-1e30f, -1e30f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (sx >= R.X0 && sx < R.X0 + R.PW && sy >= R.Y0 && sy < R.Y0 + R.PH) {
+          const uint32_t pix = (uint32_t)(sy - R.Y0) * (uint32_t)R.PW + (uint32_t)(sx - R.X0);
+          const float2 *o = (const float2 *)(out + 6 * ((size_t)R.path_base +
+                                                        (size_t)pix * R.sb + g0 + k));
+          const float2 a = o[0], b = o[1], c = o[2];
+          v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y;
         }
+        const uint32_t slot = cell * gn + k;
+        for (int f = 0; f < 6; f++) rec[f][slot] = v[f];
       }
-    float *a = R.accum + 5 * ((size_t)qy * R.width + (size_t)qx);
-    a[0] += acc[0]; a[1] += acc[1]; a[2] += acc[2]; a[3] += acc[3]; a[4] += acc[4];
+      __syncthreads();
+      if (!live) continue;
+      for (int dy = -fr; dy <= fr; dy++)
+        for (int dx = -fr; dx <= fr; dx++) {
+          const uint32_t cell = (uint32_t)((ly + fr + dy) * RW + (lx + fr + dx));
+          for (uint32_t k = 0; k < gn; k++) {
+            const uint32_t slot = cell * gn + k;
+            const float fx = rec[0][slot], fy = rec[1][slot];
+            if (fx < -1e29f) continue;  // outside the pattern window
+            const int ix = (int)fx, iy = (int)fy;
+            if (ix - qx > fr || qx - ix > fr || iy - qy > fr || qy - iy > fr) continue;
+            const float w = mitchell1(fx - ((float)qx + 0.5f), R.inv_radius, R.fb, R.fc) *
+                            mitchell1(fy - ((float)qy + 0.5f), R.inv_radius, R.fb, R.fc);
+            float ws0 = rec[2][slot] * w, ws1 = rec[3][slot] * w, ws2 = rec[4][slot] * w;
+            float ws3 = rec[5][slot] * w, wr = w;
+            if (w < 0.0f) {
+              float mc = ws0;
+              mc = ws1 < mc ? ws1 : mc;
+              mc = ws2 < mc ? ws2 : mc;
+              if (mc < R.neg_clamp) {
+                const float cs = R.neg_clamp / mc;
+                wr = wr * cs;
+                ws0 = ws0 * cs; ws1 = ws1 * cs; ws2 = ws2 * cs; ws3 = ws3 * cs;
+              }
+            }
+            acc[0] += ws0; acc[1] += ws1; acc[2] += ws2; acc[3] += ws3; acc[4] += wr;
+          }
+        }
+    }
+    if (live) {
+      float *a = R.accum + 5 * ((size_t)qy * R.width + (size_t)qx);
+      a[0] += acc[0]; a[1] += acc[1]; a[2] += acc[2]; a[3] += acc[3]; a[4] += acc[4];
+    }
   }
 }
 

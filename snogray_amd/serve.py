@@ -158,8 +158,8 @@ class Server:
     """Drives one context's job engine for a stream of (arrival time,
     JobSpec) requests; see the module docstring."""
 
-    def __init__(self, ctx, budget, max_scenes=32, path_capacity=1 << 22, check_every=50,
-                 check_fn=None, use_graph=True):
+    def __init__(self, ctx, budget, max_scenes=32, path_capacity=1 << 24, check_every=50,
+                 check_fn=None, use_graph=True, builders=6):
         self.ctx = ctx
         self.engine = native.Engine(ctx, path_capacity=path_capacity, use_graph=use_graph)
         # the engine's path buffers are held for the whole run
@@ -167,16 +167,42 @@ class Server:
         self.planner = ResidencyPlanner(budget, max_scenes, reserved=per_path * path_capacity)
         self.scenes = {}        # SceneSpec -> native.Scene
         self.building = {}      # SceneSpec -> Future
-        self.builder = cf.ThreadPoolExecutor(1)
+        self.builder = cf.ThreadPoolExecutor(builders)
+        self.host_descs = {}    # SceneSpec -> SceneDesc (host copies outlive eviction)
         self.checker = cf.ThreadPoolExecutor(1)
         self.check_every = check_every if check_fn else 0
         self.check_fn = check_fn  # (job, scene desc, accumulators) -> dict
         self.descs = {}
 
     def _build(self, spec):
-        sd, _ = spec.build()
+        sd = self.host_descs.get(spec)
+        if sd is None:
+            sd, _ = spec.build()
+            self.host_descs[spec] = sd
         sc = native.Scene(self.ctx, sd)
         return sd, sc
+
+    def prewarm(self, specs):
+        """Make SPECS resident (up to the scene cap and budget) before
+        requests start; returns the seconds it took."""
+        t0 = time.perf_counter()
+        futs = {}
+        for spec in specs:
+            if len(self.planner.scenes) >= self.planner.max_scenes:
+                break
+            if spec in self.planner.scenes:
+                continue
+            if self.planner.plan_scene(spec, spec.estimated_bytes()) != []:
+                break
+            self.planner.add_scene(spec, spec.estimated_bytes())
+            futs[spec] = self.builder.submit(self._build, spec)
+        for spec, f in futs.items():
+            sd, sc = f.result()
+            self.descs[spec] = sd
+            self.scenes[spec] = sc
+            if not self.planner.resize_scene(spec, sc.footprint()):
+                raise RuntimeError("prewarmed scenes exceed the budget")
+        return time.perf_counter() - t0
 
     def run(self, requests, drain=True, log=None, keep_results=False):
         """REQUESTS: list of (arrival seconds, JobSpec), sorted by time.
@@ -212,7 +238,7 @@ class Server:
                     if not self.planner.resize_scene(spec, fp):
                         # measured above the estimate: evict idle scenes
                         ev = self.planner.make_room(fp - self.planner.scenes[spec][0],
-                                                    protect=(spec,))
+                                                    protect={spec} | set(self.building))
                         if ev is None:
                             raise RuntimeError("scene does not fit the HBM budget")
                         for k in ev:
@@ -236,7 +262,10 @@ class Server:
                 elif spec not in self.building and spec not in self.scenes:
                     if spec.estimated_bytes() + nbytes > self.planner.budget - self.planner.reserved:
                         raise RuntimeError(f"job {idx}: scene and accumulators exceed the budget")
-                    ev = self.planner.plan_scene(spec, spec.estimated_bytes())
+                    # scenes being built or wanted by waiting jobs stay
+                    wanted = set(self.building) | {j.scene for _, _, j in pending}
+                    wanted |= {j.scene for _, _, j in keep}
+                    ev = self.planner.plan_scene(spec, spec.estimated_bytes(), protect=wanted)
                     if ev is not None:
                         for k in ev:
                             self._evict(k)
@@ -310,9 +339,12 @@ class Server:
             self.scenes.pop(spec).close()
 
 
-def serve_poisson(ctx, rate, duration, n_scene_pool=48, seed=1, budget=None, log=None, **kw):
+def serve_poisson(ctx, rate, duration, n_scene_pool=48, seed=1, budget=None, log=None,
+                  prewarm=True, **kw):
     """Poisson arrivals at RATE jobs/s for DURATION s, jobs drawn from the
-    batched-job distribution over a pool of scenes."""
+    batched-job distribution over a pool of scenes.  With PREWARM the
+    first max_scenes scenes of the pool are made resident before the
+    first arrival (the steady state of a long-running server)."""
     from . import workloads
     pool = workloads.scene_pool(n_scene_pool, seed)
     times = workloads.poisson_arrivals(rate, duration, seed + 1)
@@ -321,8 +353,10 @@ def serve_poisson(ctx, rate, duration, n_scene_pool=48, seed=1, budget=None, log
         budget = 64 << 30
     srv = Server(ctx, budget, **kw)
     try:
+        t_warm = srv.prewarm(pool) if prewarm else 0.0
         rep = srv.run(list(zip(times, jobs)), log=log)
     finally:
         srv.close()
-    rep.update({"rate_jobs_per_s": rate, "duration_s": duration, "scene_pool": n_scene_pool})
+    rep.update({"rate_jobs_per_s": rate, "duration_s": duration, "scene_pool": n_scene_pool,
+                "prewarm_s": t_warm})
     return rep
