@@ -1,7 +1,7 @@
 """Benchmarks of the GPU path tracer on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                    [--workload render|cornell|batch|serve]
+                    [--workload render|cornell|batch|serve|learned]
 
 render (default): plain path tracing of the seeded 1.31M-triangle
 icosphere soup at 1920x1080, 64 spp, max path length 10.  One step is
@@ -15,6 +15,8 @@ batch: 64 seeded jobs of uneven resolution, spp and scene size in the
 multi-job engine, configuration chosen by the autotuner.
 serve: Poisson arrivals of the same job distribution (4, 16, 64 jobs/s),
 continuous batching, p50/p99 latency.
+learned: the learned proposal on the Cornell box at 1920x1080 against
+plain path tracing at equal time (relMSE vs the FP64 CPU reference).
 Prints one JSON line per run (per rate for serve).
 """
 import argparse
@@ -363,6 +365,92 @@ def serve_bench(args):
     return lines
 
 
+def learned_bench(args):
+    """Learned proposal on the Cornell box at 1920x1080: a fresh guide
+    trained online while it renders 64 spp (16 chunks of 4 spp), against
+    plain GPU path tracing given the same wall time; relMSE of both
+    against the FP64 CPU reference at 4096 spp on 16 crops."""
+    import torch
+    from snogray_amd import abi, native, scenes
+    torch.zeros(1, device="cuda")
+    width, height, spp, maxp, minp = 1920, 1080, 64, 8, 3
+    sd, cam = scenes.cornell_box(aspect=width / height)
+    ctx = native.Context(0)
+    sc = native.Scene(ctx, sd)
+    acc = torch.zeros((height, width, 5), dtype=torch.float32, device="cuda")
+
+    def plain(n_spp, seed):
+        acc.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sc.render_device(cam, abi.make_params(width, height, n_spp, max_path_len=maxp,
+                                              min_path_len=minp, seed=seed), acc.data_ptr())
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, acc.cpu().numpy()
+
+    def guided(n_spp, seed, pif):
+        g = native.Guide(ctx, seed=seed)
+        acc.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sc.render_guided_device(cam, abi.make_params(width, height, n_spp, max_path_len=maxp,
+                                                     min_path_len=minp, seed=seed,
+                                                     paths_in_flight=pif), g, acc.data_ptr())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        st = g.stats()
+        g.close()
+        return dt, acc.cpu().numpy(), st
+
+    pif = (width + 4) * (height + 4) * 4
+    log("warm-up")
+    plain(4, 1)
+    guided(4, 1, pif)
+    log("plain 64 spp")
+    t_p64, img_p64 = plain(spp, args.seed)
+    log("guided 64 spp (training online)")
+    t_g, img_g, gst = guided(spp, args.seed + 1, pif)
+    spp_eq = spp * t_g / t_p64
+    log(f"plain at equal time: {spp_eq:.1f} spp")
+    t_peq, img_peq = plain(max(1, int(round(spp_eq))), args.seed + 2)
+    line = {
+        "metric": "equal-time relMSE gain of the learned proposal (Cornell box 1920x1080)",
+        "value": None, "unit": "x (plain relMSE / guided relMSE at equal time)", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(1e3 * t_g, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 render, bf16 MLP with fp32 accumulation",
+        "data": "synthetic: Cornell box built in code",
+        "config": {"workload": "Cornell box 1920x1080, 64 spp guided (fresh guide, online "
+                               "training, 16 chunks), max path length 8; plain path tracing "
+                               "given the same time",
+                   "parallelism": "single GPU"},
+        "guided_s": round(t_g, 3), "plain_64spp_s": round(t_p64, 3),
+        "plain_equal_time_spp": round(spp_eq, 2), "plain_equal_time_s": round(t_peq, 3),
+        "guided_samples_per_s": round(width * height * spp / t_g, 1),
+        "plain_samples_per_s": round(width * height * spp / t_p64, 1),
+        "guide_stats": gst,
+    }
+    if not args.no_cpu:
+        cpu = CpuBaseline(sd)
+        log("fp64 reference crops")
+        ref, t_ref = cpu.crops(cam, width, height, REF_SPP, maxp, minp, args.seed + 1000,
+                               fp64=True)
+        wins = crop_windows(width, height)
+        r_g = relmse(finalize(take_crops(img_g, wins)), ref)
+        r_p64 = relmse(finalize(take_crops(img_p64, wins)), ref)
+        r_peq = relmse(finalize(take_crops(img_peq, wins)), ref)
+        line["value"] = round(r_peq / r_g, 3)
+        line["relmse"] = {"reference": f"FP64 CPU oracle at {REF_SPP} spp on {len(wins)} crops "
+                                       f"of {CROP_W}x{CROP_H} ({t_ref:.1f} s)",
+                          "guided_64spp": float(f"{r_g:.4g}"),
+                          "plain_64spp": float(f"{r_p64:.4g}"),
+                          "plain_equal_time": float(f"{r_peq:.4g}"),
+                          "equal_spp_gain": round(r_p64 / r_g, 3)}
+    sc.close()
+    ctx.close()
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,7 +459,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--workload", default="render",
-                    choices=("render", "cornell", "batch", "serve"))
+                    choices=("render", "cornell", "batch", "serve", "learned"))
     ap.add_argument("--rates", type=float, nargs="+", default=[4.0, 16.0, 64.0])
     ap.add_argument("--paths-in-flight", type=int, default=0,
                     help="paths per chunk of a frame (0: the library default)")
@@ -407,6 +495,8 @@ def main():
     elif args.workload == "serve":
         for line in serve_bench(args):
             print(json.dumps(line), flush=True)
+    elif args.workload == "learned":
+        print(json.dumps(learned_bench(args)), flush=True)
 
 
 if __name__ == "__main__":

@@ -365,6 +365,51 @@ int sgr_render_batch(sgr_ctx *ctx, uint32_t n, sgr_scene *const *scenes,
                      const sgr_camera *cams, const sgr_render_params *params,
                      float *const *accum, int on_device, sgr_stats *stats);
 
+/* ---- learned directional proposal -------------------------------------
+ * No counterpart in the reference (its continuation direction is always
+ * a BSDF sample, render/path-integ.cc:308-401).  A guide is a small MLP
+ * (position in the scene box and outgoing direction, 6-frequency sin/cos
+ * encoding, 3 x 64 ReLU, 8 von Mises-Fisher lobes; bf16 weights, fp32
+ * accumulation) trained online on the tracer's own radiance samples.
+ * A guided render picks each non-delta continuation direction by
+ * one-sample MIS: the BSDF with probability 0.5, else the lobes, and
+ * divides by the mixture density, so the estimate stays unbiased; glass
+ * and mirror vertices sample their BSDF only. */
+typedef struct sgr_guide sgr_guide;
+
+typedef struct sgr_guide_stats {
+    uint64_t train_steps;     /* Adam steps taken */
+    uint64_t records;         /* training records consumed */
+    double   last_loss;       /* weighted mean -log q of the last step */
+} sgr_guide_stats;
+
+int sgr_guide_create(sgr_ctx *ctx, uint64_t seed, sgr_guide **out);
+int sgr_guide_destroy(sgr_guide *guide);
+/* fp32 parameters: count = 16624 (W[out][in] row-major then bias, for
+ * 80->64, 64->64, 64->64, 64->48; padding entries are zero).  out may be
+ * NULL to query the count. */
+int sgr_guide_get_params(sgr_guide *guide, float *out, uint64_t *count);
+/* Replace the parameters (bf16 copy refreshed, Adam state reset). */
+int sgr_guide_set_params(sgr_guide *guide, const float *in, uint64_t count);
+/* Forward pass of n <= 4096 rows of 6 inputs (position normalised to the
+ * scene box, unit outgoing direction), host memory: 40 outputs per row,
+ * raw (lobes == 0) or as lobes (mu xyz, kappa, weight) x 8. */
+int sgr_guide_forward(sgr_guide *guide, const float *in6, uint64_t n, float *out,
+                      int lobes);
+/* One training step on n <= 16384 records of 16 floats (0-5 inputs,
+ * 6-8 unit incoming direction, 9 weight >= 0; the rest ignored): the
+ * gradient of sum(w * -log q) / sum(w) into grad (16624 floats, may be
+ * NULL), loss[0..1] = (sum w * -log q, sum w), and an Adam step with
+ * learning rate lr unless lr == 0. */
+int sgr_guide_train_batch(sgr_guide *guide, const float *rec, uint64_t n, float lr,
+                          float *grad, float *loss);
+int sgr_guide_stats_get(const sgr_guide *guide, sgr_guide_stats *stats);
+/* sgr_render with the guide; train != 0 takes Adam steps (lr 1e-3) on
+ * each chunk's records before the next chunk. */
+int sgr_render_guided(sgr_scene *scene, const sgr_camera *cam,
+                      const sgr_render_params *params, sgr_guide *guide, int train,
+                      float *accum, int accum_on_device, sgr_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

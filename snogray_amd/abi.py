@@ -131,6 +131,17 @@ class StepInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class GuideStats(C.Structure):
+    _fields_ = [("train_steps", C.c_uint64), ("records", C.c_uint64), ("last_loss", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+GD_NPARAM = 16624
+GD_REC = 16
+
+
 class OctreeStats(C.Structure):
     _fields_ = [("num_nodes", C.c_uint64), ("num_leaf_nodes", C.c_uint64),
                 ("num_surfaces", C.c_uint64),

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "dev_common.h"
+#include "guide.h"
 
 namespace sgr {
 
@@ -98,13 +99,15 @@ void launch_raygen(bool multi, const RenderView &R, const StepView &J, const Pat
                    unsigned grid, hipStream_t st);
 void launch_extend(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                    uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st);
+// G: the learned proposal's view, or NULL for plain BSDF sampling
+// (single-job renders only).
 void launch_shade(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
                   const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
-                  hipStream_t st);
+                  hipStream_t st, const GuideView *G = nullptr);
 void launch_occlude(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                     uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st);
 void launch_gather(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
-                   uint32_t b, unsigned grid, hipStream_t st);
+                   uint32_t b, unsigned grid, hipStream_t st, const GuideView *G = nullptr);
 void launch_splat(bool multi, const RenderView &R, const StepView &J, const float *out,
                   unsigned grid, hipStream_t st);
 void launch_count_queues(uint32_t maxp, const PathBufs &P, unsigned long long *stats,

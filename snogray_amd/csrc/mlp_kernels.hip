@@ -1,0 +1,421 @@
+// MLP kernels of the learned proposal: input rows of the ray queue,
+// forward pass (MFMA bf16 x bf16 -> fp32, 16 rows per wave), the
+// training passes (per-row forward + backward, weight gradients as
+// row-reduction GEMMs in fixed order, Adam) -- all deterministic.
+#include <hip/hip_runtime.h>
+
+#include "dev_common.h"
+#include "dev_guide.h"
+#include "guide.h"
+
+namespace sgr {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+#define GD_WAVES 4  // waves per workgroup
+
+// Orders this wave's LDS writes before its later LDS reads (lanes of one
+// wave exchange tiles through LDS; no other wave touches them).
+#define WAVE_SYNC()                                        \
+  do {                                                     \
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); \
+    __builtin_amdgcn_wave_barrier();                       \
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
+  } while (0)
+
+// Per-wave LDS of a 16-row tile.
+struct TileLds {
+  uint16_t a0[16 * GD_FEAT];
+  uint16_t a1[16 * GD_H];
+  uint16_t a2[16 * GD_H];
+  uint16_t a3[16 * GD_H];
+  float out[16 * GD_OUTP];
+  uint16_t d[16 * GD_H];   // gradient of the current layer's output
+  uint16_t e[16 * GD_H];   // gradient being formed
+  float rl[16], rw[16];    // per-row loss and weight
+};
+
+// out[16][N] = act[16][K] . W[N][K]^T + b (optional ReLU), MFMA
+// v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)+i] and
+// B[4(l/16)+i][l%16]; D[4(l/16)+i][l%16] comes back in acc[i].
+template<int K, int N, bool RELU, bool F32OUT>
+__device__ __forceinline__ void mlp_layer(const uint16_t *act, const uint16_t *W, const float *bias,
+                                          uint16_t *outb, float *outf, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+  for (int nt = 0; nt < N / 16; nt++) {
+    f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kt = 0; kt < K / 16; kt++) {
+      const s4v a = *(const s4v *)(act + r * K + kt * 16 + 4 * g);
+      const s4v b = *(const s4v *)(W + (size_t)(nt * 16 + r) * K + kt * 16 + 4 * g);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    const int col = nt * 16 + r;
+    const float bb = bias[col];
+    for (int i = 0; i < 4; i++) {
+      const int row = 4 * g + i;
+      float z = acc[i] + bb;
+      if (RELU) z = fmaxf(z, 0.0f);
+      if (F32OUT) outf[row * N + col] = z;
+      else outb[row * N + col] = f2bf(z);
+    }
+  }
+}
+
+// Backward through one layer: e[16][KIN] = (d[16][NOUT] . W[NOUT][KIN])
+// masked by act > 0 (the ReLU of the layer below).
+template<int NOUT, int KIN>
+__device__ __forceinline__ void mlp_back(const uint16_t *d, const uint16_t *W, const uint16_t *act,
+                                         uint16_t *e, int lane) {
+  const int r = lane & 15, g = lane >> 4;
+  for (int nt = 0; nt < KIN / 16; nt++) {
+    f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kt = 0; kt < NOUT / 16; kt++) {
+      const s4v a = *(const s4v *)(d + r * NOUT + kt * 16 + 4 * g);
+      s4v b;
+      for (int i = 0; i < 4; i++)
+        b[i] = (short)W[(size_t)(kt * 16 + 4 * g + i) * KIN + nt * 16 + r];
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    const int col = nt * 16 + r;
+    for (int i = 0; i < 4; i++) {
+      const int row = 4 * g + i;
+      const bool on = (int16_t)act[row * KIN + col] > 0;  // bf16 of a ReLU output > 0
+      e[row * KIN + col] = on ? f2bf(acc[i]) : (uint16_t)0;
+    }
+  }
+}
+
+__device__ __forceinline__ void encode_tile(const float *in6, uint32_t row0, uint32_t n,
+                                            uint16_t *a0, int lane) {
+  for (int e = lane; e < 16 * GD_FEAT; e += 64) {
+    const int row = e / GD_FEAT, f = e - row * GD_FEAT;
+    float v = 0.0f;
+    if (row0 + row < n) v = gd_feature(in6 + (size_t)(row0 + row) * GD_IN, f);
+    a0[e] = f2bf(v);
+  }
+}
+
+// Post-processing of one row's raw outputs into lobes (mu normalised,
+// kappa = exp(clamped), softmax weights).
+__device__ __forceinline__ void to_lobes(const float *raw, float *L) {
+  float mx = -1e30f;
+  for (int k = 0; k < GD_LOBES; k++) mx = fmaxf(mx, raw[5 * k + 4]);
+  float se = 0.0f;
+  for (int k = 0; k < GD_LOBES; k++) se += expf(raw[5 * k + 4] - mx);
+  for (int k = 0; k < GD_LOBES; k++) {
+    const float *o = raw + 5 * k;
+    float nn = sqrtf(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+    float ix = nn > 1e-12f ? 1.0f / nn : 0.0f;
+    L[5 * k + 0] = nn > 1e-12f ? o[0] * ix : 0.0f;
+    L[5 * k + 1] = nn > 1e-12f ? o[1] * ix : 0.0f;
+    L[5 * k + 2] = nn > 1e-12f ? o[2] * ix : 1.0f;
+    L[5 * k + 3] = expf(fminf(fmaxf(o[3], GD_KAPPA_LO), GD_KAPPA_HI));
+    L[5 * k + 4] = expf(o[4] - mx) / se;
+  }
+}
+
+// Forward pass over rows.  COUNT (device) overrides N when given.
+__global__ void __launch_bounds__(64 * GD_WAVES)
+k_mlp_forward(GuideNet net, const float *in6, uint32_t n0, const uint32_t *count, float *out,
+              int lobes) {
+  __shared__ TileLds lds[GD_WAVES];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  TileLds &T = lds[wv];
+  const uint32_t n = count ? *count : n0;
+  const uint32_t ntiles = (n + 15) / 16;
+  for (uint32_t t = blockIdx.x * GD_WAVES + wv; t < ntiles; t += gridDim.x * GD_WAVES) {
+    const uint32_t row0 = t * 16;
+    encode_tile(in6, row0, n, T.a0, lane);
+    WAVE_SYNC();
+    mlp_layer<GD_FEAT, GD_H, true, false>(T.a0, net.w16 + GD_W1, net.w32 + GD_B1, T.a1, nullptr, lane);
+    WAVE_SYNC();
+    mlp_layer<GD_H, GD_H, true, false>(T.a1, net.w16 + GD_W2, net.w32 + GD_B2, T.a2, nullptr, lane);
+    WAVE_SYNC();
+    mlp_layer<GD_H, GD_H, true, false>(T.a2, net.w16 + GD_W3, net.w32 + GD_B3, T.a3, nullptr, lane);
+    WAVE_SYNC();
+    mlp_layer<GD_H, GD_OUTP, false, true>(T.a3, net.w16 + GD_W4, net.w32 + GD_B4, nullptr, T.out,
+                                          lane);
+    WAVE_SYNC();
+    if (lane < 16 && row0 + lane < n) {
+      float *o = out + (size_t)(row0 + lane) * GD_OUT;
+      if (lobes) to_lobes(T.out + lane * GD_OUTP, o);
+      else
+        for (int k = 0; k < GD_OUT; k++) o[k] = T.out[lane * GD_OUTP + k];
+    }
+    WAVE_SYNC();
+  }
+}
+
+// d(-log q)/d(raw outputs) of one row (40 values) for incoming
+// direction W; returns log q.
+__device__ __forceinline__ float mix_grad(const float *raw, float wx, float wy, float wz, float *g) {
+  float L[GD_OUT], v[GD_LOBES];
+  to_lobes(raw, L);
+  float q = 0.0f;
+  for (int k = 0; k < GD_LOBES; k++) {
+    const float *o = L + 5 * k;
+    v[k] = vmf_pdf(o[3], o[0] * wx + o[1] * wy + o[2] * wz);
+    q += o[4] * v[k];
+  }
+  q = fmaxf(q, 1e-30f);
+  const float iq = 1.0f / q;
+  for (int k = 0; k < GD_LOBES; k++) {
+    const float *o = L + 5 * k;
+    const float *ro = raw + 5 * k;
+    const float gam = o[4] * v[k] * iq;
+    const float kap = o[3];
+    const float c = o[0] * wx + o[1] * wy + o[2] * wz;
+    // mean direction through the normalisation u -> u / |u|
+    const float nn = sqrtf(ro[0] * ro[0] + ro[1] * ro[1] + ro[2] * ro[2]);
+    const float s = nn > 1e-12f ? gam * kap / nn : 0.0f;
+    g[5 * k + 0] = -s * (wx - c * o[0]);
+    g[5 * k + 1] = -s * (wy - c * o[1]);
+    g[5 * k + 2] = -s * (wz - c * o[2]);
+    // log-concentration (zero where the clamp is active)
+    float dk = 0.0f;
+    if (ro[3] > GD_KAPPA_LO && ro[3] < GD_KAPPA_HI) {
+      const float e2 = expf(-2.0f * kap);
+      dk = gam * kap * (1.0f / kap - 2.0f * e2 / (1.0f - e2) + (c - 1.0f));
+    }
+    g[5 * k + 3] = -dk;
+    // mixture logit
+    g[5 * k + 4] = -(gam - o[4]);
+  }
+  return logf(q);
+}
+
+// Training rows: forward, output gradient (weighted by the record's
+// weight), backward to every layer; activations and gradients stored
+// for the weight-gradient pass.
+__global__ void __launch_bounds__(64 * GD_WAVES)
+k_mlp_rows(GuideNet net, GuideTrain TR, const float *rec, uint32_t n) {
+  __shared__ TileLds lds[GD_WAVES];
+  __shared__ float rowin[GD_WAVES][16 * GD_IN];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  TileLds &T = lds[wv];
+  const uint32_t ntiles = (n + 15) / 16;
+  for (uint32_t t = blockIdx.x * GD_WAVES + wv; t < ntiles; t += gridDim.x * GD_WAVES) {
+    const uint32_t row0 = t * 16;
+    for (int e = lane; e < 16 * GD_IN; e += 64) {
+      const int row = e / GD_IN, f = e - row * GD_IN;
+      rowin[wv][e] = row0 + row < n ? rec[(size_t)(row0 + row) * GD_REC + f] : 0.0f;
+    }
+    WAVE_SYNC();
+    encode_tile(rowin[wv], 0, 16, T.a0, lane);
+    WAVE_SYNC();
+    mlp_layer<GD_FEAT, GD_H, true, false>(T.a0, net.w16 + GD_W1, net.w32 + GD_B1, T.a1, nullptr, lane);
+    WAVE_SYNC();
+    mlp_layer<GD_H, GD_H, true, false>(T.a1, net.w16 + GD_W2, net.w32 + GD_B2, T.a2, nullptr, lane);
+    WAVE_SYNC();
+    mlp_layer<GD_H, GD_H, true, false>(T.a2, net.w16 + GD_W3, net.w32 + GD_B3, T.a3, nullptr, lane);
+    WAVE_SYNC();
+    mlp_layer<GD_H, GD_OUTP, false, true>(T.a3, net.w16 + GD_W4, net.w32 + GD_B4, nullptr, T.out,
+                                          lane);
+    WAVE_SYNC();
+    if (lane < 16) {
+      float g[GD_OUT];
+      float w = 0.0f, lq = 0.0f;
+      const uint32_t row = row0 + lane;
+      if (row < n) {
+        const float *r = rec + (size_t)row * GD_REC;
+        w = r[9] > 0.0f ? r[9] : 0.0f;
+        lq = mix_grad(T.out + lane * GD_OUTP, r[6], r[7], r[8], g);
+      }
+      for (int k = 0; k < GD_OUTP; k++)
+        T.d[lane * GD_OUTP + k] = (k < GD_OUT && w > 0.0f) ? f2bf(w * g[k]) : (uint16_t)0;
+      T.rl[lane] = w > 0.0f ? -w * lq : 0.0f;
+      T.rw[lane] = w;
+    }
+    WAVE_SYNC();
+    if (lane == 0) {
+      float sl = 0.0f, sw = 0.0f;
+      for (int i = 0; i < 16; i++) {
+        sl += T.rl[i];
+        sw += T.rw[i];
+      }
+      TR.tile_loss[2 * t] = sl;
+      TR.tile_loss[2 * t + 1] = sw;
+    }
+    // store activations and the output gradient
+    for (int e = lane; e < 16 * GD_FEAT; e += 64) TR.a0[(size_t)row0 * GD_FEAT + e] = T.a0[e];
+    for (int e = lane; e < 16 * GD_H; e += 64) {
+      TR.a1[(size_t)row0 * GD_H + e] = T.a1[e];
+      TR.a2[(size_t)row0 * GD_H + e] = T.a2[e];
+      TR.a3[(size_t)row0 * GD_H + e] = T.a3[e];
+    }
+    for (int e = lane; e < 16 * GD_OUTP; e += 64) TR.d4[(size_t)row0 * GD_OUTP + e] = T.d[e];
+    WAVE_SYNC();
+    mlp_back<GD_OUTP, GD_H>(T.d, net.w16 + GD_W4, T.a3, T.e, lane);
+    WAVE_SYNC();
+    for (int e = lane; e < 16 * GD_H; e += 64) TR.d3[(size_t)row0 * GD_H + e] = T.e[e];
+    mlp_back<GD_H, GD_H>(T.e, net.w16 + GD_W3, T.a2, T.d, lane);
+    WAVE_SYNC();
+    for (int e = lane; e < 16 * GD_H; e += 64) TR.d2[(size_t)row0 * GD_H + e] = T.d[e];
+    mlp_back<GD_H, GD_H>(T.d, net.w16 + GD_W2, T.a1, T.e, lane);
+    WAVE_SYNC();
+    for (int e = lane; e < 16 * GD_H; e += 64) TR.d1[(size_t)row0 * GD_H + e] = T.e[e];
+    WAVE_SYNC();
+  }
+}
+
+// Weight gradients: dW[n][k] = sum over rows of d[row][n] * a[row][k],
+// one 16x16 tile of one layer per wave over one slice of the rows
+// (MFMA with the rows as the reduction dimension); bias gradients
+// dB[n] = sum over rows of d[row][n].  Partials go to slice-major slots.
+struct WTile {
+  int layer, mt, kt;
+};
+__device__ __forceinline__ WTile wtile(int i) {
+  // layer 1: 4 x 5 tiles, layers 2, 3: 4 x 4, layer 4: 3 x 4
+  if (i < 20) return WTile{1, i / 5, i % 5};
+  i -= 20;
+  if (i < 16) return WTile{2, i / 4, i % 4};
+  i -= 16;
+  if (i < 16) return WTile{3, i / 4, i % 4};
+  i -= 16;
+  return WTile{4, i / 4, i % 4};
+}
+#define GD_WTILES 64
+
+__global__ void __launch_bounds__(64 * GD_WAVES) k_mlp_wgrad(GuideTrain TR, uint32_t n) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t job = blockIdx.x * GD_WAVES + wv;
+  if (job >= GD_WTILES * GD_WG_SLICES) return;
+  const int slice = (int)(job / GD_WTILES);
+  const WTile T = wtile((int)(job % GD_WTILES));
+  const uint16_t *d, *a;
+  int nout, kin, woff, boff;
+  switch (T.layer) {
+    case 1: d = TR.d1; a = TR.a0; nout = GD_H; kin = GD_FEAT; woff = GD_W1; boff = GD_B1; break;
+    case 2: d = TR.d2; a = TR.a1; nout = GD_H; kin = GD_H; woff = GD_W2; boff = GD_B2; break;
+    case 3: d = TR.d3; a = TR.a2; nout = GD_H; kin = GD_H; woff = GD_W3; boff = GD_B3; break;
+    default: d = TR.d4; a = TR.a3; nout = GD_OUTP; kin = GD_H; woff = GD_W4; boff = GD_B4; break;
+  }
+  const uint32_t rows = (n + 15) / 16 * 16;  // whole tiles (padding rows are zero)
+  const uint32_t per = (rows / 16 + GD_WG_SLICES - 1) / GD_WG_SLICES * 16;
+  const uint32_t r0 = slice * per, r1 = r0 + per < rows ? r0 + per : rows;
+  const int r = lane & 15, g = lane >> 4;
+  f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (uint32_t rb = r0; rb < r1; rb += 16) {
+    s4v av, bv;
+    for (int i = 0; i < 4; i++) {
+      const uint32_t row = rb + 4 * g + i;
+      av[i] = (short)d[(size_t)row * nout + T.mt * 16 + r];
+      bv[i] = (short)a[(size_t)row * kin + T.kt * 16 + r];
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+  }
+  float *out = TR.wpart + (size_t)slice * GD_NPARAM;
+  for (int i = 0; i < 4; i++) {
+    const int m = T.mt * 16 + 4 * g + i, k = T.kt * 16 + r;
+    out[woff + m * kin + k] = acc[i];
+  }
+  if (T.kt == 0 && lane < 16) {
+    float s = 0.0f;
+    for (uint32_t row = r0; row < r1; row++) s += bf2f(d[(size_t)row * nout + T.mt * 16 + lane]);
+    out[boff + T.mt * 16 + lane] = s;
+  }
+}
+
+// Loss and weight sums over the tiles, in tile order (one wave).
+__global__ void __launch_bounds__(64) k_mlp_loss(const float *tile_loss, uint32_t ntiles,
+                                                 float *aux) {
+  if (threadIdx.x != 0) return;
+  float sl = 0.0f, sw = 0.0f;
+  for (uint32_t t = 0; t < ntiles; t++) {
+    sl += tile_loss[2 * t];
+    sw += tile_loss[2 * t + 1];
+  }
+  aux[0] = sl;
+  aux[1] = sw;
+}
+
+// grad = (sum of slice partials in slice order) / sum of weights; Adam
+// (beta1 0.9, beta2 0.999, eps 1e-8) when lr > 0.
+__global__ void __launch_bounds__(256) k_mlp_adam(const float *wpart, const float *aux, float *grad,
+                                                  float *w32, uint16_t *w16, float *m, float *v,
+                                                  float lr, uint32_t step) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= GD_NPARAM) return;
+  float s = 0.0f;
+  for (int k = 0; k < GD_WG_SLICES; k++) s += wpart[(size_t)k * GD_NPARAM + p];
+  const float sw = aux[1];
+  const float gr = sw > 0.0f ? s / sw : 0.0f;
+  grad[p] = gr;
+  if (lr > 0.0f) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float mm = b1 * m[p] + (1.0f - b1) * gr;
+    const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
+    m[p] = mm;
+    v[p] = vv;
+    const float mh = mm / (1.0f - powf(b1, (float)step));
+    const float vh = vv / (1.0f - powf(b2, (float)step));
+    const float w = w32[p] - lr * mh / (sqrtf(vh) + 1e-8f);
+    w32[p] = w;
+    w16[p] = f2bf(w);
+  }
+}
+
+// Network inputs of the hit rows of a ray queue: hit position
+// normalised to the scene box and the outgoing direction -d/|d|.
+__global__ void __launch_bounds__(256)
+k_guide_inputs(GuideView G, const uint32_t *queue, const uint32_t *count, RayPtrs R,
+               const float *ht, const uint32_t *hprim) {
+  const uint32_t n = *count;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint32_t p = queue[j];
+    float *o = G.in6 + (size_t)j * GD_IN;
+    if (hprim[p] == 0xffffffffu) {  // miss
+      for (int k = 0; k < GD_IN; k++) o[k] = 0.0f;
+      continue;
+    }
+    const float t = ht[p];
+    const float dx = R.r[3][p], dy = R.r[4][p], dz = R.r[5][p];
+    const float x = R.r[0][p] + dx * t, y = R.r[1][p] + dy * t, z = R.r[2][p] + dz * t;
+    const float il = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    o[0] = (x - G.box_lo[0]) * G.box_inv[0];
+    o[1] = (y - G.box_lo[1]) * G.box_inv[1];
+    o[2] = (z - G.box_lo[2]) * G.box_inv[2];
+    o[3] = -dx * il;
+    o[4] = -dy * il;
+    o[5] = -dz * il;
+  }
+}
+
+// ---------------------------------------------------------------------------
+void launch_guide_inputs(const GuideView &G, const uint32_t *queue, const uint32_t *count,
+                         const float *const ray[8], const float *ht, const uint32_t *hprim,
+                         unsigned grid, hipStream_t st) {
+  RayPtrs R;
+  for (int k = 0; k < 8; k++) R.r[k] = ray[k];
+  hipLaunchKernelGGL(k_guide_inputs, dim3(grid), dim3(256), 0, st, G, queue, count, R, ht, hprim);
+}
+
+void launch_mlp_forward(const GuideNet &net, const float *in6, uint32_t n, const uint32_t *count,
+                        float *out, int lobes, unsigned grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64 * GD_WAVES), 0, st, net, in6, n, count,
+                     out, lobes);
+}
+
+void launch_mlp_rows(const GuideNet &net, const GuideTrain &T, const float *rec, uint32_t n,
+                     hipStream_t st) {
+  const uint32_t tiles = (n + 15) / 16;
+  unsigned grid = (tiles + GD_WAVES - 1) / GD_WAVES;
+  if (grid == 0) grid = 1;
+  hipLaunchKernelGGL(k_mlp_rows, dim3(grid), dim3(64 * GD_WAVES), 0, st, net, T, rec, n);
+}
+
+void launch_mlp_wgrad(const GuideTrain &T, uint32_t n, hipStream_t st) {
+  const unsigned grid = (GD_WTILES * GD_WG_SLICES + GD_WAVES - 1) / GD_WAVES;
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3(grid), dim3(64 * GD_WAVES), 0, st, T, n);
+}
+
+void launch_mlp_adam(const GuideTrain &T, uint32_t n, float *grad, float *aux, float *w32,
+                     uint16_t *w16, float *m, float *v, float lr, uint32_t step, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_loss, dim3(1), dim3(64), 0, st, T.tile_loss, (n + 15) / 16, aux);
+  hipLaunchKernelGGL(k_mlp_adam, dim3((GD_NPARAM + 255) / 256), dim3(256), 0, st, T.wpart, aux,
+                     grad, w32, w16, m, v, lr, step);
+}
+
+}  // namespace sgr

@@ -809,7 +809,7 @@ namespace {
 
 // Runs the chunks; per chunk, optional host copy of the per-sample output.
 int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_host,
-               sgr_stats *stats) {
+               sgr_stats *stats, sgr_guide *guide = nullptr, bool train = false) {
   sgr_ctx *ctx = sc->ctx;
   RenderView R = pl.R;
   size_t cap = (size_t)pl.npix * pl.sb_max;
@@ -818,6 +818,12 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   if (rc) return rc;
   PathBufs P;
   path_bytes(cap, R.K, &P, (char *)ctx->work);
+  GuideView GV;
+  if (guide) {
+    rc = guide_prepare(guide, sc, cap, GV);
+    if (rc) return rc;
+  }
+  const GuideView *G = guide ? &GV : nullptr;
   HIPCHK(hipMemsetAsync(ctx->stats, 0, SGR_NUM_STATS * sizeof(unsigned long long), ctx->stream));
   unsigned grid = (unsigned)std::min<size_t>((cap + 255) / 256, (size_t)ctx->num_cu * 16);
   grid = (grid + 7) & ~7u;  // whole blocks per XCD (see block_range)
@@ -862,6 +868,10 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     R.sb = std::min(pl.sb_max, R.spp - s0);
     R.npaths = pl.npix * R.sb;
     P.n = (uint32_t)cap;
+    if (guide) {
+      rc = guide_begin_chunk(guide, GV);
+      if (rc) return rc;
+    }
     t_begin();
     launch_raygen(false, R, J, P, grid, ctx->stream);
     t_end(K_OTHER);
@@ -869,14 +879,19 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
       t_begin();
       launch_extend(false, sc->view, J, P, b, ctx->stats, grid, ctx->stream);
       t_end(K_EXT);
+      if (guide) {
+        t_begin();
+        guide_infer(guide, GV, P, b, grid);
+        t_end(K_OTHER);
+      }
       t_begin();
-      launch_shade(false, sc->view, R, J, P, b, ctx->stats, grid, ctx->stream);
+      launch_shade(false, sc->view, R, J, P, b, ctx->stats, grid, ctx->stream, G);
       t_end(K_SHADE);
       t_begin();
       launch_occlude(false, sc->view, J, P, b, ctx->stats, grid, ctx->stream);
       t_end(K_OCC);
       t_begin();
-      launch_gather(false, R, J, P, b, grid, ctx->stream);
+      launch_gather(false, R, J, P, b, grid, ctx->stream, G);
       t_end(K_OTHER);
       n_ext++;
       n_occ++;
@@ -888,6 +903,13 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     }
     launch_count_queues(R.maxp, P, ctx->stats, ctx->stream);
     HIPCHK(hipGetLastError());
+    if (guide && train) {
+      uint32_t nrec = (R.npaths + GV.rec_stride - 1) / GV.rec_stride;
+      t_begin();
+      rc = guide_train_chunk(guide, GV, nrec);
+      t_end(K_OTHER);
+      if (rc) return rc;
+    }
     if (samples_host) {
       chunk_out.resize((size_t)R.npaths * 6);
       HIPCHK(hipMemcpyAsync(chunk_out.data(), P.out, chunk_out.size() * 4, hipMemcpyDeviceToHost,
@@ -901,6 +923,10 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     }
   }
   HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+  if (guide) {
+    rc = guide_finish(guide);
+    if (rc) return rc;
+  }
   unsigned long long st[SGR_NUM_STATS];
   HIPCHK(hipMemcpyAsync(st, ctx->stats, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -936,12 +962,10 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   return SGR_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int sgr_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p, float *accum,
-               int accum_on_device, sgr_stats *stats) {
+// sgr_render / sgr_render_guided: host accumulators go through a
+// temporary device copy.
+int render_accum(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p, float *accum,
+                 int accum_on_device, sgr_stats *stats, sgr_guide *guide, bool train) {
   RenderPlan pl;
   int rc = plan_render(sc, cam, p, pl);
   if (rc) return rc;
@@ -958,15 +982,32 @@ int sgr_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p,
       return fail(SGR_ERR_HIP, std::string("sgr_render: ") + hipGetErrorString(e));
     }
   }
-  rc = run_render(sc, pl, dacc, nullptr, stats);
+  rc = run_render(sc, pl, dacc, nullptr, stats, guide, train);
   if (!accum_on_device) {
     if (!rc) {
-      hipError_t e = hipMemcpy(accum, dacc, abytes, hipMemcpyDeviceToHost);
+      hipError_t e = hipMemcpyAsync(accum, dacc, abytes, hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
       if (e != hipSuccess) rc = fail(SGR_ERR_HIP, std::string("sgr_render: ") + hipGetErrorString(e));
     }
     (void)hipFree(dacc);
   }
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sgr_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p, float *accum,
+               int accum_on_device, sgr_stats *stats) {
+  return render_accum(sc, cam, p, accum, accum_on_device, stats, nullptr, false);
+}
+
+int sgr_render_guided(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p,
+                      sgr_guide *guide, int train, float *accum, int accum_on_device,
+                      sgr_stats *stats) {
+  if (!guide) return fail(SGR_ERR_INVALID, "sgr_render_guided: null guide");
+  return render_accum(sc, cam, p, accum, accum_on_device, stats, guide, train != 0);
 }
 
 int sgr_render_samples(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p,

@@ -65,4 +65,13 @@ size_t path_bytes(size_t cap, size_t K, PathBufs *P, char *base);
 int ensure_work(sgr_ctx *ctx, size_t bytes);
 void engine_destroy(sgr_ctx *ctx);
 
+// Learned proposal hooks of a render (sgr_guide.cpp): buffers for CAP
+// paths, clearing a chunk's training records, inference before shading
+// bounce B, training after a chunk, final statistics.
+int guide_prepare(sgr_guide *g, sgr_scene *sc, size_t cap, GuideView &V);
+int guide_begin_chunk(sgr_guide *g, const GuideView &V);
+void guide_infer(sgr_guide *g, const GuideView &V, const PathBufs &P, uint32_t b, unsigned grid);
+int guide_train_chunk(sgr_guide *g, const GuideView &V, uint32_t nrec);
+int guide_finish(sgr_guide *g);
+
 }  // namespace sgr

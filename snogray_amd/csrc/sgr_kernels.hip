@@ -17,8 +17,11 @@
 //              writes finished paths
 // then k_splat gathers the chunk's samples into the image accumulators
 // with the Mitchell filter (image/image-filter-conv.h:109-186).
+#include <cstring>
+
 #include "dev_shade.h"
 #include "kernels.h"
+#include "dev_guide.h"
 
 namespace sgr {
 
@@ -201,9 +204,10 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathB
 }
 
 // One bounce of PathInteg::Li for every path in the ray queue.
-template<bool MULTI>
+template<bool MULTI, bool GUIDED>
 __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, StepView J, PathBufs P,
-                                               uint32_t b, unsigned long long *stats) {
+                                               uint32_t b, unsigned long long *stats,
+                                               GuideView G) {
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   const uint32_t *q = P.q[b & 1];
   uint32_t *qn = P.q[(b + 1) & 1];
@@ -230,6 +234,15 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, Step
     uint32_t prim = P.hprim[p];
     bool done = false, cont = false;
     uint32_t nst = st & ~(SGR_ST_PENDING | SGR_ST_DONE | SGR_ST_STRAT);
+    float *grec = nullptr;  // this path's training record (learned proposal)
+    if (GUIDED && p % G.rec_stride == 0) {
+      grec = G.rec + (size_t)(p / G.rec_stride) * GD_REC;
+      if (grec[9] == -1.0f) {
+        // radiance gathered before the recorded continuation's first hit
+        grec[13] = rad.r; grec[14] = rad.g; grec[15] = rad.b;
+        grec[9] = -2.0f;
+      }
+    }
     if (prim == SGR_MISS) {
       if (include) rad = rad + background(S, ray.d) * T;
       if (path_len == 0 && czero(rad)) alpha = R.bg_alpha;
@@ -332,7 +345,36 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, Step
           cu = draw(qk, (path_len << 20) | 1u);
           cv = draw(qk, (path_len << 20) | 2u);
         }
-        BSample bs = bsdf_sample(is, M, cu, cv, F_ALL);
+        BSample bs;
+        // Learned proposal: one-sample MIS of the BSDF (probability
+        // G.mix) and the vMF mixture at this vertex; delta BSDFs (glass,
+        // mirror) bypass it.
+        const bool guided = GUIDED && (M.type == 1 || M.type == 2);  // Lambert, Cook-Torrance
+        if (guided) {
+          const float *L = G.lobes + (size_t)j * GD_OUT;
+          const float usel = draw(qk, (path_len << 20) | 3u);
+          if (usel < G.mix) {
+            bs = bsdf_sample(is, M, cu, cv, F_ALL);
+            if (bs.pdf > 0.0f) {
+              const V3 wd = is.nf.from(bs.dir);
+              bs.pdf = G.mix * bs.pdf + (1.0f - G.mix) * mix_pdf(L, wd.x, wd.y, wd.z);
+            }
+          } else {
+            V3 wd;
+            mix_sample(L, draw(qk, (path_len << 20) | 4u), draw(qk, (path_len << 20) | 5u),
+                       draw(qk, (path_len << 20) | 6u), wd.x, wd.y, wd.z);
+            const V3 ld = is.nf.to(wd);
+            const BValue bv = bsdf_eval(is, M, ld, F_ALL);
+            bs.val = bv.val;
+            bs.dir = ld;
+            bs.flags = F_REFLECTIVE | F_DIFFUSE;
+            bs.pdf = czero(bv.val) ? 0.0f
+                                   : G.mix * bv.pdf + (1.0f - G.mix) * mix_pdf(L, wd.x, wd.y, wd.z);
+          }
+        } else {
+          bs = bsdf_sample(is, M, cu, cv, F_ALL);
+        }
+        const C3 T_vertex = T;
         if (bs.pdf == 0.0f || czero(bs.val)) {
           done = true;
         } else {
@@ -372,6 +414,13 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, Step
             nst = (nst & (SGR_ST_PENDING | SGR_ST_STRAT)) | (path_len + 1) |
                   ((spec ? 1u : 0u) << SGR_ST_SPEC_SHIFT) | (depth << SGR_ST_DEPTH_SHIFT);
             cont = true;
+            if (guided && grec && path_len == (p / G.rec_stride) % 3u) {
+              const float *in = G.in6 + (size_t)j * GD_IN;
+              for (int k = 0; k < GD_IN; k++) grec[k] = in[k];
+              grec[6] = nr.d.x; grec[7] = nr.d.y; grec[8] = nr.d.z;
+              grec[10] = T_vertex.r; grec[11] = T_vertex.g; grec[12] = T_vertex.b;
+              grec[9] = -1.0f;
+            }
           }
         }
       }
@@ -391,8 +440,9 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, Step
 // Adds the unoccluded direct terms (render/direct-illum.cc:96-113 sums,
 // render/path-integ.cc:277-306 product with the path transmittance) and
 // writes finished paths: sx, sy, Tint(radiance, alpha).
-template<bool MULTI>
-__global__ void __launch_bounds__(256) k_gather(RenderView R1, StepView J, PathBufs P, uint32_t b) {
+template<bool MULTI, bool GUIDED>
+__global__ void __launch_bounds__(256) k_gather(RenderView R1, StepView J, PathBufs P, uint32_t b,
+                                                GuideView G) {
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   const uint32_t *q = P.q[b & 1];
   uint32_t jb, je;
@@ -424,6 +474,21 @@ __global__ void __launch_bounds__(256) k_gather(RenderView R1, StepView J, PathB
       P.state[p] = st;
     }
     if (st & SGR_ST_DONE) {
+      if (GUIDED && p % G.rec_stride == 0) {
+        // training weight: radiance the recorded continuation brought,
+        // over the throughput at its vertex (= f cos L / pdf, averaged)
+        float *grec = G.rec + (size_t)(p / G.rec_stride) * GD_REC;
+        if (grec[9] == -2.0f) {
+          const float d0 = rad.r - grec[13], d1 = rad.g - grec[14], d2 = rad.b - grec[15];
+          float w = 0.0f;
+          w += grec[10] > 0.0f ? d0 / grec[10] : 0.0f;
+          w += grec[11] > 0.0f ? d1 / grec[11] : 0.0f;
+          w += grec[12] > 0.0f ? d2 / grec[12] : 0.0f;
+          grec[9] = fmaxf(w * (1.0f / 3.0f), 0.0f);
+        } else if (grec[9] < 0.0f) {
+          grec[9] = 0.0f;
+        }
+      }
       float a = P.alpha[p];
       float *o = P.out + 6 * (size_t)p;
       o[2] = rad.r * a;
@@ -594,11 +659,19 @@ void launch_extend(bool multi, const SceneView &S, const StepView &J, const Path
 }
 void launch_shade(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
                   const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
-                  hipStream_t st) {
+                  hipStream_t st, const GuideView *G) {
+  GuideView g0;
+  if (!G) {
+    std::memset(&g0, 0, sizeof g0);
+    g0.rec_stride = 1;
+  }
+  const GuideView &g = G ? *G : g0;
   if (multi)
-    hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats);
+    hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats, g);
+  else if (G)
+    hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats, g);
   else
-    hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats);
+    hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats, g);
 }
 void launch_occlude(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                     uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
@@ -606,9 +679,19 @@ void launch_occlude(bool multi, const SceneView &S, const StepView &J, const Pat
   else hipLaunchKernelGGL(k_occlude<false>, dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
 }
 void launch_gather(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
-                   uint32_t b, unsigned grid, hipStream_t st) {
-  if (multi) hipLaunchKernelGGL(k_gather<true>, dim3(grid), dim3(256), 0, st, R, J, P, b);
-  else hipLaunchKernelGGL(k_gather<false>, dim3(grid), dim3(256), 0, st, R, J, P, b);
+                   uint32_t b, unsigned grid, hipStream_t st, const GuideView *G) {
+  GuideView g0;
+  if (!G) {
+    std::memset(&g0, 0, sizeof g0);
+    g0.rec_stride = 1;
+  }
+  const GuideView &g = G ? *G : g0;
+  if (multi)
+    hipLaunchKernelGGL((k_gather<true, false>), dim3(grid), dim3(256), 0, st, R, J, P, b, g);
+  else if (G)
+    hipLaunchKernelGGL((k_gather<false, true>), dim3(grid), dim3(256), 0, st, R, J, P, b, g);
+  else
+    hipLaunchKernelGGL((k_gather<false, false>), dim3(grid), dim3(256), 0, st, R, J, P, b, g);
 }
 void launch_splat(bool multi, const RenderView &R, const StepView &J, const float *out,
                   unsigned grid, hipStream_t st) {

@@ -24,6 +24,8 @@ EXPORTS = [
     "sgr_engine_configure", "sgr_job_footprint", "sgr_job_submit", "sgr_engine_step",
     "sgr_job_poll", "sgr_job_result", "sgr_job_release", "sgr_engine_stats",
     "sgr_render_batch",
+    "sgr_guide_create", "sgr_guide_destroy", "sgr_guide_get_params", "sgr_guide_set_params",
+    "sgr_guide_forward", "sgr_guide_train_batch", "sgr_guide_stats_get", "sgr_render_guided",
 ]
 
 _lib = None
@@ -81,6 +83,15 @@ def lib():
     L.sgr_render_batch.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(abi.Camera),
                                    C.POINTER(abi.RenderParams), C.POINTER(abi.f32p),
                                    C.c_int, C.POINTER(abi.Stats)]
+    L.sgr_guide_create.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+    L.sgr_guide_destroy.argtypes = [vp]
+    L.sgr_guide_get_params.argtypes = [vp, abi.f32p, C.POINTER(C.c_uint64)]
+    L.sgr_guide_set_params.argtypes = [vp, abi.f32p, C.c_uint64]
+    L.sgr_guide_forward.argtypes = [vp, abi.f32p, C.c_uint64, abi.f32p, C.c_int]
+    L.sgr_guide_train_batch.argtypes = [vp, abi.f32p, C.c_uint64, C.c_float, abi.f32p, abi.f32p]
+    L.sgr_guide_stats_get.argtypes = [vp, C.POINTER(abi.GuideStats)]
+    L.sgr_render_guided.argtypes = [vp, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), vp,
+                                    C.c_int, abi.f32p, C.c_int, C.POINTER(abi.Stats)]
     _lib = L
     return L
 
@@ -111,12 +122,15 @@ class Context:
         self.h = h
         self.scenes = weakref.WeakSet()
         self.engines = weakref.WeakSet()
+        self.guides = weakref.WeakSet()
         _live_contexts.add(self)
 
     def close(self):
         if getattr(self, "h", None):
             for eng in list(self.engines):
                 eng.release_all()
+            for g in list(self.guides):
+                g.close()
             for sc in list(self.scenes):
                 sc.close()
             check(lib().sgr_ctx_destroy(self.h), "sgr_ctx_destroy")
@@ -212,6 +226,23 @@ class Scene:
               "sgr_render")
         return st.as_dict()
 
+    def render_guided(self, cam, params, guide, train=True, accum=None):
+        """sgr_render_guided into host accumulators; returns (accum, stats)."""
+        if accum is None:
+            accum = np.zeros((params.height, params.width, 5), np.float32)
+        st = abi.Stats()
+        check(lib().sgr_render_guided(self.h, C.byref(cam), C.byref(params), guide.h,
+                                      int(train), abi.fptr(accum), 0, C.byref(st)),
+              "sgr_render_guided")
+        return accum, st.as_dict()
+
+    def render_guided_device(self, cam, params, guide, accum_ptr, train=True):
+        st = abi.Stats()
+        check(lib().sgr_render_guided(self.h, C.byref(cam), C.byref(params), guide.h,
+                                      int(train), C.cast(C.c_void_p(accum_ptr), abi.f32p), 1,
+                                      C.byref(st)), "sgr_render_guided")
+        return st.as_dict()
+
     def render_samples(self, cam, params):
         x0, y0, w, h, m = abi.render_window(params)
         out = np.zeros(((h + 2 * m) * (w + 2 * m) * params.spp, 6), np.float32)
@@ -294,6 +325,56 @@ class Engine:
             if si["jobs"] == 0:
                 return steps
             steps.append(si)
+
+
+class Guide:
+    """The learned directional proposal (sgr_guide_*)."""
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().sgr_guide_create(ctx.h, seed, C.byref(h)), "sgr_guide_create")
+        self.h = h
+        ctx.guides.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sgr_guide_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def params(self):
+        out = np.zeros(abi.GD_NPARAM, np.float32)
+        n = C.c_uint64()
+        check(lib().sgr_guide_get_params(self.h, abi.fptr(out), C.byref(n)), "sgr_guide_get_params")
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        check(lib().sgr_guide_set_params(self.h, abi.fptr(w), len(w)), "sgr_guide_set_params")
+
+    def forward(self, in6, lobes=False):
+        x = np.ascontiguousarray(in6, np.float32).reshape(-1, 6)
+        out = np.zeros((len(x), 40), np.float32)
+        check(lib().sgr_guide_forward(self.h, abi.fptr(x), len(x), abi.fptr(out), int(lobes)),
+              "sgr_guide_forward")
+        return out
+
+    def train_batch(self, rec, lr=0.0):
+        """rec: (n, 16) records.  Returns (gradient, (loss sum, weight sum))."""
+        r = np.ascontiguousarray(rec, np.float32).reshape(-1, abi.GD_REC)
+        g = np.zeros(abi.GD_NPARAM, np.float32)
+        loss = np.zeros(2, np.float32)
+        check(lib().sgr_guide_train_batch(self.h, abi.fptr(r), len(r), lr, abi.fptr(g),
+                                          abi.fptr(loss)), "sgr_guide_train_batch")
+        return g, (float(loss[0]), float(loss[1]))
+
+    def stats(self):
+        st = abi.GuideStats()
+        check(lib().sgr_guide_stats_get(self.h, C.byref(st)), "sgr_guide_stats_get")
+        return st.as_dict()
 
 
 def render_batch(ctx, jobs):

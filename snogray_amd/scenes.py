@@ -243,7 +243,7 @@ def _box_mesh(center_xz, size, height, angle_deg):
     return v, np.array(tris, np.uint32)
 
 
-def cornell_box(light_emission=(17.0, 12.0, 4.0)):
+def cornell_box(light_emission=(17.0, 12.0, 4.0), aspect=1.0):
     """Cornell box, 5.5 m, y up; returns (SceneDesc, camera)."""
     S = 5.5
     sd = SceneDesc()
@@ -275,8 +275,8 @@ def cornell_box(light_emission=(17.0, 12.0, 4.0)):
     lx, lz = 1.3, 1.05
     sd.tripar((S / 2 - lx / 2, S - 0.01, S / 2 - lz / 2), (0.0, 0.0, lz),
               (lx, 0.0, 0.0), light, parallelogram=True)
-    cam = camera((S / 2, S / 2, -8.0), (0, 0, 1), (0, 1, 0), (1, 0, 0),
-                 math.tan(math.radians(39.3) / 2), math.tan(math.radians(39.3) / 2))
+    t = math.tan(math.radians(39.3) / 2)
+    cam = camera((S / 2, S / 2, -8.0), (0, 0, 1), (0, 1, 0), (1, 0, 0), t * aspect, t)
     return sd, cam
 
 
