@@ -30,6 +30,19 @@ __device__ __forceinline__ float gd_feature(const float *u, int f) {
   return (r & 1) ? cosf(x) : sinf(x);
 }
 
+// The same feature with the hardware sine / cosine (argument in
+// revolutions: 2^k pi u = 2^(k-1) u turns, |.| <= 32 turns for |u| <= 1);
+// differs from gd_feature by about 1e-6, far below the bf16 rounding
+// the features then get.
+__device__ __forceinline__ float gd_feature_fast(const float *u, int f) {
+  if (f < GD_IN) return u[f];
+  if (f >= GD_IN + GD_IN * GD_NFREQ * 2) return 0.0f;
+  const int e = f - GD_IN, i = e / (GD_NFREQ * 2), r = e - i * (GD_NFREQ * 2);
+  const int k = r >> 1;
+  const float turns = u[i] * (float)(1 << k) * 0.5f;
+  return (r & 1) ? __builtin_amdgcn_cosf(turns) : __builtin_amdgcn_sinf(turns);
+}
+
 // vMF density on the unit sphere with mean MU and concentration KAPPA
 // at cosine C = mu . w: kappa / (2 pi (1 - e^{-2 kappa})) e^{kappa (c - 1)}.
 __device__ __forceinline__ float vmf_pdf(float kappa, float c) {

@@ -56,7 +56,8 @@ struct GuideView {
   float *lobes;              // per queue row: 8 x (mu xyz, kappa, weight)
   float *rec;                // training records, GD_REC floats each
   uint32_t rec_stride;       // paths per training record
-  uint32_t rec_count;
+  uint32_t rec_count;        // records of the current chunk
+  uint64_t rec_key;          // shuffles record slots (keyed permutation)
   float box_lo[3], box_inv[3];
   float mix;                 // probability of sampling the BSDF (0.5)
 };

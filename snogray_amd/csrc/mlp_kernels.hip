@@ -115,33 +115,86 @@ __device__ __forceinline__ void to_lobes(const float *raw, float *L) {
   }
 }
 
-// Forward pass over rows.  COUNT (device) overrides N when given.
+// Forward pass over rows, transposed form: out^T = W . act^T, with the
+// weights as the MFMA A operand (lane l: W[m0 + l%16][k0 + 4(l/16) + i])
+// and the activations as the B operand (lane l: act[row l%16][k0 +
+// 4(l/16) + i]).  The D fragment of output tile m (lane l: out[row
+// l%16][m0 + 4(l/16) + i]) is then exactly the B fragment of the next
+// layer's k-tile m, so activations stay in registers between layers;
+// each weight fragment is loaded once per FT row tiles.  COUNT (device)
+// overrides N when given.
+#define FT 4  // 16-row tiles per wave and pass
+
+template<int KT, int MT, bool RELU>
+__device__ __forceinline__ void layer_t(const uint16_t *W, const float *bias, const s4v (&in)[FT][KT],
+                                        f4v (&out)[FT][MT], int lane) {
+  const int r = lane & 15, g = lane >> 4;
+  for (int mt = 0; mt < MT; mt++) {
+    f4v acc[FT];
+    for (int t = 0; t < FT; t++) acc[t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kt = 0; kt < KT; kt++) {
+      const s4v a = *(const s4v *)(W + (size_t)(mt * 16 + r) * (KT * 16) + kt * 16 + 4 * g);
+      for (int t = 0; t < FT; t++)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, in[t][kt], acc[t], 0, 0, 0);
+    }
+    for (int t = 0; t < FT; t++)
+      for (int i = 0; i < 4; i++) {
+        float z = acc[t][i] + bias[mt * 16 + 4 * g + i];
+        out[t][mt][i] = RELU ? fmaxf(z, 0.0f) : z;
+      }
+  }
+}
+
+template<int MT>
+__device__ __forceinline__ void to_bf16_frag(const f4v (&x)[FT][MT], s4v (&y)[FT][MT]) {
+  for (int t = 0; t < FT; t++)
+    for (int m = 0; m < MT; m++)
+      for (int i = 0; i < 4; i++) y[t][m][i] = (short)f2bf(x[t][m][i]);
+}
+
 __global__ void __launch_bounds__(64 * GD_WAVES)
 k_mlp_forward(GuideNet net, const float *in6, uint32_t n0, const uint32_t *count, float *out,
               int lobes) {
-  __shared__ TileLds lds[GD_WAVES];
+  __shared__ float obuf[GD_WAVES][FT * 16 * GD_OUTP];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  TileLds &T = lds[wv];
+  const int r = lane & 15, g = lane >> 4;
   const uint32_t n = count ? *count : n0;
-  const uint32_t ntiles = (n + 15) / 16;
-  for (uint32_t t = blockIdx.x * GD_WAVES + wv; t < ntiles; t += gridDim.x * GD_WAVES) {
-    const uint32_t row0 = t * 16;
-    encode_tile(in6, row0, n, T.a0, lane);
+  const uint32_t ngroups = (n + 16 * FT - 1) / (16 * FT);
+  float *ob = obuf[wv];
+  for (uint32_t grp = blockIdx.x * GD_WAVES + wv; grp < ngroups; grp += gridDim.x * GD_WAVES) {
+    const uint32_t row0 = grp * 16 * FT;
+    // encoded features straight into B fragments: this lane's row r of
+    // each tile, features kt*16 + 4g + i
+    s4v x0[FT][GD_FEAT / 16];
+    for (int t = 0; t < FT; t++) {
+      const uint32_t row = row0 + t * 16 + r;
+      float u[GD_IN];
+      for (int k = 0; k < GD_IN; k++) u[k] = row < n ? in6[(size_t)row * GD_IN + k] : 0.0f;
+      for (int kt = 0; kt < GD_FEAT / 16; kt++)
+        for (int i = 0; i < 4; i++)
+          x0[t][kt][i] = (short)f2bf(row < n ? gd_feature_fast(u, kt * 16 + 4 * g + i) : 0.0f);
+    }
+    f4v h[FT][GD_H / 16];
+    s4v hb[FT][GD_H / 16];
+    layer_t<GD_FEAT / 16, GD_H / 16, true>(net.w16 + GD_W1, net.w32 + GD_B1, x0, h, lane);
+    to_bf16_frag(h, hb);
+    layer_t<GD_H / 16, GD_H / 16, true>(net.w16 + GD_W2, net.w32 + GD_B2, hb, h, lane);
+    to_bf16_frag(h, hb);
+    layer_t<GD_H / 16, GD_H / 16, true>(net.w16 + GD_W3, net.w32 + GD_B3, hb, h, lane);
+    to_bf16_frag(h, hb);
+    f4v o[FT][GD_OUTP / 16];
+    layer_t<GD_H / 16, GD_OUTP / 16, false>(net.w16 + GD_W4, net.w32 + GD_B4, hb, o, lane);
+    // rows gather their 40 outputs through LDS
+    for (int t = 0; t < FT; t++)
+      for (int m = 0; m < GD_OUTP / 16; m++)
+        for (int i = 0; i < 4; i++) ob[(t * 16 + r) * GD_OUTP + m * 16 + 4 * g + i] = o[t][m][i];
     WAVE_SYNC();
-    mlp_layer<GD_FEAT, GD_H, true, false>(T.a0, net.w16 + GD_W1, net.w32 + GD_B1, T.a1, nullptr, lane);
-    WAVE_SYNC();
-    mlp_layer<GD_H, GD_H, true, false>(T.a1, net.w16 + GD_W2, net.w32 + GD_B2, T.a2, nullptr, lane);
-    WAVE_SYNC();
-    mlp_layer<GD_H, GD_H, true, false>(T.a2, net.w16 + GD_W3, net.w32 + GD_B3, T.a3, nullptr, lane);
-    WAVE_SYNC();
-    mlp_layer<GD_H, GD_OUTP, false, true>(T.a3, net.w16 + GD_W4, net.w32 + GD_B4, nullptr, T.out,
-                                          lane);
-    WAVE_SYNC();
-    if (lane < 16 && row0 + lane < n) {
-      float *o = out + (size_t)(row0 + lane) * GD_OUT;
-      if (lobes) to_lobes(T.out + lane * GD_OUTP, o);
+    const uint32_t row = row0 + lane;
+    if (row < n) {
+      float *dst = out + (size_t)row * GD_OUT;
+      if (lobes) to_lobes(ob + lane * GD_OUTP, dst);
       else
-        for (int k = 0; k < GD_OUT; k++) o[k] = T.out[lane * GD_OUTP + k];
+        for (int k = 0; k < GD_OUT; k++) dst[k] = ob[lane * GD_OUTP + k];
     }
     WAVE_SYNC();
   }

@@ -869,7 +869,7 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     R.npaths = pl.npix * R.sb;
     P.n = (uint32_t)cap;
     if (guide) {
-      rc = guide_begin_chunk(guide, GV);
+      rc = guide_begin_chunk(guide, GV, R.npaths, R.seed * 0x9e3779b97f4a7c15ULL + s0 + 1);
       if (rc) return rc;
     }
     t_begin();
@@ -904,9 +904,8 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     launch_count_queues(R.maxp, P, ctx->stats, ctx->stream);
     HIPCHK(hipGetLastError());
     if (guide && train) {
-      uint32_t nrec = (R.npaths + GV.rec_stride - 1) / GV.rec_stride;
       t_begin();
-      rc = guide_train_chunk(guide, GV, nrec);
+      rc = guide_train_chunk(guide, GV, GV.rec_count);
       t_end(K_OTHER);
       if (rc) return rc;
     }

@@ -149,8 +149,11 @@ int guide_prepare(sgr_guide *g, sgr_scene *sc, size_t cap, GuideView &V) {
   return SGR_OK;
 }
 
-// Clears the chunk's training records (before raygen).
-int guide_begin_chunk(sgr_guide *g, const GuideView &V) {
+// Record count and slot key of a chunk of NPATHS paths; clears its
+// records (before raygen).
+int guide_begin_chunk(sgr_guide *g, GuideView &V, uint32_t npaths, uint64_t key) {
+  V.rec_count = (npaths + V.rec_stride - 1) / V.rec_stride;
+  V.rec_key = key;
   HIPCHK(hipMemsetAsync(V.rec, 0, (size_t)V.rec_count * GD_REC * 4, g->ctx->stream));
   return SGR_OK;
 }

@@ -69,7 +69,7 @@ void engine_destroy(sgr_ctx *ctx);
 // paths, clearing a chunk's training records, inference before shading
 // bounce B, training after a chunk, final statistics.
 int guide_prepare(sgr_guide *g, sgr_scene *sc, size_t cap, GuideView &V);
-int guide_begin_chunk(sgr_guide *g, const GuideView &V);
+int guide_begin_chunk(sgr_guide *g, GuideView &V, uint32_t npaths, uint64_t key);
 void guide_infer(sgr_guide *g, const GuideView &V, const PathBufs &P, uint32_t b, unsigned grid);
 int guide_train_chunk(sgr_guide *g, const GuideView &V, uint32_t nrec);
 int guide_finish(sgr_guide *g);

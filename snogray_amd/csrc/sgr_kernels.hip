@@ -236,7 +236,8 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, Step
     uint32_t nst = st & ~(SGR_ST_PENDING | SGR_ST_DONE | SGR_ST_STRAT);
     float *grec = nullptr;  // this path's training record (learned proposal)
     if (GUIDED && p % G.rec_stride == 0) {
-      grec = G.rec + (size_t)(p / G.rec_stride) * GD_REC;
+      // slots are shuffled so that every training minibatch spans the image
+      grec = G.rec + (size_t)permute(p / G.rec_stride, G.rec_count, G.rec_key) * GD_REC;
       if (grec[9] == -1.0f) {
         // radiance gathered before the recorded continuation's first hit
         grec[13] = rad.r; grec[14] = rad.g; grec[15] = rad.b;
@@ -477,7 +478,7 @@ __global__ void __launch_bounds__(256) k_gather(RenderView R1, StepView J, PathB
       if (GUIDED && p % G.rec_stride == 0) {
         // training weight: radiance the recorded continuation brought,
         // over the throughput at its vertex (= f cos L / pdf, averaged)
-        float *grec = G.rec + (size_t)(p / G.rec_stride) * GD_REC;
+        float *grec = G.rec + (size_t)permute(p / G.rec_stride, G.rec_count, G.rec_key) * GD_REC;
         if (grec[9] == -2.0f) {
           const float d0 = rad.r - grec[13], d1 = rad.g - grec[14], d2 = rad.b - grec[15];
           float w = 0.0f;
