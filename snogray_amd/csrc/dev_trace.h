@@ -118,9 +118,20 @@ __device__ __forceinline__ uint32_t visit_mask(uint32_t mask, uint32_t oct) {
 // hits wins exactly as on the CPU.  Node-call counts include the pruned
 // calls, as the reference counts them.
 // ANY: stop at the first hit (space/space.cc:103-116 / :160-180).
+// Traversal stack: the two newest entries of each lane live in LDS
+// (slot i of the lane at lds[(2 * i + half) * SGR_TRACE_BLOCK]); older ones are
+// spilled to the lane's private array when a third is pushed and read
+// back from there when popped.  Entries are 2 x float4: (first | child
+// cursor << 28, child mask | pending siblings << 8, x slab) and (y slab,
+// z slab).
+#define SGR_TRACE_BLOCK 256
+struct LdsStack {
+  float4 *base;  // &shared[threadIdx.x] of a [4][SGR_TRACE_BLOCK] float4 array
+};
+
 template<bool ANY>
 __device__ uint32_t trace_ray(const SceneView &S, RayF &ray, float &hu, float &hv,
-                              TraceCount &cnt) {
+                              TraceCount &cnt, LdsStack L) {
   uint32_t hit = SGR_MISS;
   float t, u, v;
   uint32_t prim;
@@ -158,7 +169,7 @@ __device__ uint32_t trace_ray(const SceneView &S, RayF &ray, float &hu, float &h
   cnt.nodes++;
   if (fmx(ray.t0, fmx(bx0, fmx(by0, bz0))) >= fmn(ray.t1, fmn(bx1, fmn(by1, bz1)))) return hit;
   float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
-  int sp = 0;
+  int sp = 0, nl = 0;  // stack depth, entries resident in LDS (top nl)
   uint32_t cn = 0;
   for (;;) {
     // Enter node cn: its surfaces, then set up the child walk.
@@ -224,9 +235,16 @@ __device__ uint32_t trace_ray(const SceneView &S, RayF &ray, float &hu, float &h
           later |= 1u << j;
         }
         if (later) {  // siblings left: keep this node's walk state
-          st0[sp] = make_float4(__uint_as_float(first | (ci << 28)),
+          float4 *slot = L.base + (size_t)(2 * (sp & 1)) * SGR_TRACE_BLOCK;
+          if (nl == 2) {  // the slot holds entry sp - 2: spill it first
+            st0[sp - 2] = slot[0];
+            st1[sp - 2] = slot[SGR_TRACE_BLOCK];
+          } else {
+            nl++;
+          }
+          slot[0] = make_float4(__uint_as_float(first | (ci << 28)),
                                 __uint_as_float(mask | (later << 8)), bx0, bx1);
-          st1[sp] = make_float4(by0, by1, bz0, bz1);
+          slot[SGR_TRACE_BLOCK] = make_float4(by0, by1, bz0, bz1);
           sp++;
         }
         cn = first + __popc(mask & ((1u << real) - 1u));
@@ -239,7 +257,16 @@ __device__ uint32_t trace_ray(const SceneView &S, RayF &ray, float &hu, float &h
       // This node is done: resume the nearest pending ancestor.
       if (sp == 0) break;
       sp--;
-      float4 a = st0[sp], b = st1[sp];
+      float4 a, b;
+      if (nl > 0) {
+        const float4 *slot = L.base + (size_t)(2 * (sp & 1)) * SGR_TRACE_BLOCK;
+        a = slot[0];
+        b = slot[SGR_TRACE_BLOCK];
+        nl--;
+      } else {
+        a = st0[sp];
+        b = st1[sp];
+      }
       uint32_t w0 = __float_as_uint(a.x), w1 = __float_as_uint(a.y);
       first = w0 & 0x0fffffffu;
       ci = w0 >> 28;

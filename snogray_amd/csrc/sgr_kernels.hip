@@ -30,6 +30,8 @@ namespace sgr {
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_trace_closest(SceneView S, RayBatchDev rb, uint32_t n,
                                                        unsigned long long *stats) {
+  __shared__ float4 stk_lds[4 * SGR_TRACE_BLOCK];
+  const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     RayF r;
@@ -38,7 +40,7 @@ __global__ void __launch_bounds__(256) k_trace_closest(SceneView S, RayBatchDev 
     r.t0 = rb.t0[i];
     r.t1 = rb.t1[i];
     float u = 0.0f, v = 0.0f;
-    uint32_t p = trace_ray<false>(S, r, u, v, cnt);
+    uint32_t p = trace_ray<false>(S, r, u, v, cnt, stk);
     rb.prim[i] = p;
     rb.t[i] = p == SGR_MISS ? 0.0f : r.t1;
     rb.u[i] = u;
@@ -50,6 +52,8 @@ __global__ void __launch_bounds__(256) k_trace_closest(SceneView S, RayBatchDev 
 
 __global__ void __launch_bounds__(256) k_trace_any(SceneView S, RayBatchDev rb, uint32_t n,
                                                    unsigned long long *stats) {
+  __shared__ float4 stk_lds[4 * SGR_TRACE_BLOCK];
+  const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     RayF r;
@@ -58,7 +62,7 @@ __global__ void __launch_bounds__(256) k_trace_any(SceneView S, RayBatchDev rb, 
     r.t0 = rb.t0[i];
     r.t1 = rb.t1[i];
     float u, v;
-    uint32_t p = trace_ray<true>(S, r, u, v, cnt);
+    uint32_t p = trace_ray<true>(S, r, u, v, cnt, stk);
     rb.occ[i] = p == SGR_MISS ? 0 : 1;
   }
   wave_count(&stats[0], cnt.nodes);
@@ -159,6 +163,8 @@ __global__ void __launch_bounds__(256) k_raygen(RenderView R1, StepView J, PathB
 template<bool MULTI>
 __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                 unsigned long long *stats) {
+  __shared__ float4 stk_lds[4 * SGR_TRACE_BLOCK];
+  const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   const uint32_t *q = P.q[b & 1];
@@ -170,7 +176,7 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
     RayF r;
     load_ray(P, p, r);
     float u = 0.0f, v = 0.0f;
-    uint32_t h = trace_ray<false>(S, r, u, v, cnt);
+    uint32_t h = trace_ray<false>(S, r, u, v, cnt, stk);
     P.hprim[p] = h;
     P.ht[p] = r.t1;
     P.hu[p] = u;
@@ -183,6 +189,8 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
 template<bool MULTI>
 __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                  unsigned long long *stats) {
+  __shared__ float4 stk_lds[4 * SGR_TRACE_BLOCK];
+  const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_S + b];
   uint32_t jb, je;
@@ -196,7 +204,7 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathB
     r.t0 = P.sray[6][slot];
     r.t1 = P.sray[7][slot];
     float u, v;
-    uint32_t h = trace_ray<true>(S, r, u, v, cnt);
+    uint32_t h = trace_ray<true>(S, r, u, v, cnt, stk);
     P.svis[slot] = h == SGR_MISS ? 1 : 0;
   }
   wave_count(&stats[SGR_STAT_NODES_S], cnt.nodes);

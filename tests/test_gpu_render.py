@@ -188,26 +188,28 @@ def test_invalid_parameters_raise(ctx):
 def test_full_size_frame_properties(ctx):
     """The 1.31M-triangle soup at 1920x1080, 64 spp (the benchmark frame):
     deterministic, chunking changes only the summation order, every pixel
-    gets filter weight, and the weights add up to the sample count (the
-    Mitchell filter's integer shifts sum to one)."""
-    import torch
+    gets filter weight, and without the negative-lobe clamp the weights
+    add up to the sample count (the Mitchell filter's integer shifts sum
+    to one)."""
     sd, cam = scenes.icosphere_soup()
     sc = native.Scene(ctx, sd)
     W, H, S = 1920, 1080, 64
     accs = []
     for pif in (0, 0, 1 << 24):
-        acc = torch.zeros((H, W, 5), dtype=torch.float32, device="cuda")
-        sc.render_device(cam, abi.make_params(W, H, S, max_path_len=10, seed=1,
-                                              paths_in_flight=pif), acc.data_ptr())
-        torch.cuda.synchronize()
-        accs.append(acc.cpu().numpy())
+        acc, _ = sc.render(cam, abi.make_params(W, H, S, max_path_len=10, seed=1,
+                                                paths_in_flight=pif))
+        accs.append(acc)
     a, b, c = accs
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     np.testing.assert_allclose(c, a, rtol=2e-5, atol=1e-5)
     assert np.isfinite(a).all()
     w = a[..., 4]
     assert np.all(w > 0)
-    assert w.sum() / (W * H * S) == pytest.approx(1.0, abs=5e-3)
     alpha = a[..., 3] / w
     assert np.all((alpha > 0.999) & (alpha < 1.001))   # bg_alpha 1, no holes
+    # Without the negative-lobe clamp (which shrinks negative weights of
+    # bright samples) each sample's weights over the image sum to one.
+    d, _ = sc.render(cam, abi.make_params(W, H, 4, max_path_len=10, seed=2, neg_clamp=-1e30))
+    assert d[..., 4].sum() / (W * H * 4) == pytest.approx(1.0, abs=2e-3)
+    assert w.sum() / (W * H * S) >= 1.0 - 2e-3
     sc.close()
