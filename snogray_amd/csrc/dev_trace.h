@@ -285,6 +285,229 @@ __device__ uint32_t trace_ray(const SceneView &S, RayF &ray, float &hu, float &h
   return hit;
 }
 
+// ---------------------------------------------------------------------------
+// The same walk as a resumable state machine, for lane-refill traversal:
+// trav_init sets up a ray (false: it misses the root cube), trav_step
+// enters the pending node and walks to the next node to enter (true: the
+// ray is finished).  The decisions, their order and the counters are
+// those of trace_ray, so results are bit-identical.
+struct TravState {
+  RayF ray;
+  uint32_t oct, cn, first, mask, vis, ci;
+  float bx0, bx1, by0, by1, bz0, bz1;
+  float xh, yh, zh, xm, ym, zm;
+  int sp, nl;
+  bool enter;
+  uint32_t hit;
+  float hu, hv;
+};
+
+__device__ __forceinline__ bool trav_init(const SceneView &S, TravState &T, TraceCount &cnt) {
+  T.hit = SGR_MISS;
+  T.hu = 0.0f;
+  T.hv = 0.0f;
+  if (S.n_nodes == 0) return false;
+  const RayF &ray = T.ray;
+  T.oct = (ray.d.x >= 0.0f ? 0u : 4u) | (ray.d.y >= 0.0f ? 0u : 2u) | (ray.d.z >= 0.0f ? 0u : 1u);
+  float ix = ray.d.x == 0.0f ? 1e9f : 1.0f / ray.d.x;
+  float bx0 = (S.origin[0] - ray.o.x) * ix, bx1 = (S.origin[0] - ray.o.x + S.size) * ix;
+  if (bx0 > bx1) { float q = bx0; bx0 = bx1; bx1 = q; }
+  float iy = ray.d.y == 0.0f ? 1e9f : 1.0f / ray.d.y;
+  float by0 = (S.origin[1] - ray.o.y) * iy, by1 = (S.origin[1] - ray.o.y + S.size) * iy;
+  if (by0 > by1) { float q = by0; by0 = by1; by1 = q; }
+  float iz = ray.d.z == 0.0f ? 1e9f : 1.0f / ray.d.z;
+  float bz0 = (S.origin[2] - ray.o.z) * iz, bz1 = (S.origin[2] - ray.o.z + S.size) * iz;
+  if (bz0 > bz1) { float q = bz0; bz0 = bz1; bz1 = q; }
+  cnt.nodes++;
+  if (fmx(ray.t0, fmx(bx0, fmx(by0, bz0))) >= fmn(ray.t1, fmn(bx1, fmn(by1, bz1)))) return false;
+  T.bx0 = bx0; T.bx1 = bx1; T.by0 = by0; T.by1 = by1; T.bz0 = bz0; T.bz1 = bz1;
+  T.sp = 0;
+  T.nl = 0;
+  T.cn = 0;
+  T.enter = true;
+  return true;
+}
+
+template<bool ANY>
+__device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, TraceCount &cnt,
+                                          LdsStack L, float4 *st0, float4 *st1) {
+  if (T.enter) {
+    float t, u, v;
+    uint32_t prim;
+    const uint4 nd = S.nodes[T.cn];
+    for (uint32_t k = nd.z, e = nd.z + nd.w; k < e; k++) {
+      cnt.tests++;
+      if (ref_hit(S.refs, k, T.ray, t, u, v, prim)) {
+        T.hit = prim;
+        if (ANY) return true;
+        T.ray.t1 = t;
+        T.hu = u;
+        T.hv = v;
+      }
+    }
+    T.mask = nd.y;
+    T.first = nd.x;
+    T.vis = visit_mask(T.mask, T.oct);
+    T.ci = 0;
+    if (T.vis) {
+      T.xh = (T.bx1 - T.bx0) / 2.0f;
+      T.yh = (T.by1 - T.by0) / 2.0f;
+      T.zh = (T.bz1 - T.bz0) / 2.0f;
+      T.xm = T.bx0 + T.xh;
+      T.ym = T.by0 + T.yh;
+      T.zm = T.bz0 + T.zh;
+    }
+    T.enter = false;
+  }
+  for (;;) {
+    const uint32_t rest = T.vis >> T.ci;
+    if (rest) {
+      const uint32_t i = T.ci + (uint32_t)__builtin_ctz(rest);
+      T.ci = i + 1;
+      const uint32_t child = (i == 3) ? 4u : (i == 4) ? 3u : i;
+      const float t0 = T.ray.t0, t1 = T.ray.t1;
+      const float xm = T.xm, ym = T.ym, zm = T.zm, xh = T.xh, yh = T.yh, zh = T.zh;
+      if (!(((child & 4) ? (t1 > xm) : (t0 < xm)) && ((child & 2) ? (t1 > ym) : (t0 < ym)) &&
+            ((child & 1) ? (t1 > zm) : (t0 < zm))))
+        continue;
+      const float cx = T.bx0 + ((child & 4) ? xh : 0.0f);
+      const float cy = T.by0 + ((child & 2) ? yh : 0.0f);
+      const float cz = T.bz0 + ((child & 1) ? zh : 0.0f);
+      const float cx1 = cx + xh, cy1 = cy + yh, cz1 = cz + zh;
+      cnt.nodes++;
+      if (fmx(t0, fmx(cx, fmx(cy, cz))) >= fmn(t1, fmn(cx1, fmn(cy1, cz1)))) continue;
+      const uint32_t real = child ^ T.oct;
+      uint32_t later = 0;
+      for (uint32_t r2 = T.vis >> T.ci; r2; r2 &= r2 - 1) {
+        const uint32_t j = T.ci + (uint32_t)__builtin_ctz(r2);
+        const uint32_t ch = (j == 3) ? 4u : (j == 4) ? 3u : j;
+        if (!(((ch & 4) ? (t1 > xm) : (t0 < xm)) && ((ch & 2) ? (t1 > ym) : (t0 < ym)) &&
+              ((ch & 1) ? (t1 > zm) : (t0 < zm))))
+          continue;
+        const float sx = T.bx0 + ((ch & 4) ? xh : 0.0f);
+        const float sy = T.by0 + ((ch & 2) ? yh : 0.0f);
+        const float sz = T.bz0 + ((ch & 1) ? zh : 0.0f);
+        if (fmx(t0, fmx(sx, fmx(sy, sz))) >= fmn(t1, fmn(sx + xh, fmn(sy + yh, sz + zh)))) {
+          cnt.nodes++;
+          continue;
+        }
+        later |= 1u << j;
+      }
+      if (later) {
+        float4 *slot = L.base + (size_t)(2 * (T.sp & 1)) * SGR_TRACE_BLOCK;
+        if (T.nl == 2) {
+          st0[T.sp - 2] = slot[0];
+          st1[T.sp - 2] = slot[SGR_TRACE_BLOCK];
+        } else {
+          T.nl++;
+        }
+        slot[0] = make_float4(__uint_as_float(T.first | (T.ci << 28)),
+                              __uint_as_float(T.mask | (later << 8)), T.bx0, T.bx1);
+        slot[SGR_TRACE_BLOCK] = make_float4(T.by0, T.by1, T.bz0, T.bz1);
+        T.sp++;
+      }
+      T.cn = T.first + __popc(T.mask & ((1u << real) - 1u));
+      T.bx0 = cx; T.bx1 = cx1;
+      T.by0 = cy; T.by1 = cy1;
+      T.bz0 = cz; T.bz1 = cz1;
+      T.enter = true;
+      return false;
+    }
+    if (T.sp == 0) return true;
+    T.sp--;
+    float4 a, b;
+    if (T.nl > 0) {
+      const float4 *slot = L.base + (size_t)(2 * (T.sp & 1)) * SGR_TRACE_BLOCK;
+      a = slot[0];
+      b = slot[SGR_TRACE_BLOCK];
+      T.nl--;
+    } else {
+      a = st0[T.sp];
+      b = st1[T.sp];
+    }
+    const uint32_t w0 = __float_as_uint(a.x), w1 = __float_as_uint(a.y);
+    T.first = w0 & 0x0fffffffu;
+    T.ci = w0 >> 28;
+    T.mask = w1 & 0xffu;
+    T.vis = (w1 >> 8) & 0xffu;
+    T.bx0 = a.z; T.bx1 = a.w; T.by0 = b.x; T.by1 = b.y; T.bz0 = b.z; T.bz1 = b.w;
+    T.xh = (T.bx1 - T.bx0) / 2.0f;
+    T.yh = (T.by1 - T.by0) / 2.0f;
+    T.zh = (T.bz1 - T.bz0) / 2.0f;
+    T.xm = T.bx0 + T.xh;
+    T.ym = T.by0 + T.yh;
+    T.zm = T.bz0 + T.zh;
+  }
+}
+
+// Lane-refill driver: every lane of a wave walks its own ray; when at
+// least SGR_REFILL lanes are idle the wave takes that many more ray
+// indices of [0, len) with one atomic on *head (a wave-wide ballot and
+// rank), so no lane idles while its wave still has rays to finish.
+// GET(k, ray) loads ray k and returns its scene (used when MULTI; S1
+// otherwise); PUT(k, state) stores the result.  The loop exits when the
+// wave's last ray is done and the range is exhausted (every wave gets
+// there: each step advances a finite walk).
+#define SGR_REFILL 16
+template<bool ANY, bool MULTI, typename Get, typename Put>
+__device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t len, uint32_t *head,
+                                             Get get, Put put, TraceCount &cnt, LdsStack L) {
+  TravState T;
+  float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
+  bool active = false, drained = false;
+  uint32_t id = 0;
+  const SceneView *Sp = &S1;
+  const uint32_t lane = __lane_id();
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = (uint32_t)__popcll(idle);
+    if (drained && nidle == 64u) break;
+    if (!drained && nidle >= SGR_REFILL) {
+      const int leader = __ffsll((unsigned long long)idle) - 1;
+      uint32_t base = 0;
+      if ((int)lane == leader) base = atomicAdd(head, nidle);
+      base = (uint32_t)__shfl((int)base, leader);
+      if (base + nidle >= len) drained = true;
+      if (!active) {
+        const uint32_t k = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        if (k < len) {
+          id = k;
+          const SceneView *g = get(k, T.ray);
+          if (MULTI) Sp = g;
+          const SceneView &S = MULTI ? *Sp : S1;
+          if (S.use_triv) {
+            // brute force (TrivSpace, space/triv-space.h:48-58), in one go
+            T.hit = SGR_MISS;
+            T.hu = 0.0f;
+            T.hv = 0.0f;
+            float t, u, v;
+            uint32_t prim;
+            for (uint32_t q = 0; q < S.n_surfs; q++) {
+              cnt.tests++;
+              if (ref_hit(S.refs, q, T.ray, t, u, v, prim)) {
+                T.hit = prim;
+                if (ANY) break;
+                T.ray.t1 = t;
+                T.hu = u;
+                T.hv = v;
+              }
+            }
+            put(id, T);
+          } else if (trav_init(S, T, cnt)) {
+            active = true;
+          } else {
+            put(id, T);
+          }
+        }
+      }
+    }
+    if (active && trav_step<ANY>(MULTI ? *Sp : S1, T, cnt, L, st0, st1)) {
+      put(id, T);
+      active = false;
+    }
+  }
+}
+
 // Sum a per-lane counter over the wave and add it once to *dst.
 __device__ __forceinline__ void wave_count(unsigned long long *dst, uint32_t v) {
   unsigned long long x = v;

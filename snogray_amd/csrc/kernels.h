@@ -19,11 +19,15 @@ namespace sgr {
 #define SGR_ST_DONE       (1u << 17)
 #define SGR_ST_STRAT      (1u << 18)
 
-// Counter block: ray-queue lengths per bounce, then shadow-queue lengths.
+// Counter block: ray-queue lengths per bounce, shadow-queue lengths, then
+// the fetch heads of the lane-refill traversal (per bounce, one per XCD
+// slice of the queue) for closest-hit and any-hit rays.
 #define SGR_MAX_BOUNCES   64
 #define SGR_CNT_Q         0
 #define SGR_CNT_S         (SGR_MAX_BOUNCES + 2)
-#define SGR_NUM_COUNTERS  (2 * SGR_MAX_BOUNCES + 4)
+#define SGR_CNT_HE        (2 * SGR_MAX_BOUNCES + 4)
+#define SGR_CNT_HS        (SGR_CNT_HE + 8 * SGR_MAX_BOUNCES)
+#define SGR_NUM_COUNTERS  (SGR_CNT_HS + 8 * SGR_MAX_BOUNCES)
 
 // Device statistics slots (unsigned long long).
 #define SGR_STAT_NODES_C 0
@@ -34,6 +38,8 @@ namespace sgr {
 #define SGR_STAT_ISECT   5
 #define SGR_STAT_SHADOW  6
 #define SGR_NUM_STATS    8
+// The statistics buffer also holds the 8 fetch heads of a ray-batch query.
+#define SGR_STATS_BYTES  (SGR_NUM_STATS * 8 + 8 * 4)
 
 struct RayBatchDev {
   const float *ox, *oy, *oz, *dx, *dy, *dz, *t0, *t1;

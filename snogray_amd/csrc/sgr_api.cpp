@@ -147,7 +147,7 @@ int sgr_ctx_create(int device, sgr_ctx **out) {
     return fail(SGR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
   for (auto &ev : c->ev) (void)hipEventCreate(&ev);
-  if (hipMalloc((void **)&c->stats, SGR_NUM_STATS * sizeof(unsigned long long)) != hipSuccess) {
+  if (hipMalloc((void **)&c->stats, SGR_STATS_BYTES) != hipSuccess) {
     delete c;
     return fail(SGR_ERR_NOMEM, "sgr_ctx_create: stats buffer");
   }
@@ -611,7 +611,7 @@ static int trace_common(sgr_scene *sc, const sgr_rays *rays, size_t n, sgr_hits 
   for (int k = 0; k < 8 && e == hipSuccess; k++)
     e = hipMemcpyAsync(f + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
-    e = hipMemsetAsync(ctx->stats, 0, SGR_NUM_STATS * sizeof(unsigned long long), ctx->stream);
+    e = hipMemsetAsync(ctx->stats, 0, SGR_STATS_BYTES, ctx->stream);
   RayBatchDev rb;
   rb.ox = f; rb.oy = f + n; rb.oz = f + 2 * n; rb.dx = f + 3 * n; rb.dy = f + 4 * n;
   rb.dz = f + 5 * n; rb.t0 = f + 6 * n; rb.t1 = f + 7 * n;

@@ -28,43 +28,46 @@ namespace sgr {
 // ---------------------------------------------------------------------------
 // Ray batches
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_trace_closest(SceneView S, RayBatchDev rb, uint32_t n,
-                                                       unsigned long long *stats) {
-  __shared__ float4 stk_lds[4 * SGR_TRACE_BLOCK];
-  const LdsStack stk{&stk_lds[threadIdx.x]};
-  TraceCount cnt{0, 0};
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    RayF r;
-    r.o = v3(rb.ox[i], rb.oy[i], rb.oz[i]);
-    r.d = v3(rb.dx[i], rb.dy[i], rb.dz[i]);
-    r.t0 = rb.t0[i];
-    r.t1 = rb.t1[i];
-    float u = 0.0f, v = 0.0f;
-    uint32_t p = trace_ray<false>(S, r, u, v, cnt, stk);
-    rb.prim[i] = p;
-    rb.t[i] = p == SGR_MISS ? 0.0f : r.t1;
-    rb.u[i] = u;
-    rb.v[i] = v;
-  }
-  wave_count(&stats[0], cnt.nodes);
-  wave_count(&stats[1], cnt.tests);
+// Ray-batch queries with lane-refill traversal: each XCD's blocks share
+// one eighth of the batch and its fetch head (heads[0..7], zeroed by the
+// caller).
+__device__ __forceinline__ void xcd_slice(uint32_t n, uint32_t &beg, uint32_t &end) {
+  const uint32_t x = blockIdx.x & 7u;
+  beg = (uint32_t)((uint64_t)n * x / 8u);
+  end = (uint32_t)((uint64_t)n * (x + 1u) / 8u);
 }
 
-__global__ void __launch_bounds__(256) k_trace_any(SceneView S, RayBatchDev rb, uint32_t n,
-                                                   unsigned long long *stats) {
+template<bool ANY>
+__global__ void __launch_bounds__(256) k_trace(SceneView S, RayBatchDev rb, uint32_t n,
+                                               unsigned long long *stats) {
   __shared__ float4 stk_lds[4 * SGR_TRACE_BLOCK];
   const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    RayF r;
-    r.o = v3(rb.ox[i], rb.oy[i], rb.oz[i]);
-    r.d = v3(rb.dx[i], rb.dy[i], rb.dz[i]);
-    r.t0 = rb.t0[i];
-    r.t1 = rb.t1[i];
-    float u, v;
-    uint32_t p = trace_ray<true>(S, r, u, v, cnt, stk);
-    rb.occ[i] = p == SGR_MISS ? 0 : 1;
-  }
+  uint32_t beg, end;
+  xcd_slice(n, beg, end);
+  uint32_t *heads = (uint32_t *)(stats + SGR_NUM_STATS);
+  trace_refill<ANY, false>(
+      S, end - beg, &heads[blockIdx.x & 7u],
+      [&](uint32_t k, RayF &r) -> const SceneView * {
+        const uint32_t i = beg + k;
+        r.o = v3(rb.ox[i], rb.oy[i], rb.oz[i]);
+        r.d = v3(rb.dx[i], rb.dy[i], rb.dz[i]);
+        r.t0 = rb.t0[i];
+        r.t1 = rb.t1[i];
+        return nullptr;
+      },
+      [&](uint32_t k, const TravState &T) {
+        const uint32_t i = beg + k;
+        if (ANY) {
+          rb.occ[i] = T.hit == SGR_MISS ? 0 : 1;
+        } else {
+          rb.prim[i] = T.hit;
+          rb.t[i] = T.hit == SGR_MISS ? 0.0f : T.ray.t1;
+          rb.u[i] = T.hu;
+          rb.v[i] = T.hv;
+        }
+      },
+      cnt, stk);
   wave_count(&stats[0], cnt.nodes);
   wave_count(&stats[1], cnt.tests);
 }
@@ -122,7 +125,8 @@ template<bool MULTI>
 __global__ void __launch_bounds__(256) k_raygen(RenderView R1, StepView J, PathBufs P) {
   uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t total = MULTI ? J.hdr[1] : R1.npaths;
-  if (gid < SGR_NUM_COUNTERS) P.counts[gid] = gid == 0 ? total : 0u;
+  for (uint32_t c = gid; c < SGR_NUM_COUNTERS; c += gridDim.x * blockDim.x)
+    P.counts[c] = c == 0 ? total : 0u;
   for (uint32_t i = gid; i < total; i += gridDim.x * blockDim.x) {
     uint32_t seg = 0;
     if (MULTI) {
@@ -168,20 +172,23 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   const uint32_t *q = P.q[b & 1];
-  uint32_t jb, je;
-  block_range(n, jb, je);
-  for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
-    uint32_t p = q[j];
-    const SceneView S = MULTI ? J.S[J.R[P.pseg[p]].scene] : S1;
-    RayF r;
-    load_ray(P, p, r);
-    float u = 0.0f, v = 0.0f;
-    uint32_t h = trace_ray<false>(S, r, u, v, cnt, stk);
-    P.hprim[p] = h;
-    P.ht[p] = r.t1;
-    P.hu[p] = u;
-    P.hv[p] = v;
-  }
+  uint32_t beg, end;
+  xcd_slice(n, beg, end);
+  trace_refill<false, MULTI>(
+      S1, end - beg, &P.counts[SGR_CNT_HE + 8 * b + (blockIdx.x & 7u)],
+      [&](uint32_t k, RayF &r) -> const SceneView * {
+        const uint32_t p = q[beg + k];
+        load_ray(P, p, r);
+        return MULTI ? &J.S[J.R[P.pseg[p]].scene] : nullptr;
+      },
+      [&](uint32_t k, const TravState &T) {
+        const uint32_t p = q[beg + k];
+        P.hprim[p] = T.hit;
+        P.ht[p] = T.ray.t1;
+        P.hu[p] = T.hu;
+        P.hv[p] = T.hv;
+      },
+      cnt, stk);
   wave_count(&stats[SGR_STAT_NODES_C], cnt.nodes);
   wave_count(&stats[SGR_STAT_TESTS_C], cnt.tests);
 }
@@ -193,25 +200,26 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathB
   const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_S + b];
-  uint32_t jb, je;
-  block_range(n, jb, je);
-  for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
-    uint32_t slot = P.sq[j];
-    const SceneView S = MULTI ? J.S[J.R[P.pseg[slot / P.K]].scene] : S1;
-    RayF r;
-    r.o = v3(P.sray[0][slot], P.sray[1][slot], P.sray[2][slot]);
-    r.d = v3(P.sray[3][slot], P.sray[4][slot], P.sray[5][slot]);
-    r.t0 = P.sray[6][slot];
-    r.t1 = P.sray[7][slot];
-    float u, v;
-    uint32_t h = trace_ray<true>(S, r, u, v, cnt, stk);
-    P.svis[slot] = h == SGR_MISS ? 1 : 0;
-  }
+  uint32_t beg, end;
+  xcd_slice(n, beg, end);
+  trace_refill<true, MULTI>(
+      S1, end - beg, &P.counts[SGR_CNT_HS + 8 * b + (blockIdx.x & 7u)],
+      [&](uint32_t k, RayF &r) -> const SceneView * {
+        const uint32_t slot = P.sq[beg + k];
+        r.o = v3(P.sray[0][slot], P.sray[1][slot], P.sray[2][slot]);
+        r.d = v3(P.sray[3][slot], P.sray[4][slot], P.sray[5][slot]);
+        r.t0 = P.sray[6][slot];
+        r.t1 = P.sray[7][slot];
+        return MULTI ? &J.S[J.R[P.pseg[slot / P.K]].scene] : nullptr;
+      },
+      [&](uint32_t k, const TravState &T) {
+        P.svis[P.sq[beg + k]] = T.hit == SGR_MISS ? 1 : 0;
+      },
+      cnt, stk);
   wave_count(&stats[SGR_STAT_NODES_S], cnt.nodes);
   wave_count(&stats[SGR_STAT_TESTS_S], cnt.tests);
 }
 
-// One bounce of PathInteg::Li for every path in the ray queue.
 template<bool MULTI, bool GUIDED>
 __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, StepView J, PathBufs P,
                                                uint32_t b, unsigned long long *stats,
@@ -652,8 +660,9 @@ static inline unsigned grid_for(uint32_t n, unsigned cap) {
 void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool any,
                   unsigned long long *stats, hipStream_t st) {
   unsigned g = grid_for(n, 65535);
-  if (any) hipLaunchKernelGGL(k_trace_any, dim3(g), dim3(256), 0, st, S, rb, n, stats);
-  else hipLaunchKernelGGL(k_trace_closest, dim3(g), dim3(256), 0, st, S, rb, n, stats);
+  g = (g + 7) & ~7u;  // whole blocks per XCD slice
+  if (any) hipLaunchKernelGGL(k_trace<true>, dim3(g), dim3(256), 0, st, S, rb, n, stats);
+  else hipLaunchKernelGGL(k_trace<false>, dim3(g), dim3(256), 0, st, S, rb, n, stats);
 }
 
 void launch_raygen(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
