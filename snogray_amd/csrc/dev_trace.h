@@ -442,38 +442,46 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
 
 // Lane-refill driver: every lane of a wave walks its own ray; when at
 // least SGR_REFILL (32: the best of 8, 16, 32, 48 on the 1.31 M-triangle
-// workload) lanes are idle the wave takes that many more ray
-// indices of [0, len) with one atomic on *head (a wave-wide ballot and
-// rank), so no lane idles while its wave still has rays to finish.
-// GET(k, ray) loads ray k and returns its scene (used when MULTI; S1
-// otherwise); PUT(k, state) stores the result.  The loop exits when the
-// wave's last ray is done and the range is exhausted (every wave gets
-// there: each step advances a finite walk).
+// workload) lanes are idle the wave takes that many more rays with one
+// atomic on a slice head (a wave-wide ballot ranks the idle lanes), so
+// no lane idles while its wave still has rays to finish.  The N rays are
+// cut into 8 slices, one per XCD (blockIdx.x & 7 under round-robin
+// dispatch), so neighbouring rays share an L2; a wave whose slice is
+// used up moves on to the next slices in turn, so no XCD idles while
+// another still has rays.  GET(i, ray) loads ray i and returns its scene
+// (used when MULTI; S1 otherwise); PUT(i, state) stores the result.  The
+// loop exits when all 8 slices are used up and the wave's last ray is
+// done (every wave gets there: each step advances a finite walk).
 #define SGR_REFILL 32
 template<bool ANY, bool MULTI, typename Get, typename Put>
-__device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t len, uint32_t *head,
+__device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t n, uint32_t *heads,
                                              Get get, Put put, TraceCount &cnt, LdsStack L) {
   TravState T;
   float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
-  bool active = false, drained = false;
-  uint32_t id = 0;
+  bool active = false;
+  uint32_t id = 0, tried = 0, slice = blockIdx.x & 7u;
   const SceneView *Sp = &S1;
   const uint32_t lane = __lane_id();
   for (;;) {
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = (uint32_t)__popcll(idle);
-    if (drained && nidle == 64u) break;
-    if (!drained && nidle >= SGR_REFILL) {
+    if (tried == 8u && nidle == 64u) break;
+    if (tried < 8u && nidle >= SGR_REFILL) {
+      const uint32_t beg = (uint32_t)((uint64_t)n * slice / 8u);
+      const uint32_t len = (uint32_t)((uint64_t)n * (slice + 1u) / 8u) - beg;
       const int leader = __ffsll((unsigned long long)idle) - 1;
       uint32_t base = 0;
-      if ((int)lane == leader) base = atomicAdd(head, nidle);
+      if ((int)lane == leader) base = atomicAdd(&heads[slice], nidle);
       base = (uint32_t)__shfl((int)base, leader);
-      if (base + nidle >= len) drained = true;
+      if (base + nidle >= len) {  // this slice is used up: next time, the next one
+        tried++;
+        slice = (slice + 1u) & 7u;
+      }
       if (!active) {
         const uint32_t k = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
         if (k < len) {
-          id = k;
-          const SceneView *g = get(k, T.ray);
+          id = beg + k;
+          const SceneView *g = get(id, T.ray);
           if (MULTI) Sp = g;
           const SceneView &S = MULTI ? *Sp : S1;
           if (S.use_triv) {

@@ -28,36 +28,25 @@ namespace sgr {
 // ---------------------------------------------------------------------------
 // Ray batches
 // ---------------------------------------------------------------------------
-// Ray-batch queries with lane-refill traversal: each XCD's blocks share
-// one eighth of the batch and its fetch head (heads[0..7], zeroed by the
-// caller).
-__device__ __forceinline__ void xcd_slice(uint32_t n, uint32_t &beg, uint32_t &end) {
-  const uint32_t x = blockIdx.x & 7u;
-  beg = (uint32_t)((uint64_t)n * x / 8u);
-  end = (uint32_t)((uint64_t)n * (x + 1u) / 8u);
-}
-
+// Ray-batch queries with lane-refill traversal (heads[0..7], zeroed by
+// the caller, are the fetch heads of the 8 slices of the batch).
 template<bool ANY>
 __global__ void __launch_bounds__(256) k_trace(SceneView S, RayBatchDev rb, uint32_t n,
                                                unsigned long long *stats) {
   __shared__ float4 stk_lds[4 * SGR_TRACE_BLOCK];
   const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
-  uint32_t beg, end;
-  xcd_slice(n, beg, end);
   uint32_t *heads = (uint32_t *)(stats + SGR_NUM_STATS);
   trace_refill<ANY, false>(
-      S, end - beg, &heads[blockIdx.x & 7u],
-      [&](uint32_t k, RayF &r) -> const SceneView * {
-        const uint32_t i = beg + k;
+      S, n, heads,
+      [&](uint32_t i, RayF &r) -> const SceneView * {
         r.o = v3(rb.ox[i], rb.oy[i], rb.oz[i]);
         r.d = v3(rb.dx[i], rb.dy[i], rb.dz[i]);
         r.t0 = rb.t0[i];
         r.t1 = rb.t1[i];
         return nullptr;
       },
-      [&](uint32_t k, const TravState &T) {
-        const uint32_t i = beg + k;
+      [&](uint32_t i, const TravState &T) {
         if (ANY) {
           rb.occ[i] = T.hit == SGR_MISS ? 0 : 1;
         } else {
@@ -172,17 +161,15 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   const uint32_t *q = P.q[b & 1];
-  uint32_t beg, end;
-  xcd_slice(n, beg, end);
   trace_refill<false, MULTI>(
-      S1, end - beg, &P.counts[SGR_CNT_HE + 8 * b + (blockIdx.x & 7u)],
-      [&](uint32_t k, RayF &r) -> const SceneView * {
-        const uint32_t p = q[beg + k];
+      S1, n, &P.counts[SGR_CNT_HE + 8 * b],
+      [&](uint32_t i, RayF &r) -> const SceneView * {
+        const uint32_t p = q[i];
         load_ray(P, p, r);
         return MULTI ? &J.S[J.R[P.pseg[p]].scene] : nullptr;
       },
-      [&](uint32_t k, const TravState &T) {
-        const uint32_t p = q[beg + k];
+      [&](uint32_t i, const TravState &T) {
+        const uint32_t p = q[i];
         P.hprim[p] = T.hit;
         P.ht[p] = T.ray.t1;
         P.hu[p] = T.hu;
@@ -200,20 +187,18 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathB
   const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_S + b];
-  uint32_t beg, end;
-  xcd_slice(n, beg, end);
   trace_refill<true, MULTI>(
-      S1, end - beg, &P.counts[SGR_CNT_HS + 8 * b + (blockIdx.x & 7u)],
-      [&](uint32_t k, RayF &r) -> const SceneView * {
-        const uint32_t slot = P.sq[beg + k];
+      S1, n, &P.counts[SGR_CNT_HS + 8 * b],
+      [&](uint32_t i, RayF &r) -> const SceneView * {
+        const uint32_t slot = P.sq[i];
         r.o = v3(P.sray[0][slot], P.sray[1][slot], P.sray[2][slot]);
         r.d = v3(P.sray[3][slot], P.sray[4][slot], P.sray[5][slot]);
         r.t0 = P.sray[6][slot];
         r.t1 = P.sray[7][slot];
         return MULTI ? &J.S[J.R[P.pseg[slot / P.K]].scene] : nullptr;
       },
-      [&](uint32_t k, const TravState &T) {
-        P.svis[P.sq[beg + k]] = T.hit == SGR_MISS ? 1 : 0;
+      [&](uint32_t i, const TravState &T) {
+        P.svis[P.sq[i]] = T.hit == SGR_MISS ? 1 : 0;
       },
       cnt, stk);
   wave_count(&stats[SGR_STAT_NODES_S], cnt.nodes);
