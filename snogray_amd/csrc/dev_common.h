@@ -41,6 +41,9 @@ struct DevMesh {
   int left_handed;
 };
 
+#define SGR_ENV_GUIDE_ROW 256
+#define SGR_ENV_GUIDE_COL 32
+
 struct SceneView {
   const uint4 *nodes;        // first_child, mask, surf_begin, surf_count
   const uint32_t *surfs;     // primitive ids (host export only)
@@ -55,6 +58,11 @@ struct SceneView {
   const float4 *env_tex;     // rgb_ per texel, row 0 = top
   const float *env_row_cum;
   const float *env_col_cum;
+  // Guide tables of the two searches: entry g of a table of G + 1 is the
+  // lower_bound of g / G in its cumulative array (rows: one table of
+  // SGR_ENV_GUIDE_ROW + 1; columns: SGR_ENV_GUIDE_COL + 1 per row).
+  const uint32_t *env_row_guide;
+  const uint32_t *env_col_guide;
   uint32_t n_tri_like;       // triangles + tripars
   uint32_t n_tri;            // mesh triangles
   uint32_t n_prims;

@@ -468,16 +468,34 @@ __device__ __forceinline__ uint32_t lower_bound_f(const float *a, uint32_t n, fl
   }
   return lo;
 }
+// lower_bound of VAL in A[N] through a guide table GD of G + 1 entries:
+// with g = floor(val * G), g / G <= val < (g + 1) / G, so the answer lies
+// in [GD[g], GD[g + 1]] and a search of that range gives the same index.
+__device__ __forceinline__ uint32_t guided_lower_bound(const float *a, uint32_t n,
+                                                       const uint32_t *gd, uint32_t G, float val) {
+  int g = (int)(val * (float)G);
+  g = g < 0 ? 0 : (g > (int)G - 1 ? (int)G - 1 : g);
+  // the float product can round across a boundary: step to the cell
+  // whose bounds hold val exactly as the table was built
+  if ((float)g / (float)G > val) g--;
+  else if (g + 1 < (int)G && (float)(g + 1) / (float)G <= val) g++;
+  uint32_t lo = gd[g], hi = gd[g + 1];
+  if (g + 1 == (int)G) hi = n;
+  return lo + lower_bound_f(a + lo, hi - lo, val);
+}
+
 __device__ __forceinline__ bool env_hist_sample(const SceneView &S, float pu, float pv,
                                                 float &ou, float &ov, float &pdf) {
   float u = fmn(pu, 1.0f), v = fmn(pv, 1.0f);
-  uint32_t row = lower_bound_f(S.env_row_cum, S.env_hh, v);
+  uint32_t row = guided_lower_bound(S.env_row_cum, S.env_hh, S.env_row_guide, SGR_ENV_GUIDE_ROW, v);
   if (row == S.env_hh) {
     pdf = 0.0f;
     ou = ov = 0.0f;
     return false;
   }
-  uint32_t col = lower_bound_f(S.env_col_cum + (size_t)row * S.env_hw, S.env_hw, u);
+  uint32_t col = guided_lower_bound(S.env_col_cum + (size_t)row * S.env_hw, S.env_hw,
+                                    S.env_col_guide + (size_t)row * (SGR_ENV_GUIDE_COL + 1),
+                                    SGR_ENV_GUIDE_COL, u);
   pdf = env_bin_pdf(S, col, row);
   float cw = 1.0f / (float)S.env_hw, rh = 1.0f / (float)S.env_hh;
   ou = (float)col * cw + fmodf(pu, cw);

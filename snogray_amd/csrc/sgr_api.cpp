@@ -112,6 +112,15 @@ void build_env_cdf(const sgr_envmap &em, std::vector<float> &row_cum,
   }
 }
 
+// Guide table of a cumulative array A[N]: entry g (g = 0..G) is the
+// std::lower_bound of float(g) / G in A, appended to OUT.
+void build_guide(const float *a, uint32_t n, uint32_t G, std::vector<uint32_t> &out) {
+  for (uint32_t g = 0; g <= G; g++) {
+    const float val = (float)g / (float)G;
+    out.push_back((uint32_t)(std::lower_bound(a, a + n, val) - a));
+  }
+}
+
 inline float fmin_t(float a, float b) { return a < b ? a : b; }
 inline float fmax_t(float a, float b) { return a > b ? a : b; }
 
@@ -192,6 +201,7 @@ struct HostScene {
   std::vector<DevMesh> meshes;
   std::vector<DevLight> lights;
   std::vector<float> row_cum, col_cum;
+  std::vector<uint32_t> row_guide, col_guide;
   FlatOctree oct;
   SceneView V{};
   float blo[3], bhi[3];
@@ -412,6 +422,10 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     for (size_t i = 0; i < H.env_tex.size(); i++)
       H.env_tex[i] = make_float4(em.texels[3 * i], em.texels[3 * i + 1], em.texels[3 * i + 2], 0.f);
     build_env_cdf(em, H.row_cum, H.col_cum, V.env_hw, V.env_hh);
+    build_guide(H.row_cum.data(), V.env_hh, SGR_ENV_GUIDE_ROW, H.row_guide);
+    for (uint32_t r = 0; r < V.env_hh; r++)
+      build_guide(H.col_cum.data() + (size_t)r * V.env_hw, V.env_hw, SGR_ENV_GUIDE_COL,
+                  H.col_guide);
     V.env_w = em.width;
     V.env_h = em.height;
     V.has_env = 1;
@@ -520,6 +534,8 @@ int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
   if (!rc) rc = upload(ctx, sc, up, H.env_tex, &V.env_tex);
   if (!rc) rc = upload(ctx, sc, up, H.row_cum, &V.env_row_cum);
   if (!rc) rc = upload(ctx, sc, up, H.col_cum, &V.env_col_cum);
+  if (!rc) rc = upload(ctx, sc, up, H.row_guide, &V.env_row_guide);
+  if (!rc) rc = upload(ctx, sc, up, H.col_guide, &V.env_col_guide);
   if (up) {
     hipError_t e = hipStreamSynchronize(up);
     if (!rc && e != hipSuccess) rc = fail(SGR_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
