@@ -116,181 +116,26 @@ __device__ __forceinline__ uint32_t visit_mask(uint32_t mask, uint32_t oct) {
 // at the same point of the walk, before it is entered, so the surfaces
 // are tested in the reference's order and the first of several equal-t
 // hits wins exactly as on the CPU.  Node-call counts include the pruned
-// calls, as the reference counts them.
-// ANY: stop at the first hit (space/space.cc:103-116 / :160-180).
-// Traversal stack: the two newest entries of each lane live in LDS
-// (slot i of the lane at lds[(2 * i + half) * SGR_TRACE_BLOCK]); older ones are
-// spilled to the lane's private array when a third is pushed and read
-// back from there when popped.  Entries are 2 x float4: (first | child
-// cursor << 28, child mask | pending siblings << 8, x slab) and (y slab,
-// z slab).
+// calls, as the reference counts them.  ANY: stop at the first hit
+// (space/space.cc:103-116 / :160-180).
+//
+// Traversal stack: the SGR_LDS_SLOTS newest entries of each lane live in
+// LDS (slot i of the lane at lds[(2 * i + half) * SGR_TRACE_BLOCK]);
+// older ones are spilled to the lane's private array when a newer one
+// needs their slot and read back from there when popped.  Entries are
+// 2 x float4: (first | child cursor << 28, child mask | pending siblings
+// << 8, x slab) and (y slab, z slab).
 #define SGR_TRACE_BLOCK 256
+#define SGR_LDS_SLOTS 3
 struct LdsStack {
-  float4 *base;  // &shared[threadIdx.x] of a [4][SGR_TRACE_BLOCK] float4 array
+  float4 *base;  // &shared[threadIdx.x] of a [2 * SGR_LDS_SLOTS][SGR_TRACE_BLOCK] array
 };
 
-template<bool ANY>
-__device__ uint32_t trace_ray(const SceneView &S, RayF &ray, float &hu, float &hv,
-                              TraceCount &cnt, LdsStack L) {
-  uint32_t hit = SGR_MISS;
-  float t, u, v;
-  uint32_t prim;
-  if (S.use_triv) {  // TrivSpace (space/triv-space.h:48-58)
-    for (uint32_t k = 0; k < S.n_surfs; k++) {
-      cnt.tests++;
-      if (ref_hit(S.refs, k, ray, t, u, v, prim)) {
-        if (ANY) return prim;
-        ray.t1 = t;
-        hit = prim;
-        hu = u;
-        hv = v;
-      }
-    }
-    return hit;
-  }
-  if (S.n_nodes == 0) return hit;
-  const uint32_t oct = (ray.d.x >= 0.0f ? 0u : 4u) | (ray.d.y >= 0.0f ? 0u : 2u) |
-                       (ray.d.z >= 0.0f ? 0u : 1u);
-  float bx0, bx1, by0, by1, bz0, bz1;
-  {
-    float ix = ray.d.x == 0.0f ? 1e9f : 1.0f / ray.d.x;
-    bx0 = (S.origin[0] - ray.o.x) * ix;
-    bx1 = (S.origin[0] - ray.o.x + S.size) * ix;
-    if (bx0 > bx1) { float q = bx0; bx0 = bx1; bx1 = q; }
-    float iy = ray.d.y == 0.0f ? 1e9f : 1.0f / ray.d.y;
-    by0 = (S.origin[1] - ray.o.y) * iy;
-    by1 = (S.origin[1] - ray.o.y + S.size) * iy;
-    if (by0 > by1) { float q = by0; by0 = by1; by1 = q; }
-    float iz = ray.d.z == 0.0f ? 1e9f : 1.0f / ray.d.z;
-    bz0 = (S.origin[2] - ray.o.z) * iz;
-    bz1 = (S.origin[2] - ray.o.z + S.size) * iz;
-    if (bz0 > bz1) { float q = bz0; bz0 = bz1; bz1 = q; }
-  }
-  cnt.nodes++;
-  if (fmx(ray.t0, fmx(bx0, fmx(by0, bz0))) >= fmn(ray.t1, fmn(bx1, fmn(by1, bz1)))) return hit;
-  float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
-  int sp = 0, nl = 0;  // stack depth, entries resident in LDS (top nl)
-  uint32_t cn = 0;
-  for (;;) {
-    // Enter node cn: its surfaces, then set up the child walk.
-    uint4 nd = S.nodes[cn];
-    for (uint32_t k = nd.z, e = nd.z + nd.w; k < e; k++) {
-      cnt.tests++;
-      if (ref_hit(S.refs, k, ray, t, u, v, prim)) {
-        if (ANY) return prim;
-        ray.t1 = t;
-        hit = prim;
-        hu = u;
-        hv = v;
-      }
-    }
-    uint32_t mask = nd.y, first = nd.x;
-    uint32_t vis = visit_mask(mask, oct);
-    uint32_t ci = 0;
-    float xh = 0, yh = 0, zh = 0, xm = 0, ym = 0, zm = 0;
-    if (vis) {
-      xh = (bx1 - bx0) / 2.0f;
-      yh = (by1 - by0) / 2.0f;
-      zh = (bz1 - bz0) / 2.0f;
-      xm = bx0 + xh;
-      ym = by0 + yh;
-      zm = bz0 + zh;
-    }
-    bool entered = false;
-    for (;;) {
-      // Next existing child in visiting order.
-      uint32_t rest = vis >> ci;
-      if (rest) {
-        uint32_t i = ci + (uint32_t)__builtin_ctz(rest);
-        ci = i + 1;
-        uint32_t child = (i == 3) ? 4u : (i == 4) ? 3u : i;
-        float t0 = ray.t0, t1 = ray.t1;
-        if (!(((child & 4) ? (t1 > xm) : (t0 < xm)) && ((child & 2) ? (t1 > ym) : (t0 < ym)) &&
-              ((child & 1) ? (t1 > zm) : (t0 < zm))))
-          continue;
-        float cx = bx0 + ((child & 4) ? xh : 0.0f);
-        float cy = by0 + ((child & 2) ? yh : 0.0f);
-        float cz = bz0 + ((child & 1) ? zh : 0.0f);
-        float cx1 = cx + xh, cy1 = cy + yh, cz1 = cz + zh;
-        cnt.nodes++;
-        if (fmx(t0, fmx(cx, fmx(cy, cz))) >= fmn(t1, fmn(cx1, fmn(cy1, cz1)))) continue;
-        uint32_t real = child ^ oct;
-        // Siblings still to visit: drop those that already fail the prune
-        // test with the current ray extent -- t0 is fixed and t1 only
-        // shrinks, so they would fail when their turn comes too.
-        uint32_t later = 0;
-        for (uint32_t r2 = vis >> ci; r2; r2 &= r2 - 1) {
-          uint32_t j = ci + (uint32_t)__builtin_ctz(r2);
-          uint32_t ch = (j == 3) ? 4u : (j == 4) ? 3u : j;
-          if (!(((ch & 4) ? (t1 > xm) : (t0 < xm)) && ((ch & 2) ? (t1 > ym) : (t0 < ym)) &&
-                ((ch & 1) ? (t1 > zm) : (t0 < zm))))
-            continue;
-          float sx = bx0 + ((ch & 4) ? xh : 0.0f);
-          float sy = by0 + ((ch & 2) ? yh : 0.0f);
-          float sz = bz0 + ((ch & 1) ? zh : 0.0f);
-          if (fmx(t0, fmx(sx, fmx(sy, sz))) >= fmn(t1, fmn(sx + xh, fmn(sy + yh, sz + zh)))) {
-            cnt.nodes++;  // a call the recursive search makes and prunes at once
-            continue;
-          }
-          later |= 1u << j;
-        }
-        if (later) {  // siblings left: keep this node's walk state
-          float4 *slot = L.base + (size_t)(2 * (sp & 1)) * SGR_TRACE_BLOCK;
-          if (nl == 2) {  // the slot holds entry sp - 2: spill it first
-            st0[sp - 2] = slot[0];
-            st1[sp - 2] = slot[SGR_TRACE_BLOCK];
-          } else {
-            nl++;
-          }
-          slot[0] = make_float4(__uint_as_float(first | (ci << 28)),
-                                __uint_as_float(mask | (later << 8)), bx0, bx1);
-          slot[SGR_TRACE_BLOCK] = make_float4(by0, by1, bz0, bz1);
-          sp++;
-        }
-        cn = first + __popc(mask & ((1u << real) - 1u));
-        bx0 = cx; bx1 = cx1;
-        by0 = cy; by1 = cy1;
-        bz0 = cz; bz1 = cz1;
-        entered = true;
-        break;
-      }
-      // This node is done: resume the nearest pending ancestor.
-      if (sp == 0) break;
-      sp--;
-      float4 a, b;
-      if (nl > 0) {
-        const float4 *slot = L.base + (size_t)(2 * (sp & 1)) * SGR_TRACE_BLOCK;
-        a = slot[0];
-        b = slot[SGR_TRACE_BLOCK];
-        nl--;
-      } else {
-        a = st0[sp];
-        b = st1[sp];
-      }
-      uint32_t w0 = __float_as_uint(a.x), w1 = __float_as_uint(a.y);
-      first = w0 & 0x0fffffffu;
-      ci = w0 >> 28;
-      mask = w1 & 0xffu;
-      vis = (w1 >> 8) & 0xffu;
-      bx0 = a.z; bx1 = a.w; by0 = b.x; by1 = b.y; bz0 = b.z; bz1 = b.w;
-      xh = (bx1 - bx0) / 2.0f;
-      yh = (by1 - by0) / 2.0f;
-      zh = (bz1 - bz0) / 2.0f;
-      xm = bx0 + xh;
-      ym = by0 + yh;
-      zm = bz0 + zh;
-    }
-    if (!entered) break;
-  }
-  return hit;
-}
-
 // ---------------------------------------------------------------------------
-// The same walk as a resumable state machine, for lane-refill traversal:
+// The walk is a resumable state machine, for lane-refill traversal:
 // trav_init sets up a ray (false: it misses the root cube), trav_step
 // enters the pending node and walks to the next node to enter (true: the
-// ray is finished).  The decisions, their order and the counters are
-// those of trace_ray, so results are bit-identical.
+// ray is finished).
 struct TravState {
   RayF ray;
   uint32_t oct, cn, first, mask, vis, ci;
@@ -394,10 +239,10 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
         later |= 1u << j;
       }
       if (later) {
-        float4 *slot = L.base + (size_t)(2 * (T.sp & 1)) * SGR_TRACE_BLOCK;
-        if (T.nl == 2) {
-          st0[T.sp - 2] = slot[0];
-          st1[T.sp - 2] = slot[SGR_TRACE_BLOCK];
+        float4 *slot = L.base + (size_t)(2 * (T.sp % SGR_LDS_SLOTS)) * SGR_TRACE_BLOCK;
+        if (T.nl == SGR_LDS_SLOTS) {  // the slot holds entry sp - SLOTS: spill it
+          st0[T.sp - SGR_LDS_SLOTS] = slot[0];
+          st1[T.sp - SGR_LDS_SLOTS] = slot[SGR_TRACE_BLOCK];
         } else {
           T.nl++;
         }
@@ -417,7 +262,7 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
     T.sp--;
     float4 a, b;
     if (T.nl > 0) {
-      const float4 *slot = L.base + (size_t)(2 * (T.sp & 1)) * SGR_TRACE_BLOCK;
+      const float4 *slot = L.base + (size_t)(2 * (T.sp % SGR_LDS_SLOTS)) * SGR_TRACE_BLOCK;
       a = slot[0];
       b = slot[SGR_TRACE_BLOCK];
       T.nl--;
@@ -437,6 +282,42 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
     T.xm = T.bx0 + T.xh;
     T.ym = T.by0 + T.yh;
     T.zm = T.bz0 + T.zh;
+  }
+}
+
+// Plain driver: lane i of the grid walks rays i, i + G, ... to the end,
+// one at a time (ray ranges by block_range order).  Cheaper per step than
+// trace_refill when walks are short (shallow octrees), where the waves
+// are balanced anyway.
+template<bool ANY, bool MULTI, typename Get, typename Put>
+__device__ __forceinline__ void trace_plain(const SceneView &S1, uint32_t jb, uint32_t je,
+                                            Get get, Put put, TraceCount &cnt, LdsStack L) {
+  TravState T;
+  float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
+  for (uint32_t i = jb + threadIdx.x; i < je; i += blockDim.x) {
+    const SceneView *g = get(i, T.ray);
+    const SceneView &S = MULTI ? *g : S1;
+    if (S.use_triv) {
+      T.hit = SGR_MISS;
+      T.hu = 0.0f;
+      T.hv = 0.0f;
+      float t, u, v;
+      uint32_t prim;
+      for (uint32_t q = 0; q < S.n_surfs; q++) {
+        cnt.tests++;
+        if (ref_hit(S.refs, q, T.ray, t, u, v, prim)) {
+          T.hit = prim;
+          if (ANY) break;
+          T.ray.t1 = t;
+          T.hu = u;
+          T.hv = v;
+        }
+      }
+    } else if (trav_init(S, T, cnt)) {
+      while (!trav_step<ANY>(S, T, cnt, L, st0, st1)) {
+      }
+    }
+    put(i, T);
   }
 }
 

@@ -486,6 +486,10 @@ int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
   sc->n_lights = H.V.n_lights;
   SceneView &V = sc->view;
   V = H.V;
+  // Lane refill pays off once walks are long enough to leave waves
+  // unbalanced: on the 1.31 M-triangle soup (depth 19) it is 16 % faster,
+  // on the Cornell box (depth 5) 30 % slower than one ray per lane.
+  V.walk_refill = sc->oct.max_depth >= 8 ? 1u : 0u;
   std::vector<uint4> nodes4(V.n_nodes);
   for (uint32_t i = 0; i < V.n_nodes; i++)
     nodes4[i] = make_uint4(sc->oct.nodes[4 * i], sc->oct.nodes[4 * i + 1],

@@ -28,16 +28,42 @@ namespace sgr {
 // ---------------------------------------------------------------------------
 // Ray batches
 // ---------------------------------------------------------------------------
+// Queue range of this block.  Hardware deals workgroups to the 8 XCDs
+// round-robin; remapping so that the blocks of one XCD take one
+// contiguous slice of the queue keeps neighbouring rays (neighbouring
+// pixels) on the same L2.  The grid is a multiple of 8.
+__device__ __forceinline__ void block_range(uint32_t n, uint32_t &beg, uint32_t &end) {
+  const uint32_t G = gridDim.x;
+  const uint32_t lb = (blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
+  const uint32_t per = (n + G - 1) / G;
+  beg = lb * per;
+  end = beg + per < n ? beg + per : n;
+}
+
+// Walk the N rays of a launch: lane refill over XCD slices, or one ray
+// per lane over this block's queue range.
+template<bool ANY, bool MULTI, bool REFILL, typename Get, typename Put>
+__device__ __forceinline__ void walk(const SceneView &S1, uint32_t n, uint32_t *heads, Get get,
+                                     Put put, TraceCount &cnt, LdsStack L) {
+  if (REFILL) {
+    trace_refill<ANY, MULTI>(S1, n, heads, get, put, cnt, L);
+  } else {
+    uint32_t jb, je;
+    block_range(n, jb, je);
+    trace_plain<ANY, MULTI>(S1, jb, je, get, put, cnt, L);
+  }
+}
+
 // Ray-batch queries with lane-refill traversal (heads[0..7], zeroed by
 // the caller, are the fetch heads of the 8 slices of the batch).
-template<bool ANY>
+template<bool ANY, bool REFILL>
 __global__ void __launch_bounds__(256) k_trace(SceneView S, RayBatchDev rb, uint32_t n,
                                                unsigned long long *stats) {
-  __shared__ float4 stk_lds[4 * SGR_TRACE_BLOCK];
+  __shared__ float4 stk_lds[2 * SGR_LDS_SLOTS * SGR_TRACE_BLOCK];
   const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
   uint32_t *heads = (uint32_t *)(stats + SGR_NUM_STATS);
-  trace_refill<ANY, false>(
+  walk<ANY, false, REFILL>(
       S, n, heads,
       [&](uint32_t i, RayF &r) -> const SceneView * {
         r.o = v3(rb.ox[i], rb.oy[i], rb.oz[i]);
@@ -59,18 +85,6 @@ __global__ void __launch_bounds__(256) k_trace(SceneView S, RayBatchDev rb, uint
       cnt, stk);
   wave_count(&stats[0], cnt.nodes);
   wave_count(&stats[1], cnt.tests);
-}
-
-// Queue range of this block.  Hardware deals workgroups to the 8 XCDs
-// round-robin; remapping so that the blocks of one XCD take one
-// contiguous slice of the queue keeps neighbouring rays (neighbouring
-// pixels) on the same L2.  The grid is a multiple of 8.
-__device__ __forceinline__ void block_range(uint32_t n, uint32_t &beg, uint32_t &end) {
-  const uint32_t G = gridDim.x;
-  const uint32_t lb = (blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
-  const uint32_t per = (n + G - 1) / G;
-  beg = lb * per;
-  end = beg + per < n ? beg + per : n;
 }
 
 // ---------------------------------------------------------------------------
@@ -153,15 +167,15 @@ __global__ void __launch_bounds__(256) k_raygen(RenderView R1, StepView J, PathB
   }
 }
 
-template<bool MULTI>
+template<bool MULTI, bool REFILL>
 __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                 unsigned long long *stats) {
-  __shared__ float4 stk_lds[4 * SGR_TRACE_BLOCK];
+  __shared__ float4 stk_lds[2 * SGR_LDS_SLOTS * SGR_TRACE_BLOCK];
   const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   const uint32_t *q = P.q[b & 1];
-  trace_refill<false, MULTI>(
+  walk<false, MULTI, REFILL>(
       S1, n, &P.counts[SGR_CNT_HE + 8 * b],
       [&](uint32_t i, RayF &r) -> const SceneView * {
         const uint32_t p = q[i];
@@ -180,14 +194,14 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
   wave_count(&stats[SGR_STAT_TESTS_C], cnt.tests);
 }
 
-template<bool MULTI>
+template<bool MULTI, bool REFILL>
 __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                  unsigned long long *stats) {
-  __shared__ float4 stk_lds[4 * SGR_TRACE_BLOCK];
+  __shared__ float4 stk_lds[2 * SGR_LDS_SLOTS * SGR_TRACE_BLOCK];
   const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_S + b];
-  trace_refill<true, MULTI>(
+  walk<true, MULTI, REFILL>(
       S1, n, &P.counts[SGR_CNT_HS + 8 * b],
       [&](uint32_t i, RayF &r) -> const SceneView * {
         const uint32_t slot = P.sq[i];
@@ -646,8 +660,13 @@ void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool an
                   unsigned long long *stats, hipStream_t st) {
   unsigned g = grid_for(n, 65535);
   g = (g + 7) & ~7u;  // whole blocks per XCD slice
-  if (any) hipLaunchKernelGGL(k_trace<true>, dim3(g), dim3(256), 0, st, S, rb, n, stats);
-  else hipLaunchKernelGGL(k_trace<false>, dim3(g), dim3(256), 0, st, S, rb, n, stats);
+  if (S.walk_refill) {
+    if (any) hipLaunchKernelGGL((k_trace<true, true>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
+    else hipLaunchKernelGGL((k_trace<false, true>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
+  } else {
+    if (any) hipLaunchKernelGGL((k_trace<true, false>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
+    else hipLaunchKernelGGL((k_trace<false, false>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
+  }
 }
 
 void launch_raygen(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
@@ -657,8 +676,12 @@ void launch_raygen(bool multi, const RenderView &R, const StepView &J, const Pat
 }
 void launch_extend(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                    uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
-  if (multi) hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
-  else hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  if (multi)
+    hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else if (S.walk_refill)
+    hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else
+    hipLaunchKernelGGL((k_extend<false, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
 }
 void launch_shade(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
                   const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
@@ -678,8 +701,12 @@ void launch_shade(bool multi, const SceneView &S, const RenderView &R, const Ste
 }
 void launch_occlude(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                     uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
-  if (multi) hipLaunchKernelGGL(k_occlude<true>, dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
-  else hipLaunchKernelGGL(k_occlude<false>, dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  if (multi)
+    hipLaunchKernelGGL((k_occlude<true, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else if (S.walk_refill)
+    hipLaunchKernelGGL((k_occlude<false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else
+    hipLaunchKernelGGL((k_occlude<false, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
 }
 void launch_gather(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
                    uint32_t b, unsigned grid, hipStream_t st, const GuideView *G) {
