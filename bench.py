@@ -330,6 +330,9 @@ def batch_bench(args):
     return line
 
 
+SERVE_PATHS = 1 << 26   # engine path capacity of the serving loop
+
+
 def serve_bench(args):
     from snogray_amd import native, serve
     ctx = native.Context(0)
@@ -342,7 +345,8 @@ def serve_bench(args):
             checker = cpu.pixel_check
         log(f"serving at {rate} jobs/s for {args.duration} s")
         rep = serve.serve_poisson(ctx, rate, args.duration, seed=args.seed, check_fn=checker,
-                                  log=log)
+                                  log=log, drain_limit=args.drain_limit,
+                                  path_capacity=args.paths_in_flight or SERVE_PATHS)
         lines.append({
             "metric": "served samples/s; p50/p99 job latency (Poisson arrivals)",
             "value": round(rep["samples_per_s"], 1), "unit": "samples/s", "n_gpus": 1,
@@ -353,9 +357,11 @@ def serve_bench(args):
             "config": {"workload": f"Poisson {rate} jobs/s for {args.duration} s, <=32 resident "
                                    "scenes (prewarmed) of a 48-scene pool, admission by HBM "
                                    "footprint",
+                       "path_capacity": args.paths_in_flight or SERVE_PATHS,
                        "parallelism": "single GPU"},
             "p50_ms": round(rep["p50_ms"], 1), "p99_ms": round(rep["p99_ms"], 1),
-            "jobs": rep["jobs"], "wall_s": round(rep["wall_s"], 2),
+            "jobs": rep["jobs"], "unfinished": rep["unfinished"],
+            "wall_s": round(rep["wall_s"], 2),
             "max_active_jobs": rep["max_active_jobs"],
             "prewarm_s": round(rep["prewarm_s"], 1),
             "peak_budget_gb": round(rep["peak_budget_bytes"] / 2 ** 30, 2),
@@ -489,6 +495,9 @@ def main():
     ap.add_argument("--paths-in-flight", type=int, default=0,
                     help="paths per chunk of a frame (0: the library default)")
     ap.add_argument("--duration", type=float, default=120.0)
+    ap.add_argument("--drain-limit", type=float, default=60.0,
+                    help="serve: seconds after the last arrival before unfinished jobs "
+                         "are counted and the run ends")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one plain 256 MiB device copy (reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")

@@ -62,7 +62,7 @@ struct Engine {
   StepTables *dev = nullptr;
   unsigned long long *stats = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  std::map<uint32_t, hipGraphExec_t> graphs;  // by bounce count
+  std::map<uint32_t, hipGraphExec_t> graphs;  // by bounce count | refill << 31
   std::map<uint64_t, Job> jobs;
   std::deque<uint64_t> queued;   // admission order
   std::vector<uint64_t> active;  // round-robin order
@@ -159,7 +159,7 @@ uint32_t splat_tiles(const RenderView &R) {
   return (uint32_t)((ow + 15) / 16) * (uint32_t)((oh + 15) / 16);
 }
 
-void enqueue_step(sgr_ctx *ctx, Engine *E, uint32_t maxp, unsigned grid) {
+void enqueue_step(sgr_ctx *ctx, Engine *E, uint32_t maxp, bool refill, unsigned grid) {
   StepView J;
   J.R = E->dev->R;
   J.S = E->dev->S;
@@ -170,6 +170,7 @@ void enqueue_step(sgr_ctx *ctx, Engine *E, uint32_t maxp, unsigned grid) {
   std::memset(&R0, 0, sizeof R0);
   SceneView S0;
   std::memset(&S0, 0, sizeof S0);
+  S0.walk_refill = refill ? 1u : 0u;  // selects the walk of the multi-scene kernels
   const PathBufs &P = E->P;
   hipStream_t st = ctx->stream;
   launch_raygen(true, R0, J, P, grid, st);
@@ -338,6 +339,7 @@ int sgr_engine_step(sgr_ctx *ctx, sgr_step_info *info) {
   uint64_t paths = 0;
   size_t K = 1;
   uint32_t maxp = 0;
+  bool refill = false;  // any scene of the step deep enough for lane refill
   size_t n = E->active.size();
   size_t stop = n;
   for (size_t k = 0; k < n && run.size() < SGR_MAX_SEGMENTS; k++) {
@@ -354,6 +356,7 @@ int sgr_engine_step(sgr_ctx *ctx, sgr_step_info *info) {
     paths += np;
     K = std::max<size_t>(K, j.pl.R.K);
     maxp = std::max(maxp, j.pl.R.maxp);
+    refill = refill || j.sc->view.walk_refill;
   }
   if (run.empty()) return SGR_OK;
   // next step starts with the first job that did not fit, else the job
@@ -396,11 +399,12 @@ int sgr_engine_step(sgr_ctx *ctx, sgr_step_info *info) {
   HIPCHK(hipEventRecord(E->e0, ctx->stream));
   bool graphed = false;
   if (E->cfg.use_graph) {
-    auto it = E->graphs.find(maxp);
+    const uint32_t key = maxp | (refill ? 1u << 31 : 0u);
+    auto it = E->graphs.find(key);
     if (it == E->graphs.end()) {
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_step(ctx, E, maxp, grid);
+      enqueue_step(ctx, E, maxp, refill, grid);
       hipError_t le = hipGetLastError();
       hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
       if (le != hipSuccess) return fail(SGR_ERR_HIP, std::string("step capture: ") + hipGetErrorString(le));
@@ -409,12 +413,12 @@ int sgr_engine_step(sgr_ctx *ctx, sgr_step_info *info) {
       hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (ie != hipSuccess) return fail(SGR_ERR_HIP, std::string("step graph: ") + hipGetErrorString(ie));
-      it = E->graphs.emplace(maxp, x).first;
+      it = E->graphs.emplace(key, x).first;
     }
     HIPCHK(hipGraphLaunch(it->second, ctx->stream));
     graphed = true;
   } else {
-    enqueue_step(ctx, E, maxp, grid);
+    enqueue_step(ctx, E, maxp, refill, grid);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(E->e1, ctx->stream));

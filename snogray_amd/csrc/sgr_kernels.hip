@@ -676,8 +676,10 @@ void launch_raygen(bool multi, const RenderView &R, const StepView &J, const Pat
 }
 void launch_extend(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                    uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
-  if (multi)
+  if (multi && S.walk_refill)
     hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else if (multi)
+    hipLaunchKernelGGL((k_extend<true, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else if (S.walk_refill)
     hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else
@@ -701,8 +703,10 @@ void launch_shade(bool multi, const SceneView &S, const RenderView &R, const Ste
 }
 void launch_occlude(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                     uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
-  if (multi)
+  if (multi && S.walk_refill)
     hipLaunchKernelGGL((k_occlude<true, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else if (multi)
+    hipLaunchKernelGGL((k_occlude<true, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else if (S.walk_refill)
     hipLaunchKernelGGL((k_occlude<false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else

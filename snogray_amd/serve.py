@@ -100,6 +100,25 @@ class ResidencyPlanner:
         assert self.scenes[key][1] == 0, "scene still has active jobs"
         del self.scenes[key]
 
+    def drain_target(self, n_starving):
+        """How many scenes to keep draining while N_STARVING scenes that
+        waiting jobs need find no room: one per starving scene, at most a
+        quarter of the resident scenes (at least one), so that most
+        scenes keep taking jobs while room is made."""
+        return min(int(n_starving), max(1, len(self.scenes) // 4))
+
+    def drain_victims(self, n, demand, draining=(), protect=()):
+        """N resident scenes to stop admitting to, so that their active
+        jobs finish and the scenes become evictable for scenes that
+        waiting jobs need.  Scenes that no waiting job wants (DEMAND:
+        key -> number of waiting jobs) go first, then the least
+        recently admitted; scenes already DRAINING or in PROTECT are
+        never chosen."""
+        cand = [k for k in self.scenes if k not in draining and k not in protect]
+        # stable sort: LRU order within each demand class
+        cand.sort(key=lambda k: demand.get(k, 0) > 0)
+        return cand[:max(0, int(n))]
+
     def can_admit(self, key, nbytes):
         return key in self.scenes and self.used() + nbytes <= self.budget
 
@@ -158,7 +177,7 @@ class Server:
     """Drives one context's job engine for a stream of (arrival time,
     JobSpec) requests; see the module docstring."""
 
-    def __init__(self, ctx, budget, max_scenes=32, path_capacity=1 << 24, check_every=50,
+    def __init__(self, ctx, budget, max_scenes=32, path_capacity=1 << 26, check_every=50,
                  check_fn=None, use_graph=True, builders=6):
         self.ctx = ctx
         self.engine = native.Engine(ctx, path_capacity=path_capacity, use_graph=use_graph)
@@ -173,6 +192,9 @@ class Server:
         self.check_every = check_every if check_fn else 0
         self.check_fn = check_fn  # (job, scene desc, accumulators) -> dict
         self.descs = {}
+        # resident scenes that take no new jobs until their active jobs
+        # finish and they can be evicted for a scene a waiting job needs
+        self.draining = set()
 
     def _build(self, spec):
         sd = self.host_descs.get(spec)
@@ -204,10 +226,13 @@ class Server:
                 raise RuntimeError("prewarmed scenes exceed the budget")
         return time.perf_counter() - t0
 
-    def run(self, requests, drain=True, log=None, keep_results=False):
+    def run(self, requests, drain=True, log=None, keep_results=False, drain_limit=None):
         """REQUESTS: list of (arrival seconds, JobSpec), sorted by time.
         Returns a report dict (with every job's accumulators under
-        "results" if KEEP_RESULTS)."""
+        "results" if KEEP_RESULTS).  With DRAIN the loop runs until every
+        request has finished, or at most DRAIN_LIMIT seconds after the
+        last arrival; requests still unfinished then are counted under
+        "unfinished"."""
         results = {}
         reqs = collections.deque((a, i, job) for i, (a, job) in enumerate(requests))
         pending = collections.deque()   # (arrival, index, spec)
@@ -219,8 +244,13 @@ class Server:
         step_ms = 0.0
         max_active = 0
         t_report = 0.0
+        t_last = requests[-1][0] if requests else 0.0
+        unfinished = 0
         while reqs or pending or active or self.building:
             now = time.perf_counter() - t0
+            if drain_limit is not None and not reqs and now > t_last + drain_limit:
+                unfinished = len(pending) + len(active)
+                break
             if log and now - t_report >= 10.0:
                 t_report = now
                 log(f"serving t={now:.0f}s: {len(lat)} done, {len(active)} active, "
@@ -248,11 +278,12 @@ class Server:
             # not block later jobs whose scenes are resident
             keep = collections.deque()
             blocked_mem = False
+            starving = set()    # scenes waiting jobs need that find no room
             while pending:
                 a, idx, job = pending.popleft()
                 spec = job.scene
                 nbytes = native.Engine.footprint(job.params())
-                if spec in self.scenes and not blocked_mem:
+                if spec in self.scenes and spec not in self.draining and not blocked_mem:
                     if self.planner.can_admit(spec, nbytes):
                         self.planner.admit(spec, nbytes)
                         jid = self.engine.submit(self.scenes[spec], job.camera(), job.params())
@@ -265,14 +296,30 @@ class Server:
                     # scenes being built or wanted by waiting jobs stay
                     wanted = set(self.building) | {j.scene for _, _, j in pending}
                     wanted |= {j.scene for _, _, j in keep}
+                    wanted -= self.draining
                     ev = self.planner.plan_scene(spec, spec.estimated_bytes(), protect=wanted)
                     if ev is not None:
                         for k in ev:
                             self._evict(k)
                         self.planner.add_scene(spec, spec.estimated_bytes())
                         self.building[spec] = self.builder.submit(self._build, spec)
+                    else:
+                        starving.add(spec)
                 keep.append((a, idx, job))
             pending = keep
+            # Without this, scenes whose jobs keep arriving never go idle
+            # and a job on a scene that is not resident waits for ever.
+            # Once fewer scenes starve, the most wanted drained scenes take
+            # jobs again.
+            need = self.planner.drain_target(len(starving)) - len(self.draining)
+            if need:
+                demand = collections.Counter(j.scene for _, _, j in pending)
+                if need > 0:
+                    self.draining.update(self.planner.drain_victims(
+                        need, demand, self.draining, protect=set(self.building)))
+                else:
+                    back = sorted(self.draining, key=lambda k: -demand.get(k, 0))[:-need]
+                    self.draining.difference_update(back)
             max_active = max(max_active, len(active))
             if active:
                 si = self.engine.step()
@@ -321,11 +368,13 @@ class Server:
             "peak_budget_bytes": self.planner.peak, "budget_bytes": self.planner.budget,
             "resident_scenes_max": self.planner.max_scenes,
             "checks": checks,
+            "unfinished": unfinished,
             **({"results": results} if keep_results else {}),
         }
 
     def _evict(self, spec):
         self.planner.evict(spec)
+        self.draining.discard(spec)
         sc = self.scenes.pop(spec, None)
         if sc is not None:
             sc.close()
@@ -340,7 +389,7 @@ class Server:
 
 
 def serve_poisson(ctx, rate, duration, n_scene_pool=48, seed=1, budget=None, log=None,
-                  prewarm=True, **kw):
+                  prewarm=True, drain_limit=None, **kw):
     """Poisson arrivals at RATE jobs/s for DURATION s, jobs drawn from the
     batched-job distribution over a pool of scenes.  With PREWARM the
     first max_scenes scenes of the pool are made resident before the
@@ -354,7 +403,7 @@ def serve_poisson(ctx, rate, duration, n_scene_pool=48, seed=1, budget=None, log
     srv = Server(ctx, budget, **kw)
     try:
         t_warm = srv.prewarm(pool) if prewarm else 0.0
-        rep = srv.run(list(zip(times, jobs)), log=log)
+        rep = srv.run(list(zip(times, jobs)), log=log, drain_limit=drain_limit)
     finally:
         srv.close()
     rep.update({"rate_jobs_per_s": rate, "duration_s": duration, "scene_pool": n_scene_pool,

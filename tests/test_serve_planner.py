@@ -77,3 +77,73 @@ def test_pixel_test_accepts_same_distribution_and_rejects_shift():
     assert ok and frac > 0.97
     frac, ok = pixel_test(gpu * 1.5, cpu)
     assert not ok
+
+
+def test_drain_victims_prefer_unwanted_then_least_recent():
+    P = ResidencyPlanner(10_000, max_scenes=4)
+    for k in "abcd":
+        P.add_scene(k, 100)
+    for k in "dcba":          # admission order: a most recent, d least
+        P.admit(k, 10)
+    assert P.drain_victims(1, {}) == ["d"]
+    assert P.drain_victims(2, {"d": 3, "b": 1}) == ["c", "a"]
+    assert P.drain_victims(3, {"d": 3, "b": 1}) == ["c", "a", "d"]
+    assert P.drain_victims(2, {}, draining={"d"}, protect={"c"}) == ["b", "a"]
+    assert P.drain_victims(0, {}) == []
+    assert P.drain_target(0) == 0 and P.drain_target(3) == 1
+    Q = ResidencyPlanner(10_000, max_scenes=16)
+    for k in range(12):
+        Q.add_scene(k, 100)
+    assert Q.drain_target(2) == 2 and Q.drain_target(9) == 3
+
+
+def test_drained_scenes_free_room_for_starving_jobs():
+    """Simulated serving in which every resident scene stays busy: jobs
+    arrive for 5 scenes (2 resident at most) faster than a scene's jobs
+    finish, so no resident scene ever goes idle by itself.  With the
+    drain policy no job waits long for its scene; without it the jobs of
+    the other scenes wait until arrivals stop."""
+    rng = np.random.default_rng(5)
+    P = ResidencyPlanner(100_000, max_scenes=2)
+    draining, pending, active = set(), [], []
+    served, max_wait = set(), 0
+    for t in range(4000):
+        if t < 3000:
+            pending.append((int(rng.integers(5)), t))
+        active = [(k, r - 1) for k, r in active]
+        for k, r in active:
+            if r <= 0:
+                P.release(k, 10)
+                served.add(k)
+        active = [(k, r) for k, r in active if r > 0]
+        keep, starving = [], set()
+        for k, t0 in pending:
+            if P.resident(k) and k not in draining and P.can_admit(k, 10):
+                P.admit(k, 10)
+                active.append((k, int(rng.integers(20, 40))))
+                max_wait = max(max_wait, t - t0)
+                continue
+            if not P.resident(k):
+                wanted = {j for j, _ in pending} - draining
+                ev = P.plan_scene(k, 100, protect=wanted - {k})
+                if ev is None:
+                    starving.add(k)
+                else:
+                    for e in ev:
+                        assert all(a != e for a, _ in active)   # only idle scenes
+                        P.evict(e)
+                        draining.discard(e)
+                    P.add_scene(k, 100)
+            keep.append((k, t0))
+        pending = keep
+        need = P.drain_target(len(starving)) - len(draining)
+        demand = {}
+        for k, _ in pending:
+            demand[k] = demand.get(k, 0) + 1
+        if need > 0:
+            draining.update(P.drain_victims(need, demand, draining))
+        elif need < 0:
+            draining.difference_update(sorted(draining, key=lambda k: -demand.get(k, 0))[:-need])
+    assert served == set(range(5))
+    assert not pending and not active
+    assert max_wait < 400, max_wait
