@@ -110,8 +110,9 @@ __device__ __forceinline__ uint32_t visit_mask(uint32_t mask, uint32_t oct) {
 // children in perturbed parametric order, pruning a child by its
 // parametric interval against the live ray extent (ray.t1 shrinks on
 // every closer hit).  Here the current node lives in registers and a
-// parent is pushed (first child | next position, child masks, parametric
-// bounds) only when it still has children left to visit.  A child is
+// parent is pushed (first child, child mask and the siblings still to
+// visit, parametric bounds) only when it still has children left to
+// visit.  A child is
 // tested against the same prune conditions, with the same ray extent,
 // at the same point of the walk, before it is entered, so the surfaces
 // are tested in the reference's order and the first of several equal-t
@@ -123,8 +124,8 @@ __device__ __forceinline__ uint32_t visit_mask(uint32_t mask, uint32_t oct) {
 // LDS (slot i of the lane at lds[(2 * i + half) * SGR_TRACE_BLOCK]);
 // older ones are spilled to the lane's private array when a newer one
 // needs their slot and read back from there when popped.  Entries are
-// 2 x float4: (first | child cursor << 28, child mask | pending siblings
-// << 8, x slab) and (y slab, z slab).
+// 2 x float4: (first child, child mask | pending siblings << 8, x slab)
+// and (y slab, z slab); the pending siblings are a visit-order mask.
 #define SGR_TRACE_BLOCK 256
 #define SGR_LDS_SLOTS 3
 struct LdsStack {
@@ -134,13 +135,15 @@ struct LdsStack {
 // ---------------------------------------------------------------------------
 // The walk is a resumable state machine, for lane-refill traversal:
 // trav_init sets up a ray (false: it misses the root cube), trav_step
-// enters the pending node and walks to the next node to enter (true: the
-// ray is finished).
+// does one step (true: the ray is finished).  A step enters the pending
+// node (its surfaces, in list order) unless the last step popped it,
+// then picks the child to enter next or pops one stack entry.  Every
+// step thus costs one node load at most and one evaluation of the
+// eight children, so the lanes of a wave stay on the same instructions.
 struct TravState {
   RayF ray;
-  uint32_t oct, cn, first, mask, vis, ci;
+  uint32_t oct, cn, first, mask, vis;
   float bx0, bx1, by0, by1, bz0, bz1;
-  float xh, yh, zh, xm, ym, zm;
   int sp, nl;
   bool enter;
   uint32_t hit;
@@ -173,6 +176,45 @@ __device__ __forceinline__ bool trav_init(const SceneView &S, TravState &T, Trac
   return true;
 }
 
+// Visit-order bit i <-> child index (parametric order with 3 and 4
+// swapped): exchange bits 3 and 4 of an 8-bit child mask.
+__device__ __forceinline__ uint32_t swap34(uint32_t m) {
+  return (m & 0xe7u) | ((m >> 1) & 0x08u) | ((m << 1) & 0x10u);
+}
+
+// The children of the current node the recursive search would call
+// (C1: the child-call test of space/octree.cc:269-272 holds) and enter
+// (OK: the entry test of :186-188 on the child slabs of :276-291), for
+// all eight children at once, as visit-order bit masks.  The operands and the
+// order of the fmx / fmn chains are those of the per-child tests (child
+// slab = low half [b0 + 0, (b0 + 0) + h] or high half [b0 + h,
+// (b0 + h) + h], h = (b1 - b0) / 2), so the decisions are the same bit
+// for bit, NaN operands included.
+__device__ __forceinline__ void child_masks(const TravState &T, uint32_t &c1, uint32_t &ok) {
+  const float t0 = T.ray.t0, t1 = T.ray.t1;
+  const float xh = (T.bx1 - T.bx0) / 2.0f, yh = (T.by1 - T.by0) / 2.0f,
+              zh = (T.bz1 - T.bz0) / 2.0f;
+  const float xm = T.bx0 + xh, ym = T.by0 + yh, zm = T.bz0 + zh;
+  const float xl = T.bx0 + 0.0f, yl = T.by0 + 0.0f, zl = T.bz0 + 0.0f;
+  const float xe[2] = {xl + xh, xm + xh}, ye[2] = {yl + yh, ym + yh}, ze[2] = {zl + zh, zm + zh};
+  const float xs[2] = {xl, xm}, ys[2] = {yl, ym}, zs[2] = {zl, zm};
+  uint32_t m = 0;
+#pragma unroll
+  for (int yz = 0; yz < 4; yz++) {
+    const float a = fmx(ys[yz >> 1], zs[yz & 1]), b = fmn(ye[yz >> 1], ze[yz & 1]);
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      const bool in = !(fmx(t0, fmx(xs[x], a)) >= fmn(t1, fmn(xe[x], b)));
+      m |= in ? 1u << (x * 4 + yz) : 0u;
+    }
+  }
+  const uint32_t cx = (t0 < xm ? 0x0fu : 0u) | (t1 > xm ? 0xf0u : 0u);
+  const uint32_t cy = (t0 < ym ? 0x33u : 0u) | (t1 > ym ? 0xccu : 0u);
+  const uint32_t cz = (t0 < zm ? 0x55u : 0u) | (t1 > zm ? 0xaau : 0u);
+  c1 = swap34(cx & cy & cz);
+  ok = swap34(m);
+}
+
 template<bool ANY>
 __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, TraceCount &cnt,
                                           LdsStack L, float4 *st0, float4 *st1) {
@@ -193,96 +235,63 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
     T.mask = nd.y;
     T.first = nd.x;
     T.vis = visit_mask(T.mask, T.oct);
-    T.ci = 0;
-    if (T.vis) {
-      T.xh = (T.bx1 - T.bx0) / 2.0f;
-      T.yh = (T.by1 - T.by0) / 2.0f;
-      T.zh = (T.bz1 - T.bz0) / 2.0f;
-      T.xm = T.bx0 + T.xh;
-      T.ym = T.by0 + T.yh;
-      T.zm = T.bz0 + T.zh;
-    }
-    T.enter = false;
   }
-  for (;;) {
-    const uint32_t rest = T.vis >> T.ci;
-    if (rest) {
-      const uint32_t i = T.ci + (uint32_t)__builtin_ctz(rest);
-      T.ci = i + 1;
-      const uint32_t child = (i == 3) ? 4u : (i == 4) ? 3u : i;
-      const float t0 = T.ray.t0, t1 = T.ray.t1;
-      const float xm = T.xm, ym = T.ym, zm = T.zm, xh = T.xh, yh = T.yh, zh = T.zh;
-      if (!(((child & 4) ? (t1 > xm) : (t0 < xm)) && ((child & 2) ? (t1 > ym) : (t0 < ym)) &&
-            ((child & 1) ? (t1 > zm) : (t0 < zm))))
-        continue;
-      const float cx = T.bx0 + ((child & 4) ? xh : 0.0f);
-      const float cy = T.by0 + ((child & 2) ? yh : 0.0f);
-      const float cz = T.bz0 + ((child & 1) ? zh : 0.0f);
-      const float cx1 = cx + xh, cy1 = cy + yh, cz1 = cz + zh;
-      cnt.nodes++;
-      if (fmx(t0, fmx(cx, fmx(cy, cz))) >= fmn(t1, fmn(cx1, fmn(cy1, cz1)))) continue;
-      const uint32_t real = child ^ T.oct;
-      uint32_t later = 0;
-      for (uint32_t r2 = T.vis >> T.ci; r2; r2 &= r2 - 1) {
-        const uint32_t j = T.ci + (uint32_t)__builtin_ctz(r2);
-        const uint32_t ch = (j == 3) ? 4u : (j == 4) ? 3u : j;
-        if (!(((ch & 4) ? (t1 > xm) : (t0 < xm)) && ((ch & 2) ? (t1 > ym) : (t0 < ym)) &&
-              ((ch & 1) ? (t1 > zm) : (t0 < zm))))
-          continue;
-        const float sx = T.bx0 + ((ch & 4) ? xh : 0.0f);
-        const float sy = T.by0 + ((ch & 2) ? yh : 0.0f);
-        const float sz = T.bz0 + ((ch & 1) ? zh : 0.0f);
-        if (fmx(t0, fmx(sx, fmx(sy, sz))) >= fmn(t1, fmn(sx + xh, fmn(sy + yh, sz + zh)))) {
-          cnt.nodes++;
-          continue;
-        }
-        later |= 1u << j;
+  // Children still to visit that the search would call (counted) and
+  // enter, with the ray extent as it is now.  Siblings after the one
+  // entered are tested again when their entry is popped: the extent can
+  // only shrink, so one that fails now would fail then too.
+  uint32_t c1, ok;
+  child_masks(T, c1, ok);
+  const uint32_t cand = T.vis & c1, go = cand & ok;
+  cnt.nodes += (uint32_t)__popc(cand & ~ok) + (go ? 1u : 0u);
+  if (go) {
+    const uint32_t i = (uint32_t)__builtin_ctz(go), later = go & (go - 1u);
+    if (later) {
+      float4 *slot = L.base + (size_t)(2 * (T.sp % SGR_LDS_SLOTS)) * SGR_TRACE_BLOCK;
+      if (T.nl == SGR_LDS_SLOTS) {  // the slot holds entry sp - SLOTS: spill it
+        st0[T.sp - SGR_LDS_SLOTS] = slot[0];
+        st1[T.sp - SGR_LDS_SLOTS] = slot[SGR_TRACE_BLOCK];
+      } else {
+        T.nl++;
       }
-      if (later) {
-        float4 *slot = L.base + (size_t)(2 * (T.sp % SGR_LDS_SLOTS)) * SGR_TRACE_BLOCK;
-        if (T.nl == SGR_LDS_SLOTS) {  // the slot holds entry sp - SLOTS: spill it
-          st0[T.sp - SGR_LDS_SLOTS] = slot[0];
-          st1[T.sp - SGR_LDS_SLOTS] = slot[SGR_TRACE_BLOCK];
-        } else {
-          T.nl++;
-        }
-        slot[0] = make_float4(__uint_as_float(T.first | (T.ci << 28)),
-                              __uint_as_float(T.mask | (later << 8)), T.bx0, T.bx1);
-        slot[SGR_TRACE_BLOCK] = make_float4(T.by0, T.by1, T.bz0, T.bz1);
-        T.sp++;
-      }
-      T.cn = T.first + __popc(T.mask & ((1u << real) - 1u));
-      T.bx0 = cx; T.bx1 = cx1;
-      T.by0 = cy; T.by1 = cy1;
-      T.bz0 = cz; T.bz1 = cz1;
-      T.enter = true;
-      return false;
+      slot[0] = make_float4(__uint_as_float(T.first), __uint_as_float(T.mask | (later << 8)),
+                            T.bx0, T.bx1);
+      slot[SGR_TRACE_BLOCK] = make_float4(T.by0, T.by1, T.bz0, T.bz1);
+      T.sp++;
     }
-    if (T.sp == 0) return true;
-    T.sp--;
-    float4 a, b;
-    if (T.nl > 0) {
-      const float4 *slot = L.base + (size_t)(2 * (T.sp % SGR_LDS_SLOTS)) * SGR_TRACE_BLOCK;
-      a = slot[0];
-      b = slot[SGR_TRACE_BLOCK];
-      T.nl--;
-    } else {
-      a = st0[T.sp];
-      b = st1[T.sp];
-    }
-    const uint32_t w0 = __float_as_uint(a.x), w1 = __float_as_uint(a.y);
-    T.first = w0 & 0x0fffffffu;
-    T.ci = w0 >> 28;
-    T.mask = w1 & 0xffu;
-    T.vis = (w1 >> 8) & 0xffu;
-    T.bx0 = a.z; T.bx1 = a.w; T.by0 = b.x; T.by1 = b.y; T.bz0 = b.z; T.bz1 = b.w;
-    T.xh = (T.bx1 - T.bx0) / 2.0f;
-    T.yh = (T.by1 - T.by0) / 2.0f;
-    T.zh = (T.bz1 - T.bz0) / 2.0f;
-    T.xm = T.bx0 + T.xh;
-    T.ym = T.by0 + T.yh;
-    T.zm = T.bz0 + T.zh;
+    const uint32_t child = (i == 3) ? 4u : (i == 4) ? 3u : i;
+    const uint32_t real = child ^ T.oct;
+    const float xh = (T.bx1 - T.bx0) / 2.0f, yh = (T.by1 - T.by0) / 2.0f,
+                zh = (T.bz1 - T.bz0) / 2.0f;
+    const float cx = T.bx0 + ((child & 4) ? xh : 0.0f);
+    const float cy = T.by0 + ((child & 2) ? yh : 0.0f);
+    const float cz = T.bz0 + ((child & 1) ? zh : 0.0f);
+    T.cn = T.first + __popc(T.mask & ((1u << real) - 1u));
+    T.bx0 = cx; T.bx1 = cx + xh;
+    T.by0 = cy; T.by1 = cy + yh;
+    T.bz0 = cz; T.bz1 = cz + zh;
+    T.enter = true;
+    return false;
   }
+  if (T.sp == 0) return true;
+  T.sp--;
+  float4 a, b;
+  if (T.nl > 0) {
+    const float4 *slot = L.base + (size_t)(2 * (T.sp % SGR_LDS_SLOTS)) * SGR_TRACE_BLOCK;
+    a = slot[0];
+    b = slot[SGR_TRACE_BLOCK];
+    T.nl--;
+  } else {
+    a = st0[T.sp];
+    b = st1[T.sp];
+  }
+  const uint32_t w1 = __float_as_uint(a.y);
+  T.first = __float_as_uint(a.x);
+  T.mask = w1 & 0xffu;
+  T.vis = (w1 >> 8) & 0xffu;
+  T.bx0 = a.z; T.bx1 = a.w; T.by0 = b.x; T.by1 = b.y; T.bz0 = b.z; T.bz1 = b.w;
+  T.enter = false;
+  return false;
 }
 
 // Plain driver: lane i of the grid walks rays i, i + G, ... to the end,
