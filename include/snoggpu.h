@@ -186,9 +186,11 @@ typedef struct sgr_render_params {
      * margin (ceil(radius - 0.5001) pixels, snogray.lua:546-550) is added
      * around it by the renderer. w == 0 means the whole image. */
     uint32_t x0, y0, w, h;
-    /* Paths kept in flight per wavefront step (0: the default below);
-     * a frame runs in chunks of whole-image x floor(this / pixels)
-     * samples. */
+    /* Paths per chunk (0: the default below): a frame runs in chunks of
+     * whole-image x floor(this / pixels) samples.  sgr_render keeps two
+     * chunks in flight on two streams when a frame has several (path
+     * buffers for both), so that one chunk's sparse late bounces overlap
+     * the next chunk's full early ones; results do not depend on it. */
     uint32_t paths_in_flight;
     uint32_t flags;                /* SGR_RENDER_* */
 } sgr_render_params;
@@ -198,6 +200,9 @@ typedef struct sgr_render_params {
 /* sgr_render_params.flags: time every kernel launch with HIP events on
  * the render stream (no host waits) into sgr_stats.ms_* */
 #define SGR_RENDER_KERNEL_TIMING 1u
+/* ... run the chunks one after another on one stream (one chunk of path
+ * buffers); the image is the same bit for bit */
+#define SGR_RENDER_ONE_CHUNK_IN_FLIGHT 2u
 
 /* Counters named after RenderStats::IsecStats (render/render-stats.h:30-52)
  * plus device timings. */

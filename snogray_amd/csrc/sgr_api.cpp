@@ -151,7 +151,9 @@ int sgr_ctx_create(int device, sgr_ctx **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->lane_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(SGR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
@@ -175,8 +177,11 @@ int sgr_ctx_destroy(sgr_ctx *ctx) {
   if (ctx->stats) (void)hipFree(ctx->stats);
   for (auto &ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
+  (void)hipStreamSynchronize(ctx->lane_stream);
   for (auto &ev : ctx->evpool) (void)hipEventDestroy(ev);
+  for (auto &ev : ctx->syncpool) (void)hipEventDestroy(ev);
   sgr::engine_destroy(ctx);
+  (void)hipStreamDestroy(ctx->lane_stream);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SGR_OK;
@@ -833,11 +838,19 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   sgr_ctx *ctx = sc->ctx;
   RenderView R = pl.R;
   size_t cap = (size_t)pl.npix * pl.sb_max;
-  size_t need = path_bytes(cap, R.K, nullptr, nullptr);
-  int rc = ensure_work(ctx, need);
+  // Two chunk lanes (own path buffers, own stream) when the frame has
+  // several chunks: the next chunk's full first bounces then overlap the
+  // sparse, latency-bound last bounces of the one before.  Guided and
+  // per-sample renders keep one lane (they depend on chunk order).
+  const uint32_t nchunks = (R.spp + pl.sb_max - 1) / pl.sb_max;
+  const int lanes = (!guide && !samples_host && nchunks >= 2 &&
+                     !(pl.flags & SGR_RENDER_ONE_CHUNK_IN_FLIGHT)) ? 2 : 1;
+  size_t need = (path_bytes(cap, R.K, nullptr, nullptr) + 255) & ~(size_t)255;
+  int rc = ensure_work(ctx, need * lanes);
   if (rc) return rc;
-  PathBufs P;
-  path_bytes(cap, R.K, &P, (char *)ctx->work);
+  PathBufs PL[2];
+  for (int l = 0; l < lanes; l++) path_bytes(cap, R.K, &PL[l], (char *)ctx->work + need * l);
+  const hipStream_t lane_st[2] = {ctx->stream, ctx->lane_stream};
   GuideView GV;
   if (guide) {
     rc = guide_prepare(guide, sc, cap, GV);
@@ -865,39 +878,62 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     return ctx->evpool[next_ev++];
   };
   size_t mark0 = 0;
+  hipStream_t cur = ctx->stream;  // stream of the chunk being enqueued
   auto t_begin = [&]() {
     if (!kt) return;
     mark0 = next_ev;
-    (void)hipEventRecord(ev_get(), ctx->stream);
+    (void)hipEventRecord(ev_get(), cur);
   };
   auto t_end = [&](int kind) {
     if (!kt) return;
-    (void)hipEventRecord(ev_get(), ctx->stream);
+    (void)hipEventRecord(ev_get(), cur);
     marks.push_back({kind, mark0});
   };
+  size_t next_sync = 0;
+  auto sync_get = [&]() -> hipEvent_t {
+    if (next_sync == ctx->syncpool.size()) {
+      hipEvent_t e;
+      (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+      ctx->syncpool.push_back(e);
+    }
+    return ctx->syncpool[next_sync++];
+  };
   HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+  hipEvent_t ev_bulk = nullptr, ev_splat = nullptr;  // of the previous chunk
+  if (lanes > 1) {  // the second lane starts after the stats are cleared
+    hipEvent_t e = sync_get();
+    HIPCHK(hipEventRecord(e, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->lane_stream, e, 0));
+  }
+  const uint32_t stagger_b = R.maxp < 1 ? R.maxp : 1u;
   StepView J;
   std::memset(&J, 0, sizeof J);
   R.path_base = 0;
   R.out_base = 0;
   R.scene = 0;
   R.accum = accum_dev;
-  P.K = R.K ? R.K : 1;
-  for (uint32_t s0 = 0; s0 < R.spp; s0 += pl.sb_max) {
+  for (int l = 0; l < lanes; l++) PL[l].K = R.K ? R.K : 1;
+  uint32_t chunk = 0;
+  for (uint32_t s0 = 0; s0 < R.spp; s0 += pl.sb_max, chunk++) {
+    const int lane = (int)(chunk % (uint32_t)lanes);
+    PathBufs &P = PL[lane];
+    cur = lane_st[lane];
     R.s0 = s0;
     R.sb = std::min(pl.sb_max, R.spp - s0);
     R.npaths = pl.npix * R.sb;
     P.n = (uint32_t)cap;
+    // a chunk starts once the one before has passed its first bounces
+    if (ev_bulk) HIPCHK(hipStreamWaitEvent(cur, ev_bulk, 0));
     if (guide) {
       rc = guide_begin_chunk(guide, GV, R.npaths, R.seed * 0x9e3779b97f4a7c15ULL + s0 + 1);
       if (rc) return rc;
     }
     t_begin();
-    launch_raygen(false, R, J, P, grid, ctx->stream);
+    launch_raygen(false, R, J, P, grid, cur);
     t_end(K_OTHER);
     for (uint32_t b = 0; b <= R.maxp; b++) {
       t_begin();
-      launch_extend(false, sc->view, J, P, b, ctx->stats, grid, ctx->stream);
+      launch_extend(false, sc->view, J, P, b, ctx->stats, grid, cur);
       t_end(K_EXT);
       if (guide) {
         t_begin();
@@ -905,23 +941,33 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
         t_end(K_OTHER);
       }
       t_begin();
-      launch_shade(false, sc->view, R, J, P, b, ctx->stats, grid, ctx->stream, G);
+      launch_shade(false, sc->view, R, J, P, b, ctx->stats, grid, cur, G);
       t_end(K_SHADE);
       t_begin();
-      launch_occlude(false, sc->view, J, P, b, ctx->stats, grid, ctx->stream);
+      launch_occlude(false, sc->view, J, P, b, ctx->stats, grid, cur);
       t_end(K_OCC);
       t_begin();
-      launch_gather(false, R, J, P, b, grid, ctx->stream, G);
+      launch_gather(false, R, J, P, b, grid, cur, G);
       t_end(K_OTHER);
       n_ext++;
       n_occ++;
+      if (lanes > 1 && b == stagger_b) {
+        ev_bulk = sync_get();
+        HIPCHK(hipEventRecord(ev_bulk, cur));
+      }
     }
     if (accum_dev) {
+      // splats reach the accumulators in chunk order
+      if (ev_splat) HIPCHK(hipStreamWaitEvent(cur, ev_splat, 0));
       t_begin();
-      launch_splat(false, R, J, P.out, grid, ctx->stream);
+      launch_splat(false, R, J, P.out, grid, cur);
       t_end(K_SPLAT);
+      if (lanes > 1) {
+        ev_splat = sync_get();
+        HIPCHK(hipEventRecord(ev_splat, cur));
+      }
     }
-    launch_count_queues(R.maxp, P, ctx->stats, ctx->stream);
+    launch_count_queues(R.maxp, P, ctx->stats, cur);
     HIPCHK(hipGetLastError());
     if (guide && train) {
       t_begin();
@@ -932,14 +978,19 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     if (samples_host) {
       chunk_out.resize((size_t)R.npaths * 6);
       HIPCHK(hipMemcpyAsync(chunk_out.data(), P.out, chunk_out.size() * 4, hipMemcpyDeviceToHost,
-                            ctx->stream));
+                            cur));
     }
     if (samples_host) {
-      HIPCHK(hipStreamSynchronize(ctx->stream));
+      HIPCHK(hipStreamSynchronize(cur));
       for (uint32_t pix = 0; pix < pl.npix; pix++)
         std::memcpy(samples_host + ((size_t)pix * R.spp + s0) * 6, &chunk_out[(size_t)pix * R.sb * 6],
                     (size_t)R.sb * 6 * 4);
     }
+  }
+  if (lanes > 1) {  // join the second lane back into the render stream
+    hipEvent_t e = sync_get();
+    HIPCHK(hipEventRecord(e, ctx->lane_stream));
+    HIPCHK(hipStreamWaitEvent(ctx->stream, e, 0));
   }
   HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
   if (guide) {

@@ -24,6 +24,7 @@ struct Engine;
 struct sgr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t lane_stream = nullptr;  // second chunk lane of sgr_render
   std::atomic<uint64_t> bytes{0};  // device bytes held (scenes + buffers)
   void *work = nullptr;        // path buffers, grown on demand
   size_t work_bytes = 0;
@@ -31,6 +32,7 @@ struct sgr_ctx {
   int num_cu = 256;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<hipEvent_t> evpool;  // per-launch timing events
+  std::vector<hipEvent_t> syncpool;  // ordering events between the chunk lanes
   std::atomic<uint32_t> live_scenes{0};
   sgr::Engine *engine = nullptr;   // multi-job engine (sgr_jobs.cpp)
 };

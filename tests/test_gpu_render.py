@@ -154,6 +154,12 @@ def test_determinism_and_chunking(ctx):
     assert np.array_equal(x, y)
     z, _ = sc.render(cam, p2)
     np.testing.assert_allclose(x, z, rtol=1e-5, atol=1e-6)
+    # two chunks in flight (the default for several chunks) or one: the
+    # same splats in the same order, so the same bits
+    p3 = abi.make_params(40, 30, 9, max_path_len=8, seed=5, paths_in_flight=2000,
+                         flags=abi.SGR_RENDER_ONE_CHUNK_IN_FLIGHT)
+    w, _ = sc.render(cam, p3)
+    assert np.array_equal(w.view(np.uint32), z.view(np.uint32))
 
 
 def test_window_tiles_equal_full_render(ctx):
@@ -187,7 +193,8 @@ def test_invalid_parameters_raise(ctx):
 
 def test_full_size_frame_properties(ctx):
     """The 1.31M-triangle soup at 1920x1080, 64 spp (the benchmark frame):
-    deterministic, chunking changes only the summation order, every pixel
+    deterministic (with one or two chunks in flight), chunking changes
+    only the summation order, every pixel
     gets filter weight, and without the negative-lobe clamp the weights
     add up to the sample count (the Mitchell filter's integer shifts sum
     to one)."""
@@ -195,9 +202,9 @@ def test_full_size_frame_properties(ctx):
     sc = native.Scene(ctx, sd)
     W, H, S = 1920, 1080, 64
     accs = []
-    for pif in (0, 0, 1 << 24):
+    for pif, fl in ((0, 0), (0, abi.SGR_RENDER_ONE_CHUNK_IN_FLIGHT), (1 << 24, 0)):
         acc, _ = sc.render(cam, abi.make_params(W, H, S, max_path_len=10, seed=1,
-                                                paths_in_flight=pif))
+                                                paths_in_flight=pif, flags=fl))
         accs.append(acc)
     a, b, c = accs
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
