@@ -151,8 +151,11 @@ int sgr_ctx_create(int device, sgr_ctx **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->lane_stream, hipStreamNonBlocking);
+  for (auto &ls : c->lane_stream)
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ls, hipStreamNonBlocking);
   if (e != hipSuccess) {
+    for (auto &ls : c->lane_stream)
+      if (ls) (void)hipStreamDestroy(ls);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(SGR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -177,11 +180,11 @@ int sgr_ctx_destroy(sgr_ctx *ctx) {
   if (ctx->stats) (void)hipFree(ctx->stats);
   for (auto &ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
-  (void)hipStreamSynchronize(ctx->lane_stream);
+  for (auto &ls : ctx->lane_stream) (void)hipStreamSynchronize(ls);
   for (auto &ev : ctx->evpool) (void)hipEventDestroy(ev);
   for (auto &ev : ctx->syncpool) (void)hipEventDestroy(ev);
   sgr::engine_destroy(ctx);
-  (void)hipStreamDestroy(ctx->lane_stream);
+  for (auto &ls : ctx->lane_stream) (void)hipStreamDestroy(ls);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SGR_OK;
@@ -838,19 +841,23 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   sgr_ctx *ctx = sc->ctx;
   RenderView R = pl.R;
   size_t cap = (size_t)pl.npix * pl.sb_max;
-  // Two chunk lanes (own path buffers, own stream) when the frame has
-  // several chunks: the next chunk's full first bounces then overlap the
-  // sparse, latency-bound last bounces of the one before.  Guided and
-  // per-sample renders keep one lane (they depend on chunk order).
+  // Up to SGR_CHUNK_LANES chunk lanes (own path buffers, own stream)
+  // when the frame has several chunks: chunks run concurrently, so one
+  // chunk's sparse, latency-bound last bounces overlap another's full
+  // ones.  Guided and per-sample renders keep one lane (they depend on
+  // chunk order).
   const uint32_t nchunks = (R.spp + pl.sb_max - 1) / pl.sb_max;
-  const int lanes = (!guide && !samples_host && nchunks >= 2 &&
-                     !(pl.flags & SGR_RENDER_ONE_CHUNK_IN_FLIGHT)) ? 2 : 1;
+  const int lanes = (guide || samples_host || (pl.flags & SGR_RENDER_ONE_CHUNK_IN_FLIGHT))
+                        ? 1 : (int)std::min<uint32_t>(nchunks, SGR_CHUNK_LANES);
   size_t need = (path_bytes(cap, R.K, nullptr, nullptr) + 255) & ~(size_t)255;
   int rc = ensure_work(ctx, need * lanes);
   if (rc) return rc;
-  PathBufs PL[2];
-  for (int l = 0; l < lanes; l++) path_bytes(cap, R.K, &PL[l], (char *)ctx->work + need * l);
-  const hipStream_t lane_st[2] = {ctx->stream, ctx->lane_stream};
+  PathBufs PL[SGR_CHUNK_LANES];
+  hipStream_t lane_st[SGR_CHUNK_LANES];
+  for (int l = 0; l < lanes; l++) {
+    path_bytes(cap, R.K, &PL[l], (char *)ctx->work + need * l);
+    lane_st[l] = l ? ctx->lane_stream[l - 1] : ctx->stream;
+  }
   GuideView GV;
   if (guide) {
     rc = guide_prepare(guide, sc, cap, GV);
@@ -899,13 +906,12 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     return ctx->syncpool[next_sync++];
   };
   HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-  hipEvent_t ev_bulk = nullptr, ev_splat = nullptr;  // of the previous chunk
-  if (lanes > 1) {  // the second lane starts after the stats are cleared
+  hipEvent_t ev_splat = nullptr;  // of the previous chunk
+  if (lanes > 1) {  // the other lanes start after the stats are cleared
     hipEvent_t e = sync_get();
     HIPCHK(hipEventRecord(e, ctx->stream));
-    HIPCHK(hipStreamWaitEvent(ctx->lane_stream, e, 0));
+    for (int l = 1; l < lanes; l++) HIPCHK(hipStreamWaitEvent(lane_st[l], e, 0));
   }
-  const uint32_t stagger_b = R.maxp < 1 ? R.maxp : 1u;
   StepView J;
   std::memset(&J, 0, sizeof J);
   R.path_base = 0;
@@ -922,8 +928,6 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     R.sb = std::min(pl.sb_max, R.spp - s0);
     R.npaths = pl.npix * R.sb;
     P.n = (uint32_t)cap;
-    // a chunk starts once the one before has passed its first bounces
-    if (ev_bulk) HIPCHK(hipStreamWaitEvent(cur, ev_bulk, 0));
     if (guide) {
       rc = guide_begin_chunk(guide, GV, R.npaths, R.seed * 0x9e3779b97f4a7c15ULL + s0 + 1);
       if (rc) return rc;
@@ -951,10 +955,6 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
       t_end(K_OTHER);
       n_ext++;
       n_occ++;
-      if (lanes > 1 && b == stagger_b) {
-        ev_bulk = sync_get();
-        HIPCHK(hipEventRecord(ev_bulk, cur));
-      }
     }
     if (accum_dev) {
       // splats reach the accumulators in chunk order
@@ -987,9 +987,9 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
                     (size_t)R.sb * 6 * 4);
     }
   }
-  if (lanes > 1) {  // join the second lane back into the render stream
+  for (int l = 1; l < lanes; l++) {  // join the lanes back into the render stream
     hipEvent_t e = sync_get();
-    HIPCHK(hipEventRecord(e, ctx->lane_stream));
+    HIPCHK(hipEventRecord(e, lane_st[l]));
     HIPCHK(hipStreamWaitEvent(ctx->stream, e, 0));
   }
   HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));

@@ -21,10 +21,14 @@ namespace sgr {
 struct Engine;
 }
 
+// Chunks sgr_render keeps in flight at most (one stream and one set of
+// path buffers each).
+#define SGR_CHUNK_LANES 2
+
 struct sgr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t lane_stream = nullptr;  // second chunk lane of sgr_render
+  hipStream_t lane_stream[SGR_CHUNK_LANES - 1] = {};  // chunk lanes of sgr_render after the first
   std::atomic<uint64_t> bytes{0};  // device bytes held (scenes + buffers)
   void *work = nullptr;        // path buffers, grown on demand
   size_t work_bytes = 0;
