@@ -198,12 +198,15 @@ def frame_bench(args, sd, cam, width, height, spp, maxp, minp, workload, traffic
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / args.steps
     assert torch.isfinite(acc).all().item()
-    # Kernel roofline: one more frame with every launch timed by HIP events
-    # and the chunks run one at a time on one stream, so that a launch's
-    # duration is the kernel's own and not shared with a concurrent chunk.
+    # Kernel roofline: one more frame with every launch timed by HIP events,
+    # the chunks run one at a time on one stream (a launch's duration is
+    # the kernel's own, not shared with a concurrent chunk) and every
+    # bounce through the per-bounce kernels (so that all closest-hit walks
+    # the counters count are k_extend's).  The image is the same.
     pk = abi.make_params(width, height, spp, max_path_len=maxp, min_path_len=minp,
                          seed=args.seed, paths_in_flight=args.paths_in_flight,
-                         flags=abi.SGR_RENDER_KERNEL_TIMING | abi.SGR_RENDER_ONE_CHUNK_IN_FLIGHT)
+                         flags=(abi.SGR_RENDER_KERNEL_TIMING | abi.SGR_RENDER_ONE_CHUNK_IN_FLIGHT
+                                | abi.SGR_RENDER_WAVEFRONT_ONLY))
     tot = sc.render_device(cam, pk, acc.data_ptr())
     log(f"timed-kernels frame: {tot['ms_render']:.1f} ms")
     # one more frame alone for the image-quality comparison
@@ -433,14 +436,25 @@ def learned_bench(args):
         log(f"seed {seed}: guided {t_g:.3f} s, plain 64 spp {t_p64:.3f} s, "
             f"equal time {spp_eq:.1f} spp")
 
-    def mean_rel(key, ref, crop=None):
+    def seed_rel(key, ref, crop=None):
         vals = []
         for r in runs:
             img = finalize(r[key]) if crop is None else finalize(take_crops(r[key], crop))
             vals.append(relmse(img, ref))
-        return float(np.mean(vals))
+        return vals
 
-    full = {k: mean_rel(k, ref_full) for k in ("img_g", "img_p64", "img_peq")}
+    def summary(vals, gain_of):
+        """Means over the seeds (the headline), and per-seed values with
+        the median: the path tracer's Russian roulette makes rare,
+        very bright samples, so one seed's relMSE can dominate a mean."""
+        m = {k: float(np.mean(v)) for k, v in vals.items()}
+        med = {k: float(np.median(v)) for k, v in vals.items()}
+        return m, {
+            "per_seed": {k: [float(f"{x:.4g}") for x in v] for k, v in vals.items()},
+            "median_equal_time_gain": round(med[gain_of[0]] / med[gain_of[1]], 3)}
+
+    full_v = {k: seed_rel(k, ref_full) for k in ("img_g", "img_p64", "img_peq")}
+    full, full_extra = summary(full_v, ("img_peq", "img_g"))
     gain_full = full["img_peq"] / full["img_g"]
     t_g = float(np.mean([r["t_g"] for r in runs]))
     t_p64 = float(np.mean([r["t_p64"] for r in runs]))
@@ -464,7 +478,8 @@ def learned_bench(args):
                               "guided_64spp": float(f"{full['img_g']:.4g}"),
                               "plain_64spp": float(f"{full['img_p64']:.4g}"),
                               "plain_equal_time": float(f"{full['img_peq']:.4g}"),
-                              "equal_spp_gain": round(full["img_p64"] / full["img_g"], 3)},
+                              "equal_spp_gain": round(full["img_p64"] / full["img_g"], 3),
+                              **full_extra},
     }
     if not args.no_cpu:
         cpu = CpuBaseline(sd)
@@ -472,14 +487,16 @@ def learned_bench(args):
         ref, t_ref = cpu.crops(cam, width, height, REF_SPP, maxp, minp, args.seed + 1000,
                                fp64=True)
         wins = crop_windows(width, height)
-        crops = {k: mean_rel(k, ref, wins) for k in ("img_g", "img_p64", "img_peq")}
+        crops_v = {k: seed_rel(k, ref, wins) for k in ("img_g", "img_p64", "img_peq")}
+        crops, crops_extra = summary(crops_v, ("img_peq", "img_g"))
         line["relmse_crops_fp64_oracle"] = {
             "reference": f"FP64 CPU oracle at {REF_SPP} spp on {len(wins)} crops of "
                          f"{CROP_W}x{CROP_H} ({t_ref:.1f} s)",
             "guided_64spp": float(f"{crops['img_g']:.4g}"),
             "plain_64spp": float(f"{crops['img_p64']:.4g}"),
             "plain_equal_time": float(f"{crops['img_peq']:.4g}"),
-            "equal_time_gain": round(crops["img_peq"] / crops["img_g"], 3)}
+            "equal_time_gain": round(crops["img_peq"] / crops["img_g"], 3),
+            **crops_extra}
     sc.close()
     ctx.close()
     return line

@@ -203,6 +203,10 @@ typedef struct sgr_render_params {
 /* ... run the chunks one after another on one stream (one chunk of path
  * buffers); the image is the same bit for bit */
 #define SGR_RENDER_ONE_CHUNK_IN_FLIGHT 2u
+/* ... run every bounce through the per-bounce kernels: without it, once
+ * a late bounce's ray queue is short, one kernel carries each remaining
+ * path through all its remaining bounces (same image bit for bit) */
+#define SGR_RENDER_WAVEFRONT_ONLY 4u
 
 /* Counters named after RenderStats::IsecStats (render/render-stats.h:30-52)
  * plus device timings. */

@@ -19,6 +19,7 @@ SGR_ACCEL_TRIV = 1
 MISS = 0xFFFFFFFF
 SGR_RENDER_KERNEL_TIMING = 1
 SGR_RENDER_ONE_CHUNK_IN_FLIGHT = 2
+SGR_RENDER_WAVEFRONT_ONLY = 4
 SGR_DEFAULT_PATHS_IN_FLIGHT = 1 << 26
 
 f32p = C.POINTER(C.c_float)

@@ -294,6 +294,33 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
   return false;
 }
 
+// One ray from start to end: brute force over the surface list
+// (TrivSpace, space/triv-space.h:48-58) or the octree walk.
+template<bool ANY>
+__device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, TraceCount &cnt,
+                                          LdsStack L, float4 *st0, float4 *st1) {
+  if (S.use_triv) {
+    T.hit = SGR_MISS;
+    T.hu = 0.0f;
+    T.hv = 0.0f;
+    float t, u, v;
+    uint32_t prim;
+    for (uint32_t q = 0; q < S.n_surfs; q++) {
+      cnt.tests++;
+      if (ref_hit(S.refs, q, T.ray, t, u, v, prim)) {
+        T.hit = prim;
+        if (ANY) break;
+        T.ray.t1 = t;
+        T.hu = u;
+        T.hv = v;
+      }
+    }
+  } else if (trav_init(S, T, cnt)) {
+    while (!trav_step<ANY>(S, T, cnt, L, st0, st1)) {
+    }
+  }
+}
+
 // Plain driver: lane i of the grid walks rays i, i + G, ... to the end,
 // one at a time (ray ranges by block_range order).  Cheaper per step than
 // trace_refill when walks are short (shallow octrees), where the waves
@@ -305,27 +332,7 @@ __device__ __forceinline__ void trace_plain(const SceneView &S1, uint32_t jb, ui
   float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
   for (uint32_t i = jb + threadIdx.x; i < je; i += blockDim.x) {
     const SceneView *g = get(i, T.ray);
-    const SceneView &S = MULTI ? *g : S1;
-    if (S.use_triv) {
-      T.hit = SGR_MISS;
-      T.hu = 0.0f;
-      T.hv = 0.0f;
-      float t, u, v;
-      uint32_t prim;
-      for (uint32_t q = 0; q < S.n_surfs; q++) {
-        cnt.tests++;
-        if (ref_hit(S.refs, q, T.ray, t, u, v, prim)) {
-          T.hit = prim;
-          if (ANY) break;
-          T.ray.t1 = t;
-          T.hu = u;
-          T.hv = v;
-        }
-      }
-    } else if (trav_init(S, T, cnt)) {
-      while (!trav_step<ANY>(S, T, cnt, L, st0, st1)) {
-      }
-    }
+    trace_one<ANY>(MULTI ? *g : S1, T, cnt, L, st0, st1);
     put(i, T);
   }
 }

@@ -97,7 +97,16 @@ struct PathBufs {
   uint32_t *pseg;     // segment of each path (multi-job steps)
   uint32_t n;         // path capacity (stride of the media stack)
   uint32_t K;         // shadow slots per path
+  // From bounce fin_from on, a ray queue shorter than fin_max rays is
+  // finished by k_finish in one launch instead of bounce by bounce
+  // (fin_max 0: never).
+  uint32_t fin_from = 0, fin_max = 0;
 };
+
+// Bounce from which, and queue length below which, renders and engine
+// steps finish a queue's paths in one launch (k_finish).
+#define SGR_FINISH_FROM 1u
+#define SGR_FINISH_MAX (1u << 18)
 
 void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool any,
                   unsigned long long *stats, hipStream_t st);
@@ -114,6 +123,11 @@ void launch_occlude(bool multi, const SceneView &S, const StepView &J, const Pat
                     uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st);
 void launch_gather(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
                    uint32_t b, unsigned grid, hipStream_t st, const GuideView *G = nullptr);
+// Finishes the paths of the bounce-B queue when P.fin_* select it
+// (unguided renders and multi-job steps).
+void launch_finish(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
+                   const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
+                   hipStream_t st);
 void launch_splat(bool multi, const RenderView &R, const StepView &J, const float *out,
                   unsigned grid, hipStream_t st);
 void launch_count_queues(uint32_t maxp, const PathBufs &P, unsigned long long *stats,

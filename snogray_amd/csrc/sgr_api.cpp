@@ -918,7 +918,14 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   R.out_base = 0;
   R.scene = 0;
   R.accum = accum_dev;
-  for (int l = 0; l < lanes; l++) PL[l].K = R.K ? R.K : 1;
+  // Short late queues go to k_finish (unguided renders): from bounce
+  // SGR_FINISH_FROM on, a queue of fewer than SGR_FINISH_MAX rays.
+  const bool finish = !guide && !(pl.flags & SGR_RENDER_WAVEFRONT_ONLY);
+  for (int l = 0; l < lanes; l++) {
+    PL[l].K = R.K ? R.K : 1;
+    PL[l].fin_from = SGR_FINISH_FROM;
+    PL[l].fin_max = finish ? SGR_FINISH_MAX : 0u;
+  }
   uint32_t chunk = 0;
   for (uint32_t s0 = 0; s0 < R.spp; s0 += pl.sb_max, chunk++) {
     const int lane = (int)(chunk % (uint32_t)lanes);
@@ -936,6 +943,11 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     launch_raygen(false, R, J, P, grid, cur);
     t_end(K_OTHER);
     for (uint32_t b = 0; b <= R.maxp; b++) {
+      if (finish && b >= SGR_FINISH_FROM) {
+        t_begin();
+        launch_finish(false, sc->view, R, J, P, b, ctx->stats, grid, cur);
+        t_end(K_OTHER);
+      }
       t_begin();
       launch_extend(false, sc->view, J, P, b, ctx->stats, grid, cur);
       t_end(K_EXT);

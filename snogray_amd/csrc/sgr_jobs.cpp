@@ -142,6 +142,8 @@ int ensure_engine_work(sgr_ctx *ctx, Engine *E, size_t cap, size_t K) {
   path_bytes(cap, K, &E->P, (char *)E->work);
   E->P.n = (uint32_t)cap;
   E->P.K = (uint32_t)K;
+  E->P.fin_from = SGR_FINISH_FROM;
+  E->P.fin_max = SGR_FINISH_MAX;
   return SGR_OK;
 }
 
@@ -175,6 +177,7 @@ void enqueue_step(sgr_ctx *ctx, Engine *E, uint32_t maxp, bool refill, unsigned 
   hipStream_t st = ctx->stream;
   launch_raygen(true, R0, J, P, grid, st);
   for (uint32_t b = 0; b <= maxp; b++) {
+    if (b >= SGR_FINISH_FROM) launch_finish(true, S0, R0, J, P, b, E->stats, grid, st);
     launch_extend(true, S0, J, P, b, E->stats, grid, st);
     launch_shade(true, S0, R0, J, P, b, E->stats, grid, st);
     launch_occlude(true, S0, J, P, b, E->stats, grid, st);

@@ -167,6 +167,12 @@ __global__ void __launch_bounds__(256) k_raygen(RenderView R1, StepView J, PathB
   }
 }
 
+// Bounce B's queue of N rays goes to k_finish instead of the per-bounce
+// kernels.
+__device__ __forceinline__ bool finishing(const PathBufs &P, uint32_t b, uint32_t n) {
+  return P.fin_max != 0 && b >= P.fin_from && n < P.fin_max;
+}
+
 template<bool MULTI, bool REFILL>
 __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                 unsigned long long *stats) {
@@ -174,6 +180,7 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
   const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
   const uint32_t n = P.counts[SGR_CNT_Q + b];
+  if (finishing(P, b, n)) return;
   const uint32_t *q = P.q[b & 1];
   walk<false, MULTI, REFILL>(
       S1, n, &P.counts[SGR_CNT_HE + 8 * b],
@@ -200,6 +207,7 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathB
   __shared__ float4 stk_lds[2 * SGR_LDS_SLOTS * SGR_TRACE_BLOCK];
   const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
+  if (finishing(P, b, P.counts[SGR_CNT_Q + b])) return;
   const uint32_t n = P.counts[SGR_CNT_S + b];
   walk<true, MULTI, REFILL>(
       S1, n, &P.counts[SGR_CNT_HS + 8 * b],
@@ -219,235 +227,252 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathB
   wave_count(&stats[SGR_STAT_TESTS_S], cnt.tests);
 }
 
+// One path's vertex (render/path-integ.cc:133-407 for one bounce): the
+// hit at P.hprim / P.ht of path P, emission, direct lighting (shadow rays
+// written to the path's slots, each announced by EMIT(slot)), the BSDF
+// continuation and Russian roulette.  Returns whether the path goes on
+// with the ray it stored.  J is the queue position (learned proposal
+// lobes); ILLUM counts direct-lighting calls.
+template<bool MULTI, bool GUIDED, typename Emit>
+__device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView &R1,
+                                           const StepView &J, const PathBufs &P,
+                                           const GuideView &G, uint32_t p, uint32_t j,
+                                           uint32_t &illum, Emit emit) {
+  const uint32_t N = P.n;
+  const RenderView R = MULTI ? J.R[P.pseg[p]] : R1;
+  const SceneView S = MULTI ? J.S[R.scene] : S1;
+  uint32_t st = P.state[p];
+  uint32_t path_len = st & SGR_ST_LEN_MASK;
+  bool after_spec = (st >> SGR_ST_SPEC_SHIFT) & 1u;
+  uint32_t depth = (st >> SGR_ST_DEPTH_SHIFT) & 0xfu;
+  bool include = path_len == 0 || after_spec || R.ds == 0;
+  C3 T = c3(P.T[0][p], P.T[1][p], P.T[2][p]);
+  C3 rad = c3(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
+  float alpha = P.alpha[p];
+  RayF ray;
+  load_ray(P, p, ray);
+  uint32_t prim = P.hprim[p];
+  bool done = false, cont = false;
+  uint32_t nst = st & ~(SGR_ST_PENDING | SGR_ST_DONE | SGR_ST_STRAT);
+  float *grec = nullptr;  // this path's training record (learned proposal)
+  if (GUIDED && p % G.rec_stride == 0) {
+    // slots are shuffled so that every training minibatch spans the image
+    grec = G.rec + (size_t)permute(p / G.rec_stride, G.rec_count, G.rec_key) * GD_REC;
+    if (grec[9] == -1.0f) {
+      // radiance gathered before the recorded continuation's first hit
+      grec[13] = rad.r; grec[14] = rad.g; grec[15] = rad.b;
+      grec[9] = -2.0f;
+    }
+  }
+  if (prim == SGR_MISS) {
+    if (include) rad = rad + background(S, ray.d) * T;
+    if (path_len == 0 && czero(rad)) alpha = R.bg_alpha;
+    done = true;
+  } else {
+    ray.t1 = P.ht[p];
+    Isec is;
+    float mtop = P.media[(size_t)(depth - 1) * N + p];
+    float mencl = depth >= 2 ? P.media[(size_t)(depth - 2) * N + p] : 1.0f;
+    make_isec(S, prim, ray, mtop, mencl, is);
+    const DevMaterial &HM = S.mats[is.mat];
+    if (include && HM.type == 5 && !is.back) rad = rad + cload(HM.color) * T;
+    if (is.bmat < 0) {
+      done = true;
+    } else {
+      const DevMaterial &M = S.mats[is.bmat];
+      int px, py;
+      uint32_t s;
+      path_ident(R, p, px, py, s);
+      uint64_t pk = pixel_key(R.seed, px, py);
+      uint64_t qk = path_key(pk, s);
+      // ---- direct lighting (DirectIllum::sample_all_lights) ----
+      if (R.ds != 0) {
+        illum++;
+        const unsigned dflags = F_ALL & ~F_SPECULAR;
+        bool strat = path_len < R.minp;
+        uint32_t nsub = strat ? R.nsub_strat : R.nsub_rand;
+        for (uint32_t l = 0; l < R.L; l++) {
+          const DevLight &L = S.lights[l];
+          for (uint32_t jj = 0; jj < nsub; jj++) {
+            float lu, lv, bu, bv;
+            uint32_t base = 16 + (l * nsub + jj) * 8;
+            if (strat) {
+              uint32_t chan = 1 + path_len * (2 * R.L + 1) + 2 * l;
+              strat_uv(pk, chan, s * nsub + jj, R.g_dir, lu, lv);
+              strat_uv(pk, chan + 1, s * nsub + jj, R.g_dir, bu, bv);
+            } else {
+              lu = draw(qk, (path_len << 20) | (base + 0));
+              lv = draw(qk, (path_len << 20) | (base + 1));
+              bu = draw(qk, (path_len << 20) | (base + 2));
+              bv = draw(qk, (path_len << 20) | (base + 3));
+            }
+            float extra = draw(qk, (path_len << 20) | (base + 4));
+            size_t slot_a = (size_t)p * P.K + 2 * (l * nsub + jj), slot_b = slot_a + 1;
+            P.svis[slot_a] = 0;
+            P.svis[slot_b] = 0;
+            // light sample (render/direct-illum.cc:147-197)
+            LSample ls = light_sample(S, L, is, lu, lv, extra);
+            if (ls.pdf > 0.0f && cgt0(ls.val)) {
+              BValue bval = bsdf_eval(is, M, ls.dir, dflags);
+              if (cgt0(bval.val)) {
+                C3 c = ls.val;
+                c = c * mis_weight(ls.pdf, bval.pdf);
+                c = c * bval.val;
+                c = c * fabsf(ls.dir.z);
+                c = c * (1.0f / ls.pdf);
+                float md = ls.dist == 0.0f ? S.horizon : ls.dist;
+                V3 wd = is.nf.from(ls.dir);
+                P.sray[0][slot_a] = is.nf.o.x; P.sray[1][slot_a] = is.nf.o.y;
+                P.sray[2][slot_a] = is.nf.o.z; P.sray[3][slot_a] = wd.x;
+                P.sray[4][slot_a] = wd.y; P.sray[5][slot_a] = wd.z;
+                P.sray[6][slot_a] = S.min_trace; P.sray[7][slot_a] = md - S.min_trace;
+                P.sc[0][slot_a] = c.r; P.sc[1][slot_a] = c.g; P.sc[2][slot_a] = c.b;
+                emit(slot_a);
+              }
+            }
+            // BSDF sample (render/direct-illum.cc:199-252)
+            BSample bs = bsdf_sample(is, M, bu, bv, dflags);
+            if (bs.pdf > 0.0f && cgt0(bs.val)) {
+              LSample lval = light_eval(S, L, is, bs.dir);
+              if (lval.pdf > 0.0f && cgt0(lval.val)) {
+                C3 c = lval.val;
+                c = c * mis_weight(bs.pdf, lval.pdf);
+                c = c * bs.val;
+                c = c * fabsf(bs.dir.z);
+                c = c * (1.0f / bs.pdf);
+                float md = lval.dist == 0.0f ? S.horizon : lval.dist;
+                V3 wd = is.nf.from(bs.dir);
+                P.sray[0][slot_b] = is.nf.o.x; P.sray[1][slot_b] = is.nf.o.y;
+                P.sray[2][slot_b] = is.nf.o.z; P.sray[3][slot_b] = wd.x;
+                P.sray[4][slot_b] = wd.y; P.sray[5][slot_b] = wd.z;
+                P.sray[6][slot_b] = S.min_trace; P.sray[7][slot_b] = md - S.min_trace;
+                P.sc[0][slot_b] = c.r; P.sc[1][slot_b] = c.g; P.sc[2][slot_b] = c.b;
+                emit(slot_b);
+              }
+            }
+          }
+        }
+        nst |= SGR_ST_PENDING | (strat ? SGR_ST_STRAT : 0u);
+        P.Tp[0][p] = T.r; P.Tp[1][p] = T.g; P.Tp[2][p] = T.b;
+      }
+      // ---- continuation (render/path-integ.cc:308-401) ----
+      float cu, cv;
+      if (path_len < R.minp) {
+        uint32_t chan = 1 + path_len * (2 * R.L + 1) + 2 * R.L;
+        strat_uv(pk, chan, s, R.g_cont, cu, cv);
+      } else {
+        cu = draw(qk, (path_len << 20) | 1u);
+        cv = draw(qk, (path_len << 20) | 2u);
+      }
+      BSample bs;
+      // Learned proposal: one-sample MIS of the BSDF (probability
+      // G.mix) and the vMF mixture at this vertex; delta BSDFs (glass,
+      // mirror) bypass it.
+      const bool guided = GUIDED && (M.type == 1 || M.type == 2);  // Lambert, Cook-Torrance
+      if (guided) {
+        const float *L = G.lobes + (size_t)j * GD_OUT;
+        const float usel = draw(qk, (path_len << 20) | 3u);
+        if (usel < G.mix) {
+          bs = bsdf_sample(is, M, cu, cv, F_ALL);
+          if (bs.pdf > 0.0f) {
+            const V3 wd = is.nf.from(bs.dir);
+            bs.pdf = G.mix * bs.pdf + (1.0f - G.mix) * mix_pdf(L, wd.x, wd.y, wd.z);
+          }
+        } else {
+          V3 wd;
+          mix_sample(L, draw(qk, (path_len << 20) | 4u), draw(qk, (path_len << 20) | 5u),
+                     draw(qk, (path_len << 20) | 6u), wd.x, wd.y, wd.z);
+          const V3 ld = is.nf.to(wd);
+          const BValue bv = bsdf_eval(is, M, ld, F_ALL);
+          bs.val = bv.val;
+          bs.dir = ld;
+          bs.flags = F_REFLECTIVE | F_DIFFUSE;
+          bs.pdf = czero(bv.val) ? 0.0f
+                                 : G.mix * bv.pdf + (1.0f - G.mix) * mix_pdf(L, wd.x, wd.y, wd.z);
+        }
+      } else {
+        bs = bsdf_sample(is, M, cu, cv, F_ALL);
+      }
+      const C3 T_vertex = T;
+      if (bs.pdf == 0.0f || czero(bs.val)) {
+        done = true;
+      } else {
+        C3 f = bs.val * fabsf(bs.dir.z);
+        f = f * (1.0f / bs.pdf);
+        T = T * f;
+        float rrr = P.rr[p];
+        if (path_len > R.minp) {
+          float pc = fmn(((T.r + T.g) + T.b) / 3.0f * rrr, 1.0f);
+          float rr = draw(qk, (path_len << 20) | 0u);
+          if (rr > pc) {
+            done = true;
+          } else {
+            T = T * (1.0f / pc);
+            rrr = rrr * pc;
+          }
+        }
+        if (!done && path_len == R.maxp) done = true;
+        if (!done) {
+          RayF nr;
+          nr.o = is.nf.o;
+          nr.d = is.nf.from(bs.dir);
+          nr.t0 = S.min_trace;
+          nr.t1 = S.horizon;
+          store_ray(P, p, nr);
+          P.rr[p] = rrr;
+          bool spec = (bs.flags & F_SPECULAR) != 0;
+          if ((bs.flags & F_TRANSMISSIVE) && HM.medium) {
+            // Media::update_stack_for_transmission (material/media.cc:75-118)
+            if (is.back) {
+              if (depth >= 2) depth--;
+            } else if (depth < SGR_MEDIA_MAX) {
+              P.media[(size_t)depth * N + p] = M.ior_n;
+              depth++;
+            }
+          }
+          nst = (nst & (SGR_ST_PENDING | SGR_ST_STRAT)) | (path_len + 1) |
+                ((spec ? 1u : 0u) << SGR_ST_SPEC_SHIFT) | (depth << SGR_ST_DEPTH_SHIFT);
+          cont = true;
+          if (guided && grec && path_len == (p / G.rec_stride) % 3u) {
+            const float *in = G.in6 + (size_t)j * GD_IN;
+            for (int k = 0; k < GD_IN; k++) grec[k] = in[k];
+            grec[6] = nr.d.x; grec[7] = nr.d.y; grec[8] = nr.d.z;
+            grec[10] = T_vertex.r; grec[11] = T_vertex.g; grec[12] = T_vertex.b;
+            grec[9] = -1.0f;
+          }
+        }
+      }
+    }
+  }
+  if (done) nst |= SGR_ST_DONE;
+  P.state[p] = nst;
+  P.T[0][p] = T.r; P.T[1][p] = T.g; P.T[2][p] = T.b;
+  P.rad[0][p] = rad.r; P.rad[1][p] = rad.g; P.rad[2][p] = rad.b;
+  P.alpha[p] = alpha;
+  return cont;
+}
+
 template<bool MULTI, bool GUIDED>
 __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, StepView J, PathBufs P,
                                                uint32_t b, unsigned long long *stats,
                                                GuideView G) {
   const uint32_t n = P.counts[SGR_CNT_Q + b];
+  if (finishing(P, b, n)) return;
   const uint32_t *q = P.q[b & 1];
   uint32_t *qn = P.q[(b + 1) & 1];
   uint32_t *qn_count = &P.counts[SGR_CNT_Q + b + 1];
   uint32_t *sq_count = &P.counts[SGR_CNT_S + b];
   uint32_t illum = 0;
-  const uint32_t N = P.n;
   uint32_t jb, je;
   block_range(n, jb, je);
   for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
-    uint32_t p = q[j];
-    const RenderView R = MULTI ? J.R[P.pseg[p]] : R1;
-    const SceneView S = MULTI ? J.S[R.scene] : S1;
-    uint32_t st = P.state[p];
-    uint32_t path_len = st & SGR_ST_LEN_MASK;
-    bool after_spec = (st >> SGR_ST_SPEC_SHIFT) & 1u;
-    uint32_t depth = (st >> SGR_ST_DEPTH_SHIFT) & 0xfu;
-    bool include = path_len == 0 || after_spec || R.ds == 0;
-    C3 T = c3(P.T[0][p], P.T[1][p], P.T[2][p]);
-    C3 rad = c3(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
-    float alpha = P.alpha[p];
-    RayF ray;
-    load_ray(P, p, ray);
-    uint32_t prim = P.hprim[p];
-    bool done = false, cont = false;
-    uint32_t nst = st & ~(SGR_ST_PENDING | SGR_ST_DONE | SGR_ST_STRAT);
-    float *grec = nullptr;  // this path's training record (learned proposal)
-    if (GUIDED && p % G.rec_stride == 0) {
-      // slots are shuffled so that every training minibatch spans the image
-      grec = G.rec + (size_t)permute(p / G.rec_stride, G.rec_count, G.rec_key) * GD_REC;
-      if (grec[9] == -1.0f) {
-        // radiance gathered before the recorded continuation's first hit
-        grec[13] = rad.r; grec[14] = rad.g; grec[15] = rad.b;
-        grec[9] = -2.0f;
-      }
-    }
-    if (prim == SGR_MISS) {
-      if (include) rad = rad + background(S, ray.d) * T;
-      if (path_len == 0 && czero(rad)) alpha = R.bg_alpha;
-      done = true;
-    } else {
-      ray.t1 = P.ht[p];
-      Isec is;
-      float mtop = P.media[(size_t)(depth - 1) * N + p];
-      float mencl = depth >= 2 ? P.media[(size_t)(depth - 2) * N + p] : 1.0f;
-      make_isec(S, prim, ray, mtop, mencl, is);
-      const DevMaterial &HM = S.mats[is.mat];
-      if (include && HM.type == 5 && !is.back) rad = rad + cload(HM.color) * T;
-      if (is.bmat < 0) {
-        done = true;
-      } else {
-        const DevMaterial &M = S.mats[is.bmat];
-        int px, py;
-        uint32_t s;
-        path_ident(R, p, px, py, s);
-        uint64_t pk = pixel_key(R.seed, px, py);
-        uint64_t qk = path_key(pk, s);
-        // ---- direct lighting (DirectIllum::sample_all_lights) ----
-        if (R.ds != 0) {
-          illum++;
-          const unsigned dflags = F_ALL & ~F_SPECULAR;
-          bool strat = path_len < R.minp;
-          uint32_t nsub = strat ? R.nsub_strat : R.nsub_rand;
-          for (uint32_t l = 0; l < R.L; l++) {
-            const DevLight &L = S.lights[l];
-            for (uint32_t jj = 0; jj < nsub; jj++) {
-              float lu, lv, bu, bv;
-              uint32_t base = 16 + (l * nsub + jj) * 8;
-              if (strat) {
-                uint32_t chan = 1 + path_len * (2 * R.L + 1) + 2 * l;
-                strat_uv(pk, chan, s * nsub + jj, R.g_dir, lu, lv);
-                strat_uv(pk, chan + 1, s * nsub + jj, R.g_dir, bu, bv);
-              } else {
-                lu = draw(qk, (path_len << 20) | (base + 0));
-                lv = draw(qk, (path_len << 20) | (base + 1));
-                bu = draw(qk, (path_len << 20) | (base + 2));
-                bv = draw(qk, (path_len << 20) | (base + 3));
-              }
-              float extra = draw(qk, (path_len << 20) | (base + 4));
-              size_t slot_a = (size_t)p * P.K + 2 * (l * nsub + jj), slot_b = slot_a + 1;
-              P.svis[slot_a] = 0;
-              P.svis[slot_b] = 0;
-              // light sample (render/direct-illum.cc:147-197)
-              LSample ls = light_sample(S, L, is, lu, lv, extra);
-              if (ls.pdf > 0.0f && cgt0(ls.val)) {
-                BValue bval = bsdf_eval(is, M, ls.dir, dflags);
-                if (cgt0(bval.val)) {
-                  C3 c = ls.val;
-                  c = c * mis_weight(ls.pdf, bval.pdf);
-                  c = c * bval.val;
-                  c = c * fabsf(ls.dir.z);
-                  c = c * (1.0f / ls.pdf);
-                  float md = ls.dist == 0.0f ? S.horizon : ls.dist;
-                  V3 wd = is.nf.from(ls.dir);
-                  P.sray[0][slot_a] = is.nf.o.x; P.sray[1][slot_a] = is.nf.o.y;
-                  P.sray[2][slot_a] = is.nf.o.z; P.sray[3][slot_a] = wd.x;
-                  P.sray[4][slot_a] = wd.y; P.sray[5][slot_a] = wd.z;
-                  P.sray[6][slot_a] = S.min_trace; P.sray[7][slot_a] = md - S.min_trace;
-                  P.sc[0][slot_a] = c.r; P.sc[1][slot_a] = c.g; P.sc[2][slot_a] = c.b;
-                  uint32_t at = wave_append(sq_count, true);
-                  P.sq[at] = (uint32_t)slot_a;
-                }
-              }
-              // BSDF sample (render/direct-illum.cc:199-252)
-              BSample bs = bsdf_sample(is, M, bu, bv, dflags);
-              if (bs.pdf > 0.0f && cgt0(bs.val)) {
-                LSample lval = light_eval(S, L, is, bs.dir);
-                if (lval.pdf > 0.0f && cgt0(lval.val)) {
-                  C3 c = lval.val;
-                  c = c * mis_weight(bs.pdf, lval.pdf);
-                  c = c * bs.val;
-                  c = c * fabsf(bs.dir.z);
-                  c = c * (1.0f / bs.pdf);
-                  float md = lval.dist == 0.0f ? S.horizon : lval.dist;
-                  V3 wd = is.nf.from(bs.dir);
-                  P.sray[0][slot_b] = is.nf.o.x; P.sray[1][slot_b] = is.nf.o.y;
-                  P.sray[2][slot_b] = is.nf.o.z; P.sray[3][slot_b] = wd.x;
-                  P.sray[4][slot_b] = wd.y; P.sray[5][slot_b] = wd.z;
-                  P.sray[6][slot_b] = S.min_trace; P.sray[7][slot_b] = md - S.min_trace;
-                  P.sc[0][slot_b] = c.r; P.sc[1][slot_b] = c.g; P.sc[2][slot_b] = c.b;
-                  uint32_t at = wave_append(sq_count, true);
-                  P.sq[at] = (uint32_t)slot_b;
-                }
-              }
-            }
-          }
-          nst |= SGR_ST_PENDING | (strat ? SGR_ST_STRAT : 0u);
-          P.Tp[0][p] = T.r; P.Tp[1][p] = T.g; P.Tp[2][p] = T.b;
-        }
-        // ---- continuation (render/path-integ.cc:308-401) ----
-        float cu, cv;
-        if (path_len < R.minp) {
-          uint32_t chan = 1 + path_len * (2 * R.L + 1) + 2 * R.L;
-          strat_uv(pk, chan, s, R.g_cont, cu, cv);
-        } else {
-          cu = draw(qk, (path_len << 20) | 1u);
-          cv = draw(qk, (path_len << 20) | 2u);
-        }
-        BSample bs;
-        // Learned proposal: one-sample MIS of the BSDF (probability
-        // G.mix) and the vMF mixture at this vertex; delta BSDFs (glass,
-        // mirror) bypass it.
-        const bool guided = GUIDED && (M.type == 1 || M.type == 2);  // Lambert, Cook-Torrance
-        if (guided) {
-          const float *L = G.lobes + (size_t)j * GD_OUT;
-          const float usel = draw(qk, (path_len << 20) | 3u);
-          if (usel < G.mix) {
-            bs = bsdf_sample(is, M, cu, cv, F_ALL);
-            if (bs.pdf > 0.0f) {
-              const V3 wd = is.nf.from(bs.dir);
-              bs.pdf = G.mix * bs.pdf + (1.0f - G.mix) * mix_pdf(L, wd.x, wd.y, wd.z);
-            }
-          } else {
-            V3 wd;
-            mix_sample(L, draw(qk, (path_len << 20) | 4u), draw(qk, (path_len << 20) | 5u),
-                       draw(qk, (path_len << 20) | 6u), wd.x, wd.y, wd.z);
-            const V3 ld = is.nf.to(wd);
-            const BValue bv = bsdf_eval(is, M, ld, F_ALL);
-            bs.val = bv.val;
-            bs.dir = ld;
-            bs.flags = F_REFLECTIVE | F_DIFFUSE;
-            bs.pdf = czero(bv.val) ? 0.0f
-                                   : G.mix * bv.pdf + (1.0f - G.mix) * mix_pdf(L, wd.x, wd.y, wd.z);
-          }
-        } else {
-          bs = bsdf_sample(is, M, cu, cv, F_ALL);
-        }
-        const C3 T_vertex = T;
-        if (bs.pdf == 0.0f || czero(bs.val)) {
-          done = true;
-        } else {
-          C3 f = bs.val * fabsf(bs.dir.z);
-          f = f * (1.0f / bs.pdf);
-          T = T * f;
-          float rrr = P.rr[p];
-          if (path_len > R.minp) {
-            float pc = fmn(((T.r + T.g) + T.b) / 3.0f * rrr, 1.0f);
-            float rr = draw(qk, (path_len << 20) | 0u);
-            if (rr > pc) {
-              done = true;
-            } else {
-              T = T * (1.0f / pc);
-              rrr = rrr * pc;
-            }
-          }
-          if (!done && path_len == R.maxp) done = true;
-          if (!done) {
-            RayF nr;
-            nr.o = is.nf.o;
-            nr.d = is.nf.from(bs.dir);
-            nr.t0 = S.min_trace;
-            nr.t1 = S.horizon;
-            store_ray(P, p, nr);
-            P.rr[p] = rrr;
-            bool spec = (bs.flags & F_SPECULAR) != 0;
-            if ((bs.flags & F_TRANSMISSIVE) && HM.medium) {
-              // Media::update_stack_for_transmission (material/media.cc:75-118)
-              if (is.back) {
-                if (depth >= 2) depth--;
-              } else if (depth < SGR_MEDIA_MAX) {
-                P.media[(size_t)depth * N + p] = M.ior_n;
-                depth++;
-              }
-            }
-            nst = (nst & (SGR_ST_PENDING | SGR_ST_STRAT)) | (path_len + 1) |
-                  ((spec ? 1u : 0u) << SGR_ST_SPEC_SHIFT) | (depth << SGR_ST_DEPTH_SHIFT);
-            cont = true;
-            if (guided && grec && path_len == (p / G.rec_stride) % 3u) {
-              const float *in = G.in6 + (size_t)j * GD_IN;
-              for (int k = 0; k < GD_IN; k++) grec[k] = in[k];
-              grec[6] = nr.d.x; grec[7] = nr.d.y; grec[8] = nr.d.z;
-              grec[10] = T_vertex.r; grec[11] = T_vertex.g; grec[12] = T_vertex.b;
-              grec[9] = -1.0f;
-            }
-          }
-        }
-      }
-    }
-    if (done) nst |= SGR_ST_DONE;
-    P.state[p] = nst;
-    P.T[0][p] = T.r; P.T[1][p] = T.g; P.T[2][p] = T.b;
-    P.rad[0][p] = rad.r; P.rad[1][p] = rad.g; P.rad[2][p] = rad.b;
-    P.alpha[p] = alpha;
+    const uint32_t p = q[j];
+    const bool cont = shade_path<MULTI, GUIDED>(S1, R1, J, P, G, p, j, illum, [&](size_t slot) {
+      const uint32_t at = wave_append(sq_count, true);
+      P.sq[at] = (uint32_t)slot;
+    });
     // Next ray queue: wave ballot + prefix sum, one atomic per wave.
-    uint32_t at = wave_append(qn_count, cont);
+    const uint32_t at = wave_append(qn_count, cont);
     if (cont) qn[at] = p;
   }
   wave_count(&stats[SGR_STAT_ILLUM], illum);
@@ -457,62 +482,132 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, Step
 // render/path-integ.cc:277-306 product with the path transmittance) and
 // writes finished paths: sx, sy, Tint(radiance, alpha).
 template<bool MULTI, bool GUIDED>
+__device__ __forceinline__ void gather_path(const RenderView &R1, const StepView &J,
+                                            const PathBufs &P, const GuideView &G, uint32_t p) {
+  uint32_t st = P.state[p];
+  if (!(st & (SGR_ST_PENDING | SGR_ST_DONE))) return;
+  const RenderView R = MULTI ? J.R[P.pseg[p]] : R1;
+  C3 rad = c3(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
+  if (st & SGR_ST_PENDING) {
+    uint32_t nsub = (st & SGR_ST_STRAT) ? R.nsub_strat : R.nsub_rand;
+    float inv_n = 1.0f / (float)nsub;
+    C3 D = c3(0.0f);
+    for (uint32_t l = 0; l < R.L; l++) {
+      C3 lrad = c3(0.0f);
+      for (uint32_t jj = 0; jj < nsub; jj++) {
+        size_t sa = (size_t)p * P.K + 2 * (l * nsub + jj);
+        C3 r = c3(0.0f);
+        if (P.svis[sa]) r = r + c3(P.sc[0][sa], P.sc[1][sa], P.sc[2][sa]);
+        if (P.svis[sa + 1]) r = r + c3(P.sc[0][sa + 1], P.sc[1][sa + 1], P.sc[2][sa + 1]);
+        lrad = lrad + r;
+      }
+      D = D + lrad * inv_n;
+    }
+    rad = rad + D * c3(P.Tp[0][p], P.Tp[1][p], P.Tp[2][p]);
+    P.rad[0][p] = rad.r; P.rad[1][p] = rad.g; P.rad[2][p] = rad.b;
+    st &= ~(SGR_ST_PENDING | SGR_ST_STRAT);
+    P.state[p] = st;
+  }
+  if (st & SGR_ST_DONE) {
+    if (GUIDED && p % G.rec_stride == 0) {
+      // training weight: radiance the recorded continuation brought,
+      // over the throughput at its vertex (= f cos L / pdf, averaged)
+      float *grec = G.rec + (size_t)permute(p / G.rec_stride, G.rec_count, G.rec_key) * GD_REC;
+      if (grec[9] == -2.0f) {
+        const float d0 = rad.r - grec[13], d1 = rad.g - grec[14], d2 = rad.b - grec[15];
+        float w = 0.0f;
+        w += grec[10] > 0.0f ? d0 / grec[10] : 0.0f;
+        w += grec[11] > 0.0f ? d1 / grec[11] : 0.0f;
+        w += grec[12] > 0.0f ? d2 / grec[12] : 0.0f;
+        grec[9] = fmaxf(w * (1.0f / 3.0f), 0.0f);
+      } else if (grec[9] < 0.0f) {
+        grec[9] = 0.0f;
+      }
+    }
+    float a = P.alpha[p];
+    float *o = P.out + 6 * (size_t)p;
+    o[2] = rad.r * a;
+    o[3] = rad.g * a;
+    o[4] = rad.b * a;
+    o[5] = a;
+  }
+}
+
+template<bool MULTI, bool GUIDED>
 __global__ void __launch_bounds__(256) k_gather(RenderView R1, StepView J, PathBufs P, uint32_t b,
                                                 GuideView G) {
   const uint32_t n = P.counts[SGR_CNT_Q + b];
+  if (finishing(P, b, n)) return;
   const uint32_t *q = P.q[b & 1];
   uint32_t jb, je;
   block_range(n, jb, je);
+  for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x)
+    gather_path<MULTI, GUIDED>(R1, J, P, G, q[j]);
+}
+
+// Finishes every path of a short bounce-B queue in one launch: each
+// thread takes one path through all its remaining bounces (closest hit,
+// the vertex, its shadow rays, the gather), in the same order and with
+// the same per-path code as the per-bounce kernels, so the results are
+// the same bit for bit.  Late bounces hold few rays whose walks are long
+// chains of dependent loads; bounce by bounce, every launch would wait
+// for its slowest ray, here only each path's own chain counts.  Rays and
+// shadow rays traced after bounce B are counted here (the queue counts
+// of those bounces stay zero).
+template<bool MULTI>
+__global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, StepView J,
+                                                PathBufs P, uint32_t b,
+                                                unsigned long long *stats) {
+  const uint32_t n = P.counts[SGR_CNT_Q + b];
+  if (!finishing(P, b, n)) return;
+  __shared__ float4 stk_lds[2 * SGR_LDS_SLOTS * SGR_TRACE_BLOCK];
+  const LdsStack stk{&stk_lds[threadIdx.x]};
+  float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
+  const uint32_t *q = P.q[b & 1];
+  const GuideView G{};
+  TraceCount cc{0, 0}, cs{0, 0};
+  uint32_t illum = 0, rays = 0, shadow = 0;
+  uint32_t jb, je;
+  block_range(n, jb, je);
   for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
-    uint32_t p = q[j];
-    uint32_t st = P.state[p];
-    if (!(st & (SGR_ST_PENDING | SGR_ST_DONE))) continue;
-    const RenderView R = MULTI ? J.R[P.pseg[p]] : R1;
-    C3 rad = c3(P.rad[0][p], P.rad[1][p], P.rad[2][p]);
-    if (st & SGR_ST_PENDING) {
-      uint32_t nsub = (st & SGR_ST_STRAT) ? R.nsub_strat : R.nsub_rand;
-      float inv_n = 1.0f / (float)nsub;
-      C3 D = c3(0.0f);
-      for (uint32_t l = 0; l < R.L; l++) {
-        C3 lrad = c3(0.0f);
-        for (uint32_t jj = 0; jj < nsub; jj++) {
-          size_t sa = (size_t)p * P.K + 2 * (l * nsub + jj);
-          C3 r = c3(0.0f);
-          if (P.svis[sa]) r = r + c3(P.sc[0][sa], P.sc[1][sa], P.sc[2][sa]);
-          if (P.svis[sa + 1]) r = r + c3(P.sc[0][sa + 1], P.sc[1][sa + 1], P.sc[2][sa + 1]);
-          lrad = lrad + r;
-        }
-        D = D + lrad * inv_n;
+    const uint32_t p = q[j];
+    const SceneView *Sp = MULTI ? &J.S[J.R[P.pseg[p]].scene] : &S1;
+    for (;;) {
+      const SceneView &S = *Sp;
+      TravState T;
+      load_ray(P, p, T.ray);
+      trace_one<false>(S, T, cc, stk, st0, st1);
+      P.hprim[p] = T.hit;
+      P.ht[p] = T.ray.t1;
+      P.hu[p] = T.hu;
+      P.hv[p] = T.hv;
+      uint64_t em = 0;  // shadow slots of the path written at this vertex
+      const bool cont = shade_path<MULTI, false>(S1, R1, J, P, G, p, j, illum, [&](size_t slot) {
+        em |= 1ull << (uint32_t)(slot - (size_t)p * P.K);
+      });
+      for (uint64_t m = em; m; m &= m - 1) {
+        const size_t slot = (size_t)p * P.K + (uint32_t)__builtin_ctzll(m);
+        TravState U;
+        U.ray.o = v3(P.sray[0][slot], P.sray[1][slot], P.sray[2][slot]);
+        U.ray.d = v3(P.sray[3][slot], P.sray[4][slot], P.sray[5][slot]);
+        U.ray.t0 = P.sray[6][slot];
+        U.ray.t1 = P.sray[7][slot];
+        trace_one<true>(S, U, cs, stk, st0, st1);
+        P.svis[slot] = U.hit == SGR_MISS ? 1 : 0;
+        shadow++;
       }
-      rad = rad + D * c3(P.Tp[0][p], P.Tp[1][p], P.Tp[2][p]);
-      P.rad[0][p] = rad.r; P.rad[1][p] = rad.g; P.rad[2][p] = rad.b;
-      st &= ~(SGR_ST_PENDING | SGR_ST_STRAT);
-      P.state[p] = st;
-    }
-    if (st & SGR_ST_DONE) {
-      if (GUIDED && p % G.rec_stride == 0) {
-        // training weight: radiance the recorded continuation brought,
-        // over the throughput at its vertex (= f cos L / pdf, averaged)
-        float *grec = G.rec + (size_t)permute(p / G.rec_stride, G.rec_count, G.rec_key) * GD_REC;
-        if (grec[9] == -2.0f) {
-          const float d0 = rad.r - grec[13], d1 = rad.g - grec[14], d2 = rad.b - grec[15];
-          float w = 0.0f;
-          w += grec[10] > 0.0f ? d0 / grec[10] : 0.0f;
-          w += grec[11] > 0.0f ? d1 / grec[11] : 0.0f;
-          w += grec[12] > 0.0f ? d2 / grec[12] : 0.0f;
-          grec[9] = fmaxf(w * (1.0f / 3.0f), 0.0f);
-        } else if (grec[9] < 0.0f) {
-          grec[9] = 0.0f;
-        }
-      }
-      float a = P.alpha[p];
-      float *o = P.out + 6 * (size_t)p;
-      o[2] = rad.r * a;
-      o[3] = rad.g * a;
-      o[4] = rad.b * a;
-      o[5] = a;
+      gather_path<MULTI, false>(R1, J, P, G, p);
+      if (!cont) break;
+      rays++;
     }
   }
+  wave_count(&stats[SGR_STAT_NODES_C], cc.nodes);
+  wave_count(&stats[SGR_STAT_TESTS_C], cc.tests);
+  wave_count(&stats[SGR_STAT_NODES_S], cs.nodes);
+  wave_count(&stats[SGR_STAT_TESTS_S], cs.tests);
+  wave_count(&stats[SGR_STAT_ILLUM], illum);
+  wave_count(&stats[SGR_STAT_ISECT], rays);
+  wave_count(&stats[SGR_STAT_SHADOW], shadow);
 }
 
 // image/image-mitchell-filt.h:45-65
@@ -732,6 +827,15 @@ void launch_splat(bool multi, const RenderView &R, const StepView &J, const floa
   if (multi) hipLaunchKernelGGL(k_splat<true>, dim3(grid), dim3(256), 0, st, R, J, out);
   else hipLaunchKernelGGL(k_splat<false>, dim3(grid), dim3(256), 0, st, R, J, out);
 }
+void launch_finish(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
+                   const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
+                   hipStream_t st) {
+  if (multi)
+    hipLaunchKernelGGL((k_finish<true>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats);
+  else
+    hipLaunchKernelGGL((k_finish<false>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats);
+}
+
 void launch_count_queues(uint32_t maxp, const PathBufs &P, unsigned long long *stats,
                          hipStream_t st) {
   hipLaunchKernelGGL(k_count_queues, dim3(1), dim3(64), 0, st, maxp, P, stats);

@@ -139,6 +139,10 @@ def test_pixel_means_within_3se_of_oracle(ctx):
     assert np.mean(z <= 3) > 0.97
 
 
+COUNTERS = ("scene_intersect_calls", "shadow_tests", "node_visits_closest", "surf_tests_closest",
+            "node_visits_shadow", "surf_tests_shadow", "illum_calls")
+
+
 def test_determinism_and_chunking(ctx):
     """Same seed: identical samples whatever the number of paths in flight;
     repeated renders give identical accumulators."""
@@ -160,6 +164,15 @@ def test_determinism_and_chunking(ctx):
                          flags=abi.SGR_RENDER_ONE_CHUNK_IN_FLIGHT)
     w, _ = sc.render(cam, p3)
     assert np.array_equal(w.view(np.uint32), z.view(np.uint32))
+    # short late queues finished in one launch or bounce by bounce: the
+    # same per-path work, so the same bits and the same counters
+    p4 = abi.make_params(40, 30, 9, max_path_len=8, seed=5, paths_in_flight=2000,
+                         flags=abi.SGR_RENDER_WAVEFRONT_ONLY)
+    v, sv = sc.render(cam, p4)
+    _, sz = sc.render(cam, p2)
+    assert np.array_equal(v.view(np.uint32), z.view(np.uint32))
+    for k in COUNTERS:
+        assert sv[k] == sz[k], k
 
 
 def test_window_tiles_equal_full_render(ctx):
@@ -193,21 +206,27 @@ def test_invalid_parameters_raise(ctx):
 
 def test_full_size_frame_properties(ctx):
     """The 1.31M-triangle soup at 1920x1080, 64 spp (the benchmark frame):
-    deterministic (with one or two chunks in flight), chunking changes
-    only the summation order, every pixel
+    deterministic (with one or two chunks in flight, with or without the
+    one-launch finish of short late queues), chunking changes only the
+    summation order, every pixel
     gets filter weight, and without the negative-lobe clamp the weights
     add up to the sample count (the Mitchell filter's integer shifts sum
     to one)."""
     sd, cam = scenes.icosphere_soup()
     sc = native.Scene(ctx, sd)
     W, H, S = 1920, 1080, 64
-    accs = []
-    for pif, fl in ((0, 0), (0, abi.SGR_RENDER_ONE_CHUNK_IN_FLIGHT), (1 << 24, 0)):
-        acc, _ = sc.render(cam, abi.make_params(W, H, S, max_path_len=10, seed=1,
-                                                paths_in_flight=pif, flags=fl))
+    accs, sts = [], []
+    for pif, fl in ((0, 0), (0, abi.SGR_RENDER_ONE_CHUNK_IN_FLIGHT), (1 << 24, 0),
+                    (0, abi.SGR_RENDER_WAVEFRONT_ONLY)):
+        acc, st = sc.render(cam, abi.make_params(W, H, S, max_path_len=10, seed=1,
+                                                 paths_in_flight=pif, flags=fl))
         accs.append(acc)
-    a, b, c = accs
+        sts.append(st)
+    a, b, c, e = accs
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(a.view(np.uint32), e.view(np.uint32))
+    for k in COUNTERS:
+        assert sts[0][k] == sts[3][k], k
     np.testing.assert_allclose(c, a, rtol=2e-5, atol=1e-5)
     assert np.isfinite(a).all()
     w = a[..., 4]
