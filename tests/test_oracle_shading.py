@@ -374,6 +374,55 @@ def test_quad_light_direct_closed_form():
     assert abs(diff.mean()) < 3 * se + 1e-4 * expect.mean()
 
 
+def _envmap_direct_expectation(rho, rows):
+    """Expectation (per unit radiance) of the direct-lighting estimate on
+    a Lambert floor (normal = the map's pole) under a uniform envmap of
+    ROWS latitude rows, with the reference's envmap pdf: light samples
+    drawn from the latitude-row histogram (true solid-angle density
+    q = pdf_uv / (2 pi^2 cos lat)) but weighted with p_l = pdf_uv / (4 pi)
+    (light/envmap-light.cc), MIS (power heuristic) against cosine-weighted
+    BSDF samples p_b = cos t / pi:
+        rho / pi * Int_{H+} cos t (q p_l + p_b^2) / (p_l^2 + p_b^2) dw,
+    by quadrature in latitude within each row."""
+    lat_r = ((np.arange(rows) + 0.5) / rows - 0.5) * np.pi
+    pdf_uv = np.cos(lat_r) / np.mean(np.cos(lat_r))   # bin pdf of a uniform map
+    total = 0.0
+    for r in range(rows):
+        lo = -np.pi / 2 + r * np.pi / rows
+        hi = lo + np.pi / rows
+        if hi <= 0:
+            continue
+        edges = np.linspace(max(lo, 0.0), hi, 4001)
+        lat = 0.5 * (edges[1:] + edges[:-1])
+        dl = edges[1] - edges[0]
+        pl = pdf_uv[r] / (4 * np.pi)
+        q = pdf_uv[r] / (2 * np.pi ** 2 * np.cos(lat))
+        pb = np.sin(lat) / np.pi
+        f = np.sin(lat) * (q * pl + pb ** 2) / (pl ** 2 + pb ** 2)
+        total += np.sum(f * np.cos(lat)) * dl * 2 * np.pi
+    return rho / np.pi * total
+
+
+def test_envmap_light_direct_matches_quadrature():
+    """Uniform envmap over a Lambert floor, direct lighting only: the
+    oracle's mean matches the quadrature of the estimator with the
+    reference's envmap pdf pdf_uv / (4 pi) (0.474 E for rho = 0.5), and
+    not rho E, which the true density would give -- so this pins that
+    expression, the histogram sampling under it and the MIS weights."""
+    E, rho = 2.0, 0.5
+    sd = scenes.SceneDesc()
+    floor = sd.lambert((rho, rho, rho))
+    sd.tripar((-50, 0, -50), (100, 0, 0), (0, 0, 100), floor, parallelogram=True)
+    sd.envmap(np.full((32, 64, 3), E, np.float32))
+    cam = scenes.camera((0.0, 1.5, 0.0), (0, -1, 0), (0, 0, 1), (1, 0, 0), 0.6, 0.6)
+    smp, _ = _direct_only(sd, cam, spp=256, W=16, H=16)
+    got = smp[:, 2:5].astype(float).mean(1)
+    mean, se = got.mean(), got.std() / math.sqrt(len(got))
+    expect = E * _envmap_direct_expectation(rho, 32)
+    assert abs(mean - expect) < 3 * se + 1e-4 * expect, (mean, expect, se)
+    assert abs(mean - rho * E) > 10 * se, (mean, se)
+
+
 def test_furnace_closed_form_on_oracle():
     """Closed box of emitting Lambert walls (E = 1, albedo a): the path
     estimate has expectation E * sum_{j=0}^{M+1} a^j; also in FP64."""
