@@ -10,6 +10,7 @@ namespace sgr {
 
 #define SGR_MEDIA_MAX 8       // media stack levels per path
 #define SGR_MAX_SLOTS 64      // shadow-ray slots per path (2 per light sample)
+#define SGR_COMPACT_ITEMS 4096  // queue positions per compaction block (256 x 16)
 
 // Path state word.
 #define SGR_ST_LEN_MASK   0xffu
@@ -92,6 +93,12 @@ struct PathBufs {
   uint8_t *svis;
   uint32_t *q[2];     // ray queues (ping-pong)
   uint32_t *sq;       // shadow queue
+  // Ordered compaction of the queues k_shade produces: per queue
+  // position, whether the path goes on and which of its shadow slots
+  // got a ray; per block of SGR_COMPACT_ITEMS positions, the two counts
+  // (then their exclusive prefixes).
+  uint8_t *qflag, *sflag;
+  uint32_t *cblk;
   uint32_t *counts;   // SGR_NUM_COUNTERS
   float *out;         // 6 floats per path: sx sy r g b a
   uint32_t *pseg;     // segment of each path (multi-job steps)

@@ -745,6 +745,10 @@ size_t path_bytes(size_t cap, size_t K, PathBufs *P, char *base) {
   r = take(cap * 4); if (P) P->q[0] = (uint32_t *)r;
   r = take(cap * 4); if (P) P->q[1] = (uint32_t *)r;
   r = take(ns * 4); if (P) P->sq = (uint32_t *)r;
+  r = take(cap); if (P) P->qflag = (uint8_t *)r;
+  r = take(ns); if (P) P->sflag = (uint8_t *)r;
+  r = take(((cap + SGR_COMPACT_ITEMS - 1) / SGR_COMPACT_ITEMS) * 2 * 4);
+  if (P) P->cblk = (uint32_t *)r;
   r = take(SGR_NUM_COUNTERS * 4); if (P) P->counts = (uint32_t *)r;
   r = take(cap * 6 * 4); if (P) P->out = (float *)r;
   r = take(cap * 4); if (P) P->pseg = (uint32_t *)r;

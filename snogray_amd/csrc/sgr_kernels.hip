@@ -459,23 +459,165 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, Step
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   if (finishing(P, b, n)) return;
   const uint32_t *q = P.q[b & 1];
-  uint32_t *qn = P.q[(b + 1) & 1];
-  uint32_t *qn_count = &P.counts[SGR_CNT_Q + b + 1];
-  uint32_t *sq_count = &P.counts[SGR_CNT_S + b];
+  const size_t K = P.K;
   uint32_t illum = 0;
   uint32_t jb, je;
   block_range(n, jb, je);
   for (uint32_t j = jb + threadIdx.x; j < je; j += blockDim.x) {
     const uint32_t p = q[j];
+    uint8_t *sf = P.sflag + (size_t)j * K;
+    for (size_t k = 0; k < K; k++) sf[k] = 0;
     const bool cont = shade_path<MULTI, GUIDED>(S1, R1, J, P, G, p, j, illum, [&](size_t slot) {
-      const uint32_t at = wave_append(sq_count, true);
-      P.sq[at] = (uint32_t)slot;
+      sf[slot - (size_t)p * K] = 1;
     });
-    // Next ray queue: wave ballot + prefix sum, one atomic per wave.
-    const uint32_t at = wave_append(qn_count, cont);
-    if (cont) qn[at] = p;
+    P.qflag[j] = cont ? 1 : 0;
   }
   wave_count(&stats[SGR_STAT_ILLUM], illum);
+}
+
+// Ordered compaction of what k_shade flagged: the next ray queue (paths
+// that go on) and the shadow queue (slots with a ray), both in the order
+// of the queue they come from.  Queues thus stay in raygen's pixel order
+// bounce after bounce, so the slice of a queue each XCD takes (see
+// block_range) keeps covering the same part of the image, and k_shade
+// needs no atomics.  A block takes SGR_COMPACT_ITEMS consecutive
+// positions, each of its 4 waves a run of 1024, 64 at a time (lane l
+// position base + l, coalesced flag loads): a count pass, one block
+// scanning the block counts, a scatter pass.
+#define SGR_COMPACT_WAVE (SGR_COMPACT_ITEMS / 4)
+
+// Exclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
+  const uint32_t lane = __lane_id();
+  uint32_t x = v;
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+    if (lane >= off) x += y;
+  }
+  return x - v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  for (uint32_t off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
+  return v;
+}
+
+// Continuing paths and shadow rays among the wave's run of positions.
+__device__ __forceinline__ void compact_wave_counts(const PathBufs &P, uint32_t n, uint32_t base,
+                                                    uint32_t &cq, uint32_t &cs) {
+  const uint32_t lane = __lane_id(), K = P.K;
+  uint32_t q = 0, sh = 0;
+  for (uint32_t it = 0; it < SGR_COMPACT_WAVE; it += 64) {
+    const uint32_t j = base + it + lane;
+    if (j < n) {
+      q += P.qflag[j];
+      const uint8_t *sf = P.sflag + (size_t)j * K;
+      for (uint32_t k = 0; k < K; k++) sh += sf[k];
+    }
+  }
+  cq = wave_sum(q);
+  cs = wave_sum(sh);
+}
+
+__global__ void __launch_bounds__(256) k_compact_count(PathBufs P, uint32_t b) {
+  __shared__ uint32_t part[8];
+  const uint32_t n = P.counts[SGR_CNT_Q + b];
+  if (finishing(P, b, n) || blockIdx.x * SGR_COMPACT_ITEMS >= n) return;
+  const uint32_t w = threadIdx.x >> 6;
+  uint32_t cq, cs;
+  compact_wave_counts(P, n, blockIdx.x * SGR_COMPACT_ITEMS + w * SGR_COMPACT_WAVE, cq, cs);
+  if (__lane_id() == 0) {
+    part[2 * w] = cq;
+    part[2 * w + 1] = cs;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    P.cblk[2 * blockIdx.x] = (part[0] + part[2]) + (part[4] + part[6]);
+    P.cblk[2 * blockIdx.x + 1] = (part[1] + part[3]) + (part[5] + part[7]);
+  }
+}
+
+// One block: exclusive prefixes of the block counts (in place) and the
+// queue lengths.
+__global__ void __launch_bounds__(256) k_compact_scan(PathBufs P, uint32_t b) {
+  __shared__ uint32_t lds[512];
+  const uint32_t n = P.counts[SGR_CNT_Q + b];
+  if (finishing(P, b, n)) return;
+  const uint32_t t = threadIdx.x;
+  const uint32_t nblk = (n + SGR_COMPACT_ITEMS - 1) / SGR_COMPACT_ITEMS;
+  const uint32_t per = (nblk + 255) / 256;
+  const uint32_t i0 = t * per < nblk ? t * per : nblk;
+  const uint32_t i1 = i0 + per < nblk ? i0 + per : nblk;
+  uint32_t cq = 0, cs = 0;
+  for (uint32_t i = i0; i < i1; i++) {
+    cq += P.cblk[2 * i];
+    cs += P.cblk[2 * i + 1];
+  }
+  // inclusive scan over the 256 threads (Hillis-Steele in LDS)
+  lds[t] = cq;
+  lds[256 + t] = cs;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint32_t va = t >= off ? lds[t - off] : 0u, vb = t >= off ? lds[256 + t - off] : 0u;
+    __syncthreads();
+    lds[t] += va;
+    lds[256 + t] += vb;
+    __syncthreads();
+  }
+  uint32_t oq = lds[t] - cq, os = lds[256 + t] - cs;
+  for (uint32_t i = i0; i < i1; i++) {
+    const uint32_t a = P.cblk[2 * i], c = P.cblk[2 * i + 1];
+    P.cblk[2 * i] = oq;
+    P.cblk[2 * i + 1] = os;
+    oq += a;
+    os += c;
+  }
+  if (t == 0) {
+    P.counts[SGR_CNT_Q + b + 1] = lds[255];
+    P.counts[SGR_CNT_S + b] = lds[511];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_compact_scatter(PathBufs P, uint32_t b) {
+  __shared__ uint32_t part[8];
+  const uint32_t n = P.counts[SGR_CNT_Q + b];
+  if (finishing(P, b, n) || blockIdx.x * SGR_COMPACT_ITEMS >= n) return;
+  const uint32_t w = threadIdx.x >> 6, lane = __lane_id(), K = P.K;
+  const uint32_t base = blockIdx.x * SGR_COMPACT_ITEMS + w * SGR_COMPACT_WAVE;
+  uint32_t cq, cs;
+  compact_wave_counts(P, n, base, cq, cs);
+  if (lane == 0) {
+    part[2 * w] = cq;
+    part[2 * w + 1] = cs;
+  }
+  __syncthreads();
+  uint32_t oq = P.cblk[2 * blockIdx.x], os = P.cblk[2 * blockIdx.x + 1];
+  for (uint32_t v = 0; v < w; v++) {
+    oq += part[2 * v];
+    os += part[2 * v + 1];
+  }
+  const uint32_t *q = P.q[b & 1];
+  uint32_t *qn = P.q[(b + 1) & 1];
+  for (uint32_t it = 0; it < SGR_COMPACT_WAVE; it += 64) {
+    const uint32_t j = base + it + lane;
+    const bool in = j < n;
+    const uint32_t p = in ? q[j] : 0u;
+    const bool go = in && P.qflag[j];
+    const uint64_t m = __ballot(go);
+    if (go)
+      qn[oq + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = p;
+    oq += (uint32_t)__popcll(m);
+    const uint8_t *sf = P.sflag + (size_t)(in ? j : 0u) * K;
+    uint32_t c = 0;
+    if (in)
+      for (uint32_t k = 0; k < K; k++) c += sf[k];
+    uint32_t at = os + wave_excl_scan(c);
+    if (c)
+      for (uint32_t k = 0; k < K; k++)
+        if (sf[k]) P.sq[at++] = (uint32_t)((size_t)p * K + k);
+    os += wave_sum(c);
+  }
 }
 
 // Adds the unoccluded direct terms (render/direct-illum.cc:96-113 sums,
@@ -795,6 +937,11 @@ void launch_shade(bool multi, const SceneView &S, const RenderView &R, const Ste
     hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats, g);
   else
     hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats, g);
+  // the next ray queue and this bounce's shadow queue, in order
+  const unsigned nblk = (P.n + SGR_COMPACT_ITEMS - 1) / SGR_COMPACT_ITEMS;
+  hipLaunchKernelGGL(k_compact_count, dim3(nblk), dim3(256), 0, st, P, b);
+  hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(256), 0, st, P, b);
+  hipLaunchKernelGGL(k_compact_scatter, dim3(nblk), dim3(256), 0, st, P, b);
 }
 void launch_occlude(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                     uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
