@@ -176,21 +176,4 @@ __device__ __forceinline__ void strat_uv(uint64_t pk, uint32_t chan, uint32_t e,
   v = clamp01f((float)i * vs + draw(ck, 2 * e + 1) * vs);
 }
 
-// ---------------------------------------------------------------------------
-// Wave-level queue append: ballot + lane prefix + one atomic per wave.
-// Must be called by all active lanes of the wave.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
-  uint64_t mask = __ballot(pred);
-  if (mask == 0) return 0xffffffffu;
-  uint32_t lane = __lane_id();
-  uint32_t prefix = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-  int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader);
-  return pred ? base + prefix : 0xffffffffu;
-}
-
 }  // namespace sgr

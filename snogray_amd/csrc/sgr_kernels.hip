@@ -1,7 +1,7 @@
 // gfx950 kernels of the path-tracing core.
 //
-// Ray batches: k_trace_closest / k_trace_any (Space::intersect /
-// Space::occludes, space/space.cc:55-66, :182-194).
+// Ray batches: k_trace<ANY> (Space::intersect / Space::occludes,
+// space/space.cc:55-66, :182-194).
 //
 // Wavefront path tracer (PathInteg::Li, render/path-integ.cc:133-407,
 // restructured): one path per (pattern pixel, sample) of a chunk of
@@ -10,13 +10,16 @@
 //   k_shade    emission, direct lighting (DirectIllum::sample_light,
 //              render/direct-illum.cc:126-255: one light sample and one
 //              BSDF sample per light, power-heuristic MIS), BSDF
-//              continuation, Russian roulette; appends shadow rays and the
-//              next ray queue with wave ballots + prefix sums
+//              continuation, Russian roulette; flags the shadow rays it
+//              wrote and whether the path goes on
+//   k_compact_* ordered compaction of those flags into the shadow queue
+//              and the next ray queue (block scan + wave ballots)
 //   k_occlude  any hit of every queued shadow ray
 //   k_gather   adds the unoccluded direct terms in the reference's order,
 //              writes finished paths
-// then k_splat gathers the chunk's samples into the image accumulators
-// with the Mitchell filter (image/image-filter-conv.h:109-186).
+// with k_finish taking over once a late queue is short; then k_splat
+// gathers the chunk's samples into the image accumulators with the
+// Mitchell filter (image/image-filter-conv.h:109-186).
 #include <cstring>
 
 #include "dev_shade.h"
