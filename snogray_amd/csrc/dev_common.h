@@ -11,6 +11,7 @@
 
 namespace sgr {
 
+#define SGR_REF_STRIDE 3  // float4s per surface reference on the device
 #define SGR_MAX_DEPTH 64  // octree levels a traversal can hold (checked at build)
 
 // ---------------------------------------------------------------------------

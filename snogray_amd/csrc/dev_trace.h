@@ -75,11 +75,11 @@ __device__ __forceinline__ bool sphere_hit(V3 c, float radius, V3 ro, V3 rd, flo
 // kind: 0 triangle, 1 parallelogram, 2 sphere.
 __device__ __forceinline__ bool ref_hit(const float4 *refs, uint32_t k, const RayF &r,
                                         float &t, float &u, float &v, uint32_t &prim) {
-  float4 a = refs[3 * k], b = refs[3 * k + 1];
+  float4 a = refs[SGR_REF_STRIDE * k], b = refs[SGR_REF_STRIDE * k + 1];
   uint32_t kind = __float_as_uint(b.w);
   prim = __float_as_uint(a.w);
   if (kind != 2u) {
-    float4 c = refs[3 * k + 2];
+    float4 c = refs[SGR_REF_STRIDE * k + 2];
     return tripar_hit(ld3(a), ld3(b), ld3(c), kind == 1u, r, t, u, v);
   }
   u = 0.0f;

@@ -505,7 +505,7 @@ int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
   const std::vector<uint32_t> &surfs = V.use_triv ? H.order : sc->oct.surf_ids;
   // Surface references in list order with their geometry inline, so a
   // node's surfaces are contiguous (kind in the second record's w).
-  std::vector<float4> refs(surfs.size() * 3);
+  std::vector<float4> refs(surfs.size() * SGR_REF_STRIDE, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
   for (size_t k = 0; k < surfs.size(); k++) {
     uint32_t q = surfs[k];
     float pb, kind;
@@ -515,16 +515,16 @@ int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
       const float4 &a = H.tri[3 * q], &b = H.tri[3 * q + 1], &c = H.tri[3 * q + 2];
       kd = a.w != 0.0f ? 1u : 0u;
       std::memcpy(&kind, &kd, 4);
-      refs[3 * k] = make_float4(a.x, a.y, a.z, pb);
-      refs[3 * k + 1] = make_float4(b.x, b.y, b.z, kind);
-      refs[3 * k + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+      refs[SGR_REF_STRIDE * k] = make_float4(a.x, a.y, a.z, pb);
+      refs[SGR_REF_STRIDE * k + 1] = make_float4(b.x, b.y, b.z, kind);
+      refs[SGR_REF_STRIDE * k + 2] = make_float4(c.x, c.y, c.z, 0.0f);
     } else {
       const float4 &sp = H.sph[q - V.n_tri_like];
       kd = 2u;
       std::memcpy(&kind, &kd, 4);
-      refs[3 * k] = make_float4(sp.x, sp.y, sp.z, pb);
-      refs[3 * k + 1] = make_float4(sp.w, 0.0f, 0.0f, kind);
-      refs[3 * k + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      refs[SGR_REF_STRIDE * k] = make_float4(sp.x, sp.y, sp.z, pb);
+      refs[SGR_REF_STRIDE * k + 1] = make_float4(sp.w, 0.0f, 0.0f, kind);
+      refs[SGR_REF_STRIDE * k + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
   }
   // Uploads go through a stream of their own: a scene may be created by
