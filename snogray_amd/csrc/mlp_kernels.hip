@@ -189,12 +189,15 @@ k_mlp_forward(GuideNet net, const float *in6, uint32_t n0, const uint32_t *count
       for (int m = 0; m < GD_OUTP / 16; m++)
         for (int i = 0; i < 4; i++) ob[(t * 16 + r) * GD_OUTP + m * 16 + 4 * g + i] = o[t][m][i];
     WAVE_SYNC();
-    const uint32_t row = row0 + lane;
-    if (row < n) {
-      float *dst = out + (size_t)row * GD_OUT;
-      if (lobes) to_lobes(ob + lane * GD_OUTP, dst);
-      else
-        for (int k = 0; k < GD_OUT; k++) dst[k] = ob[lane * GD_OUTP + k];
+    // each lane turns its row into lobes in place, then the wave writes
+    // the 64 rows' outputs (contiguous in memory) with coalesced stores
+    if (lobes && row0 + lane < n) to_lobes(ob + lane * GD_OUTP, ob + lane * GD_OUTP);
+    WAVE_SYNC();
+    const uint32_t rows = n - row0 < 16 * FT ? n - row0 : 16 * FT;
+    float *dst = out + (size_t)row0 * GD_OUT;
+    for (uint32_t idx = lane; idx < rows * GD_OUT; idx += 64) {
+      const uint32_t rr = idx / GD_OUT, c = idx - rr * GD_OUT;
+      dst[idx] = ob[rr * GD_OUTP + c];
     }
     WAVE_SYNC();
   }
