@@ -848,11 +848,13 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   // Up to SGR_CHUNK_LANES chunk lanes (own path buffers, own stream)
   // when the frame has several chunks: chunks run concurrently, so one
   // chunk's sparse, latency-bound last bounces overlap another's full
-  // ones.  Guided and per-sample renders keep one lane (they depend on
-  // chunk order).
+  // ones.  Per-sample renders keep one lane (they copy each chunk out).
+  // Guided renders use two: the chunks of a pair render concurrently with
+  // the same weights, and the guide trains on both after the pair.
   const uint32_t nchunks = (R.spp + pl.sb_max - 1) / pl.sb_max;
-  const int lanes = (guide || samples_host || (pl.flags & SGR_RENDER_ONE_CHUNK_IN_FLIGHT))
-                        ? 1 : (int)std::min<uint32_t>(nchunks, SGR_CHUNK_LANES);
+  const int max_lanes = guide ? std::min(SGR_CHUNK_LANES, 2) : SGR_CHUNK_LANES;
+  const int lanes = (samples_host || (pl.flags & SGR_RENDER_ONE_CHUNK_IN_FLIGHT))
+                        ? 1 : (int)std::min<uint32_t>(nchunks, (uint32_t)max_lanes);
   size_t need = (path_bytes(cap, R.K, nullptr, nullptr) + 255) & ~(size_t)255;
   int rc = ensure_work(ctx, need * lanes);
   if (rc) return rc;
@@ -862,12 +864,12 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     path_bytes(cap, R.K, &PL[l], (char *)ctx->work + need * l);
     lane_st[l] = l ? ctx->lane_stream[l - 1] : ctx->stream;
   }
-  GuideView GV;
-  if (guide) {
-    rc = guide_prepare(guide, sc, cap, GV);
+  GuideView GV[2];
+  for (int l = 0; guide && l < lanes; l++) {
+    rc = guide_prepare(guide, sc, cap, l, GV[l]);
     if (rc) return rc;
   }
-  const GuideView *G = guide ? &GV : nullptr;
+  hipEvent_t ev_trained = nullptr;  // the last training step of the guide
   HIPCHK(hipMemsetAsync(ctx->stats, 0, SGR_NUM_STATS * sizeof(unsigned long long), ctx->stream));
   unsigned grid = (unsigned)std::min<size_t>((cap + 255) / 256, (size_t)ctx->num_cu * 16);
   grid = (grid + 7) & ~7u;  // whole blocks per XCD (see block_range)
@@ -939,8 +941,13 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     R.sb = std::min(pl.sb_max, R.spp - s0);
     R.npaths = pl.npix * R.sb;
     P.n = (uint32_t)cap;
+    const GuideView *G = guide ? &GV[lane] : nullptr;
     if (guide) {
-      rc = guide_begin_chunk(guide, GV, R.npaths, R.seed * 0x9e3779b97f4a7c15ULL + s0 + 1);
+      // lane 1 waits for the training on the previous pair (which has also
+      // read its records); lane 0 is the training stream itself
+      if (lane && ev_trained) HIPCHK(hipStreamWaitEvent(cur, ev_trained, 0));
+      rc = guide_begin_chunk(guide, GV[lane], R.npaths, R.seed * 0x9e3779b97f4a7c15ULL + s0 + 1,
+                             cur);
       if (rc) return rc;
     }
     t_begin();
@@ -957,7 +964,7 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
       t_end(K_EXT);
       if (guide) {
         t_begin();
-        guide_infer(guide, GV, P, b, grid);
+        guide_infer(guide, *G, P, b, grid, cur);
         t_end(K_OTHER);
       }
       t_begin();
@@ -985,11 +992,22 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     }
     launch_count_queues(R.maxp, P, ctx->stats, cur);
     HIPCHK(hipGetLastError());
-    if (guide && train) {
+    if (guide && train && (lane == lanes - 1 || s0 + pl.sb_max >= R.spp)) {
+      // the pair (or last chunk) is done: train on its chunks in order
+      if (lane) {
+        hipEvent_t e = sync_get();
+        HIPCHK(hipEventRecord(e, cur));
+        HIPCHK(hipStreamWaitEvent(ctx->stream, e, 0));
+      }
+      cur = ctx->stream;
       t_begin();
-      rc = guide_train_chunk(guide, GV, GV.rec_count);
+      for (int l = 0; l <= lane && !rc; l++) rc = guide_train_chunk(guide, GV[l], GV[l].rec_count);
       t_end(K_OTHER);
       if (rc) return rc;
+      if (lanes > 1) {
+        ev_trained = sync_get();
+        HIPCHK(hipEventRecord(ev_trained, ctx->stream));
+      }
     }
     if (samples_host) {
       chunk_out.resize((size_t)R.npaths * 6);

@@ -16,8 +16,9 @@ struct sgr_guide {
   uint16_t *w16 = nullptr;
   sgr::GuideTrain T{};
   float *upload = nullptr;      // host-supplied rows / records
-  float *in6 = nullptr, *lobes = nullptr, *rec = nullptr;
-  size_t rows_cap = 0, rec_cap = 0;
+  // inference rows and training records of each chunk lane
+  float *in6[2] = {nullptr, nullptr}, *lobes[2] = {nullptr, nullptr}, *rec[2] = {nullptr, nullptr};
+  size_t rows_cap[2] = {0, 0}, rec_cap[2] = {0, 0};
   uint32_t step = 0;
   std::vector<void *> allocs;
   uint64_t bytes = 0;
@@ -110,34 +111,34 @@ int train_batch(sgr_guide *g, const float *rec, uint32_t n, float lr) {
 
 }  // namespace
 
-int guide_prepare(sgr_guide *g, sgr_scene *sc, size_t cap, GuideView &V) {
+int guide_prepare(sgr_guide *g, sgr_scene *sc, size_t cap, int lane, GuideView &V) {
   if (g->ctx != sc->ctx) return fail(SGR_ERR_INVALID, "guide and scene belong to different contexts");
   // about 256 k training records per chunk
   uint32_t stride = (uint32_t)std::max<size_t>(1, cap / (256u * 1024u));
   size_t nrec = (cap + stride - 1) / stride;
-  if (cap > g->rows_cap) {
-    gfree(g, g->in6, g->rows_cap * GD_IN * 4);
-    gfree(g, g->lobes, g->rows_cap * GD_OUT * 4);
-    g->in6 = g->lobes = nullptr;
-    g->rows_cap = 0;
-    int rc = galloc(g, cap * GD_IN * 4, (void **)&g->in6);
-    if (!rc) rc = galloc(g, cap * GD_OUT * 4, (void **)&g->lobes);
+  if (cap > g->rows_cap[lane]) {
+    gfree(g, g->in6[lane], g->rows_cap[lane] * GD_IN * 4);
+    gfree(g, g->lobes[lane], g->rows_cap[lane] * GD_OUT * 4);
+    g->in6[lane] = g->lobes[lane] = nullptr;
+    g->rows_cap[lane] = 0;
+    int rc = galloc(g, cap * GD_IN * 4, (void **)&g->in6[lane]);
+    if (!rc) rc = galloc(g, cap * GD_OUT * 4, (void **)&g->lobes[lane]);
     if (rc) return rc;
-    g->rows_cap = cap;
+    g->rows_cap[lane] = cap;
   }
-  if (nrec > g->rec_cap) {
-    gfree(g, g->rec, g->rec_cap * GD_REC * 4);
-    g->rec = nullptr;
-    g->rec_cap = 0;
-    int rc = galloc(g, nrec * GD_REC * 4, (void **)&g->rec);
+  if (nrec > g->rec_cap[lane]) {
+    gfree(g, g->rec[lane], g->rec_cap[lane] * GD_REC * 4);
+    g->rec[lane] = nullptr;
+    g->rec_cap[lane] = 0;
+    int rc = galloc(g, nrec * GD_REC * 4, (void **)&g->rec[lane]);
     if (rc) return rc;
-    g->rec_cap = nrec;
+    g->rec_cap[lane] = nrec;
   }
   std::memset(&V, 0, sizeof V);
   V.net = GuideNet{g->w32, g->w16};
-  V.in6 = g->in6;
-  V.lobes = g->lobes;
-  V.rec = g->rec;
+  V.in6 = g->in6[lane];
+  V.lobes = g->lobes[lane];
+  V.rec = g->rec[lane];
   V.rec_stride = stride;
   V.rec_count = (uint32_t)nrec;
   for (int a = 0; a < 3; a++) {
@@ -150,17 +151,17 @@ int guide_prepare(sgr_guide *g, sgr_scene *sc, size_t cap, GuideView &V) {
 }
 
 // Record count and slot key of a chunk of NPATHS paths; clears its
-// records (before raygen).
-int guide_begin_chunk(sgr_guide *g, GuideView &V, uint32_t npaths, uint64_t key) {
+// records (before raygen, on the chunk's stream S).
+int guide_begin_chunk(sgr_guide *g, GuideView &V, uint32_t npaths, uint64_t key, hipStream_t s) {
   V.rec_count = (npaths + V.rec_stride - 1) / V.rec_stride;
   V.rec_key = key;
-  HIPCHK(hipMemsetAsync(V.rec, 0, (size_t)V.rec_count * GD_REC * 4, g->ctx->stream));
+  HIPCHK(hipMemsetAsync(V.rec, 0, (size_t)V.rec_count * GD_REC * 4, s));
   return SGR_OK;
 }
 
 // Network inputs and lobes of bounce B's queue rows (before shading).
-void guide_infer(sgr_guide *g, const GuideView &V, const PathBufs &P, uint32_t b, unsigned grid) {
-  hipStream_t s = g->ctx->stream;
+void guide_infer(sgr_guide *g, const GuideView &V, const PathBufs &P, uint32_t b, unsigned grid,
+                 hipStream_t s) {
   launch_guide_inputs(V, P.q[b & 1], &P.counts[SGR_CNT_Q + b], P.ray, P.ht, P.hprim, grid, s);
   launch_mlp_forward(V.net, V.in6, 0, &P.counts[SGR_CNT_Q + b], V.lobes, 1, grid, s);
 }

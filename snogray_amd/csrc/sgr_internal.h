@@ -74,9 +74,11 @@ void engine_destroy(sgr_ctx *ctx);
 // Learned proposal hooks of a render (sgr_guide.cpp): buffers for CAP
 // paths, clearing a chunk's training records, inference before shading
 // bounce B, training after a chunk, final statistics.
-int guide_prepare(sgr_guide *g, sgr_scene *sc, size_t cap, GuideView &V);
-int guide_begin_chunk(sgr_guide *g, GuideView &V, uint32_t npaths, uint64_t key);
-void guide_infer(sgr_guide *g, const GuideView &V, const PathBufs &P, uint32_t b, unsigned grid);
+// LANE: chunk lane (0 or 1) whose inference rows and records V uses.
+int guide_prepare(sgr_guide *g, sgr_scene *sc, size_t cap, int lane, GuideView &V);
+int guide_begin_chunk(sgr_guide *g, GuideView &V, uint32_t npaths, uint64_t key, hipStream_t s);
+void guide_infer(sgr_guide *g, const GuideView &V, const PathBufs &P, uint32_t b, unsigned grid,
+                 hipStream_t s);
 int guide_train_chunk(sgr_guide *g, const GuideView &V, uint32_t nrec);
 int guide_finish(sgr_guide *g);
 

@@ -189,3 +189,38 @@ def test_trained_guide_gain_at_equal_spp(ctx):
     assert gain > 0.8   # the guide never makes the estimate much worse
     g.close()
     sc.close()
+
+
+def test_guided_chunk_pairs_deterministic(ctx):
+    """Several chunks: the chunks of a pair render concurrently with the
+    same weights and the guide trains on both after the pair; the same
+    seeds give the same image bit for bit and the same training steps,
+    and the image agrees with plain rendering within 3 standard errors
+    of the whole-image mean."""
+    sd, cam = scenes.cornell_box()
+    sc = native.Scene(ctx, sd)
+    W, spp = 48, 32
+    pif = (W + 4) * (W + 4) * 8          # 4 chunks of 8 spp
+    outs = []
+    for _ in range(2):
+        g = native.Guide(ctx, seed=4)
+        acc, _ = sc.render_guided(cam, abi.make_params(W, W, spp, max_path_len=8, seed=9,
+                                                       paths_in_flight=pif), g)
+        outs.append((acc, g.stats()["train_steps"]))
+        g.close()
+    (a, ta), (b, tb) = outs
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert ta == tb > 0
+    K = 8
+    gm = []
+    for s in range(K):
+        g = native.Guide(ctx, seed=60 + s)
+        p = abi.make_params(W, W, spp, max_path_len=8, seed=50 + s, paths_in_flight=pif)
+        gm.append(native.finalize(sc.render_guided(cam, p, g)[0])[..., :3].mean())
+        g.close()
+    pm = [native.finalize(sc.render(cam, abi.make_params(W, W, spp, max_path_len=8,
+                                                          seed=80 + s))[0])[..., :3].mean()
+          for s in range(K)]
+    se = math.sqrt(np.var(gm, ddof=1) / K + np.var(pm, ddof=1) / K)
+    assert abs(np.mean(gm) - np.mean(pm)) < 3 * se + 1e-6, (np.mean(gm), np.mean(pm), se)
+    sc.close()
