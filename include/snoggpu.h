@@ -184,7 +184,9 @@ typedef struct sgr_render_params {
     float    bg_alpha;             /* "background_alpha", 1 */
     /* Rendered window [x0, x0+w) x [y0, y0+h) of the image; the filter
      * margin (ceil(radius - 0.5001) pixels, snogray.lua:546-550) is added
-     * around it by the renderer. w == 0 means the whole image. */
+     * around it by the renderer, and only the window's accumulators
+     * receive samples (a limited render's output is the window,
+     * snogray.lua:447-454). w == 0 means the whole image. */
     uint32_t x0, y0, w, h;
     /* Paths per chunk (0: the default below): a frame runs in chunks of
      * whole-image x floor(this / pixels) samples.  sgr_render keeps two

@@ -365,12 +365,15 @@ static void render_impl(const Scene<R> &sc, const sgr_camera &cam,
       const float *o = &buf[k * 6];
       float sx = o[0], sy = o[1];
       int x = int(sx), y = int(sy);
+      // valid_x / valid_y (image/image-sampled-output.h:155-156) of the
+      // output, which for a limited render is the window alone
+      // (snogray.lua:447-454): margin samples only reach window pixels
       for (int fy = -fr; fy <= fr; fy++) {
         int py = y + fy;
-        if (py < 0 || py >= (int)P.height) continue;
+        if (py < (int)P.y0 || py >= (int)(P.y0 + P.h)) continue;
         for (int fx = -fr; fx <= fr; fx++) {
           int px = x + fx;
-          if (px < 0 || px >= (int)P.width) continue;
+          if (px < (int)P.x0 || px >= (int)(P.x0 + P.w)) continue;
           float w = mitchell1(sx - (float(px) + 0.5f), inv_r, P.filter_b, P.filter_c) *
                     mitchell1(sy - (float(py) + 0.5f), inv_r, P.filter_b, P.filter_c);
           R ws[4] = {R(o[2]) * R(w), R(o[3]) * R(w), R(o[4]) * R(w), R(o[5]) * R(w)};

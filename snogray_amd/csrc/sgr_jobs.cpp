@@ -151,12 +151,10 @@ uint64_t accum_bytes(const sgr_render_params *p) {
   return (uint64_t)p->width * p->height * 5 * sizeof(float);
 }
 
-// 16 x 16 tiles of image pixels the splat of a chunk writes (the pattern
-// window clipped to the image), as the splat kernel enumerates them.
+// 16 x 16 tiles of image pixels the splat of a chunk writes (the
+// window), as the splat kernel enumerates them.
 uint32_t splat_tiles(const RenderView &R) {
-  int ox0 = std::max(R.X0, 0), oy0 = std::max(R.Y0, 0);
-  int ox1 = std::min(R.X0 + R.PW, (int)R.width), oy1 = std::min(R.Y0 + R.PH, (int)R.height);
-  int ow = ox1 - ox0, oh = oy1 - oy0;
+  int ow = R.PW - 2 * R.fr, oh = R.PH - 2 * R.fr;  // the window
   if (ow <= 0 || oh <= 0) return 0;
   return (uint32_t)((ow + 15) / 16) * (uint32_t)((oh + 15) / 16);
 }

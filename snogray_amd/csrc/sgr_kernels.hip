@@ -770,8 +770,11 @@ __device__ __forceinline__ float mitchell1(float x, float inv_radius, float B, f
 // weighted samples of the pattern pixels within the filter radius (a
 // sample of pattern pixel x reaches image pixels int(sx) +- radius; the
 // taps that truncation adds or drops all carry weight 0).  No atomics,
-// fixed summation order.  The image pixels of a segment are those its
-// pattern window covers, clipped to the image.
+// fixed summation order.  The image pixels of a segment are those of its
+// window: a limited render's output is the window alone
+// (snogray.lua:447-454, valid_x / valid_y of
+// image/image-sampled-output.h:155-156), so the samples of the margin
+// around it only reach window pixels.
 //
 // A workgroup takes a 16 x 16 tile of image pixels.  Per pass it stages
 // the records (sx, sy, r, g, b, a) of G consecutive samples of every
@@ -786,10 +789,10 @@ __device__ __forceinline__ float mitchell1(float x, float inv_radius, float B, f
 
 __device__ __forceinline__ void splat_bounds(const RenderView &R, int &ox0, int &oy0, int &ow,
                                              int &oh) {
-  ox0 = R.X0 > 0 ? R.X0 : 0;
-  oy0 = R.Y0 > 0 ? R.Y0 : 0;
-  int ox1 = R.X0 + R.PW < (int)R.width ? R.X0 + R.PW : (int)R.width;
-  int oy1 = R.Y0 + R.PH < (int)R.height ? R.Y0 + R.PH : (int)R.height;
+  // the window is the pattern without its filter margin (inside the image)
+  ox0 = R.X0 + R.fr;
+  oy0 = R.Y0 + R.fr;
+  const int ox1 = R.X0 + R.PW - R.fr, oy1 = R.Y0 + R.PH - R.fr;
   ow = ox1 - ox0;
   oh = oy1 - oy0;
 }
