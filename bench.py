@@ -524,7 +524,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is a design in DESIGN.md, not a measured path.",
+              "sharding by sample index with one all-reduce of the accumulators is "
+              "designed, not built or measured.",
               file=sys.stderr)
         sys.exit(2)
 
