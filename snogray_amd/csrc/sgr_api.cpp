@@ -829,7 +829,11 @@ int ensure_work(sgr_ctx *ctx, size_t bytes) {
     ctx->work_bytes = 0;
   }
   hipError_t e = hipMalloc(&ctx->work, bytes);
-  if (e != hipSuccess) return fail(SGR_ERR_NOMEM, std::string("render buffers: ") + hipGetErrorString(e));
+  if (e != hipSuccess) {
+    ctx->work = nullptr;
+    (void)hipGetLastError();  // a caller may retry smaller: leave no error behind
+    return fail(SGR_ERR_NOMEM, std::string("render buffers: ") + hipGetErrorString(e));
+  }
   ctx->work_bytes = bytes;
   ctx->bytes += bytes;
   return SGR_OK;
@@ -853,10 +857,14 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   // the same weights, and the guide trains on both after the pair.
   const uint32_t nchunks = (R.spp + pl.sb_max - 1) / pl.sb_max;
   const int max_lanes = guide ? std::min(SGR_CHUNK_LANES, 2) : SGR_CHUNK_LANES;
-  const int lanes = (samples_host || (pl.flags & SGR_RENDER_ONE_CHUNK_IN_FLIGHT))
-                        ? 1 : (int)std::min<uint32_t>(nchunks, (uint32_t)max_lanes);
+  int lanes = (samples_host || (pl.flags & SGR_RENDER_ONE_CHUNK_IN_FLIGHT))
+                  ? 1 : (int)std::min<uint32_t>(nchunks, (uint32_t)max_lanes);
   size_t need = (path_bytes(cap, R.K, nullptr, nullptr) + 255) & ~(size_t)255;
   int rc = ensure_work(ctx, need * lanes);
+  if (rc && lanes > 1) {  // not enough device memory for a second lane: one
+    lanes = 1;
+    rc = ensure_work(ctx, need);
+  }
   if (rc) return rc;
   PathBufs PL[SGR_CHUNK_LANES];
   hipStream_t lane_st[SGR_CHUNK_LANES];
