@@ -91,12 +91,13 @@ struct PathBufs {
   float *sray[8];     // shadow rays, K per path
   float *sc[3];       // shadow contributions
   uint8_t *svis;
-  uint32_t *q[2];     // ray queues (ping-pong)
-  uint32_t *sq;       // shadow queue
+  uint32_t *q[2];     // ray queues (ping-pong), in raygen's pixel order
+  uint32_t *sq;       // shadow queue, grouped by direction octant per block
   // Ordered compaction of the queues k_shade produces: per queue
-  // position, whether the path goes on and which of its shadow slots
-  // got a ray; per block of SGR_COMPACT_ITEMS positions, the two counts
-  // (then their exclusive prefixes).
+  // position, whether the path goes on (1 or 0) and which of its shadow
+  // slots got a ray (1 | direction octant << 1, or 0); per block of
+  // SGR_COMPACT_ITEMS positions, the two counts (then their exclusive
+  // prefixes).
   uint8_t *qflag, *sflag;
   uint32_t *cblk;
   uint32_t *counts;   // SGR_NUM_COUNTERS

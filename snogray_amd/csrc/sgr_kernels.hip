@@ -93,6 +93,12 @@ __global__ void __launch_bounds__(256) k_trace(SceneView S, RayBatchDev rb, uint
 // ---------------------------------------------------------------------------
 // Wavefront path tracer
 // ---------------------------------------------------------------------------
+// Direction octant of a ray (sign bits x, y, z as 4, 2, 1), the key the
+// shadow queue is grouped by.
+__device__ __forceinline__ uint32_t dir_octant(V3 d) {
+  return (d.x >= 0.0f ? 0u : 4u) | (d.y >= 0.0f ? 0u : 2u) | (d.z >= 0.0f ? 0u : 1u);
+}
+
 __device__ __forceinline__ void path_ident(const RenderView &R, uint32_t p, int &px, int &py,
                                            uint32_t &s) {
   uint32_t i = p - R.path_base;
@@ -232,10 +238,10 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathB
 
 // One path's vertex (render/path-integ.cc:133-407 for one bounce): the
 // hit at P.hprim / P.ht of path P, emission, direct lighting (shadow rays
-// written to the path's slots, each announced by EMIT(slot)), the BSDF
-// continuation and Russian roulette.  Returns whether the path goes on
-// with the ray it stored.  J is the queue position (learned proposal
-// lobes); ILLUM counts direct-lighting calls.
+// written to the path's slots, each announced by EMIT(slot, octant of its
+// direction)), the BSDF continuation and Russian roulette.  Returns
+// whether the path goes on with the ray it stored.  J is the queue
+// position (learned proposal lobes); ILLUM counts direct-lighting calls.
 template<bool MULTI, bool GUIDED, typename Emit>
 __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView &R1,
                                            const StepView &J, const PathBufs &P,
@@ -330,7 +336,7 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
                 P.sray[4][slot_a] = wd.y; P.sray[5][slot_a] = wd.z;
                 P.sray[6][slot_a] = S.min_trace; P.sray[7][slot_a] = md - S.min_trace;
                 P.sc[0][slot_a] = c.r; P.sc[1][slot_a] = c.g; P.sc[2][slot_a] = c.b;
-                emit(slot_a);
+                emit(slot_a, dir_octant(wd));
               }
             }
             // BSDF sample (render/direct-illum.cc:199-252)
@@ -350,7 +356,7 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
                 P.sray[4][slot_b] = wd.y; P.sray[5][slot_b] = wd.z;
                 P.sray[6][slot_b] = S.min_trace; P.sray[7][slot_b] = md - S.min_trace;
                 P.sc[0][slot_b] = c.r; P.sc[1][slot_b] = c.g; P.sc[2][slot_b] = c.b;
-                emit(slot_b);
+                emit(slot_b, dir_octant(wd));
               }
             }
           }
@@ -470,8 +476,9 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, Step
     const uint32_t p = q[j];
     uint8_t *sf = P.sflag + (size_t)j * K;
     for (size_t k = 0; k < K; k++) sf[k] = 0;
-    const bool cont = shade_path<MULTI, GUIDED>(S1, R1, J, P, G, p, j, illum, [&](size_t slot) {
-      sf[slot - (size_t)p * K] = 1;
+    const bool cont = shade_path<MULTI, GUIDED>(S1, R1, J, P, G, p, j, illum,
+                                                [&](size_t slot, uint32_t oct) {
+      sf[slot - (size_t)p * K] = (uint8_t)(1u | oct << 1);
     });
     P.qflag[j] = cont ? 1 : 0;
   }
@@ -480,25 +487,15 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, Step
 
 // Ordered compaction of what k_shade flagged: the next ray queue (paths
 // that go on) and the shadow queue (slots with a ray), both in the order
-// of the queue they come from.  Queues thus stay in raygen's pixel order
-// bounce after bounce, so the slice of a queue each XCD takes (see
+// of the queue they come from (the shadow queue up to a grouping by
+// direction octant within a block).  Queues thus stay in raygen's pixel
+// order bounce after bounce, so the slice of a queue each XCD takes (see
 // block_range) keeps covering the same part of the image, and k_shade
 // needs no atomics.  A block takes SGR_COMPACT_ITEMS consecutive
 // positions, each of its 4 waves a run of 1024, 64 at a time (lane l
 // position base + l, coalesced flag loads): a count pass, one block
 // scanning the block counts, a scatter pass.
 #define SGR_COMPACT_WAVE (SGR_COMPACT_ITEMS / 4)
-
-// Exclusive prefix sum over the 64 lanes of a wave.
-__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
-  const uint32_t lane = __lane_id();
-  uint32_t x = v;
-  for (uint32_t off = 1; off < 64; off <<= 1) {
-    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
-    if (lane >= off) x += y;
-  }
-  return x - v;
-}
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   for (uint32_t off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
@@ -515,7 +512,7 @@ __device__ __forceinline__ void compact_wave_counts(const PathBufs &P, uint32_t 
     if (j < n) {
       q += P.qflag[j];
       const uint8_t *sf = P.sflag + (size_t)j * K;
-      for (uint32_t k = 0; k < K; k++) sh += sf[k];
+      for (uint32_t k = 0; k < K; k++) sh += sf[k] != 0 ? 1u : 0u;
     }
   }
   cq = wave_sum(q);
@@ -581,23 +578,53 @@ __global__ void __launch_bounds__(256) k_compact_scan(PathBufs P, uint32_t b) {
   }
 }
 
+// Scatter pass.  The next ray queue keeps the order of the queue it
+// comes from: k_shade, k_gather and the learned proposal read their
+// per-path records through it.  The shadow queue, read by k_occlude
+// alone, is written with each block's shadow rays grouped by direction
+// octant (shadow flags hold 1 | octant << 1; 0: no ray): rays of one
+// octant from neighbouring pixels share the octree nodes they walk, so a
+// wave of them touches fewer cache lines (1.31 M-triangle frame 158 ->
+// 155 ms; the same grouping of the ray queue, for k_extend alone, gained
+// no more there and lost 1.5 % on the Cornell box, its scattered ray
+// loads and hit stores outweighing the shorter walks).
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __global__ void __launch_bounds__(256) k_compact_scatter(PathBufs P, uint32_t b) {
-  __shared__ uint32_t part[8];
+  __shared__ uint32_t part[4][9];  // per wave: rays, then shadow rays per octant
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   if (finishing(P, b, n) || blockIdx.x * SGR_COMPACT_ITEMS >= n) return;
   const uint32_t w = threadIdx.x >> 6, lane = __lane_id(), K = P.K;
   const uint32_t base = blockIdx.x * SGR_COMPACT_ITEMS + w * SGR_COMPACT_WAVE;
-  uint32_t cq, cs;
-  compact_wave_counts(P, n, base, cq, cs);
-  if (lane == 0) {
-    part[2 * w] = cq;
-    part[2 * w + 1] = cs;
+  // Counts of the wave's run: continuing paths, and shadow rays per
+  // octant (lane o < 8 holds octant o's).
+  uint32_t cq = 0, cs = 0;
+  for (uint32_t it = 0; it < SGR_COMPACT_WAVE; it += 64) {
+    const uint32_t j = base + it + lane;
+    const bool in = j < n;
+    cq += (uint32_t)__popcll(__ballot(in && P.qflag[j]));
+    const uint8_t *sf = P.sflag + (size_t)(in ? j : 0u) * K;
+    for (uint32_t k = 0; k < K; k++) {
+      const uint32_t fs = in ? sf[k] : 0u;
+#pragma unroll
+      for (uint32_t o = 0; o < 8; o++) {
+        const uint32_t c = (uint32_t)__popcll(__ballot(fs == (1u | o << 1)));
+        cs += lane == o ? c : 0u;
+      }
+    }
   }
+  if (lane == 0) part[w][0] = cq;
+  if (lane < 8) part[w][1 + lane] = cs;
   __syncthreads();
-  uint32_t oq = P.cblk[2 * blockIdx.x], os = P.cblk[2 * blockIdx.x + 1];
-  for (uint32_t v = 0; v < w; v++) {
-    oq += part[2 * v];
-    os += part[2 * v + 1];
+  // Starts: the wave's run of the ray queue, and per octant (lane o) the
+  // wave's place in the block's octant-grouped shadow runs.
+  uint32_t oq = P.cblk[2 * blockIdx.x], offs = P.cblk[2 * blockIdx.x + 1];
+  for (uint32_t v = 0; v < 4; v++) {
+    if (v < w) oq += part[v][0];
+    for (uint32_t o = 0; o < 8; o++)
+      if (o < lane || (o == lane && v < w)) offs += part[v][1 + o];
   }
   const uint32_t *q = P.q[b & 1];
   uint32_t *qn = P.q[(b + 1) & 1];
@@ -607,19 +634,19 @@ __global__ void __launch_bounds__(256) k_compact_scatter(PathBufs P, uint32_t b)
     const uint32_t p = in ? q[j] : 0u;
     const bool go = in && P.qflag[j];
     const uint64_t m = __ballot(go);
-    if (go)
-      qn[oq + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = p;
+    if (go) qn[oq + lane_rank(m)] = p;
     oq += (uint32_t)__popcll(m);
     const uint8_t *sf = P.sflag + (size_t)(in ? j : 0u) * K;
-    uint32_t c = 0;
-    if (in)
-      for (uint32_t k = 0; k < K; k++) c += sf[k];
-    uint32_t at = os + wave_excl_scan(c);
-    if (c)
-      for (uint32_t k = 0; k < K; k++)
-        if (sf[k]) P.sq[at++] = (uint32_t)((size_t)p * K + k);
-    os += wave_sum(c);
+    for (uint32_t k = 0; k < K; k++) {
+      const uint32_t fs = in ? sf[k] : 0u;
+#pragma unroll
+      for (uint32_t o = 0; o < 8; o++) {
+        const uint64_t mo = __ballot(fs == (1u | o << 1));
+        const uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)offs, (int)o);
+        if (fs == (1u | o << 1)) P.sq[at + lane_rank(mo)] = (uint32_t)((size_t)p * K + k);
+        offs += lane == o ? (uint32_t)__popcll(mo) : 0u;
+      }
+    }
   }
 }
 
@@ -727,7 +754,8 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
       P.hu[p] = T.hu;
       P.hv[p] = T.hv;
       uint64_t em = 0;  // shadow slots of the path written at this vertex
-      const bool cont = shade_path<MULTI, false>(S1, R1, J, P, G, p, j, illum, [&](size_t slot) {
+      const bool cont = shade_path<MULTI, false>(S1, R1, J, P, G, p, j, illum,
+                                                 [&](size_t slot, uint32_t) {
         em |= 1ull << (uint32_t)(slot - (size_t)p * P.K);
       });
       for (uint64_t m = em; m; m &= m - 1) {
