@@ -64,7 +64,8 @@ struct SceneView {
   // SGR_ENV_GUIDE_ROW + 1; columns: SGR_ENV_GUIDE_COL + 1 per row).
   const uint32_t *env_row_guide;
   const uint32_t *env_col_guide;
-  // Walk with lane refill (deep octrees) or one ray per lane at a time.
+  // Walk with lane refill (deep octrees; camera rays excepted) or one ray
+  // per lane at a time.
   uint32_t walk_refill;
   uint32_t n_tri_like;       // triangles + tripars
   uint32_t n_tri;            // mesh triangles

@@ -330,7 +330,9 @@ __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, Trac
 // Plain driver: lane i of the grid walks rays i, i + G, ... to the end,
 // one at a time (ray ranges by block_range order).  Cheaper per step than
 // trace_refill when walks are short (shallow octrees), where the waves
-// are balanced anyway.
+// are balanced anyway, and for camera rays, whose lanes (the samples of
+// one or two pixels) walk the same nodes in step, so their loads
+// coalesce.
 template<bool ANY, bool MULTI, typename Get, typename Put>
 __device__ __forceinline__ void trace_plain(const SceneView &S1, uint32_t jb, uint32_t je,
                                             Get get, Put put, TraceCount &cnt, LdsStack L) {
