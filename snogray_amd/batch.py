@@ -61,8 +61,17 @@ def autotune(ctx, jobs, capacities=DEFAULT_CAPACITIES, grids=DEFAULT_GRIDS, step
         table.append(({"path_capacity": cap, "grid_blocks": grid}, rate))
         if log:
             log(f"autotune capacity {cap} grid {grid or 'auto'}: {rate / 1e3:.1f} M paths/s")
-    best = max(table, key=lambda t: t[1])[0]
-    return best, table
+    return pick(table), table
+
+
+def pick(table, tol=0.03):
+    """The largest path capacity among the configurations within TOL of
+    the best probe rate (the best rate among those).  A probe whose work
+    fits the smaller capacities measures them alike, while the whole
+    batch then needs more steps at the smaller one."""
+    top = max(r for _, r in table)
+    near = [(c, r) for c, r in table if r >= (1.0 - tol) * top]
+    return max(near, key=lambda t: (t[0]["path_capacity"], t[1]))[0]
 
 
 def run_batch(ctx, jobs, path_capacity=abi.SGR_DEFAULT_PATHS_IN_FLIGHT, grid_blocks=0, keep_results=False):

@@ -346,10 +346,12 @@ __device__ __forceinline__ void trace_plain(const SceneView &S1, uint32_t jb, ui
 }
 
 // Lane-refill driver: every lane of a wave walks its own ray; when at
-// least SGR_REFILL (32: the best of 8, 16, 32, 48 on the 1.31 M-triangle
-// workload) lanes are idle the wave takes that many more rays with one
-// atomic on a slice head (a wave-wide ballot ranks the idle lanes), so
-// no lane idles while its wave still has rays to finish.  The N rays are
+// least SGR_REFILL lanes are idle the wave takes that many more rays with
+// one atomic on a slice head (a wave-wide ballot ranks the idle lanes),
+// so no lane idles for long while its wave still has rays to finish.
+// 48 is the best of 16 .. 64 on the 1.31 M-triangle frame (16 / 32 / 40 /
+// 48 / 56 / 64: 167 / 140 / 136 / 134 / 134 / 146 ms): refilled lanes
+// start together, and lanes that walk in step share their loads.  The N rays are
 // cut into 8 slices, one per XCD (blockIdx.x & 7 under round-robin
 // dispatch), so neighbouring rays share an L2; a wave whose slice is
 // used up moves on to the next slices in turn, so no XCD idles while
@@ -357,7 +359,7 @@ __device__ __forceinline__ void trace_plain(const SceneView &S1, uint32_t jb, ui
 // (used when MULTI; S1 otherwise); PUT(i, state) stores the result.  The
 // loop exits when all 8 slices are used up and the wave's last ray is
 // done (every wave gets there: each step advances a finite walk).
-#define SGR_REFILL 32
+#define SGR_REFILL 48
 template<bool ANY, bool MULTI, typename Get, typename Put>
 __device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t n, uint32_t *heads,
                                              Get get, Put put, TraceCount &cnt, LdsStack L) {
