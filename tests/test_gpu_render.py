@@ -67,6 +67,33 @@ def test_lights_and_materials_paths_match_oracle(ctx):
     assert compare_samples(got, ref, 1e-3) > 0.97
 
 
+def test_many_lights_match_oracle(ctx):
+    """Five lights (three sphere lights, an area light, an envmap) at
+    direct_samples = 3: 30 shadow slots per path, so the shadow queue's
+    compaction handles many slots per position."""
+    sd = scenes.SceneDesc()
+    wall = sd.lambert((0.6, 0.6, 0.6))
+    ct = sd.cook_torrance((0.3, 0.2, 0.1), (0.6, 0.6, 0.6), 0.2, 1.6)
+    sd.tripar((-5, 0, -5), (0, 0, 10), (10, 0, 0), wall, parallelogram=True)
+    sd.tripar((-5, 0, 5), (10, 0, 0), (0, 6, 0), ct, parallelogram=True)
+    for i, col in enumerate(((12.0, 4.0, 4.0), (4.0, 12.0, 4.0), (4.0, 4.0, 12.0))):
+        sd.sphere((-2.0 + 2.0 * i, 2.5, -0.5 + 0.5 * i), 0.4, sd.glow(col))
+    sd.tripar((-1, 5.5, -1), (2, 0, 0), (0, 0, 2), sd.glow((6.0, 6.0, 5.0)), parallelogram=True)
+    sd.sphere((0.5, 1, 0.5), 1.0, wall)
+    sd.envmap(scenes.gaussian_lobe_envmap(64, 32, 5))
+    cam = scenes.look_at((0, 2, -8), (0, 1.5, 0), vfov_deg=45.0)
+    sc, osc = native.Scene(ctx, sd), oracle.OracleScene(sd)
+    p = abi.make_params(40, 32, 8, max_path_len=6, direct_samples=3)
+    got, gst = sc.render_samples(cam, p)
+    _, ref, ost = osc.render(cam, p, samples=True)
+    assert compare_samples(got, ref, 1e-3) > 0.97
+    for k in ("scene_intersect_calls", "shadow_tests", "illum_calls"):
+        assert abs(gst[k] - ost[k]) <= 0.005 * ost[k], k
+    acc, _ = sc.render(cam, p)  # accumulating path, default flags
+    oacc, _, _ = osc.render(cam, p)
+    np.testing.assert_allclose(acc[..., 4], oacc[..., 4], rtol=1e-5, atol=1e-5)
+
+
 def test_accumulators_match_oracle(ctx):
     sd, cam = scenes.cornell_box()
     sc, osc = native.Scene(ctx, sd), oracle.OracleScene(sd)
