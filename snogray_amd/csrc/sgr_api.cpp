@@ -162,6 +162,11 @@ int sgr_ctx_create(int device, sgr_ctx **out) {
   }
   for (auto &ev : c->ev) (void)hipEventCreate(&ev);
   if (hipMalloc((void **)&c->stats, SGR_STATS_BYTES) != hipSuccess) {
+    (void)hipGetLastError();
+    for (auto &ev : c->ev)
+      if (ev) (void)hipEventDestroy(ev);
+    for (auto &ls : c->lane_stream) (void)hipStreamDestroy(ls);
+    (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(SGR_ERR_NOMEM, "sgr_ctx_create: stats buffer");
   }
@@ -175,12 +180,13 @@ int sgr_ctx_destroy(sgr_ctx *ctx) {
     return sgr::fail(SGR_ERR_INVALID, "sgr_ctx_destroy: " + std::to_string(ctx->live_scenes.load()) +
                                      " scene(s) still alive; destroy them first");
   (void)hipSetDevice(ctx->device);
+  // every stream idle before any buffer its kernels use is freed
   (void)hipStreamSynchronize(ctx->stream);
+  for (auto &ls : ctx->lane_stream) (void)hipStreamSynchronize(ls);
   if (ctx->work) (void)hipFree(ctx->work);
   if (ctx->stats) (void)hipFree(ctx->stats);
   for (auto &ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
-  for (auto &ls : ctx->lane_stream) (void)hipStreamSynchronize(ls);
   for (auto &ev : ctx->evpool) (void)hipEventDestroy(ev);
   for (auto &ev : ctx->syncpool) (void)hipEventDestroy(ev);
   sgr::engine_destroy(ctx);
@@ -497,6 +503,7 @@ int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
   // Lane refill pays off once walks are long enough to leave waves
   // unbalanced: on the 1.31 M-triangle soup (depth 19) it is 16 % faster,
   // on the Cornell box (depth 5) 30 % slower than one ray per lane.
+  // Camera rays take the plain walk on every scene (launch_extend).
   V.walk_refill = sc->oct.max_depth >= 8 ? 1u : 0u;
   std::vector<uint4> nodes4(V.n_nodes);
   for (uint32_t i = 0; i < V.n_nodes; i++)
@@ -823,6 +830,7 @@ int ensure_work(sgr_ctx *ctx, size_t bytes) {
   if (ctx->work_bytes >= bytes) return SGR_OK;
   if (ctx->work) {
     (void)hipStreamSynchronize(ctx->stream);
+    for (auto &ls : ctx->lane_stream) (void)hipStreamSynchronize(ls);
     (void)hipFree(ctx->work);
     ctx->bytes -= ctx->work_bytes;
     ctx->work = nullptr;
