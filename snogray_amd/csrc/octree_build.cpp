@@ -9,15 +9,29 @@ namespace {
 
 enum : unsigned { XH = 4, YH = 2, ZH = 1 };
 
+struct Work {
+  uint32_t node;
+  float x, y, z, sz;
+};
+
+#define SGR_NO_LINK 0xffffffffu
+
+// A node's surfaces are a head-linked list through Builder::links, newest
+// first, as the reference keeps them (space/octree-builder.cc:128-154).
 struct BNode {
   uint32_t kid[8];
-  std::vector<uint32_t> items;  // insertion order; emitted newest-first
-  BNode() { std::memset(kid, 0, sizeof kid); }
+  uint32_t head, count;
+  BNode() : head(SGR_NO_LINK), count(0) { std::memset(kid, 0, sizeof kid); }
+};
+
+struct Link {
+  uint32_t id, next;
 };
 
 class Builder {
  public:
   std::vector<BNode> nodes;
+  std::vector<Link> links;
   float org[3] = {0, 0, 0};
   float size = 0;
 
@@ -30,7 +44,7 @@ class Builder {
       if (ey > s) s = ey;
       if (ez > s) s = ez;
       size = s;
-      nodes[0].items.push_back(id);
+      add_item(0, id);
       return;
     }
     // Grow the root until the box fits (:180-196 and :227-317).
@@ -42,6 +56,14 @@ class Builder {
   }
 
  private:
+  std::vector<Work> work;  // descent work list, reused across inserts
+
+  void add_item(uint32_t node, uint32_t id) {
+    links.push_back(Link{id, nodes[node].head});
+    nodes[node].head = (uint32_t)links.size() - 1;
+    nodes[node].count++;
+  }
+
   void grow(const Box3 &b) {
     uint32_t moved = (uint32_t)nodes.size();
     nodes.push_back(std::move(nodes[0]));
@@ -61,15 +83,10 @@ class Builder {
     size *= 2;
   }
 
-  struct Work {
-    uint32_t node;
-    float x, y, z, sz;
-  };
-
   // Eager descent (:323-525), iterative over a work list.
   void descend(uint32_t id, const Box3 &b) {
     float avg = ((b.hi[0] - b.lo[0]) + (b.hi[1] - b.lo[1]) + (b.hi[2] - b.lo[2])) / 3;
-    std::vector<Work> work;
+    work.clear();
     work.push_back(Work{0, org[0], org[1], org[2], size});
     while (!work.empty()) {
       Work w = work.back();
@@ -101,7 +118,7 @@ class Builder {
                             hz ? mid[2] : w.z, sub});
         placed = true;
       }
-      if (!placed) nodes[w.node].items.push_back(id);
+      if (!placed) add_item(w.node, id);
     }
   }
 };
@@ -133,6 +150,8 @@ void subtree_stats(const FlatOctree &t, uint32_t ni, unsigned &max_depth,
 void build_octree(const std::vector<Box3> &boxes,
                   const std::vector<uint32_t> &ids, FlatOctree &out) {
   Builder bld;
+  bld.nodes.reserve(ids.size() + 1);
+  bld.links.reserve(2 * ids.size() + 16);
   for (uint32_t id : ids) bld.insert(id, boxes[id]);
   out = FlatOctree();
   out.num_real = ids.size();
@@ -147,8 +166,9 @@ void build_octree(const std::vector<Box3> &boxes,
     uint32_t from = q.front();
     q.pop_front();
     const BNode &bn = bld.nodes[from];
-    uint32_t rec[4] = {0, 0, (uint32_t)out.surf_ids.size(), (uint32_t)bn.items.size()};
-    for (size_t k = bn.items.size(); k-- > 0;) out.surf_ids.push_back(bn.items[k]);
+    uint32_t rec[4] = {0, 0, (uint32_t)out.surf_ids.size(), bn.count};
+    for (uint32_t l = bn.head; l != SGR_NO_LINK; l = bld.links[l].next)
+      out.surf_ids.push_back(bld.links[l].id);
     for (unsigned c = 0; c < 8; c++)
       if (bn.kid[c]) {
         if (!rec[1]) rec[0] = next_free;
