@@ -1,8 +1,11 @@
 // Host octree builder (rules cited in octree_build.h).
 #include "octree_build.h"
 
+#include <algorithm>
 #include <cstring>
-#include <deque>
+#include <memory>
+#include <stdexcept>
+#include <thread>
 
 namespace sgr {
 namespace {
@@ -18,56 +21,132 @@ struct Work {
 
 // A node's surfaces are a head-linked list through Builder::links, newest
 // first, as the reference keeps them (space/octree-builder.cc:128-154).
+// kid[c] == 0: no child c (the root, id 0, is nobody's child).
 struct BNode {
   uint32_t kid[8];
   uint32_t head, count;
-  BNode() : head(SGR_NO_LINK), count(0) { std::memset(kid, 0, sizeof kid); }
 };
 
 struct Link {
   uint32_t id, next;
 };
 
+// Node storage: one pool per builder thread, in blocks that never move,
+// so a node id (pool << 26 | index) stays valid while other threads add
+// nodes to their own pools.
+constexpr uint32_t kPoolShift = 26, kBlockShift = 16;
+constexpr uint32_t kBlock = 1u << kBlockShift, kMaxBlocks = 1u << (kPoolShift - kBlockShift);
+constexpr unsigned kMaxThreads = 16;
+
+struct Pool {
+  std::vector<std::unique_ptr<BNode[]>> blocks = std::vector<std::unique_ptr<BNode[]>>(kMaxBlocks);
+  uint32_t used = 0;
+};
+
+struct Placement {
+  uint32_t node, id;
+};
+
 class Builder {
  public:
-  std::vector<BNode> nodes;
   std::vector<Link> links;
   float org[3] = {0, 0, 0};
   float size = 0;
+  bool empty = true;
+  Pool pools[kMaxThreads];
 
-  void insert(uint32_t id, const Box3 &b) {
-    if (nodes.empty()) {  // space/octree-builder.cc:205-218
-      nodes.emplace_back();
-      for (int a = 0; a < 3; a++) org[a] = b.lo[a];
-      float ex = b.hi[0] - b.lo[0], ey = b.hi[1] - b.lo[1], ez = b.hi[2] - b.lo[2];
-      float s = ex;
-      if (ey > s) s = ey;
-      if (ez > s) s = ez;
-      size = s;
-      add_item(0, id);
-      return;
+  BNode &node(uint32_t id) {
+    Pool &p = pools[id >> kPoolShift];
+    const uint32_t i = id & ((1u << kPoolShift) - 1);
+    return p.blocks[i >> kBlockShift][i & (kBlock - 1)];
+  }
+
+  // Inserts ids in order: the root grows sequentially, and each run of
+  // surfaces that fit the current root descends in parallel.  Within a
+  // run the descents are independent: where a surface goes depends only
+  // on its box and the cell geometry, never on which children already
+  // exist, so the tree is the one sequential insertion builds; the
+  // surfaces then join their nodes' lists in insertion order.
+  void build(const std::vector<Box3> &boxes, const std::vector<uint32_t> &ids) {
+    const size_t n = ids.size();
+    if (!n) return;
+    root_from(ids[0], boxes[ids[0]]);
+    unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nt = std::max(1u, std::min(hw ? hw : 1u, kMaxThreads));
+    std::vector<std::vector<Placement>> placed(nt);
+    for (size_t i = 1; i < n;) {
+      const Box3 &b = boxes[ids[i]];
+      while (!fits(b)) grow(b);  // space/octree-builder.cc:180-196 and :227-317
+      size_t j = i + 1;
+      while (j < n && fits(boxes[ids[j]])) j++;
+      const size_t len = j - i;
+      const unsigned use = len < 4096 ? 1u : nt;
+      auto run = [&](unsigned t) {
+        std::vector<Work> work;
+        uint32_t spare = SGR_NO_LINK;
+        placed[t].clear();
+        for (size_t k = i + len * t / use, e = i + len * (t + 1) / use; k < e; k++)
+          descend(t, ids[k], boxes[ids[k]], work, spare, placed[t]);
+      };
+      if (use == 1) {
+        run(0);
+      } else {
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < use; t++) th.emplace_back(run, t);
+        run(0);
+        for (auto &x : th) x.join();
+      }
+      for (unsigned t = 0; t < use; t++)
+        for (const Placement &pl : placed[t]) add_item(pl.node, pl.id);
+      i = j;
     }
-    // Grow the root until the box fits (:180-196 and :227-317).
-    while (!(org[0] <= b.lo[0] && org[1] <= b.lo[1] && org[2] <= b.lo[2] &&
-             (org[0] + size) >= b.hi[0] && (org[1] + size) >= b.hi[1] &&
-             (org[2] + size) >= b.hi[2]))
-      grow(b);
-    descend(id, b);
   }
 
  private:
-  std::vector<Work> work;  // descent work list, reused across inserts
+  uint32_t alloc(unsigned t) {
+    Pool &p = pools[t];
+    if (p.used >= (1u << kPoolShift)) throw std::length_error("octree: too many nodes");
+    const uint32_t i = p.used++;
+    auto &blk = p.blocks[i >> kBlockShift];
+    if (!blk) blk.reset(new BNode[kBlock]);
+    BNode &nd = blk[i & (kBlock - 1)];
+    std::memset(nd.kid, 0, sizeof nd.kid);
+    nd.head = SGR_NO_LINK;
+    nd.count = 0;
+    return (uint32_t)t << kPoolShift | i;
+  }
 
-  void add_item(uint32_t node, uint32_t id) {
-    links.push_back(Link{id, nodes[node].head});
-    nodes[node].head = (uint32_t)links.size() - 1;
-    nodes[node].count++;
+  bool fits(const Box3 &b) const {
+    return org[0] <= b.lo[0] && org[1] <= b.lo[1] && org[2] <= b.lo[2] &&
+           (org[0] + size) >= b.hi[0] && (org[1] + size) >= b.hi[1] &&
+           (org[2] + size) >= b.hi[2];
+  }
+
+  void add_item(uint32_t nd, uint32_t id) {
+    links.push_back(Link{id, node(nd).head});
+    node(nd).head = (uint32_t)links.size() - 1;
+    node(nd).count++;
+  }
+
+  void root_from(uint32_t id, const Box3 &b) {  // space/octree-builder.cc:205-218
+    alloc(0);
+    for (int a = 0; a < 3; a++) org[a] = b.lo[a];
+    float ex = b.hi[0] - b.lo[0], ey = b.hi[1] - b.lo[1], ez = b.hi[2] - b.lo[2];
+    float s = ex;
+    if (ey > s) s = ey;
+    if (ez > s) s = ez;
+    size = s;
+    empty = false;
+    add_item(0, id);
   }
 
   void grow(const Box3 &b) {
-    uint32_t moved = (uint32_t)nodes.size();
-    nodes.push_back(std::move(nodes[0]));
-    nodes[0] = BNode();
+    const uint32_t moved = alloc(0);
+    node(moved) = node(0);
+    BNode &root = node(0);
+    std::memset(root.kid, 0, sizeof root.kid);
+    root.head = SGR_NO_LINK;
+    root.count = 0;
     unsigned slot = 0;
     bool lo_side[3];
     for (int a = 0; a < 3; a++) {
@@ -77,14 +156,16 @@ class Builder {
       lo_side[a] = hi_grow <= lo_grow;
       if (lo_side[a]) slot |= (a == 0 ? XH : a == 1 ? YH : ZH);
     }
-    nodes[0].kid[slot] = moved;
+    root.kid[slot] = moved;
     for (int a = 0; a < 3; a++)
       if (lo_side[a]) org[a] -= size;
     size *= 2;
   }
 
-  // Eager descent (:323-525), iterative over a work list.
-  void descend(uint32_t id, const Box3 &b) {
+  // Eager descent (:323-525), iterative over a work list; a missing child
+  // is created by whichever thread gets there first.
+  void descend(unsigned t, uint32_t id, const Box3 &b, std::vector<Work> &work,
+               uint32_t &spare, std::vector<Placement> &out) {
     float avg = ((b.hi[0] - b.lo[0]) + (b.hi[1] - b.lo[1]) + (b.hi[2] - b.lo[2])) / 3;
     work.clear();
     work.push_back(Work{0, org[0], org[1], org[2], size});
@@ -108,17 +189,24 @@ class Builder {
         if (!(hx ? in_hi[0] : in_lo[0]) || !(hy ? in_hi[1] : in_lo[1]) ||
             !(hz ? in_hi[2] : in_lo[2]))
           continue;
-        uint32_t k = nodes[w.node].kid[c];
+        uint32_t *slot = &node(w.node).kid[c];
+        uint32_t k = __atomic_load_n(slot, __ATOMIC_ACQUIRE);
         if (!k) {
-          k = (uint32_t)nodes.size();
-          nodes.emplace_back();
-          nodes[w.node].kid[c] = k;
+          if (spare == SGR_NO_LINK) spare = alloc(t);
+          uint32_t expect = 0;
+          if (__atomic_compare_exchange_n(slot, &expect, spare, false, __ATOMIC_ACQ_REL,
+                                          __ATOMIC_ACQUIRE)) {
+            k = spare;
+            spare = SGR_NO_LINK;
+          } else {
+            k = expect;  // another thread made it; keep ours for later
+          }
         }
         work.push_back(Work{k, hx ? mid[0] : w.x, hy ? mid[1] : w.y,
                             hz ? mid[2] : w.z, sub});
         placed = true;
       }
-      if (!placed) add_item(w.node, id);
+      if (!placed) out.push_back(Placement{w.node, id});
     }
   }
 };
@@ -149,32 +237,34 @@ void subtree_stats(const FlatOctree &t, uint32_t ni, unsigned &max_depth,
 
 void build_octree(const std::vector<Box3> &boxes,
                   const std::vector<uint32_t> &ids, FlatOctree &out) {
-  Builder bld;
-  bld.nodes.reserve(ids.size() + 1);
-  bld.links.reserve(2 * ids.size() + 16);
-  for (uint32_t id : ids) bld.insert(id, boxes[id]);
+  std::unique_ptr<Builder> bld(new Builder);
+  bld->links.reserve(2 * ids.size() + 16);
+  bld->build(boxes, ids);
   out = FlatOctree();
   out.num_real = ids.size();
-  for (int a = 0; a < 3; a++) out.origin[a] = bld.org[a];
-  out.size = bld.size;
-  if (bld.nodes.empty()) return;
-  // Breadth-first copy (:538-650).
-  std::deque<uint32_t> q;
-  q.push_back(0);
+  for (int a = 0; a < 3; a++) out.origin[a] = bld->org[a];
+  out.size = bld->size;
+  if (bld->empty) return;
+  // Breadth-first copy (:538-650); ORDER is the queue, kept whole.
+  size_t made = 0;
+  for (const Pool &p : bld->pools) made += p.used;
+  out.nodes.reserve(4 * made);
+  out.surf_ids.reserve(bld->links.size());
+  std::vector<uint32_t> order;
+  order.reserve(made);
+  order.push_back(0);
   uint32_t next_free = 1;
-  while (!q.empty()) {
-    uint32_t from = q.front();
-    q.pop_front();
-    const BNode &bn = bld.nodes[from];
+  for (size_t qi = 0; qi < order.size(); qi++) {
+    const BNode &bn = bld->node(order[qi]);
     uint32_t rec[4] = {0, 0, (uint32_t)out.surf_ids.size(), bn.count};
-    for (uint32_t l = bn.head; l != SGR_NO_LINK; l = bld.links[l].next)
-      out.surf_ids.push_back(bld.links[l].id);
+    for (uint32_t l = bn.head; l != SGR_NO_LINK; l = bld->links[l].next)
+      out.surf_ids.push_back(bld->links[l].id);
     for (unsigned c = 0; c < 8; c++)
       if (bn.kid[c]) {
         if (!rec[1]) rec[0] = next_free;
         rec[1] |= 1u << c;
         next_free++;
-        q.push_back(bn.kid[c]);
+        order.push_back(bn.kid[c]);
       }
     if (!rec[1]) out.num_leaf++;
     out.num_surfaces += rec[3];

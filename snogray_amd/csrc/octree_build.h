@@ -6,7 +6,10 @@
 // into the child that holds it, with exact-midpoint tie rules, and is
 // forced into every overlapped child when its average extent is below a
 // quarter of the node size (:323-500); each node's surface list is kept
-// newest-first (:128-154).  The result is flattened breadth-first so
+// newest-first (:128-154).  Between root growths, the surfaces descend
+// on up to 16 host threads (the tree and list order are those of
+// sequential insertion; see Builder::build).  The result is flattened
+// breadth-first so
 // that the children of a node are consecutive (:538-650), which lets a
 // device node be 16 bytes: first child, 8-bit child mask, first surface
 // and surface count.

@@ -57,7 +57,7 @@ def assert_same_tree(sd):
 
 
 @pytest.mark.parametrize("name", ["cornell", "furnace", "soup_rand", "icosoup_small",
-                                  "degenerate", "coincident"])
+                                  "icosoup_327k", "degenerate", "coincident"])
 def test_host_octree_matches_oracle(name):
     if name == "cornell":
         sd, _ = scenes.cornell_box()
@@ -67,6 +67,9 @@ def test_host_octree_matches_oracle(name):
         sd = helpers.triangle_soup(5000, seed=21, extent=20.0, size=0.3)
     elif name == "icosoup_small":
         sd, _ = helpers.small_soup(level=4, n_spheres=4)
+    elif name == "icosoup_327k":
+        # large enough that the builder descends in parallel runs
+        sd, _ = scenes.icosphere_soup(level=5)
     elif name == "degenerate":
         # zero-area and repeated triangles are skipped / kept like the
         # reference (surface/mesh.cc:832-846)
