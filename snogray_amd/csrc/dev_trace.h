@@ -363,6 +363,10 @@ __device__ __forceinline__ void trace_plain(const SceneView &S1, uint32_t jb, ui
 template<bool ANY, bool MULTI, typename Get, typename Put>
 __device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t n, uint32_t *heads,
                                              Get get, Put put, TraceCount &cnt, LdsStack L) {
+  // Blocks beyond what a short batch needs (a ray per lane, a block per
+  // XCD at least) leave at once; the waves that stay take every slice.
+  const uint32_t need = ((n + blockDim.x - 1u) / blockDim.x + 7u) & ~7u;
+  if (blockIdx.x >= (need > 8u ? need : 8u)) return;
   TravState T;
   float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
   bool active = false;
@@ -378,7 +382,7 @@ __device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t n, ui
       const uint32_t len = (uint32_t)((uint64_t)n * (slice + 1u) / 8u) - beg;
       const int leader = __ffsll((unsigned long long)idle) - 1;
       uint32_t base = 0;
-      if ((int)lane == leader) base = atomicAdd(&heads[slice], nidle);
+      if ((int)lane == leader) base = atomicAdd(&heads[slice * SGR_HEAD_STRIDE], nidle);
       base = (uint32_t)__shfl((int)base, leader);
       if (base + nidle >= len) {  // this slice is used up: next time, the next one
         tried++;

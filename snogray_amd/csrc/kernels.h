@@ -22,13 +22,17 @@ namespace sgr {
 
 // Counter block: ray-queue lengths per bounce, shadow-queue lengths, then
 // the fetch heads of the lane-refill traversal (per bounce, one per XCD
-// slice of the queue) for closest-hit and any-hit rays.
+// slice of the queue, SGR_HEAD_STRIDE words apart) for closest-hit and
+// any-hit rays.
 #define SGR_MAX_BOUNCES   64
 #define SGR_CNT_Q         0
 #define SGR_CNT_S         (SGR_MAX_BOUNCES + 2)
-#define SGR_CNT_HE        (2 * SGR_MAX_BOUNCES + 4)
-#define SGR_CNT_HS        (SGR_CNT_HE + 8 * SGR_MAX_BOUNCES)
-#define SGR_NUM_COUNTERS  (SGR_CNT_HS + 8 * SGR_MAX_BOUNCES)
+#define SGR_CNT_HE        160  // a multiple of SGR_HEAD_STRIDE past the lengths
+#define SGR_HEADS_PER_B   (8 * SGR_HEAD_STRIDE)
+#define SGR_CNT_HS        (SGR_CNT_HE + SGR_HEADS_PER_B * SGR_MAX_BOUNCES)
+#define SGR_NUM_COUNTERS  (SGR_CNT_HS + SGR_HEADS_PER_B * SGR_MAX_BOUNCES)
+static_assert(SGR_CNT_HE >= 2 * SGR_MAX_BOUNCES + 4 && SGR_CNT_HE % SGR_HEAD_STRIDE == 0,
+              "counter block layout");
 
 // Device statistics slots (unsigned long long).
 #define SGR_STAT_NODES_C 0
@@ -39,8 +43,11 @@ namespace sgr {
 #define SGR_STAT_ISECT   5
 #define SGR_STAT_SHADOW  6
 #define SGR_NUM_STATS    8
-// The statistics buffer also holds the 8 fetch heads of a ray-batch query.
-#define SGR_STATS_BYTES  (SGR_NUM_STATS * 8 + 8 * 4)
+// The statistics buffer also holds the 8 fetch heads of a ray-batch query,
+// from byte SGR_STATS_HEADS_OFF on, SGR_HEAD_STRIDE words apart.
+#define SGR_STATS_HEADS_OFF 128
+#define SGR_STATS_BYTES  (SGR_STATS_HEADS_OFF + 8 * SGR_HEAD_STRIDE * 4)
+static_assert(SGR_NUM_STATS * 8 <= SGR_STATS_HEADS_OFF, "statistics layout");
 
 struct RayBatchDev {
   const float *ox, *oy, *oz, *dx, *dy, *dz, *t0, *t1;

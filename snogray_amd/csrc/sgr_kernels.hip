@@ -57,15 +57,16 @@ __device__ __forceinline__ void walk(const SceneView &S1, uint32_t n, uint32_t *
   }
 }
 
-// Ray-batch queries with lane-refill traversal (heads[0..7], zeroed by
-// the caller, are the fetch heads of the 8 slices of the batch).
+// Ray-batch queries with lane-refill traversal (heads[SGR_HEAD_STRIDE * i],
+// i < 8, zeroed by the caller, are the fetch heads of the 8 slices of the
+// batch).
 template<bool ANY, bool REFILL>
 __global__ void __launch_bounds__(256) k_trace(SceneView S, RayBatchDev rb, uint32_t n,
                                                unsigned long long *stats) {
   __shared__ float4 stk_lds[2 * SGR_LDS_SLOTS * SGR_TRACE_BLOCK];
   const LdsStack stk{&stk_lds[threadIdx.x]};
   TraceCount cnt{0, 0};
-  uint32_t *heads = (uint32_t *)(stats + SGR_NUM_STATS);
+  uint32_t *heads = (uint32_t *)((char *)stats + SGR_STATS_HEADS_OFF);
   walk<ANY, false, REFILL>(
       S, n, heads,
       [&](uint32_t i, RayF &r) -> const SceneView * {
@@ -192,7 +193,7 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
   if (finishing(P, b, n)) return;
   const uint32_t *q = P.q[b & 1];
   walk<false, MULTI, REFILL>(
-      S1, n, &P.counts[SGR_CNT_HE + 8 * b],
+      S1, n, &P.counts[SGR_CNT_HE + SGR_HEADS_PER_B * b],
       [&](uint32_t i, RayF &r) -> const SceneView * {
         const uint32_t p = q[i];
         load_ray(P, p, r);
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathB
   if (finishing(P, b, P.counts[SGR_CNT_Q + b])) return;
   const uint32_t n = P.counts[SGR_CNT_S + b];
   walk<true, MULTI, REFILL>(
-      S1, n, &P.counts[SGR_CNT_HS + 8 * b],
+      S1, n, &P.counts[SGR_CNT_HS + SGR_HEADS_PER_B * b],
       [&](uint32_t i, RayF &r) -> const SceneView * {
         const uint32_t slot = P.sq[i];
         r.o = v3(P.sray[0][slot], P.sray[1][slot], P.sray[2][slot]);
