@@ -950,9 +950,10 @@ void launch_extend(bool multi, const SceneView &S, const StepView &J, const Path
                    uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
   // Camera rays take the plain walk even where later bounces refill: a
   // wave holds the samples of one or two pixels, whose walks stay in step
-  // and share their node and surface loads (1.31 M-triangle frame 155 ->
-  // 140 ms; the plain walk for the first bounce's shadow rays or for
-  // bounce 1 lost 3 % and 7 %).
+  // and share their node and surface loads, so the refill's balancing
+  // buys nothing there (the two tie on the 1.31 M-triangle frame) and
+  // its atomics are saved; for bounce 1 and for shadow rays, whose
+  // directions are random, the plain walk loses 3-9 %.
   const bool refill = S.walk_refill && b > 0;
   if (multi && refill)
     hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
