@@ -349,9 +349,10 @@ __device__ __forceinline__ void trace_plain(const SceneView &S1, uint32_t jb, ui
 // least SGR_REFILL lanes are idle the wave takes that many more rays with
 // one atomic on a slice head (a wave-wide ballot ranks the idle lanes),
 // so no lane idles for long while its wave still has rays to finish.
-// 48 is the best of 16 .. 64 on the 1.31 M-triangle frame (16 / 32 / 40 /
-// 48 / 56 / 64: 167 / 140 / 136 / 134 / 134 / 146 ms): refilled lanes
-// start together, and lanes that walk in step share their loads.  The N rays are
+// 48: with the slice heads on lines of their own, batches of 32 / 40 /
+// 48 / 56 gave 124.3 / 123.3 / 123.3 / 123.7 ms on the 1.31 M-triangle
+// frame (64 lanes, a wave idle until all its lanes are, lost 10 %
+// before).  The N rays are
 // cut into 8 slices, one per XCD (blockIdx.x & 7 under round-robin
 // dispatch), so neighbouring rays share an L2; a wave whose slice is
 // used up moves on to the next slices in turn, so no XCD idles while
