@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <stdexcept>
 #include <thread>
@@ -81,12 +82,17 @@ class Builder {
       while (j < n && fits(boxes[ids[j]])) j++;
       const size_t len = j - i;
       const unsigned use = len < 4096 ? 1u : nt;
+      std::vector<std::exception_ptr> err(use);
       auto run = [&](unsigned t) {
-        std::vector<Work> work;
-        uint32_t spare = SGR_NO_LINK;
-        placed[t].clear();
-        for (size_t k = i + len * t / use, e = i + len * (t + 1) / use; k < e; k++)
-          descend(t, ids[k], boxes[ids[k]], work, spare, placed[t]);
+        try {  // an exception must not leave a thread: carried to the caller
+          std::vector<Work> work;
+          uint32_t spare = SGR_NO_LINK;
+          placed[t].clear();
+          for (size_t k = i + len * t / use, e = i + len * (t + 1) / use; k < e; k++)
+            descend(t, ids[k], boxes[ids[k]], work, spare, placed[t]);
+        } catch (...) {
+          err[t] = std::current_exception();
+        }
       };
       if (use == 1) {
         run(0);
@@ -96,6 +102,8 @@ class Builder {
         run(0);
         for (auto &x : th) x.join();
       }
+      for (auto &e : err)
+        if (e) std::rethrow_exception(e);
       for (unsigned t = 0; t < use; t++)
         for (const Placement &pl : placed[t]) add_item(pl.node, pl.id);
       i = j;

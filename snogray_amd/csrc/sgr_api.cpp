@@ -9,6 +9,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -389,7 +391,13 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   std::memset(&V, 0, sizeof V);
   // ---- acceleration structure ----
   if (d->accel == SGR_ACCEL_OCTREE) {
-    build_octree(boxes, order, H.oct);
+    try {
+      build_octree(boxes, order, H.oct);
+    } catch (const std::bad_alloc &) {
+      return fail(SGR_ERR_NOMEM, "sgr_scene_create: out of host memory building the octree");
+    } catch (const std::exception &e) {
+      return fail(SGR_ERR_LIMIT, std::string("sgr_scene_create: ") + e.what());
+    }
     if (H.oct.max_depth > SGR_MAX_DEPTH)
       return fail(SGR_ERR_LIMIT, "sgr_scene_create: octree depth " +
                                      std::to_string(H.oct.max_depth) + " exceeds " +
