@@ -491,7 +491,8 @@ int sgr_octree_build(const sgr_scene_desc *d, uint32_t *nodes, uint64_t *num_nod
   return SGR_OK;
 }
 
-int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
+static int scene_create_impl(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out,
+                             sgr_scene *&made) {
   if (!ctx || !d || !out) return fail(SGR_ERR_INVALID, "sgr_scene_create: null argument");
   HostScene H;
   int rc = prepare_scene(d, H);
@@ -500,6 +501,7 @@ int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
   sgr_scene *sc = new sgr_scene;
   sc->ctx = ctx;
   ctx->live_scenes++;
+  made = sc;
   for (int a = 0; a < 3; a++) {
     sc->bbox_lo[a] = H.blo[a];
     sc->bbox_hi[a] = H.bhi[a];
@@ -568,12 +570,23 @@ int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
     if (!rc && e != hipSuccess) rc = fail(SGR_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
     (void)hipStreamDestroy(up);
   }
+  made = nullptr;
   if (rc) {
     sgr_scene_destroy(sc);
     return rc;
   }
   *out = sc;
   return SGR_OK;
+}
+
+// No C++ exception leaves the C ABI: host allocation failures become
+// SGR_ERR_NOMEM, and a half-made scene is destroyed.
+int sgr_scene_create(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out) {
+  sgr_scene *made = nullptr;
+  return sgr::guarded("sgr_scene_create", [&] { return scene_create_impl(ctx, d, out, made); },
+                      [&] {
+                        if (made) sgr_scene_destroy(made);
+                      });
 }
 
 int sgr_scene_destroy(sgr_scene *sc) {
@@ -1128,14 +1141,18 @@ extern "C" {
 
 int sgr_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p, float *accum,
                int accum_on_device, sgr_stats *stats) {
-  return render_accum(sc, cam, p, accum, accum_on_device, stats, nullptr, false);
+  return sgr::guarded("sgr_render", [&] {
+    return render_accum(sc, cam, p, accum, accum_on_device, stats, nullptr, false);
+  });
 }
 
 int sgr_render_guided(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p,
                       sgr_guide *guide, int train, float *accum, int accum_on_device,
                       sgr_stats *stats) {
   if (!guide) return fail(SGR_ERR_INVALID, "sgr_render_guided: null guide");
-  return render_accum(sc, cam, p, accum, accum_on_device, stats, guide, train != 0);
+  return sgr::guarded("sgr_render_guided", [&] {
+    return render_accum(sc, cam, p, accum, accum_on_device, stats, guide, train != 0);
+  });
 }
 
 int sgr_render_samples(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p,
@@ -1145,7 +1162,7 @@ int sgr_render_samples(sgr_scene *sc, const sgr_camera *cam, const sgr_render_pa
   if (rc) return rc;
   if (!out) return fail(SGR_ERR_INVALID, "sgr_render_samples: null output");
   HIPCHK(hipSetDevice(sc->ctx->device));
-  return run_render(sc, pl, nullptr, out, stats);
+  return sgr::guarded("sgr_render_samples", [&] { return run_render(sc, pl, nullptr, out, stats); });
 }
 
 // ImageSampledOutput::set_raw_min_y (image/image-sampled-output.cc:63-126)

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -19,6 +21,26 @@
 
 namespace sgr {
 struct Engine;
+int fail(int code, const std::string &msg);
+
+// Runs F (returning an sgr status); a C++ exception thrown inside turns
+// into an error code after CLEANUP runs, so none crosses the C ABI.
+template<class F, class C>
+int guarded(const char *what, F &&f, C &&cleanup) {
+  try {
+    return f();
+  } catch (const std::bad_alloc &) {
+    cleanup();
+    return fail(SGR_ERR_NOMEM, std::string(what) + ": out of host memory");
+  } catch (const std::exception &e) {
+    cleanup();
+    return fail(SGR_ERR_INVALID, std::string(what) + ": " + e.what());
+  }
+}
+template<class F>
+int guarded(const char *what, F &&f) {
+  return guarded(what, static_cast<F &&>(f), [] {});
+}
 }
 
 // Chunks sgr_render keeps in flight at most (one stream and one set of
