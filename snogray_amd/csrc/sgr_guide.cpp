@@ -192,7 +192,7 @@ using namespace sgr;
 
 extern "C" {
 
-int sgr_guide_create(sgr_ctx *ctx, uint64_t seed, sgr_guide **out) {
+static int sgr_guide_create_impl(sgr_ctx *ctx, uint64_t seed, sgr_guide **out) {
   if (!ctx || !out) return fail(SGR_ERR_INVALID, "sgr_guide_create: null argument");
   HIPCHK(hipSetDevice(ctx->device));
   sgr_guide *g = new sgr_guide;
@@ -230,6 +230,11 @@ int sgr_guide_create(sgr_ctx *ctx, uint64_t seed, sgr_guide **out) {
   }
   *out = g;
   return SGR_OK;
+}
+
+int sgr_guide_create(sgr_ctx *ctx, uint64_t seed, sgr_guide **out) {
+  return sgr::guarded("sgr_guide_create",
+                      [&] { return sgr_guide_create_impl(ctx, seed, out); });
 }
 
 int sgr_guide_destroy(sgr_guide *g) {

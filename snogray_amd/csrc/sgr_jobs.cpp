@@ -286,8 +286,8 @@ int sgr_job_footprint(const sgr_render_params *p, int caller_accum, uint64_t *by
   return SGR_OK;
 }
 
-int sgr_job_submit(sgr_ctx *ctx, sgr_scene *sc, const sgr_camera *cam,
-                   const sgr_render_params *p, float *accum_dev, uint64_t *job_id) {
+static int sgr_job_submit_impl(sgr_ctx *ctx, sgr_scene *sc, const sgr_camera *cam,
+                               const sgr_render_params *p, float *accum_dev, uint64_t *job_id) {
   if (!job_id) return fail(SGR_ERR_INVALID, "sgr_job_submit: null job_id");
   if (!sc || sc->ctx != ctx) return fail(SGR_ERR_INVALID, "sgr_job_submit: scene of another context");
   Engine *E;
@@ -329,7 +329,13 @@ int sgr_job_submit(sgr_ctx *ctx, sgr_scene *sc, const sgr_camera *cam,
   return SGR_OK;
 }
 
-int sgr_engine_step(sgr_ctx *ctx, sgr_step_info *info) {
+int sgr_job_submit(sgr_ctx *ctx, sgr_scene *sc, const sgr_camera *cam,
+                   const sgr_render_params *p, float *accum_dev, uint64_t *job_id) {
+  return sgr::guarded("sgr_job_submit",
+                      [&] { return sgr_job_submit_impl(ctx, sc, cam, p, accum_dev, job_id); });
+}
+
+static int sgr_engine_step_impl(sgr_ctx *ctx, sgr_step_info *info) {
   Engine *E;
   int rc = engine_get(ctx, E);
   if (rc) return rc;
@@ -469,6 +475,11 @@ int sgr_engine_step(sgr_ctx *ctx, sgr_step_info *info) {
   return SGR_OK;
 }
 
+int sgr_engine_step(sgr_ctx *ctx, sgr_step_info *info) {
+  return sgr::guarded("sgr_engine_step",
+                      [&] { return sgr_engine_step_impl(ctx, info); });
+}
+
 int sgr_job_poll(sgr_ctx *ctx, uint64_t id, sgr_job_status *s) {
   if (!ctx || !s) return fail(SGR_ERR_INVALID, "sgr_job_poll: null argument");
   Engine *E = ctx->engine;
@@ -548,9 +559,9 @@ int sgr_engine_stats(sgr_ctx *ctx, sgr_stats *stats) {
   return SGR_OK;
 }
 
-int sgr_render_batch(sgr_ctx *ctx, uint32_t n, sgr_scene *const *scenes, const sgr_camera *cams,
-                     const sgr_render_params *params, float *const *accum, int on_device,
-                     sgr_stats *stats) {
+static int sgr_render_batch_impl(sgr_ctx *ctx, uint32_t n, sgr_scene *const *scenes,
+                                 const sgr_camera *cams, const sgr_render_params *params,
+                                 float *const *accum, int on_device, sgr_stats *stats) {
   if (n && (!scenes || !cams || !params || !accum))
     return fail(SGR_ERR_INVALID, "sgr_render_batch: null argument");
   Engine *E;
@@ -623,6 +634,16 @@ int sgr_render_batch(sgr_ctx *ctx, uint32_t n, sgr_scene *const *scenes, const s
   }
   cleanup();
   return SGR_OK;
+}
+
+int sgr_render_batch(sgr_ctx *ctx, uint32_t n, sgr_scene *const *scenes, const sgr_camera *cams,
+                     const sgr_render_params *params, float *const *accum, int on_device,
+                     sgr_stats *stats) {
+  return sgr::guarded("sgr_render_batch",
+                      [&] {
+                        return sgr_render_batch_impl(ctx, n, scenes, cams, params, accum,
+                                                     on_device, stats);
+                      });
 }
 
 }  // extern "C"
