@@ -1019,10 +1019,16 @@ void launch_splat(bool multi, const RenderView &R, const StepView &J, const floa
 void launch_finish(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
                    const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
                    hipStream_t st) {
+  // A queue it takes has under SGR_FINISH_MAX rays: one per thread at most
+  // needs no more blocks, and a launch that finds its queue too long (or
+  // already finished) then dispatches a quarter of the blocks of the
+  // per-bounce kernels (this kernel holds 2 waves per SIMD).
+  unsigned g = SGR_FINISH_MAX / 256u;
+  g = grid < g ? grid : g;
   if (multi)
-    hipLaunchKernelGGL((k_finish<true>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats);
+    hipLaunchKernelGGL((k_finish<true>), dim3(g), dim3(256), 0, st, S, R, J, P, b, stats);
   else
-    hipLaunchKernelGGL((k_finish<false>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats);
+    hipLaunchKernelGGL((k_finish<false>), dim3(g), dim3(256), 0, st, S, R, J, P, b, stats);
 }
 
 void launch_count_queues(uint32_t maxp, const PathBufs &P, unsigned long long *stats,
