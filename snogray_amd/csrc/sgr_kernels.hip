@@ -946,6 +946,15 @@ void launch_raygen(bool multi, const RenderView &R, const StepView &J, const Pat
   if (multi) hipLaunchKernelGGL(k_raygen<true>, dim3(grid), dim3(256), 0, st, R, J, P);
   else hipLaunchKernelGGL(k_raygen<false>, dim3(grid), dim3(256), 0, st, R, J, P);
 }
+// Grid of the plain walk on a shallow octree: 3/8 of the default, the
+// blocks six waves per SIMD keep resident (24 KB of LDS stack each), so
+// every block starts at once and loops over its range; the default
+// grid's extra generations of short-lived blocks cost more than the
+// walks (Cornell box 256x256 frame 4.45 -> 3.40 ms).  Deep octrees keep
+// the default: there the later blocks even out the camera rays' uneven
+// ranges (1.31 M-triangle frame 119.3 -> 120.7 ms with the smaller grid).
+static inline unsigned plain_walk_grid(unsigned grid) { return (grid * 3u / 8u + 7u) & ~7u; }
+
 void launch_extend(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                    uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
   // Camera rays take the plain walk even where later bounces refill: a
@@ -955,6 +964,7 @@ void launch_extend(bool multi, const SceneView &S, const StepView &J, const Path
   // its atomics are saved; for bounce 1 and for shadow rays, whose
   // directions are random, the plain walk loses 3-9 %.
   const bool refill = S.walk_refill && b > 0;
+  if (!S.walk_refill) grid = plain_walk_grid(grid);
   if (multi && refill)
     hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else if (multi)
@@ -973,6 +983,13 @@ void launch_shade(bool multi, const SceneView &S, const RenderView &R, const Ste
     g0.rec_stride = 1;
   }
   const GuideView &g = G ? *G : g0;
+  // This kernel holds 2 waves per SIMD: 512 blocks of 4 waves fill the
+  // 256 CUs once, each wave looping over its block's range, and a launch
+  // with a short queue (or none left) dispatches an eighth of the blocks
+  // (grids of 4096 / 1024 / 512 / 256 blocks: Cornell 256x256 frame 4.94
+  // / 4.53 / 4.42 / 4.55 ms, 1.31 M-triangle frame 121.8 / 120.8 / 119.1
+  // / 127.2 ms).
+  grid = grid < 512u ? grid : 512u;
   if (multi)
     hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats, g);
   else if (G)
@@ -987,6 +1004,7 @@ void launch_shade(bool multi, const SceneView &S, const RenderView &R, const Ste
 }
 void launch_occlude(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                     uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
+  if (!S.walk_refill) grid = plain_walk_grid(grid);
   if (multi && S.walk_refill)
     hipLaunchKernelGGL((k_occlude<true, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else if (multi)
