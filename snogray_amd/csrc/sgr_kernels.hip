@@ -1016,6 +1016,9 @@ void launch_occlude(bool multi, const SceneView &S, const StepView &J, const Pat
 }
 void launch_gather(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
                    uint32_t b, unsigned grid, hipStream_t st, const GuideView *G) {
+  // 8 waves per SIMD keep 2048 blocks resident: one generation of blocks
+  // (1.31 M-triangle frame 119.6 -> 119.0 ms against 4096)
+  grid = grid < 2048u ? grid : 2048u;
   GuideView g0;
   if (!G) {
     std::memset(&g0, 0, sizeof g0);
