@@ -231,6 +231,10 @@ typedef struct sgr_stats {
     double   ms_other;
     uint32_t extend_launches;
     uint32_t shadow_launches;
+    /* transmissions into a medium nested deeper than 8 levels: the
+     * device keeps 8 media per path (the reference's stack is unbounded,
+     * material/media.cc:75-118) and does not push these */
+    uint64_t media_overflows;
 } sgr_stats;
 
 /* Octree statistics (space/octree.h:53-67). */
@@ -319,10 +323,18 @@ typedef struct sgr_engine_config {
                                  default SGR_DEFAULT_PATHS_IN_FLIGHT) */
     uint64_t memory_budget;   /* bytes of job accumulators admitted at
                                  once (0: no limit) */
-    uint32_t grid_blocks;     /* workgroups per launch (0: 16 per CU);
-                                 rounded up to a multiple of 8 */
+    uint32_t grid_blocks;     /* workgroups of the camera-ray, refill-walk
+                                 and splat launches (0: 16 per CU);
+                                 rounded up to a multiple of 8.  The
+                                 shading, gather, finishing and plain-walk
+                                 launches are capped at the workgroups
+                                 their occupancy keeps resident (2, 8, 4
+                                 and 6 per CU) */
     uint32_t use_graph;       /* replay each step from a captured
                                  hipGraph (one per bounce count) */
+    uint32_t finish_max;      /* ray-queue length below which a late
+                                 bounce's paths are finished in one launch
+                                 (0: 2^18; results do not depend on it) */
 } sgr_engine_config;
 
 enum { SGR_JOB_QUEUED = 0, SGR_JOB_ACTIVE = 1, SGR_JOB_DONE = 2 };

@@ -20,11 +20,11 @@ namespace {
 
 struct Counters {
   uint64_t eye = 0, isect = 0, shadow = 0, nc_c = 0, st_c = 0, nc_s = 0,
-           st_s = 0, illum = 0, paths = 0;
+           st_s = 0, illum = 0, paths = 0, media_ovf = 0;
   void add(const Counters &o) {
     eye += o.eye; isect += o.isect; shadow += o.shadow; nc_c += o.nc_c;
     st_c += o.st_c; nc_s += o.nc_s; st_s += o.st_s; illum += o.illum;
-    paths += o.paths;
+    paths += o.paths; media_ovf += o.media_ovf;
   }
 };
 
@@ -161,8 +161,13 @@ template<class R> struct Tracer {
 
   // PathInteg::Li (render/path-integ.cc:133-407).  Returns radiance and
   // sets alpha.
+  // The media stack holds kMediaMax levels, as the device's does: a
+  // transmission into a deeper medium is not pushed (counted in
+  // media_ovf).  The reference's stack is unbounded
+  // (material/media.cc:75-118); the two agree up to that depth.
+  static constexpr int kMediaMax = 8;
   C3<R> Li(Ray<R> ray, uint64_t pk, uint64_t qk, uint32_t s, R &alpha) {
-    R media[64];
+    R media[kMediaMax];
     int depth = 1;
     media[0] = 1;  // RenderContext::default_medium (ior 1)
     C3<R> radiance(0), T(1);
@@ -220,8 +225,10 @@ template<class R> struct Tracer {
         if (has) {
           if (is.back) {
             if (depth >= 2) depth--;
-          } else if (depth < 64) {
+          } else if (depth < kMediaMax) {
             media[depth++] = mior;
+          } else {
+            cnt.media_ovf++;
           }
         }
       }
@@ -411,6 +418,7 @@ static void render_impl(const Scene<R> &sc, const sgr_camera &cam,
     stats->illum_calls = tot.illum;
     stats->samples = (uint64_t)P.w * P.h * P.spp;
     stats->paths_traced = tot.paths;
+    stats->media_overflows = tot.media_ovf;
   }
 }
 

@@ -100,7 +100,8 @@ class Stats(C.Structure):
         "surf_tests_shadow", "illum_calls", "samples", "paths_traced")] + \
         [(k, C.c_double) for k in ("ms_render", "ms_extend", "ms_shade",
                                    "ms_shadow", "ms_splat", "ms_other")] + \
-        [("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32)]
+        [("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
+         ("media_overflows", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -108,7 +109,8 @@ class Stats(C.Structure):
 
 class EngineConfig(C.Structure):
     _fields_ = [("path_capacity", C.c_uint64), ("memory_budget", C.c_uint64),
-                ("grid_blocks", C.c_uint32), ("use_graph", C.c_uint32)]
+                ("grid_blocks", C.c_uint32), ("use_graph", C.c_uint32),
+                ("finish_max", C.c_uint32)]
 
 
 SGR_JOB_QUEUED, SGR_JOB_ACTIVE, SGR_JOB_DONE = 0, 1, 2

@@ -1,13 +1,16 @@
 """Batched uneven jobs on the multi-job engine, with an autotuner for the
-engine's path capacity (paths in flight per step, i.e. the ray / shade /
-shadow queue length) and its launch width (workgroups per launch).
+engine's queue and launch sizes: the path capacity (paths in flight per
+step, i.e. the length of the ray, shade and shadow queues), the launch
+width of the camera-ray, refill-walk and splat kernels, and the
+ray-queue length below which a late bounce's paths are finished in one
+launch.
 
 The tuner times a few engine steps of a probe batch (the real jobs with
 their sample counts cut down) per candidate and keeps the configuration
 with the most paths per device-millisecond.  A job's chunk is
 min(its own chunk, capacity / pattern pixels) samples, so tuner choices
 change a job's result only in the fp32 summation order of its
-accumulators."""
+accumulators (the launch width and finishing threshold not at all)."""
 import itertools
 import time
 
@@ -16,13 +19,17 @@ import numpy as np
 from . import abi, native
 
 DEFAULT_CAPACITIES = (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
-DEFAULT_GRIDS = (0, 4096 * 2, 4096 * 4)  # 0: 16 workgroups per CU
+# 0: 16 workgroups per CU.  The width reaches raygen, the refill walks and
+# the splat; the other kernels are capped at what stays resident.
+DEFAULT_GRIDS = (0, 4096 * 2)
+DEFAULT_FINISH = (1 << 16, 1 << 18, 1 << 20)
 
 
-def _probe(ctx, jobs, capacity, grid, max_steps, spp_cap):
+def _probe(ctx, jobs, capacity, grid, max_steps, spp_cap, finish_max=0):
     """Paths per device-ms of the probe batch: a first pass captures the
     step graphs, the second pass is timed."""
-    eng = native.Engine(ctx, path_capacity=capacity, grid_blocks=grid, use_graph=True)
+    eng = native.Engine(ctx, path_capacity=capacity, grid_blocks=grid, use_graph=True,
+                        finish_max=finish_max)
     rate = 0.0
     try:
         for _ in range(2):
@@ -44,10 +51,11 @@ def _probe(ctx, jobs, capacity, grid, max_steps, spp_cap):
     return rate
 
 
-def autotune(ctx, jobs, capacities=DEFAULT_CAPACITIES, grids=DEFAULT_GRIDS, steps=6,
-             spp_cap=16, probe_jobs=24, log=None):
-    """JOBS: [(native.Scene, camera, params)].  Returns (best config
-    dict, [(config, paths per ms)])."""
+def autotune(ctx, jobs, capacities=DEFAULT_CAPACITIES, grids=DEFAULT_GRIDS,
+             finish=DEFAULT_FINISH, steps=6, spp_cap=16, probe_jobs=24, log=None):
+    """JOBS: [(native.Scene, camera, params)].  Capacity and launch width
+    are searched together, then the finishing threshold at the chosen
+    pair.  Returns (best config dict, [(config, paths per ms)])."""
     def npix(p):
         _, _, w, h, m = abi.render_window(p)
         return (w + 2 * m) * (h + 2 * m)
@@ -58,10 +66,19 @@ def autotune(ctx, jobs, capacities=DEFAULT_CAPACITIES, grids=DEFAULT_GRIDS, step
         if cap < max_npix:
             continue
         rate = _probe(ctx, probe, cap, grid, steps, spp_cap)
-        table.append(({"path_capacity": cap, "grid_blocks": grid}, rate))
+        table.append(({"path_capacity": cap, "grid_blocks": grid, "finish_max": 0}, rate))
         if log:
             log(f"autotune capacity {cap} grid {grid or 'auto'}: {rate / 1e3:.1f} M paths/s")
-    return pick(table), table
+    best = pick(table)
+    stage2 = [(c, r) for c, r in table if c == best]
+    for fm in finish:
+        cfg = dict(best, finish_max=fm)
+        rate = _probe(ctx, probe, cfg["path_capacity"], cfg["grid_blocks"], steps, spp_cap, fm)
+        table.append((cfg, rate))
+        stage2.append((cfg, rate))
+        if log:
+            log(f"autotune finish below {fm} rays: {rate / 1e3:.1f} M paths/s")
+    return max(stage2, key=lambda t: t[1])[0], table
 
 
 def pick(table, tol=0.03):
@@ -74,11 +91,12 @@ def pick(table, tol=0.03):
     return max(near, key=lambda t: (t[0]["path_capacity"], t[1]))[0]
 
 
-def run_batch(ctx, jobs, path_capacity=abi.SGR_DEFAULT_PATHS_IN_FLIGHT, grid_blocks=0, keep_results=False):
+def run_batch(ctx, jobs, path_capacity=abi.SGR_DEFAULT_PATHS_IN_FLIGHT, grid_blocks=0,
+              keep_results=False, finish_max=0):
     """Submit every job, step until all are done.  Returns a report with
     aggregate samples/s and per-job latency (submit to done)."""
     eng = native.Engine(ctx, path_capacity=path_capacity, grid_blocks=grid_blocks,
-                        use_graph=True)
+                        use_graph=True, finish_max=finish_max)
     results = {}
     t0 = time.perf_counter()
     ids = [eng.submit(sc, cam, p) for sc, cam, p in jobs]

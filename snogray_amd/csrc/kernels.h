@@ -8,7 +8,11 @@
 
 namespace sgr {
 
-#define SGR_MEDIA_MAX 8       // media stack levels per path
+// Media stack levels per path.  The reference's stack is unbounded
+// (material/media.cc:75-118); here a transmission into a ninth nested
+// medium is not pushed (the path keeps refracting against the eighth)
+// and is counted in SGR_STAT_MEDIA_OVF.  The oracle applies the same cap.
+#define SGR_MEDIA_MAX 8
 #define SGR_MAX_SLOTS 64      // shadow-ray slots per path (2 per light sample)
 #define SGR_COMPACT_ITEMS 4096  // queue positions per compaction block (256 x 16)
 
@@ -42,6 +46,7 @@ static_assert(SGR_CNT_HE >= 2 * SGR_MAX_BOUNCES + 4 && SGR_CNT_HE % SGR_HEAD_STR
 #define SGR_STAT_ILLUM   4
 #define SGR_STAT_ISECT   5
 #define SGR_STAT_SHADOW  6
+#define SGR_STAT_MEDIA_OVF 7  // media pushes dropped at SGR_MEDIA_MAX
 #define SGR_NUM_STATS    8
 // The statistics buffer also holds the 8 fetch heads of a ray-batch query,
 // from byte SGR_STATS_HEADS_OFF on, SGR_HEAD_STRIDE words apart.
@@ -116,6 +121,8 @@ struct PathBufs {
   // finished by k_finish in one launch instead of bounce by bounce
   // (fin_max 0: never).
   uint32_t fin_from = 0, fin_max = 0;
+  // Statistics slot counting dropped media pushes (set by the launchers).
+  unsigned long long *media_ovf = nullptr;
 };
 
 // Bounce from which, and queue length below which, renders and engine
@@ -123,6 +130,8 @@ struct PathBufs {
 #define SGR_FINISH_FROM 1u
 #define SGR_FINISH_MAX (1u << 18)
 
+// Sets the CU count the per-kernel launch caps are derived from.
+void set_launch_cus(unsigned num_cu);
 void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool any,
                   unsigned long long *stats, hipStream_t st);
 void launch_raygen(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,

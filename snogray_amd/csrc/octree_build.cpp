@@ -97,9 +97,18 @@ class Builder {
       if (use == 1) {
         run(0);
       } else {
+        // A thread that cannot be started leaves its range to this one;
+        // the threads already running are joined before anything here
+        // can leave the scope their ranges refer to.
         std::vector<std::thread> th;
-        for (unsigned t = 1; t < use; t++) th.emplace_back(run, t);
+        unsigned started = 1;
+        try {
+          th.reserve(use - 1);
+          for (; started < use; started++) th.emplace_back(run, started);
+        } catch (...) {
+        }
         run(0);
+        for (unsigned t = started; t < use; t++) run(t);
         for (auto &x : th) x.join();
       }
       for (auto &e : err)

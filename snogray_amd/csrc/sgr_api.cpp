@@ -152,6 +152,7 @@ int sgr_ctx_create(int device, sgr_ctx **out) {
   c->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+  sgr::set_launch_cus((unsigned)c->num_cu);
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   for (auto &ls : c->lane_stream)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ls, hipStreamNonBlocking);
@@ -311,6 +312,8 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     float mhi[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
     for (uint32_t k = 0; k < m.num_vertices; k++)
       for (int a = 0; a < 3; a++) {
+        if (!std::isfinite(m.vertices[3 * k + a]))  // it would poison the scene box and horizon
+          return fail(SGR_ERR_INVALID, "mesh " + std::to_string(mi) + ": non-finite vertex");
         mlo[a] = fmin_t(mlo[a], m.vertices[3 * k + a]);
         mhi[a] = fmax_t(mhi[a], m.vertices[3 * k + a]);
       }
@@ -387,6 +390,12 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     grow(bx.lo, bx.hi);
     order.push_back(p);
   }
+  // A surface whose box is not finite would grow the octree root without
+  // end (space/octree-builder.cc:180-196 doubles it until the box fits).
+  for (uint32_t q : order)
+    for (int k = 0; k < 3; k++)
+      if (!std::isfinite(boxes[q].lo[k]) || !std::isfinite(boxes[q].hi[k]))
+        return fail(SGR_ERR_INVALID, "primitive " + std::to_string(q) + ": non-finite coordinates");
   SceneView &V = H.V;
   std::memset(&V, 0, sizeof V);
   // ---- acceleration structure ----
@@ -466,9 +475,9 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
 
 extern "C" {
 
-int sgr_octree_build(const sgr_scene_desc *d, uint32_t *nodes, uint64_t *num_nodes,
-                     uint32_t *surf_ids, uint64_t *num_surf_ids, float *origin_size4,
-                     sgr_octree_stats *stats) {
+static int octree_build_impl(const sgr_scene_desc *d, uint32_t *nodes, uint64_t *num_nodes,
+                             uint32_t *surf_ids, uint64_t *num_surf_ids, float *origin_size4,
+                             sgr_octree_stats *stats) {
   HostScene H;
   int rc = prepare_scene(d, H);
   if (rc) return rc;
@@ -489,6 +498,14 @@ int sgr_octree_build(const sgr_scene_desc *d, uint32_t *nodes, uint64_t *num_nod
     stats->avg_depth = H.oct.avg_depth;
   }
   return SGR_OK;
+}
+
+int sgr_octree_build(const sgr_scene_desc *d, uint32_t *nodes, uint64_t *num_nodes,
+                     uint32_t *surf_ids, uint64_t *num_surf_ids, float *origin_size4,
+                     sgr_octree_stats *stats) {
+  return sgr::guarded("sgr_octree_build", [&] {
+    return octree_build_impl(d, nodes, num_nodes, surf_ids, num_surf_ids, origin_size4, stats);
+  });
 }
 
 static int scene_create_impl(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out,
@@ -1086,6 +1103,7 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     stats->node_visits_shadow = st[SGR_STAT_NODES_S];
     stats->surf_tests_shadow = st[SGR_STAT_TESTS_S];
     stats->illum_calls = st[SGR_STAT_ILLUM];
+    stats->media_overflows = st[SGR_STAT_MEDIA_OVF];
     for (auto &m : marks) {
       float e = 0;
       (void)hipEventElapsedTime(&e, ctx->evpool[m.second], ctx->evpool[m.second + 1]);

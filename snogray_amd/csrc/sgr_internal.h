@@ -5,6 +5,7 @@
 #include <atomic>
 #include <exception>
 #include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,9 @@ int guarded(const char *what, F &&f, C &&cleanup) {
   } catch (const std::bad_alloc &) {
     cleanup();
     return fail(SGR_ERR_NOMEM, std::string(what) + ": out of host memory");
+  } catch (const std::length_error &e) {  // a structure outgrew its index range
+    cleanup();
+    return fail(SGR_ERR_LIMIT, std::string(what) + ": " + e.what());
   } catch (const std::exception &e) {
     cleanup();
     return fail(SGR_ERR_INVALID, std::string(what) + ": " + e.what());

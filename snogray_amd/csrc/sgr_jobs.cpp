@@ -143,7 +143,7 @@ int ensure_engine_work(sgr_ctx *ctx, Engine *E, size_t cap, size_t K) {
   E->P.n = (uint32_t)cap;
   E->P.K = (uint32_t)K;
   E->P.fin_from = SGR_FINISH_FROM;
-  E->P.fin_max = SGR_FINISH_MAX;
+  E->P.fin_max = E->cfg.finish_max ? E->cfg.finish_max : SGR_FINISH_MAX;
   return SGR_OK;
 }
 
@@ -262,6 +262,7 @@ int sgr_engine_configure(sgr_ctx *ctx, const sgr_engine_config *cfg) {
   uint64_t cap = cfg->path_capacity ? cfg->path_capacity : (SGR_DEFAULT_PATHS_IN_FLIGHT);
   if (cap >= (1ull << 31)) return fail(SGR_ERR_LIMIT, "sgr_engine_configure: path_capacity too large");
   if (cfg->grid_blocks > (1u << 20)) return fail(SGR_ERR_LIMIT, "sgr_engine_configure: grid_blocks too large");
+  if (cfg->finish_max > (1u << 24)) return fail(SGR_ERR_LIMIT, "sgr_engine_configure: finish_max too large");
   HIPCHK(hipStreamSynchronize(ctx->stream));
   drop_graphs(E);
   if (E->work) {
@@ -306,25 +307,36 @@ static int sgr_job_submit_impl(sgr_ctx *ctx, sgr_scene *sc, const sgr_camera *ca
   if (E->cfg.memory_budget && j.footprint > E->cfg.memory_budget)
     return fail(SGR_ERR_LIMIT, "sgr_job_submit: job footprint " + std::to_string(j.footprint) +
                                    " exceeds the memory budget");
-  if (accum_dev) {
-    j.accum = accum_dev;
-  } else {
-    hipError_t e = hipMalloc((void **)&j.accum, j.footprint);
-    if (e == hipSuccess) e = hipMemsetAsync(j.accum, 0, j.footprint, ctx->stream);
-    if (e != hipSuccess) {
-      if (j.accum) (void)hipFree(j.accum);
-      return fail(SGR_ERR_NOMEM, std::string("sgr_job_submit: accumulators: ") + hipGetErrorString(e));
-    }
-    j.owns_accum = true;
-  }
-  j.pl.R.accum = j.accum;
-  j.tiles_out = splat_tiles(j.pl.R);
+  // The engine's containers take the job first: if they throw, nothing
+  // has been acquired yet.  The accumulators and the scene reference come
+  // after, and a failure there removes the entries again.
   j.id = E->next_id++;
   j.sc = sc;
   j.t_submit = E->now_ms();
+  auto it = E->jobs.emplace(j.id, j).first;
+  try {
+    E->queued.push_back(j.id);
+  } catch (...) {
+    E->jobs.erase(it);
+    throw;
+  }
+  Job &jj = it->second;
+  if (accum_dev) {
+    jj.accum = accum_dev;
+  } else {
+    hipError_t e = hipMalloc((void **)&jj.accum, jj.footprint);
+    if (e == hipSuccess) e = hipMemsetAsync(jj.accum, 0, jj.footprint, ctx->stream);
+    if (e != hipSuccess) {
+      if (jj.accum) (void)hipFree(jj.accum);
+      E->queued.pop_back();
+      E->jobs.erase(it);
+      return fail(SGR_ERR_NOMEM, std::string("sgr_job_submit: accumulators: ") + hipGetErrorString(e));
+    }
+    jj.owns_accum = true;
+  }
+  jj.pl.R.accum = jj.accum;
+  jj.tiles_out = splat_tiles(jj.pl.R);
   sc->jobs_using++;
-  E->jobs[j.id] = j;
-  E->queued.push_back(j.id);
   *job_id = j.id;
   return SGR_OK;
 }
@@ -555,13 +567,17 @@ int sgr_engine_stats(sgr_ctx *ctx, sgr_stats *stats) {
   stats->node_visits_shadow = st[SGR_STAT_NODES_S];
   stats->surf_tests_shadow = st[SGR_STAT_TESTS_S];
   stats->illum_calls = st[SGR_STAT_ILLUM];
+  stats->media_overflows = st[SGR_STAT_MEDIA_OVF];
   stats->ms_render = E->ms_device;
   return SGR_OK;
 }
 
+// IDS: the jobs submitted so far; the caller releases them on every exit,
+// exceptions included.
 static int sgr_render_batch_impl(sgr_ctx *ctx, uint32_t n, sgr_scene *const *scenes,
                                  const sgr_camera *cams, const sgr_render_params *params,
-                                 float *const *accum, int on_device, sgr_stats *stats) {
+                                 float *const *accum, int on_device, sgr_stats *stats,
+                                 std::vector<uint64_t> &ids) {
   if (n && (!scenes || !cams || !params || !accum))
     return fail(SGR_ERR_INVALID, "sgr_render_batch: null argument");
   Engine *E;
@@ -573,9 +589,10 @@ static int sgr_render_batch_impl(sgr_ctx *ctx, uint32_t n, sgr_scene *const *sce
     rc = sgr_engine_stats(ctx, &before);
     if (rc) return rc;
   }
-  std::vector<uint64_t> ids;
+  ids.reserve(n);  // push_back after a successful submit cannot throw
   auto cleanup = [&]() {
     for (uint64_t id : ids) sgr_job_release(ctx, id);
+    ids.clear();
   };
   for (uint32_t i = 0; i < n; i++) {
     if (!accum[i]) {
@@ -630,6 +647,7 @@ static int sgr_render_batch_impl(sgr_ctx *ctx, uint32_t n, sgr_scene *const *sce
     stats->node_visits_shadow -= before.node_visits_shadow;
     stats->surf_tests_shadow -= before.surf_tests_shadow;
     stats->illum_calls -= before.illum_calls;
+    stats->media_overflows -= before.media_overflows;
     stats->ms_render -= before.ms_render;
   }
   cleanup();
@@ -639,11 +657,16 @@ static int sgr_render_batch_impl(sgr_ctx *ctx, uint32_t n, sgr_scene *const *sce
 int sgr_render_batch(sgr_ctx *ctx, uint32_t n, sgr_scene *const *scenes, const sgr_camera *cams,
                      const sgr_render_params *params, float *const *accum, int on_device,
                      sgr_stats *stats) {
-  return sgr::guarded("sgr_render_batch",
-                      [&] {
-                        return sgr_render_batch_impl(ctx, n, scenes, cams, params, accum,
-                                                     on_device, stats);
-                      });
+  std::vector<uint64_t> ids;
+  return sgr::guarded(
+      "sgr_render_batch",
+      [&] {
+        return sgr_render_batch_impl(ctx, n, scenes, cams, params, accum, on_device, stats, ids);
+      },
+      [&] {
+        for (uint64_t id : ids) sgr_job_release(ctx, id);
+        ids.clear();
+      });
 }
 
 }  // extern "C"

@@ -438,6 +438,8 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
             } else if (depth < SGR_MEDIA_MAX) {
               P.media[(size_t)depth * N + p] = M.ior_n;
               depth++;
+            } else if (P.media_ovf) {
+              atomicAdd(P.media_ovf, 1ull);
             }
           }
           nst = (nst & (SGR_ST_PENDING | SGR_ST_STRAT)) | (path_len + 1) |
@@ -922,6 +924,15 @@ __global__ void __launch_bounds__(64) k_count_queues(uint32_t maxp, PathBufs P,
 // ---------------------------------------------------------------------------
 // Host-callable launchers
 // ---------------------------------------------------------------------------
+// Compute units of the device the launches go to.  The per-kernel caps
+// below are resident workgroups per CU at each kernel's occupancy (set by
+// its register use): a launch of that many blocks fills the chip once and
+// each block loops over its range.  They apply whatever grid the caller
+// asks for; the caller's grid sets raygen, the refill walks and the
+// compaction-free kernels.
+static unsigned s_num_cu = 256;
+void set_launch_cus(unsigned num_cu) { s_num_cu = num_cu ? num_cu : 256u; }
+
 static inline unsigned grid_for(uint32_t n, unsigned cap) {
   unsigned g = (n + 255) / 256;
   if (g == 0) g = 1;
@@ -946,14 +957,18 @@ void launch_raygen(bool multi, const RenderView &R, const StepView &J, const Pat
   if (multi) hipLaunchKernelGGL(k_raygen<true>, dim3(grid), dim3(256), 0, st, R, J, P);
   else hipLaunchKernelGGL(k_raygen<false>, dim3(grid), dim3(256), 0, st, R, J, P);
 }
-// Grid of the plain walk on a shallow octree: 3/8 of the default, the
-// blocks six waves per SIMD keep resident (24 KB of LDS stack each), so
-// every block starts at once and loops over its range; the default
-// grid's extra generations of short-lived blocks cost more than the
-// walks (Cornell box 256x256 frame 4.45 -> 3.40 ms).  Deep octrees keep
-// the default: there the later blocks even out the camera rays' uneven
-// ranges (1.31 M-triangle frame 119.3 -> 120.7 ms with the smaller grid).
-static inline unsigned plain_walk_grid(unsigned grid) { return (grid * 3u / 8u + 7u) & ~7u; }
+// Grid of the plain walk on a shallow octree: at most the blocks six
+// waves per SIMD keep resident (24 KB of LDS stack each, 6 per CU), so
+// every block starts at once and loops over its range: more generations
+// of short-lived blocks cost more than the walks (Cornell box 256x256
+// frame 4.45 -> 3.40 ms).  On deep octrees the camera rays' plain walk
+// keeps the caller's grid, whose later blocks
+// even out the camera rays' uneven ranges (1.31 M-triangle frame 119.3 ->
+// 120.7 ms with the smaller grid).
+static inline unsigned plain_walk_grid(unsigned grid) {
+  const unsigned cap = (6u * s_num_cu + 7u) & ~7u;
+  return grid < cap ? grid : cap;
+}
 
 void launch_extend(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                    uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
@@ -989,13 +1004,15 @@ void launch_shade(bool multi, const SceneView &S, const RenderView &R, const Ste
   // (grids of 4096 / 1024 / 512 / 256 blocks: Cornell 256x256 frame 4.94
   // / 4.53 / 4.42 / 4.55 ms, 1.31 M-triangle frame 121.8 / 120.8 / 119.1
   // / 127.2 ms).
-  grid = grid < 512u ? grid : 512u;
+  grid = grid < 2u * s_num_cu ? grid : 2u * s_num_cu;
+  PathBufs Pk = P;
+  Pk.media_ovf = stats + SGR_STAT_MEDIA_OVF;
   if (multi)
-    hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats, g);
+    hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
   else if (G)
-    hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats, g);
+    hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
   else
-    hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(256), 0, st, S, R, J, P, b, stats, g);
+    hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
   // the next ray queue and this bounce's shadow queue, in order
   const unsigned nblk = (P.n + SGR_COMPACT_ITEMS - 1) / SGR_COMPACT_ITEMS;
   hipLaunchKernelGGL(k_compact_count, dim3(nblk), dim3(256), 0, st, P, b);
@@ -1018,7 +1035,7 @@ void launch_gather(bool multi, const RenderView &R, const StepView &J, const Pat
                    uint32_t b, unsigned grid, hipStream_t st, const GuideView *G) {
   // 8 waves per SIMD keep 2048 blocks resident: one generation of blocks
   // (1.31 M-triangle frame 119.6 -> 119.0 ms against 4096)
-  grid = grid < 2048u ? grid : 2048u;
+  grid = grid < 8u * s_num_cu ? grid : 8u * s_num_cu;
   GuideView g0;
   if (!G) {
     std::memset(&g0, 0, sizeof g0);
@@ -1045,11 +1062,14 @@ void launch_finish(bool multi, const SceneView &S, const RenderView &R, const St
   // already finished) then dispatches a quarter of the blocks of the
   // per-bounce kernels (this kernel holds 2 waves per SIMD).
   unsigned g = SGR_FINISH_MAX / 256u;
+  g = g < 4u * s_num_cu ? g : 4u * s_num_cu;
   g = grid < g ? grid : g;
+  PathBufs Pk = P;
+  Pk.media_ovf = stats + SGR_STAT_MEDIA_OVF;
   if (multi)
-    hipLaunchKernelGGL((k_finish<true>), dim3(g), dim3(256), 0, st, S, R, J, P, b, stats);
+    hipLaunchKernelGGL((k_finish<true>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);
   else
-    hipLaunchKernelGGL((k_finish<false>), dim3(g), dim3(256), 0, st, S, R, J, P, b, stats);
+    hipLaunchKernelGGL((k_finish<false>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);
 }
 
 void launch_count_queues(uint32_t maxp, const PathBufs &P, unsigned long long *stats,

@@ -259,14 +259,17 @@ class Engine:
     fits the memory budget and runs the next sample chunk of every active
     job that fits the path capacity in one launch sequence."""
 
-    def __init__(self, ctx, path_capacity=0, memory_budget=0, grid_blocks=0, use_graph=True):
+    def __init__(self, ctx, path_capacity=0, memory_budget=0, grid_blocks=0, use_graph=True,
+                 finish_max=0):
         self.ctx = ctx
         self.jobs = set()
         ctx.engines.add(self)
-        self.configure(path_capacity, memory_budget, grid_blocks, use_graph)
+        self.configure(path_capacity, memory_budget, grid_blocks, use_graph, finish_max)
 
-    def configure(self, path_capacity=0, memory_budget=0, grid_blocks=0, use_graph=True):
-        cfg = abi.EngineConfig(path_capacity, memory_budget, grid_blocks, int(bool(use_graph)))
+    def configure(self, path_capacity=0, memory_budget=0, grid_blocks=0, use_graph=True,
+                  finish_max=0):
+        cfg = abi.EngineConfig(path_capacity, memory_budget, grid_blocks, int(bool(use_graph)),
+                               finish_max)
         check(lib().sgr_engine_configure(self.ctx.h, C.byref(cfg)), "sgr_engine_configure")
         self.cfg = cfg
 

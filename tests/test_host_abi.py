@@ -41,6 +41,24 @@ def test_errors_are_reported_not_raised():
     sd.mesh(np.zeros((3, 3), np.float32), np.array([[0, 1, 2]], np.uint32), 3)
     with pytest.raises(native.SgrError, match="bad material"):
         native.octree_build(sd)
+    # a non-finite corner would grow the root without end
+    for bad in (np.nan, np.inf):
+        sd = scenes.SceneDesc()
+        m = sd.lambert((0.5, 0.5, 0.5))
+        v = np.array([(0, 0, 0), (1, 0, 0), (0, bad, 0)], np.float32)
+        sd.mesh(v, np.array([[0, 1, 2]], np.uint32), m)
+        with pytest.raises(native.SgrError, match="non-finite"):
+            native.octree_build(sd)
+    sd = scenes.SceneDesc()
+    m = sd.lambert((0.5, 0.5, 0.5))
+    sd.sphere((0, 0, 0), np.inf, m)
+    with pytest.raises(native.SgrError, match="non-finite"):
+        native.octree_build(sd)
+    sd = scenes.SceneDesc()
+    m = sd.lambert((0.5, 0.5, 0.5))
+    sd.tripar((0, 0, 0), (1, 0, 0), (0, np.nan, 0), m)
+    with pytest.raises(native.SgrError, match="non-finite"):
+        native.octree_build(sd)
 
 
 def assert_same_tree(sd):
