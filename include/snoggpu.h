@@ -303,6 +303,24 @@ int sgr_render(sgr_scene *scene, const sgr_camera *cam,
 int sgr_render_samples(sgr_scene *scene, const sgr_camera *cam,
                        const sgr_render_params *params, float *out,
                        sgr_stats *stats);
+/* sgr_render that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  accum
+ * is device memory of the context's device (added to).  The path state of
+ * the render lives in a fixed scratch array of SGR_NOALLOC_SCRATCH_BYTES
+ * per device (held until the process ends); the frame runs in chunks of
+ * as many samples per pattern pixel as fit there (at most the chunk
+ * params->paths_in_flight gives), one chunk in flight, so the image is the
+ * one sgr_render gives with that chunk size.  A window whose pattern
+ * pixels (window plus filter margin) need more than the scratch for one
+ * sample each fails with SGR_ERR_LIMIT.  Every call on a device shares
+ * that device's scratch: calls that overlap on one device race (one
+ * context per device, calls serialised, as for every entry point).
+ * SGR_RENDER_KERNEL_TIMING is ignored. */
+#define SGR_NOALLOC_SCRATCH_BYTES (32u << 20)
+int sgr_render_noalloc(sgr_scene *scene, const sgr_camera *cam,
+                       const sgr_render_params *params, float *accum_dev,
+                       sgr_stats *stats);
 /* Normalise accumulators into an RGBA image (4 floats per pixel), as
  * ImageSampledOutput::set_raw_min_y does: divide by weight, clamp alpha,
  * clamp colour at 0, un-premultiply, exposure, contrast. */

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -137,6 +138,36 @@ Grid grid_shape(uint32_t n) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Scratch of sgr_render_noalloc: the path state of its chunks lives in
+// this array, so that entry point needs no device allocation.  Each
+// device holds its own copy for the life of the process; host code uses
+// the per-device address resolved at context creation, never the array's
+// own name (a host-side stand-in).
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[SGR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// With DEVICE current: its copy's address, resolved once per device.
+hipError_t scratch_address(int device, void **out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
+
 extern "C" {
 
 const char *sgr_last_error(void) { return sgr::g_err.c_str(); }
@@ -172,6 +203,11 @@ int sgr_ctx_create(int device, sgr_ctx **out) {
     (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(SGR_ERR_NOMEM, "sgr_ctx_create: stats buffer");
+  }
+  // The no-alloc entry point reports the failure if this did not work.
+  if (scratch_address(device, &c->noalloc_scratch) != hipSuccess) {
+    c->noalloc_scratch = nullptr;
+    (void)hipGetLastError();
   }
   *out = c;
   return SGR_OK;
@@ -890,8 +926,11 @@ int ensure_work(sgr_ctx *ctx, size_t bytes) {
 namespace {
 
 // Runs the chunks; per chunk, optional host copy of the per-sample output.
+// WORK_EXT: path buffers supplied by the caller (one chunk lane, no
+// per-launch timing: nothing here allocates device memory or events).
 int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_host,
-               sgr_stats *stats, sgr_guide *guide = nullptr, bool train = false) {
+               sgr_stats *stats, sgr_guide *guide = nullptr, bool train = false,
+               void *work_ext = nullptr) {
   sgr_ctx *ctx = sc->ctx;
   RenderView R = pl.R;
   size_t cap = (size_t)pl.npix * pl.sb_max;
@@ -903,19 +942,23 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   // the same weights, and the guide trains on both after the pair.
   const uint32_t nchunks = (R.spp + pl.sb_max - 1) / pl.sb_max;
   const int max_lanes = guide ? std::min(SGR_CHUNK_LANES, 2) : SGR_CHUNK_LANES;
-  int lanes = (samples_host || (pl.flags & SGR_RENDER_ONE_CHUNK_IN_FLIGHT))
+  int lanes = (samples_host || work_ext || (pl.flags & SGR_RENDER_ONE_CHUNK_IN_FLIGHT))
                   ? 1 : (int)std::min<uint32_t>(nchunks, (uint32_t)max_lanes);
   size_t need = (path_bytes(cap, R.K, nullptr, nullptr) + 255) & ~(size_t)255;
-  int rc = ensure_work(ctx, need * lanes);
-  if (rc && lanes > 1) {  // not enough device memory for a second lane: one
-    lanes = 1;
-    rc = ensure_work(ctx, need);
+  int rc = SGR_OK;
+  if (!work_ext) {
+    rc = ensure_work(ctx, need * lanes);
+    if (rc && lanes > 1) {  // not enough device memory for a second lane: one
+      lanes = 1;
+      rc = ensure_work(ctx, need);
+    }
   }
   if (rc) return rc;
+  char *work = work_ext ? (char *)work_ext : (char *)ctx->work;
   PathBufs PL[SGR_CHUNK_LANES];
   hipStream_t lane_st[SGR_CHUNK_LANES];
   for (int l = 0; l < lanes; l++) {
-    path_bytes(cap, R.K, &PL[l], (char *)ctx->work + need * l);
+    path_bytes(cap, R.K, &PL[l], work + need * l);
     lane_st[l] = l ? ctx->lane_stream[l - 1] : ctx->stream;
   }
   GuideView GV[2];
@@ -933,7 +976,7 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   // host waits, read after the final synchronisation.
   enum { K_EXT, K_SHADE, K_OCC, K_SPLAT, K_OTHER, K_N };
   double kms[K_N] = {0, 0, 0, 0, 0};
-  const bool kt = stats && (pl.flags & SGR_RENDER_KERNEL_TIMING);
+  const bool kt = stats && !work_ext && (pl.flags & SGR_RENDER_KERNEL_TIMING);
   std::vector<std::pair<int, size_t>> marks;
   size_t next_ev = 0;
   auto ev_get = [&]() -> hipEvent_t {
@@ -1181,6 +1224,40 @@ int sgr_render_samples(sgr_scene *sc, const sgr_camera *cam, const sgr_render_pa
   if (!out) return fail(SGR_ERR_INVALID, "sgr_render_samples: null output");
   HIPCHK(hipSetDevice(sc->ctx->device));
   return sgr::guarded("sgr_render_samples", [&] { return run_render(sc, pl, nullptr, out, stats); });
+}
+
+// sgr_render into device accumulators with the path state in the
+// device's scratch array: chunks of as many samples per pattern pixel as
+// the scratch holds (at most the plan's own chunk), one chunk in flight.
+static int render_noalloc_impl(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p,
+                               float *accum_dev, sgr_stats *stats) {
+  if (!accum_dev) return fail(SGR_ERR_INVALID, "sgr_render_noalloc: null accumulators");
+  RenderPlan pl;
+  int rc = plan_render(sc, cam, p, pl);
+  if (rc) return rc;
+  sgr_ctx *ctx = sc->ctx;
+  if (!ctx->noalloc_scratch)
+    return fail(SGR_ERR_HIP, "sgr_render_noalloc: the scratch array's device address is unknown");
+  HIPCHK(hipSetDevice(ctx->device));
+  const size_t K = pl.R.K;
+  uint32_t sb = pl.sb_max;
+  while (sb > 0 && path_bytes((size_t)pl.npix * sb, K, nullptr, nullptr) > SGR_NOALLOC_SCRATCH_BYTES)
+    sb = sb > 8 ? sb - sb / 8 : sb - 1;  // shrink by eighths, then one at a time
+  if (sb == 0)
+    return fail(SGR_ERR_LIMIT,
+                "sgr_render_noalloc: " + std::to_string(pl.npix) +
+                    " pattern pixels (window plus filter margin) need " +
+                    std::to_string(path_bytes(pl.npix, K, nullptr, nullptr)) +
+                    " bytes of path state for one sample each; the scratch holds " +
+                    std::to_string((unsigned long long)SGR_NOALLOC_SCRATCH_BYTES));
+  pl.sb_max = sb;
+  return run_render(sc, pl, accum_dev, nullptr, stats, nullptr, false, ctx->noalloc_scratch);
+}
+
+int sgr_render_noalloc(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p,
+                       float *accum_dev, sgr_stats *stats) {
+  return sgr::guarded("sgr_render_noalloc",
+                      [&] { return render_noalloc_impl(sc, cam, p, accum_dev, stats); });
 }
 
 // ImageSampledOutput::set_raw_min_y (image/image-sampled-output.cc:63-126)

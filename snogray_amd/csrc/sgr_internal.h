@@ -65,6 +65,9 @@ struct sgr_ctx {
   std::vector<hipEvent_t> syncpool;  // ordering events between the chunk lanes
   std::atomic<uint32_t> live_scenes{0};
   sgr::Engine *engine = nullptr;   // multi-job engine (sgr_jobs.cpp)
+  // Device address of this device's copy of the no-alloc render scratch
+  // (sgr_render_noalloc), or null when it could not be resolved.
+  void *noalloc_scratch = nullptr;
 };
 
 struct sgr_scene {

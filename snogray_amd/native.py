@@ -20,7 +20,7 @@ EXPORTS = [
     "sgr_scene_create", "sgr_scene_destroy", "sgr_scene_octree_stats",
     "sgr_octree_export", "sgr_octree_build", "sgr_scene_bounds", "sgr_scene_footprint",
     "sgr_trace_closest", "sgr_trace_occluded", "sgr_render",
-    "sgr_render_samples", "sgr_finalize_image",
+    "sgr_render_samples", "sgr_finalize_image", "sgr_render_noalloc",
     "sgr_engine_configure", "sgr_job_footprint", "sgr_job_submit", "sgr_engine_step",
     "sgr_job_poll", "sgr_job_result", "sgr_job_release", "sgr_engine_stats",
     "sgr_render_batch",
@@ -68,6 +68,8 @@ def lib():
     L.sgr_render_samples.argtypes = [vp, C.POINTER(abi.Camera),
                                      C.POINTER(abi.RenderParams), abi.f32p,
                                      C.POINTER(abi.Stats)]
+    L.sgr_render_noalloc.argtypes = [vp, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams),
+                                     abi.f32p, C.POINTER(abi.Stats)]
     L.sgr_finalize_image.argtypes = [abi.f32p, C.c_uint32, C.c_uint32, C.c_float,
                                      C.c_float, abi.f32p]
     L.sgr_engine_configure.argtypes = [vp, C.POINTER(abi.EngineConfig)]

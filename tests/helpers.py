@@ -49,3 +49,42 @@ def small_soup(level=4, n_spheres=2, seed=7):
     """Reduced icosphere soup (same generator as the full workload)."""
     return scenes.icosphere_soup(n_spheres=n_spheres, level=level, seed=seed,
                                  env_size=(128, 64))
+
+
+class DeviceMem:
+    """Device memory through the HIP runtime's C API (ctypes), for tests
+    that hand device pointers to the C ABI."""
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            import ctypes as C
+            h = C.CDLL("libamdhip64.so")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes):
+        import ctypes as C
+        self.nbytes = nbytes
+        p = C.c_void_p()
+        assert self.hip().hipMalloc(C.byref(p), nbytes) == 0, "hipMalloc"
+        self.ptr = p.value
+        assert self.hip().hipMemset(self.ptr, 0, nbytes) == 0, "hipMemset"
+
+    def to_host(self, shape, dtype=np.float32):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        assert self.hip().hipDeviceSynchronize() == 0
+        assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0, "hipMemcpy"
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None
