@@ -59,6 +59,9 @@ def lib():
         L.orc_envmap_probe.argtypes = [abi.f32p, C.c_uint32, C.c_uint32,
                                        C.POINTER(C.c_double), C.c_int,
                                        C.POINTER(C.c_double)]
+        L.orc_envmap_lookup.argtypes = [abi.f32p, C.c_uint32, C.c_uint32,
+                                        C.POINTER(C.c_double), C.c_int,
+                                        C.POINTER(C.c_double)]
         L.orc_stream_probe.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_uint32,
                                        C.c_uint32, abi.f32p, C.c_uint32,
                                        C.c_uint32, abi.f32p]
@@ -175,6 +178,15 @@ def envmap_probe(texels, params):
     out = np.zeros((len(p), 10))
     lib().orc_envmap_probe(abi.fptr(t), t.shape[1], t.shape[0], dptr(p), len(p),
                            dptr(out))
+    return out
+
+
+def envmap_lookup(texels, uv):
+    """Bilinear lookup of an (H, W, 3) map (row 0 = top) at (u, v) pairs."""
+    t = np.ascontiguousarray(texels, np.float32)
+    p = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+    out = np.zeros((len(p), 3))
+    lib().orc_envmap_lookup(abi.fptr(t), t.shape[1], t.shape[0], dptr(p), len(p), dptr(out))
     return out
 
 

@@ -588,6 +588,24 @@ int orc_envmap_probe(const float *texels, uint32_t w, uint32_t h,
   return 0;
 }
 
+// Envmap lookup probe: the bilinear map value at n (u, v) positions of
+// the lat-long parameter square (texture/matrix-linterp.h:30-70).
+// out: r, g, b (3 per position).
+int orc_envmap_lookup(const float *texels, uint32_t w, uint32_t h, const double *uv, int n,
+                      double *out) {
+  Envmap<float> E;
+  E.w = w;
+  E.h = h;
+  E.tex.assign(texels, texels + (size_t)3 * w * h);
+  for (int i = 0; i < n; i++) {
+    C3<float> c = E.lookup((float)uv[2 * i], (float)uv[2 * i + 1]);
+    out[3 * i] = c.r;
+    out[3 * i + 1] = c.g;
+    out[3 * i + 2] = c.b;
+  }
+  return 0;
+}
+
 // Sample-stream probe: stratified UV values of channel CHAN for all SPP
 // samples of pixel (px, py), and NR plain draws of path S.
 int orc_stream_probe(uint64_t seed, int px, int py, uint32_t chan, uint32_t spp,
