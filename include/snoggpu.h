@@ -235,6 +235,10 @@ typedef struct sgr_stats {
      * device keeps 8 media per path (the reference's stack is unbounded,
      * material/media.cc:75-118) and does not push these */
     uint64_t media_overflows;
+    /* octree node records loaded (node_visits_* also count the children
+     * the reference calls and prunes at once, which load nothing) */
+    uint64_t node_fetches_closest;
+    uint64_t node_fetches_shadow;
 } sgr_stats;
 
 /* Octree statistics (space/octree.h:53-67). */

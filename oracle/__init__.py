@@ -14,20 +14,22 @@ from snogray_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+# the shipped-flags build with the negative mailbox: CPU-baseline timing only
+LIB_FAST = os.path.join(HERE, "liboracle_fast.so")
 
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(fast=False):
+    if fast not in _libs:
+        path = LIB_FAST if fast else LIB
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         L.orc_scene_create.restype = C.c_void_p
         L.orc_scene_create.argtypes = [C.POINTER(abi.SceneDescC), C.c_int,
                                        C.c_char_p, C.c_int]
@@ -67,8 +69,8 @@ def lib():
                                        C.c_uint32, abi.f32p]
         L.orc_permute.restype = C.c_uint32
         L.orc_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
-        _lib = L
-    return _lib
+        _libs[fast] = L
+    return _libs[fast]
 
 
 def dptr(a):
@@ -77,53 +79,56 @@ def dptr(a):
 
 class OracleScene:
     """CPU-oracle scene (fp32 reproduces the reference arithmetic, fp64 is
-    the high-precision oracle)."""
+    the high-precision oracle).  fast: the shipped-flags build with the
+    negative mailbox, for timing only (same images up to fast-math
+    rounding; surface-test counts then exclude the mailbox's skips)."""
 
-    def __init__(self, desc, fp64=False):
+    def __init__(self, desc, fp64=False, fast=False):
         err = C.create_string_buffer(512)
         self.desc = desc
-        self.h = lib().orc_scene_create(C.byref(desc.c()), 1 if fp64 else 0,
-                                        err, 512)
+        self.L = lib(fast)
+        self.h = self.L.orc_scene_create(C.byref(desc.c()), 1 if fp64 else 0,
+                                         err, 512)
         if not self.h:
             raise RuntimeError("oracle scene: " + err.value.decode())
         self.fp64 = fp64
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_scene_destroy(self.h)
+            self.L.orc_scene_destroy(self.h)
             self.h = None
 
     def octree(self):
         nn, ns = C.c_uint64(), C.c_uint64()
-        lib().orc_octree_export(self.h, None, C.byref(nn), None, C.byref(ns), None)
+        self.L.orc_octree_export(self.h, None, C.byref(nn), None, C.byref(ns), None)
         nodes = np.zeros((nn.value, 4), np.uint32)
         surfs = np.zeros(ns.value, np.uint32)
         osz = np.zeros(4, np.float64)
-        lib().orc_octree_export(self.h, abi.uptr(nodes), C.byref(nn),
+        self.L.orc_octree_export(self.h, abi.uptr(nodes), C.byref(nn),
                                 abi.uptr(surfs), C.byref(ns), dptr(osz))
         return nodes, surfs, osz
 
     def octree_stats(self):
         st = abi.OctreeStats()
-        lib().orc_octree_stats(self.h, C.byref(st))
+        self.L.orc_octree_stats(self.h, C.byref(st))
         return {k: getattr(st, k) for k, _ in st._fields_}
 
     def bounds(self):
         a, b, h = np.zeros(3), np.zeros(3), C.c_double()
-        lib().orc_scene_bounds(self.h, dptr(a), dptr(b), C.byref(h))
+        self.L.orc_scene_bounds(self.h, dptr(a), dptr(b), C.byref(h))
         return a, b, h.value
 
     def trace_closest(self, batch, nthreads=8):
         hb = abi.HitBuffer(batch.n)
         cnt = (C.c_uint64 * 2)()
-        lib().orc_trace(self.h, C.byref(batch.c), batch.n, C.byref(hb.c), None,
+        self.L.orc_trace(self.h, C.byref(batch.c), batch.n, C.byref(hb.c), None,
                         nthreads, cnt)
         return hb, (cnt[0], cnt[1])
 
     def trace_occluded(self, batch, nthreads=8):
         occ = np.zeros(batch.n, np.uint8)
         cnt = (C.c_uint64 * 2)()
-        lib().orc_trace(self.h, C.byref(batch.c), batch.n, None,
+        self.L.orc_trace(self.h, C.byref(batch.c), batch.n, None,
                         occ.ctypes.data_as(C.POINTER(C.c_uint8)), nthreads, cnt)
         return occ, (cnt[0], cnt[1])
 
@@ -135,7 +140,7 @@ class OracleScene:
             out = np.zeros(((h + 2 * m) * (w + 2 * m) * params.spp, 6), np.float32)
         st = abi.Stats()
         err = C.create_string_buffer(512)
-        rc = lib().orc_render(self.h, C.byref(cam), C.byref(params), dptr(acc),
+        rc = self.L.orc_render(self.h, C.byref(cam), C.byref(params), dptr(acc),
                               abi.fptr(out) if out is not None else None,
                               nthreads, C.byref(st), err, 512)
         if rc:

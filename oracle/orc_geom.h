@@ -406,6 +406,21 @@ template<class R> struct Prim {
   R radius;
 };
 
+#ifdef ORC_MAILBOX
+// Negative mailbox of one search (space/isec-cache.h:37-114, used at
+// space/octree.cc:195-216): a surface that was tested and missed is not
+// tested again in the same search, which cannot change the result (the
+// ray extent only shrinks).  1024 direct-mapped entries keyed by the
+// primitive index, cleared per search by a generation count.  Built only
+// into the shipped-flags CPU baseline, whose surface-test counts then
+// count the tests made, as the reference's do.
+struct Mailbox {
+  unsigned gen = 1;
+  struct Entry { unsigned gen, id; } e[1024] = {};
+};
+static thread_local Mailbox tl_mailbox;
+#endif
+
 // Closest-hit / any-hit search over either the octree or a brute-force
 // list.  The callback semantics follow space/space.cc:36-194.
 template<class R> struct Accel {
@@ -443,6 +458,10 @@ template<class R> struct Accel {
 
     bool cb(unsigned p) {  // space/space.cc:42-53 / :103-116
       R t, u, v;
+#ifdef ORC_MAILBOX
+      Mailbox::Entry &me = tl_mailbox.e[p & 1023u];
+      if (me.gen == tl_mailbox.gen && me.id == p) return false;
+#endif
       surf_tests++;
       if (acc->test(p, *ray, t, u, v)) {
         if (any) { stop = true; hit = p; return true; }
@@ -452,6 +471,10 @@ template<class R> struct Accel {
         hv = v;
         return true;
       }
+#ifdef ORC_MAILBOX
+      me.gen = tl_mailbox.gen;
+      me.id = p;
+#endif
       return false;
     }
 
@@ -494,6 +517,12 @@ template<class R> struct Accel {
   // TrivSpace (space/triv-space.h:48-58).
   void search(Search &ss) const {
     Ray<R> &ray = *ss.ray;
+#ifdef ORC_MAILBOX
+    if (++tl_mailbox.gen == 0) {  // generation wrapped: clear the table
+      for (auto &e : tl_mailbox.e) e.gen = 0;
+      tl_mailbox.gen = 1;
+    }
+#endif
     if (use_triv) {
       for (unsigned p : triv_order) {
         ss.cb(p);

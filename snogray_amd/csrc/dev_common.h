@@ -12,7 +12,7 @@
 namespace sgr {
 
 #define SGR_REF_STRIDE 3  // float4s per surface reference on the device
-#define SGR_MAX_DEPTH 64  // octree levels a traversal can hold (checked at build)
+#define SGR_MAX_DEPTH 42  // octree levels a traversal can hold (checked at scene creation)
 // uint32 words between the fetch heads of the lane-refill walk: each head
 // has a 128-byte line to itself, so the atomics of different XCD slices
 // do not queue on one line.

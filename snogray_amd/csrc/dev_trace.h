@@ -94,8 +94,10 @@ __device__ __forceinline__ bool ref_hit(const float4 *refs, uint32_t k, const Ra
   return false;
 }
 
+// nodes: node calls as the reference counts them (pruned children
+// included); tests: primitive tests; fetch: node records loaded.
 struct TraceCount {
-  uint32_t nodes, tests;
+  uint32_t nodes, tests, fetch;
 };
 
 // Child occupancy of a node in visiting order: bit j is set when the
@@ -126,17 +128,32 @@ __device__ __forceinline__ uint32_t visit_mask(uint32_t mask, uint32_t oct) {
 // calls, as the reference counts them.  ANY: stop at the first hit
 // (space/space.cc:103-116 / :160-180).
 //
-// Traversal stack: the SGR_LDS_SLOTS newest entries of each lane live in
-// LDS (slot i of the lane at lds[(2 * i + half) * SGR_TRACE_BLOCK]);
-// older ones are spilled to the lane's private array when a newer one
-// needs their slot and read back from there when popped.  Entries are
-// 2 x float4: (first child, child mask | pending siblings << 8, x slab)
-// and (y slab, z slab); the pending siblings are a visit-order mask.
+// Traversal stack.  An entry is 8 bytes: the node's first child, its
+// child mask, the siblings still to visit (a visit-order mask) and the
+// node's depth.  Its parametric bounds are not stored: the bounds of a
+// node follow from the root's by halving along the path to it, so a pop
+// recomputes them from the root slabs and the lane's trail (3 bits per
+// level: the child taken at each level of the current path, whose
+// prefixes are the paths to every stack entry), with the same operations
+// in the same order as the descent, hence the same bits.  The newest
+// pushed entry's bounds are also kept (6 floats in LDS): a pop straight
+// after a push, the common case, needs no recomputation.  The
+// SGR_LDS_SLOTS newest entries of each lane live in LDS; older ones are
+// spilled to the lane's private array when a newer one needs their slot
+// and read back from there when popped.  LDS per 256-lane block: 9 x 8 B
+// entries + 24 B of bounds per lane, 24 KB.
 #define SGR_TRACE_BLOCK 256
-#define SGR_LDS_SLOTS 3
+#define SGR_LDS_SLOTS 9
+#define SGR_TRAIL_LO 21  // levels in the first 64-bit trail word
 struct LdsStack {
-  float4 *base;  // &shared[threadIdx.x] of a [2 * SGR_LDS_SLOTS][SGR_TRACE_BLOCK] array
+  uint2 *ent;    // &ent_lds[threadIdx.x] of a [SGR_LDS_SLOTS][SGR_TRACE_BLOCK] array
+  float *bnd;    // &bnd_lds[threadIdx.x] of a [6][SGR_TRACE_BLOCK] array
 };
+#define SGR_STACK_LDS(name)                                             \
+  __shared__ uint2 name##_ent[SGR_LDS_SLOTS * SGR_TRACE_BLOCK];         \
+  __shared__ float name##_bnd[6 * SGR_TRACE_BLOCK];                     \
+  const LdsStack name{&name##_ent[threadIdx.x], &name##_bnd[threadIdx.x]}
+static_assert(SGR_MAX_DEPTH <= 2 * SGR_TRAIL_LO, "the trail holds 2 x 21 levels");
 
 // ---------------------------------------------------------------------------
 // The walk is a resumable state machine, for lane-refill traversal:
@@ -150,11 +167,44 @@ struct TravState {
   RayF ray;
   uint32_t oct, cn, first, mask, vis;
   float bx0, bx1, by0, by1, bz0, bz1;
-  int sp, nl;
-  bool enter;
+  uint64_t tr0, tr1;  // trail: child taken at levels 0-20, 21-41
+  int sp, nl, dep;    // stack size, entries in LDS, depth of the current node
+  bool enter, top_ok; // top_ok: the LDS bounds are the top entry's
   uint32_t hit;
   float hu, hv;
 };
+
+// Root slabs of the ray (space/octree.cc:101-162): parametric bounds of
+// the root cube per axis, ordered.
+__device__ __forceinline__ void root_slabs(const SceneView &S, const RayF &ray, float &bx0,
+                                           float &bx1, float &by0, float &by1, float &bz0,
+                                           float &bz1) {
+  float ix = ray.d.x == 0.0f ? 1e9f : 1.0f / ray.d.x;
+  bx0 = (S.origin[0] - ray.o.x) * ix;
+  bx1 = (S.origin[0] - ray.o.x + S.size) * ix;
+  if (bx0 > bx1) { float q = bx0; bx0 = bx1; bx1 = q; }
+  float iy = ray.d.y == 0.0f ? 1e9f : 1.0f / ray.d.y;
+  by0 = (S.origin[1] - ray.o.y) * iy;
+  by1 = (S.origin[1] - ray.o.y + S.size) * iy;
+  if (by0 > by1) { float q = by0; by0 = by1; by1 = q; }
+  float iz = ray.d.z == 0.0f ? 1e9f : 1.0f / ray.d.z;
+  bz0 = (S.origin[2] - ray.o.z) * iz;
+  bz1 = (S.origin[2] - ray.o.z + S.size) * iz;
+  if (bz0 > bz1) { float q = bz0; bz0 = bz1; bz1 = q; }
+}
+
+// Bounds of child CHILD (parametric index, before the octant flip) of a
+// node with bounds b: the lower or upper half per axis.
+__device__ __forceinline__ void child_bounds(uint32_t child, float &bx0, float &bx1, float &by0,
+                                             float &by1, float &bz0, float &bz1) {
+  const float xh = (bx1 - bx0) / 2.0f, yh = (by1 - by0) / 2.0f, zh = (bz1 - bz0) / 2.0f;
+  const float cx = bx0 + ((child & 4) ? xh : 0.0f);
+  const float cy = by0 + ((child & 2) ? yh : 0.0f);
+  const float cz = bz0 + ((child & 1) ? zh : 0.0f);
+  bx0 = cx; bx1 = cx + xh;
+  by0 = cy; by1 = cy + yh;
+  bz0 = cz; bz1 = cz + zh;
+}
 
 __device__ __forceinline__ bool trav_init(const SceneView &S, TravState &T, TraceCount &cnt) {
   T.hit = SGR_MISS;
@@ -163,22 +213,19 @@ __device__ __forceinline__ bool trav_init(const SceneView &S, TravState &T, Trac
   if (S.n_nodes == 0) return false;
   const RayF &ray = T.ray;
   T.oct = (ray.d.x >= 0.0f ? 0u : 4u) | (ray.d.y >= 0.0f ? 0u : 2u) | (ray.d.z >= 0.0f ? 0u : 1u);
-  float ix = ray.d.x == 0.0f ? 1e9f : 1.0f / ray.d.x;
-  float bx0 = (S.origin[0] - ray.o.x) * ix, bx1 = (S.origin[0] - ray.o.x + S.size) * ix;
-  if (bx0 > bx1) { float q = bx0; bx0 = bx1; bx1 = q; }
-  float iy = ray.d.y == 0.0f ? 1e9f : 1.0f / ray.d.y;
-  float by0 = (S.origin[1] - ray.o.y) * iy, by1 = (S.origin[1] - ray.o.y + S.size) * iy;
-  if (by0 > by1) { float q = by0; by0 = by1; by1 = q; }
-  float iz = ray.d.z == 0.0f ? 1e9f : 1.0f / ray.d.z;
-  float bz0 = (S.origin[2] - ray.o.z) * iz, bz1 = (S.origin[2] - ray.o.z + S.size) * iz;
-  if (bz0 > bz1) { float q = bz0; bz0 = bz1; bz1 = q; }
+  float bx0, bx1, by0, by1, bz0, bz1;
+  root_slabs(S, ray, bx0, bx1, by0, by1, bz0, bz1);
   cnt.nodes++;
   if (fmx(ray.t0, fmx(bx0, fmx(by0, bz0))) >= fmn(ray.t1, fmn(bx1, fmn(by1, bz1)))) return false;
   T.bx0 = bx0; T.bx1 = bx1; T.by0 = by0; T.by1 = by1; T.bz0 = bz0; T.bz1 = bz1;
   T.sp = 0;
   T.nl = 0;
+  T.dep = 0;
+  T.tr0 = 0;
+  T.tr1 = 0;
   T.cn = 0;
   T.enter = true;
+  T.top_ok = false;
   return true;
 }
 
@@ -221,12 +268,27 @@ __device__ __forceinline__ void child_masks(const TravState &T, uint32_t &c1, ui
   ok = swap34(m);
 }
 
+// Bounds of the node at depth DEP of the current path: the root slabs
+// halved along the trail's first DEP levels.
+__device__ __forceinline__ void replay_bounds(const SceneView &S, TravState &T, int dep) {
+  float bx0, bx1, by0, by1, bz0, bz1;
+  root_slabs(S, T.ray, bx0, bx1, by0, by1, bz0, bz1);
+  uint64_t t = T.tr0;
+  for (int l = 0; l < dep; l++) {
+    if (l == SGR_TRAIL_LO) t = T.tr1;
+    child_bounds((uint32_t)t & 7u, bx0, bx1, by0, by1, bz0, bz1);
+    t >>= 3;
+  }
+  T.bx0 = bx0; T.bx1 = bx1; T.by0 = by0; T.by1 = by1; T.bz0 = bz0; T.bz1 = bz1;
+}
+
 template<bool ANY>
 __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, TraceCount &cnt,
-                                          LdsStack L, float4 *st0, float4 *st1) {
+                                          LdsStack L, uint2 *spill) {
   if (T.enter) {
     float t, u, v;
     uint32_t prim;
+    cnt.fetch++;
     const uint4 nd = S.nodes[T.cn];
     for (uint32_t k = nd.z, e = nd.z + nd.w; k < e; k++) {
       cnt.tests++;
@@ -253,49 +315,57 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
   if (go) {
     const uint32_t i = (uint32_t)__builtin_ctz(go), later = go & (go - 1u);
     if (later) {
-      float4 *slot = L.base + (size_t)(2 * (T.sp % SGR_LDS_SLOTS)) * SGR_TRACE_BLOCK;
+      uint2 *slot = L.ent + (size_t)(T.sp % SGR_LDS_SLOTS) * SGR_TRACE_BLOCK;
       if (T.nl == SGR_LDS_SLOTS) {  // the slot holds entry sp - SLOTS: spill it
-        st0[T.sp - SGR_LDS_SLOTS] = slot[0];
-        st1[T.sp - SGR_LDS_SLOTS] = slot[SGR_TRACE_BLOCK];
+        spill[T.sp - SGR_LDS_SLOTS] = *slot;
       } else {
         T.nl++;
       }
-      slot[0] = make_float4(__uint_as_float(T.first), __uint_as_float(T.mask | (later << 8)),
-                            T.bx0, T.bx1);
-      slot[SGR_TRACE_BLOCK] = make_float4(T.by0, T.by1, T.bz0, T.bz1);
+      *slot = make_uint2(T.first, T.mask | (later << 8) | ((uint32_t)T.dep << 16));
+      L.bnd[0] = T.bx0; L.bnd[SGR_TRACE_BLOCK] = T.bx1;
+      L.bnd[2 * SGR_TRACE_BLOCK] = T.by0; L.bnd[3 * SGR_TRACE_BLOCK] = T.by1;
+      L.bnd[4 * SGR_TRACE_BLOCK] = T.bz0; L.bnd[5 * SGR_TRACE_BLOCK] = T.bz1;
+      T.top_ok = true;
       T.sp++;
     }
     const uint32_t child = (i == 3) ? 4u : (i == 4) ? 3u : i;
     const uint32_t real = child ^ T.oct;
-    const float xh = (T.bx1 - T.bx0) / 2.0f, yh = (T.by1 - T.by0) / 2.0f,
-                zh = (T.bz1 - T.bz0) / 2.0f;
-    const float cx = T.bx0 + ((child & 4) ? xh : 0.0f);
-    const float cy = T.by0 + ((child & 2) ? yh : 0.0f);
-    const float cz = T.bz0 + ((child & 1) ? zh : 0.0f);
     T.cn = T.first + __popc(T.mask & ((1u << real) - 1u));
-    T.bx0 = cx; T.bx1 = cx + xh;
-    T.by0 = cy; T.by1 = cy + yh;
-    T.bz0 = cz; T.bz1 = cz + zh;
+    child_bounds(child, T.bx0, T.bx1, T.by0, T.by1, T.bz0, T.bz1);
+    // the trail's entry for this level (stale bits of an earlier path
+    // cleared)
+    if (T.dep < SGR_TRAIL_LO) {
+      const int sh = 3 * T.dep;
+      T.tr0 = (T.tr0 & ~(7ull << sh)) | ((uint64_t)child << sh);
+    } else {
+      const int sh = 3 * (T.dep - SGR_TRAIL_LO);
+      T.tr1 = (T.tr1 & ~(7ull << sh)) | ((uint64_t)child << sh);
+    }
+    T.dep++;
     T.enter = true;
     return false;
   }
   if (T.sp == 0) return true;
   T.sp--;
-  float4 a, b;
+  uint2 a;
   if (T.nl > 0) {
-    const float4 *slot = L.base + (size_t)(2 * (T.sp % SGR_LDS_SLOTS)) * SGR_TRACE_BLOCK;
-    a = slot[0];
-    b = slot[SGR_TRACE_BLOCK];
+    a = L.ent[(size_t)(T.sp % SGR_LDS_SLOTS) * SGR_TRACE_BLOCK];
     T.nl--;
   } else {
-    a = st0[T.sp];
-    b = st1[T.sp];
+    a = spill[T.sp];
   }
-  const uint32_t w1 = __float_as_uint(a.y);
-  T.first = __float_as_uint(a.x);
-  T.mask = w1 & 0xffu;
-  T.vis = (w1 >> 8) & 0xffu;
-  T.bx0 = a.z; T.bx1 = a.w; T.by0 = b.x; T.by1 = b.y; T.bz0 = b.z; T.bz1 = b.w;
+  T.first = a.x;
+  T.mask = a.y & 0xffu;
+  T.vis = (a.y >> 8) & 0xffu;
+  T.dep = (int)(a.y >> 16);
+  if (T.top_ok) {  // the entry pushed last: its bounds are in LDS
+    T.bx0 = L.bnd[0]; T.bx1 = L.bnd[SGR_TRACE_BLOCK];
+    T.by0 = L.bnd[2 * SGR_TRACE_BLOCK]; T.by1 = L.bnd[3 * SGR_TRACE_BLOCK];
+    T.bz0 = L.bnd[4 * SGR_TRACE_BLOCK]; T.bz1 = L.bnd[5 * SGR_TRACE_BLOCK];
+  } else {
+    replay_bounds(S, T, T.dep);
+  }
+  T.top_ok = false;
   T.enter = false;
   return false;
 }
@@ -304,7 +374,7 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
 // (TrivSpace, space/triv-space.h:48-58) or the octree walk.
 template<bool ANY>
 __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, TraceCount &cnt,
-                                          LdsStack L, float4 *st0, float4 *st1) {
+                                          LdsStack L, uint2 *spill) {
   if (S.use_triv) {
     T.hit = SGR_MISS;
     T.hu = 0.0f;
@@ -322,7 +392,7 @@ __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, Trac
       }
     }
   } else if (trav_init(S, T, cnt)) {
-    while (!trav_step<ANY>(S, T, cnt, L, st0, st1)) {
+    while (!trav_step<ANY>(S, T, cnt, L, spill)) {
     }
   }
 }
@@ -337,10 +407,10 @@ template<bool ANY, bool MULTI, typename Get, typename Put>
 __device__ __forceinline__ void trace_plain(const SceneView &S1, uint32_t jb, uint32_t je,
                                             Get get, Put put, TraceCount &cnt, LdsStack L) {
   TravState T;
-  float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
+  uint2 spill[SGR_MAX_DEPTH];
   for (uint32_t i = jb + threadIdx.x; i < je; i += blockDim.x) {
     const SceneView *g = get(i, T.ray);
-    trace_one<ANY>(MULTI ? *g : S1, T, cnt, L, st0, st1);
+    trace_one<ANY>(MULTI ? *g : S1, T, cnt, L, spill);
     put(i, T);
   }
 }
@@ -369,7 +439,7 @@ __device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t n, ui
   const uint32_t need = ((n + blockDim.x - 1u) / blockDim.x + 7u) & ~7u;
   if (blockIdx.x >= (need > 8u ? need : 8u)) return;
   TravState T;
-  float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
+  uint2 spill[SGR_MAX_DEPTH];
   bool active = false;
   uint32_t id = 0, tried = 0, slice = blockIdx.x & 7u;
   const SceneView *Sp = &S1;
@@ -422,7 +492,7 @@ __device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t n, ui
         }
       }
     }
-    if (active && trav_step<ANY>(MULTI ? *Sp : S1, T, cnt, L, st0, st1)) {
+    if (active && trav_step<ANY>(MULTI ? *Sp : S1, T, cnt, L, spill)) {
       put(id, T);
       active = false;
     }

@@ -47,7 +47,9 @@ static_assert(SGR_CNT_HE >= 2 * SGR_MAX_BOUNCES + 4 && SGR_CNT_HE % SGR_HEAD_STR
 #define SGR_STAT_ISECT   5
 #define SGR_STAT_SHADOW  6
 #define SGR_STAT_MEDIA_OVF 7  // media pushes dropped at SGR_MEDIA_MAX
-#define SGR_NUM_STATS    8
+#define SGR_STAT_FETCH_C 8    // octree node records loaded (closest hit)
+#define SGR_STAT_FETCH_S 9    // ... (any hit)
+#define SGR_NUM_STATS    16
 // The statistics buffer also holds the 8 fetch heads of a ray-batch query,
 // from byte SGR_STATS_HEADS_OFF on, SGR_HEAD_STRIDE words apart.
 #define SGR_STATS_HEADS_OFF 128
@@ -94,7 +96,7 @@ struct StepView {
 
 struct PathBufs {
   float *ray[8];      // ox oy oz dx dy dz t0 t1
-  uint32_t *hprim;
+  uint32_t *hprim;    // closest hit of queue position j of the current bounce
   float *ht, *hu, *hv;
   float *T[3], *rad[3], *Tp[3];
   float *rr, *alpha;

@@ -422,11 +422,11 @@ k_guide_inputs(GuideView G, const uint32_t *queue, const uint32_t *count, RayPtr
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     const uint32_t p = queue[j];
     float *o = G.in6 + (size_t)j * GD_IN;
-    if (hprim[p] == 0xffffffffu) {  // miss
+    if (hprim[j] == 0xffffffffu) {  // miss (hits are stored by queue position)
       for (int k = 0; k < GD_IN; k++) o[k] = 0.0f;
       continue;
     }
-    const float t = ht[p];
+    const float t = ht[j];
     const float dx = R.r[3][p], dy = R.r[4][p], dz = R.r[5][p];
     const float x = R.r[0][p] + dx * t, y = R.r[1][p] + dy * t, z = R.r[2][p] + dz * t;
     const float il = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);

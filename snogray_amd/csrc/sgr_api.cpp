@@ -759,12 +759,14 @@ static int trace_common(sgr_scene *sc, const sgr_rays *rays, size_t n, sgr_hits 
       stats->scene_intersect_calls = n;
       stats->node_visits_closest = st[0];
       stats->surf_tests_closest = st[1];
+      stats->node_fetches_closest = st[SGR_STAT_FETCH_C];
       stats->ms_extend = ms;
       stats->extend_launches = 1;
     } else {
       stats->shadow_tests = n;
       stats->node_visits_shadow = st[0];
       stats->surf_tests_shadow = st[1];
+      stats->node_fetches_shadow = st[SGR_STAT_FETCH_C];
       stats->ms_shadow = ms;
       stats->shadow_launches = 1;
     }
@@ -1147,6 +1149,8 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     stats->surf_tests_shadow = st[SGR_STAT_TESTS_S];
     stats->illum_calls = st[SGR_STAT_ILLUM];
     stats->media_overflows = st[SGR_STAT_MEDIA_OVF];
+    stats->node_fetches_closest = st[SGR_STAT_FETCH_C];
+    stats->node_fetches_shadow = st[SGR_STAT_FETCH_S];
     for (auto &m : marks) {
       float e = 0;
       (void)hipEventElapsedTime(&e, ctx->evpool[m.second], ctx->evpool[m.second + 1]);

@@ -568,6 +568,8 @@ int sgr_engine_stats(sgr_ctx *ctx, sgr_stats *stats) {
   stats->surf_tests_shadow = st[SGR_STAT_TESTS_S];
   stats->illum_calls = st[SGR_STAT_ILLUM];
   stats->media_overflows = st[SGR_STAT_MEDIA_OVF];
+  stats->node_fetches_closest = st[SGR_STAT_FETCH_C];
+  stats->node_fetches_shadow = st[SGR_STAT_FETCH_S];
   stats->ms_render = E->ms_device;
   return SGR_OK;
 }
@@ -648,6 +650,8 @@ static int sgr_render_batch_impl(sgr_ctx *ctx, uint32_t n, sgr_scene *const *sce
     stats->surf_tests_shadow -= before.surf_tests_shadow;
     stats->illum_calls -= before.illum_calls;
     stats->media_overflows -= before.media_overflows;
+    stats->node_fetches_closest -= before.node_fetches_closest;
+    stats->node_fetches_shadow -= before.node_fetches_shadow;
     stats->ms_render -= before.ms_render;
   }
   cleanup();

@@ -63,9 +63,8 @@ __device__ __forceinline__ void walk(const SceneView &S1, uint32_t n, uint32_t *
 template<bool ANY, bool REFILL>
 __global__ void __launch_bounds__(256) k_trace(SceneView S, RayBatchDev rb, uint32_t n,
                                                unsigned long long *stats) {
-  __shared__ float4 stk_lds[2 * SGR_LDS_SLOTS * SGR_TRACE_BLOCK];
-  const LdsStack stk{&stk_lds[threadIdx.x]};
-  TraceCount cnt{0, 0};
+  SGR_STACK_LDS(stk);
+  TraceCount cnt{0, 0, 0};
   uint32_t *heads = (uint32_t *)((char *)stats + SGR_STATS_HEADS_OFF);
   walk<ANY, false, REFILL>(
       S, n, heads,
@@ -89,6 +88,7 @@ __global__ void __launch_bounds__(256) k_trace(SceneView S, RayBatchDev rb, uint
       cnt, stk);
   wave_count(&stats[0], cnt.nodes);
   wave_count(&stats[1], cnt.tests);
+  wave_count(&stats[SGR_STAT_FETCH_C], cnt.fetch);
 }
 
 // ---------------------------------------------------------------------------
@@ -186,9 +186,8 @@ __device__ __forceinline__ bool finishing(const PathBufs &P, uint32_t b, uint32_
 template<bool MULTI, bool REFILL>
 __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                 unsigned long long *stats) {
-  __shared__ float4 stk_lds[2 * SGR_LDS_SLOTS * SGR_TRACE_BLOCK];
-  const LdsStack stk{&stk_lds[threadIdx.x]};
-  TraceCount cnt{0, 0};
+  SGR_STACK_LDS(stk);
+  TraceCount cnt{0, 0, 0};
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   if (finishing(P, b, n)) return;
   const uint32_t *q = P.q[b & 1];
@@ -200,23 +199,24 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
         return MULTI ? &J.S[J.R[P.pseg[p]].scene] : nullptr;
       },
       [&](uint32_t i, const TravState &T) {
-        const uint32_t p = q[i];
-        P.hprim[p] = T.hit;
-        P.ht[p] = T.ray.t1;
-        P.hu[p] = T.hu;
-        P.hv[p] = T.hv;
+        // by queue position: the lanes of a wave store neighbouring words
+        // (by path the stores scatter over the sparse late queues)
+        P.hprim[i] = T.hit;
+        P.ht[i] = T.ray.t1;
+        P.hu[i] = T.hu;
+        P.hv[i] = T.hv;
       },
       cnt, stk);
   wave_count(&stats[SGR_STAT_NODES_C], cnt.nodes);
   wave_count(&stats[SGR_STAT_TESTS_C], cnt.tests);
+  wave_count(&stats[SGR_STAT_FETCH_C], cnt.fetch);
 }
 
 template<bool MULTI, bool REFILL>
 __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                  unsigned long long *stats) {
-  __shared__ float4 stk_lds[2 * SGR_LDS_SLOTS * SGR_TRACE_BLOCK];
-  const LdsStack stk{&stk_lds[threadIdx.x]};
-  TraceCount cnt{0, 0};
+  SGR_STACK_LDS(stk);
+  TraceCount cnt{0, 0, 0};
   if (finishing(P, b, P.counts[SGR_CNT_Q + b])) return;
   const uint32_t n = P.counts[SGR_CNT_S + b];
   walk<true, MULTI, REFILL>(
@@ -235,14 +235,16 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathB
       cnt, stk);
   wave_count(&stats[SGR_STAT_NODES_S], cnt.nodes);
   wave_count(&stats[SGR_STAT_TESTS_S], cnt.tests);
+  wave_count(&stats[SGR_STAT_FETCH_S], cnt.fetch);
 }
 
 // One path's vertex (render/path-integ.cc:133-407 for one bounce): the
-// hit at P.hprim / P.ht of path P, emission, direct lighting (shadow rays
-// written to the path's slots, each announced by EMIT(slot, octant of its
-// direction)), the BSDF continuation and Russian roulette.  Returns
-// whether the path goes on with the ray it stored.  J is the queue
-// position (learned proposal lobes); ILLUM counts direct-lighting calls.
+// hit of path P at queue position J (P.hprim / P.ht [J]), emission,
+// direct lighting (shadow rays written to the path's slots, each
+// announced by EMIT(slot, octant of its direction)), the BSDF
+// continuation and Russian roulette.  Returns whether the path goes on
+// with the ray it stored.  J also indexes the learned proposal's lobes;
+// ILLUM counts direct-lighting calls.
 template<bool MULTI, bool GUIDED, typename Emit>
 __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView &R1,
                                            const StepView &J, const PathBufs &P,
@@ -261,7 +263,7 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
   float alpha = P.alpha[p];
   RayF ray;
   load_ray(P, p, ray);
-  uint32_t prim = P.hprim[p];
+  uint32_t prim = P.hprim[j];
   bool done = false, cont = false;
   uint32_t nst = st & ~(SGR_ST_PENDING | SGR_ST_DONE | SGR_ST_STRAT);
   float *grec = nullptr;  // this path's training record (learned proposal)
@@ -279,7 +281,7 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
     if (path_len == 0 && czero(rad)) alpha = R.bg_alpha;
     done = true;
   } else {
-    ray.t1 = P.ht[p];
+    ray.t1 = P.ht[j];
     Isec is;
     float mtop = P.media[(size_t)(depth - 1) * N + p];
     float mencl = depth >= 2 ? P.media[(size_t)(depth - 2) * N + p] : 1.0f;
@@ -735,12 +737,11 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
                                                 unsigned long long *stats) {
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   if (!finishing(P, b, n)) return;
-  __shared__ float4 stk_lds[2 * SGR_LDS_SLOTS * SGR_TRACE_BLOCK];
-  const LdsStack stk{&stk_lds[threadIdx.x]};
-  float4 st0[SGR_MAX_DEPTH], st1[SGR_MAX_DEPTH];
+  SGR_STACK_LDS(stk);
+  uint2 spill[SGR_MAX_DEPTH];
   const uint32_t *q = P.q[b & 1];
   const GuideView G{};
-  TraceCount cc{0, 0}, cs{0, 0};
+  TraceCount cc{0, 0, 0}, cs{0, 0, 0};
   uint32_t illum = 0, rays = 0, shadow = 0;
   uint32_t jb, je;
   block_range(n, jb, je);
@@ -751,11 +752,11 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
       const SceneView &S = *Sp;
       TravState T;
       load_ray(P, p, T.ray);
-      trace_one<false>(S, T, cc, stk, st0, st1);
-      P.hprim[p] = T.hit;
-      P.ht[p] = T.ray.t1;
-      P.hu[p] = T.hu;
-      P.hv[p] = T.hv;
+      trace_one<false>(S, T, cc, stk, spill);
+      P.hprim[j] = T.hit;
+      P.ht[j] = T.ray.t1;
+      P.hu[j] = T.hu;
+      P.hv[j] = T.hv;
       uint64_t em = 0;  // shadow slots of the path written at this vertex
       const bool cont = shade_path<MULTI, false>(S1, R1, J, P, G, p, j, illum,
                                                  [&](size_t slot, uint32_t) {
@@ -768,7 +769,7 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
         U.ray.d = v3(P.sray[3][slot], P.sray[4][slot], P.sray[5][slot]);
         U.ray.t0 = P.sray[6][slot];
         U.ray.t1 = P.sray[7][slot];
-        trace_one<true>(S, U, cs, stk, st0, st1);
+        trace_one<true>(S, U, cs, stk, spill);
         P.svis[slot] = U.hit == SGR_MISS ? 1 : 0;
         shadow++;
       }
@@ -781,6 +782,8 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
   wave_count(&stats[SGR_STAT_TESTS_C], cc.tests);
   wave_count(&stats[SGR_STAT_NODES_S], cs.nodes);
   wave_count(&stats[SGR_STAT_TESTS_S], cs.tests);
+  wave_count(&stats[SGR_STAT_FETCH_C], cc.fetch);
+  wave_count(&stats[SGR_STAT_FETCH_S], cs.fetch);
   wave_count(&stats[SGR_STAT_ILLUM], illum);
   wave_count(&stats[SGR_STAT_ISECT], rays);
   wave_count(&stats[SGR_STAT_SHADOW], shadow);
