@@ -198,8 +198,8 @@ class CpuBaseline:
             p.seed = 1000 + 7919 * k + job.seed % 1000
             a, _, _ = osc.render(job.camera(), p, nthreads=self.cores)
             imgs.append(finalize(a[y0:y0 + hh, x0:x0 + ww]))
-        frac, ok = serve.pixel_test(finalize(acc[y0:y0 + hh, x0:x0 + ww]), imgs)
-        return {"pass_fraction": frac, "passed": bool(ok)}
+        frac, ok, ref = serve.pixel_test(finalize(acc[y0:y0 + hh, x0:x0 + ww]), imgs)
+        return {"pass_fraction": frac, "cpu_pass_fraction": ref, "passed": bool(ok)}
 
 
 # --------------------------------------------------------------------------
@@ -359,8 +359,8 @@ def batch_bench(args):
         "sequential_render_s": round(seq, 3),
         "batch_vs_sequential": round(seq / rep["wall_s"], 3),
         "scene_build_s": round(t_build, 1),
-        "autotune_table": [[c["path_capacity"], c["grid_blocks"], round(r * 1e3)]
-                           for c, r in table],
+        "autotune_table": [[c["path_capacity"], c["grid_blocks"], c["finish_max"],
+                            round(r * 1e3)] for c, r in table],
         "graph_steps": rep["graph_steps"], "max_jobs_per_step": rep["max_jobs_per_step"],
     }
     for sc in S.values():

@@ -24,6 +24,24 @@ def compare_samples(got, ref, rtol):
     return float(np.mean(rel.max(1) <= rtol))
 
 
+def check_paths(got, ref, rtol, min_frac, name):
+    """At least MIN_FRAC of the paths agree within RTOL (measured: every
+    path of these scenes within 1e-4, 97.8-99.9 % within 1e-5); the rest took
+    another branch after a last-ulp difference of a transcendental, so
+    their radiance differences are independent draws of mean zero: their
+    mean per channel must lie within 3 standard errors of zero (a bias on
+    some branch would not)."""
+    frac = compare_samples(got, ref, rtol)
+    print(f"{name}: {frac:.4f} of {len(got)} paths within {rtol:g}", flush=True)
+    assert frac >= min_frac, (name, frac)
+    rel = np.abs(got[:, 2:] - ref[:, 2:]) / (np.abs(ref[:, 2:]) + 1e-3)
+    div = rel.max(1) > rtol
+    if div.sum() >= 8:
+        d = got[div, 2:5].astype(float) - ref[div, 2:5].astype(float)
+        se = d.std(0, ddof=1) / np.sqrt(len(d)) + 1e-9
+        assert np.all(np.abs(d.mean(0)) <= 3.5 * se), (name, d.mean(0), se)
+
+
 def test_cornell_paths_match_oracle(ctx):
     sd, cam = scenes.cornell_box()
     sc, osc = native.Scene(ctx, sd), oracle.OracleScene(sd)
@@ -42,7 +60,7 @@ def test_soup_paths_match_oracle(ctx):
     p = abi.make_params(64, 36, 8, max_path_len=10)
     got, _ = sc.render_samples(cam, p)
     _, ref, _ = osc.render(cam, p, samples=True)
-    assert compare_samples(got, ref, 1e-3) > 0.97
+    check_paths(got, ref, 1e-4, 0.999, "soup")
 
 
 def test_lights_and_materials_paths_match_oracle(ctx):
@@ -64,7 +82,7 @@ def test_lights_and_materials_paths_match_oracle(ctx):
     p = abi.make_params(40, 40, 8, max_path_len=8, direct_samples=2)
     got, _ = sc.render_samples(cam, p)
     _, ref, _ = osc.render(cam, p, samples=True)
-    assert compare_samples(got, ref, 1e-3) > 0.97
+    check_paths(got, ref, 1e-4, 0.999, "lights and materials")
 
 
 def test_many_lights_match_oracle(ctx):
@@ -86,7 +104,7 @@ def test_many_lights_match_oracle(ctx):
     p = abi.make_params(40, 32, 8, max_path_len=6, direct_samples=3)
     got, gst = sc.render_samples(cam, p)
     _, ref, ost = osc.render(cam, p, samples=True)
-    assert compare_samples(got, ref, 1e-3) > 0.97
+    check_paths(got, ref, 1e-4, 0.999, "many lights")
     for k in ("scene_intersect_calls", "shadow_tests", "illum_calls"):
         assert abs(gst[k] - ost[k]) <= 0.005 * ost[k], k
     acc, _ = sc.render(cam, p)  # accumulating path, default flags
@@ -277,4 +295,25 @@ def test_edge_cases_match_oracle(ctx, case):
     np.testing.assert_allclose(acc[..., 3:], acc_ref[..., 3:], rtol=1e-5, atol=1e-5)
     close = np.abs(acc[..., :3] - acc_ref[..., :3]) <= 1e-3 * (np.abs(acc_ref[..., :3]) + 1e-2)
     assert close.mean() >= (0.95 if close.size > 3 else 0.0)
+    sc.close()
+
+
+def test_nested_media_beyond_the_stack_cap_match_oracle(ctx):
+    """Nine concentric glass spheres: the eighth and ninth media cannot be
+    pushed (the stack holds eight levels, air included), on the device as
+    in the oracle; paths and the overflow count agree."""
+    sd = scenes.SceneDesc()
+    iors = [1.3, 1.6, 1.4, 1.8, 1.5, 2.0, 1.7, 1.45, 1.9]
+    for i, n in enumerate(iors):
+        sd.sphere((0, 0, 0), 1.0 - 0.07 * i, sd.glass(n))
+    sd.envmap(scenes.gaussian_lobe_envmap(64, 32, 7))
+    cam = scenes.look_at((0, 0.3, -4), (0, 0, 0), vfov_deg=30.0)
+    sc, osc = native.Scene(ctx, sd), oracle.OracleScene(sd)
+    p = abi.make_params(32, 32, 8, max_path_len=40, min_path_len=50)
+    got, gst = sc.render_samples(cam, p)
+    _, ref, ost = osc.render(cam, p, samples=True)
+    check_paths(got, ref, 1e-4, 0.999, "nested media")
+    assert ost["media_overflows"] > 0
+    assert abs(gst["media_overflows"] - ost["media_overflows"]) <= 0.01 * ost["media_overflows"]
+    assert 0 < gst["node_fetches_closest"] < gst["node_visits_closest"]
     sc.close()

@@ -24,8 +24,8 @@ def oracle_pixel_check(job, desc, acc, window=8, seeds=24):
         p.seed = 1000 + 7919 * k + job.seed % 1000
         a, _, _ = osc.render(job.camera(), p)
         imgs.append(serve.finalize_numpy(a[y0:y0 + hh, x0:x0 + ww]))
-    frac, ok = serve.pixel_test(serve.finalize_numpy(acc[y0:y0 + hh, x0:x0 + ww]), imgs)
-    return {"pass_fraction": frac, "passed": bool(ok)}
+    frac, ok, ref = serve.pixel_test(serve.finalize_numpy(acc[y0:y0 + hh, x0:x0 + ww]), imgs)
+    return {"pass_fraction": frac, "cpu_pass_fraction": ref, "passed": bool(ok)}
 
 
 def small_requests(n=24, rate=60.0, seed=3):
