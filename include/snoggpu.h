@@ -83,7 +83,11 @@ typedef struct sgr_material {
  * are vertex-index triples.  tri_materials may be NULL, then every
  * triangle uses `material`.  axis is the mesh axis used for tangent
  * frames (surface/mesh.cc:563-600); left_handed selects the normal
- * convention (surface/mesh.h:267-271). */
+ * convention (surface/mesh.h:267-271).  normals: per-vertex normals
+ * (float xyz, num_vertices of them; surface/mesh.h:144-149) or NULL;
+ * with them the shading frame follows the barycentric interpolation of
+ * the three vertex normals and the geometric normal is flipped to its
+ * side (surface/mesh.cc:620-641). */
 typedef struct sgr_mesh {
     const float    *vertices;
     uint32_t        num_vertices;
@@ -93,6 +97,7 @@ typedef struct sgr_mesh {
     uint32_t        material;
     float           axis[3];
     int32_t         left_handed;
+    const float    *normals;
 } sgr_mesh;
 
 /* Triangle or parallelogram primitive (surface/tripar.h). */

@@ -36,6 +36,7 @@ template<class R> struct V3 {
   V3() : x(0), y(0), z(0) {}
   V3(R a, R b, R c) : x(a), y(b), z(c) {}
   V3 operator+(const V3 &o) const { return V3(x + o.x, y + o.y, z + o.z); }
+  V3 &operator+=(const V3 &o) { x = x + o.x; y = y + o.y; z = z + o.z; return *this; }
   V3 operator-(const V3 &o) const { return V3(x - o.x, y - o.y, z - o.z); }
   V3 operator-() const { return V3(-x, -y, -z); }
   V3 operator*(R s) const { return V3(x * s, y * s, z * s); }

@@ -532,7 +532,8 @@ template<class R> struct Scene {
   std::vector<Prim<R>> prims;
   std::vector<int> prim_mat;
   std::vector<int> prim_mesh;
-  struct MeshInfo { V3<R> axis, center; bool left_handed; };
+  struct MeshInfo { V3<R> axis, center; bool left_handed, has_normals; };
+  std::vector<V3<R>> tri_vn;  // 3 vertex normals per mesh triangle (meshes with normals)
   std::vector<MeshInfo> meshes;
   std::vector<Material<R>> mats;
   std::vector<Light<R>> lights;
@@ -575,7 +576,7 @@ template<class R> struct Scene {
       V3<R> cen((mb.min.x + mb.max.x) / R(2), (mb.min.y + mb.max.y) / R(2),
                 (mb.min.z + mb.max.z) / R(2));
       meshes.push_back(MeshInfo{V3<R>(m.axis[0], m.axis[1], m.axis[2]), cen,
-                                m.left_handed != 0});
+                                m.left_handed != 0, m.normals != nullptr});
       bbox.add(mb);
       for (unsigned t = 0; t < m.num_triangles; t++) {
         const uint32_t *ix = &m.triangles[3 * t];
@@ -592,6 +593,13 @@ template<class R> struct Scene {
         prims.push_back(p);
         prim_mat.push_back(m.tri_materials ? (int)m.tri_materials[t] : (int)m.material);
         prim_mesh.push_back((int)mi);
+        if (m.normals) {  // surface/mesh.h:144-149 vertex_normals
+          tri_vn.resize(3 * (prims.size() - 1));
+          for (int c = 0; c < 3; c++) {
+            const float *nv = &m.normals[3 * ix[c]];
+            tri_vn.push_back(V3<R>(nv[0], nv[1], nv[2]));
+          }
+        }
         BBox<R> tb(a);
         tb.add(b);
         tb.add(c);
@@ -699,27 +707,40 @@ template<class R> struct Scene {
   // Frame construction for a hit (surface/mesh.cc:563-709,
   // surface/tripar.cc:45-66, surface/sphere.cc IsecInfo::make_intersect)
   // followed by Intersect::finish_init (render/intersect.cc:86-136).
+  // Mesh::Part::Triangle::IsecInfo::make_frame (surface/mesh.cc:563-600)
+  static Frame<R> mesh_frame(const MeshInfo &mi, const V3<R> &point, const V3<R> &norm) {
+    V3<R> s = cross(norm, mi.axis);
+    if (s.length_squared() < R(1e-15)) {
+      V3<R> cent = mi.center - point;
+      s = cross(norm, cent);
+      if (s.length_squared() < R(1e-15)) s = norm.perpendicular();
+    }
+    s = s.unit();
+    V3<R> t = cross(s, norm);
+    return Frame<R>(point, s, t, norm);
+  }
+
   void make_isec(unsigned p, const Ray<R> &ray, R hu, R hv, R media_ior,
                  R enclosing_ior, Isec<R> &is) const {
-    (void)hu;
-    (void)hv;
     const Prim<R> &pr = prims[p];
     V3<R> point = ray.origin + ray.dir * ray.t1;
     Frame<R> nf, gf;
     if (pr.kind == 0) {
       const MeshInfo &mi = meshes[prim_mesh[p]];
       V3<R> raw = mi.left_handed ? cross(pr.e2, pr.e1) : cross(pr.e1, pr.e2);
-      V3<R> norm = raw.unit();
-      V3<R> s = cross(norm, mi.axis);
-      if (s.length_squared() < R(1e-15)) {
-        V3<R> cent = mi.center - point;
-        s = cross(norm, cent);
-        if (s.length_squared() < R(1e-15)) s = norm.perpendicular();
+      gf = mesh_frame(mi, point, raw.unit());
+      if (mi.has_normals) {
+        // surface/mesh.cc:620-641: interpolated shading normal; the
+        // geometric normal is flipped to its side
+        V3<R> norm = tri_vn[3 * p] * (R(1) - hu - hv);
+        norm += tri_vn[3 * p + 1] * hu;
+        norm += tri_vn[3 * p + 2] * hv;
+        norm = norm.unit();
+        if (gf.to(norm).z < 0) gf.z = -gf.z;
+        nf = mesh_frame(mi, point, norm);
+      } else {
+        nf = gf;
       }
-      s = s.unit();
-      V3<R> t = cross(s, norm);
-      gf = Frame<R>(point, s, t, norm);
-      nf = gf;
     } else if (pr.kind == 1) {
       R e1_len = pr.e1.length(), inv_e1_len = R(1) / e1_len;
       V3<R> norm = cross(pr.e2, pr.e1).unit();

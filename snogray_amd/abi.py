@@ -37,7 +37,8 @@ class Mesh(C.Structure):
     _fields_ = [("vertices", f32p), ("num_vertices", C.c_uint32),
                 ("triangles", u32p), ("num_triangles", C.c_uint32),
                 ("tri_materials", u32p), ("material", C.c_uint32),
-                ("axis", C.c_float * 3), ("left_handed", C.c_int32)]
+                ("axis", C.c_float * 3), ("left_handed", C.c_int32),
+                ("normals", f32p)]
 
 
 class Tripar(C.Structure):

@@ -44,6 +44,7 @@ struct DevMesh {
   float axis[3];
   float center[3];
   int left_handed;
+  int has_normals;  // vertex normals in SceneView::tri_vn
 };
 
 #define SGR_ENV_GUIDE_ROW 256
@@ -55,6 +56,7 @@ struct SceneView {
   const float4 *refs;        // 3 per surface reference, surface-list order
   const float4 *tri;         // 3 per triangle-like prim: v0|flags, e1, e2
   const float4 *sph;         // centre, radius
+  const float4 *tri_vn;      // 3 per mesh triangle: its vertex normals (or null: none)
   const uint32_t *prim_mat;  // material index per prim
   const uint32_t *prim_mesh; // mesh index per triangle (0xffffffff others)
   const DevMesh *meshes;

@@ -36,10 +36,31 @@ struct Isec {
   float media_ior, encl_ior;
 };
 
+// Mesh::Part::Triangle::IsecInfo::make_frame (surface/mesh.cc:563-600):
+// tangent from NORM x axis, else from the direction to the mesh centre,
+// else any perpendicular.
+__device__ __forceinline__ Frame mesh_frame(const DevMesh &M, V3 point, V3 norm) {
+  V3 axis = v3(M.axis[0], M.axis[1], M.axis[2]);
+  V3 s = cross(norm, axis);
+  if (len2(s) < 1e-15f) {
+    V3 cent = v3(M.center[0], M.center[1], M.center[2]) - point;
+    s = cross(norm, cent);
+    if (len2(s) < 1e-15f) {  // geometry/vec.h:119-123 perpendicular
+      float nz = (norm.x == norm.z) ? norm.z : -norm.z;
+      s = v3(norm.y * norm.y + norm.x * nz, -nz * nz - norm.x * norm.y,
+             norm.x * norm.x - norm.y * nz);
+    }
+  }
+  s = unit(s);
+  V3 t = cross(s, norm);
+  return Frame{point, s, t, norm};
+}
+
 // Hit frames: surface/mesh.cc:563-709, surface/tripar.cc:45-66,
 // surface/sphere.cc make_intersect; then Intersect::finish_init.
 __device__ __forceinline__ void make_isec(const SceneView &S, uint32_t p, const RayF &ray,
-                                          float media_ior, float encl_ior, Isec &is) {
+                                          float hu, float hv, float media_ior, float encl_ior,
+                                          Isec &is) {
   V3 point = ray.o + ray.d * ray.t1;
   Frame nf, gf;
   if (p < S.n_tri_like) {
@@ -47,22 +68,21 @@ __device__ __forceinline__ void make_isec(const SceneView &S, uint32_t p, const 
     if (p < S.n_tri) {
       const DevMesh &M = S.meshes[S.prim_mesh[p]];
       V3 raw = M.left_handed ? cross(e2, e1) : cross(e1, e2);
-      V3 norm = unit(raw);
-      V3 axis = v3(M.axis[0], M.axis[1], M.axis[2]);
-      V3 s = cross(norm, axis);
-      if (len2(s) < 1e-15f) {
-        V3 cent = v3(M.center[0], M.center[1], M.center[2]) - point;
-        s = cross(norm, cent);
-        if (len2(s) < 1e-15f) {  // geometry/vec.h:119-123 perpendicular
-          float nz = (norm.x == norm.z) ? norm.z : -norm.z;
-          s = v3(norm.y * norm.y + norm.x * nz, -nz * nz - norm.x * norm.y,
-                 norm.x * norm.x - norm.y * nz);
-        }
+      gf = mesh_frame(M, point, unit(raw));
+      if (M.has_normals) {
+        // surface/mesh.cc:620-641: shading normal interpolated from the
+        // vertex normals; the geometric normal takes its side
+        const V3 n0 = ld3(S.tri_vn[3 * p]), n1 = ld3(S.tri_vn[3 * p + 1]),
+                 n2 = ld3(S.tri_vn[3 * p + 2]);
+        V3 norm = n0 * (1.0f - hu - hv);
+        norm = norm + n1 * hu;
+        norm = norm + n2 * hv;
+        norm = unit(norm);
+        if (gf.to(norm).z < 0.0f) gf.z = -gf.z;
+        nf = mesh_frame(M, point, norm);
+      } else {
+        nf = gf;
       }
-      s = unit(s);
-      V3 t = cross(s, norm);
-      gf = Frame{point, s, t, norm};
-      nf = gf;
     } else {
       float e1_len = vlen(e1), inv = 1.0f / e1_len;
       V3 norm = unit(cross(e2, e1));

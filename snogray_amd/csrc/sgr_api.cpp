@@ -250,6 +250,7 @@ namespace {
 struct HostScene {
   std::vector<DevMaterial> mats;
   std::vector<float4> tri, sph, env_tex;
+  std::vector<float4> tri_vn;  // vertex normals of mesh triangles, if any mesh has them
   std::vector<uint32_t> prim_mat, prim_mesh, order;
   std::vector<DevMesh> meshes;
   std::vector<DevLight> lights;
@@ -360,6 +361,13 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
       dm.center[a] = (mlo[a] + mhi[a]) / 2;  // geometry/pos.h:123-126
     }
     dm.left_handed = m.left_handed ? 1 : 0;
+    dm.has_normals = m.normals ? 1 : 0;
+    if (m.normals) {
+      for (uint32_t k = 0; k < 3 * m.num_vertices; k++)
+        if (!std::isfinite(m.normals[k]))
+          return fail(SGR_ERR_INVALID, "mesh " + std::to_string(mi) + ": non-finite normal");
+      if (H.tri_vn.empty()) H.tri_vn.assign((size_t)n_tri * 3, make_float4(0.f, 0.f, 0.f, 0.f));
+    }
     for (uint32_t t = 0; t < m.num_triangles; t++, p++) {
       uint32_t ix[3] = {m.triangles[3 * t], m.triangles[3 * t + 1], m.triangles[3 * t + 2]};
       for (int c = 0; c < 3; c++)
@@ -372,6 +380,11 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
       tri[3 * p] = make_float4(a[0], a[1], a[2], 0.0f);
       tri[3 * p + 1] = make_float4(e1[0], e1[1], e1[2], 0.0f);
       tri[3 * p + 2] = make_float4(e2[0], e2[1], e2[2], 0.0f);
+      if (m.normals)
+        for (int c = 0; c < 3; c++) {
+          const float *nv = m.normals + 3 * (size_t)ix[c];
+          H.tri_vn[3 * p + c] = make_float4(nv[0], nv[1], nv[2], 0.0f);
+        }
       uint32_t mat = m.tri_materials ? m.tri_materials[t] : m.material;
       if (mat >= nm) return fail(SGR_ERR_INVALID, "mesh " + std::to_string(mi) + ": bad material");
       prim_mat[p] = mat;
@@ -608,6 +621,7 @@ static int scene_create_impl(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **
   if (!rc) rc = upload(ctx, sc, up, refs, &V.refs);
   if (!rc) rc = upload(ctx, sc, up, H.tri, &V.tri);
   if (!rc) rc = upload(ctx, sc, up, H.sph, &V.sph);
+  if (!rc && !H.tri_vn.empty()) rc = upload(ctx, sc, up, H.tri_vn, &V.tri_vn);
   if (!rc) rc = upload(ctx, sc, up, H.prim_mat, &V.prim_mat);
   if (!rc) rc = upload(ctx, sc, up, H.prim_mesh, &V.prim_mesh);
   if (!rc) rc = upload(ctx, sc, up, H.meshes, &V.meshes);

@@ -285,7 +285,9 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
     Isec is;
     float mtop = P.media[(size_t)(depth - 1) * N + p];
     float mencl = depth >= 2 ? P.media[(size_t)(depth - 2) * N + p] : 1.0f;
-    make_isec(S, prim, ray, mtop, mencl, is);
+    // barycentrics only matter for interpolated vertex normals
+    const float hu = S.tri_vn ? P.hu[j] : 0.0f, hv = S.tri_vn ? P.hv[j] : 0.0f;
+    make_isec(S, prim, ray, hu, hv, mtop, mencl, is);
     const DevMaterial &HM = S.mats[is.mat];
     if (include && HM.type == 5 && !is.back) rad = rad + cload(HM.color) * T;
     if (is.bmat < 0) {

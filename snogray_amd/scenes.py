@@ -71,13 +71,16 @@ class SceneDesc:
 
     # --- surfaces ------------------------------------------------------
     def mesh(self, vertices, triangles, material, tri_materials=None,
-             axis=(0.0, 0.0, 1.0), left_handed=True):
+             axis=(0.0, 0.0, 1.0), left_handed=True, normals=None):
         v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
         t = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1, 3)
         tm = None if tri_materials is None else np.ascontiguousarray(
             tri_materials, dtype=np.uint32)
+        vn = None if normals is None else np.ascontiguousarray(
+            normals, dtype=np.float32).reshape(-1, 3)
+        assert vn is None or len(vn) == len(v), "one normal per vertex"
         self.meshes.append(dict(v=v, t=t, tm=tm, material=material,
-                                axis=axis, left_handed=left_handed))
+                                axis=axis, left_handed=left_handed, vn=vn))
         self._c = None
         return len(self.meshes) - 1
 
@@ -128,6 +131,7 @@ class SceneDesc:
             meshes[i].material = m["material"]
             meshes[i].axis[:] = [float(x) for x in m["axis"]]
             meshes[i].left_handed = 1 if m["left_handed"] else 0
+            meshes[i].normals = abi.fptr(m["vn"]) if m.get("vn") is not None else None
         tps = (abi.Tripar * max(1, len(self.tripars)))()
         for i, t in enumerate(self.tripars):
             tps[i].v0[:] = [float(x) for x in t["v0"]]
