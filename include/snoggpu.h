@@ -336,6 +336,34 @@ int sgr_render_noalloc(sgr_scene *scene, const sgr_camera *cam,
 int sgr_finalize_image(const float *accum, uint32_t width, uint32_t height,
                        float exposure, float contrast, float *rgba);
 
+/* ---- image files (host only) ------------------------------------------
+ * The formats the reference always supports (image/image-dispatch.cc:
+ * 70-157): "pfm" (image/image-pfm.cc) and "rgbe" / "hdr" / "pic"
+ * (Radiance, image/image-rgbe.cc).  format == NULL: from the file name's
+ * extension (ImageIo::find_format). */
+/* Write a finished RGBA image (4 floats per pixel, row-major, row 0 at
+ * the top; sgr_finalize_image output): each pixel's alpha-scaled colour,
+ * as ImageSink::write_row writes it.  Replaces PfmImageSink /
+ * RgbeImageSink (image/image-pfm.cc:58-94, image/image-rgbe.cc:44-150). */
+int sgr_image_write(const char *path, const char *format, const float *rgba,
+                    uint32_t width, uint32_t height);
+/* Read an image as RGB floats (3 per pixel, row 0 at the top).  rgb ==
+ * NULL: only *width and *height are set; else they must match the file.
+ * Replaces PfmImageSource / RgbeImageSource (image/image-pfm.cc:100-147,
+ * image/image-rgbe.cc:156-333). */
+int sgr_image_read(const char *path, const char *format, float *rgb,
+                   uint32_t *width, uint32_t *height);
+/* Recover the rows of a partially written output image into accumulators
+ * (5 floats per pixel, width * height): each complete row, from the top,
+ * becomes (r, g, b, alpha 1, weight 1); if the file ends or breaks inside
+ * a row, the last SGR_RECOVER_DISCARD_ROWS rows read are dropped too.
+ * *rows is the number of rows recovered (0 for a missing or unreadable
+ * file); a render of the rest of the image continues at that row.
+ * Replaces recover_image (image/recover-image.cc:22-124). */
+#define SGR_RECOVER_DISCARD_ROWS 4
+int sgr_image_recover(const char *path, const char *format, float *accum,
+                      uint32_t width, uint32_t height, uint32_t *rows);
+
 /* ---- job engine: many renders sharing one stream of launches ----------
  * The reference renders one image per RenderMgr::render call
  * (render-mgr/render-mgr.cc:35-63); these calls have no counterpart

@@ -21,6 +21,7 @@ EXPORTS = [
     "sgr_octree_export", "sgr_octree_build", "sgr_scene_bounds", "sgr_scene_footprint",
     "sgr_trace_closest", "sgr_trace_occluded", "sgr_render",
     "sgr_render_samples", "sgr_finalize_image", "sgr_render_noalloc",
+    "sgr_image_write", "sgr_image_read", "sgr_image_recover",
     "sgr_engine_configure", "sgr_job_footprint", "sgr_job_submit", "sgr_engine_step",
     "sgr_job_poll", "sgr_job_result", "sgr_job_release", "sgr_engine_stats",
     "sgr_render_batch",
@@ -68,6 +69,11 @@ def lib():
     L.sgr_render_samples.argtypes = [vp, C.POINTER(abi.Camera),
                                      C.POINTER(abi.RenderParams), abi.f32p,
                                      C.POINTER(abi.Stats)]
+    L.sgr_image_write.argtypes = [C.c_char_p, C.c_char_p, abi.f32p, C.c_uint32, C.c_uint32]
+    L.sgr_image_read.argtypes = [C.c_char_p, C.c_char_p, abi.f32p, C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32)]
+    L.sgr_image_recover.argtypes = [C.c_char_p, C.c_char_p, abi.f32p, C.c_uint32, C.c_uint32,
+                                    C.POINTER(C.c_uint32)]
     L.sgr_render_noalloc.argtypes = [vp, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams),
                                      abi.f32p, C.POINTER(abi.Stats)]
     L.sgr_finalize_image.argtypes = [abi.f32p, C.c_uint32, C.c_uint32, C.c_float,
@@ -419,3 +425,39 @@ def finalize(accum, exposure=1.0, contrast=1.0):
     check(lib().sgr_finalize_image(abi.fptr(a), w, h, exposure, contrast, abi.fptr(out)),
           "sgr_finalize_image")
     return out
+
+
+def _fmt(fmt):
+    return None if fmt is None else fmt.encode()
+
+
+def write_image(path, rgba, fmt=None):
+    """Write a finished (H, W, 4) RGBA image as PFM or Radiance RGBE
+    (format from the extension unless FMT is given)."""
+    a = np.ascontiguousarray(rgba, np.float32)
+    h, w = a.shape[:2]
+    check(lib().sgr_image_write(str(path).encode(), _fmt(fmt), abi.fptr(a), w, h),
+          "sgr_image_write")
+
+
+def read_image(path, fmt=None):
+    """(H, W, 3) float32 RGB of a PFM or RGBE file."""
+    w, h = C.c_uint32(), C.c_uint32()
+    check(lib().sgr_image_read(str(path).encode(), _fmt(fmt), None, C.byref(w), C.byref(h)),
+          "sgr_image_read")
+    out = np.zeros((h.value, w.value, 3), np.float32)
+    check(lib().sgr_image_read(str(path).encode(), _fmt(fmt), abi.fptr(out), C.byref(w),
+                               C.byref(h)), "sgr_image_read")
+    return out
+
+
+def recover_image(path, accum, fmt=None):
+    """Fill the leading complete rows of a partially written output image
+    into ACCUM ((H, W, 5) float32, modified in place); returns the number
+    of rows recovered (a render of the rest continues at that row)."""
+    assert accum.dtype == np.float32 and accum.flags.c_contiguous
+    rows = C.c_uint32()
+    h, w = accum.shape[:2]
+    check(lib().sgr_image_recover(str(path).encode(), _fmt(fmt), abi.fptr(accum), w, h,
+                                  C.byref(rows)), "sgr_image_recover")
+    return rows.value

@@ -356,3 +356,30 @@ def test_vertex_normals_match_oracle(ctx):
     ofin = native.finalize(oacc.astype(np.float32))[..., :3]
     assert np.mean(np.abs(fin - ofin) <= 1e-3 * (np.abs(ofin) + 1e-2)) > 0.99
     sc.close()
+
+
+def test_resume_from_a_partial_image(ctx, tmp_path):
+    """--continue: the rows recovered from a cut-off RGBE output image,
+    then a render of the rest of the image as a window, finalize to the
+    recovered rows above and to the uninterrupted render's rows below,
+    bit for bit (image/recover-image.cc:22-124, snogray.lua:447-454)."""
+    sd, cam = scenes.cornell_box()
+    sc = native.Scene(ctx, sd)
+    W, H = 48, 40
+    p = abi.make_params(W, H, 8, max_path_len=8, seed=4)
+    full, _ = sc.render(cam, p)
+    img = native.finalize(full)
+    path = tmp_path / "out.hdr"
+    native.write_image(path, img)
+    data = path.read_bytes()
+    path.write_bytes(data[:int(len(data) * 0.55)])   # the render was cut off
+    acc = np.zeros((H, W, 5), np.float32)
+    R = native.recover_image(path, acc)
+    assert 0 < R < H
+    q = abi.make_params(W, H, 8, max_path_len=8, seed=4, window=(0, R, W, H - R))
+    sc.render(cam, q, accum=acc)
+    out = native.finalize(acc)
+    np.testing.assert_array_equal(out[R:], img[R:])
+    native.write_image(tmp_path / "whole.hdr", img)
+    np.testing.assert_array_equal(out[:R, :, :3], native.read_image(tmp_path / "whole.hdr")[:R])
+    sc.close()
