@@ -364,6 +364,24 @@ int sgr_image_read(const char *path, const char *format, float *rgb,
 int sgr_image_recover(const char *path, const char *format, float *accum,
                       uint32_t width, uint32_t height, uint32_t *rows);
 
+/* ---- mesh files (host only) --------------------------------------------
+ * PLY (load/mesh/load-ply.cc: vertex x y z [nx ny nz], triangle faces;
+ * ASCII and binary) and Wavefront OBJ (load/mesh/obj.lua: v, vn, f with
+ * i or i//i indices, polygons fanned into triangles).  format == NULL:
+ * from the extension.  Meshes from both are right-handed (left_handed =
+ * 0 in the sgr_mesh they fill). */
+typedef struct sgr_mesh_file sgr_mesh_file;
+/* Parse a file; reports the vertex and triangle counts and whether it
+ * carries per-vertex normals (any pointer but path and out may be NULL). */
+int sgr_mesh_file_open(const char *path, const char *format, sgr_mesh_file **out,
+                       uint64_t *num_vertices, uint64_t *num_triangles,
+                       int *has_normals);
+/* Copy the arrays (3 floats per vertex, 3 indices per triangle, 3 floats
+ * per vertex normal) into caller buffers; NULL skips one. */
+int sgr_mesh_file_copy(const sgr_mesh_file *mesh, float *vertices,
+                       uint32_t *triangles, float *normals);
+int sgr_mesh_file_close(sgr_mesh_file *mesh);
+
 /* ---- job engine: many renders sharing one stream of launches ----------
  * The reference renders one image per RenderMgr::render call
  * (render-mgr/render-mgr.cc:35-63); these calls have no counterpart

@@ -22,6 +22,7 @@ EXPORTS = [
     "sgr_trace_closest", "sgr_trace_occluded", "sgr_render",
     "sgr_render_samples", "sgr_finalize_image", "sgr_render_noalloc",
     "sgr_image_write", "sgr_image_read", "sgr_image_recover",
+    "sgr_mesh_file_open", "sgr_mesh_file_copy", "sgr_mesh_file_close",
     "sgr_engine_configure", "sgr_job_footprint", "sgr_job_submit", "sgr_engine_step",
     "sgr_job_poll", "sgr_job_result", "sgr_job_release", "sgr_engine_stats",
     "sgr_render_batch",
@@ -74,6 +75,11 @@ def lib():
                                  C.POINTER(C.c_uint32)]
     L.sgr_image_recover.argtypes = [C.c_char_p, C.c_char_p, abi.f32p, C.c_uint32, C.c_uint32,
                                     C.POINTER(C.c_uint32)]
+    L.sgr_mesh_file_open.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(vp),
+                                     C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_int)]
+    L.sgr_mesh_file_copy.argtypes = [vp, abi.f32p, abi.u32p, abi.f32p]
+    L.sgr_mesh_file_close.argtypes = [vp]
     L.sgr_render_noalloc.argtypes = [vp, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams),
                                      abi.f32p, C.POINTER(abi.Stats)]
     L.sgr_finalize_image.argtypes = [abi.f32p, C.c_uint32, C.c_uint32, C.c_float,
@@ -461,3 +467,22 @@ def recover_image(path, accum, fmt=None):
     check(lib().sgr_image_recover(str(path).encode(), _fmt(fmt), abi.fptr(accum), w, h,
                                   C.byref(rows)), "sgr_image_recover")
     return rows.value
+
+
+def load_mesh(path, fmt=None):
+    """(vertices (V, 3) float32, triangles (T, 3) uint32, normals (V, 3)
+    float32 or None) of a PLY or OBJ file."""
+    h = C.c_void_p()
+    nv, nt, hn = C.c_uint64(), C.c_uint64(), C.c_int()
+    check(lib().sgr_mesh_file_open(str(path).encode(), _fmt(fmt), C.byref(h), C.byref(nv),
+                                   C.byref(nt), C.byref(hn)), "sgr_mesh_file_open")
+    try:
+        v = np.zeros((nv.value, 3), np.float32)
+        t = np.zeros((nt.value, 3), np.uint32)
+        n = np.zeros((nv.value, 3), np.float32) if hn.value else None
+        check(lib().sgr_mesh_file_copy(h, abi.fptr(v), abi.uptr(t),
+                                       abi.fptr(n) if n is not None else None),
+              "sgr_mesh_file_copy")
+    finally:
+        lib().sgr_mesh_file_close(h)
+    return v, t, n

@@ -84,6 +84,14 @@ class SceneDesc:
         self._c = None
         return len(self.meshes) - 1
 
+    def mesh_file(self, path, material, axis=(0.0, 0.0, 1.0), fmt=None):
+        """A PLY or OBJ mesh file (with its vertex normals, if any) as one
+        mesh; right-handed, as the reference's loaders make it
+        (load/mesh/load-ply.cc, load/mesh/obj.lua)."""
+        from . import native
+        v, t, n = native.load_mesh(path, fmt)
+        return self.mesh(v, t, material, axis=axis, left_handed=False, normals=n)
+
     def tripar(self, v0, e1, e2, material, parallelogram=False):
         self.tripars.append(dict(v0=v0, e1=e1, e2=e2, material=material,
                                  parallelogram=parallelogram))
