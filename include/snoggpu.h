@@ -127,11 +127,38 @@ typedef struct sgr_envmap {
     float        frame[9];
 } sgr_envmap;
 
+/* A model: geometry with its own acceleration structure in its own
+ * coordinate system, placed in the scene by instances (surface/model.h,
+ * surface/model.cc:26-47).  Material indices refer to the scene's table.
+ * A model holds no instances and supplies no light samplers (an emitting
+ * surface in it glows where a path hits it). */
+typedef struct sgr_model {
+    const sgr_mesh     *meshes;
+    uint32_t            num_meshes;
+    const sgr_tripar   *tripars;
+    uint32_t            num_tripars;
+    const sgr_sphere   *spheres;
+    uint32_t            num_spheres;
+} sgr_model;
+
+/* An instance of a model (surface/instance.cc:145-216): local_to_world is
+ * the 3 x 4 row-major affine transform (rows x, y, z: linear part, then
+ * translation).  A ray is intersected with the model in model space
+ * (the inverse transform, computed by the library); the shading frame is
+ * mapped back with the transform (the normal with the inverse
+ * transpose). */
+typedef struct sgr_instance {
+    uint32_t model;
+    float    local_to_world[12];
+} sgr_instance;
+
 /* Whole scene.  Surfaces are inserted into the acceleration structure in
- * this order: meshes (triangle order), then tripars, then spheres.  The
- * global primitive id of a hit is its index in that order.  Light
- * samplers are created for GLOW tripars and spheres (insertion order),
- * then for each envmap. */
+ * this order: meshes (triangle order), then tripars, then spheres, then
+ * instances.  The primitive id of a hit is its index in that order; for
+ * a surface of an instanced model it is the index in the model's own
+ * order (meshes, tripars, spheres) and the hit also names the instance.
+ * Light samplers are created for GLOW tripars and spheres (insertion
+ * order), then for each envmap. */
 typedef struct sgr_scene_desc {
     const sgr_mesh     *meshes;
     uint32_t            num_meshes;
@@ -145,6 +172,10 @@ typedef struct sgr_scene_desc {
     uint32_t            num_envmaps;
     float               min_trace;   /* render/render-params.h:32-36 */
     int32_t             accel;       /* SGR_ACCEL_* */
+    const sgr_model    *models;
+    uint32_t            num_models;
+    const sgr_instance *instances;
+    uint32_t            num_instances;
 } sgr_scene_desc;
 
 /* Rays and hits, structure-of-arrays (geometry/ray.h:84-89).  Pointers
@@ -160,6 +191,7 @@ typedef struct sgr_hits {
     float    *t;
     float    *u;
     float    *v;
+    uint32_t *inst;   /* instance of the hit (0xffffffff: none); may be NULL */
 } sgr_hits;
 
 /* Pinhole camera (camera/camera.h:322-389): eye_vec = forward

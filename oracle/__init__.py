@@ -67,6 +67,7 @@ def lib(fast=False):
         L.orc_stream_probe.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_uint32,
                                        C.c_uint32, abi.f32p, C.c_uint32,
                                        C.c_uint32, abi.f32p]
+        L.orc_isec_probe.argtypes = [C.c_void_p, abi.f32p, C.c_int, C.POINTER(C.c_double)]
         L.orc_permute.restype = C.c_uint32
         L.orc_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
         _libs[fast] = L
@@ -131,6 +132,20 @@ class OracleScene:
         self.L.orc_trace(self.h, C.byref(batch.c), batch.n, None,
                         occ.ctypes.data_as(C.POINTER(C.c_uint8)), nthreads, cnt)
         return occ, (cnt[0], cnt[1])
+
+    def isec_probe(self, origins, dirs, t0=1e-3, t1=1e9):
+        """Closest hit and hit frame of each ray (fp32 scene): dict of
+        prim, inst (-1: none), t, origin, normal (frame z), back, u, v."""
+        o = np.asarray(origins, np.float32).reshape(-1, 3)
+        d = np.asarray(dirs, np.float32).reshape(-1, 3)
+        rays = np.zeros((len(o), 8), np.float32)
+        rays[:, :3], rays[:, 3:6], rays[:, 6], rays[:, 7] = o, d, t0, t1
+        out = np.zeros((len(o), 12))
+        if self.L.orc_isec_probe(self.h, abi.fptr(rays), len(o), dptr(out)):
+            raise RuntimeError("isec probe needs an fp32 oracle scene")
+        return {"prim": out[:, 0].astype(np.int64), "inst": out[:, 1].astype(np.int64),
+                "t": out[:, 2], "origin": out[:, 3:6], "normal": out[:, 6:9],
+                "back": out[:, 9] > 0, "u": out[:, 10], "v": out[:, 11]}
 
     def render(self, cam, params, nthreads=8, samples=False):
         acc = np.zeros((params.height, params.width, 5), np.float64)

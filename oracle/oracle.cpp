@@ -178,7 +178,8 @@ template<class R> struct Tracer {
     for (;;) {
       R hu, hv;
       cnt.isect++;
-      unsigned prim = sc.accel.closest(ray, hu, hv, &cnt.nc_c, &cnt.st_c);
+      unsigned inst = 0xffffffffu;
+      unsigned prim = sc.accel.closest(ray, hu, hv, &cnt.nc_c, &cnt.st_c, &inst);
       bool include = path_len == 0 || after_spec || P.direct_samples == 0;
       if (prim == 0xffffffffu) {
         if (include) radiance += sc.background(ray.dir) * T;
@@ -187,7 +188,7 @@ template<class R> struct Tracer {
       }
       Isec<R> is;
       R enclosing = depth >= 2 ? media[depth - 2] : R(1);
-      sc.make_isec(prim, ray, hu, hv, media[depth - 1], enclosing, is);
+      sc.make_isec(prim, inst, ray, hu, hv, media[depth - 1], enclosing, is);
       const Material<R> &hm = sc.mats[is.mat];
       if (include && hm.type == SGR_MAT_GLOW && !is.back)  // material/glow.cc:36-40
         radiance += hm.color * T;
@@ -317,11 +318,13 @@ static void trace_impl(const Scene<R> &sc, const sgr_rays *rays, size_t n,
       r.t1 = rays->t1[i];
       if (hits) {
         R u = 0, v = 0;
-        unsigned p = sc.accel.closest(r, u, v, &cs[t].nc_c, &cs[t].st_c);
+        unsigned inst = 0xffffffffu;
+        unsigned p = sc.accel.closest(r, u, v, &cs[t].nc_c, &cs[t].st_c, &inst);
         hits->prim[i] = p;
         hits->t[i] = p == 0xffffffffu ? 0.f : float(r.t1);
         hits->u[i] = float(u);
         hits->v[i] = float(v);
+        if (hits->inst) hits->inst[i] = inst;
       }
       if (occ) occ[i] = sc.accel.occluded(r, &cs[t].nc_s, &cs[t].st_s) ? 1 : 0;
     }
@@ -629,5 +632,43 @@ int orc_stream_probe(uint64_t seed, int px, int py, uint32_t chan, uint32_t spp,
 }
 
 uint32_t orc_permute(uint32_t i, uint32_t n, uint64_t key) { return permute(i, n, key); }
+
+}  // extern "C"
+
+extern "C" {
+
+// Intersection probe: for each of n rays (ox oy oz dx dy dz t0 t1), the
+// closest hit and its Isec.  out (12 per ray): prim, instance (as
+// doubles; -1 for none), t, normal-frame origin xyz, normal-frame z xyz,
+// back flag, u, v.
+int orc_isec_probe(void *h, const float *rays, int n, double *out) {
+  OrcScene *s = (OrcScene *)h;
+  if (s->fp64) return 1;
+  const Scene<float> &sc = *s->sf;
+  for (int i = 0; i < n; i++) {
+    const float *r = rays + 8 * i;
+    Ray<float> ray;
+    ray.origin = V3<float>(r[0], r[1], r[2]);
+    ray.dir = V3<float>(r[3], r[4], r[5]);
+    ray.t0 = r[6];
+    ray.t1 = r[7];
+    float u, v;
+    unsigned inst = 0xffffffffu;
+    unsigned p = sc.accel.closest(ray, u, v, nullptr, nullptr, &inst);
+    double *o = out + 12 * i;
+    for (int k = 0; k < 12; k++) o[k] = 0;
+    o[0] = p == 0xffffffffu ? -1.0 : (double)p;
+    o[1] = inst == 0xffffffffu ? -1.0 : (double)inst;
+    if (p == 0xffffffffu) continue;
+    Isec<float> is;
+    sc.make_isec(p, inst, ray, u, v, 1.0f, 1.0f, is);
+    o[2] = ray.t1;
+    o[3] = is.nf.origin.x; o[4] = is.nf.origin.y; o[5] = is.nf.origin.z;
+    o[6] = is.nf.z.x; o[7] = is.nf.z.y; o[8] = is.nf.z.z;
+    o[9] = is.back ? 1 : 0;
+    o[10] = u; o[11] = v;
+  }
+  return 0;
+}
 
 }  // extern "C"

@@ -420,13 +420,42 @@ struct Mailbox {
   struct Entry { unsigned gen, id; } e[1024] = {};
 };
 static thread_local Mailbox tl_mailbox;
+static inline void mailbox_new_generation() {
+  if (++tl_mailbox.gen == 0) {  // generation wrapped: clear the table
+    for (auto &e : tl_mailbox.e) e.gen = 0;
+    tl_mailbox.gen = 1;
+  }
+}
 #endif
+
+// An affine 3 x 4 row-major transform applied to a position or a vector
+// (geometry/pos.h:80-97, geometry/vec.h:81-95), and transposed to a
+// vector (geometry/xform.h:295-309).
+template<class R> static inline V3<R> xform_pos(const R *m, const V3<R> &p) {
+  return V3<R>(p.x * m[0] + p.y * m[1] + p.z * m[2] + m[3],
+               p.x * m[4] + p.y * m[5] + p.z * m[6] + m[7],
+               p.x * m[8] + p.y * m[9] + p.z * m[10] + m[11]);
+}
+template<class R> static inline V3<R> xform_vec(const R *m, const V3<R> &v) {
+  return V3<R>(v.x * m[0] + v.y * m[1] + v.z * m[2], v.x * m[4] + v.y * m[5] + v.z * m[6],
+               v.x * m[8] + v.y * m[9] + v.z * m[10]);
+}
+template<class R> static inline V3<R> xform_transpose(const R *m, const V3<R> &v) {
+  return V3<R>(v.x * m[0] + v.y * m[4] + v.z * m[8], v.x * m[1] + v.y * m[5] + v.z * m[9],
+               v.x * m[2] + v.y * m[6] + v.z * m[10]);
+}
 
 // Closest-hit / any-hit search over either the octree or a brute-force
 // list.  The callback semantics follow space/space.cc:36-194.
 template<class R> struct Accel {
   const std::vector<Prim<R>> *prims = nullptr;
   bool use_triv = false;
+  // Two-level instancing (surface/instance.cc:114-146): surface
+  // n_prims + i is instance i of a model with its own Accel.
+  struct Inst { unsigned model; R l2w[12], w2l[12]; };
+  const std::vector<Inst> *insts = nullptr;
+  std::vector<const Accel *> models;
+  unsigned n_prims = 0;
   std::vector<FlatNode> nodes;
   std::vector<unsigned> surfs;
   std::vector<unsigned> triv_order;
@@ -453,6 +482,7 @@ template<class R> struct Accel {
     bool any;
     bool stop = false;
     unsigned hit = 0xffffffffu;
+    unsigned hit_inst = 0xffffffffu;
     R hu = 0, hv = 0;
     unsigned octant;
     uint64_t node_calls = 0, surf_tests = 0;
@@ -464,10 +494,41 @@ template<class R> struct Accel {
       if (me.gen == tl_mailbox.gen && me.id == p) return false;
 #endif
       surf_tests++;
+      if (acc->insts && p >= acc->n_prims) {
+        // an instance: the ray in model space (t kept), the model's own
+        // search, whose node calls and tests count with this one's
+        const unsigned i = p - acc->n_prims;
+        const Inst &I = (*acc->insts)[i];
+        Ray<R> r2;
+        r2.origin = xform_pos(I.w2l, ray->origin);
+        r2.dir = xform_vec(I.w2l, ray->dir);
+        r2.t0 = ray->t0;
+        r2.t1 = ray->t1;
+        Search s2;
+        s2.acc = acc->models[I.model];
+        s2.ray = &r2;
+        s2.any = any;
+        s2.acc->search(s2);
+        node_calls += s2.node_calls;
+        surf_tests += s2.surf_tests;
+#ifdef ORC_MAILBOX
+        // the model's surfaces share the table's keys: start afresh
+        mailbox_new_generation();
+#endif
+        if (s2.hit == 0xffffffffu) return false;
+        hit = s2.hit;
+        hit_inst = i;
+        if (any) { stop = true; return true; }
+        ray->t1 = r2.t1;
+        hu = s2.hu;
+        hv = s2.hv;
+        return true;
+      }
       if (acc->test(p, *ray, t, u, v)) {
-        if (any) { stop = true; hit = p; return true; }
+        if (any) { stop = true; hit = p; hit_inst = 0xffffffffu; return true; }
         ray->t1 = t;
         hit = p;
+        hit_inst = 0xffffffffu;
         hu = u;
         hv = v;
         return true;
@@ -519,10 +580,7 @@ template<class R> struct Accel {
   void search(Search &ss) const {
     Ray<R> &ray = *ss.ray;
 #ifdef ORC_MAILBOX
-    if (++tl_mailbox.gen == 0) {  // generation wrapped: clear the table
-      for (auto &e : tl_mailbox.e) e.gen = 0;
-      tl_mailbox.gen = 1;
-    }
+    mailbox_new_generation();
 #endif
     if (use_triv) {
       for (unsigned p : triv_order) {
@@ -551,7 +609,7 @@ template<class R> struct Accel {
 
   // Closest hit; updates ray.t1 (space/space.cc:55-66).
   unsigned closest(Ray<R> &ray, R &u, R &v, uint64_t *nc = nullptr,
-                   uint64_t *st = nullptr) const {
+                   uint64_t *st = nullptr, unsigned *inst = nullptr) const {
     Search ss;
     ss.acc = this;
     ss.ray = &ray;
@@ -561,6 +619,7 @@ template<class R> struct Accel {
     v = ss.hv;
     if (nc) *nc += ss.node_calls;
     if (st) *st += ss.surf_tests;
+    if (inst) *inst = ss.hit_inst;
     return ss.hit;
   }
 

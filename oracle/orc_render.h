@@ -8,6 +8,8 @@
 #include "orc_geom.h"
 #include "orc_rng.h"
 #include "../include/snoggpu.h"
+#include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -528,6 +530,27 @@ template<class R> struct Light {
   C3<R> intensity;
 };
 
+// Inverse of an affine 3 x 4 transform: linear part by cofactors in
+// double, rounded to float (the scene's instances keep both).
+static inline bool invert_affine(const float *m, float *inv) {
+  const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9],
+               i = m[10];
+  const double A = e * i - f * h, B = f * g - d * i, Cc = d * h - e * g;
+  const double det = a * A + b * B + c * Cc;
+  if (!(std::isfinite(det) && det != 0.0)) return false;
+  const double r[9] = {A / det, (c * h - b * i) / det, (b * f - c * e) / det,
+                       B / det, (a * i - c * g) / det, (c * d - a * f) / det,
+                       Cc / det, (b * g - a * h) / det, (a * e - b * d) / det};
+  const double t[3] = {m[3], m[7], m[11]};
+  for (int row = 0; row < 3; row++) {
+    for (int col = 0; col < 3; col++) inv[4 * row + col] = (float)r[3 * row + col];
+    inv[4 * row + 3] = (float)-(r[3 * row] * t[0] + r[3 * row + 1] * t[1] + r[3 * row + 2] * t[2]);
+  }
+  for (int k = 0; k < 12; k++)
+    if (!std::isfinite(inv[k])) return false;
+  return true;
+}
+
 template<class R> struct Scene {
   std::vector<Prim<R>> prims;
   std::vector<int> prim_mat;
@@ -540,6 +563,9 @@ template<class R> struct Scene {
   std::vector<Envmap<R>> envs;
   Accel<R> accel;
   OctreeBuilder<R> builder;
+  // two-level instancing: models are scenes of their own (geometry only)
+  std::vector<std::unique_ptr<Scene>> models;
+  std::vector<typename Accel<R>::Inst> insts;
   BBox<R> bbox;
   R horizon = 0;
   R min_trace = R(1e-3);
@@ -641,6 +667,59 @@ template<class R> struct Scene {
       pb.push_back(sb);
       bbox.add(sb);
     }
+    // Models and their instances (surface/model.cc:26-47,
+    // surface/instance.cc:145-216): the instance's box is the model's box
+    // with its 8 corners taken to world space (geometry/bbox.cc:25-42).
+    const unsigned n_direct = (unsigned)prims.size();
+    for (unsigned mi = 0; mi < d.num_models; mi++) {
+      const sgr_model &m = d.models[mi];
+      sgr_scene_desc md;
+      std::memset(&md, 0, sizeof md);
+      md.meshes = m.meshes; md.num_meshes = m.num_meshes;
+      md.tripars = m.tripars; md.num_tripars = m.num_tripars;
+      md.spheres = m.spheres; md.num_spheres = m.num_spheres;
+      md.materials = d.materials; md.num_materials = d.num_materials;
+      md.min_trace = d.min_trace;
+      md.accel = d.accel;
+      models.emplace_back(new Scene);
+      models.back()->build(md);
+    }
+    for (unsigned i = 0; i < d.num_instances; i++) {
+      const sgr_instance &in = d.instances[i];
+      if (in.model >= d.num_models) throw std::runtime_error("instance: bad model");
+      typename Accel<R>::Inst I;
+      I.model = in.model;
+      float w2l[12];
+      if (!invert_affine(in.local_to_world, w2l)) throw std::runtime_error("instance: singular transform");
+      for (int k = 0; k < 12; k++) {
+        I.l2w[k] = R(in.local_to_world[k]);
+        I.w2l[k] = R(w2l[k]);
+      }
+      insts.push_back(I);
+      const BBox<R> &mb = models[in.model]->bbox;
+      BBox<R> ib;
+      for (int c = 0; c < 8; c++) {
+        V3<R> corner((c & 4) ? mb.max.x : mb.min.x, (c & 2) ? mb.max.y : mb.min.y,
+                     (c & 1) ? mb.max.z : mb.min.z);
+        V3<R> w = xform_pos(I.l2w, corner);
+        if (c == 0) ib = BBox<R>(w);
+        else ib.add(w);
+      }
+      Prim<R> p;
+      p.kind = 3;
+      p.parallelogram = false;
+      p.radius = 0;
+      prims.push_back(p);
+      prim_mat.push_back(-1);
+      prim_mesh.push_back(-1);
+      pb.push_back(ib);
+      bbox.add(ib);
+    }
+    accel.n_prims = n_direct;
+    if (!insts.empty()) {
+      accel.insts = &insts;
+      for (auto &m : models) accel.models.push_back(&m->accel);
+    }
     // Acceleration structure; degenerate mesh triangles are not inserted
     // (surface/mesh.cc:832-846).
     accel.prims = &prims;
@@ -718,6 +797,27 @@ template<class R> struct Scene {
     s = s.unit();
     V3<R> t = cross(s, norm);
     return Frame<R>(point, s, t, norm);
+  }
+
+  // The hit of instance INST (0xffffffff: none): the model's Isec with
+  // the model-space ray, then its normal frame to world space -- origin
+  // and tangents by the transform, the normal by the inverse transpose,
+  // each renormalised (surface/instance.cc:87-103).
+  void make_isec(unsigned p, unsigned inst, const Ray<R> &ray, R hu, R hv, R media_ior,
+                 R enclosing_ior, Isec<R> &is) const {
+    if (inst == 0xffffffffu) {
+      make_isec(p, ray, hu, hv, media_ior, enclosing_ior, is);
+      return;
+    }
+    const typename Accel<R>::Inst &I = insts[inst];
+    Ray<R> r2 = ray;
+    r2.origin = xform_pos(I.w2l, ray.origin);
+    r2.dir = xform_vec(I.w2l, ray.dir);
+    models[I.model]->make_isec(p, r2, hu, hv, media_ior, enclosing_ior, is);
+    is.nf.origin = xform_pos(I.l2w, is.nf.origin);
+    is.nf.x = xform_vec(I.l2w, is.nf.x).unit();
+    is.nf.y = xform_vec(I.l2w, is.nf.y).unit();
+    is.nf.z = xform_transpose(I.w2l, is.nf.z).unit();
   }
 
   void make_isec(unsigned p, const Ray<R> &ray, R hu, R hv, R media_ior,

@@ -64,7 +64,19 @@ class SceneDescC(C.Structure):
                 ("materials", C.POINTER(Material)),
                 ("num_materials", C.c_uint32),
                 ("envmaps", C.POINTER(Envmap)), ("num_envmaps", C.c_uint32),
-                ("min_trace", C.c_float), ("accel", C.c_int32)]
+                ("min_trace", C.c_float), ("accel", C.c_int32),
+                ("models", C.c_void_p), ("num_models", C.c_uint32),
+                ("instances", C.c_void_p), ("num_instances", C.c_uint32)]
+
+
+class Model(C.Structure):
+    _fields_ = [("meshes", C.POINTER(Mesh)), ("num_meshes", C.c_uint32),
+                ("tripars", C.POINTER(Tripar)), ("num_tripars", C.c_uint32),
+                ("spheres", C.POINTER(Sphere)), ("num_spheres", C.c_uint32)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("model", C.c_uint32), ("local_to_world", C.c_float * 12)]
 
 
 class Rays(C.Structure):
@@ -73,7 +85,7 @@ class Rays(C.Structure):
 
 
 class Hits(C.Structure):
-    _fields_ = [("prim", u32p), ("t", f32p), ("u", f32p), ("v", f32p)]
+    _fields_ = [("prim", u32p), ("t", f32p), ("u", f32p), ("v", f32p), ("inst", u32p)]
 
 
 class Camera(C.Structure):
@@ -210,4 +222,6 @@ class HitBuffer:
         self.t = np.zeros(n, np.float32)
         self.u = np.zeros(n, np.float32)
         self.v = np.zeros(n, np.float32)
-        self.c = Hits(uptr(self.prim), fptr(self.t), fptr(self.u), fptr(self.v))
+        self.inst = np.zeros(n, np.uint32)
+        self.c = Hits(uptr(self.prim), fptr(self.t), fptr(self.u), fptr(self.v),
+                      uptr(self.inst))

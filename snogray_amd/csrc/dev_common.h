@@ -50,6 +50,14 @@ struct DevMesh {
 #define SGR_ENV_GUIDE_ROW 256
 #define SGR_ENV_GUIDE_COL 32
 
+// An instance of a model (surface/instance.cc): 3 x 4 row-major affine
+// transforms, local to world and its inverse.
+struct DevInstance {
+  uint32_t model;
+  float l2w[12];
+  float w2l[12];
+};
+
 struct SceneView {
   const uint4 *nodes;        // first_child, mask, surf_begin, surf_count
   const uint32_t *surfs;     // primitive ids (host export only)
@@ -73,6 +81,12 @@ struct SceneView {
   // Walk with lane refill (deep octrees; camera rays excepted) or one ray
   // per lane at a time.
   uint32_t walk_refill;
+  // Two-level instancing: the models' own views (geometry and octree in
+  // model space; materials shared with this scene) and the instances,
+  // whose primitive ids in this scene's surface lists are n_prims + i.
+  const SceneView *models;
+  const DevInstance *insts;
+  uint32_t n_insts;
   uint32_t n_tri_like;       // triangles + tripars
   uint32_t n_tri;            // mesh triangles
   uint32_t n_prims;
@@ -117,6 +131,24 @@ __device__ __forceinline__ float clamp01f(float v) {
   return v > 0.0f ? v : 0.0f;
 }
 __device__ __forceinline__ V3 ld3(const float4 &f) { return V3{f.x, f.y, f.z}; }
+
+// A 3 x 4 row-major affine transform M applied to a position, a vector
+// (geometry/pos.h:80-97, geometry/vec.h:81-95) and, transposed, to a
+// vector (geometry/xform.h:295-309, the normal transform when M is the
+// inverse).
+__device__ __forceinline__ V3 xform_pos(const float *m, V3 p) {
+  return V3{p.x * m[0] + p.y * m[1] + p.z * m[2] + m[3],
+            p.x * m[4] + p.y * m[5] + p.z * m[6] + m[7],
+            p.x * m[8] + p.y * m[9] + p.z * m[10] + m[11]};
+}
+__device__ __forceinline__ V3 xform_vec(const float *m, V3 v) {
+  return V3{v.x * m[0] + v.y * m[1] + v.z * m[2], v.x * m[4] + v.y * m[5] + v.z * m[6],
+            v.x * m[8] + v.y * m[9] + v.z * m[10]};
+}
+__device__ __forceinline__ V3 xform_transpose(const float *m, V3 v) {
+  return V3{v.x * m[0] + v.y * m[4] + v.z * m[8], v.x * m[1] + v.y * m[5] + v.z * m[9],
+            v.x * m[2] + v.y * m[6] + v.z * m[10]};
+}
 
 struct Frame {
   V3 o, x, y, z;

@@ -58,9 +58,9 @@ __device__ __forceinline__ Frame mesh_frame(const DevMesh &M, V3 point, V3 norm)
 
 // Hit frames: surface/mesh.cc:563-709, surface/tripar.cc:45-66,
 // surface/sphere.cc make_intersect; then Intersect::finish_init.
-__device__ __forceinline__ void make_isec(const SceneView &S, uint32_t p, const RayF &ray,
-                                          float hu, float hv, float media_ior, float encl_ior,
-                                          Isec &is) {
+__device__ __forceinline__ void make_isec_local(const SceneView &S, uint32_t p, const RayF &ray,
+                                                float hu, float hv, float media_ior,
+                                                float encl_ior, Isec &is) {
   V3 point = ray.o + ray.d * ray.t1;
   Frame nf, gf;
   if (p < S.n_tri_like) {
@@ -121,6 +121,32 @@ __device__ __forceinline__ void make_isec(const SceneView &S, uint32_t p, const 
   is.bmat = S.mats[is.mat].bsdf;
   is.media_ior = media_ior;
   is.encl_ior = encl_ior;
+}
+
+// The hit's Isec; INST (SGR_MISS: none) names the instance whose model
+// holds primitive P: the intersection is made in model space with the
+// model-space ray, then its normal frame is taken to world space --
+// origin and tangents by the transform, the normal by the inverse
+// transpose, each renormalised (Instance::Renderable::IsecInfo::
+// make_intersect, surface/instance.cc:87-103).
+__device__ __forceinline__ void make_isec(const SceneView &S, uint32_t p, uint32_t inst,
+                                          const RayF &ray, float hu, float hv, float media_ior,
+                                          float encl_ior, Isec &is) {
+  if (inst == 0xffffffffu) {
+    make_isec_local(S, p, ray, hu, hv, media_ior, encl_ior, is);
+    return;
+  }
+  const DevInstance &I = S.insts[inst];
+  RayF r2;
+  r2.o = xform_pos(I.w2l, ray.o);
+  r2.d = xform_vec(I.w2l, ray.d);
+  r2.t0 = ray.t0;
+  r2.t1 = ray.t1;
+  make_isec_local(S.models[I.model], p, r2, hu, hv, media_ior, encl_ior, is);
+  is.nf.o = xform_pos(I.l2w, is.nf.o);
+  is.nf.x = unit(xform_vec(I.l2w, is.nf.x));
+  is.nf.y = unit(xform_vec(I.l2w, is.nf.y));
+  is.nf.z = unit(xform_transpose(I.w2l, is.nf.z));
 }
 
 __device__ __forceinline__ float cos_geom(const Isec &is, V3 d) { return dot(d, is.gn); }

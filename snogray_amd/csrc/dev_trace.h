@@ -165,6 +165,7 @@ static_assert(SGR_MAX_DEPTH <= 2 * SGR_TRAIL_LO, "the trail holds 2 x 21 levels"
 // eight children, so the lanes of a wave stay on the same instructions.
 struct TravState {
   RayF ray;
+  uint32_t hinst;     // instance of the hit (SGR_MISS: a surface of the scene itself)
   uint32_t oct, cn, first, mask, vis;
   float bx0, bx1, by0, by1, bz0, bz1;
   uint64_t tr0, tr1;  // trail: child taken at levels 0-20, 21-41
@@ -208,6 +209,7 @@ __device__ __forceinline__ void child_bounds(uint32_t child, float &bx0, float &
 
 __device__ __forceinline__ bool trav_init(const SceneView &S, TravState &T, TraceCount &cnt) {
   T.hit = SGR_MISS;
+  T.hinst = SGR_MISS;
   T.hu = 0.0f;
   T.hv = 0.0f;
   if (S.n_nodes == 0) return false;
@@ -282,9 +284,50 @@ __device__ __forceinline__ void replay_bounds(const SceneView &S, TravState &T, 
   T.bx0 = bx0; T.bx1 = bx1; T.by0 = by0; T.by1 = by1; T.bz0 = bz0; T.bz1 = bz1;
 }
 
-template<bool ANY>
+// Stack of the walk inside an instanced model (a second LDS ring and
+// spill array: the scene's walk keeps its own entries while it waits).
+struct Nest {
+  LdsStack L;
+  uint2 *spill;
+};
+
+template<bool ANY, bool INST>
+__device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, TraceCount &cnt,
+                                          LdsStack L, uint2 *spill, Nest N);
+
+// Surface reference K of the scene: a primitive, or (INST) an instance of
+// a model: the ray is taken to model space by the inverse transform (the
+// parametric t is kept, the direction is not renormalised) and the
+// model's octree is searched (Instance::Renderable::intersect /
+// intersects, surface/instance.cc:114-146); its node calls and surface
+// tests count with the scene's, as they share the render's IsecStats.
+template<bool ANY, bool INST>
+__device__ __forceinline__ bool ref_test(const SceneView &S, uint32_t k, const RayF &r, float &t,
+                                         float &u, float &v, uint32_t &prim, uint32_t &inst,
+                                         TraceCount &cnt, Nest N) {
+  if (!INST) return ref_hit(S.refs, k, r, t, u, v, prim);
+  const float4 *rec = S.refs + SGR_REF_STRIDE * k;
+  if (__float_as_uint(rec[1].w) != 3u) return ref_hit(S.refs, k, r, t, u, v, prim);
+  const uint32_t i = __float_as_uint(rec[0].w) - S.n_prims;
+  const DevInstance &I = S.insts[i];
+  TravState T2;
+  T2.ray.o = xform_pos(I.w2l, r.o);
+  T2.ray.d = xform_vec(I.w2l, r.d);
+  T2.ray.t0 = r.t0;
+  T2.ray.t1 = r.t1;
+  trace_one<ANY, false>(S.models[I.model], T2, cnt, N.L, N.spill, N);
+  if (T2.hit == SGR_MISS) return false;
+  t = T2.ray.t1;
+  u = T2.hu;
+  v = T2.hv;
+  prim = T2.hit;
+  inst = i;
+  return true;
+}
+
+template<bool ANY, bool INST>
 __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, TraceCount &cnt,
-                                          LdsStack L, uint2 *spill) {
+                                          LdsStack L, uint2 *spill, Nest N) {
   if (T.enter) {
     float t, u, v;
     uint32_t prim;
@@ -292,8 +335,10 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
     const uint4 nd = S.nodes[T.cn];
     for (uint32_t k = nd.z, e = nd.z + nd.w; k < e; k++) {
       cnt.tests++;
-      if (ref_hit(S.refs, k, T.ray, t, u, v, prim)) {
+      uint32_t inst = SGR_MISS;
+      if (ref_test<ANY, INST>(S, k, T.ray, t, u, v, prim, inst, cnt, N)) {
         T.hit = prim;
+        T.hinst = inst;
         if (ANY) return true;
         T.ray.t1 = t;
         T.hu = u;
@@ -372,19 +417,22 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
 
 // One ray from start to end: brute force over the surface list
 // (TrivSpace, space/triv-space.h:48-58) or the octree walk.
-template<bool ANY>
+template<bool ANY, bool INST>
 __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, TraceCount &cnt,
-                                          LdsStack L, uint2 *spill) {
+                                          LdsStack L, uint2 *spill, Nest N) {
   if (S.use_triv) {
     T.hit = SGR_MISS;
+    T.hinst = SGR_MISS;
     T.hu = 0.0f;
     T.hv = 0.0f;
     float t, u, v;
     uint32_t prim;
     for (uint32_t q = 0; q < S.n_surfs; q++) {
       cnt.tests++;
-      if (ref_hit(S.refs, q, T.ray, t, u, v, prim)) {
+      uint32_t inst = SGR_MISS;
+      if (ref_test<ANY, INST>(S, q, T.ray, t, u, v, prim, inst, cnt, N)) {
         T.hit = prim;
+        T.hinst = inst;
         if (ANY) break;
         T.ray.t1 = t;
         T.hu = u;
@@ -392,7 +440,7 @@ __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, Trac
       }
     }
   } else if (trav_init(S, T, cnt)) {
-    while (!trav_step<ANY>(S, T, cnt, L, spill)) {
+    while (!trav_step<ANY, INST>(S, T, cnt, L, spill, N)) {
     }
   }
 }
@@ -403,19 +451,23 @@ __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, Trac
 // are balanced anyway, and for camera rays, whose lanes (the samples of
 // one or two pixels) walk the same nodes in step, so their loads
 // coalesce.
-template<bool ANY, bool MULTI, typename Get, typename Put>
+// INST: the scene may hold instances (N is the stack of the walk inside
+// a model).
+template<bool ANY, bool MULTI, bool INST, typename Get, typename Put>
 __device__ __forceinline__ void trace_plain(const SceneView &S1, uint32_t jb, uint32_t je,
-                                            Get get, Put put, TraceCount &cnt, LdsStack L) {
+                                            Get get, Put put, TraceCount &cnt, LdsStack L,
+                                            Nest N) {
   TravState T;
   uint2 spill[SGR_MAX_DEPTH];
   for (uint32_t i = jb + threadIdx.x; i < je; i += blockDim.x) {
     const SceneView *g = get(i, T.ray);
-    trace_one<ANY>(MULTI ? *g : S1, T, cnt, L, spill);
+    trace_one<ANY, INST>(MULTI ? *g : S1, T, cnt, L, spill, N);
     put(i, T);
   }
 }
 
-// Lane-refill driver: every lane of a wave walks its own ray; when at
+// Lane-refill driver (scenes without instances): every lane of a wave
+// walks its own ray; when at
 // least SGR_REFILL lanes are idle the wave takes that many more rays with
 // one atomic on a slice head (a wave-wide ballot ranks the idle lanes),
 // so no lane idles for long while its wave still has rays to finish.
@@ -469,6 +521,7 @@ __device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t n, ui
           if (S.use_triv) {
             // brute force (TrivSpace, space/triv-space.h:48-58), in one go
             T.hit = SGR_MISS;
+            T.hinst = SGR_MISS;
             T.hu = 0.0f;
             T.hv = 0.0f;
             float t, u, v;
@@ -492,7 +545,7 @@ __device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t n, ui
         }
       }
     }
-    if (active && trav_step<ANY>(MULTI ? *Sp : S1, T, cnt, L, spill)) {
+    if (active && trav_step<ANY, false>(MULTI ? *Sp : S1, T, cnt, L, spill, Nest{})) {
       put(id, T);
       active = false;
     }

@@ -60,6 +60,7 @@ struct RayBatchDev {
   const float *ox, *oy, *oz, *dx, *dy, *dz, *t0, *t1;
   uint32_t *prim;
   float *t, *u, *v;
+  uint32_t *inst;   // may be null
   uint8_t *occ;
 };
 
@@ -98,6 +99,7 @@ struct PathBufs {
   float *ray[8];      // ox oy oz dx dy dz t0 t1
   uint32_t *hprim;    // closest hit of queue position j of the current bounce
   float *ht, *hu, *hv;
+  uint32_t *hinst;    // ... its instance (instanced scenes)
   float *T[3], *rad[3], *Tp[3];
   float *rr, *alpha;
   uint32_t *state;

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -259,7 +260,38 @@ struct HostScene {
   FlatOctree oct;
   SceneView V{};
   float blo[3], bhi[3];
+  // two-level instancing: the models (prepared as scenes of their own)
+  // and the instances placing them
+  std::vector<std::unique_ptr<HostScene>> models;
+  std::vector<DevInstance> insts;
 };
+
+// Inverse of the affine 3 x 4 transform M (linear part by cofactors, in
+// double, then rounded): false if it is singular or not finite.
+bool invert_affine(const float *m, float *inv) {
+  const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9],
+               i = m[10];
+  const double A = e * i - f * h, B = f * g - d * i, Cc = d * h - e * g;
+  const double det = a * A + b * B + c * Cc;
+  if (!(std::isfinite(det) && det != 0.0)) return false;
+  const double r[9] = {A / det, (c * h - b * i) / det, (b * f - c * e) / det,
+                       B / det, (a * i - c * g) / det, (c * d - a * f) / det,
+                       Cc / det, (b * g - a * h) / det, (a * e - b * d) / det};
+  const double t[3] = {m[3], m[7], m[11]};
+  for (int row = 0; row < 3; row++) {
+    for (int col = 0; col < 3; col++) inv[4 * row + col] = (float)r[3 * row + col];
+    inv[4 * row + 3] = (float)-(r[3 * row] * t[0] + r[3 * row + 1] * t[1] + r[3 * row + 2] * t[2]);
+  }
+  for (int k = 0; k < 12; k++)
+    if (!std::isfinite(inv[k])) return false;
+  return true;
+}
+
+// Pos::transformed (geometry/pos.h:80-97) in float.
+void xform_point(const float *m, const float *p, float *o) {
+  for (int r = 0; r < 3; r++)
+    o[r] = p[0] * m[4 * r] + p[1] * m[4 * r + 1] + p[2] * m[4 * r + 2] + m[4 * r + 3];
+}
 
 int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   if (!d) return fail(SGR_ERR_INVALID, "sgr_scene_create: null descriptor");
@@ -327,7 +359,29 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   prim_mesh.assign(n_prims, 0xffffffffu);
   std::vector<DevMesh> &meshes = H.meshes;
   meshes.resize(d->num_meshes);
-  std::vector<Box3> boxes(n_prims);
+  // ---- models (their own geometry and octree, the scene's materials) ----
+  if (d->num_instances && !d->instances) return fail(SGR_ERR_INVALID, "sgr_scene_create: null instances");
+  if (d->num_models && !d->models) return fail(SGR_ERR_INVALID, "sgr_scene_create: null models");
+  for (uint32_t mi = 0; mi < d->num_models; mi++) {
+    const sgr_model &m = d->models[mi];
+    sgr_scene_desc md;
+    std::memset(&md, 0, sizeof md);
+    md.meshes = m.meshes; md.num_meshes = m.num_meshes;
+    md.tripars = m.tripars; md.num_tripars = m.num_tripars;
+    md.spheres = m.spheres; md.num_spheres = m.num_spheres;
+    md.materials = d->materials; md.num_materials = nm;
+    md.min_trace = d->min_trace;
+    md.accel = d->accel;
+    H.models.emplace_back(new HostScene);
+    int rc = prepare_scene(&md, *H.models.back());
+    if (rc) return fail(rc, "model " + std::to_string(mi) + ": " + g_err);
+    if (H.models.back()->order.empty())
+      return fail(SGR_ERR_INVALID, "model " + std::to_string(mi) + " has no surfaces");
+  }
+  const uint32_t n_inst = d->num_instances;
+  if ((uint64_t)n_prims + n_inst >= (1ull << 28))
+    return fail(SGR_ERR_LIMIT, "sgr_scene_create: more than 2^28 primitives and instances");
+  std::vector<Box3> boxes(n_prims + n_inst);
   order.reserve(n_prims);
   float *blo = H.blo, *bhi = H.bhi;
   for (int a = 0; a < 3; a++) {
@@ -439,6 +493,33 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     grow(bx.lo, bx.hi);
     order.push_back(p);
   }
+  // ---- instances (surface/instance.cc:145-216): the model's box, its 8
+  // corners taken to world space (geometry/bbox.cc:25-42) ----
+  for (uint32_t i = 0; i < n_inst; i++) {
+    const sgr_instance &in = d->instances[i];
+    if (in.model >= d->num_models)
+      return fail(SGR_ERR_INVALID, "instance " + std::to_string(i) + ": bad model");
+    DevInstance I;
+    I.model = in.model;
+    std::memcpy(I.l2w, in.local_to_world, sizeof I.l2w);
+    if (!invert_affine(I.l2w, I.w2l))
+      return fail(SGR_ERR_INVALID, "instance " + std::to_string(i) + ": singular transform");
+    H.insts.push_back(I);
+    const HostScene &M = *H.models[in.model];
+    Box3 &bx = boxes[n_prims + i];
+    for (int c = 0; c < 8; c++) {
+      const float corner[3] = {(c & 4) ? M.bhi[0] : M.blo[0], (c & 2) ? M.bhi[1] : M.blo[1],
+                               (c & 1) ? M.bhi[2] : M.blo[2]};
+      float w[3];
+      xform_point(I.l2w, corner, w);
+      for (int k = 0; k < 3; k++) {
+        bx.lo[k] = c ? fmin_t(bx.lo[k], w[k]) : w[k];
+        bx.hi[k] = c ? fmax_t(bx.hi[k], w[k]) : w[k];
+      }
+    }
+    grow(bx.lo, bx.hi);
+    order.push_back((uint32_t)n_prims + i);
+  }
   // A surface whose box is not finite would grow the octree root without
   // end (space/octree-builder.cc:180-196 doubles it until the box fits).
   for (uint32_t q : order)
@@ -477,7 +558,7 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   V.min_trace = d->min_trace;
   {  // render/scene.cc:28 horizon = bbox diameter
     float ex = bhi[0] - blo[0], ey = bhi[1] - blo[1], ez = bhi[2] - blo[2];
-    V.horizon = p ? std::sqrt(ex * ex + ey * ey + ez * ez) : 0.0f;
+    V.horizon = (p || n_inst) ? std::sqrt(ex * ex + ey * ey + ez * ez) : 0.0f;
   }
   for (int a = 0; a < 3; a++) {
     V.bbox_lo[a] = blo[a];
@@ -517,6 +598,7 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     H.lights.push_back(L);
   }
   V.n_lights = (uint32_t)H.lights.size();
+  V.n_insts = n_inst;
   return SGR_OK;
 }
 
@@ -557,6 +639,54 @@ int sgr_octree_build(const sgr_scene_desc *d, uint32_t *nodes, uint64_t *num_nod
   });
 }
 
+// Device copies of a prepared scene's (or model's) geometry: octree
+// nodes, surface references with their geometry inline in list order (a
+// node's surfaces contiguous; kind in the second record's w: 0 triangle,
+// 1 parallelogram, 2 sphere, 3 instance), primitives and meshes.
+static int upload_geometry(sgr_ctx *ctx, sgr_scene *sc, hipStream_t up, const HostScene &H,
+                           const FlatOctree &oct, SceneView &V) {
+  std::vector<uint4> nodes4(V.n_nodes);
+  for (uint32_t i = 0; i < V.n_nodes; i++)
+    nodes4[i] = make_uint4(oct.nodes[4 * i], oct.nodes[4 * i + 1], oct.nodes[4 * i + 2],
+                           oct.nodes[4 * i + 3]);
+  const std::vector<uint32_t> &surfs = V.use_triv ? H.order : oct.surf_ids;
+  std::vector<float4> refs(surfs.size() * SGR_REF_STRIDE, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  for (size_t k = 0; k < surfs.size(); k++) {
+    uint32_t q = surfs[k];
+    float pb, kind;
+    std::memcpy(&pb, &q, 4);
+    uint32_t kd = 0;
+    if (q < V.n_tri_like) {
+      const float4 &a = H.tri[3 * q], &b = H.tri[3 * q + 1], &c = H.tri[3 * q + 2];
+      kd = a.w != 0.0f ? 1u : 0u;
+      std::memcpy(&kind, &kd, 4);
+      refs[SGR_REF_STRIDE * k] = make_float4(a.x, a.y, a.z, pb);
+      refs[SGR_REF_STRIDE * k + 1] = make_float4(b.x, b.y, b.z, kind);
+      refs[SGR_REF_STRIDE * k + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+    } else if (q < V.n_prims) {
+      const float4 &sp = H.sph[q - V.n_tri_like];
+      kd = 2u;
+      std::memcpy(&kind, &kd, 4);
+      refs[SGR_REF_STRIDE * k] = make_float4(sp.x, sp.y, sp.z, pb);
+      refs[SGR_REF_STRIDE * k + 1] = make_float4(sp.w, 0.0f, 0.0f, kind);
+    } else {  // instance n_prims + i
+      kd = 3u;
+      std::memcpy(&kind, &kd, 4);
+      refs[SGR_REF_STRIDE * k] = make_float4(0.0f, 0.0f, 0.0f, pb);
+      refs[SGR_REF_STRIDE * k + 1] = make_float4(0.0f, 0.0f, 0.0f, kind);
+    }
+  }
+  int rc = upload(ctx, sc, up, nodes4, &V.nodes);
+  if (!rc) rc = upload(ctx, sc, up, refs, &V.refs);
+  if (!rc) rc = upload(ctx, sc, up, H.tri, &V.tri);
+  if (!rc) rc = upload(ctx, sc, up, H.sph, &V.sph);
+  if (!rc && !H.tri_vn.empty()) rc = upload(ctx, sc, up, H.tri_vn, &V.tri_vn);
+  if (!rc) rc = upload(ctx, sc, up, H.prim_mat, &V.prim_mat);
+  if (!rc) rc = upload(ctx, sc, up, H.prim_mesh, &V.prim_mesh);
+  if (!rc) rc = upload(ctx, sc, up, H.meshes, &V.meshes);
+  return rc;
+}
+
 static int scene_create_impl(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **out,
                              sgr_scene *&made) {
   if (!ctx || !d || !out) return fail(SGR_ERR_INVALID, "sgr_scene_create: null argument");
@@ -580,36 +710,7 @@ static int scene_create_impl(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **
   // unbalanced: on the 1.31 M-triangle soup (depth 19) it is 16 % faster,
   // on the Cornell box (depth 5) 30 % slower than one ray per lane.
   // Camera rays take the plain walk on every scene (launch_extend).
-  V.walk_refill = sc->oct.max_depth >= 8 ? 1u : 0u;
-  std::vector<uint4> nodes4(V.n_nodes);
-  for (uint32_t i = 0; i < V.n_nodes; i++)
-    nodes4[i] = make_uint4(sc->oct.nodes[4 * i], sc->oct.nodes[4 * i + 1],
-                           sc->oct.nodes[4 * i + 2], sc->oct.nodes[4 * i + 3]);
-  const std::vector<uint32_t> &surfs = V.use_triv ? H.order : sc->oct.surf_ids;
-  // Surface references in list order with their geometry inline, so a
-  // node's surfaces are contiguous (kind in the second record's w).
-  std::vector<float4> refs(surfs.size() * SGR_REF_STRIDE, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
-  for (size_t k = 0; k < surfs.size(); k++) {
-    uint32_t q = surfs[k];
-    float pb, kind;
-    std::memcpy(&pb, &q, 4);
-    uint32_t kd = 0;
-    if (q < V.n_tri_like) {
-      const float4 &a = H.tri[3 * q], &b = H.tri[3 * q + 1], &c = H.tri[3 * q + 2];
-      kd = a.w != 0.0f ? 1u : 0u;
-      std::memcpy(&kind, &kd, 4);
-      refs[SGR_REF_STRIDE * k] = make_float4(a.x, a.y, a.z, pb);
-      refs[SGR_REF_STRIDE * k + 1] = make_float4(b.x, b.y, b.z, kind);
-      refs[SGR_REF_STRIDE * k + 2] = make_float4(c.x, c.y, c.z, 0.0f);
-    } else {
-      const float4 &sp = H.sph[q - V.n_tri_like];
-      kd = 2u;
-      std::memcpy(&kind, &kd, 4);
-      refs[SGR_REF_STRIDE * k] = make_float4(sp.x, sp.y, sp.z, pb);
-      refs[SGR_REF_STRIDE * k + 1] = make_float4(sp.w, 0.0f, 0.0f, kind);
-      refs[SGR_REF_STRIDE * k + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-  }
+  V.walk_refill = sc->oct.max_depth >= 8 && V.n_insts == 0 ? 1u : 0u;
   // Uploads go through a stream of their own: a scene may be created by
   // one host thread while another drives renders on the context's stream.
   hipStream_t up = nullptr;
@@ -617,15 +718,24 @@ static int scene_create_impl(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **
     hipError_t e = hipStreamCreateWithFlags(&up, hipStreamNonBlocking);
     if (e != hipSuccess) rc = fail(SGR_ERR_HIP, std::string("upload stream: ") + hipGetErrorString(e));
   }
-  if (!rc) rc = upload(ctx, sc, up, nodes4, &V.nodes);
-  if (!rc) rc = upload(ctx, sc, up, refs, &V.refs);
-  if (!rc) rc = upload(ctx, sc, up, H.tri, &V.tri);
-  if (!rc) rc = upload(ctx, sc, up, H.sph, &V.sph);
-  if (!rc && !H.tri_vn.empty()) rc = upload(ctx, sc, up, H.tri_vn, &V.tri_vn);
-  if (!rc) rc = upload(ctx, sc, up, H.prim_mat, &V.prim_mat);
-  if (!rc) rc = upload(ctx, sc, up, H.prim_mesh, &V.prim_mesh);
-  if (!rc) rc = upload(ctx, sc, up, H.meshes, &V.meshes);
+  if (!rc) rc = upload_geometry(ctx, sc, up, H, sc->oct, V);
   if (!rc) rc = upload(ctx, sc, up, H.mats, &V.mats);
+  if (!rc && !H.models.empty()) {
+    // the models' own views: geometry and octree in model space, the
+    // scene's materials, no lights
+    std::vector<SceneView> mv(H.models.size());
+    for (size_t m = 0; m < H.models.size() && !rc; m++) {
+      HostScene &Hm = *H.models[m];
+      mv[m] = Hm.V;
+      rc = upload_geometry(ctx, sc, up, Hm, Hm.oct, mv[m]);
+      mv[m].mats = V.mats;
+      mv[m].walk_refill = 0;
+      mv[m].n_lights = 0;
+      mv[m].has_env = 0;
+    }
+    if (!rc) rc = upload(ctx, sc, up, mv, &V.models);
+    if (!rc) rc = upload(ctx, sc, up, H.insts, &V.insts);
+  }
   if (!rc) rc = upload(ctx, sc, up, H.lights, &V.lights);
   if (!rc) rc = upload(ctx, sc, up, H.env_tex, &V.env_tex);
   if (!rc) rc = upload(ctx, sc, up, H.row_cum, &V.env_row_cum);
@@ -725,7 +835,7 @@ static int trace_common(sgr_scene *sc, const sgr_rays *rays, size_t n, sgr_hits 
   sgr_ctx *ctx = sc->ctx;
   HIPCHK(hipSetDevice(ctx->device));
   size_t in_bytes = n * 8 * sizeof(float);
-  size_t out_bytes = hits ? n * 16 : n;
+  size_t out_bytes = hits ? n * 20 : n;
   void *buf = nullptr;
   HIPCHK(hipMalloc(&buf, in_bytes + out_bytes));
   float *f = (float *)buf;
@@ -743,6 +853,7 @@ static int trace_common(sgr_scene *sc, const sgr_rays *rays, size_t n, sgr_hits 
   rb.t = (float *)(ob + n);
   rb.u = (float *)(ob + 2 * n);
   rb.v = (float *)(ob + 3 * n);
+  rb.inst = hits ? ob + 4 * n : nullptr;
   rb.occ = (uint8_t *)ob;
   if (e == hipSuccess) {
     hipEventRecord(ctx->ev[0], ctx->stream);
@@ -756,6 +867,8 @@ static int trace_common(sgr_scene *sc, const sgr_rays *rays, size_t n, sgr_hits 
       if (e == hipSuccess) e = hipMemcpyAsync(hits->t, rb.t, n * 4, hipMemcpyDeviceToHost, ctx->stream);
       if (e == hipSuccess) e = hipMemcpyAsync(hits->u, rb.u, n * 4, hipMemcpyDeviceToHost, ctx->stream);
       if (e == hipSuccess) e = hipMemcpyAsync(hits->v, rb.v, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess && hits->inst)
+        e = hipMemcpyAsync(hits->inst, rb.inst, n * 4, hipMemcpyDeviceToHost, ctx->stream);
     } else {
       e = hipMemcpyAsync(occ, rb.occ, n, hipMemcpyDeviceToHost, ctx->stream);
     }
@@ -828,6 +941,7 @@ size_t path_bytes(size_t cap, size_t K, PathBufs *P, char *base) {
   r = take(cap * 4); if (P) P->ht = (float *)r;
   r = take(cap * 4); if (P) P->hu = (float *)r;
   r = take(cap * 4); if (P) P->hv = (float *)r;
+  r = take(cap * 4); if (P) P->hinst = (uint32_t *)r;
   for (int k = 0; k < 3; k++) { r = take(cap * 4); if (P) P->T[k] = (float *)r; }
   for (int k = 0; k < 3; k++) { r = take(cap * 4); if (P) P->rad[k] = (float *)r; }
   for (int k = 0; k < 3; k++) { r = take(cap * 4); if (P) P->Tp[k] = (float *)r; }

@@ -62,7 +62,7 @@ struct Engine {
   StepTables *dev = nullptr;
   unsigned long long *stats = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  std::map<uint32_t, hipGraphExec_t> graphs;  // by bounce count | refill << 31
+  std::map<uint32_t, hipGraphExec_t> graphs;  // by bounce count | refill << 31 | inst << 30
   std::map<uint64_t, Job> jobs;
   std::deque<uint64_t> queued;   // admission order
   std::vector<uint64_t> active;  // round-robin order
@@ -159,7 +159,8 @@ uint32_t splat_tiles(const RenderView &R) {
   return (uint32_t)((ow + 15) / 16) * (uint32_t)((oh + 15) / 16);
 }
 
-void enqueue_step(sgr_ctx *ctx, Engine *E, uint32_t maxp, bool refill, unsigned grid) {
+void enqueue_step(sgr_ctx *ctx, Engine *E, uint32_t maxp, bool refill, bool inst,
+                  unsigned grid) {
   StepView J;
   J.R = E->dev->R;
   J.S = E->dev->S;
@@ -170,7 +171,10 @@ void enqueue_step(sgr_ctx *ctx, Engine *E, uint32_t maxp, bool refill, unsigned 
   std::memset(&R0, 0, sizeof R0);
   SceneView S0;
   std::memset(&S0, 0, sizeof S0);
-  S0.walk_refill = refill ? 1u : 0u;  // selects the walk of the multi-scene kernels
+  // select the walk of the multi-scene kernels: instanced scenes take the
+  // plain walk with a stack for the models
+  S0.walk_refill = refill && !inst ? 1u : 0u;
+  S0.n_insts = inst ? 1u : 0u;
   const PathBufs &P = E->P;
   hipStream_t st = ctx->stream;
   launch_raygen(true, R0, J, P, grid, st);
@@ -359,6 +363,7 @@ static int sgr_engine_step_impl(sgr_ctx *ctx, sgr_step_info *info) {
   size_t K = 1;
   uint32_t maxp = 0;
   bool refill = false;  // any scene of the step deep enough for lane refill
+  bool inst = false;    // any scene of the step with instances
   size_t n = E->active.size();
   size_t stop = n;
   for (size_t k = 0; k < n && run.size() < SGR_MAX_SEGMENTS; k++) {
@@ -376,6 +381,7 @@ static int sgr_engine_step_impl(sgr_ctx *ctx, sgr_step_info *info) {
     K = std::max<size_t>(K, j.pl.R.K);
     maxp = std::max(maxp, j.pl.R.maxp);
     refill = refill || j.sc->view.walk_refill;
+    inst = inst || j.sc->view.n_insts > 0;
   }
   if (run.empty()) return SGR_OK;
   // next step starts with the first job that did not fit, else the job
@@ -418,12 +424,12 @@ static int sgr_engine_step_impl(sgr_ctx *ctx, sgr_step_info *info) {
   HIPCHK(hipEventRecord(E->e0, ctx->stream));
   bool graphed = false;
   if (E->cfg.use_graph) {
-    const uint32_t key = maxp | (refill ? 1u << 31 : 0u);
+    const uint32_t key = maxp | (refill ? 1u << 31 : 0u) | (inst ? 1u << 30 : 0u);
     auto it = E->graphs.find(key);
     if (it == E->graphs.end()) {
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_step(ctx, E, maxp, refill, grid);
+      enqueue_step(ctx, E, maxp, refill, inst, grid);
       hipError_t le = hipGetLastError();
       hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
       if (le != hipSuccess) return fail(SGR_ERR_HIP, std::string("step capture: ") + hipGetErrorString(le));
@@ -437,7 +443,7 @@ static int sgr_engine_step_impl(sgr_ctx *ctx, sgr_step_info *info) {
     HIPCHK(hipGraphLaunch(it->second, ctx->stream));
     graphed = true;
   } else {
-    enqueue_step(ctx, E, maxp, refill, grid);
+    enqueue_step(ctx, E, maxp, refill, inst, grid);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(E->e1, ctx->stream));

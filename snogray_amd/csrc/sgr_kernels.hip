@@ -45,28 +45,37 @@ __device__ __forceinline__ void block_range(uint32_t n, uint32_t &beg, uint32_t 
 
 // Walk the N rays of a launch: lane refill over XCD slices, or one ray
 // per lane over this block's queue range.
-template<bool ANY, bool MULTI, bool REFILL, typename Get, typename Put>
+// INST: scenes may hold instances (plain walk only; the walk inside a
+// model takes a second LDS stack, declared here only for such kernels).
+template<bool ANY, bool MULTI, bool REFILL, bool INST, typename Get, typename Put>
 __device__ __forceinline__ void walk(const SceneView &S1, uint32_t n, uint32_t *heads, Get get,
                                      Put put, TraceCount &cnt, LdsStack L) {
-  if (REFILL) {
+  static_assert(!(REFILL && INST), "instanced scenes take the plain walk");
+  if constexpr (REFILL) {
     trace_refill<ANY, MULTI>(S1, n, heads, get, put, cnt, L);
   } else {
     uint32_t jb, je;
     block_range(n, jb, je);
-    trace_plain<ANY, MULTI>(S1, jb, je, get, put, cnt, L);
+    if constexpr (INST) {
+      SGR_STACK_LDS(stk2);
+      uint2 spill2[SGR_MAX_DEPTH];
+      trace_plain<ANY, MULTI, true>(S1, jb, je, get, put, cnt, L, Nest{stk2, spill2});
+    } else {
+      trace_plain<ANY, MULTI, false>(S1, jb, je, get, put, cnt, L, Nest{});
+    }
   }
 }
 
 // Ray-batch queries with lane-refill traversal (heads[SGR_HEAD_STRIDE * i],
 // i < 8, zeroed by the caller, are the fetch heads of the 8 slices of the
 // batch).
-template<bool ANY, bool REFILL>
+template<bool ANY, bool REFILL, bool INST>
 __global__ void __launch_bounds__(256) k_trace(SceneView S, RayBatchDev rb, uint32_t n,
                                                unsigned long long *stats) {
   SGR_STACK_LDS(stk);
   TraceCount cnt{0, 0, 0};
   uint32_t *heads = (uint32_t *)((char *)stats + SGR_STATS_HEADS_OFF);
-  walk<ANY, false, REFILL>(
+  walk<ANY, false, REFILL, INST>(
       S, n, heads,
       [&](uint32_t i, RayF &r) -> const SceneView * {
         r.o = v3(rb.ox[i], rb.oy[i], rb.oz[i]);
@@ -83,6 +92,7 @@ __global__ void __launch_bounds__(256) k_trace(SceneView S, RayBatchDev rb, uint
           rb.t[i] = T.hit == SGR_MISS ? 0.0f : T.ray.t1;
           rb.u[i] = T.hu;
           rb.v[i] = T.hv;
+          if (rb.inst) rb.inst[i] = T.hinst;
         }
       },
       cnt, stk);
@@ -183,7 +193,7 @@ __device__ __forceinline__ bool finishing(const PathBufs &P, uint32_t b, uint32_
   return P.fin_max != 0 && b >= P.fin_from && n < P.fin_max;
 }
 
-template<bool MULTI, bool REFILL>
+template<bool MULTI, bool REFILL, bool INST>
 __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                 unsigned long long *stats) {
   SGR_STACK_LDS(stk);
@@ -191,7 +201,7 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   if (finishing(P, b, n)) return;
   const uint32_t *q = P.q[b & 1];
-  walk<false, MULTI, REFILL>(
+  walk<false, MULTI, REFILL, INST>(
       S1, n, &P.counts[SGR_CNT_HE + SGR_HEADS_PER_B * b],
       [&](uint32_t i, RayF &r) -> const SceneView * {
         const uint32_t p = q[i];
@@ -205,6 +215,7 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
         P.ht[i] = T.ray.t1;
         P.hu[i] = T.hu;
         P.hv[i] = T.hv;
+        if (INST) P.hinst[i] = T.hinst;
       },
       cnt, stk);
   wave_count(&stats[SGR_STAT_NODES_C], cnt.nodes);
@@ -212,14 +223,14 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
   wave_count(&stats[SGR_STAT_FETCH_C], cnt.fetch);
 }
 
-template<bool MULTI, bool REFILL>
+template<bool MULTI, bool REFILL, bool INST>
 __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                  unsigned long long *stats) {
   SGR_STACK_LDS(stk);
   TraceCount cnt{0, 0, 0};
   if (finishing(P, b, P.counts[SGR_CNT_Q + b])) return;
   const uint32_t n = P.counts[SGR_CNT_S + b];
-  walk<true, MULTI, REFILL>(
+  walk<true, MULTI, REFILL, INST>(
       S1, n, &P.counts[SGR_CNT_HS + SGR_HEADS_PER_B * b],
       [&](uint32_t i, RayF &r) -> const SceneView * {
         const uint32_t slot = P.sq[i];
@@ -287,7 +298,8 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
     float mencl = depth >= 2 ? P.media[(size_t)(depth - 2) * N + p] : 1.0f;
     // barycentrics only matter for interpolated vertex normals
     const float hu = S.tri_vn ? P.hu[j] : 0.0f, hv = S.tri_vn ? P.hv[j] : 0.0f;
-    make_isec(S, prim, ray, hu, hv, mtop, mencl, is);
+    const uint32_t hinst = S.n_insts ? P.hinst[j] : SGR_MISS;
+    make_isec(S, prim, hinst, ray, hu, hv, mtop, mencl, is);
     const DevMaterial &HM = S.mats[is.mat];
     if (include && HM.type == 5 && !is.back) rad = rad + cload(HM.color) * T;
     if (is.bmat < 0) {
@@ -733,7 +745,7 @@ __global__ void __launch_bounds__(256) k_gather(RenderView R1, StepView J, PathB
 // for its slowest ray, here only each path's own chain counts.  Rays and
 // shadow rays traced after bounce B are counted here (the queue counts
 // of those bounces stay zero).
-template<bool MULTI>
+template<bool MULTI, bool INST>
 __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, StepView J,
                                                 PathBufs P, uint32_t b,
                                                 unsigned long long *stats) {
@@ -741,6 +753,14 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
   if (!finishing(P, b, n)) return;
   SGR_STACK_LDS(stk);
   uint2 spill[SGR_MAX_DEPTH];
+  Nest N{};
+  if constexpr (INST) {  // stack of the walk inside an instanced model (LDS
+                         // arrays are static: they outlive the block)
+    SGR_STACK_LDS(stk2);
+    N.L = stk2;
+  }
+  uint2 spill2[INST ? SGR_MAX_DEPTH : 1];
+  N.spill = spill2;
   const uint32_t *q = P.q[b & 1];
   const GuideView G{};
   TraceCount cc{0, 0, 0}, cs{0, 0, 0};
@@ -754,11 +774,12 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
       const SceneView &S = *Sp;
       TravState T;
       load_ray(P, p, T.ray);
-      trace_one<false>(S, T, cc, stk, spill);
+      trace_one<false, INST>(S, T, cc, stk, spill, N);
       P.hprim[j] = T.hit;
       P.ht[j] = T.ray.t1;
       P.hu[j] = T.hu;
       P.hv[j] = T.hv;
+      if (INST) P.hinst[j] = T.hinst;
       uint64_t em = 0;  // shadow slots of the path written at this vertex
       const bool cont = shade_path<MULTI, false>(S1, R1, J, P, G, p, j, illum,
                                                  [&](size_t slot, uint32_t) {
@@ -771,7 +792,7 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
         U.ray.d = v3(P.sray[3][slot], P.sray[4][slot], P.sray[5][slot]);
         U.ray.t0 = P.sray[6][slot];
         U.ray.t1 = P.sray[7][slot];
-        trace_one<true>(S, U, cs, stk, spill);
+        trace_one<true, INST>(S, U, cs, stk, spill, N);
         P.svis[slot] = U.hit == SGR_MISS ? 1 : 0;
         shadow++;
       }
@@ -948,12 +969,15 @@ void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool an
                   unsigned long long *stats, hipStream_t st) {
   unsigned g = grid_for(n, 65535);
   g = (g + 7) & ~7u;  // whole blocks per XCD slice
-  if (S.walk_refill) {
-    if (any) hipLaunchKernelGGL((k_trace<true, true>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
-    else hipLaunchKernelGGL((k_trace<false, true>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
+  if (S.n_insts) {  // instanced scenes: plain walk, second stack for the models
+    if (any) hipLaunchKernelGGL((k_trace<true, false, true>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
+    else hipLaunchKernelGGL((k_trace<false, false, true>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
+  } else if (S.walk_refill) {
+    if (any) hipLaunchKernelGGL((k_trace<true, true, false>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
+    else hipLaunchKernelGGL((k_trace<false, true, false>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
   } else {
-    if (any) hipLaunchKernelGGL((k_trace<true, false>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
-    else hipLaunchKernelGGL((k_trace<false, false>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
+    if (any) hipLaunchKernelGGL((k_trace<true, false, false>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
+    else hipLaunchKernelGGL((k_trace<false, false, false>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
   }
 }
 
@@ -985,14 +1009,18 @@ void launch_extend(bool multi, const SceneView &S, const StepView &J, const Path
   // directions are random, the plain walk loses 3-9 %.
   const bool refill = S.walk_refill && b > 0;
   if (!S.walk_refill) grid = plain_walk_grid(grid);
-  if (multi && refill)
-    hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  if (S.n_insts && multi)  // instanced scenes take the plain walk
+    hipLaunchKernelGGL((k_extend<true, false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else if (S.n_insts)
+    hipLaunchKernelGGL((k_extend<false, false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else if (multi && refill)
+    hipLaunchKernelGGL((k_extend<true, true, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else if (multi)
-    hipLaunchKernelGGL((k_extend<true, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+    hipLaunchKernelGGL((k_extend<true, false, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else if (refill)
-    hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+    hipLaunchKernelGGL((k_extend<false, true, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else
-    hipLaunchKernelGGL((k_extend<false, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+    hipLaunchKernelGGL((k_extend<false, false, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
 }
 void launch_shade(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
                   const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
@@ -1027,14 +1055,18 @@ void launch_shade(bool multi, const SceneView &S, const RenderView &R, const Ste
 void launch_occlude(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                     uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
   if (!S.walk_refill) grid = plain_walk_grid(grid);
-  if (multi && S.walk_refill)
-    hipLaunchKernelGGL((k_occlude<true, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  if (S.n_insts && multi)
+    hipLaunchKernelGGL((k_occlude<true, false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else if (S.n_insts)
+    hipLaunchKernelGGL((k_occlude<false, false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+  else if (multi && S.walk_refill)
+    hipLaunchKernelGGL((k_occlude<true, true, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else if (multi)
-    hipLaunchKernelGGL((k_occlude<true, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+    hipLaunchKernelGGL((k_occlude<true, false, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else if (S.walk_refill)
-    hipLaunchKernelGGL((k_occlude<false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+    hipLaunchKernelGGL((k_occlude<false, true, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else
-    hipLaunchKernelGGL((k_occlude<false, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+    hipLaunchKernelGGL((k_occlude<false, false, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
 }
 void launch_gather(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,
                    uint32_t b, unsigned grid, hipStream_t st, const GuideView *G) {
@@ -1071,10 +1103,14 @@ void launch_finish(bool multi, const SceneView &S, const RenderView &R, const St
   g = grid < g ? grid : g;
   PathBufs Pk = P;
   Pk.media_ovf = stats + SGR_STAT_MEDIA_OVF;
-  if (multi)
-    hipLaunchKernelGGL((k_finish<true>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);
+  if (multi && S.n_insts)
+    hipLaunchKernelGGL((k_finish<true, true>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);
+  else if (multi)
+    hipLaunchKernelGGL((k_finish<true, false>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);
+  else if (S.n_insts)
+    hipLaunchKernelGGL((k_finish<false, true>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);
   else
-    hipLaunchKernelGGL((k_finish<false>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);
+    hipLaunchKernelGGL((k_finish<false, false>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);
 }
 
 void launch_count_queues(uint32_t maxp, const PathBufs &P, unsigned long long *stats,

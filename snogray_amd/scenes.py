@@ -38,6 +38,8 @@ class SceneDesc:
         self.tripars = []
         self.spheres = []
         self.envmaps = []
+        self.models = []       # SceneDesc objects holding geometry only
+        self.instances = []    # (model index, 3x4 local-to-world)
         self.min_trace = min_trace
         self.accel = accel
         self._c = None
@@ -92,6 +94,20 @@ class SceneDesc:
         v, t, n = native.load_mesh(path, fmt)
         return self.mesh(v, t, material, axis=axis, left_handed=False, normals=n)
 
+    def model(self):
+        """A new model: add geometry to the returned SceneDesc (its
+        material indices are this scene's), then place it with instance()."""
+        m = SceneDesc(min_trace=self.min_trace, accel=self.accel)
+        self.models.append(m)
+        self._c = None
+        return len(self.models) - 1, m
+
+    def instance(self, model, local_to_world):
+        """Place MODEL by a 3x4 (or 4x4) affine local-to-world transform."""
+        x = np.asarray(local_to_world, np.float64)[:3, :4]
+        self.instances.append((int(model), x.astype(np.float32)))
+        self._c = None
+
     def tripar(self, v0, e1, e2, material, parallelogram=False):
         self.tripars.append(dict(v0=v0, e1=e1, e2=e2, material=material,
                                  parallelogram=parallelogram))
@@ -118,16 +134,8 @@ class SceneDesc:
         return self.num_triangles + len(self.tripars) + len(self.spheres)
 
     # --- C descriptor ----------------------------------------------------
-    def c(self):
-        if self._c is not None:
-            return self._c
-        mats = (abi.Material * max(1, len(self.materials)))()
-        for i, m in enumerate(self.materials):
-            mats[i].type = m["type"]
-            mats[i].color[:] = [float(x) for x in m["color"]]
-            mats[i].gloss_color[:] = [float(x) for x in m["gloss_color"]]
-            mats[i].m, mats[i].ior_n, mats[i].ior_k = m["m"], m["ior_n"], m["ior_k"]
-            mats[i].underlying = m["underlying"]
+    def _geometry(self):
+        """C arrays of this description's meshes, tripars and spheres."""
         meshes = (abi.Mesh * max(1, len(self.meshes)))()
         for i, m in enumerate(self.meshes):
             meshes[i].vertices = abi.fptr(m["v"])
@@ -152,6 +160,19 @@ class SceneDesc:
             sps[i].center[:] = [float(x) for x in s["center"]]
             sps[i].radius = s["radius"]
             sps[i].material = s["material"]
+        return meshes, tps, sps
+
+    def c(self):
+        if self._c is not None:
+            return self._c
+        mats = (abi.Material * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            mats[i].type = m["type"]
+            mats[i].color[:] = [float(x) for x in m["color"]]
+            mats[i].gloss_color[:] = [float(x) for x in m["gloss_color"]]
+            mats[i].m, mats[i].ior_n, mats[i].ior_k = m["m"], m["ior_n"], m["ior_k"]
+            mats[i].underlying = m["underlying"]
+        meshes, tps, sps = self._geometry()
         envs = (abi.Envmap * max(1, len(self.envmaps)))()
         for i, e in enumerate(self.envmaps):
             envs[i].texels = abi.fptr(e["tex"])
@@ -164,7 +185,23 @@ class SceneDesc:
         d.materials, d.num_materials = mats, len(self.materials)
         d.envmaps, d.num_envmaps = envs, len(self.envmaps)
         d.min_trace, d.accel = self.min_trace, self.accel
-        self._keep = (mats, meshes, tps, sps, envs)
+        models = (abi.Model * max(1, len(self.models)))()
+        keep_models = []
+        for i, m in enumerate(self.models):
+            g = m._geometry()
+            keep_models.append(g)
+            models[i].meshes, models[i].num_meshes = g[0], len(m.meshes)
+            models[i].tripars, models[i].num_tripars = g[1], len(m.tripars)
+            models[i].spheres, models[i].num_spheres = g[2], len(m.spheres)
+        insts = (abi.Instance * max(1, len(self.instances)))()
+        for i, (mi, x) in enumerate(self.instances):
+            insts[i].model = mi
+            insts[i].local_to_world[:] = [float(v) for v in x.ravel()]
+        d.models = C.cast(models, C.c_void_p) if self.models else None
+        d.num_models = len(self.models)
+        d.instances = C.cast(insts, C.c_void_p) if self.instances else None
+        d.num_instances = len(self.instances)
+        self._keep = (mats, meshes, tps, sps, envs, models, keep_models, insts)
         self._c = d
         return d
 

@@ -383,3 +383,25 @@ def test_resume_from_a_partial_image(ctx, tmp_path):
     native.write_image(tmp_path / "whole.hdr", img)
     np.testing.assert_array_equal(out[:R, :, :3], native.read_image(tmp_path / "whole.hdr")[:R])
     sc.close()
+
+
+def test_instanced_paths_match_oracle(ctx):
+    """Paths through a scene of instanced models (rotated, scaled and
+    mirrored boxes, an ellipsoid from a scaled sphere, 20 placed copies
+    of a triangle soup) agree with the oracle, through the per-bounce
+    kernels and the finishing kernel."""
+    from tests.test_gpu_trace import instanced_scene
+    sd, cam = instanced_scene()
+    sc, osc = native.Scene(ctx, sd), oracle.OracleScene(sd)
+    p = abi.make_params(48, 40, 8, max_path_len=8)
+    got, gst = sc.render_samples(cam, p)
+    _, ref, ost = osc.render(cam, p, samples=True)
+    check_paths(got, ref, 1e-4, 0.999, "instances")
+    for k in ("eye_rays", "scene_intersect_calls", "shadow_tests"):
+        assert abs(gst[k] - ost[k]) <= 0.002 * ost[k], k
+    acc, _ = sc.render(cam, p)
+    oacc, _, _ = osc.render(cam, p)
+    fin = native.finalize(acc)[..., :3]
+    ofin = native.finalize(oacc.astype(np.float32))[..., :3]
+    assert np.mean(np.abs(fin - ofin) <= 1e-3 * (np.abs(ofin) + 1e-2)) > 0.99
+    sc.close()

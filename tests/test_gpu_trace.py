@@ -124,3 +124,48 @@ def test_edge_rays(ctx):
                          np.float32(0), np.float32(1))
     h, _ = sc.trace_closest(empty)
     assert len(h.prim) == 0
+
+
+def instanced_scene():
+    """Cornell box whose two boxes are one model placed twice (rotated and
+    scaled, then mirrored), with an ellipsoid made of a scaled unit-sphere
+    model, and a model holding its own triangle soup placed 20 times."""
+    from tests.test_oracle_instances import box_model_scene, rot
+    A = 0.8 * rot((0.2, 1.0, 0.1), 0.6)
+    sd, cam = box_model_scene(True, np.hstack([A, np.array([[0.3], [0.1], [-0.2]])]))
+    sd.instance(0, np.hstack([np.diag([-0.5, 0.6, 0.5]), np.array([[4.5], [0.0], [3.0]])]))
+    mi, m = sd.model()
+    m.sphere((0, 0, 0), 1.0, 0)
+    sd.instance(mi, np.hstack([rot((1, 0.3, -0.2), 0.9) @ np.diag([0.6, 0.3, 0.15]),
+                               np.array([[2.0], [4.0], [2.5]])]))
+    soup = helpers.triangle_soup(2000, seed=9, extent=1.0, size=0.15)
+    mj, ms = sd.model()
+    ms.mesh(soup.meshes[0]["v"], soup.meshes[0]["t"], 0)
+    rng = np.random.default_rng(8)
+    for k in range(20):
+        X = rot(rng.normal(size=3), rng.uniform(0, 3)) * rng.uniform(0.2, 0.5)
+        sd.instance(mj, np.hstack([X, rng.uniform(0.8, 4.8, (3, 1))]))
+    return sd, cam
+
+
+def test_instanced_scene_bit_exact(ctx):
+    """Two-level search: closest hits (primitive within its model, the
+    instance, t, u, v), the surfaces tested and any-hit occlusion equal
+    the oracle's bit for bit."""
+    sd, cam = instanced_scene()
+    sc, osc = native.Scene(ctx, sd), oracle.OracleScene(sd)
+    lo, hi, _ = sc.bounds()
+    for b in (helpers.random_rays(lo, hi, 100000, seed=5), helpers.camera_rays(cam, 128, 96, seed=6)):
+        g, gst = sc.trace_closest(b)
+        o, (onodes, otests) = osc.trace_closest(b)
+        assert_hits_equal(g, o)
+        hit = o.prim != abi.MISS
+        assert np.array_equal(g.inst[hit], o.inst[hit])
+        assert (o.inst[hit] != abi.MISS).sum() > 0.1 * hit.sum()
+        assert gst["surf_tests_closest"] == otests
+        assert abs(gst["node_visits_closest"] - onodes) <= 0.1 * onodes
+        for sb in (helpers.shadow_batch(b, o), helpers.shadow_batch(b, o, frac=1.001)):
+            go, _ = sc.trace_occluded(sb)
+            oo, _ = osc.trace_occluded(sb)
+            assert np.array_equal(go, oo)
+    sc.close()
