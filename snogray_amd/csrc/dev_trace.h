@@ -72,7 +72,8 @@ __device__ __forceinline__ bool sphere_hit(V3 c, float radius, V3 ro, V3 rd, flo
 // One surface reference: three float4 in surface-list order.
 //   triangle / tripar: (v0, prim id bits) (e1, kind) (e2, 0)
 //   sphere:            (centre, prim id bits) (radius, 0, 0, kind)
-// kind: 0 triangle, 1 parallelogram, 2 sphere.
+// kind: 0 triangle, 1 parallelogram, 2 sphere, 3 instance of a model
+// (record (0, n_prims + instance index) (0, kind) (0); see ref_test).
 __device__ __forceinline__ bool ref_hit(const float4 *refs, uint32_t k, const RayF &r,
                                         float &t, float &u, float &v, uint32_t &prim) {
   // All three loads issued together (a sphere's third record is zeros),
