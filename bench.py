@@ -42,6 +42,9 @@ HBM_PEAK_GBS = 8000.0
 NODE_B, TEST_B, LIST_B = 16, 36, 4
 RAY_IN_B, HIT_OUT_B, OCC_OUT_B, QUEUE_B = 32, 16, 1, 4
 CROP_W, CROP_H, CROP_GRID, REF_SPP = 16, 9, 4, 4096
+# FP64 references of served / batched jobs' windows: 16 x the job's spp,
+# at least 256 and at most 2048 samples per pixel
+JOB_REF_MULT, JOB_REF_MIN, JOB_REF_MAX = 16, 256, 2048
 LEARNED_SEEDS = 9
 
 
@@ -99,6 +102,23 @@ def relmse(x, ref):
     x = np.asarray(x, np.float64)
     r = np.asarray(ref, np.float64)
     return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+
+def relmse_summary(rows, reference):
+    """Medians over jobs of the relMSE of the GPU results and of CPU renders
+    at the same spp against FP64 references, and of their ratio (about 1
+    for an unbiased GPU estimator with the CPU's variance)."""
+    rows = [r for r in rows if "relmse_gpu" in r]
+    if not rows:
+        return None
+    g = np.array([r["relmse_gpu"] for r in rows])
+    c = np.array([r["relmse_cpu_same_spp"] for r in rows])
+    return {"reference": reference, "jobs": len(rows),
+            "gpu_median": float(f"{np.median(g):.4g}"),
+            "cpu_same_spp_median": float(f"{np.median(c):.4g}"),
+            "gpu_over_cpu_median": float(f"{np.median(g / np.maximum(c, 1e-30)):.4g}"),
+            "per_job": [[r["job_spp"], float(f"{r['relmse_gpu']:.4g}"),
+                         float(f"{r['relmse_cpu_same_spp']:.4g}")] for r in rows]}
 
 
 def finalize(acc):
@@ -187,19 +207,61 @@ class CpuBaseline:
 
     def pixel_check(self, job, desc, acc, window=16, seeds=24):
         """3-standard-error pixel test of a served job's window against
-        independent-seed CPU renders."""
+        independent-seed CPU renders, and the relMSE of the GPU window and
+        of those CPU renders (same spp) against an FP64 oracle render of
+        the window at JOB_REF_MULT x the job's spp."""
         from snogray_amd import serve
         osc = self.oracle.OracleScene(desc)
-        ww, hh = min(window, job.width), min(window, job.height)
-        x0, y0 = (job.width - ww) // 2, (job.height - hh) // 2
+        x0, y0, ww, hh = self.pick_window(osc, job, window)
         imgs = []
         for k in range(seeds):
             p = job.params(window=(x0, y0, ww, hh))
             p.seed = 1000 + 7919 * k + job.seed % 1000
             a, _, _ = osc.render(job.camera(), p, nthreads=self.cores)
             imgs.append(finalize(a[y0:y0 + hh, x0:x0 + ww]))
-        frac, ok, ref = serve.pixel_test(finalize(acc[y0:y0 + hh, x0:x0 + ww]), imgs)
-        return {"pass_fraction": frac, "cpu_pass_fraction": ref, "passed": bool(ok)}
+        gpu = finalize(acc[y0:y0 + hh, x0:x0 + ww])
+        frac, ok, ref = serve.pixel_test(gpu, imgs)
+        r = self.window_relmse(desc, job, (x0, y0, ww, hh), gpu, imgs)
+        return {"pass_fraction": frac, "cpu_pass_fraction": ref, "passed": bool(ok), **r}
+
+    def pick_window(self, osc, job, window=16, grid=4, spp=4):
+        """The WINDOW x WINDOW window, of a GRID x GRID set spread over the
+        job's image, whose pixels differ most between two independent
+        SPP-sample renders: a window where the estimator is noisy (not
+        plain sky or unlit background), so that the checks test it."""
+        ww, hh = min(window, job.width), min(window, job.height)
+        best, best_score = None, -1.0
+        for j in range(grid):
+            for i in range(grid):
+                x0 = min(job.width - ww, max(0, int((i + 0.5) * job.width / grid) - ww // 2))
+                y0 = min(job.height - hh, max(0, int((j + 0.5) * job.height / grid) - hh // 2))
+                imgs = []
+                for k in range(2):
+                    p = job.params(window=(x0, y0, ww, hh))
+                    p.spp = spp
+                    p.seed = 555 + k
+                    a, _, _ = osc.render(job.camera(), p, nthreads=self.cores)
+                    imgs.append(finalize(a[y0:y0 + hh, x0:x0 + ww])[..., :3])
+                score = float(np.mean(np.abs(imgs[0] - imgs[1])))
+                if score > best_score:
+                    best, best_score = (x0, y0, ww, hh), score
+        return best
+
+    def window_relmse(self, desc, job, win, gpu, cpu_imgs):
+        """relMSE of GPU (a finalized window of the job's render) and the
+        mean relMSE of CPU_IMGS (independent renders at the job's spp)
+        against an FP64 oracle render of the window."""
+        x0, y0, ww, hh = win
+        ref_spp = int(min(JOB_REF_MAX, max(JOB_REF_MIN, JOB_REF_MULT * job.spp)))
+        p = job.params(window=win)
+        p.spp = ref_spp
+        p.seed = 99991 + job.seed % 100000
+        a, _, _ = self.oracle.OracleScene(desc, fp64=True).render(job.camera(), p,
+                                                                  nthreads=self.cores)
+        ref = finalize(a[y0:y0 + hh, x0:x0 + ww])
+        return {"relmse_gpu": relmse(gpu, ref),
+                "relmse_cpu_same_spp": float(np.mean([relmse(c, ref) for c in cpu_imgs])),
+                "ref_spp": ref_spp, "job_spp": int(job.spp)}
 
 
 # --------------------------------------------------------------------------
@@ -335,7 +397,7 @@ def batch_bench(args):
     t_build = time.perf_counter() - t0
     best, table = batch.autotune(ctx, J, log=log)
     log(f"tuned: {best}")
-    rep = batch.run_batch(ctx, J, **best)
+    rep = batch.run_batch(ctx, J, keep_results=not args.no_cpu, **best)
     # the same jobs one after another through sgr_render, for comparison
     t0 = time.perf_counter()
     for sc, cam, p in J:
@@ -363,10 +425,39 @@ def batch_bench(args):
                             round(r * 1e3)] for c, r in table],
         "graph_steps": rep["graph_steps"], "max_jobs_per_step": rep["max_jobs_per_step"],
     }
+    if not args.no_cpu:
+        line["relmse_fp64"] = batch_relmse(jobs, rep["results"])
     for sc in S.values():
         sc.close()
     ctx.close()
     return line
+
+
+def batch_relmse(jobs, results, every=8, window=16, seeds=4):
+    """relMSE of every EVERY-th batched job's central window against an
+    FP64 oracle render, beside CPU renders at the job's spp."""
+    cpu = CpuBaseline(None)
+    rows = []
+    t = time.perf_counter()
+    for i in range(0, len(jobs), every):
+        job = jobs[i]
+        desc = job.scene.build()[0]
+        osc = cpu.oracle.OracleScene(desc)
+        win = cpu.pick_window(osc, job, window)
+        x0, y0, ww, hh = win
+        imgs = []
+        for k in range(seeds):
+            p = job.params(window=win)
+            p.seed = 2000 + 7919 * k + job.seed % 1000
+            a, _, _ = osc.render(job.camera(), p, nthreads=cpu.cores)
+            imgs.append(finalize(a[y0:y0 + hh, x0:x0 + ww]))
+        gpu = finalize(results[i][y0:y0 + hh, x0:x0 + ww])
+        rows.append(cpu.window_relmse(desc, job, win, gpu, imgs))
+    log(f"batch relMSE checks: {time.perf_counter() - t:.1f} s")
+    return relmse_summary(rows, f"FP64 CPU oracle render of a 16x16 window of every {every}th "
+                                f"job (the noisiest of a 4x4 grid) at 16x its spp (256-2048); "
+                                f"CPU: {seeds} fp32 renders at the job's spp; "
+                                f"per_job = [spp, gpu, cpu same spp]")
 
 
 SERVE_PATHS = 1 << 26   # engine path capacity of the serving loop
@@ -404,7 +495,12 @@ def serve_bench(args):
             "max_active_jobs": rep["max_active_jobs"],
             "prewarm_s": round(rep["prewarm_s"], 1),
             "peak_budget_gb": round(rep["peak_budget_bytes"] / 2 ** 30, 2),
-            "cpu_pixel_checks": rep["checks"],
+            "cpu_pixel_checks": [{k: v for k, v in c.items() if k != "relmse_gpu"
+                                  and k != "relmse_cpu_same_spp"} for c in rep["checks"]],
+            "relmse_fp64": relmse_summary(
+                rep["checks"], "FP64 CPU oracle render of each checked job's 16x16 window "
+                "(the noisiest of a 4x4 grid) at 16x the job's spp (256-2048); "
+                "per_job = [spp, gpu, cpu same spp]"),
         })
     ctx.close()
     return lines
