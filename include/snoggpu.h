@@ -52,6 +52,7 @@ extern "C" {
 #define SGR_MAT_GLASS          3
 #define SGR_MAT_MIRROR         4
 #define SGR_MAT_GLOW           5
+#define SGR_MAT_PHONG          6
 
 /* Acceleration structure (render/global-render-state.cc:63-74 "accel"). */
 #define SGR_ACCEL_OCTREE  0
@@ -68,7 +69,9 @@ typedef struct sgr_scene sgr_scene;
  *   MIRROR         gloss_color = reflectance, ior_n/ior_k = IOR,
  *                  color = underlying Lambert colour (all zero: none)
  *   GLOW           color = emitted radiance; underlying = index of the
- *                  material supplying the BSDF, or -1 for none           */
+ *                  material supplying the BSDF, or -1 for none
+ *   PHONG          color = diffuse colour, gloss_color = specular colour,
+ *                  m = Phong exponent (material/phong.h:26-37)           */
 typedef struct sgr_material {
     int32_t type;
     float   color[3];

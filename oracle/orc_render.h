@@ -185,6 +185,13 @@ template<class R> struct Bsdf {
       bool entering = !is.back;
       old_ior = entering ? is.media_ior : glass_medium_ior;
       new_ior = entering ? glass_medium_ior : is.enclosing_ior;
+    } else if (type == SGR_MAT_PHONG) {  // material/phong.cc:38-45
+      color = mt.color;
+      gloss = mt.gloss;
+      m = mt.m;
+      diff_weight = color.intensity();
+      inv_diff_weight = diff_weight == 0 ? R(0) : R(1) / diff_weight;
+      inv_gloss_weight = diff_weight == 1 ? R(0) : R(1) / (R(1) - diff_weight);
     } else if (type == SGR_MAT_MIRROR) {
       gloss = mt.gloss;  // reflectance
       color = mt.color;  // underlying Lambert colour
@@ -355,12 +362,68 @@ template<class R> struct Bsdf {
     return s;
   }
 
+  // --- Phong (material/phong.cc:38-153, material/phong-dist.h:27-57)
+  R phong_pdf(R c) const { return (m + R(1)) * R(INV_PIf) * R(0.5f) * std::pow(c, m); }
+  C3<R> phong_val(const V3<R> &l, const V3<R> &h, R &pdf, unsigned flags) const {
+    R nl = isec->cos_n(l);
+    C3<R> col(0);
+    pdf = 0;
+    if (flags & DIFFUSE) {
+      R diff = R(INV_PIf);
+      R diff_pdf = nl * R(INV_PIf);
+      pdf += diff_pdf * diff_weight;
+      col += color * diff;
+    }
+    if (flags & GLOSSY) {
+      R nh = isec->cos_n(h);
+      R vh = isec->cos_v(h);
+      R spec = phong_pdf(nh);
+      R spec_pdf = spec / (R(4) * vh);
+      pdf += spec_pdf * (R(1) - diff_weight);
+      col += gloss * spec;
+    }
+    return col;
+  }
+  BsdfSample<R> phong_sample(R u, R v, unsigned desired) const {
+    V3<R> l, h;
+    if (u < diff_weight) {
+      R su = u * inv_diff_weight;
+      l = z_normal_symm_vec(std::sqrt(su), v);
+      h = (isec->v + l).unit();
+    } else {
+      R su = (u - diff_weight) * inv_gloss_weight;
+      h = z_normal_symm_vec(std::pow(su, R(1) / (m + R(1))), v);
+      if (isec->cos_v(h) < 0) h = -h;
+      l = mirror(isec->v, h);
+    }
+    BsdfSample<R> s;
+    s.dir = l;
+    s.flags = REFLECTIVE;
+    if (isec->cos_n(l) > R(float(1e-15)) && isec->cos_geom_n(l) > R(float(1e-15))) {
+      R pdf;
+      s.val = phong_val(l, h, pdf, desired);
+      s.pdf = pdf;
+    }
+    return s;
+  }
+  BsdfValue<R> phong_eval(const V3<R> &dir, unsigned flags) const {
+    BsdfValue<R> r;
+    if ((flags & REFLECTIVE) && isec->cos_n(dir) > 0) {
+      V3<R> h = (isec->v + dir).unit();
+      R pdf;
+      r.val = phong_val(dir, h, pdf, flags);
+      r.pdf = pdf;
+    }
+    return r;
+  }
+
   BsdfSample<R> sample(R u, R v, unsigned flags) const {
     switch (type) {
       case SGR_MAT_LAMBERT: return lambert_sample(u, v, flags, color);
       case SGR_MAT_COOK_TORRANCE: return ct_sample(u, v, flags);
       case SGR_MAT_GLASS: return glass_sample(u, v, flags);
       case SGR_MAT_MIRROR: return mirror_sample(u, v, flags);
+      case SGR_MAT_PHONG: return phong_sample(u, v, flags);
     }
     return BsdfSample<R>();
   }
@@ -368,6 +431,7 @@ template<class R> struct Bsdf {
     switch (type) {
       case SGR_MAT_LAMBERT: return lambert_eval(dir, flags);
       case SGR_MAT_COOK_TORRANCE: return ct_eval(dir, flags);
+      case SGR_MAT_PHONG: return phong_eval(dir, flags);
     }
     return BsdfValue<R>();  // glass and mirror: specular only
   }
@@ -586,7 +650,7 @@ template<class R> struct Scene {
   void build(const sgr_scene_desc &d) {
     for (unsigned i = 0; i < d.num_materials; i++) {
       const sgr_material &m = d.materials[i];
-      if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_GLOW)
+      if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_PHONG)
         throw std::runtime_error("bad material type");
       mats.push_back(Material<R>{m.type, C3<R>(m.color[0], m.color[1], m.color[2]),
                                  C3<R>(m.gloss_color[0], m.gloss_color[1], m.gloss_color[2]),

@@ -14,6 +14,7 @@ SGR_MAT_COOK_TORRANCE = 2
 SGR_MAT_GLASS = 3
 SGR_MAT_MIRROR = 4
 SGR_MAT_GLOW = 5
+SGR_MAT_PHONG = 6
 SGR_ACCEL_OCTREE = 0
 SGR_ACCEL_TRIV = 1
 MISS = 0xFFFFFFFF

@@ -335,6 +335,71 @@ __device__ __forceinline__ BValue ct_eval(const Isec &is, const DevMaterial &M, 
   return r;
 }
 
+// Phong (material/phong.cc:38-153, material/phong-dist.h:27-57): a
+// diffuse layer (CosDist) chosen with probability intensity(color) and
+// a Phong lobe of the half vector, pdf (e + 1) / 2pi cos^e.  val sums
+// the layers FLAGS selects; a sample picks its direction whatever FLAGS
+// says and reports REFLECTIVE only.
+__device__ __forceinline__ float phong_dist_pdf(float e, float cos_theta) {
+  return (e + 1.0f) * SGR_INV_PIf * 0.5f * powf(cos_theta, e);
+}
+__device__ __forceinline__ C3 phong_val(const Isec &is, const DevMaterial &M, V3 l, V3 h,
+                                        unsigned flags, float &pdf) {
+  const float nl = l.z;
+  C3 col = c3(0.0f);
+  pdf = 0.0f;
+  if (flags & F_DIFFUSE) {
+    const float diff_pdf = nl * SGR_INV_PIf;
+    pdf = pdf + diff_pdf * M.diff_weight;
+    col = col + cload(M.color) * SGR_INV_PIf;
+  }
+  if (flags & F_GLOSSY) {
+    const float nh = h.z;
+    const float vh = cos_v(is, h);
+    const float spec = phong_dist_pdf(M.m, nh);
+    const float spec_pdf = spec / (4.0f * vh);
+    pdf = pdf + spec_pdf * (1.0f - M.diff_weight);
+    col = col + cload(M.gloss) * spec;
+  }
+  return col;
+}
+__device__ __forceinline__ BSample phong_sample(const Isec &is, const DevMaterial &M, float u,
+                                                float v, unsigned desired) {
+  V3 l, h;
+  if (u < M.diff_weight) {
+    const float su = u * M.inv_diff_weight;
+    l = z_normal_symm_vec(sqrtf(su), v);
+    h = unit(is.v + l);
+  } else {
+    const float su = (u - M.diff_weight) * M.inv_gloss_weight;
+    h = z_normal_symm_vec(powf(su, 1.0f / (M.m + 1.0f)), v);
+    if (cos_v(is, h) < 0.0f) h = -h;
+    l = h * dot(is.v, h) * 2.0f - is.v;
+  }
+  BSample s = bsample_none();
+  s.dir = l;
+  s.flags = F_REFLECTIVE;
+  if (l.z > 1e-15f && cos_geom(is, l) > 1e-15f) {
+    float pdf;
+    s.val = phong_val(is, M, l, h, desired, pdf);
+    s.pdf = pdf;
+  }
+  return s;
+}
+__device__ __forceinline__ BValue phong_eval(const Isec &is, const DevMaterial &M, V3 d,
+                                             unsigned flags) {
+  BValue r;
+  r.val = c3(0.0f);
+  r.pdf = 0.0f;
+  if ((flags & F_REFLECTIVE) && d.z > 0.0f) {
+    const V3 h = unit(is.v + d);
+    float pdf;
+    r.val = phong_val(is, M, d, h, flags, pdf);
+    r.pdf = pdf;
+  }
+  return r;
+}
+
 // geometry/vec.h:166-186
 __device__ __forceinline__ V3 refraction(V3 v, V3 n, float ior_in, float ior_out) {
   float ratio = ior_in / ior_out;
@@ -413,6 +478,7 @@ __device__ __forceinline__ BSample bsdf_sample(const Isec &is, const DevMaterial
     case 2: return ct_sample(is, M, u, v, flags);
     case 3: return glass_sample(is, M, u, flags);
     case 4: return mirror_sample(is, M, u, v, flags);
+    case 6: return phong_sample(is, M, u, v, flags);
   }
   return bsample_none();
 }
@@ -429,6 +495,8 @@ __device__ __forceinline__ BValue bsdf_eval(const Isec &is, const DevMaterial &M
     }
   } else if (M.type == 2) {
     r = ct_eval(is, M, d, flags);
+  } else if (M.type == 6) {
+    r = phong_eval(is, M, d, flags);
   }
   return r;
 }

@@ -305,7 +305,7 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   mats.resize(nm);
   for (uint32_t i = 0; i < nm; i++) {
     const sgr_material &m = d->materials[i];
-    if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_GLOW)
+    if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_PHONG)
       return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": unknown type");
     if (m.type == SGR_MAT_GLOW && m.underlying >= (int)nm)
       return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": bad underlying");
@@ -329,6 +329,14 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
       o.inv_gloss_weight = o.diff_weight == 1 ? 0.0f : 1.0f / (1.0f - o.diff_weight);
       o.gloss_layer = m.m < 0.5f ? 0x02u : 0x04u;
       o.have_layers = (o.diff_weight > 0 ? 0x04u : 0u) | (o.diff_weight < 1 ? o.gloss_layer : 0u);
+    }
+    if (m.type == SGR_MAT_PHONG) {  // material/phong.cc:38-45
+      float s = m.color[0];  // Color::intensity (color/color.h:160-166)
+      s += m.color[1];
+      s += m.color[2];
+      o.diff_weight = s / 3.0f;
+      o.inv_diff_weight = o.diff_weight == 0 ? 0.0f : 1.0f / o.diff_weight;
+      o.inv_gloss_weight = o.diff_weight == 1 ? 0.0f : 1.0f / (1.0f - o.diff_weight);
     }
     if (m.type == SGR_MAT_MIRROR) {  // material/mirror.cc:28-33
       float s = m.color[0];

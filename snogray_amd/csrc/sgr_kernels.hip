@@ -394,7 +394,7 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
       // Learned proposal: one-sample MIS of the BSDF (probability
       // G.mix) and the vMF mixture at this vertex; delta BSDFs (glass,
       // mirror) bypass it.
-      const bool guided = GUIDED && (M.type == 1 || M.type == 2);  // Lambert, Cook-Torrance
+      const bool guided = GUIDED && (M.type == 1 || M.type == 2 || M.type == 6);  // not delta
       if (guided) {
         const float *L = G.lobes + (size_t)j * GD_OUT;
         const float usel = draw(qk, (path_len << 20) | 3u);

@@ -67,6 +67,10 @@ class SceneDesc:
         return self._mat(type=abi.SGR_MAT_MIRROR, gloss_color=reflectance,
                          ior_n=ior, ior_k=ior_k, color=color)
 
+    def phong(self, color, specular_color, exponent):
+        return self._mat(type=abi.SGR_MAT_PHONG, color=color,
+                         gloss_color=specular_color, m=exponent)
+
     def glow(self, color, underlying=-1):
         return self._mat(type=abi.SGR_MAT_GLOW, color=color,
                          underlying=underlying)
