@@ -235,7 +235,24 @@ typedef struct sgr_render_params {
      * the next chunk's full early ones; results do not depend on it. */
     uint32_t paths_in_flight;
     uint32_t flags;                /* SGR_RENDER_* */
+    /* Surface integrator (render/render.lua "surface_integ"): the path
+     * tracer (render/path-integ.cc) or direct lighting with perfectly
+     * specular recursion (render/direct-integ.h:31-67 over
+     * render/recursive-integ.cc:29-216), whose light sampling takes
+     * light_samples samples per light (0: 16, the reference's default,
+     * render/direct-integ.h:72-77).  The direct integrator runs in
+     * sgr_render / sgr_render_samples / sgr_render_noalloc; the job
+     * engine and guided renders refuse it. */
+    uint32_t integrator;           /* SGR_INTEG_* */
+    uint32_t light_samples;
 } sgr_render_params;
+
+#define SGR_INTEG_PATH    0u
+#define SGR_INTEG_DIRECT  1u
+/* recursion levels the direct integrator keeps per sample; deeper
+ * specular branches (reached with probability 2^-26 or less through
+ * its roulette) are dropped and counted in sgr_stats.direct_overflows */
+#define SGR_DIRECT_MAX_DEPTH 32u
 
 #define SGR_DEFAULT_PATHS_IN_FLIGHT (1u << 26)
 
@@ -279,6 +296,7 @@ typedef struct sgr_stats {
      * the reference calls and prunes at once, which load nothing) */
     uint64_t node_fetches_closest;
     uint64_t node_fetches_shadow;
+    uint64_t direct_overflows;            /* see SGR_DIRECT_MAX_DEPTH */
 } sgr_stats;
 
 /* Octree statistics (space/octree.h:53-67). */

@@ -20,11 +20,11 @@ namespace {
 
 struct Counters {
   uint64_t eye = 0, isect = 0, shadow = 0, nc_c = 0, st_c = 0, nc_s = 0,
-           st_s = 0, illum = 0, paths = 0, media_ovf = 0;
+           st_s = 0, illum = 0, paths = 0, media_ovf = 0, dint_ovf = 0;
   void add(const Counters &o) {
     eye += o.eye; isect += o.isect; shadow += o.shadow; nc_c += o.nc_c;
     st_c += o.st_c; nc_s += o.nc_s; st_s += o.st_s; illum += o.illum;
-    paths += o.paths; media_ovf += o.media_ovf;
+    paths += o.paths; media_ovf += o.media_ovf; dint_ovf += o.dint_ovf;
   }
 };
 
@@ -52,8 +52,14 @@ static inline float mitchell1(float x, float inv_radius, float b, float c) {
 // Sample-stream layout (see orc_rng.h).
 struct Stream {
   uint32_t spp, minp, L, ds;
-  GridShape g_cam, g_dir, g_cont, g_dir_rand;
-  uint32_t nsub_dir, nsub_dir_rand;
+  GridShape g_cam, g_dir, g_cont, g_dir_rand, g_dint;
+  uint32_t nsub_dir, nsub_dir_rand, nsub_dint;
+  // The direct integrator's light samples: light_samples per light
+  // (default 16), channels 1 + 2 l (light) and 2 + 2 l (BSDF).
+  void init_direct(uint32_t light_samples) {
+    g_dint = grid_shape(spp * (light_samples ? light_samples : 16u));
+    nsub_dint = g_dint.total / spp;
+  }
   void init(uint32_t spp_, uint32_t minp_, uint32_t L_, uint32_t ds_) {
     spp = spp_; minp = minp_; L = L_; ds = ds_;
     g_cam = grid_shape(spp);
@@ -85,6 +91,7 @@ template<class R> struct Tracer {
 
   Tracer(const Scene<R> &s, const sgr_render_params &p) : sc(s), P(p) {
     st.init(p.spp, p.min_path_len, (uint32_t)s.lights.size(), p.direct_samples);
+    st.init_direct(p.light_samples);
   }
 
   bool occluded(const Ray<R> &r) {
@@ -238,6 +245,108 @@ template<class R> struct Tracer {
     return radiance;
   }
 
+  // DirectInteg (render/direct-integ.h:31-67) on RecursiveInteg
+  // (render/recursive-integ.cc:29-216).  At each hit: emission, direct
+  // lighting of every light with light_samples samples each (the same
+  // stratified samples at every depth: Lo gets the one Sample), then a
+  // recursion into the perfectly specular reflection and the perfectly
+  // specular transmission, each weighted by val |cos|; beyond depth 5 a
+  // branch goes on with probability 1/2 and twice the weight, drawn from
+  // the path's sequential stream (context.random()), which the sphere
+  // light's extra draw also takes from.  The media of a branch are kept
+  // as (innermost, enclosing) IORs, all that Media (isec, true) reads
+  // (material/media.cc:25-52): entering pushes (material medium,
+  // innermost), leaving sets both to the enclosing one (the new layer
+  // keeps the old surrounding).  Radiance is accumulated top-down: a
+  // branch's weight times its emission and direct light, depth first,
+  // reflection before transmission; the reference sums the same terms
+  // bottom-up.
+  static constexpr uint32_t kDintDepth = SGR_DIRECT_MAX_DEPTH;
+  static constexpr uint32_t kDintRand = 0xc0000000u;  // draw dims of the sequential stream
+  uint64_t dq = 0;
+  uint32_t dn = 0;
+  R dint_draw() { return R(draw(dq, kDintRand + dn++)); }
+
+  C3<R> dint_direct(const Isec<R> &is, const Bsdf<R> &bsdf, uint64_t pk, uint32_t s) {
+    cnt.illum++;
+    C3<R> rad(0);
+    const uint32_t nsub = st.nsub_dint;
+    for (uint32_t l = 0; l < st.L; l++) {
+      C3<R> lrad(0);
+      for (uint32_t j = 0; j < nsub; j++) {
+        float lu, lv, bu, bv;
+        Stream::strat(pk, 1 + 2 * l, s * nsub + j, st.g_dint, lu, lv);
+        Stream::strat(pk, 2 + 2 * l, s * nsub + j, st.g_dint, bu, bv);
+        R extra = sc.lights[l].kind == 1 ? dint_draw() : R(0);
+        lrad += sample_light(is, bsdf, sc.lights[l], R(lu), R(lv), R(bu), R(bv), extra,
+                             ALL & ~SPECULAR);
+      }
+      rad += lrad / R(float(nsub));
+    }
+    return rad;
+  }
+
+  void dint_Li(const Ray<R> &ray, R m0, R m1, uint32_t depth, C3<R> W, uint64_t pk,
+               uint32_t s, C3<R> &total, R &alpha) {
+    if (depth > 5) {  // render/recursive-integ.cc:115-138
+      if (dint_draw() < R(0.5f)) return;
+      W *= R(2);
+    }
+    cnt.isect++;
+    R hu, hv;
+    unsigned inst = 0xffffffffu;
+    Ray<R> r = ray;
+    unsigned prim = sc.accel.closest(r, hu, hv, &cnt.nc_c, &cnt.st_c, &inst);
+    if (prim == 0xffffffffu) {
+      total += sc.background(r.dir) * W;
+      if (depth == 0) alpha = R(P.bg_alpha);
+      return;
+    }
+    Isec<R> is;
+    sc.make_isec(prim, inst, r, hu, hv, m0, m1, is);
+    const Material<R> &hm = sc.mats[is.mat];
+    if (hm.type == SGR_MAT_GLOW && !is.back) total += hm.color * W;
+    if (is.bsdf_mat < 0) return;
+    const Material<R> &bm = sc.mats[is.bsdf_mat];
+    Bsdf<R> bsdf;
+    bsdf.init(bm, is, bm.ior_n);
+    total += dint_direct(is, bsdf, pk, s) * W;
+    BsdfSample<R> rs = bsdf.sample(R(0), R(0), SPECULAR | REFLECTIVE);
+    if (rs.val.gt0()) {
+      if (depth + 1 >= kDintDepth) {
+        cnt.dint_ovf++;
+      } else {
+        C3<R> f = rs.val * std::fabs(is.cos_n(rs.dir));
+        dint_Li(sc.recursive_ray(is, rs.dir, 0, false), m0, m1, depth + 1, W * f, pk, s, total,
+                alpha);
+      }
+    }
+    BsdfSample<R> xs = bsdf.sample(R(0), R(0), SPECULAR | TRANSMISSIVE);
+    if (xs.val.gt0()) {
+      if (depth + 1 >= kDintDepth) {
+        cnt.dint_ovf++;
+      } else {
+        bool has;
+        R mior = sc.medium_ior(is.mat, has);
+        R n0 = is.back ? m1 : (has ? mior : R(1)), n1 = is.back ? m1 : m0;
+        C3<R> f = xs.val * std::fabs(is.cos_n(xs.dir));
+        dint_Li(sc.recursive_ray(is, xs.dir, 0, false), n0, n1, depth + 1, W * f, pk, s, total,
+                alpha);
+      }
+    }
+  }
+
+  // RecursiveInteg::Li, top level (render/recursive-integ.cc:176-216)
+  C3<R> dint(const Ray<R> &ray, uint64_t pk, uint64_t qk, uint32_t s, R &alpha) {
+    dq = qk;
+    dn = 0;
+    alpha = 1;
+    C3<R> total(0);
+    dint_Li(ray, R(1), R(1), 0, C3<R>(1), pk, s, total, alpha);
+    if (!total.is_zero()) alpha = 1;
+    return total;
+  }
+
   // One path: camera ray (render-mgr/renderer.cc:71-85,
   // camera/camera.h:322-389) and PathInteg::Li.  Output: sx, sy, Tint.
   void path(const sgr_camera &cam, R max_trace, int px, int py, uint32_t s,
@@ -260,7 +369,8 @@ template<class R> struct Tracer {
     cnt.eye++;
     cnt.paths++;
     R alpha;
-    C3<R> L = Li(ray, pk, qk, s, alpha);
+    C3<R> L = P.integrator == SGR_INTEG_DIRECT ? dint(ray, pk, qk, s, alpha)
+                                               : Li(ray, pk, qk, s, alpha);
     out[0] = cx;
     out[1] = cy;
     out[2] = float(L.r * alpha);  // Tint(radiance, alpha) premultiplies
@@ -422,6 +532,7 @@ static void render_impl(const Scene<R> &sc, const sgr_camera &cam,
     stats->samples = (uint64_t)P.w * P.h * P.spp;
     stats->paths_traced = tot.paths;
     stats->media_overflows = tot.media_ovf;
+    stats->direct_overflows = tot.dint_ovf;
   }
 }
 

@@ -15,6 +15,7 @@ SGR_MAT_GLASS = 3
 SGR_MAT_MIRROR = 4
 SGR_MAT_GLOW = 5
 SGR_MAT_PHONG = 6
+SGR_INTEG_PATH, SGR_INTEG_DIRECT = 0, 1
 SGR_ACCEL_OCTREE = 0
 SGR_ACCEL_TRIV = 1
 MISS = 0xFFFFFFFF
@@ -104,7 +105,8 @@ class RenderParams(C.Structure):
                 ("filter_radius", C.c_float), ("neg_clamp", C.c_float),
                 ("bg_alpha", C.c_float), ("x0", C.c_uint32),
                 ("y0", C.c_uint32), ("w", C.c_uint32), ("h", C.c_uint32),
-                ("paths_in_flight", C.c_uint32), ("flags", C.c_uint32)]
+                ("paths_in_flight", C.c_uint32), ("flags", C.c_uint32),
+                ("integrator", C.c_uint32), ("light_samples", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -116,7 +118,7 @@ class Stats(C.Structure):
                                    "ms_shadow", "ms_splat", "ms_other")] + \
         [("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
          ("media_overflows", C.c_uint64), ("node_fetches_closest", C.c_uint64),
-         ("node_fetches_shadow", C.c_uint64)]
+         ("node_fetches_shadow", C.c_uint64), ("direct_overflows", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -179,7 +181,7 @@ def uptr(a):
 def make_params(width, height, spp, max_path_len=8, min_path_len=3,
                 direct_samples=1, seed=1, window=None, filter_b=1.0 / 3,
                 filter_c=1.0 / 3, filter_radius=2.0, neg_clamp=-0.1,
-                bg_alpha=1.0, paths_in_flight=0, flags=0):
+                bg_alpha=1.0, paths_in_flight=0, flags=0, integrator=0, light_samples=0):
     p = RenderParams()
     p.width, p.height, p.spp = width, height, spp
     p.min_path_len, p.max_path_len = min_path_len, max_path_len
@@ -190,6 +192,7 @@ def make_params(width, height, spp, max_path_len=8, min_path_len=3,
         p.x0, p.y0, p.w, p.h = window
     p.paths_in_flight = paths_in_flight
     p.flags = flags
+    p.integrator, p.light_samples = integrator, light_samples
     return p
 
 
