@@ -801,4 +801,55 @@ __device__ __forceinline__ float mis_weight(float pdf, float other) {
   return (term * pdf) / (term * term + ot * ot);
 }
 
+// The two terms of DirectIllum::sample_light (render/direct-illum.cc:
+// 126-255): a sample of light L weighted against the BSDF's pdf, and a
+// sample of the BSDF weighted against the light's, with the power
+// heuristic.  A term that can contribute goes to USE(c, ray): its value
+// if unoccluded and the shadow ray that decides it (from the hit point
+// to the light sample, or to the scene horizon for an envmap).
+__device__ __forceinline__ RayF direct_shadow_ray(const SceneView &S, const Isec &is, V3 dir,
+                                                  float dist) {
+  const float md = dist == 0.0f ? S.horizon : dist;
+  RayF r;
+  r.o = is.nf.o;
+  r.d = is.nf.from(dir);
+  r.t0 = S.min_trace;
+  r.t1 = md - S.min_trace;
+  return r;
+}
+template<typename Use>
+__device__ __forceinline__ void direct_light_term(const SceneView &S, const DevLight &L,
+                                                  const Isec &is, const DevMaterial &M,
+                                                  float lu, float lv, float extra, Use use) {
+  const LSample ls = light_sample(S, L, is, lu, lv, extra);
+  if (ls.pdf > 0.0f && cgt0(ls.val)) {
+    const BValue bval = bsdf_eval(is, M, ls.dir, F_ALL & ~F_SPECULAR);
+    if (cgt0(bval.val)) {
+      C3 c = ls.val;
+      c = c * mis_weight(ls.pdf, bval.pdf);
+      c = c * bval.val;
+      c = c * fabsf(ls.dir.z);
+      c = c * (1.0f / ls.pdf);
+      use(c, direct_shadow_ray(S, is, ls.dir, ls.dist));
+    }
+  }
+}
+template<typename Use>
+__device__ __forceinline__ void direct_bsdf_term(const SceneView &S, const DevLight &L,
+                                                 const Isec &is, const DevMaterial &M, float bu,
+                                                 float bv, Use use) {
+  const BSample bs = bsdf_sample(is, M, bu, bv, F_ALL & ~F_SPECULAR);
+  if (bs.pdf > 0.0f && cgt0(bs.val)) {
+    const LSample lval = light_eval(S, L, is, bs.dir);
+    if (lval.pdf > 0.0f && cgt0(lval.val)) {
+      C3 c = lval.val;
+      c = c * mis_weight(bs.pdf, lval.pdf);
+      c = c * bs.val;
+      c = c * fabsf(bs.dir.z);
+      c = c * (1.0f / bs.pdf);
+      use(c, direct_shadow_ray(S, is, bs.dir, lval.dist));
+    }
+  }
+}
+
 }  // namespace sgr

@@ -49,6 +49,7 @@ static_assert(SGR_CNT_HE >= 2 * SGR_MAX_BOUNCES + 4 && SGR_CNT_HE % SGR_HEAD_STR
 #define SGR_STAT_MEDIA_OVF 7  // media pushes dropped at SGR_MEDIA_MAX
 #define SGR_STAT_FETCH_C 8    // octree node records loaded (closest hit)
 #define SGR_STAT_FETCH_S 9    // ... (any hit)
+#define SGR_STAT_DINT_OVF 10  // direct integrator: branches beyond SGR_DIRECT_MAX_DEPTH
 #define SGR_NUM_STATS    16
 // The statistics buffer also holds the 8 fetch heads of a ray-batch query,
 // from byte SGR_STATS_HEADS_OFF on, SGR_HEAD_STRIDE words apart.
@@ -68,6 +69,10 @@ struct RenderView {
   uint32_t width, height, spp, minp, maxp, ds, L;
   uint32_t nsub_strat, nsub_rand, K;
   Grid g_cam, g_dir, g_cont;
+  // Surface integrator (SGR_INTEG_*); the direct integrator's light
+  // samples per light and per sample, and their grid (spp x samples).
+  uint32_t integ, nsub_dint;
+  Grid g_dint;
   uint64_t seed;
   float bg_alpha, fb, fc, inv_radius, neg_clamp;
   int fr;
@@ -144,6 +149,8 @@ void launch_extend(bool multi, const SceneView &S, const StepView &J, const Path
                    uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st);
 // G: the learned proposal's view, or NULL for plain BSDF sampling
 // (single-job renders only).
+void launch_direct_integ(const SceneView &S, const RenderView &R, const PathBufs &P,
+                         unsigned long long *stats, unsigned grid, hipStream_t st);
 void launch_shade(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
                   const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
                   hipStream_t st, const GuideView *G = nullptr);

@@ -1002,6 +1002,19 @@ int plan_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p
   R.nsub_rand = grid_shape(ds).total;
   uint32_t nsub_max = std::max(R.nsub_strat, R.nsub_rand);
   R.K = 2 * R.L * nsub_max;
+  R.integ = p->integrator;
+  if (R.integ != SGR_INTEG_PATH && R.integ != SGR_INTEG_DIRECT)
+    return fail(SGR_ERR_INVALID, "sgr_render: unknown integrator " + std::to_string(R.integ));
+  if (R.integ == SGR_INTEG_DIRECT) {
+    // render/direct-integ.h:72-77: "light_samples", 16 by default; the
+    // shadow rays are traced where they are made (no shadow slots)
+    uint32_t ls = p->light_samples ? p->light_samples : 16u;
+    if ((uint64_t)R.spp * ls >= (1ull << 31))
+      return fail(SGR_ERR_LIMIT, "sgr_render: spp x light_samples above 2^31");
+    R.g_dint = grid_shape(R.spp * ls);
+    R.nsub_dint = R.g_dint.total / R.spp;
+    R.K = 0;
+  }
   if (R.K > SGR_MAX_SLOTS)
     return fail(SGR_ERR_LIMIT, "sgr_render: lights x direct samples needs " + std::to_string(R.K) +
                                    " shadow slots per path (limit " + std::to_string(SGR_MAX_SLOTS) + ")");
@@ -1188,7 +1201,12 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     t_begin();
     launch_raygen(false, R, J, P, grid, cur);
     t_end(K_OTHER);
-    for (uint32_t b = 0; b <= R.maxp; b++) {
+    if (R.integ == SGR_INTEG_DIRECT) {
+      t_begin();
+      launch_direct_integ(sc->view, R, P, ctx->stats, grid, cur);
+      t_end(K_OTHER);
+    }
+    for (uint32_t b = 0; R.integ == SGR_INTEG_PATH && b <= R.maxp; b++) {
       if (finish && b >= SGR_FINISH_FROM) {
         t_begin();
         launch_finish(false, sc->view, R, J, P, b, ctx->stats, grid, cur);
@@ -1225,7 +1243,7 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
         HIPCHK(hipEventRecord(ev_splat, cur));
       }
     }
-    launch_count_queues(R.maxp, P, ctx->stats, cur);
+    if (R.integ == SGR_INTEG_PATH) launch_count_queues(R.maxp, P, ctx->stats, cur);
     HIPCHK(hipGetLastError());
     if (guide && train && (lane == lanes - 1 || s0 + pl.sb_max >= R.spp)) {
       // the pair (or last chunk) is done: train on its chunks in order
@@ -1285,6 +1303,7 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     stats->surf_tests_shadow = st[SGR_STAT_TESTS_S];
     stats->illum_calls = st[SGR_STAT_ILLUM];
     stats->media_overflows = st[SGR_STAT_MEDIA_OVF];
+    stats->direct_overflows = st[SGR_STAT_DINT_OVF];
     stats->node_fetches_closest = st[SGR_STAT_FETCH_C];
     stats->node_fetches_shadow = st[SGR_STAT_FETCH_S];
     for (auto &m : marks) {
@@ -1311,6 +1330,8 @@ int render_accum(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *
   RenderPlan pl;
   int rc = plan_render(sc, cam, p, pl);
   if (rc) return rc;
+  if (guide && pl.R.integ != SGR_INTEG_PATH)
+    return fail(SGR_ERR_INVALID, "sgr_render_guided: the learned proposal guides the path integrator only");
   if (!accum) return fail(SGR_ERR_INVALID, "sgr_render: null accumulators");
   sgr_ctx *ctx = sc->ctx;
   HIPCHK(hipSetDevice(ctx->device));

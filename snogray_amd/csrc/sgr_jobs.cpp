@@ -301,6 +301,8 @@ static int sgr_job_submit_impl(sgr_ctx *ctx, sgr_scene *sc, const sgr_camera *ca
   Job j;
   rc = plan_render(sc, cam, p, j.pl);
   if (rc) return rc;
+  if (j.pl.R.integ != SGR_INTEG_PATH)
+    return fail(SGR_ERR_INVALID, "sgr_job_submit: the job engine renders with the path integrator only");
   uint64_t cap = E->cfg.path_capacity;
   if (j.pl.npix > cap)
     return fail(SGR_ERR_LIMIT, "sgr_job_submit: " + std::to_string(j.pl.npix) +

@@ -24,6 +24,7 @@
 
 #include "dev_shade.h"
 #include "kernels.h"
+#include "../../include/snoggpu.h"
 #include "dev_guide.h"
 
 namespace sgr {
@@ -256,6 +257,14 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathB
 // continuation and Russian roulette.  Returns whether the path goes on
 // with the ray it stored.  J also indexes the learned proposal's lobes;
 // ILLUM counts direct-lighting calls.
+// Shadow slot SLOT of the path state: the ray and its contribution.
+__device__ __forceinline__ void put_shadow(const PathBufs &P, size_t slot, C3 c, const RayF &r) {
+  P.sray[0][slot] = r.o.x; P.sray[1][slot] = r.o.y; P.sray[2][slot] = r.o.z;
+  P.sray[3][slot] = r.d.x; P.sray[4][slot] = r.d.y; P.sray[5][slot] = r.d.z;
+  P.sray[6][slot] = r.t0; P.sray[7][slot] = r.t1;
+  P.sc[0][slot] = c.r; P.sc[1][slot] = c.g; P.sc[2][slot] = c.b;
+}
+
 template<bool MULTI, bool GUIDED, typename Emit>
 __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView &R1,
                                            const StepView &J, const PathBufs &P,
@@ -314,7 +323,6 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
       // ---- direct lighting (DirectIllum::sample_all_lights) ----
       if (R.ds != 0) {
         illum++;
-        const unsigned dflags = F_ALL & ~F_SPECULAR;
         bool strat = path_len < R.minp;
         uint32_t nsub = strat ? R.nsub_strat : R.nsub_rand;
         for (uint32_t l = 0; l < R.L; l++) {
@@ -336,46 +344,15 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
             size_t slot_a = (size_t)p * P.K + 2 * (l * nsub + jj), slot_b = slot_a + 1;
             P.svis[slot_a] = 0;
             P.svis[slot_b] = 0;
-            // light sample (render/direct-illum.cc:147-197)
-            LSample ls = light_sample(S, L, is, lu, lv, extra);
-            if (ls.pdf > 0.0f && cgt0(ls.val)) {
-              BValue bval = bsdf_eval(is, M, ls.dir, dflags);
-              if (cgt0(bval.val)) {
-                C3 c = ls.val;
-                c = c * mis_weight(ls.pdf, bval.pdf);
-                c = c * bval.val;
-                c = c * fabsf(ls.dir.z);
-                c = c * (1.0f / ls.pdf);
-                float md = ls.dist == 0.0f ? S.horizon : ls.dist;
-                V3 wd = is.nf.from(ls.dir);
-                P.sray[0][slot_a] = is.nf.o.x; P.sray[1][slot_a] = is.nf.o.y;
-                P.sray[2][slot_a] = is.nf.o.z; P.sray[3][slot_a] = wd.x;
-                P.sray[4][slot_a] = wd.y; P.sray[5][slot_a] = wd.z;
-                P.sray[6][slot_a] = S.min_trace; P.sray[7][slot_a] = md - S.min_trace;
-                P.sc[0][slot_a] = c.r; P.sc[1][slot_a] = c.g; P.sc[2][slot_a] = c.b;
-                emit(slot_a, dir_octant(wd));
-              }
-            }
-            // BSDF sample (render/direct-illum.cc:199-252)
-            BSample bs = bsdf_sample(is, M, bu, bv, dflags);
-            if (bs.pdf > 0.0f && cgt0(bs.val)) {
-              LSample lval = light_eval(S, L, is, bs.dir);
-              if (lval.pdf > 0.0f && cgt0(lval.val)) {
-                C3 c = lval.val;
-                c = c * mis_weight(bs.pdf, lval.pdf);
-                c = c * bs.val;
-                c = c * fabsf(bs.dir.z);
-                c = c * (1.0f / bs.pdf);
-                float md = lval.dist == 0.0f ? S.horizon : lval.dist;
-                V3 wd = is.nf.from(bs.dir);
-                P.sray[0][slot_b] = is.nf.o.x; P.sray[1][slot_b] = is.nf.o.y;
-                P.sray[2][slot_b] = is.nf.o.z; P.sray[3][slot_b] = wd.x;
-                P.sray[4][slot_b] = wd.y; P.sray[5][slot_b] = wd.z;
-                P.sray[6][slot_b] = S.min_trace; P.sray[7][slot_b] = md - S.min_trace;
-                P.sc[0][slot_b] = c.r; P.sc[1][slot_b] = c.g; P.sc[2][slot_b] = c.b;
-                emit(slot_b, dir_octant(wd));
-              }
-            }
+            // light sample, then BSDF sample (render/direct-illum.cc:147-252)
+            direct_light_term(S, L, is, M, lu, lv, extra, [&](C3 c, const RayF &r) {
+              put_shadow(P, slot_a, c, r);
+              emit(slot_a, dir_octant(r.d));
+            });
+            direct_bsdf_term(S, L, is, M, bu, bv, [&](C3 c, const RayF &r) {
+              put_shadow(P, slot_b, c, r);
+              emit(slot_b, dir_octant(r.d));
+            });
           }
         }
         nst |= SGR_ST_PENDING | (strat ? SGR_ST_STRAT : 0u);
@@ -812,6 +789,173 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
   wave_count(&stats[SGR_STAT_SHADOW], shadow);
 }
 
+// Direct-lighting integrator (render/direct-integ.h:31-67 over
+// render/recursive-integ.cc:29-216), one thread per path of the chunk:
+// the sample's whole recursion.  At each hit: emission, direct lighting
+// of every light with nsub_dint light and BSDF samples (the same
+// stratified samples at every depth, as Lo gets the one Sample; shadow
+// rays traced at once), then the perfectly specular reflection and
+// transmission, each weighted by val |cos|.  Beyond depth 5 a branch goes
+// on with probability 1/2 and twice the weight.  The roulette and the
+// sphere light's extra coordinate come from the path's sequential
+// stream (the reference's context.random()).  The recursion runs depth
+// first, reflection before transmission, on a stack of pending
+// transmissions (at most one per level); a branch keeps its weight and
+// its media as (innermost, enclosing) IORs: entering a medium pushes
+// (its IOR, innermost), leaving sets both to the enclosing IOR, as
+// Media (isec, true) does (material/media.cc:25-52).  Radiance is
+// accumulated top-down, weight times emission and direct light.
+#define SGR_DINT_RAND 0xc0000000u  // draw dims of the sequential stream
+struct DintBranch {
+  RayF ray;
+  C3 W;
+  float m0, m1;  // innermost and enclosing IOR
+  uint32_t depth;
+};
+
+__device__ __forceinline__ RayF dint_ray(const SceneView &S, const Isec &is, V3 dir) {
+  RayF r;  // Intersect::recursive_ray (render/intersect.cc:195-218)
+  r.o = is.nf.o;
+  r.d = is.nf.from(dir);
+  r.t0 = S.min_trace;
+  r.t1 = S.horizon;
+  return r;
+}
+
+template<bool INST>
+__global__ void __launch_bounds__(256) k_direct_integ(SceneView S, RenderView R, PathBufs P,
+                                                      unsigned long long *stats) {
+  SGR_STACK_LDS(stk);
+  uint2 spill[SGR_MAX_DEPTH];
+  Nest N{};
+  if constexpr (INST) {
+    SGR_STACK_LDS(stk2);
+    N.L = stk2;
+  }
+  uint2 spill2[INST ? SGR_MAX_DEPTH : 1];
+  N.spill = spill2;
+  DintBranch pend[SGR_DIRECT_MAX_DEPTH];
+  TraceCount cc{0, 0, 0}, cs{0, 0, 0};
+  uint32_t illum = 0, rays = 0, shadow = 0, ovf = 0;
+  const uint32_t nsub = R.nsub_dint;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < R.npaths;
+       p += gridDim.x * blockDim.x) {
+    int px, py;
+    uint32_t s;
+    path_ident(R, p, px, py, s);
+    const uint64_t pk = pixel_key(R.seed, px, py), qk = path_key(pk, s);
+    uint32_t nd = 0;  // sequential draws taken
+    C3 total = c3(0.0f);
+    float alpha = 1.0f;
+    DintBranch cur;
+    load_ray(P, p, cur.ray);
+    cur.W = c3(1.0f);
+    cur.m0 = 1.0f;  // RenderContext::default_medium
+    cur.m1 = 1.0f;
+    cur.depth = 0;
+    uint32_t sp = 0;
+    for (;;) {
+      bool child = false;  // CUR was replaced by its reflection branch
+      bool live = true;
+      if (cur.depth > 5) {  // render/recursive-integ.cc:115-138
+        if (draw(qk, SGR_DINT_RAND + nd++) < 0.5f) live = false;
+        else cur.W = cur.W * 2.0f;
+      }
+      if (live) {
+        TravState T;
+        T.ray = cur.ray;
+        trace_one<false, INST>(S, T, cc, stk, spill, N);
+        rays++;
+        if (T.hit == SGR_MISS) {
+          total = total + background(S, cur.ray.d) * cur.W;
+          if (cur.depth == 0) alpha = R.bg_alpha;
+        } else {
+          RayF hr = cur.ray;
+          hr.t1 = T.ray.t1;
+          Isec is;
+          make_isec(S, T.hit, T.hinst, hr, S.tri_vn ? T.hu : 0.0f, S.tri_vn ? T.hv : 0.0f,
+                    cur.m0, cur.m1, is);
+          const DevMaterial &HM = S.mats[is.mat];
+          if (HM.type == 5 && !is.back) total = total + cload(HM.color) * cur.W;
+          if (is.bmat >= 0) {
+            const DevMaterial &M = S.mats[is.bmat];
+            illum++;
+            C3 D = c3(0.0f);
+            for (uint32_t l = 0; l < R.L; l++) {
+              const DevLight &L = S.lights[l];
+              C3 lrad = c3(0.0f);
+              for (uint32_t jj = 0; jj < nsub; jj++) {
+                float lu, lv, bu, bv;
+                strat_uv(pk, 1 + 2 * l, s * nsub + jj, R.g_dint, lu, lv);
+                strat_uv(pk, 2 + 2 * l, s * nsub + jj, R.g_dint, bu, bv);
+                const float extra = L.kind == 1 ? draw(qk, SGR_DINT_RAND + nd++) : 0.0f;
+                C3 r = c3(0.0f);
+                auto visible = [&](C3 c, const RayF &ray) {
+                  TravState U;
+                  U.ray = ray;
+                  trace_one<true, INST>(S, U, cs, stk, spill, N);
+                  shadow++;
+                  if (U.hit == SGR_MISS) r = r + c;
+                };
+                direct_light_term(S, L, is, M, lu, lv, extra, visible);
+                direct_bsdf_term(S, L, is, M, bu, bv, visible);
+                lrad = lrad + r;
+              }
+              const float fn = (float)nsub;
+              D = D + C3{lrad.r / fn, lrad.g / fn, lrad.b / fn};
+            }
+            total = total + D * cur.W;
+            const BSample xs = bsdf_sample(is, M, 0.0f, 0.0f, F_SPECULAR | F_TRANSMISSIVE);
+            const BSample rs = bsdf_sample(is, M, 0.0f, 0.0f, F_SPECULAR | F_REFLECTIVE);
+            if (cgt0(xs.val)) {
+              if (cur.depth + 1 >= SGR_DIRECT_MAX_DEPTH) {
+                ovf++;
+              } else {
+                DintBranch &b = pend[sp++];
+                b.ray = dint_ray(S, is, xs.dir);
+                b.W = cur.W * (xs.val * fabsf(xs.dir.z));
+                b.m0 = is.back ? cur.m1 : (HM.medium ? M.ior_n : 1.0f);
+                b.m1 = is.back ? cur.m1 : cur.m0;
+                b.depth = cur.depth + 1;
+              }
+            }
+            if (cgt0(rs.val)) {
+              if (cur.depth + 1 >= SGR_DIRECT_MAX_DEPTH) {
+                ovf++;
+              } else {
+                cur.ray = dint_ray(S, is, rs.dir);
+                cur.W = cur.W * (rs.val * fabsf(rs.dir.z));
+                cur.depth++;
+                child = true;
+              }
+            }
+          }
+        }
+      }
+      if (!child) {
+        if (sp == 0) break;
+        cur = pend[--sp];
+      }
+    }
+    if (!czero(total)) alpha = 1.0f;  // render/recursive-integ.cc:212-213
+    float *o = P.out + 6 * (size_t)p;
+    o[2] = total.r * alpha;
+    o[3] = total.g * alpha;
+    o[4] = total.b * alpha;
+    o[5] = alpha;
+  }
+  wave_count(&stats[SGR_STAT_NODES_C], cc.nodes);
+  wave_count(&stats[SGR_STAT_TESTS_C], cc.tests);
+  wave_count(&stats[SGR_STAT_NODES_S], cs.nodes);
+  wave_count(&stats[SGR_STAT_TESTS_S], cs.tests);
+  wave_count(&stats[SGR_STAT_FETCH_C], cc.fetch);
+  wave_count(&stats[SGR_STAT_FETCH_S], cs.fetch);
+  wave_count(&stats[SGR_STAT_ILLUM], illum);
+  wave_count(&stats[SGR_STAT_ISECT], rays);
+  wave_count(&stats[SGR_STAT_SHADOW], shadow);
+  wave_count(&stats[SGR_STAT_DINT_OVF], ovf);
+}
+
 // image/image-mitchell-filt.h:45-65
 __device__ __forceinline__ float mitchell1(float x, float inv_radius, float B, float Cc) {
   x = fabsf(2.0f * x * inv_radius);
@@ -1021,6 +1165,13 @@ void launch_extend(bool multi, const SceneView &S, const StepView &J, const Path
     hipLaunchKernelGGL((k_extend<false, true, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else
     hipLaunchKernelGGL((k_extend<false, false, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
+}
+void launch_direct_integ(const SceneView &S, const RenderView &R, const PathBufs &P,
+                         unsigned long long *stats, unsigned grid, hipStream_t st) {
+  if (S.n_insts)
+    hipLaunchKernelGGL((k_direct_integ<true>), dim3(grid), dim3(256), 0, st, S, R, P, stats);
+  else
+    hipLaunchKernelGGL((k_direct_integ<false>), dim3(grid), dim3(256), 0, st, S, R, P, stats);
 }
 void launch_shade(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
                   const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
