@@ -141,6 +141,7 @@ struct PathBufs {
 
 // Sets the CU count the per-kernel launch caps are derived from.
 void set_launch_cus(unsigned num_cu);
+unsigned launch_cus();
 void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool any,
                   unsigned long long *stats, hipStream_t st);
 void launch_raygen(bool multi, const RenderView &R, const StepView &J, const PathBufs &P,

@@ -7,11 +7,40 @@
 #include "dev_common.h"
 #include "dev_guide.h"
 #include "guide.h"
+#include "kernels.h"
 
 namespace sgr {
 
 typedef short s4v __attribute__((ext_vector_type(4)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+
+// Products accumulate over k in tiles of 16: lane l holds the tile's
+// A[l%16][4(l/16)+i] and B[4(l/16)+i][l%16], i < 4 (the fragments of
+// v_mfma_f32_16x16x16_bf16).  gfx950's v_mfma_f32_16x16x32_bf16 takes
+// two such tiles in one instruction: each lane passes its fragment of
+// tile 2j followed by its fragment of tile 2j+1 as A and as B.  Both
+// operands then list k in the same order, so the instruction sums A.B
+// over both tiles; D[4(l/16)+i][l%16] comes back in acc[i] either way.
+__device__ __forceinline__ f4v mfma_2tiles(s4v a0, s4v a1, s4v b0, s4v b1, f4v acc) {
+  const s8v a = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+  const s8v b = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a),
+                                                 __builtin_bit_cast(bf8v, b), acc, 0, 0, 0);
+}
+__device__ __forceinline__ f4v mfma_tile(s4v a, s4v b, f4v acc) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+}
+// acc + sum over the KT k-tiles of A(kt).B(kt), two tiles per instruction
+// (an odd last tile alone); FA / FB give a lane's fragments of tile kt.
+template<int KT, typename FA, typename FB>
+__device__ __forceinline__ f4v mfma_k(FA fa, FB fb, f4v acc) {
+#pragma unroll
+  for (int kt = 0; kt + 1 < KT; kt += 2) acc = mfma_2tiles(fa(kt), fa(kt + 1), fb(kt), fb(kt + 1), acc);
+  if (KT & 1) acc = mfma_tile(fa(KT - 1), fb(KT - 1), acc);
+  return acc;
+}
 
 #define GD_WAVES 4  // waves per workgroup
 
@@ -36,20 +65,17 @@ struct TileLds {
   float rl[16], rw[16];    // per-row loss and weight
 };
 
-// out[16][N] = act[16][K] . W[N][K]^T + b (optional ReLU), MFMA
-// v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)+i] and
-// B[4(l/16)+i][l%16]; D[4(l/16)+i][l%16] comes back in acc[i].
+// out[16][N] = act[16][K] . W[N][K]^T + b (optional ReLU), on MFMA
+// (mfma_k above).
 template<int K, int N, bool RELU, bool F32OUT>
 __device__ __forceinline__ void mlp_layer(const uint16_t *act, const uint16_t *W, const float *bias,
                                           uint16_t *outb, float *outf, int lane) {
   const int r = lane & 15, g = lane >> 4;
   for (int nt = 0; nt < N / 16; nt++) {
-    f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kt = 0; kt < K / 16; kt++) {
-      const s4v a = *(const s4v *)(act + r * K + kt * 16 + 4 * g);
-      const s4v b = *(const s4v *)(W + (size_t)(nt * 16 + r) * K + kt * 16 + 4 * g);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
+    const f4v acc = mfma_k<K / 16>(
+        [&](int kt) { return *(const s4v *)(act + r * K + kt * 16 + 4 * g); },
+        [&](int kt) { return *(const s4v *)(W + (size_t)(nt * 16 + r) * K + kt * 16 + 4 * g); },
+        f4v{0.0f, 0.0f, 0.0f, 0.0f});
     const int col = nt * 16 + r;
     const float bb = bias[col];
     for (int i = 0; i < 4; i++) {
@@ -69,14 +95,15 @@ __device__ __forceinline__ void mlp_back(const uint16_t *d, const uint16_t *W, c
                                          uint16_t *e, int lane) {
   const int r = lane & 15, g = lane >> 4;
   for (int nt = 0; nt < KIN / 16; nt++) {
-    f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kt = 0; kt < NOUT / 16; kt++) {
-      const s4v a = *(const s4v *)(d + r * NOUT + kt * 16 + 4 * g);
-      s4v b;
-      for (int i = 0; i < 4; i++)
-        b[i] = (short)W[(size_t)(kt * 16 + 4 * g + i) * KIN + nt * 16 + r];
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
+    const f4v acc = mfma_k<NOUT / 16>(
+        [&](int kt) { return *(const s4v *)(d + r * NOUT + kt * 16 + 4 * g); },
+        [&](int kt) {
+          s4v b;
+          for (int i = 0; i < 4; i++)
+            b[i] = (short)W[(size_t)(kt * 16 + 4 * g + i) * KIN + nt * 16 + r];
+          return b;
+        },
+        f4v{0.0f, 0.0f, 0.0f, 0.0f});
     const int col = nt * 16 + r;
     for (int i = 0; i < 4; i++) {
       const int row = 4 * g + i;
@@ -120,22 +147,52 @@ __device__ __forceinline__ void to_lobes(const float *raw, float *L) {
 // and the activations as the B operand (lane l: act[row l%16][k0 +
 // 4(l/16) + i]).  The D fragment of output tile m (lane l: out[row
 // l%16][m0 + 4(l/16) + i]) is then exactly the B fragment of the next
-// layer's k-tile m, so activations stay in registers between layers;
-// each weight fragment is loaded once per FT row tiles.  COUNT (device)
-// overrides N when given.
-#define FT 4  // 16-row tiles per wave and pass
+// layer's k-tile m, so activations stay in registers between layers.
+// The weights (bf16) and biases are staged in LDS once per workgroup,
+// which then loops over row groups: every wave reads its weight
+// fragments from LDS (rows padded by GD_WPAD so that the 16 rows a
+// fragment spans fall in different banks) instead of re-reading 33 KB
+// per group from the caches.  COUNT (device) overrides N when given.
+#define FT 2              // 16-row tiles per wave and pass
+#define GD_FWD_WAVES 8    // waves per forward workgroup
+#define GD_WPAD 8         // bf16 padding per LDS weight row
+struct FwdLds {
+  uint16_t w1[GD_H * (GD_FEAT + GD_WPAD)];
+  uint16_t w2[GD_H * (GD_H + GD_WPAD)];
+  uint16_t w3[GD_H * (GD_H + GD_WPAD)];
+  uint16_t w4[GD_OUTP * (GD_H + GD_WPAD)];
+  float b1[GD_H], b2[GD_H], b3[GD_H], b4[GD_OUTP];
+  float out[GD_FWD_WAVES][FT * 16 * GD_OUT];  // a pass's raw outputs, row-major
+};
+
+// Row-major N x K bf16 matrix from global memory into LDS rows of K +
+// GD_WPAD, 4 values (8 bytes) per copy.
+template<int N, int K>
+__device__ __forceinline__ void stage_weights(const uint16_t *src, uint16_t *dst) {
+  for (int e = threadIdx.x; e < N * K / 4; e += blockDim.x) {
+    const int row = e / (K / 4), c = e - row * (K / 4);
+    *(uint2 *)(dst + row * (K + GD_WPAD) + 4 * c) = *(const uint2 *)(src + row * K + 4 * c);
+  }
+}
 
 template<int KT, int MT, bool RELU>
 __device__ __forceinline__ void layer_t(const uint16_t *W, const float *bias, const s4v (&in)[FT][KT],
                                         f4v (&out)[FT][MT], int lane) {
   const int r = lane & 15, g = lane >> 4;
+  constexpr int LDW = KT * 16 + GD_WPAD;
   for (int mt = 0; mt < MT; mt++) {
     f4v acc[FT];
     for (int t = 0; t < FT; t++) acc[t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kt = 0; kt < KT; kt++) {
-      const s4v a = *(const s4v *)(W + (size_t)(mt * 16 + r) * (KT * 16) + kt * 16 + 4 * g);
+    const s4v *wrow = (const s4v *)(W + (mt * 16 + r) * LDW + 4 * g);
+#pragma unroll
+    for (int kt = 0; kt + 1 < KT; kt += 2) {  // two k-tiles per MFMA (mfma_k)
+      const s4v a0 = wrow[kt * 4], a1 = wrow[(kt + 1) * 4];
       for (int t = 0; t < FT; t++)
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, in[t][kt], acc[t], 0, 0, 0);
+        acc[t] = mfma_2tiles(a0, a1, in[t][kt], in[t][kt + 1], acc[t]);
+    }
+    if (KT & 1) {
+      const s4v a = wrow[(KT - 1) * 4];
+      for (int t = 0; t < FT; t++) acc[t] = mfma_tile(a, in[t][KT - 1], acc[t]);
     }
     for (int t = 0; t < FT; t++)
       for (int i = 0; i < 4; i++) {
@@ -152,16 +209,29 @@ __device__ __forceinline__ void to_bf16_frag(const f4v (&x)[FT][MT], s4v (&y)[FT
       for (int i = 0; i < 4; i++) y[t][m][i] = (short)f2bf(x[t][m][i]);
 }
 
-__global__ void __launch_bounds__(64 * GD_WAVES)
+__global__ void __launch_bounds__(64 * GD_FWD_WAVES)
 k_mlp_forward(GuideNet net, const float *in6, uint32_t n0, const uint32_t *count, float *out,
               int lobes) {
-  __shared__ float obuf[GD_WAVES][FT * 16 * GD_OUTP];
+  __shared__ FwdLds S;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = lane & 15, g = lane >> 4;
   const uint32_t n = count ? *count : n0;
   const uint32_t ngroups = (n + 16 * FT - 1) / (16 * FT);
-  float *ob = obuf[wv];
-  for (uint32_t grp = blockIdx.x * GD_WAVES + wv; grp < ngroups; grp += gridDim.x * GD_WAVES) {
+  if (blockIdx.x * GD_FWD_WAVES >= ngroups) return;  // the whole block: no group to take
+  stage_weights<GD_H, GD_FEAT>(net.w16 + GD_W1, S.w1);
+  stage_weights<GD_H, GD_H>(net.w16 + GD_W2, S.w2);
+  stage_weights<GD_H, GD_H>(net.w16 + GD_W3, S.w3);
+  stage_weights<GD_OUTP, GD_H>(net.w16 + GD_W4, S.w4);
+  for (int e = threadIdx.x; e < GD_H; e += blockDim.x) {
+    S.b1[e] = net.w32[GD_B1 + e];
+    S.b2[e] = net.w32[GD_B2 + e];
+    S.b3[e] = net.w32[GD_B3 + e];
+  }
+  for (int e = threadIdx.x; e < GD_OUTP; e += blockDim.x) S.b4[e] = net.w32[GD_B4 + e];
+  __syncthreads();
+  float *ob = S.out[wv];
+  for (uint32_t grp = blockIdx.x * GD_FWD_WAVES + wv; grp < ngroups;
+       grp += gridDim.x * GD_FWD_WAVES) {
     const uint32_t row0 = grp * 16 * FT;
     // encoded features straight into B fragments: this lane's row r of
     // each tile, features kt*16 + 4g + i
@@ -176,29 +246,31 @@ k_mlp_forward(GuideNet net, const float *in6, uint32_t n0, const uint32_t *count
     }
     f4v h[FT][GD_H / 16];
     s4v hb[FT][GD_H / 16];
-    layer_t<GD_FEAT / 16, GD_H / 16, true>(net.w16 + GD_W1, net.w32 + GD_B1, x0, h, lane);
+    layer_t<GD_FEAT / 16, GD_H / 16, true>(S.w1, S.b1, x0, h, lane);
     to_bf16_frag(h, hb);
-    layer_t<GD_H / 16, GD_H / 16, true>(net.w16 + GD_W2, net.w32 + GD_B2, hb, h, lane);
+    layer_t<GD_H / 16, GD_H / 16, true>(S.w2, S.b2, hb, h, lane);
     to_bf16_frag(h, hb);
-    layer_t<GD_H / 16, GD_H / 16, true>(net.w16 + GD_W3, net.w32 + GD_B3, hb, h, lane);
+    layer_t<GD_H / 16, GD_H / 16, true>(S.w3, S.b3, hb, h, lane);
     to_bf16_frag(h, hb);
     f4v o[FT][GD_OUTP / 16];
-    layer_t<GD_H / 16, GD_OUTP / 16, false>(net.w16 + GD_W4, net.w32 + GD_B4, hb, o, lane);
-    // rows gather their 40 outputs through LDS
+    layer_t<GD_H / 16, GD_OUTP / 16, false>(S.w4, S.b4, hb, o, lane);
+    // rows gather their 40 outputs through LDS (the padding columns are
+    // not kept)
     for (int t = 0; t < FT; t++)
       for (int m = 0; m < GD_OUTP / 16; m++)
-        for (int i = 0; i < 4; i++) ob[(t * 16 + r) * GD_OUTP + m * 16 + 4 * g + i] = o[t][m][i];
+        for (int i = 0; i < 4; i++) {
+          const int c = m * 16 + 4 * g + i;
+          if (c < GD_OUT) ob[(t * 16 + r) * GD_OUT + c] = o[t][m][i];
+        }
     WAVE_SYNC();
-    // each lane turns its row into lobes in place, then the wave writes
-    // the 64 rows' outputs (contiguous in memory) with coalesced stores
-    if (lobes && row0 + lane < n) to_lobes(ob + lane * GD_OUTP, ob + lane * GD_OUTP);
+    // a lane per row turns the row into lobes in place, then the wave
+    // writes the pass's rows (contiguous in memory) with coalesced stores
+    if (lobes && lane < 16 * FT && row0 + lane < n)
+      to_lobes(ob + lane * GD_OUT, ob + lane * GD_OUT);
     WAVE_SYNC();
     const uint32_t rows = n - row0 < 16 * FT ? n - row0 : 16 * FT;
     float *dst = out + (size_t)row0 * GD_OUT;
-    for (uint32_t idx = lane; idx < rows * GD_OUT; idx += 64) {
-      const uint32_t rr = idx / GD_OUT, c = idx - rr * GD_OUT;
-      dst[idx] = ob[rr * GD_OUTP + c];
-    }
+    for (uint32_t idx = lane; idx < rows * GD_OUT; idx += 64) dst[idx] = ob[idx];
     WAVE_SYNC();
   }
 }
@@ -352,16 +424,19 @@ __global__ void __launch_bounds__(64 * GD_WAVES) k_mlp_wgrad(GuideTrain TR, uint
   const uint32_t per = (rows / 16 + GD_WG_SLICES - 1) / GD_WG_SLICES * 16;
   const uint32_t r0 = slice * per, r1 = r0 + per < rows ? r0 + per : rows;
   const int r = lane & 15, g = lane >> 4;
+  // k = the rows of the slice, 16 per tile, two tiles per MFMA
+  auto frag = [&](const uint16_t *m, int ld, int col, uint32_t rb) {
+    s4v v;
+    for (int i = 0; i < 4; i++) v[i] = (short)m[(size_t)(rb + 4 * g + i) * ld + col];
+    return v;
+  };
+  const int dc = T.mt * 16 + r, ac = T.kt * 16 + r;
   f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (uint32_t rb = r0; rb < r1; rb += 16) {
-    s4v av, bv;
-    for (int i = 0; i < 4; i++) {
-      const uint32_t row = rb + 4 * g + i;
-      av[i] = (short)d[(size_t)row * nout + T.mt * 16 + r];
-      bv[i] = (short)a[(size_t)row * kin + T.kt * 16 + r];
-    }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-  }
+  uint32_t rb = r0;
+  for (; rb + 32 <= r1; rb += 32)
+    acc = mfma_2tiles(frag(d, nout, dc, rb), frag(d, nout, dc, rb + 16), frag(a, kin, ac, rb),
+                      frag(a, kin, ac, rb + 16), acc);
+  if (rb < r1) acc = mfma_tile(frag(d, nout, dc, rb), frag(a, kin, ac, rb), acc);
   float *out = TR.wpart + (size_t)slice * GD_NPARAM;
   for (int i = 0; i < 4; i++) {
     const int m = T.mt * 16 + 4 * g + i, k = T.kt * 16 + r;
@@ -450,8 +525,11 @@ void launch_guide_inputs(const GuideView &G, const uint32_t *queue, const uint32
 
 void launch_mlp_forward(const GuideNet &net, const float *in6, uint32_t n, const uint32_t *count,
                         float *out, int lobes, unsigned grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64 * GD_WAVES), 0, st, net, in6, n, count,
-                     out, lobes);
+  // two workgroups fit a CU (their LDS); each loops over row groups
+  const unsigned cap = 2u * launch_cus();
+  grid = grid < cap ? grid : cap;
+  hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64 * GD_FWD_WAVES), 0, st, net, in6, n,
+                     count, out, lobes);
 }
 
 void launch_mlp_rows(const GuideNet &net, const GuideTrain &T, const float *rec, uint32_t n,

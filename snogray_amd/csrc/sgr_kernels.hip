@@ -1102,6 +1102,7 @@ __global__ void __launch_bounds__(64) k_count_queues(uint32_t maxp, PathBufs P,
 // compaction-free kernels.
 static unsigned s_num_cu = 256;
 void set_launch_cus(unsigned num_cu) { s_num_cu = num_cu ? num_cu : 256u; }
+unsigned launch_cus() { return s_num_cu; }
 
 static inline unsigned grid_for(uint32_t n, unsigned cap) {
   unsigned g = (n + 255) / 256;

@@ -166,27 +166,36 @@ def test_delta_bsdfs_bypass_the_guide(ctx):
 
 
 def test_trained_guide_gain_at_equal_spp(ctx):
-    """relMSE against a 2048-spp plain render: guided (trained online on
-    this view) vs plain, both at 16 spp over several seeds."""
+    """relMSE against a 2048-spp plain render of guided renders (the
+    guide trained online on the same view, four 512 x 512 renders of 16
+    spp as in a benchmark frame) and of plain renders, both at 16 spp, on
+    a fixed set of 8 seeds (renders and training are deterministic): the
+    median of the per-seed ratios must not fall below 0.8.  Per seed the
+    ratio is decided by whether the plain render holds one of the Russian
+    roulette's rare, very bright samples (4 of these 8 do: median 3.8);
+    in pixels without one the two estimators are about equal per sample,
+    so this asserts that the guide does no harm, not the equal-time
+    target, whose measurement needs the 1080p frame's many pixels."""
     sd, cam = scenes.cornell_box()
     sc = native.Scene(ctx, sd)
-    W = 64
+    W = 96
     ref = native.finalize(sc.render(cam, abi.make_params(W, W, 2048, max_path_len=8,
                                                          seed=77))[0])[..., :3]
     g = native.Guide(ctx, seed=8)
-    for s in range(6):
-        sc.render_guided(cam, abi.make_params(W, W, 16, max_path_len=8, seed=200 + s), g)
+    for s in range(4):
+        sc.render_guided(cam, abi.make_params(512, 512, 16, max_path_len=8, seed=200 + s), g)
 
     def relmse(x):
         return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
-    rg, rp = [], []
-    for s in range(6):
+    ratios, rg, rp = [], [], []
+    for s in range(8):
         p = abi.make_params(W, W, 16, max_path_len=8, seed=300 + s)
         rg.append(relmse(native.finalize(sc.render_guided(cam, p, g, train=False)[0])[..., :3]))
         rp.append(relmse(native.finalize(sc.render(cam, p)[0])[..., :3]))
-    gain = np.mean(rp) / np.mean(rg)
-    print(f"relMSE plain {np.mean(rp):.5f} guided {np.mean(rg):.5f} gain {gain:.3f}")
-    assert gain > 0.8   # the guide never makes the estimate much worse
+        ratios.append(rp[-1] / rg[-1])
+    print("guided", [f"{x:.4g}" for x in rg], "plain", [f"{x:.4g}" for x in rp],
+          "median gain", f"{np.median(ratios):.3f}")
+    assert np.median(ratios) >= 0.8
     g.close()
     sc.close()
 
