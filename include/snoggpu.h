@@ -53,6 +53,7 @@ extern "C" {
 #define SGR_MAT_MIRROR         4
 #define SGR_MAT_GLOW           5
 #define SGR_MAT_PHONG          6
+#define SGR_MAT_THIN_GLASS     7
 
 /* Acceleration structure (render/global-render-state.cc:63-74 "accel"). */
 #define SGR_ACCEL_OCTREE  0
@@ -71,7 +72,12 @@ typedef struct sgr_scene sgr_scene;
  *   GLOW           color = emitted radiance; underlying = index of the
  *                  material supplying the BSDF, or -1 for none
  *   PHONG          color = diffuse colour, gloss_color = specular colour,
- *                  m = Phong exponent (material/phong.h:26-37)           */
+ *                  m = Phong exponent (material/phong.h:26-37)
+ *   THIN_GLASS     color = transmitted colour, ior_n = IOR
+ *                  (material/thin-glass.h:30-32, ThinGlass (col, ior)):
+ *                  specular reflection and straight-through transmission,
+ *                  and shadow rays pass it, multiplied by its
+ *                  transmittance; not allowed in a model              */
 typedef struct sgr_material {
     int32_t type;
     float   color[3];

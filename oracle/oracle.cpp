@@ -94,9 +94,12 @@ template<class R> struct Tracer {
     st.init_direct(p.light_samples);
   }
 
-  bool occluded(const Ray<R> &r) {
+  // Scene::occludes: TRANS is multiplied by the partially occluding
+  // surfaces the ray passes (MEDIUM: the index of the medium at its
+  // origin, isec.media.medium).
+  bool occluded(const Ray<R> &r, R medium, C3<R> &trans) {
     cnt.shadow++;
-    return sc.accel.occluded(r, &cnt.nc_s, &cnt.st_s);
+    return sc.occludes(r, medium, trans, &cnt.nc_s, &cnt.st_s);
   }
 
   // DirectIllum::sample_light (render/direct-illum.cc:126-255)
@@ -108,8 +111,9 @@ template<class R> struct Tracer {
       BsdfValue<R> bval = bsdf.eval(ls.dir, flags);
       if (bval.val.gt0()) {
         Ray<R> ray = sc.recursive_ray(is, ls.dir, ls.dist, true);
-        if (!occluded(ray)) {
-          C3<R> r = ls.val;
+        C3<R> trans(1);
+        if (!occluded(ray, is.media_ior, trans)) {
+          C3<R> r = ls.val * trans;
           r *= mis_weight(ls.pdf, bval.pdf);
           r *= bval.val;
           r *= std::fabs(is.cos_n(ls.dir));
@@ -123,8 +127,9 @@ template<class R> struct Tracer {
       LightSample<R> lval = sc.light_eval(L, is, bs.dir);
       if (lval.pdf > 0 && lval.val.gt0()) {
         Ray<R> ray = sc.recursive_ray(is, bs.dir, lval.dist, true);
-        if (!occluded(ray)) {
-          C3<R> r = lval.val;
+        C3<R> trans(1);
+        if (!occluded(ray, is.media_ior, trans)) {
+          C3<R> r = lval.val * trans;
           r *= mis_weight(bs.pdf, lval.pdf);
           r *= bs.val;
           r *= std::fabs(is.cos_n(bs.dir));
@@ -436,7 +441,10 @@ static void trace_impl(const Scene<R> &sc, const sgr_rays *rays, size_t n,
         hits->v[i] = float(v);
         if (hits->inst) hits->inst[i] = inst;
       }
-      if (occ) occ[i] = sc.accel.occluded(r, &cs[t].nc_s, &cs[t].st_s) ? 1 : 0;
+      if (occ) {  // Scene::occludes from a point in air
+        C3<R> trans(1);
+        occ[i] = sc.occludes(r, R(1), trans, &cs[t].nc_s, &cs[t].st_s) ? 1 : 0;
+      }
     }
   });
   if (counters) {

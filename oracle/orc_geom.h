@@ -480,6 +480,14 @@ template<class R> struct Accel {
     const Accel *acc;
     Ray<R> *ray;
     bool any;
+    // Shadow rays through partially occluding surfaces (space/space.cc:
+    // 131-194, material/material.h:168-192): for a surface the ray meets,
+    // PARTIAL (PCTX, p, t) multiplies the ray's transmittance by the
+    // surface's and returns whether the ray is blocked (a fully occluding
+    // surface, or a transmittance below Eps); a surface it lets through
+    // counts as a miss, as the reference's callback reports it.
+    int (*partial)(void *, unsigned, R) = nullptr;
+    void *pctx = nullptr;
     bool stop = false;
     unsigned hit = 0xffffffffu;
     unsigned hit_inst = 0xffffffffu;
@@ -525,13 +533,21 @@ template<class R> struct Accel {
         return true;
       }
       if (acc->test(p, *ray, t, u, v)) {
-        if (any) { stop = true; hit = p; hit_inst = 0xffffffffu; return true; }
-        ray->t1 = t;
-        hit = p;
-        hit_inst = 0xffffffffu;
-        hu = u;
-        hv = v;
-        return true;
+        if (any) {
+          if (!partial || partial(pctx, p, t)) {
+            stop = true;
+            hit = p;
+            hit_inst = 0xffffffffu;
+            return true;
+          }
+        } else {
+          ray->t1 = t;
+          hit = p;
+          hit_inst = 0xffffffffu;
+          hu = u;
+          hv = v;
+          return true;
+        }
       }
 #ifdef ORC_MAILBOX
       me.gen = tl_mailbox.gen;
@@ -625,13 +641,15 @@ template<class R> struct Accel {
 
   // Any hit (space/space.cc:182-194; every material here fully occludes,
   // material/material.h:101-112).
-  bool occluded(const Ray<R> &ray_in, uint64_t *nc = nullptr,
-                uint64_t *st = nullptr) const {
+  bool occluded(const Ray<R> &ray_in, uint64_t *nc = nullptr, uint64_t *st = nullptr,
+                int (*partial)(void *, unsigned, R) = nullptr, void *pctx = nullptr) const {
     Ray<R> ray = ray_in;
     Search ss;
     ss.acc = this;
     ss.ray = &ray;
     ss.any = true;
+    ss.partial = partial;
+    ss.pctx = pctx;
     search(ss);
     if (nc) *nc += ss.node_calls;
     if (st) *st += ss.surf_tests;

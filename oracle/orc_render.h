@@ -185,6 +185,10 @@ template<class R> struct Bsdf {
       bool entering = !is.back;
       old_ior = entering ? is.media_ior : glass_medium_ior;
       new_ior = entering ? glass_medium_ior : is.enclosing_ior;
+    } else if (type == SGR_MAT_THIN_GLASS) {  // material/thin-glass.cc:30-46
+      color = mt.color;
+      fr_n1 = is.media_ior;
+      fr_n = mt.ior_n;
     } else if (type == SGR_MAT_PHONG) {  // material/phong.cc:38-45
       color = mt.color;
       gloss = mt.gloss;
@@ -417,6 +421,38 @@ template<class R> struct Bsdf {
     return r;
   }
 
+  // --- Thin glass (material/thin-glass.cc:30-70): Fresnel reflection at
+  // the view angle; the rest, times the colour, goes straight through.
+  // The choice compares u with the colour of xmit / (xmit + refl) as
+  // Color < Color does (component sums), and a single allowed direction
+  // is taken only when it is TRANSMISSIVE: asked for REFLECTIVE alone,
+  // the sample still transmits whenever u (0 in a recursion) is below
+  // that ratio.  Both branches report pdf 1.
+  BsdfSample<R> thin_glass_sample(R u, R, unsigned flags) const {
+    BsdfSample<R> s;
+    if (flags & SPECULAR) {
+      flags &= ALL_DIRECTIONS;
+      R cos_xmit = isec->cos_n(isec->v);
+      R refl = Fresnel<R>(fr_n1, fr_n).reflectance(cos_xmit);
+      C3<R> xmit = color * (R(1) - refl);
+      C3<R> sum = xmit + C3<R>(refl);  // Color / Color: 0 where the divisor is 0
+      C3<R> ratio(sum.r == 0 ? R(0) : xmit.r / sum.r, sum.g == 0 ? R(0) : xmit.g / sum.g,
+                  sum.b == 0 ? R(0) : xmit.b / sum.b);
+      if (flags == TRANSMISSIVE || C3<R>(u).sum() < ratio.sum()) {
+        s.val = xmit;
+        s.pdf = 1;
+        s.dir = -isec->v;
+        s.flags = SPECULAR | TRANSMISSIVE;
+      } else {
+        s.val = C3<R>(refl);
+        s.pdf = 1;
+        s.dir = mirror(isec->v, V3<R>(0, 0, 1));
+        s.flags = SPECULAR | REFLECTIVE;
+      }
+    }
+    return s;
+  }
+
   BsdfSample<R> sample(R u, R v, unsigned flags) const {
     switch (type) {
       case SGR_MAT_LAMBERT: return lambert_sample(u, v, flags, color);
@@ -424,6 +460,7 @@ template<class R> struct Bsdf {
       case SGR_MAT_GLASS: return glass_sample(u, v, flags);
       case SGR_MAT_MIRROR: return mirror_sample(u, v, flags);
       case SGR_MAT_PHONG: return phong_sample(u, v, flags);
+      case SGR_MAT_THIN_GLASS: return thin_glass_sample(u, v, flags);
     }
     return BsdfSample<R>();
   }
@@ -640,6 +677,57 @@ template<class R> struct Scene {
     while (m >= 0 && mats[m].type == SGR_MAT_GLOW && guard++ < 8) m = mats[m].underlying;
     return m;
   }
+  // Material::fully_occluding (material/material.h:98-100): thin glass
+  // lets shadow rays through.
+  bool partial_mat(int m) const { return m >= 0 && mats[m].type == SGR_MAT_THIN_GLASS; }
+  bool has_partial = false;
+
+  // Material::transmittance of partially occluding surface P met by a
+  // shadow ray at T, in a medium of index MEDIUM (material/thin-glass.cc:
+  // 123-140): the colour times one minus the Fresnel reflectance at the
+  // angle between the surface normal (IsecInfo::normal: the unit
+  // geometric normal, surface/mesh.cc:714-718, surface/tripar.cc:78-81,
+  // surface/sphere.cc:95-98) and the reversed ray direction.
+  C3<R> transmittance(unsigned p, const Ray<R> &ray, R t, R medium) const {
+    const Prim<R> &pr = prims[p];
+    const Material<R> &m = mats[prim_mat[p]];
+    V3<R> n = pr.kind == 2 ? (ray.origin + ray.dir * t - pr.center).unit()
+                           : cross(pr.e2, pr.e1).unit();
+    R c = float(dot(n, -ray.dir));  // cos_angle returns a float
+    R refl = Fresnel<R>(medium, m.ior_n).reflectance(c);
+    return m.color * (R(1) - refl);
+  }
+
+  // Scene::occludes (render/scene.h:68-87 -> space/space.cc:131-194): a
+  // fully occluding surface on the ray blocks it; each partially
+  // occluding one multiplies TRANS by its transmittance, once however
+  // many octree nodes list it (the reference's negative mailbox skips a
+  // surface it already met), and blocks the ray once TRANS falls below
+  // Eps (Color < Eps: the component sum below 3 Eps).
+  struct Occl {
+    const Scene *sc;
+    const Ray<R> *ray;
+    R medium;
+    C3<R> trans;
+    std::vector<unsigned> seen;
+  };
+  static int occl_partial(void *c, unsigned p, R t) {
+    Occl &o = *(Occl *)c;
+    if (!o.sc->partial_mat(o.sc->prim_mat[p])) return 1;
+    for (unsigned q : o.seen)
+      if (q == p) return 0;
+    o.seen.push_back(p);
+    o.trans *= o.sc->transmittance(p, *o.ray, t, o.medium);
+    return o.trans.sum() < R(3e-15f) ? 1 : 0;
+  }
+  bool occludes(const Ray<R> &ray, R medium, C3<R> &trans, uint64_t *nc, uint64_t *st) const {
+    if (!has_partial) return accel.occluded(ray, nc, st);
+    Occl o{this, &ray, medium, trans, {}};
+    const bool b = accel.occluded(ray, nc, st, &Scene::occl_partial, &o);
+    trans = o.trans;
+    return b;
+  }
+
   // material/material.h medium(): the glass medium, through GLOW wrappers.
   R medium_ior(int m, bool &has) const {
     int b = bsdf_material(m);
@@ -650,7 +738,7 @@ template<class R> struct Scene {
   void build(const sgr_scene_desc &d) {
     for (unsigned i = 0; i < d.num_materials; i++) {
       const sgr_material &m = d.materials[i];
-      if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_PHONG)
+      if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_THIN_GLASS)
         throw std::runtime_error("bad material type");
       mats.push_back(Material<R>{m.type, C3<R>(m.color[0], m.color[1], m.color[2]),
                                  C3<R>(m.gloss_color[0], m.gloss_color[1], m.gloss_color[2]),
@@ -747,6 +835,8 @@ template<class R> struct Scene {
       md.accel = d.accel;
       models.emplace_back(new Scene);
       models.back()->build(md);
+      if (models.back()->has_partial)
+        throw std::runtime_error("a model's surfaces must occlude fully (no thin glass)");
     }
     for (unsigned i = 0; i < d.num_instances; i++) {
       const sgr_instance &in = d.instances[i];
@@ -780,6 +870,7 @@ template<class R> struct Scene {
       bbox.add(ib);
     }
     accel.n_prims = n_direct;
+    for (unsigned p = 0; p < n_direct; p++) has_partial = has_partial || partial_mat(prim_mat[p]);
     if (!insts.empty()) {
       accel.insts = &insts;
       for (auto &m : models) accel.models.push_back(&m->accel);

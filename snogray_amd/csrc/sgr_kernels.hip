@@ -901,8 +901,7 @@ __global__ void __launch_bounds__(256) k_direct_integ(SceneView S, RenderView R,
                 direct_bsdf_term(S, L, is, M, bu, bv, visible);
                 lrad = lrad + r;
               }
-              const float fn = (float)nsub;
-              D = D + C3{lrad.r / fn, lrad.g / fn, lrad.b / fn};
+              D = D + lrad * (1.0f / (float)nsub);  // Color / n: times 1 / n
             }
             total = total + D * cur.W;
             const BSample xs = bsdf_sample(is, M, 0.0f, 0.0f, F_SPECULAR | F_TRANSMISSIVE);

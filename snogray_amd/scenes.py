@@ -71,6 +71,9 @@ class SceneDesc:
         return self._mat(type=abi.SGR_MAT_PHONG, color=color,
                          gloss_color=specular_color, m=exponent)
 
+    def thin_glass(self, color=(1.0, 1.0, 1.0), ior=1.5):
+        return self._mat(type=abi.SGR_MAT_THIN_GLASS, color=color, ior_n=ior)
+
     def glow(self, color, underlying=-1):
         return self._mat(type=abi.SGR_MAT_GLOW, color=color,
                          underlying=underlying)
