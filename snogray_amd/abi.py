@@ -119,7 +119,8 @@ class Stats(C.Structure):
                                    "ms_shadow", "ms_splat", "ms_other")] + \
         [("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
          ("media_overflows", C.c_uint64), ("node_fetches_closest", C.c_uint64),
-         ("node_fetches_shadow", C.c_uint64), ("direct_overflows", C.c_uint64)]
+         ("node_fetches_shadow", C.c_uint64), ("direct_overflows", C.c_uint64),
+         ("partial_overflows", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

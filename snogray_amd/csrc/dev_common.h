@@ -32,6 +32,7 @@ struct DevMaterial {
   float diff_weight, inv_diff_weight, inv_gloss_weight;
   unsigned gloss_layer, have_layers;
   int mirror_under;  // Mirror: underlying Lambert present
+  int partial;       // lets shadow rays through, times its transmittance
 };
 
 struct DevLight {
@@ -87,6 +88,10 @@ struct SceneView {
   const SceneView *models;
   const DevInstance *insts;
   uint32_t n_insts;
+  // Surfaces of partially occluding materials (thin glass) in this scene:
+  // shadow rays take the general walk, which multiplies their
+  // transmittance (such scenes, like instanced ones, use the INST kernels).
+  uint32_t has_partial;
   uint32_t n_tri_like;       // triangles + tripars
   uint32_t n_tri;            // mesh triangles
   uint32_t n_prims;
@@ -214,6 +219,58 @@ __device__ __forceinline__ void strat_uv(uint64_t pk, uint32_t chan, uint32_t e,
   float us = 1.0f / (float)g.u, vs = 1.0f / (float)g.v;
   u = clamp01f((float)j * us + draw(ck, 2 * e) * us);
   v = clamp01f((float)i * vs + draw(ck, 2 * e + 1) * vs);
+}
+
+// Colours (color/color.h; comparisons by component sum, :241-264).
+struct C3 {
+  float r, g, b;
+};
+__device__ __forceinline__ C3 c3(float v) { return C3{v, v, v}; }
+__device__ __forceinline__ C3 c3(float a, float b, float c) { return C3{a, b, c}; }
+__device__ __forceinline__ C3 operator+(C3 a, C3 b) { return C3{a.r + b.r, a.g + b.g, a.b + b.b}; }
+__device__ __forceinline__ C3 operator*(C3 a, C3 b) { return C3{a.r * b.r, a.g * b.g, a.b * b.b}; }
+__device__ __forceinline__ C3 operator*(C3 a, float s) { return C3{a.r * s, a.g * s, a.b * s}; }
+__device__ __forceinline__ float csum(C3 a) { float s = a.r; s += a.g; s += a.b; return s; }
+__device__ __forceinline__ bool cgt0(C3 a) { return csum(a) > 0.0f; }
+__device__ __forceinline__ bool czero(C3 a) { return a.r == 0.0f && a.g == 0.0f && a.b == 0.0f; }
+__device__ __forceinline__ C3 cload(const float *p) { return C3{p[0], p[1], p[2]}; }
+
+// material/fresnel.h:62-130
+__device__ __forceinline__ float fresnel_refl(float n, float k, float cosr) {
+  float c = cosr;
+  c = c < 1.0f ? c : 1.0f;
+  c = c > -1.0f ? c : -1.0f;
+  c = fabsf(c);
+  float ang = acosf(c);
+  float Rs, Rp;
+  if (k == 0.0f) {
+    float st = sinf(ang) / n;
+    st = st < 1.0f ? st : 1.0f;
+    st = st > -1.0f ? st : -1.0f;
+    float ct = sqrtf(1.0f - st * st);
+    float nc1 = n * c, nc2 = n * ct;
+    float Fs = (nc1 - ct) / (nc1 + ct);
+    float Fp = (c - nc2) / (c + nc2);
+    Rs = Fs * Fs;
+    Rp = Fp * Fp;
+  } else {
+    float n2k2 = n * n * k * k, n2mk2 = n * n - k * k;
+    float sr = sinf(ang);
+    float a0 = n2mk2 - sr * sr;
+    float stn = sr * tanf(ang);
+    float common = sqrtf(a0 * a0 + 4.0f * n2k2);
+    float a2 = (common + a0) / 2.0f, b2 = (common - a0) / 2.0f;
+    float apb = a2 + b2, a = sqrtf(a2);
+    float rs1 = apb + c * c, rs2 = 2.0f * a * c;
+    Rs = (rs1 - rs2) / (rs1 + rs2);
+    float rp1 = apb + stn * stn, rp2 = 2.0f * a * stn;
+    Rp = Rs * ((rp1 - rp2) / (rp1 + rp2));
+  }
+  return clamp01f((Rs + Rp) / 2.0f);
+}
+// Fresnel(n1, Ior(n2, k2)).reflectance
+__device__ __forceinline__ float fresnel(float n1, float n2, float k2, float c) {
+  return fresnel_refl(n2 / n1, k2 / n1, c);
 }
 
 }  // namespace sgr

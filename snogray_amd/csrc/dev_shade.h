@@ -14,19 +14,6 @@ enum : unsigned {  // material/bsdf.h:46-70
   F_SPECULAR = 0x01, F_GLOSSY = 0x02, F_DIFFUSE = 0x04, F_ALL = 0x37
 };
 
-struct C3 {
-  float r, g, b;
-};
-__device__ __forceinline__ C3 c3(float v) { return C3{v, v, v}; }
-__device__ __forceinline__ C3 c3(float a, float b, float c) { return C3{a, b, c}; }
-__device__ __forceinline__ C3 operator+(C3 a, C3 b) { return C3{a.r + b.r, a.g + b.g, a.b + b.b}; }
-__device__ __forceinline__ C3 operator*(C3 a, C3 b) { return C3{a.r * b.r, a.g * b.g, a.b * b.b}; }
-__device__ __forceinline__ C3 operator*(C3 a, float s) { return C3{a.r * s, a.g * s, a.b * s}; }
-__device__ __forceinline__ float csum(C3 a) { float s = a.r; s += a.g; s += a.b; return s; }
-__device__ __forceinline__ bool cgt0(C3 a) { return csum(a) > 0.0f; }
-__device__ __forceinline__ bool czero(C3 a) { return a.r == 0.0f && a.g == 0.0f && a.b == 0.0f; }
-__device__ __forceinline__ C3 cload(const float *p) { return C3{p[0], p[1], p[2]}; }
-
 // Intersection state (render/intersect.cc:86-136).
 struct Isec {
   Frame nf, gf;
@@ -151,44 +138,6 @@ __device__ __forceinline__ void make_isec(const SceneView &S, uint32_t p, uint32
 
 __device__ __forceinline__ float cos_geom(const Isec &is, V3 d) { return dot(d, is.gn); }
 __device__ __forceinline__ float cos_v(const Isec &is, V3 d) { return fmn(dot(is.v, d), 1.0f); }
-
-// material/fresnel.h:62-130
-__device__ __forceinline__ float fresnel_refl(float n, float k, float cosr) {
-  float c = cosr;
-  c = c < 1.0f ? c : 1.0f;
-  c = c > -1.0f ? c : -1.0f;
-  c = fabsf(c);
-  float ang = acosf(c);
-  float Rs, Rp;
-  if (k == 0.0f) {
-    float st = sinf(ang) / n;
-    st = st < 1.0f ? st : 1.0f;
-    st = st > -1.0f ? st : -1.0f;
-    float ct = sqrtf(1.0f - st * st);
-    float nc1 = n * c, nc2 = n * ct;
-    float Fs = (nc1 - ct) / (nc1 + ct);
-    float Fp = (c - nc2) / (c + nc2);
-    Rs = Fs * Fs;
-    Rp = Fp * Fp;
-  } else {
-    float n2k2 = n * n * k * k, n2mk2 = n * n - k * k;
-    float sr = sinf(ang);
-    float a0 = n2mk2 - sr * sr;
-    float stn = sr * tanf(ang);
-    float common = sqrtf(a0 * a0 + 4.0f * n2k2);
-    float a2 = (common + a0) / 2.0f, b2 = (common - a0) / 2.0f;
-    float apb = a2 + b2, a = sqrtf(a2);
-    float rs1 = apb + c * c, rs2 = 2.0f * a * c;
-    Rs = (rs1 - rs2) / (rs1 + rs2);
-    float rp1 = apb + stn * stn, rp2 = 2.0f * a * stn;
-    Rp = Rs * ((rp1 - rp2) / (rp1 + rp2));
-  }
-  return clamp01f((Rs + Rp) / 2.0f);
-}
-// Fresnel(n1, Ior(n2, k2)).reflectance
-__device__ __forceinline__ float fresnel(float n1, float n2, float k2, float c) {
-  return fresnel_refl(n2 / n1, k2 / n1, c);
-}
 
 // geometry/spherical-coords.h:155-168 via material/dist.h:28-31
 __device__ __forceinline__ V3 z_symm_vec(float cos_theta, float lng) {
@@ -445,6 +394,37 @@ __device__ __forceinline__ BSample glass_sample(const Isec &is, const DevMateria
   return s;
 }
 
+// material/thin-glass.cc:30-70: Fresnel reflection at the view angle,
+// the rest times the colour straight through.  The choice compares u
+// with the colour of xmit / (xmit + refl) by component sums (Color <
+// Color, with Color / Color giving 0 for a zero divisor), and a single
+// allowed direction is honoured only when it is TRANSMISSIVE: asked for
+// REFLECTIVE alone, the sample still transmits whenever u (0 in a
+// recursion) is below that ratio.  Both branches report pdf 1.
+__device__ __forceinline__ BSample thin_glass_sample(const Isec &is, const DevMaterial &M,
+                                                     float u, unsigned flags) {
+  BSample s = bsample_none();
+  if (flags & F_SPECULAR) {
+    flags &= F_ALL_DIRECTIONS;
+    const float refl = fresnel(is.media_ior, M.ior_n, 0.0f, is.v.z);
+    const C3 xmit = cload(M.color) * (1.0f - refl);
+    const C3 sum = xmit + c3(refl);
+    const C3 ratio = c3(sum.r == 0.0f ? 0.0f : xmit.r / sum.r, sum.g == 0.0f ? 0.0f : xmit.g / sum.g,
+                        sum.b == 0.0f ? 0.0f : xmit.b / sum.b);
+    s.pdf = 1.0f;
+    if (flags == F_TRANSMISSIVE || csum(c3(u)) < csum(ratio)) {
+      s.val = xmit;
+      s.dir = -is.v;
+      s.flags = F_SPECULAR | F_TRANSMISSIVE;
+    } else {
+      s.val = c3(refl);
+      s.dir = v3(0.0f, 0.0f, 1.0f) * (is.v.z * 2.0f) - is.v;
+      s.flags = F_SPECULAR | F_REFLECTIVE;
+    }
+  }
+  return s;
+}
+
 // material/mirror.cc:54-120
 __device__ __forceinline__ BSample mirror_sample(const Isec &is, const DevMaterial &M,
                                                  float u, float v, unsigned flags) {
@@ -479,6 +459,7 @@ __device__ __forceinline__ BSample bsdf_sample(const Isec &is, const DevMaterial
     case 3: return glass_sample(is, M, u, flags);
     case 4: return mirror_sample(is, M, u, v, flags);
     case 6: return phong_sample(is, M, u, v, flags);
+    case 7: return thin_glass_sample(is, M, u, flags);
   }
   return bsample_none();
 }

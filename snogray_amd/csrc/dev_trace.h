@@ -99,6 +99,7 @@ __device__ __forceinline__ bool ref_hit(const float4 *refs, uint32_t k, const Ra
 // included); tests: primitive tests; fetch: node records loaded.
 struct TraceCount {
   uint32_t nodes, tests, fetch;
+  uint32_t povf;  // partial occluders met beyond SGR_PARTIAL_SEEN (multiplied undeduplicated)
 };
 
 // Child occupancy of a node in visiting order: bit j is set when the
@@ -164,8 +165,16 @@ static_assert(SGR_MAX_DEPTH <= 2 * SGR_TRAIL_LO, "the trail holds 2 x 21 levels"
 // then picks the child to enter next or pops one stack entry.  Every
 // step thus costs one node load at most and one evaluation of the
 // eight children, so the lanes of a wave stay on the same instructions.
+#define SGR_PARTIAL_SEEN 8  // partial occluders a shadow ray remembers (counted once each)
 struct TravState {
   RayF ray;
+  // Shadow rays in scenes with partial occluders (INST walks): the index
+  // of the medium at the ray's origin, the transmittance so far and the
+  // partial occluders already multiplied.
+  float medium;
+  C3 trans;
+  uint32_t nseen;
+  uint32_t seen[SGR_PARTIAL_SEEN];
   uint32_t hinst;     // instance of the hit (SGR_MISS: a surface of the scene itself)
   uint32_t oct, cn, first, mask, vis;
   float bx0, bx1, by0, by1, bz0, bz1;
@@ -285,6 +294,48 @@ __device__ __forceinline__ void replay_bounds(const SceneView &S, TravState &T, 
   T.bx0 = bx0; T.bx1 = bx1; T.by0 = by0; T.by1 = by1; T.bz0 = bz0; T.bz1 = bz1;
 }
 
+// Scene::occludes at a surface a shadow ray meets (space/space.cc:
+// 131-194, material/material.h:168-192): returns true when the ray goes
+// on -- a partially occluding surface (thin glass) multiplies the ray's
+// transmittance by its own, once however many octree nodes list it (the
+// reference's negative mailbox skips a surface already met; here the
+// lane remembers SGR_PARTIAL_SEEN of them), unless the product falls
+// below Eps (Color < Eps: component sum below 3 Eps).  A fully occluding
+// surface, or a surface of an instanced model (models hold none),
+// stops it.  Thin glass transmits colour x (1 - Fresnel (medium, ior) at
+// the angle between the unit geometric normal and the reversed ray,
+// material/thin-glass.cc:123-140).
+__device__ __forceinline__ bool partial_pass(const SceneView &S, TravState &T, uint32_t prim,
+                                             uint32_t inst, float t, TraceCount &cnt) {
+  if (inst != SGR_MISS) return false;
+  const DevMaterial &M = S.mats[S.prim_mat[prim]];
+  if (!M.partial) return false;
+  bool met = false;
+#pragma unroll
+  for (int k = 0; k < SGR_PARTIAL_SEEN; k++) met = met || ((uint32_t)k < T.nseen && T.seen[k] == prim);
+  if (met) return true;
+  if (T.nseen < SGR_PARTIAL_SEEN) {
+#pragma unroll
+    for (int k = 0; k < SGR_PARTIAL_SEEN; k++)
+      if ((uint32_t)k == T.nseen) T.seen[k] = prim;
+    T.nseen++;
+  } else {
+    cnt.povf++;
+  }
+  V3 n;
+  if (prim < S.n_tri_like) {
+    const float4 a = S.tri[3 * prim + 1], b = S.tri[3 * prim + 2];
+    n = unit(cross(v3(b.x, b.y, b.z), v3(a.x, a.y, a.z)));  // e2 x e1
+  } else {
+    const float4 sp = S.sph[prim - S.n_tri_like];
+    n = unit((T.ray.o + T.ray.d * t) - v3(sp.x, sp.y, sp.z));
+  }
+  const float c = dot(n, -T.ray.d);
+  const float refl = fresnel(T.medium, M.ior_n, 0.0f, c);
+  T.trans = T.trans * (cload(M.color) * (1.0f - refl));
+  return !(csum(T.trans) < 3e-15f);
+}
+
 // Stack of the walk inside an instanced model (a second LDS ring and
 // spill array: the scene's walk keeps its own entries while it waits).
 struct Nest {
@@ -338,6 +389,7 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
       cnt.tests++;
       uint32_t inst = SGR_MISS;
       if (ref_test<ANY, INST>(S, k, T.ray, t, u, v, prim, inst, cnt, N)) {
+        if (ANY && INST && S.has_partial && partial_pass(S, T, prim, inst, t, cnt)) continue;
         T.hit = prim;
         T.hinst = inst;
         if (ANY) return true;
@@ -421,6 +473,10 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
 template<bool ANY, bool INST>
 __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, TraceCount &cnt,
                                           LdsStack L, uint2 *spill, Nest N) {
+  if (ANY && INST) {  // T.medium is the caller's
+    T.trans = c3(1.0f);
+    T.nseen = 0;
+  }
   if (S.use_triv) {
     T.hit = SGR_MISS;
     T.hinst = SGR_MISS;
@@ -432,6 +488,7 @@ __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, Trac
       cnt.tests++;
       uint32_t inst = SGR_MISS;
       if (ref_test<ANY, INST>(S, q, T.ray, t, u, v, prim, inst, cnt, N)) {
+        if (ANY && INST && S.has_partial && partial_pass(S, T, prim, inst, t, cnt)) continue;
         T.hit = prim;
         T.hinst = inst;
         if (ANY) break;
@@ -461,7 +518,8 @@ __device__ __forceinline__ void trace_plain(const SceneView &S1, uint32_t jb, ui
   TravState T;
   uint2 spill[SGR_MAX_DEPTH];
   for (uint32_t i = jb + threadIdx.x; i < je; i += blockDim.x) {
-    const SceneView *g = get(i, T.ray);
+    T.medium = 1.0f;  // a shadow ray's medium: air unless GET says otherwise
+    const SceneView *g = get(i, T);
     trace_one<ANY, INST>(MULTI ? *g : S1, T, cnt, L, spill, N);
     put(i, T);
   }
@@ -516,7 +574,8 @@ __device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t n, ui
         const uint32_t k = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
         if (k < len) {
           id = beg + k;
-          const SceneView *g = get(id, T.ray);
+          T.medium = 1.0f;
+          const SceneView *g = get(id, T);
           if (MULTI) Sp = g;
           const SceneView &S = MULTI ? *Sp : S1;
           if (S.use_triv) {

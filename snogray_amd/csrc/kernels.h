@@ -50,6 +50,7 @@ static_assert(SGR_CNT_HE >= 2 * SGR_MAX_BOUNCES + 4 && SGR_CNT_HE % SGR_HEAD_STR
 #define SGR_STAT_FETCH_C 8    // octree node records loaded (closest hit)
 #define SGR_STAT_FETCH_S 9    // ... (any hit)
 #define SGR_STAT_DINT_OVF 10  // direct integrator: branches beyond SGR_DIRECT_MAX_DEPTH
+#define SGR_STAT_PARTIAL_OVF 11  // partial occluders beyond a shadow ray's memory
 #define SGR_NUM_STATS    16
 // The statistics buffer also holds the 8 fetch heads of a ray-batch query,
 // from byte SGR_STATS_HEADS_OFF on, SGR_HEAD_STRIDE words apart.
@@ -112,6 +113,10 @@ struct PathBufs {
   float *sray[8];     // shadow rays, K per path
   float *sc[3];       // shadow contributions
   uint8_t *svis;
+  // Scenes with partial occluders (partial != 0): the medium index at
+  // each shadow ray's origin and the transmittance it found.
+  float *smed, *strans[3];
+  uint32_t partial = 0;
   uint32_t *q[2];     // ray queues (ping-pong), in raygen's pixel order
   uint32_t *sq;       // shadow queue, grouped by direction octant per block
   // Ordered compaction of the queues k_shade produces: per queue

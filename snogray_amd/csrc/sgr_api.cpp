@@ -305,7 +305,7 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   mats.resize(nm);
   for (uint32_t i = 0; i < nm; i++) {
     const sgr_material &m = d->materials[i];
-    if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_PHONG)
+    if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_THIN_GLASS)
       return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": unknown type");
     if (m.type == SGR_MAT_GLOW && m.underlying >= (int)nm)
       return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": bad underlying");
@@ -321,6 +321,10 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     o.ior_k = m.ior_k;
     if (m.type == SGR_MAT_GLASS && !(m.ior_n > 0))
       return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": glass ior must be > 0");
+    // ThinGlass (col, ior) is PARTIALLY_OCCLUDING (material/thin-glass.h:30-32)
+    if (m.type == SGR_MAT_THIN_GLASS && !(m.ior_n > 0))
+      return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": thin glass ior must be > 0");
+    o.partial = m.type == SGR_MAT_THIN_GLASS ? 1 : 0;
     if (m.type == SGR_MAT_COOK_TORRANCE) {  // material/cook-torrance.cc:38-70
       float di = ((m.color[0] + m.color[1]) + m.color[2]) / 3.0f;
       float gi = ((m.gloss_color[0] + m.gloss_color[1]) + m.gloss_color[2]) / 3.0f;
@@ -385,6 +389,10 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     if (rc) return fail(rc, "model " + std::to_string(mi) + ": " + g_err);
     if (H.models.back()->order.empty())
       return fail(SGR_ERR_INVALID, "model " + std::to_string(mi) + " has no surfaces");
+    if (H.models.back()->V.has_partial)
+      return fail(SGR_ERR_INVALID, "model " + std::to_string(mi) +
+                                       ": partially occluding materials (thin glass) are not "
+                                       "supported in models");
   }
   const uint32_t n_inst = d->num_instances;
   if ((uint64_t)n_prims + n_inst >= (1ull << 28))
@@ -607,6 +615,9 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   }
   V.n_lights = (uint32_t)H.lights.size();
   V.n_insts = n_inst;
+  V.has_partial = 0;
+  for (uint32_t p = 0; p < n_prims; p++)
+    if (mats[prim_mat[p]].partial) V.has_partial = 1;
   return SGR_OK;
 }
 
@@ -961,6 +972,8 @@ size_t path_bytes(size_t cap, size_t K, PathBufs *P, char *base) {
   for (int k = 0; k < 8; k++) { r = take(ns * 4); if (P) P->sray[k] = (float *)r; }
   for (int k = 0; k < 3; k++) { r = take(ns * 4); if (P) P->sc[k] = (float *)r; }
   r = take(ns); if (P) P->svis = (uint8_t *)r;
+  r = take(ns * 4); if (P) P->smed = (float *)r;
+  for (int k = 0; k < 3; k++) { r = take(ns * 4); if (P) P->strans[k] = (float *)r; }
   r = take(cap * 4); if (P) P->q[0] = (uint32_t *)r;
   r = take(cap * 4); if (P) P->q[1] = (uint32_t *)r;
   r = take(ns * 4); if (P) P->sq = (uint32_t *)r;
@@ -1177,6 +1190,7 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
   const bool finish = !guide && !(pl.flags & SGR_RENDER_WAVEFRONT_ONLY);
   for (int l = 0; l < lanes; l++) {
     PL[l].K = R.K ? R.K : 1;
+    PL[l].partial = sc->view.has_partial;
     PL[l].fin_from = SGR_FINISH_FROM;
     PL[l].fin_max = finish ? SGR_FINISH_MAX : 0u;
   }
@@ -1303,6 +1317,7 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     stats->surf_tests_shadow = st[SGR_STAT_TESTS_S];
     stats->illum_calls = st[SGR_STAT_ILLUM];
     stats->media_overflows = st[SGR_STAT_MEDIA_OVF];
+    stats->partial_overflows = st[SGR_STAT_PARTIAL_OVF];
     stats->direct_overflows = st[SGR_STAT_DINT_OVF];
     stats->node_fetches_closest = st[SGR_STAT_FETCH_C];
     stats->node_fetches_shadow = st[SGR_STAT_FETCH_S];

@@ -62,7 +62,8 @@ struct Engine {
   StepTables *dev = nullptr;
   unsigned long long *stats = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  std::map<uint32_t, hipGraphExec_t> graphs;  // by bounce count | refill << 31 | inst << 30
+  // by bounce count | refill << 31 | inst << 30 | partial << 29
+  std::map<uint32_t, hipGraphExec_t> graphs;
   std::map<uint64_t, Job> jobs;
   std::deque<uint64_t> queued;   // admission order
   std::vector<uint64_t> active;  // round-robin order
@@ -160,7 +161,7 @@ uint32_t splat_tiles(const RenderView &R) {
 }
 
 void enqueue_step(sgr_ctx *ctx, Engine *E, uint32_t maxp, bool refill, bool inst,
-                  unsigned grid) {
+                  bool partial, unsigned grid) {
   StepView J;
   J.R = E->dev->R;
   J.S = E->dev->S;
@@ -175,7 +176,11 @@ void enqueue_step(sgr_ctx *ctx, Engine *E, uint32_t maxp, bool refill, bool inst
   // plain walk with a stack for the models
   S0.walk_refill = refill && !inst ? 1u : 0u;
   S0.n_insts = inst ? 1u : 0u;
-  const PathBufs &P = E->P;
+  // ... and shadow rays of steps with partial occluders the walk that
+  // carries their transmittance
+  S0.has_partial = partial ? 1u : 0u;
+  PathBufs P = E->P;
+  P.partial = partial ? 1u : 0u;
   hipStream_t st = ctx->stream;
   launch_raygen(true, R0, J, P, grid, st);
   for (uint32_t b = 0; b <= maxp; b++) {
@@ -366,6 +371,7 @@ static int sgr_engine_step_impl(sgr_ctx *ctx, sgr_step_info *info) {
   uint32_t maxp = 0;
   bool refill = false;  // any scene of the step deep enough for lane refill
   bool inst = false;    // any scene of the step with instances
+  bool partial = false; // any scene of the step with partial occluders
   size_t n = E->active.size();
   size_t stop = n;
   for (size_t k = 0; k < n && run.size() < SGR_MAX_SEGMENTS; k++) {
@@ -384,6 +390,7 @@ static int sgr_engine_step_impl(sgr_ctx *ctx, sgr_step_info *info) {
     maxp = std::max(maxp, j.pl.R.maxp);
     refill = refill || j.sc->view.walk_refill;
     inst = inst || j.sc->view.n_insts > 0;
+    partial = partial || j.sc->view.has_partial;
   }
   if (run.empty()) return SGR_OK;
   // next step starts with the first job that did not fit, else the job
@@ -426,12 +433,13 @@ static int sgr_engine_step_impl(sgr_ctx *ctx, sgr_step_info *info) {
   HIPCHK(hipEventRecord(E->e0, ctx->stream));
   bool graphed = false;
   if (E->cfg.use_graph) {
-    const uint32_t key = maxp | (refill ? 1u << 31 : 0u) | (inst ? 1u << 30 : 0u);
+    const uint32_t key =
+        maxp | (refill ? 1u << 31 : 0u) | (inst ? 1u << 30 : 0u) | (partial ? 1u << 29 : 0u);
     auto it = E->graphs.find(key);
     if (it == E->graphs.end()) {
       hipGraph_t g;
       HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_step(ctx, E, maxp, refill, inst, grid);
+      enqueue_step(ctx, E, maxp, refill, inst, partial, grid);
       hipError_t le = hipGetLastError();
       hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
       if (le != hipSuccess) return fail(SGR_ERR_HIP, std::string("step capture: ") + hipGetErrorString(le));
@@ -445,7 +453,7 @@ static int sgr_engine_step_impl(sgr_ctx *ctx, sgr_step_info *info) {
     HIPCHK(hipGraphLaunch(it->second, ctx->stream));
     graphed = true;
   } else {
-    enqueue_step(ctx, E, maxp, refill, inst, grid);
+    enqueue_step(ctx, E, maxp, refill, inst, partial, grid);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(E->e1, ctx->stream));
@@ -576,6 +584,7 @@ int sgr_engine_stats(sgr_ctx *ctx, sgr_stats *stats) {
   stats->surf_tests_shadow = st[SGR_STAT_TESTS_S];
   stats->illum_calls = st[SGR_STAT_ILLUM];
   stats->media_overflows = st[SGR_STAT_MEDIA_OVF];
+  stats->partial_overflows = st[SGR_STAT_PARTIAL_OVF];
   stats->node_fetches_closest = st[SGR_STAT_FETCH_C];
   stats->node_fetches_shadow = st[SGR_STAT_FETCH_S];
   stats->ms_render = E->ms_device;

@@ -74,15 +74,15 @@ template<bool ANY, bool REFILL, bool INST>
 __global__ void __launch_bounds__(256) k_trace(SceneView S, RayBatchDev rb, uint32_t n,
                                                unsigned long long *stats) {
   SGR_STACK_LDS(stk);
-  TraceCount cnt{0, 0, 0};
+  TraceCount cnt{0, 0, 0, 0};
   uint32_t *heads = (uint32_t *)((char *)stats + SGR_STATS_HEADS_OFF);
   walk<ANY, false, REFILL, INST>(
       S, n, heads,
-      [&](uint32_t i, RayF &r) -> const SceneView * {
-        r.o = v3(rb.ox[i], rb.oy[i], rb.oz[i]);
-        r.d = v3(rb.dx[i], rb.dy[i], rb.dz[i]);
-        r.t0 = rb.t0[i];
-        r.t1 = rb.t1[i];
+      [&](uint32_t i, TravState &T) -> const SceneView * {
+        T.ray.o = v3(rb.ox[i], rb.oy[i], rb.oz[i]);
+        T.ray.d = v3(rb.dx[i], rb.dy[i], rb.dz[i]);
+        T.ray.t0 = rb.t0[i];
+        T.ray.t1 = rb.t1[i];
         return nullptr;
       },
       [&](uint32_t i, const TravState &T) {
@@ -125,6 +125,13 @@ __device__ __forceinline__ void load_ray(const PathBufs &P, uint32_t p, RayF &r)
   r.d = v3(P.ray[3][p], P.ray[4][p], P.ray[5][p]);
   r.t0 = P.ray[6][p];
   r.t1 = P.ray[7][p];
+}
+// Transmittance of shadow slot SLOT (scenes with partial occluders).
+__device__ __forceinline__ void put_trans(const PathBufs &P, size_t slot, C3 t) {
+  P.strans[0][slot] = t.r; P.strans[1][slot] = t.g; P.strans[2][slot] = t.b;
+}
+__device__ __forceinline__ C3 get_trans(const PathBufs &P, size_t slot) {
+  return c3(P.strans[0][slot], P.strans[1][slot], P.strans[2][slot]);
 }
 __device__ __forceinline__ void store_ray(const PathBufs &P, uint32_t p, const RayF &r) {
   P.ray[0][p] = r.o.x; P.ray[1][p] = r.o.y; P.ray[2][p] = r.o.z;
@@ -198,15 +205,15 @@ template<bool MULTI, bool REFILL, bool INST>
 __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                 unsigned long long *stats) {
   SGR_STACK_LDS(stk);
-  TraceCount cnt{0, 0, 0};
+  TraceCount cnt{0, 0, 0, 0};
   const uint32_t n = P.counts[SGR_CNT_Q + b];
   if (finishing(P, b, n)) return;
   const uint32_t *q = P.q[b & 1];
   walk<false, MULTI, REFILL, INST>(
       S1, n, &P.counts[SGR_CNT_HE + SGR_HEADS_PER_B * b],
-      [&](uint32_t i, RayF &r) -> const SceneView * {
+      [&](uint32_t i, TravState &T) -> const SceneView * {
         const uint32_t p = q[i];
-        load_ray(P, p, r);
+        load_ray(P, p, T.ray);
         return MULTI ? &J.S[J.R[P.pseg[p]].scene] : nullptr;
       },
       [&](uint32_t i, const TravState &T) {
@@ -228,26 +235,30 @@ template<bool MULTI, bool REFILL, bool INST>
 __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathBufs P, uint32_t b,
                                                  unsigned long long *stats) {
   SGR_STACK_LDS(stk);
-  TraceCount cnt{0, 0, 0};
+  TraceCount cnt{0, 0, 0, 0};
   if (finishing(P, b, P.counts[SGR_CNT_Q + b])) return;
   const uint32_t n = P.counts[SGR_CNT_S + b];
   walk<true, MULTI, REFILL, INST>(
       S1, n, &P.counts[SGR_CNT_HS + SGR_HEADS_PER_B * b],
-      [&](uint32_t i, RayF &r) -> const SceneView * {
+      [&](uint32_t i, TravState &T) -> const SceneView * {
         const uint32_t slot = P.sq[i];
-        r.o = v3(P.sray[0][slot], P.sray[1][slot], P.sray[2][slot]);
-        r.d = v3(P.sray[3][slot], P.sray[4][slot], P.sray[5][slot]);
-        r.t0 = P.sray[6][slot];
-        r.t1 = P.sray[7][slot];
+        T.ray.o = v3(P.sray[0][slot], P.sray[1][slot], P.sray[2][slot]);
+        T.ray.d = v3(P.sray[3][slot], P.sray[4][slot], P.sray[5][slot]);
+        T.ray.t0 = P.sray[6][slot];
+        T.ray.t1 = P.sray[7][slot];
+        if (INST && P.partial) T.medium = P.smed[slot];
         return MULTI ? &J.S[J.R[P.pseg[slot / P.K]].scene] : nullptr;
       },
       [&](uint32_t i, const TravState &T) {
-        P.svis[P.sq[i]] = T.hit == SGR_MISS ? 1 : 0;
+        const uint32_t slot = P.sq[i];
+        P.svis[slot] = T.hit == SGR_MISS ? 1 : 0;
+        if (INST && P.partial) put_trans(P, slot, T.trans);
       },
       cnt, stk);
   wave_count(&stats[SGR_STAT_NODES_S], cnt.nodes);
   wave_count(&stats[SGR_STAT_TESTS_S], cnt.tests);
   wave_count(&stats[SGR_STAT_FETCH_S], cnt.fetch);
+  if (INST) wave_count(&stats[SGR_STAT_PARTIAL_OVF], cnt.povf);
 }
 
 // One path's vertex (render/path-integ.cc:133-407 for one bounce): the
@@ -257,8 +268,11 @@ __global__ void __launch_bounds__(256) k_occlude(SceneView S1, StepView J, PathB
 // continuation and Russian roulette.  Returns whether the path goes on
 // with the ray it stored.  J also indexes the learned proposal's lobes;
 // ILLUM counts direct-lighting calls.
-// Shadow slot SLOT of the path state: the ray and its contribution.
-__device__ __forceinline__ void put_shadow(const PathBufs &P, size_t slot, C3 c, const RayF &r) {
+// Shadow slot SLOT of the path state: the ray, its contribution and (in
+// scenes with partial occluders) the index of the medium it starts in.
+__device__ __forceinline__ void put_shadow(const PathBufs &P, size_t slot, C3 c, const RayF &r,
+                                           float medium) {
+  if (P.partial) P.smed[slot] = medium;
   P.sray[0][slot] = r.o.x; P.sray[1][slot] = r.o.y; P.sray[2][slot] = r.o.z;
   P.sray[3][slot] = r.d.x; P.sray[4][slot] = r.d.y; P.sray[5][slot] = r.d.z;
   P.sray[6][slot] = r.t0; P.sray[7][slot] = r.t1;
@@ -346,11 +360,11 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
             P.svis[slot_b] = 0;
             // light sample, then BSDF sample (render/direct-illum.cc:147-252)
             direct_light_term(S, L, is, M, lu, lv, extra, [&](C3 c, const RayF &r) {
-              put_shadow(P, slot_a, c, r);
+              put_shadow(P, slot_a, c, r, is.media_ior);
               emit(slot_a, dir_octant(r.d));
             });
             direct_bsdf_term(S, L, is, M, bu, bv, [&](C3 c, const RayF &r) {
-              put_shadow(P, slot_b, c, r);
+              put_shadow(P, slot_b, c, r, is.media_ior);
               emit(slot_b, dir_octant(r.d));
             });
           }
@@ -665,8 +679,14 @@ __device__ __forceinline__ void gather_path(const RenderView &R1, const StepView
       for (uint32_t jj = 0; jj < nsub; jj++) {
         size_t sa = (size_t)p * P.K + 2 * (l * nsub + jj);
         C3 r = c3(0.0f);
-        if (P.svis[sa]) r = r + c3(P.sc[0][sa], P.sc[1][sa], P.sc[2][sa]);
-        if (P.svis[sa + 1]) r = r + c3(P.sc[0][sa + 1], P.sc[1][sa + 1], P.sc[2][sa + 1]);
+        C3 ca = c3(P.sc[0][sa], P.sc[1][sa], P.sc[2][sa]);
+        C3 cb = c3(P.sc[0][sa + 1], P.sc[1][sa + 1], P.sc[2][sa + 1]);
+        if (P.partial) {  // through partial occluders
+          ca = ca * get_trans(P, sa);
+          cb = cb * get_trans(P, sa + 1);
+        }
+        if (P.svis[sa]) r = r + ca;
+        if (P.svis[sa + 1]) r = r + cb;
         lrad = lrad + r;
       }
       D = D + lrad * inv_n;
@@ -740,7 +760,7 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
   N.spill = spill2;
   const uint32_t *q = P.q[b & 1];
   const GuideView G{};
-  TraceCount cc{0, 0, 0}, cs{0, 0, 0};
+  TraceCount cc{0, 0, 0, 0}, cs{0, 0, 0, 0};
   uint32_t illum = 0, rays = 0, shadow = 0;
   uint32_t jb, je;
   block_range(n, jb, je);
@@ -769,8 +789,10 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
         U.ray.d = v3(P.sray[3][slot], P.sray[4][slot], P.sray[5][slot]);
         U.ray.t0 = P.sray[6][slot];
         U.ray.t1 = P.sray[7][slot];
+        U.medium = (INST && P.partial) ? P.smed[slot] : 1.0f;
         trace_one<true, INST>(S, U, cs, stk, spill, N);
         P.svis[slot] = U.hit == SGR_MISS ? 1 : 0;
+        if (INST && P.partial) put_trans(P, slot, U.trans);
         shadow++;
       }
       gather_path<MULTI, false>(R1, J, P, G, p);
@@ -784,6 +806,7 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
   wave_count(&stats[SGR_STAT_TESTS_S], cs.tests);
   wave_count(&stats[SGR_STAT_FETCH_C], cc.fetch);
   wave_count(&stats[SGR_STAT_FETCH_S], cs.fetch);
+  if (INST) wave_count(&stats[SGR_STAT_PARTIAL_OVF], cs.povf);
   wave_count(&stats[SGR_STAT_ILLUM], illum);
   wave_count(&stats[SGR_STAT_ISECT], rays);
   wave_count(&stats[SGR_STAT_SHADOW], shadow);
@@ -835,7 +858,7 @@ __global__ void __launch_bounds__(256) k_direct_integ(SceneView S, RenderView R,
   uint2 spill2[INST ? SGR_MAX_DEPTH : 1];
   N.spill = spill2;
   DintBranch pend[SGR_DIRECT_MAX_DEPTH];
-  TraceCount cc{0, 0, 0}, cs{0, 0, 0};
+  TraceCount cc{0, 0, 0, 0}, cs{0, 0, 0, 0};
   uint32_t illum = 0, rays = 0, shadow = 0, ovf = 0;
   const uint32_t nsub = R.nsub_dint;
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < R.npaths;
@@ -893,9 +916,10 @@ __global__ void __launch_bounds__(256) k_direct_integ(SceneView S, RenderView R,
                 auto visible = [&](C3 c, const RayF &ray) {
                   TravState U;
                   U.ray = ray;
+                  U.medium = cur.m0;  // the medium at the hit (isec.media)
                   trace_one<true, INST>(S, U, cs, stk, spill, N);
                   shadow++;
-                  if (U.hit == SGR_MISS) r = r + c;
+                  if (U.hit == SGR_MISS) r = r + ((INST && S.has_partial) ? c * U.trans : c);
                 };
                 direct_light_term(S, L, is, M, lu, lv, extra, visible);
                 direct_bsdf_term(S, L, is, M, bu, bv, visible);
@@ -949,6 +973,7 @@ __global__ void __launch_bounds__(256) k_direct_integ(SceneView S, RenderView R,
   wave_count(&stats[SGR_STAT_TESTS_S], cs.tests);
   wave_count(&stats[SGR_STAT_FETCH_C], cc.fetch);
   wave_count(&stats[SGR_STAT_FETCH_S], cs.fetch);
+  if (INST) wave_count(&stats[SGR_STAT_PARTIAL_OVF], cs.povf);
   wave_count(&stats[SGR_STAT_ILLUM], illum);
   wave_count(&stats[SGR_STAT_ISECT], rays);
   wave_count(&stats[SGR_STAT_SHADOW], shadow);
@@ -1109,11 +1134,18 @@ static inline unsigned grid_for(uint32_t n, unsigned cap) {
   return g < cap ? g : cap;
 }
 
+// Walks whose scenes hold instances, or (any-hit walks) partial
+// occluders, take the INST kernels: the plain walk with the second stack
+// for the models and the shadow rays' transmittance.
+static inline bool general_walk(const SceneView &S, bool any) {
+  return S.n_insts || (any && S.has_partial);
+}
+
 void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool any,
                   unsigned long long *stats, hipStream_t st) {
   unsigned g = grid_for(n, 65535);
   g = (g + 7) & ~7u;  // whole blocks per XCD slice
-  if (S.n_insts) {  // instanced scenes: plain walk, second stack for the models
+  if (general_walk(S, any)) {  // plain walk, second stack for the models
     if (any) hipLaunchKernelGGL((k_trace<true, false, true>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
     else hipLaunchKernelGGL((k_trace<false, false, true>), dim3(g), dim3(256), 0, st, S, rb, n, stats);
   } else if (S.walk_refill) {
@@ -1168,7 +1200,7 @@ void launch_extend(bool multi, const SceneView &S, const StepView &J, const Path
 }
 void launch_direct_integ(const SceneView &S, const RenderView &R, const PathBufs &P,
                          unsigned long long *stats, unsigned grid, hipStream_t st) {
-  if (S.n_insts)
+  if (general_walk(S, true))
     hipLaunchKernelGGL((k_direct_integ<true>), dim3(grid), dim3(256), 0, st, S, R, P, stats);
   else
     hipLaunchKernelGGL((k_direct_integ<false>), dim3(grid), dim3(256), 0, st, S, R, P, stats);
@@ -1206,9 +1238,10 @@ void launch_shade(bool multi, const SceneView &S, const RenderView &R, const Ste
 void launch_occlude(bool multi, const SceneView &S, const StepView &J, const PathBufs &P,
                     uint32_t b, unsigned long long *stats, unsigned grid, hipStream_t st) {
   if (!S.walk_refill) grid = plain_walk_grid(grid);
-  if (S.n_insts && multi)
+  const bool gen = general_walk(S, true);
+  if (gen && multi)
     hipLaunchKernelGGL((k_occlude<true, false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
-  else if (S.n_insts)
+  else if (gen)
     hipLaunchKernelGGL((k_occlude<false, false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else if (multi && S.walk_refill)
     hipLaunchKernelGGL((k_occlude<true, true, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
@@ -1254,11 +1287,11 @@ void launch_finish(bool multi, const SceneView &S, const RenderView &R, const St
   g = grid < g ? grid : g;
   PathBufs Pk = P;
   Pk.media_ovf = stats + SGR_STAT_MEDIA_OVF;
-  if (multi && S.n_insts)
+  if (multi && general_walk(S, true))
     hipLaunchKernelGGL((k_finish<true, true>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);
   else if (multi)
     hipLaunchKernelGGL((k_finish<true, false>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);
-  else if (S.n_insts)
+  else if (general_walk(S, true))
     hipLaunchKernelGGL((k_finish<false, true>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);
   else
     hipLaunchKernelGGL((k_finish<false, false>), dim3(g), dim3(256), 0, st, S, R, J, Pk, b, stats);

@@ -164,3 +164,12 @@ def test_pane_in_sky_path_integrator_mean():
     diff = smp[:, 2:5].astype(float) - expect
     se = diff.std(0) / math.sqrt(len(diff))
     assert np.all(np.abs(diff.mean(0)) < 3 * se + 1e-4), (diff.mean(0), se)
+
+
+def test_thin_glass_in_a_model_is_refused():
+    sd = scenes.SceneDesc()
+    mi, m = sd.model()
+    m.sphere((0, 0, 0), 1.0, sd.thin_glass())
+    sd.instance(mi, np.hstack([np.eye(3), np.zeros((3, 1))]))
+    with pytest.raises(RuntimeError, match="occlude fully"):
+        oracle.OracleScene(sd)
