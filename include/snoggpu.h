@@ -54,6 +54,7 @@ extern "C" {
 #define SGR_MAT_GLOW           5
 #define SGR_MAT_PHONG          6
 #define SGR_MAT_THIN_GLASS     7
+#define SGR_MAT_STENCIL        8
 
 /* Acceleration structure (render/global-render-state.cc:63-74 "accel"). */
 #define SGR_ACCEL_OCTREE  0
@@ -77,7 +78,13 @@ typedef struct sgr_scene sgr_scene;
  *                  (material/thin-glass.h:30-32, ThinGlass (col, ior)):
  *                  specular reflection and straight-through transmission,
  *                  and shadow rays pass it, multiplied by its
- *                  transmittance; not allowed in a model              */
+ *                  transmittance; not allowed in a model
+ *   STENCIL        color = opacity, underlying = index of the material
+ *                  supplying the BSDF (not a stencil; material/stencil.h:
+ *                  30-37): the underlying BSDF with probability
+ *                  intensity(opacity), else straight through; shadow rays
+ *                  pass it with transmittance max(1 - opacity, 0); not
+ *                  allowed in a model                                  */
 typedef struct sgr_material {
     int32_t type;
     float   color[3];

@@ -205,9 +205,8 @@ template<class R> struct Tracer {
       if (include && hm.type == SGR_MAT_GLOW && !is.back)  // material/glow.cc:36-40
         radiance += hm.color * T;
       if (is.bsdf_mat < 0) break;
-      const Material<R> &bm = sc.mats[is.bsdf_mat];
       Bsdf<R> bsdf;
-      bsdf.init(bm, is, bm.ior_n);
+      sc.make_bsdf(bsdf, is.bsdf_mat, is);
       if (P.direct_samples != 0)
         radiance += direct(is, bsdf, path_len, pk, qk, s, ALL & ~SPECULAR) * T;
       float cu, cv;
@@ -312,9 +311,8 @@ template<class R> struct Tracer {
     const Material<R> &hm = sc.mats[is.mat];
     if (hm.type == SGR_MAT_GLOW && !is.back) total += hm.color * W;
     if (is.bsdf_mat < 0) return;
-    const Material<R> &bm = sc.mats[is.bsdf_mat];
     Bsdf<R> bsdf;
-    bsdf.init(bm, is, bm.ior_n);
+    sc.make_bsdf(bsdf, is.bsdf_mat, is);
     total += dint_direct(is, bsdf, pk, s) * W;
     BsdfSample<R> rs = bsdf.sample(R(0), R(0), SPECULAR | REFLECTIVE);
     if (rs.val.gt0()) {

@@ -158,6 +158,11 @@ template<class R> struct Bsdf {
   // Glass (material/glass.cc:32-42)
   R old_ior = 1, new_ior = 1;
   bool has_under = false;  // Mirror's underlying Lambert
+  // Stencil (material/stencil.cc:30-52): opacity, its intensity and
+  // reciprocal (snapped to 0 / 1 near the ends), the underlying BSDF
+  std::unique_ptr<Bsdf> under;
+  C3<R> opacity;
+  R op_int = 0, inv_op_int = 0;
 
   R fresnel_ct(R c) const { return Fresnel<R>(fr_n1, fr_n, fr_k).reflectance(c); }
 
@@ -453,6 +458,57 @@ template<class R> struct Bsdf {
     return s;
   }
 
+  // Bsdf::supports (material/lambert.cc, cook-torrance.cc:214-223,
+  // glass.cc:146-152, mirror.cc:111-119, phong.cc, thin-glass.cc): the
+  // flags of LIMIT this BSDF can produce samples for.
+  unsigned supports(unsigned limit) const {
+    switch (type) {
+      case SGR_MAT_LAMBERT:
+        return (limit & REFLECTIVE) ? ((REFLECTIVE | DIFFUSE) & limit) : 0u;
+      case SGR_MAT_COOK_TORRANCE:
+        return ((limit & REFLECTIVE) && (limit & have_layers)) ? ((REFLECTIVE | have_layers) & limit)
+                                                                : 0u;
+      case SGR_MAT_GLASS:
+      case SGR_MAT_THIN_GLASS:
+        return (limit & SPECULAR) ? ((TRANSMISSIVE | REFLECTIVE | SPECULAR) & limit) : 0u;
+      case SGR_MAT_MIRROR: {
+        unsigned f = 0;
+        if ((limit & REFLECTIVE) && (limit & SPECULAR)) f |= REFLECTIVE | SPECULAR;
+        if (has_under) f |= (limit & REFLECTIVE) ? ((REFLECTIVE | DIFFUSE) & limit) : 0u;
+        return f;
+      }
+      case SGR_MAT_PHONG:
+        return ((limit & REFLECTIVE) && (limit & (DIFFUSE | GLOSSY)))
+                   ? ((REFLECTIVE | DIFFUSE | GLOSSY) & limit) : 0u;
+    }
+    return 0;
+  }
+
+  // --- Stencil (material/stencil.cc:55-114): the underlying BSDF with
+  // probability intensity(opacity) (its value times the opacity, its pdf
+  // times 1 / intensity), else straight through with value (1 - opacity)
+  // / cos and the remaining probability, flagged TRANSLUCENT (0x08).
+  BsdfSample<R> stencil_sample(R u, R v, unsigned flags) const {
+    bool thru_ok = op_int < 1 && (flags & (TRANSMISSIVE | SPECULAR)) == (TRANSMISSIVE | SPECULAR);
+    bool undl_ok = op_int > 0 && under && under->supports(flags) != 0;
+    if (!thru_ok && !undl_ok) return BsdfSample<R>();
+    if (!thru_ok) return under->sample(u, v, flags);
+    if (!undl_ok || u > op_int) {
+      R cn = isec->cos_n(isec->v);
+      if (cn == 0) return BsdfSample<R>();
+      BsdfSample<R> s;
+      s.val = (C3<R>(1) - opacity) / cn;
+      s.pdf = undl_ok ? R(1) - op_int : R(1);
+      s.dir = -isec->v;
+      s.flags = SPECULAR | TRANSMISSIVE | 0x08u;
+      return s;
+    }
+    BsdfSample<R> s = under->sample(u * inv_op_int, v, flags);
+    s.val *= opacity;
+    s.pdf *= inv_op_int;
+    return s;
+  }
+
   BsdfSample<R> sample(R u, R v, unsigned flags) const {
     switch (type) {
       case SGR_MAT_LAMBERT: return lambert_sample(u, v, flags, color);
@@ -461,6 +517,7 @@ template<class R> struct Bsdf {
       case SGR_MAT_MIRROR: return mirror_sample(u, v, flags);
       case SGR_MAT_PHONG: return phong_sample(u, v, flags);
       case SGR_MAT_THIN_GLASS: return thin_glass_sample(u, v, flags);
+      case SGR_MAT_STENCIL: return stencil_sample(u, v, flags);
     }
     return BsdfSample<R>();
   }
@@ -469,6 +526,14 @@ template<class R> struct Bsdf {
       case SGR_MAT_LAMBERT: return lambert_eval(dir, flags);
       case SGR_MAT_COOK_TORRANCE: return ct_eval(dir, flags);
       case SGR_MAT_PHONG: return phong_eval(dir, flags);
+      case SGR_MAT_STENCIL:
+        if (under) {
+          BsdfValue<R> r = under->eval(dir, flags);
+          r.val *= opacity;
+          r.pdf *= inv_op_int;
+          return r;
+        }
+        return BsdfValue<R>();
     }
     return BsdfValue<R>();  // glass and mirror: specular only
   }
@@ -679,7 +744,9 @@ template<class R> struct Scene {
   }
   // Material::fully_occluding (material/material.h:98-100): thin glass
   // lets shadow rays through.
-  bool partial_mat(int m) const { return m >= 0 && mats[m].type == SGR_MAT_THIN_GLASS; }
+  bool partial_mat(int m) const {
+    return m >= 0 && (mats[m].type == SGR_MAT_THIN_GLASS || mats[m].type == SGR_MAT_STENCIL);
+  }
   bool has_partial = false;
 
   // Material::transmittance of partially occluding surface P met by a
@@ -691,6 +758,10 @@ template<class R> struct Scene {
   C3<R> transmittance(unsigned p, const Ray<R> &ray, R t, R medium) const {
     const Prim<R> &pr = prims[p];
     const Material<R> &m = mats[prim_mat[p]];
+    if (m.type == SGR_MAT_STENCIL) {  // material/stencil.cc:160-166
+      auto c = [](R x) { return x > R(0) ? x : R(0); };
+      return C3<R>(c(R(1) - m.color.r), c(R(1) - m.color.g), c(R(1) - m.color.b));
+    }
     V3<R> n = pr.kind == 2 ? (ray.origin + ray.dir * t - pr.center).unit()
                            : cross(pr.e2, pr.e1).unit();
     R c = float(dot(n, -ray.dir));  // cos_angle returns a float
@@ -728,22 +799,55 @@ template<class R> struct Scene {
     return b;
   }
 
+  // Material::get_bsdf of BSDF material BM at IS (a stencil wraps its
+  // underlying material's BSDF, or is it when fully opaque:
+  // material/stencil.cc:140-148).
+  void make_bsdf(Bsdf<R> &b, int bm, const Isec<R> &is) const {
+    const Material<R> &m = mats[bm];
+    if (m.type != SGR_MAT_STENCIL) {
+      b.init(m, is, m.ior_n);
+      return;
+    }
+    const int ub = bsdf_material(m.underlying);
+    if (m.color.sum() >= R(3)) {  // opac >= 1 (Color >= Color: sums)
+      if (ub >= 0) make_bsdf(b, ub, is);
+      return;
+    }
+    b.isec = &is;
+    b.type = SGR_MAT_STENCIL;
+    b.opacity = m.color;
+    b.op_int = m.color.intensity();
+    b.inv_op_int = b.op_int == 0 ? R(0) : R(1) / b.op_int;
+    if (b.op_int < R(0.001f)) b.op_int = b.inv_op_int = 0;
+    else if (b.op_int > R(0.999f)) b.op_int = b.inv_op_int = 1;
+    if (ub >= 0) {
+      b.under.reset(new Bsdf<R>);
+      make_bsdf(*b.under, ub, is);
+    }
+  }
+
   // material/material.h medium(): the glass medium, through GLOW wrappers.
   R medium_ior(int m, bool &has) const {
     int b = bsdf_material(m);
-    has = b >= 0 && mats[b].type == SGR_MAT_GLASS;
+    has = b >= 0 && mats[b].type == SGR_MAT_GLASS;  // a stencil has none
     return has ? mats[b].ior_n : R(1);
   }
 
   void build(const sgr_scene_desc &d) {
     for (unsigned i = 0; i < d.num_materials; i++) {
       const sgr_material &m = d.materials[i];
-      if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_THIN_GLASS)
+      if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_STENCIL)
         throw std::runtime_error("bad material type");
       mats.push_back(Material<R>{m.type, C3<R>(m.color[0], m.color[1], m.color[2]),
                                  C3<R>(m.gloss_color[0], m.gloss_color[1], m.gloss_color[2]),
                                  R(m.m), R(m.ior_n), R(m.ior_k), m.underlying});
     }
+    for (unsigned i = 0; i < d.num_materials; i++)
+      if (mats[i].type == SGR_MAT_STENCIL &&
+          (mats[i].underlying < 0 || mats[i].underlying >= (int)d.num_materials ||
+           (bsdf_material(mats[i].underlying) >= 0 &&
+            mats[bsdf_material(mats[i].underlying)].type == SGR_MAT_STENCIL)))
+        throw std::runtime_error("stencil: the underlying material must exist and not be a stencil");
     std::vector<BBox<R>> pb;
     for (unsigned mi = 0; mi < d.num_meshes; mi++) {
       const sgr_mesh &m = d.meshes[mi];

@@ -16,6 +16,7 @@ SGR_MAT_MIRROR = 4
 SGR_MAT_GLOW = 5
 SGR_MAT_PHONG = 6
 SGR_MAT_THIN_GLASS = 7
+SGR_MAT_STENCIL = 8
 SGR_INTEG_PATH, SGR_INTEG_DIRECT = 0, 1
 SGR_ACCEL_OCTREE = 0
 SGR_ACCEL_TRIV = 1

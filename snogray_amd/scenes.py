@@ -74,6 +74,9 @@ class SceneDesc:
     def thin_glass(self, color=(1.0, 1.0, 1.0), ior=1.5):
         return self._mat(type=abi.SGR_MAT_THIN_GLASS, color=color, ior_n=ior)
 
+    def stencil(self, opacity, underlying):
+        return self._mat(type=abi.SGR_MAT_STENCIL, color=opacity, underlying=underlying)
+
     def glow(self, color, underlying=-1):
         return self._mat(type=abi.SGR_MAT_GLOW, color=color,
                          underlying=underlying)
