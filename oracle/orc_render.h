@@ -1130,6 +1130,11 @@ template<class R> struct Scene {
     is.gf = gf;
     is.mat = prim_mat[p];
     is.bsdf_mat = bsdf_material(is.mat);
+    // A fully opaque stencil's get_bsdf is its underlying material's
+    // (material/stencil.cc:140-148), which may have none.
+    if (is.bsdf_mat >= 0 && mats[is.bsdf_mat].type == SGR_MAT_STENCIL &&
+        mats[is.bsdf_mat].color.sum() >= R(3))
+      is.bsdf_mat = bsdf_material(mats[is.bsdf_mat].underlying);
     is.media_ior = media_ior;
     is.enclosing_ior = enclosing_ior;
   }

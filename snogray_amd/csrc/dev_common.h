@@ -33,6 +33,12 @@ struct DevMaterial {
   unsigned gloss_layer, have_layers;
   int mirror_under;  // Mirror: underlying Lambert present
   int partial;       // lets shadow rays through, times its transmittance
+  // Stencil (material/stencil.cc:30-52): the underlying BSDF material at
+  // this entry's index + under_off of the table (has_under: there is
+  // one), the opacity's intensity and its reciprocal (snapped to 0 / 1
+  // below 0.001 / above 0.999); the opacity is in color.
+  int under_off, has_under;
+  float op_int, inv_op_int;
 };
 
 struct DevLight {

@@ -451,8 +451,31 @@ __device__ __forceinline__ BSample mirror_sample(const Isec &is, const DevMateri
   return s;
 }
 
-__device__ __forceinline__ BSample bsdf_sample(const Isec &is, const DevMaterial &M, float u,
-                                               float v, unsigned flags) {
+// Bsdf::supports (material/lambert.cc, cook-torrance.cc:214-223,
+// glass.cc:146-152, mirror.cc:111-119, phong.cc, thin-glass.cc): the
+// flags of LIMIT the BSDF of M can produce samples for.
+__device__ __forceinline__ unsigned bsdf_supports(const DevMaterial &M, unsigned limit) {
+  const unsigned lam = (limit & F_REFLECTIVE) ? ((F_REFLECTIVE | F_DIFFUSE) & limit) : 0u;
+  switch (M.type) {
+    case 1: return lam;
+    case 2:
+      return ((limit & F_REFLECTIVE) && (limit & M.have_layers))
+                 ? ((F_REFLECTIVE | M.have_layers) & limit) : 0u;
+    case 3:
+    case 7: return (limit & F_SPECULAR) ? ((F_TRANSMISSIVE | F_REFLECTIVE | F_SPECULAR) & limit) : 0u;
+    case 4:
+      return (((limit & F_REFLECTIVE) && (limit & F_SPECULAR)) ? (F_REFLECTIVE | F_SPECULAR) : 0u) |
+             (M.mirror_under ? lam : 0u);
+    case 6:
+      return ((limit & F_REFLECTIVE) && (limit & (F_DIFFUSE | F_GLOSSY)))
+                 ? ((F_REFLECTIVE | F_DIFFUSE | F_GLOSSY) & limit) : 0u;
+  }
+  return 0u;
+}
+
+// The BSDFs other than the stencil's.
+__device__ __forceinline__ BSample bsdf_sample_base(const Isec &is, const DevMaterial &M, float u,
+                                                    float v, unsigned flags) {
   switch (M.type) {
     case 1: return lambert_sample(is, cload(M.color), u, v, flags);
     case 2: return ct_sample(is, M, u, v, flags);
@@ -464,8 +487,8 @@ __device__ __forceinline__ BSample bsdf_sample(const Isec &is, const DevMaterial
   return bsample_none();
 }
 
-__device__ __forceinline__ BValue bsdf_eval(const Isec &is, const DevMaterial &M, V3 d,
-                                            unsigned flags) {
+__device__ __forceinline__ BValue bsdf_eval_base(const Isec &is, const DevMaterial &M, V3 d,
+                                                 unsigned flags) {
   BValue r;
   r.val = c3(0.0f);
   r.pdf = 0.0f;
@@ -478,6 +501,58 @@ __device__ __forceinline__ BValue bsdf_eval(const Isec &is, const DevMaterial &M
     r = ct_eval(is, M, d, flags);
   } else if (M.type == 6) {
     r = phong_eval(is, M, d, flags);
+  }
+  return r;
+}
+
+// Stencil (material/stencil.cc:55-114): the underlying BSDF with
+// probability intensity(opacity) -- its value times the opacity, its pdf
+// times 1 / intensity -- else straight through with value (1 - opacity)
+// / cos and the remaining probability, flagged TRANSLUCENT (0x08).  Asked
+// for no specular transmission, the underlying sample comes unscaled.
+// M is an entry of the scene's material table (the underlying one is
+// found relative to it).
+__device__ __forceinline__ BSample stencil_sample(const Isec &is, const DevMaterial &M, float u,
+                                                  float v, unsigned flags) {
+  const DevMaterial &U = (&M)[M.under_off];
+  const bool thru_ok =
+      M.op_int < 1.0f && (flags & (F_TRANSMISSIVE | F_SPECULAR)) == (F_TRANSMISSIVE | F_SPECULAR);
+  const bool undl_ok = M.op_int > 0.0f && M.has_under && bsdf_supports(U, flags) != 0u;
+  if (!thru_ok && !undl_ok) return bsample_none();
+  if (!thru_ok) return bsdf_sample_base(is, U, u, v, flags);
+  if (!undl_ok || u > M.op_int) {
+    const float cn = is.v.z;
+    if (cn == 0.0f) return bsample_none();
+    BSample s;
+    const C3 o = cload(M.color);
+    s.val = C3{1.0f - o.r, 1.0f - o.g, 1.0f - o.b} * (1.0f / cn);  // Color / float
+    s.pdf = undl_ok ? 1.0f - M.op_int : 1.0f;
+    s.dir = -is.v;
+    s.flags = F_SPECULAR | F_TRANSMISSIVE | 0x08u;
+    return s;
+  }
+  BSample s = bsdf_sample_base(is, U, u * M.inv_op_int, v, flags);
+  s.val = s.val * cload(M.color);
+  s.pdf = s.pdf * M.inv_op_int;
+  return s;
+}
+
+__device__ __forceinline__ BSample bsdf_sample(const Isec &is, const DevMaterial &M, float u,
+                                               float v, unsigned flags) {
+  if (M.type == 8) return stencil_sample(is, M, u, v, flags);
+  return bsdf_sample_base(is, M, u, v, flags);
+}
+
+__device__ __forceinline__ BValue bsdf_eval(const Isec &is, const DevMaterial &M, V3 d,
+                                            unsigned flags) {
+  if (M.type != 8) return bsdf_eval_base(is, M, d, flags);
+  BValue r;
+  r.val = c3(0.0f);
+  r.pdf = 0.0f;
+  if (M.has_under) {
+    r = bsdf_eval_base(is, (&M)[M.under_off], d, flags);
+    r.val = r.val * cload(M.color);
+    r.pdf = r.pdf * M.inv_op_int;
   }
   return r;
 }

@@ -296,7 +296,7 @@ __device__ __forceinline__ void replay_bounds(const SceneView &S, TravState &T, 
 
 // Scene::occludes at a surface a shadow ray meets (space/space.cc:
 // 131-194, material/material.h:168-192): returns true when the ray goes
-// on -- a partially occluding surface (thin glass) multiplies the ray's
+// on -- a partially occluding surface (thin glass, stencil) multiplies the ray's
 // transmittance by its own, once however many octree nodes list it (the
 // reference's negative mailbox skips a surface already met; here the
 // lane remembers SGR_PARTIAL_SEEN of them), unless the product falls
@@ -304,7 +304,7 @@ __device__ __forceinline__ void replay_bounds(const SceneView &S, TravState &T, 
 // surface, or a surface of an instanced model (models hold none),
 // stops it.  Thin glass transmits colour x (1 - Fresnel (medium, ior) at
 // the angle between the unit geometric normal and the reversed ray,
-// material/thin-glass.cc:123-140).
+// material/thin-glass.cc:123-140); a stencil max (1 - opacity, 0).
 __device__ __forceinline__ bool partial_pass(const SceneView &S, TravState &T, uint32_t prim,
                                              uint32_t inst, float t, TraceCount &cnt) {
   if (inst != SGR_MISS) return false;
@@ -321,6 +321,11 @@ __device__ __forceinline__ bool partial_pass(const SceneView &S, TravState &T, u
     T.nseen++;
   } else {
     cnt.povf++;
+  }
+  if (M.type == 8) {  // stencil: max(1 - opacity, 0) (material/stencil.cc:160-166)
+    T.trans = T.trans * C3{fmaxf(1.0f - M.color[0], 0.0f), fmaxf(1.0f - M.color[1], 0.0f),
+                           fmaxf(1.0f - M.color[2], 0.0f)};
+    return !(csum(T.trans) < 3e-15f);
   }
   V3 n;
   if (prim < S.n_tri_like) {

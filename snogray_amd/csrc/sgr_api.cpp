@@ -305,10 +305,14 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   mats.resize(nm);
   for (uint32_t i = 0; i < nm; i++) {
     const sgr_material &m = d->materials[i];
-    if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_THIN_GLASS)
+    if (m.type < SGR_MAT_LAMBERT || m.type > SGR_MAT_STENCIL)
       return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": unknown type");
     if (m.type == SGR_MAT_GLOW && m.underlying >= (int)nm)
       return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": bad underlying");
+    if (m.type == SGR_MAT_STENCIL && (m.underlying < 0 || m.underlying >= (int)nm))
+      return fail(SGR_ERR_INVALID, "material " + std::to_string(i) +
+                                       ": stencil: the underlying material must exist and not "
+                                       "be a stencil");
     DevMaterial &o = mats[i];
     std::memset(&o, 0, sizeof o);
     o.type = m.type;
@@ -324,7 +328,8 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     // ThinGlass (col, ior) is PARTIALLY_OCCLUDING (material/thin-glass.h:30-32)
     if (m.type == SGR_MAT_THIN_GLASS && !(m.ior_n > 0))
       return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": thin glass ior must be > 0");
-    o.partial = m.type == SGR_MAT_THIN_GLASS ? 1 : 0;
+    // so is Stencil (material/stencil.h:30-36)
+    o.partial = (m.type == SGR_MAT_THIN_GLASS || m.type == SGR_MAT_STENCIL) ? 1 : 0;
     if (m.type == SGR_MAT_COOK_TORRANCE) {  // material/cook-torrance.cc:38-70
       float di = ((m.color[0] + m.color[1]) + m.color[2]) / 3.0f;
       float gi = ((m.gloss_color[0] + m.gloss_color[1]) + m.gloss_color[2]) / 3.0f;
@@ -354,7 +359,38 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     while (b >= 0 && mats[b].type == SGR_MAT_GLOW && guard++ < 16) b = d->materials[b].underlying;
     if (b >= 0 && mats[b].type == SGR_MAT_GLOW) b = -1;
     mats[i].bsdf = b;
-    mats[i].medium = (b >= 0 && mats[b].type == SGR_MAT_GLASS) ? 1 : 0;
+    mats[i].medium = (b >= 0 && mats[b].type == SGR_MAT_GLASS) ? 1 : 0;  // a stencil has none
+  }
+  // Stencil (material/stencil.cc:30-52, 140-148): its underlying BSDF
+  // material (through glows), the opacity's intensity snapped to 0 / 1
+  // below 0.001 / above 0.999; fully opaque (component sum >= 3), its
+  // get_bsdf is the underlying material's, which may have none.
+  for (uint32_t i = 0; i < nm; i++) {
+    DevMaterial &o = mats[i];
+    if (o.type != SGR_MAT_STENCIL) continue;
+    const int ub = mats[d->materials[i].underlying].bsdf;
+    if (ub >= 0 && mats[ub].type == SGR_MAT_STENCIL)
+      return fail(SGR_ERR_INVALID, "material " + std::to_string(i) +
+                                       ": stencil: the underlying material must exist and not "
+                                       "be a stencil");
+    o.has_under = ub >= 0 ? 1 : 0;
+    o.under_off = ub >= 0 ? ub - (int)i : 0;
+    float s = o.color[0];  // Color::intensity (color/color.h:160-166)
+    s += o.color[1];
+    s += o.color[2];
+    o.op_int = s / 3.0f;
+    o.inv_op_int = o.op_int == 0 ? 0.0f : 1.0f / o.op_int;
+    if (o.op_int < 0.001f) o.op_int = o.inv_op_int = 0.0f;
+    else if (o.op_int > 0.999f) o.op_int = o.inv_op_int = 1.0f;
+  }
+  for (uint32_t i = 0; i < nm; i++) {
+    const int b = mats[i].bsdf;
+    if (b >= 0 && mats[b].type == SGR_MAT_STENCIL) {
+      float s = mats[b].color[0];
+      s += mats[b].color[1];
+      s += mats[b].color[2];
+      if (s >= 3.0f) mats[i].bsdf = mats[d->materials[b].underlying].bsdf;
+    }
   }
   // ---- primitives ----
   uint64_t n_tri = 0;
@@ -391,7 +427,7 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
       return fail(SGR_ERR_INVALID, "model " + std::to_string(mi) + " has no surfaces");
     if (H.models.back()->V.has_partial)
       return fail(SGR_ERR_INVALID, "model " + std::to_string(mi) +
-                                       ": partially occluding materials (thin glass) are not "
+                                       ": partially occluding materials (thin glass, stencil) are not "
                                        "supported in models");
   }
   const uint32_t n_inst = d->num_instances;
