@@ -129,3 +129,58 @@ def test_stencil_nesting_and_models_refused():
     sd.sphere((0, 0, 0), 1.0, sd.stencil((0.5, 0.5, 0.5), s1))
     with pytest.raises(RuntimeError, match="stencil"):
         oracle.OracleScene(sd)
+
+
+def _opaque_pair(under):
+    """The same room twice: its ball once of a fully opaque stencil over
+    UNDER, once of UNDER itself."""
+    out = []
+    for wrap in (True, False):
+        sd = scenes.SceneDesc()
+        sd.tripar((-50, 0, -50), (100, 0, 0), (0, 0, 100), sd.lambert((0.6, 0.6, 0.6)),
+                  parallelogram=True)
+        sd.tripar(V0, E1, E2, sd.glow((LE,) * 3), parallelogram=True)
+        m = under(sd)
+        sd.sphere((0.0, 0.8, 0.0), 0.8, sd.stencil((1.0, 1.0, 1.0), m) if wrap else m)
+        # thin glass reflects by the medium it is met in (air or the ball's)
+        sd.sphere((0.0, 0.8, 0.0), 0.4, sd.thin_glass((1.0, 1.0, 1.0), 1.3))
+        out.append(sd)
+    return out
+
+
+@pytest.mark.parametrize("under", [
+    lambda sd: sd.lambert((0.7, 0.3, 0.5)),
+    lambda sd: sd.cook_torrance((0.3, 0.2, 0.1), (0.5, 0.5, 0.5), 0.3, 1.7),
+    lambda sd: sd.glass(1.5),
+    lambda sd: sd.glow((2.0, 2.0, 2.0)),   # no BSDF
+])
+def test_fully_opaque_stencil_is_its_underlying_bsdf(under):
+    """material/stencil.cc:140-148: opacity >= 1 hands over the
+    underlying material's BSDF (none for a bare glow), with the stencil's
+    own medium (none) and emission (none).  Over Lambert or Cook-Torrance
+    every sample equals the bare material's; over glass the ball keeps no
+    medium, so a thin glass ball inside it reflects as in air and the
+    images differ; over a bare glow a camera ray that
+    meets the ball ends there with nothing, where the glow itself gives
+    its colour."""
+    a, b = _opaque_pair(under)
+    W, H = 24, 16
+    cam = scenes.look_at((0, 1.6, -5), (0, 0.8, 0), vfov_deg=40.0)
+    p = abi.make_params(W, H, 4, max_path_len=6, direct_samples=1)
+    _, sa, _ = oracle.OracleScene(a).render(cam, p, samples=True)
+    _, sb, _ = oracle.OracleScene(b).render(cam, p, samples=True)
+    probe = scenes.SceneDesc()
+    t = probe.materials[under(probe)]["type"]
+    if t == abi.SGR_MAT_GLASS:
+        assert not np.array_equal(sa[:, 2:5], sb[:, 2:5])
+    elif t == abi.SGR_MAT_GLOW:
+        d = camera_dirs(cam, sa[:, 0].astype(float), sa[:, 1].astype(float), W, H)
+        o = np.array([0.0, 1.6, -5.0]) - np.array([0.0, 0.8, 0.0])
+        bq = d @ o
+        disc = bq * bq - (d * d).sum(1) * (o @ o - 0.64)
+        ball = disc > 0.05 * (d * d).sum(1)   # clear of the silhouette
+        assert ball.sum() > 20
+        assert np.all(sa[ball, 2:5] == 0.0)
+        assert np.all(sb[ball, 2:5] == 2.0)
+    else:
+        np.testing.assert_array_equal(sa, sb)
