@@ -310,7 +310,7 @@ typedef struct sgr_stats {
     uint64_t node_fetches_closest;
     uint64_t node_fetches_shadow;
     uint64_t direct_overflows;            /* see SGR_DIRECT_MAX_DEPTH */
-    /* partially occluding surfaces (thin glass) beyond the 8 a shadow ray
+    /* partially occluding surfaces (thin glass, stencil) beyond the 8 a shadow ray
      * remembers: multiplied into its transmittance without checking for
      * a repeat (each surface once is the rule, material/material.h:168) */
     uint64_t partial_overflows;
