@@ -132,6 +132,36 @@ typedef struct sgr_sphere {
     uint32_t material;
 } sgr_sphere;
 
+/* Ellipse primitive (surface/ellipse.h:38-45): centre and two radius
+ * vectors, lying in the plane through center, center + radius1 and
+ * center + radius2 (non-orthogonal radii give a skewed ellipse).  It is
+ * the parallelogram with corner center - radius1 - radius2 and edges
+ * 2 radius1, 2 radius2 cut to its inscribed ellipse; its normal is
+ * unit(radius2 x radius1). */
+typedef struct sgr_ellipse {
+    float    center[3];
+    float    radius1[3];
+    float    radius2[3];
+    uint32_t material;
+} sgr_ellipse;
+
+/* Primitives defined as a canonical shape placed by a transform
+ * (surface/local-primitive.h): local_to_world is the 3 x 4 row-major
+ * affine transform (rows x, y, z: linear part, then translation); the
+ * library keeps its inverse (geometry/local-xform.cc:17-21).
+ *   SGR_LOCAL_CYLINDER  radius 1 around the z axis, z from -1 to 1, no
+ *                       caps (surface/cylinder.cc:114-150)
+ *   SGR_LOCAL_SPHERE2   the unit sphere at the origin (surface/sphere2.h)
+ * Their box is the local cube [-1,1]^3 with its 8 corners taken to world
+ * space (geometry/local-xform.cc:27-31). */
+#define SGR_LOCAL_CYLINDER 1
+#define SGR_LOCAL_SPHERE2  2
+typedef struct sgr_local_prim {
+    int32_t  kind;
+    float    local_to_world[12];
+    uint32_t material;
+} sgr_local_prim;
+
 /* Lat-long environment map light (light/envmap-light.cc,
  * texture/spheremap.h).  texels: RGB float, row-major, row 0 = top
  * (north).  frame: 3x3 row-major, rows are the light frame's x, y, z
@@ -155,6 +185,10 @@ typedef struct sgr_model {
     uint32_t            num_tripars;
     const sgr_sphere   *spheres;
     uint32_t            num_spheres;
+    const sgr_ellipse  *ellipses;
+    uint32_t            num_ellipses;
+    const sgr_local_prim *local_prims;
+    uint32_t            num_local_prims;
 } sgr_model;
 
 /* An instance of a model (surface/instance.cc:145-216): local_to_world is
@@ -169,12 +203,13 @@ typedef struct sgr_instance {
 } sgr_instance;
 
 /* Whole scene.  Surfaces are inserted into the acceleration structure in
- * this order: meshes (triangle order), then tripars, then spheres, then
- * instances.  The primitive id of a hit is its index in that order; for
- * a surface of an instanced model it is the index in the model's own
- * order (meshes, tripars, spheres) and the hit also names the instance.
- * Light samplers are created for GLOW tripars and spheres (insertion
- * order), then for each envmap. */
+ * this order: meshes (triangle order), then tripars, then ellipses, then
+ * spheres, then local primitives, then instances.  The primitive id of a
+ * hit is its index in that order; for a surface of an instanced model it
+ * is the index in the model's own order (meshes, tripars, ellipses,
+ * spheres, local primitives) and the hit also names the instance.
+ * Light samplers are created for GLOW tripars, ellipses, spheres and
+ * local primitives (insertion order), then for each envmap. */
 typedef struct sgr_scene_desc {
     const sgr_mesh     *meshes;
     uint32_t            num_meshes;
@@ -192,6 +227,10 @@ typedef struct sgr_scene_desc {
     uint32_t            num_models;
     const sgr_instance *instances;
     uint32_t            num_instances;
+    const sgr_ellipse  *ellipses;
+    uint32_t            num_ellipses;
+    const sgr_local_prim *local_prims;
+    uint32_t            num_local_prims;
 } sgr_scene_desc;
 
 /* Rays and hits, structure-of-arrays (geometry/ray.h:84-89).  Pointers
@@ -358,6 +397,15 @@ int sgr_scene_bounds(const sgr_scene *scene, float *bbox_min3,
                      float *bbox_max3, float *horizon);
 /* Device memory footprint of the scene in bytes. */
 int sgr_scene_footprint(const sgr_scene *scene, uint64_t *bytes);
+/* The local_to_world transform of a cylinder from its base, axis and
+ * radius (host only; Cylinder::xform, surface/cylinder.cc:24-36): the
+ * canonical cylinder moved up by 1, scaled by (radius, radius,
+ * |axis| / 2), turned to the basis (p, unit(axis) x ... ) built from
+ * p = unit(axis).perpendicular() (geometry/vec.h:119-129, not normalised,
+ * so the radius is scaled by |p| as in the reference), then moved to
+ * base.  Fails on a zero axis. */
+int sgr_cylinder_xform(const float base[3], const float axis[3], float radius,
+                       float local_to_world[12]);
 
 /* ---- ray batches (parity entry points) -------------------------------- */
 int sgr_trace_closest(sgr_scene *scene, const sgr_rays *rays, size_t n,

@@ -68,6 +68,8 @@ def lib(fast=False):
                                        C.c_uint32, abi.f32p, C.c_uint32,
                                        C.c_uint32, abi.f32p]
         L.orc_isec_probe.argtypes = [C.c_void_p, abi.f32p, C.c_int, C.POINTER(C.c_double)]
+        L.orc_light_probe.argtypes = [C.c_void_p, C.c_uint, C.c_int, C.POINTER(C.c_double),
+                                      C.c_int, C.POINTER(C.c_double)]
         L.orc_permute.restype = C.c_uint32
         L.orc_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
         _libs[fast] = L
@@ -146,6 +148,27 @@ class OracleScene:
         return {"prim": out[:, 0].astype(np.int64), "inst": out[:, 1].astype(np.int64),
                 "t": out[:, 2], "origin": out[:, 3:6], "normal": out[:, 6:9],
                 "back": out[:, 9] > 0, "u": out[:, 10], "v": out[:, 11]}
+
+    def light_probe(self, light, points, normals, uv=None, dirs=None):
+        """Light LIGHT from POINTS with NORMALS: Light::Sampler::sample at
+        parameters UV (n x 2), or, with DIRS (world directions, n x 3),
+        Light::Sampler::eval.  Returns a dict of val (n x 3), pdf, dist and
+        (sampling) the world direction."""
+        pts = np.asarray(points, np.float64).reshape(-1, 3)
+        n = len(pts)
+        q = np.zeros((n, 8))
+        q[:, 0:3] = pts
+        q[:, 3:6] = np.asarray(normals, np.float64).reshape(-1, 3)
+        if uv is not None:
+            q[:, 6:8] = np.asarray(uv, np.float64).reshape(-1, 2)
+        mode = 0 if dirs is None else 1
+        if mode:
+            q = np.concatenate([q.ravel(), np.asarray(dirs, np.float64).reshape(-1, 3).ravel()])
+        q = np.ascontiguousarray(q.ravel())
+        out = np.zeros((n, 8))
+        if self.L.orc_light_probe(self.h, light, mode, dptr(q), n, dptr(out)):
+            raise RuntimeError("light probe: no such light")
+        return {"val": out[:, 0:3], "pdf": out[:, 3], "dist": out[:, 4], "dir": out[:, 5:8]}
 
     def render(self, cam, params, nthreads=8, samples=False):
         acc = np.zeros((params.height, params.width, 5), np.float64)

@@ -789,3 +789,59 @@ int orc_isec_probe(void *h, const float *rays, int n, double *out) {
 }
 
 }  // extern "C"
+
+namespace {
+// Light L of scene SC seen from points with normals (light_probe below).
+template<class R>
+int light_probe_t(const Scene<R> &sc, unsigned light, int mode, const double *q, int n,
+                  double *out) {
+  if (light >= sc.lights.size()) return 1;
+  const Light<R> &L = sc.lights[light];
+  for (int i = 0; i < n; i++) {
+    const double *a = q + 8 * i;
+    Isec<R> is;
+    const V3<R> p = V3<R>(R(a[0]), R(a[1]), R(a[2]));
+    const V3<R> nz = V3<R>(R(a[3]), R(a[4]), R(a[5])).unit();
+    const V3<R> sx = nz.perpendicular().unit();
+    const V3<R> ty = cross(nz, sx);
+    is.nf = Frame<R>(p, sx, ty, nz);
+    is.gf = is.nf;
+    is.v = V3<R>(0, 0, 1);
+    is.geom_n = V3<R>(0, 0, 1);
+    is.back = false;
+    is.mat = is.bsdf_mat = -1;
+    is.media_ior = is.enclosing_ior = 1;
+    double *o = out + 8 * i;
+    LightSample<R> ls;
+    if (mode == 0) {
+      ls = sc.light_sample(L, is, R(a[6]), R(a[7]), R(0.5));
+    } else {
+      const V3<R> w = V3<R>(R(q[8 * n + 3 * i]), R(q[8 * n + 3 * i + 1]), R(q[8 * n + 3 * i + 2]));
+      ls = sc.light_eval(L, is, is.nf.to(w));
+    }
+    o[0] = ls.val.r; o[1] = ls.val.g; o[2] = ls.val.b;
+    o[3] = ls.pdf;
+    o[4] = ls.dist;
+    const V3<R> wdir = is.nf.from(ls.dir);
+    o[5] = wdir.x; o[6] = wdir.y; o[7] = wdir.z;
+  }
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+// Light probe: light LIGHT of the scene from n points.  q holds 8
+// doubles per point (position xyz, normal xyz, then the sample
+// parameter u v), followed, for mode 1, by 3 doubles per point (a world
+// direction).  The point's normal frame is the normal with any tangent.
+// Mode 0: Light::Sampler::sample at (u, v); mode 1: Light::Sampler::eval
+// in the direction.  out (8 per point): value rgb, pdf (solid angle),
+// distance, sample direction in world space (mode 0).
+int orc_light_probe(void *h, unsigned light, int mode, const double *q, int n, double *out) {
+  OrcScene *s = (OrcScene *)h;
+  return s->fp64 ? light_probe_t(*s->sd, light, mode, q, n, out)
+                 : light_probe_t(*s->sf, light, mode, q, n, out);
+}
+
+}  // extern "C"

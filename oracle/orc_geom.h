@@ -195,6 +195,38 @@ static inline bool sphere_intersects(const V3<R> &center, R radius,
   return false;
 }
 
+// Ellipse::intersects (surface/ellipse.h:95-110): the bounding
+// parallelogram's hit, kept when (u, v) lies in the inscribed ellipse.
+template<class R>
+static inline bool ellipse_intersects(const V3<R> &corner, const V3<R> &e1, const V3<R> &e2,
+                                      const V3<R> &ro, const V3<R> &rd, R min_t, R &t, R &u,
+                                      R &v) {
+  if (!tripar_intersects(corner, e1, e2, true, ro, rd, min_t, t, u, v)) return false;
+  const R x = u - R(0.5), y = v - R(0.5);
+  return x * x + y * y <= R(0.25);
+}
+
+// cylinder_intersects (surface/cylinder.cc:114-150): the canonical
+// cylinder (radius 1 about the z axis, z in [-1, 1], no caps); the first
+// root beyond MIN_T whose z is inside.  The constant term subtracts the
+// radius itself, not its square (both 1).
+template<class R>
+static inline bool cylinder_intersects(const V3<R> &ro, const V3<R> &rd, R min_t, R &t) {
+  const R a = rd.x * rd.x + rd.y * rd.y;
+  const R b = R(2) * (rd.x * ro.x + rd.y * ro.y);
+  const R c = ro.x * ro.x + ro.y * ro.y - R(1);
+  R roots[2];
+  const unsigned n = quadratic_roots(a, b, c, roots);
+  for (unsigned i = 0; i < n; i++) {
+    t = roots[i];
+    if (t > min_t) {
+      const R z = ro.z + t * rd.z;
+      if (z >= R(-1) && z <= R(1)) return true;
+    }
+  }
+  return false;
+}
+
 template<class R> struct Ray {
   V3<R> origin, dir;
   R t0, t1;
@@ -398,13 +430,16 @@ static inline void oct_upd_stats(const std::vector<FlatNode> &nodes,
 }
 
 // Primitive storage for intersection.  Kinds: 0 = mesh triangle,
-// 1 = tripar, 2 = sphere.
+// 1 = tripar, 2 = sphere, 3 = instance, 4 = ellipse (its bounding
+// parallelogram in v0, e1, e2), 5 = cylinder, 6 = sphere2 (local
+// primitives: a canonical shape placed by l2w, w2l its inverse).
 template<class R> struct Prim {
   int kind;
   bool parallelogram;
-  V3<R> v0, e1, e2;  // triangle / tripar
+  V3<R> v0, e1, e2;  // triangle / tripar / ellipse
   V3<R> center;      // sphere
   R radius;
+  R l2w[12], w2l[12];  // cylinder / sphere2
 };
 
 #ifdef ORC_MAILBOX
@@ -462,7 +497,8 @@ template<class R> struct Accel {
   V3<R> origin;
   R size = 0;
 
-  // Surface test (mesh.cc:730-822, tripar.cc:93-130, sphere.cc:82-120).
+  // Surface test (mesh.cc:730-822, tripar.cc:93-130, sphere.cc:82-120,
+  // ellipse.h:95-110, cylinder.cc:114-186, sphere2.cc:92-110).
   bool test(unsigned p, const Ray<R> &ray, R &t, R &u, R &v) const {
     const Prim<R> &pr = (*prims)[p];
     if (pr.kind == 2) {
@@ -471,6 +507,15 @@ template<class R> struct Accel {
         return t < ray.t1;
       return false;
     }
+    if (pr.kind == 5 || pr.kind == 6) {
+      // LocalXform::world_to_local of the ray: origin and direction, the
+      // parametric extent kept
+      u = v = 0;
+      const V3<R> o = xform_pos(pr.w2l, ray.origin), d = xform_vec(pr.w2l, ray.dir);
+      if (pr.kind == 5) return cylinder_intersects(o, d, ray.t0, t) && t < ray.t1;
+      return sphere_intersects(V3<R>(0, 0, 0), R(1), o, d, ray.t0, t) && t < ray.t1;
+    }
+    if (pr.kind == 4) return ellipse_intersects(pr.v0, pr.e1, pr.e2, ray.origin, ray.dir, ray.t0, t, u, v) && t < ray.t1;
     return tripar_intersects(pr.v0, pr.e1, pr.e2, pr.parallelogram, ray.origin,
                              ray.dir, ray.t0, t, u, v) &&
            t < ray.t1;

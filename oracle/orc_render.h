@@ -690,7 +690,10 @@ template<class R> struct Envmap {
 template<class R> struct LightSample { C3<R> val; R pdf = 0; V3<R> dir; R dist = 0; };
 
 template<class R> struct Light {
-  int kind;  // 0 = tripar area light, 1 = sphere light, 2 = envmap
+  // 0 = tripar area light, 1 = sphere light, 2 = envmap; surface lights
+  // (light/surface-light-sampler.cc) of 3 = an ellipse, 4 = a cylinder,
+  // 5 = a sphere2
+  int kind;
   unsigned prim;
   int env;
   C3<R> intensity;
@@ -749,6 +752,18 @@ template<class R> struct Scene {
   }
   bool has_partial = false;
 
+  // IsecInfo::normal of a cylinder or sphere2 hit at T (surface/
+  // cylinder.cc:84-90, surface/sphere2.cc:79-83): the local hit point's
+  // normal (its x, y for the cylinder) by the inverse transpose, unit.
+  static V3<R> local_onorm(const Prim<R> &pr, const Ray<R> &ray, R t) {
+    const V3<R> o = xform_pos(pr.w2l, ray.origin), d = xform_vec(pr.w2l, ray.dir);
+    const V3<R> ip = o + d * t;
+    return pr.kind == 5 ? V3<R>(ip.x, ip.y, R(0)) : ip;
+  }
+  static V3<R> local_normal(const Prim<R> &pr, const Ray<R> &ray, R t) {
+    return xform_transpose(pr.w2l, local_onorm(pr, ray, t)).unit();
+  }
+
   // Material::transmittance of partially occluding surface P met by a
   // shadow ray at T, in a medium of index MEDIUM (material/thin-glass.cc:
   // 123-140): the colour times one minus the Fresnel reflectance at the
@@ -763,7 +778,8 @@ template<class R> struct Scene {
       return C3<R>(c(R(1) - m.color.r), c(R(1) - m.color.g), c(R(1) - m.color.b));
     }
     V3<R> n = pr.kind == 2 ? (ray.origin + ray.dir * t - pr.center).unit()
-                           : cross(pr.e2, pr.e1).unit();
+              : (pr.kind == 5 || pr.kind == 6) ? local_normal(pr, ray, t)
+                                               : cross(pr.e2, pr.e1).unit();
     R c = float(dot(n, -ray.dir));  // cos_angle returns a float
     R refl = Fresnel<R>(medium, m.ior_n).reflectance(c);
     return m.color * (R(1) - refl);
@@ -907,6 +923,31 @@ template<class R> struct Scene {
       pb.push_back(tb);
       bbox.add(tb);
     }
+    // Ellipses (surface/ellipse.h:38-45): the bounding parallelogram
+    // corner = center - radius1 - radius2, edges 2 radius1, 2 radius2;
+    // box of its four corners (surface/ellipse.cc:171-181).
+    for (unsigned i = 0; i < d.num_ellipses; i++) {
+      const sgr_ellipse &el = d.ellipses[i];
+      const V3<R> c(el.center[0], el.center[1], el.center[2]);
+      const V3<R> r1(el.radius1[0], el.radius1[1], el.radius1[2]);
+      const V3<R> r2(el.radius2[0], el.radius2[1], el.radius2[2]);
+      Prim<R> p;
+      p.kind = 4;
+      p.parallelogram = true;
+      p.v0 = (c - r1) - r2;
+      p.e1 = r1 * R(2);
+      p.e2 = r2 * R(2);
+      p.radius = 0;
+      prims.push_back(p);
+      prim_mat.push_back((int)el.material);
+      prim_mesh.push_back(-1);
+      BBox<R> tb(p.v0);
+      tb.add(p.v0 + p.e1);
+      tb.add(p.v0 + p.e2);
+      tb.add(p.v0 + p.e1 + p.e2);
+      pb.push_back(tb);
+      bbox.add(tb);
+    }
     for (unsigned i = 0; i < d.num_spheres; i++) {
       const sgr_sphere &s = d.spheres[i];
       Prim<R> p;
@@ -923,6 +964,38 @@ template<class R> struct Scene {
       pb.push_back(sb);
       bbox.add(sb);
     }
+    // Local primitives (surface/local-primitive.h): cylinder and sphere2,
+    // a canonical shape placed by a transform; the box is the local cube
+    // [-1,1]^3 with its corners taken to world space
+    // (geometry/local-xform.cc:27-31, geometry/bbox.cc:25-42).
+    for (unsigned i = 0; i < d.num_local_prims; i++) {
+      const sgr_local_prim &lp = d.local_prims[i];
+      if (lp.kind != SGR_LOCAL_CYLINDER && lp.kind != SGR_LOCAL_SPHERE2)
+        throw std::runtime_error("local primitive: bad kind");
+      Prim<R> p;
+      p.kind = lp.kind == SGR_LOCAL_CYLINDER ? 5 : 6;
+      p.parallelogram = false;
+      p.radius = 0;
+      float w2l[12];
+      if (!invert_affine(lp.local_to_world, w2l))
+        throw std::runtime_error("local primitive: singular transform");
+      for (int k = 0; k < 12; k++) {
+        p.l2w[k] = R(lp.local_to_world[k]);
+        p.w2l[k] = R(w2l[k]);
+      }
+      prims.push_back(p);
+      prim_mat.push_back((int)lp.material);
+      prim_mesh.push_back(-1);
+      BBox<R> lb;
+      for (int c = 0; c < 8; c++) {
+        const V3<R> corner((c & 4) ? R(1) : R(-1), (c & 2) ? R(1) : R(-1), (c & 1) ? R(1) : R(-1));
+        const V3<R> w = xform_pos(p.l2w, corner);
+        if (c == 0) lb = BBox<R>(w);
+        else lb.add(w);
+      }
+      pb.push_back(lb);
+      bbox.add(lb);
+    }
     // Models and their instances (surface/model.cc:26-47,
     // surface/instance.cc:145-216): the instance's box is the model's box
     // with its 8 corners taken to world space (geometry/bbox.cc:25-42).
@@ -934,6 +1007,8 @@ template<class R> struct Scene {
       md.meshes = m.meshes; md.num_meshes = m.num_meshes;
       md.tripars = m.tripars; md.num_tripars = m.num_tripars;
       md.spheres = m.spheres; md.num_spheres = m.num_spheres;
+      md.ellipses = m.ellipses; md.num_ellipses = m.num_ellipses;
+      md.local_prims = m.local_prims; md.num_local_prims = m.num_local_prims;
       md.materials = d.materials; md.num_materials = d.num_materials;
       md.min_trace = d.min_trace;
       md.accel = d.accel;
@@ -1003,9 +1078,9 @@ template<class R> struct Scene {
     // surface/sphere.cc add_light_samplers; light/envmap-light.cc:66-68)
     for (unsigned p = 0; p < prims.size(); p++) {
       int m = prim_mat[p];
-      if (prims[p].kind != 0 && m >= 0 && mats[m].type == SGR_MAT_GLOW) {
+      if (prims[p].kind != 0 && prims[p].kind != 3 && m >= 0 && mats[m].type == SGR_MAT_GLOW) {
         Light<R> L;
-        L.kind = prims[p].kind == 1 ? 0 : 1;
+        L.kind = prims[p].kind == 1 ? 0 : prims[p].kind == 2 ? 1 : prims[p].kind - 1;
         L.prim = p;
         L.env = -1;
         L.intensity = mats[m].color;
@@ -1100,11 +1175,36 @@ template<class R> struct Scene {
       } else {
         nf = gf;
       }
-    } else if (pr.kind == 1) {
+    } else if (pr.kind == 1 || pr.kind == 4) {
+      // the tripar frame (surface/tripar.cc:45-66); an ellipse's
+      // (surface/ellipse.cc:48-75: s = radius1 / |radius1|, normal
+      // unit(edge2 x edge1)) is the same bits, its radii being the
+      // stored edges halved
       R e1_len = pr.e1.length(), inv_e1_len = R(1) / e1_len;
       V3<R> norm = cross(pr.e2, pr.e1).unit();
       V3<R> s = pr.e1 * inv_e1_len;
       V3<R> t = cross(s, norm);
+      nf = Frame<R>(point, s, t, norm);
+      gf = nf;
+    } else if (pr.kind == 5) {
+      // Cylinder::IsecInfo::make_intersect (surface/cylinder.cc:61-80):
+      // normal from the local hit point's x, y; t along the axis
+      V3<R> norm = local_normal(pr, ray, ray.t1);
+      V3<R> t = xform_vec(pr.l2w, V3<R>(0, 0, 1)).unit();
+      V3<R> s = cross(norm, t);
+      nf = Frame<R>(point, s, t, norm);
+      gf = nf;
+    } else if (pr.kind == 6) {
+      // Sphere2::IsecInfo::make_intersect (surface/sphere2.cc:50-75):
+      // local tangent onorm x z (x at the poles), taken to world space
+      const V3<R> onorm = local_onorm(pr, ray, ray.t1);
+      const R Eps = R(1e-15);
+      const V3<R> ot = (std::fabs(onorm.x) < Eps && std::fabs(onorm.y) < Eps)
+                           ? V3<R>(1, 0, 0)
+                           : cross(onorm, V3<R>(0, 0, 1));
+      V3<R> norm = xform_transpose(pr.w2l, onorm).unit();
+      V3<R> t = xform_vec(pr.l2w, ot).unit();
+      V3<R> s = cross(norm, t);
       nf = Frame<R>(point, s, t, norm);
       gf = nf;
     } else {
@@ -1204,6 +1304,29 @@ template<class R> struct Scene {
           }
         }
       }
+    } else if (L.kind >= 3) {
+      // SurfaceLightSampler::sample (light/surface-light-sampler.cc:
+      // 47-64) over the surface's own sampler
+      V3<R> pos, norm;
+      R apdf;
+      if (L.kind == 5) {
+        sphere2_from_viewpoint(prims[L.prim], is.nf.origin, u, v, pos, norm, apdf);
+      } else if (L.kind == 4) {
+        cylinder_from_viewpoint(prims[L.prim], is.nf.origin, u, v, pos, norm, apdf);
+      } else {
+        ellipse_area(prims[L.prim], u, v, pos, norm, apdf);
+      }
+      R dist, pdf;
+      V3<R> dir;
+      if (angular(pos, norm, apdf, is.nf.origin, dist, dir, pdf)) {
+        V3<R> d = is.nf.to(dir);
+        if (d.z > 0) {
+          out.val = L.intensity;
+          out.pdf = pdf;
+          out.dir = d;
+          out.dist = dist;
+        }
+      }
     } else {
       const Envmap<R> &E = envs[L.env];
       float mu, mv, pdf;
@@ -1236,6 +1359,114 @@ template<class R> struct Scene {
       }
     }
     return pdf > 0;
+  }
+
+  // disk_sample (geometry/disk-sample.h:27-71): concentric map of PARAM
+  // to the disk of RADIUS.
+  static void disk_sample(R radius, R pu, R pv, R &dx, R &dy) {
+    const R u = R(2) * pu - R(1), v = R(2) * pv - R(1);
+    if (u == 0 && v == 0) {
+      dx = dy = 0;
+      return;
+    }
+    R r, theta;
+    if (u >= -v) {
+      if (u > v) {
+        r = u;
+        theta = (v > 0) ? v / r : R(8) + v / r;
+      } else {
+        r = v;
+        theta = R(2) - u / r;
+      }
+    } else {
+      if (u <= v) {
+        r = -u;
+        theta = R(4) - v / r;
+      } else {
+        r = -v;
+        theta = R(6) + u / r;
+      }
+    }
+    theta = theta * R(PIf / 2);
+    dx = radius * (r * std::cos(theta));
+    dy = radius * (r * std::sin(theta));
+  }
+
+  // Ellipse::Sampler::sample (surface/ellipse.cc:218-229): a disk sample
+  // mapped onto the bounding parallelogram; pdf 1 / area.
+  static void ellipse_area(const Prim<R> &pr, R pu, R pv, V3<R> &pos, V3<R> &norm, R &apdf) {
+    R dx, dy;
+    disk_sample(R(0.5), pu, pv, dx, dy);
+    dx = dx + R(0.5);
+    dy = dy + R(0.5);
+    pos = pr.v0 + pr.e1 * dx + pr.e2 * dy;
+    norm = cross(pr.e2, pr.e1).unit();
+    apdf = R(4 * INV_PIf) / cross(pr.e2, pr.e1).length();
+  }
+
+  // sphere_sample / sphere_sample_inverse (geometry/sphere-sample.h,
+  // geometry/cone-sample.h:27-62 with cos_half_angle -1).
+  static V3<R> sphere_sample(R pu, R pv) {
+    const R z = R(-1) + pv * R(2);
+    const R r = std::sqrt(std::fabs(R(1) - z * z));
+    const R phi = pu * R(2) * R(PIf);
+    return V3<R>(r * std::cos(phi), r * std::sin(phi), z);
+  }
+  static void sphere_sample_inverse(const V3<R> &d, R &pu, R &pv) {
+    const R phi = std::atan2(d.y, -d.x);
+    pu = clamp01(R(0.5f) - (phi * R(INV_PIf) * R(0.5f)));
+    pv = clamp01((d.z + R(1)) / R(2));
+  }
+
+  // Surface::Sampler::sample_with_approx_area_pdf (surface/surface-
+  // sampler.h:141-152): the area pdf from the parallelogram spanned by
+  // the positions at PARAM + (delta, 0) and PARAM + (0, delta).
+  template<class F> static R approx_area_pdf(const F &f, R pu, R pv, V3<R> &pos) {
+    const R delta = R(0.0001f);
+    pos = f(pu, pv);
+    const V3<R> pdu = f(pu + delta, pv), pdv = f(pu, pv + delta);
+    const R sample_area = cross(pdu - pos, pdv - pos).length();
+    const R param_area = delta * delta;
+    return sample_area == 0 ? R(0) : param_area / sample_area;
+  }
+
+  // Cylinder::Sampler::sample_from_viewpoint (surface/cylinder.cc:
+  // 239-306): an area sample, mirrored through the axis when its normal
+  // faces away from VP, pdf doubled.
+  static void cylinder_from_viewpoint(const Prim<R> &pr, const V3<R> &vp, R pu, R pv,
+                                      V3<R> &pos, V3<R> &norm, R &apdf) {
+    const R theta = pu * R(2) * R(PIf);
+    norm = xform_transpose(pr.w2l, V3<R>(std::cos(theta), std::sin(theta), R(0))).unit();
+    auto f = [&pr](R a, R b) {
+      const R th = a * R(2) * R(PIf);
+      return xform_pos(pr.l2w, V3<R>(std::cos(th), std::sin(th), R(2) * b - R(1)));
+    };
+    apdf = approx_area_pdf(f, pu, pv, pos);
+    if (dot(norm, pos - vp) > 0) {
+      V3<R> op = xform_pos(pr.w2l, pos);
+      op.x = -op.x;
+      op.y = -op.y;
+      pos = xform_pos(pr.l2w, op);
+      norm = -norm;
+    }
+    apdf = apdf * R(2);
+  }
+
+  // Sphere2::Sampler::sample_from_viewpoint (surface/sphere2.cc:
+  // 238-270): a sample of the hemisphere facing VP (the other half
+  // mirrored through the centre), pdf doubled.
+  static void sphere2_from_viewpoint(const Prim<R> &pr, const V3<R> &vp, R pu, R pv,
+                                     V3<R> &pos, V3<R> &norm, R &apdf) {
+    const V3<R> ov = xform_pos(pr.w2l, vp);
+    V3<R> onorm = sphere_sample(pu, pv);
+    R su = pu, sv = pv;
+    if (dot(onorm, ov) < 0) {
+      onorm = -onorm;
+      sphere_sample_inverse(onorm, su, sv);
+    }
+    norm = xform_transpose(pr.w2l, onorm).unit();
+    auto f = [&pr](R a, R b) { return xform_pos(pr.l2w, sphere_sample(a, b)); };
+    apdf = approx_area_pdf(f, su, sv, pos) * R(2);
   }
 
   // light/sphere-light-sampler.cc:40-48
@@ -1279,6 +1510,50 @@ template<class R> struct Scene {
           out.pdf = R(1) / sphere_solid_angle(lc, pr.radius);
           out.dist = dist;
         }
+      }
+    } else if (L.kind >= 3) {
+      // SurfaceLightSampler::eval (light/surface-light-sampler.cc:
+      // 86-100) over the surface's eval_from_viewpoint
+      const Prim<R> &pr = prims[L.prim];
+      const V3<R> vp = is.nf.origin, wdir = is.nf.from(dir);
+      V3<R> pos, norm;
+      R apdf = 0;
+      bool hit = false;
+      if (L.kind == 3) {  // surface/ellipse.cc:233-246
+        R t, uu, vv;
+        if (ellipse_intersects(pr.v0, pr.e1, pr.e2, vp, wdir, R(0), t, uu, vv)) {
+          pos = vp + wdir * t;
+          norm = cross(pr.e2, pr.e1).unit();
+          apdf = R(4 * INV_PIf) / cross(pr.e2, pr.e1).length();
+          hit = true;
+        }
+      } else if (L.kind == 4) {  // surface/cylinder.cc:322-347
+        const V3<R> ov = xform_pos(pr.w2l, vp), od = xform_vec(pr.w2l, wdir);
+        R t;
+        if (cylinder_intersects(ov, od, R(0), t)) {
+          const V3<R> lp = ov + od * t;
+          R pu = R(std::atan2(lp.y, lp.x)) * R(INV_PIf) * R(0.5f);
+          if (pu < 0) pu += R(1);
+          const R pv = lp.z * R(0.5f) + R(0.5f);
+          cylinder_from_viewpoint(pr, vp, clamp01(pu), clamp01(pv), pos, norm, apdf);
+          hit = true;
+        }
+      } else {  // surface/sphere2.cc:278-298
+        const V3<R> ov = xform_pos(pr.w2l, vp), od = xform_vec(pr.w2l, wdir);
+        R t;
+        if (sphere_intersects(V3<R>(0, 0, 0), R(1), ov, od, R(0), t)) {
+          R pu, pv;
+          sphere_sample_inverse(ov + od * t, pu, pv);
+          sphere2_from_viewpoint(pr, vp, pu, pv, pos, norm, apdf);
+          hit = true;
+        }
+      }
+      R dist, pdf;
+      V3<R> sd;
+      if (hit && angular(pos, norm, apdf, vp, dist, sd, pdf)) {
+        out.val = L.intensity;
+        out.pdf = pdf;
+        out.dist = dist;
       }
     } else {
       const Envmap<R> &E = envs[L.env];

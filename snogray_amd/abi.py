@@ -56,6 +56,20 @@ class Sphere(C.Structure):
                 ("material", C.c_uint32)]
 
 
+class Ellipse(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("radius1", C.c_float * 3),
+                ("radius2", C.c_float * 3), ("material", C.c_uint32)]
+
+
+SGR_LOCAL_CYLINDER = 1
+SGR_LOCAL_SPHERE2 = 2
+
+
+class LocalPrim(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("local_to_world", C.c_float * 12),
+                ("material", C.c_uint32)]
+
+
 class Envmap(C.Structure):
     _fields_ = [("texels", f32p), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("frame", C.c_float * 9)]
@@ -70,13 +84,17 @@ class SceneDescC(C.Structure):
                 ("envmaps", C.POINTER(Envmap)), ("num_envmaps", C.c_uint32),
                 ("min_trace", C.c_float), ("accel", C.c_int32),
                 ("models", C.c_void_p), ("num_models", C.c_uint32),
-                ("instances", C.c_void_p), ("num_instances", C.c_uint32)]
+                ("instances", C.c_void_p), ("num_instances", C.c_uint32),
+                ("ellipses", C.POINTER(Ellipse)), ("num_ellipses", C.c_uint32),
+                ("local_prims", C.POINTER(LocalPrim)), ("num_local_prims", C.c_uint32)]
 
 
 class Model(C.Structure):
     _fields_ = [("meshes", C.POINTER(Mesh)), ("num_meshes", C.c_uint32),
                 ("tripars", C.POINTER(Tripar)), ("num_tripars", C.c_uint32),
-                ("spheres", C.POINTER(Sphere)), ("num_spheres", C.c_uint32)]
+                ("spheres", C.POINTER(Sphere)), ("num_spheres", C.c_uint32),
+                ("ellipses", C.POINTER(Ellipse)), ("num_ellipses", C.c_uint32),
+                ("local_prims", C.POINTER(LocalPrim)), ("num_local_prims", C.c_uint32)]
 
 
 class Instance(C.Structure):

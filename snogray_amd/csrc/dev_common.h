@@ -12,6 +12,15 @@
 namespace sgr {
 
 #define SGR_REF_STRIDE 3  // float4s per surface reference on the device
+// Surface-record kinds beyond triangle 0, parallelogram 1, sphere 2 and
+// instance 3: an ellipse (its bounding parallelogram inline) and the
+// local primitives, whose record holds an index into the scene's table
+// of SGR_LOC_STRIDE float4s each (world-to-local rows, local-to-world
+// rows, then the kind).
+#define SGR_REF_ELLIPSE 4u
+#define SGR_REF_CYLINDER 5u
+#define SGR_REF_SPHERE2 6u
+#define SGR_LOC_STRIDE 8
 #define SGR_MAX_DEPTH 42  // octree levels a traversal can hold (checked at scene creation)
 // uint32 words between the fetch heads of the lane-refill walk: each head
 // has a 128-byte line to itself, so the atomics of different XCD slices
@@ -42,7 +51,10 @@ struct DevMaterial {
 };
 
 struct DevLight {
-  int kind;        // 0 = tripar area light, 1 = sphere light, 2 = envmap
+  // 0 = tripar area light, 1 = sphere light, 2 = envmap; surface lights
+  // (light/surface-light-sampler.cc) of 3 = an ellipse, 4 = a cylinder,
+  // 5 = a sphere2
+  int kind;
   uint32_t prim;
   float intensity[3];
 };
@@ -71,6 +83,7 @@ struct SceneView {
   const float4 *refs;        // 3 per surface reference, surface-list order
   const float4 *tri;         // 3 per triangle-like prim: v0|flags, e1, e2
   const float4 *sph;         // centre, radius
+  const float4 *loc;         // local primitives (SGR_LOC_STRIDE per primitive)
   const float4 *tri_vn;      // 3 per mesh triangle: its vertex normals (or null: none)
   const uint32_t *prim_mat;  // material index per prim
   const uint32_t *prim_mesh; // mesh index per triangle (0xffffffff others)
@@ -98,7 +111,12 @@ struct SceneView {
   // shadow rays take the general walk, which multiplies their
   // transmittance (such scenes, like instanced ones, use the INST kernels).
   uint32_t has_partial;
-  uint32_t n_tri_like;       // triangles + tripars
+  // Ellipses or local primitives (cylinder, sphere2) in this scene or its
+  // models: its walks take the general kernels, whose surface tests,
+  // frames and light samplers cover them.
+  uint32_t has_ext;
+  uint32_t n_tri_like;       // triangles + tripars + ellipses
+  uint32_t n_sph_end;        // n_tri_like + spheres: the first local primitive
   uint32_t n_tri;            // mesh triangles
   uint32_t n_prims;
   uint32_t n_nodes;

@@ -43,8 +43,34 @@ __device__ __forceinline__ Frame mesh_frame(const DevMesh &M, V3 point, V3 norm)
   return Frame{point, s, t, norm};
 }
 
+// A local primitive's local-to-world linear part applied to a vector.
+__device__ __forceinline__ V3 l2w_vec(const float4 *lr, V3 v) {
+  return V3{v.x * lr[3].x + v.y * lr[3].y + v.z * lr[3].z,
+            v.x * lr[4].x + v.y * lr[4].y + v.z * lr[4].z,
+            v.x * lr[5].x + v.y * lr[5].y + v.z * lr[5].z};
+}
+
+// Normal frame of a local primitive hit (surface/cylinder.cc:61-80,
+// surface/sphere2.cc:50-75): the normal by the inverse transpose, the
+// second tangent t the local axis (cylinder) or onorm x z, x at the poles
+// (sphere2), by the transform, s = normal x t.
+__device__ __forceinline__ Frame local_frame(const float4 *lr, RayF ray, V3 point) {
+  const V3 onorm = local_onorm(lr, ray, ray.t1);
+  const V3 norm = unit(w2l_transpose(lr, onorm));
+  V3 ot;
+  if (__float_as_uint(lr[6].x) == SGR_REF_CYLINDER)
+    ot = v3(0.0f, 0.0f, 1.0f);
+  else
+    ot = (fabsf(onorm.x) < 1e-15f && fabsf(onorm.y) < 1e-15f) ? v3(1.0f, 0.0f, 0.0f)
+                                                               : cross(onorm, v3(0.0f, 0.0f, 1.0f));
+  const V3 t = unit(l2w_vec(lr, ot));
+  return Frame{point, cross(norm, t), t, norm};
+}
+
 // Hit frames: surface/mesh.cc:563-709, surface/tripar.cc:45-66,
 // surface/sphere.cc make_intersect; then Intersect::finish_init.
+// EXT: the scene may hold local primitives (the general kernels).
+template<bool EXT>
 __device__ __forceinline__ void make_isec_local(const SceneView &S, uint32_t p, const RayF &ray,
                                                 float hu, float hv, float media_ior,
                                                 float encl_ior, Isec &is) {
@@ -71,6 +97,10 @@ __device__ __forceinline__ void make_isec_local(const SceneView &S, uint32_t p, 
         nf = gf;
       }
     } else {
+      // tripar (surface/tripar.cc:45-66); an ellipse's frame
+      // (surface/ellipse.cc:48-75: s = radius1 / |radius1|, normal
+      // unit(edge2 x edge1)) is the same bits, its radii being the stored
+      // edges halved
       float e1_len = vlen(e1), inv = 1.0f / e1_len;
       V3 norm = unit(cross(e2, e1));
       V3 s = e1 * inv;
@@ -78,6 +108,9 @@ __device__ __forceinline__ void make_isec_local(const SceneView &S, uint32_t p, 
       nf = Frame{point, s, t, norm};
       gf = nf;
     }
+  } else if (EXT && p >= S.n_sph_end) {
+    nf = local_frame(S.loc + SGR_LOC_STRIDE * (p - S.n_sph_end), ray, point);
+    gf = nf;
   } else {
     float4 sp = S.sph[p - S.n_tri_like];
     V3 norm = unit(point - ld3(sp));
@@ -116,11 +149,12 @@ __device__ __forceinline__ void make_isec_local(const SceneView &S, uint32_t p, 
 // origin and tangents by the transform, the normal by the inverse
 // transpose, each renormalised (Instance::Renderable::IsecInfo::
 // make_intersect, surface/instance.cc:87-103).
+template<bool EXT>
 __device__ __forceinline__ void make_isec(const SceneView &S, uint32_t p, uint32_t inst,
                                           const RayF &ray, float hu, float hv, float media_ior,
                                           float encl_ior, Isec &is) {
   if (inst == 0xffffffffu) {
-    make_isec_local(S, p, ray, hu, hv, media_ior, encl_ior, is);
+    make_isec_local<EXT>(S, p, ray, hu, hv, media_ior, encl_ior, is);
     return;
   }
   const DevInstance &I = S.insts[inst];
@@ -129,7 +163,7 @@ __device__ __forceinline__ void make_isec(const SceneView &S, uint32_t p, uint32
   r2.d = xform_vec(I.w2l, ray.d);
   r2.t0 = ray.t0;
   r2.t1 = ray.t1;
-  make_isec_local(S.models[I.model], p, r2, hu, hv, media_ior, encl_ior, is);
+  make_isec_local<EXT>(S.models[I.model], p, r2, hu, hv, media_ior, encl_ior, is);
   is.nf.o = xform_pos(I.l2w, is.nf.o);
   is.nf.x = unit(xform_vec(I.l2w, is.nf.x));
   is.nf.y = unit(xform_vec(I.l2w, is.nf.y));
@@ -740,6 +774,172 @@ __device__ __forceinline__ void tripar_area(V3 v0, V3 e1, V3 e2, bool par, float
   apdf = 1.0f / area;
 }
 
+// disk_sample (geometry/disk-sample.h:27-71): concentric map of (pu, pv)
+// to the disk of RADIUS.
+__device__ __forceinline__ void disk_sample(float radius, float pu, float pv, float &dx,
+                                            float &dy) {
+  const float u = 2.0f * pu - 1.0f, v = 2.0f * pv - 1.0f;
+  if (u == 0.0f && v == 0.0f) {
+    dx = dy = 0.0f;
+    return;
+  }
+  float r, theta;
+  if (u >= -v) {
+    if (u > v) {
+      r = u;
+      theta = (v > 0.0f) ? v / r : 8.0f + v / r;
+    } else {
+      r = v;
+      theta = 2.0f - u / r;
+    }
+  } else {
+    if (u <= v) {
+      r = -u;
+      theta = 4.0f - v / r;
+    } else {
+      r = -v;
+      theta = 6.0f + u / r;
+    }
+  }
+  theta = theta * (SGR_PIf / 2.0f);
+  dx = radius * (r * cosf(theta));
+  dy = radius * (r * sinf(theta));
+}
+
+// Ellipse::Sampler::sample (surface/ellipse.cc:218-229): a disk sample
+// on the bounding parallelogram (corner A, edges E1, E2), pdf 1 / area.
+__device__ __forceinline__ void ellipse_area(V3 a, V3 e1, V3 e2, float pu, float pv, V3 &pos,
+                                             V3 &norm, float &apdf) {
+  float dx, dy;
+  disk_sample(0.5f, pu, pv, dx, dy);
+  dx = dx + 0.5f;
+  dy = dy + 0.5f;
+  pos = (a + e1 * dx) + e2 * dy;
+  const V3 n = cross(e2, e1);
+  norm = unit(n);
+  apdf = (4.0f * SGR_INV_PIf) / vlen(n);
+}
+
+// sphere_sample / sphere_sample_inverse (geometry/sphere-sample.h,
+// geometry/cone-sample.h:27-62 with cos_half_angle -1).
+__device__ __forceinline__ V3 sphere_sample(float pu, float pv) {
+  const float z = -1.0f + pv * 2.0f;
+  const float r = sqrtf(fabsf(1.0f - z * z));
+  const float phi = pu * 2.0f * SGR_PIf;
+  return v3(r * cosf(phi), r * sinf(phi), z);
+}
+__device__ __forceinline__ void sphere_sample_inverse(V3 d, float &pu, float &pv) {
+  const float phi = atan2f(d.y, -d.x);
+  pu = clamp01f(0.5f - (phi * SGR_INV_PIf * 0.5f));
+  pv = clamp01f((d.z + 1.0f) / 2.0f);
+}
+
+// A local primitive's local-to-world rows applied to a position / vector.
+__device__ __forceinline__ V3 l2w_pos(const float4 *lr, V3 p) {
+  return V3{p.x * lr[3].x + p.y * lr[3].y + p.z * lr[3].z + lr[3].w,
+            p.x * lr[4].x + p.y * lr[4].y + p.z * lr[4].z + lr[4].w,
+            p.x * lr[5].x + p.y * lr[5].y + p.z * lr[5].z + lr[5].w};
+}
+__device__ __forceinline__ V3 w2l_pos(const float4 *lr, V3 p) {
+  return V3{p.x * lr[0].x + p.y * lr[0].y + p.z * lr[0].z + lr[0].w,
+            p.x * lr[1].x + p.y * lr[1].y + p.z * lr[1].z + lr[1].w,
+            p.x * lr[2].x + p.y * lr[2].y + p.z * lr[2].z + lr[2].w};
+}
+
+// Surface::Sampler::sample_with_approx_area_pdf (surface/surface-sampler.h:
+// 141-152) for a cylinder (CYL) or a sphere2: the area pdf from the
+// parallelogram spanned by the positions at (pu + delta, pv) and
+// (pu, pv + delta).
+__device__ __forceinline__ V3 local_sample_pos(const float4 *lr, bool cyl, float pu, float pv) {
+  if (cyl) {
+    const float th = pu * 2.0f * SGR_PIf;
+    return l2w_pos(lr, v3(cosf(th), sinf(th), 2.0f * pv - 1.0f));
+  }
+  return l2w_pos(lr, sphere_sample(pu, pv));
+}
+__device__ __forceinline__ float approx_area_pdf(const float4 *lr, bool cyl, float pu, float pv,
+                                                 V3 &pos) {
+  const float delta = 0.0001f;
+  pos = local_sample_pos(lr, cyl, pu, pv);
+  const V3 pdu = local_sample_pos(lr, cyl, pu + delta, pv);
+  const V3 pdv = local_sample_pos(lr, cyl, pu, pv + delta);
+  const float sample_area = vlen(cross(pdu - pos, pdv - pos));
+  const float param_area = delta * delta;
+  return sample_area == 0.0f ? 0.0f : param_area / sample_area;
+}
+
+// Cylinder::Sampler::sample_from_viewpoint (surface/cylinder.cc:239-306)
+// and Sphere2::Sampler::sample_from_viewpoint (surface/sphere2.cc:
+// 238-270): a sample facing VP (a cylinder's mirrored through its axis, a
+// sphere2's through its centre, when it faces away), pdf doubled.
+__device__ __forceinline__ void local_from_viewpoint(const float4 *lr, V3 vp, float pu, float pv,
+                                                     V3 &pos, V3 &norm, float &apdf) {
+  if (__float_as_uint(lr[6].x) == SGR_REF_CYLINDER) {
+    const float theta = pu * 2.0f * SGR_PIf;
+    norm = unit(w2l_transpose(lr, v3(cosf(theta), sinf(theta), 0.0f)));
+    apdf = approx_area_pdf(lr, true, pu, pv, pos);
+    if (dot(norm, pos - vp) > 0.0f) {
+      V3 op = w2l_pos(lr, pos);
+      op.x = -op.x;
+      op.y = -op.y;
+      pos = l2w_pos(lr, op);
+      norm = -norm;
+    }
+    apdf = apdf * 2.0f;
+    return;
+  }
+  const V3 ov = w2l_pos(lr, vp);
+  V3 onorm = sphere_sample(pu, pv);
+  float su = pu, sv = pv;
+  if (dot(onorm, ov) < 0.0f) {
+    onorm = -onorm;
+    sphere_sample_inverse(onorm, su, sv);
+  }
+  norm = unit(w2l_transpose(lr, onorm));
+  apdf = approx_area_pdf(lr, false, su, sv, pos) * 2.0f;
+}
+
+// SurfaceLightSampler::sample (light/surface-light-sampler.cc:47-64) of
+// an ellipse, cylinder or sphere2 light from the frame's origin.
+__device__ __forceinline__ LSample surface_light_sample(const float4 *tri,
+                                                                 const float4 *loc,
+                                                                 uint32_t n_sph_end, DevLight L,
+                                                                 Frame nf, float u, float v) {
+  LSample o;
+  o.val = c3(0.0f);
+  o.pdf = 0.0f;
+  o.dir = v3(0.0f, 0.0f, 0.0f);
+  o.dist = 0.0f;
+  V3 pos, norm;
+  float apdf;
+  if (L.kind == 3) {
+    const float4 a = tri[3 * L.prim], b = tri[3 * L.prim + 1], c = tri[3 * L.prim + 2];
+    ellipse_area(ld3(a), ld3(b), ld3(c), u, v, pos, norm, apdf);
+  } else {
+    local_from_viewpoint(loc + SGR_LOC_STRIDE * (L.prim - n_sph_end), nf.o, u, v, pos, norm, apdf);
+  }
+  float dist, pdf;
+  V3 dir;
+  if (angular(pos, norm, apdf, nf.o, dist, dir, pdf)) {
+    V3 d = nf.to(dir);
+    if (d.z > 0.0f) {
+      o.val = cload(L.intensity);
+      o.pdf = pdf;
+      o.dir = d;
+      o.dist = dist;
+    }
+  }
+  return o;
+}
+
+__device__ __forceinline__ LSample surface_light_eval(const float4 *tri,
+                                                               const float4 *loc,
+                                                               uint32_t n_sph_end, DevLight L,
+                                                               Frame nf, V3 dir);
+
+// EXT: the scene may hold ellipse, cylinder and sphere2 lights (the
+// general kernels).
+template<bool EXT>
 __device__ __forceinline__ LSample light_sample(const SceneView &S, const DevLight &L,
                                                 const Isec &is, float u, float v,
                                                 float extra) {
@@ -785,6 +985,8 @@ __device__ __forceinline__ LSample light_sample(const SceneView &S, const DevLig
         }
       }
     }
+  } else if (EXT && L.kind >= 3) {
+    o = surface_light_sample(S.tri, S.loc, S.n_sph_end, L, is.nf, u, v);
   } else {  // light/envmap-light.cc:100-125
     float mu, mv, pdf;
     env_hist_sample(S, u, v, mu, mv, pdf);
@@ -800,6 +1002,7 @@ __device__ __forceinline__ LSample light_sample(const SceneView &S, const DevLig
   return o;
 }
 
+template<bool EXT>
 __device__ __forceinline__ LSample light_eval(const SceneView &S, const DevLight &L,
                                               const Isec &is, V3 dir) {
   LSample o;
@@ -838,6 +1041,8 @@ __device__ __forceinline__ LSample light_eval(const SceneView &S, const DevLight
         o.dist = dist;
       }
     }
+  } else if (EXT && L.kind >= 3) {
+    o = surface_light_eval(S.tri, S.loc, S.n_sph_end, L, is.nf, dir);
   } else {  // light/envmap-light.cc:150-170
     V3 wd = is.nf.from(dir);
     V3 ld = env_to(S, wd);
@@ -845,6 +1050,70 @@ __device__ __forceinline__ LSample light_eval(const SceneView &S, const DevLight
     latlong(ld, mu, mv);
     o.val = env_lookup(S, mu, mv);
     o.pdf = env_hist_pdf(S, mu, mv) * (0.25f * SGR_INV_PIf);
+  }
+  return o;
+}
+
+// SurfaceLightSampler::eval (light/surface-light-sampler.cc:86-100) of an
+// ellipse, cylinder or sphere2 light: its eval_from_viewpoint from the
+// frame's origin in direction DIR (frame coordinates).
+__device__ __forceinline__ LSample surface_light_eval(const float4 *tri,
+                                                               const float4 *loc,
+                                                               uint32_t n_sph_end, DevLight L,
+                                                               Frame nf, V3 dir) {
+  LSample o;
+  o.val = c3(0.0f);
+  o.pdf = 0.0f;
+  o.dir = dir;
+  o.dist = 0.0f;
+  {
+    const V3 vp = nf.o, wdir = nf.from(dir);
+    V3 pos, norm;
+    float apdf = 0.0f;
+    bool hit = false;
+    if (L.kind == 3) {  // surface/ellipse.cc:233-246
+      const float4 a = tri[3 * L.prim], b = tri[3 * L.prim + 1], c = tri[3 * L.prim + 2];
+      RayF r;
+      r.o = vp;
+      r.d = wdir;
+      r.t0 = 0.0f;
+      r.t1 = __builtin_inff();
+      float t, uu, vv;
+      if (ellipse_hit(ld3(a), ld3(b), ld3(c), r, t, uu, vv)) {
+        pos = vp + wdir * t;
+        const V3 n = cross(ld3(c), ld3(b));
+        norm = unit(n);
+        apdf = (4.0f * SGR_INV_PIf) / vlen(n);
+        hit = true;
+      }
+    } else {  // surface/cylinder.cc:322-347, surface/sphere2.cc:278-298
+      const float4 *lr = loc + SGR_LOC_STRIDE * (L.prim - n_sph_end);
+      V3 ov, od;
+      local_ray(lr, vp, wdir, ov, od);
+      float t;
+      if (L.kind == 4) {
+        if (cylinder_hit(ov, od, 0.0f, t)) {
+          const V3 lp = ov + od * t;
+          float pu = atan2f(lp.y, lp.x) * SGR_INV_PIf * 0.5f;
+          if (pu < 0.0f) pu += 1.0f;
+          const float pv = lp.z * 0.5f + 0.5f;
+          local_from_viewpoint(lr, vp, clamp01f(pu), clamp01f(pv), pos, norm, apdf);
+          hit = true;
+        }
+      } else if (sphere_hit(v3(0.0f, 0.0f, 0.0f), 1.0f, ov, od, 0.0f, t)) {
+        float pu, pv;
+        sphere_sample_inverse(ov + od * t, pu, pv);
+        local_from_viewpoint(lr, vp, pu, pv, pos, norm, apdf);
+        hit = true;
+      }
+    }
+    float dist, pdf;
+    V3 sd;
+    if (hit && angular(pos, norm, apdf, vp, dist, sd, pdf)) {
+      o.val = cload(L.intensity);
+      o.pdf = pdf;
+      o.dist = dist;
+    }
   }
   return o;
 }
@@ -873,11 +1142,11 @@ __device__ __forceinline__ RayF direct_shadow_ray(const SceneView &S, const Isec
   r.t1 = md - S.min_trace;
   return r;
 }
-template<typename Use>
+template<bool EXT, typename Use>
 __device__ __forceinline__ void direct_light_term(const SceneView &S, const DevLight &L,
                                                   const Isec &is, const DevMaterial &M,
                                                   float lu, float lv, float extra, Use use) {
-  const LSample ls = light_sample(S, L, is, lu, lv, extra);
+  const LSample ls = light_sample<EXT>(S, L, is, lu, lv, extra);
   if (ls.pdf > 0.0f && cgt0(ls.val)) {
     const BValue bval = bsdf_eval(is, M, ls.dir, F_ALL & ~F_SPECULAR);
     if (cgt0(bval.val)) {
@@ -890,13 +1159,13 @@ __device__ __forceinline__ void direct_light_term(const SceneView &S, const DevL
     }
   }
 }
-template<typename Use>
+template<bool EXT, typename Use>
 __device__ __forceinline__ void direct_bsdf_term(const SceneView &S, const DevLight &L,
                                                  const Isec &is, const DevMaterial &M, float bu,
                                                  float bv, Use use) {
   const BSample bs = bsdf_sample(is, M, bu, bv, F_ALL & ~F_SPECULAR);
   if (bs.pdf > 0.0f && cgt0(bs.val)) {
-    const LSample lval = light_eval(S, L, is, bs.dir);
+    const LSample lval = light_eval<EXT>(S, L, is, bs.dir);
     if (lval.pdf > 0.0f && cgt0(lval.val)) {
       C3 c = lval.val;
       c = c * mis_weight(bs.pdf, lval.pdf);

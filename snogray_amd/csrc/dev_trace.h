@@ -35,7 +35,31 @@ __device__ __forceinline__ bool tripar_hit(V3 corner, V3 e1, V3 e2, bool par, co
   return t > r.t0 && t < r.t1;
 }
 
-// geometry/quadratic-roots.h:30-70 + geometry/sphere-isec.h:35-60
+// geometry/quadratic-roots.h:30-70: the real roots of a t^2 + b t + cc,
+// ascending; returns how many (0, 1 or 2).
+__device__ __forceinline__ unsigned quad_roots(float a, float b, float cc, float &r0,
+                                               float &r1) {
+  if (a == 0.0f) {
+    if (b == 0.0f) return 0;
+    r0 = -cc / b;
+    r1 = r0;
+    return 1;
+  }
+  float disc = b * b - 4.0f * a * cc;
+  if (disc < 0.0f) return 0;
+  float sd = sqrtf(disc);
+  float q = -(b + (b < 0.0f ? -sd : sd)) / 2.0f;
+  r0 = q / a;
+  if (q == 0.0f) {
+    r1 = r0;
+    return 1;
+  }
+  r1 = cc / q;
+  if (r0 > r1) { float tmp = r0; r0 = r1; r1 = tmp; }
+  return 2;
+}
+
+// geometry/sphere-isec.h:35-60
 __device__ __forceinline__ bool sphere_hit(V3 c, float radius, V3 ro, V3 rd, float min_t,
                                            float &t) {
   V3 off = ro - c;
@@ -43,39 +67,94 @@ __device__ __forceinline__ bool sphere_hit(V3 c, float radius, V3 ro, V3 rd, flo
   float b = 2.0f * dot(rd, off);
   float cc = len2(off) - radius * radius;
   float r0, r1;
-  unsigned n;
-  if (a == 0.0f) {
-    if (b == 0.0f) return false;
-    r0 = -cc / b;
-    n = 1;
-    r1 = r0;
-  } else {
-    float disc = b * b - 4.0f * a * cc;
-    if (disc < 0.0f) return false;
-    float sd = sqrtf(disc);
-    float q = -(b + (b < 0.0f ? -sd : sd)) / 2.0f;
-    r0 = q / a;
-    if (q == 0.0f) {
-      n = 1;
-      r1 = r0;
-    } else {
-      r1 = cc / q;
-      if (r0 > r1) { float tmp = r0; r0 = r1; r1 = tmp; }
-      n = 2;
-    }
-  }
-  if (r0 > min_t) { t = r0; return true; }
+  const unsigned n = quad_roots(a, b, cc, r0, r1);
+  if (n >= 1 && r0 > min_t) { t = r0; return true; }
   if (n == 2 && r1 > min_t) { t = r1; return true; }
   return false;
 }
 
+// cylinder_intersects (surface/cylinder.cc:114-150): the canonical
+// cylinder (radius 1 about z, z in [-1, 1], no caps); the first root
+// beyond MIN_T whose z is inside.  The constant term subtracts the radius
+// itself, not its square (both 1).
+__device__ __forceinline__ bool cylinder_hit(V3 ro, V3 rd, float min_t, float &t) {
+  const float a = rd.x * rd.x + rd.y * rd.y;
+  const float b = 2.0f * (rd.x * ro.x + rd.y * ro.y);
+  const float cc = ro.x * ro.x + ro.y * ro.y - 1.0f;
+  float r0, r1;
+  const unsigned n = quad_roots(a, b, cc, r0, r1);
+  for (unsigned i = 0; i < n; i++) {
+    const float ti = i ? r1 : r0;
+    if (ti > min_t) {
+      const float z = ro.z + ti * rd.z;
+      if (z >= -1.0f && z <= 1.0f) { t = ti; return true; }
+    }
+  }
+  return false;
+}
+
+// Ellipse::intersects (surface/ellipse.h:95-110): the bounding
+// parallelogram's hit, kept when (u, v) lies in the inscribed ellipse.
+__device__ __forceinline__ bool ellipse_hit(V3 corner, V3 e1, V3 e2, const RayF &r, float &t,
+                                            float &u, float &v) {
+  if (!tripar_hit(corner, e1, e2, true, r, t, u, v)) return false;
+  const float x = u - 0.5f, y = v - 0.5f;
+  return x * x + y * y <= 0.25f;
+}
+
+// A local primitive's ray in its own space (LocalXform::world_to_local:
+// origin and direction transformed, the parametric extent kept).
+__device__ __forceinline__ void local_ray(const float4 *lr, V3 o, V3 d, V3 &lo, V3 &ld) {
+  const float m[12] = {lr[0].x, lr[0].y, lr[0].z, lr[0].w, lr[1].x, lr[1].y,
+                       lr[1].z, lr[1].w, lr[2].x, lr[2].y, lr[2].z, lr[2].w};
+  lo = xform_pos(m, o);
+  ld = xform_vec(m, d);
+}
+
+// Surface test of a local primitive (surface/cylinder.cc:152-186,
+// surface/sphere2.cc:92-110) with table rows LR.
+__device__ __forceinline__ bool local_hit(const float4 *lr, uint32_t kind, const RayF &r,
+                                          float &t) {
+  V3 lo, ld;
+  local_ray(lr, r.o, r.d, lo, ld);
+  const bool h = kind == SGR_REF_CYLINDER ? cylinder_hit(lo, ld, r.t0, t)
+                                          : sphere_hit(v3(0.0f, 0.0f, 0.0f), 1.0f, lo, ld, r.t0, t);
+  return h && t < r.t1;
+}
+
+// IsecInfo::normal of a local primitive hit at T (surface/cylinder.cc:
+// 84-90, surface/sphere2.cc:79-83): the local hit point (its x, y for a
+// cylinder) by the inverse transpose, unit; ONORM gets the local normal.
+__device__ __forceinline__ V3 local_onorm(const float4 *lr, const RayF &r, float t) {
+  V3 lo, ld;
+  local_ray(lr, r.o, r.d, lo, ld);
+  const V3 ip = lo + ld * t;
+  return __float_as_uint(lr[6].x) == SGR_REF_CYLINDER ? v3(ip.x, ip.y, 0.0f) : ip;
+}
+__device__ __forceinline__ V3 w2l_transpose(const float4 *lr, V3 v) {
+  return V3{v.x * lr[0].x + v.y * lr[1].x + v.z * lr[2].x,
+            v.x * lr[0].y + v.y * lr[1].y + v.z * lr[2].y,
+            v.x * lr[0].z + v.y * lr[1].z + v.z * lr[2].z};
+}
+__device__ __forceinline__ V3 local_normal(const float4 *lr, const RayF &r, float t) {
+  return unit(w2l_transpose(lr, local_onorm(lr, r, t)));
+}
+
 // One surface reference: three float4 in surface-list order.
-//   triangle / tripar: (v0, prim id bits) (e1, kind) (e2, 0)
+//   triangle / tripar / ellipse: (v0, prim id bits) (e1, kind) (e2, 0)
 //   sphere:            (centre, prim id bits) (radius, 0, 0, kind)
+//   cylinder / sphere2: (0, prim id bits) (0, kind) (table index bits, 0)
 // kind: 0 triangle, 1 parallelogram, 2 sphere, 3 instance of a model
-// (record (0, n_prims + instance index) (0, kind) (0); see ref_test).
-__device__ __forceinline__ bool ref_hit(const float4 *refs, uint32_t k, const RayF &r,
-                                        float &t, float &u, float &v, uint32_t &prim) {
+// (record (0, n_prims + instance index) (0, kind) (0); see ref_test),
+// 4 ellipse, 5 cylinder, 6 sphere2.  LOC: the local-primitive table.
+// EXT: the scene may hold ellipses and local primitives (kinds 4-6); the
+// code for them, inlined into every walk, would raise the register
+// allocation of the walks over scenes that hold none (6 -> 5 waves per
+// SIMD), so those scenes take the general walk.
+template<bool EXT>
+__device__ __forceinline__ bool ref_hit(const float4 *refs, const float4 *loc, uint32_t k,
+                                        const RayF &r, float &t, float &u, float &v,
+                                        uint32_t &prim) {
   // All three loads issued together (a sphere's third record is zeros),
   // as streaming loads: a reference is rarely read again soon by the same
   // CU, and keeping it out of the L1 leaves room for the nodes (Cornell
@@ -88,9 +167,15 @@ __device__ __forceinline__ bool ref_hit(const float4 *refs, uint32_t k, const Ra
                c = make_float4(cv.x, cv.y, cv.z, cv.w);
   uint32_t kind = __float_as_uint(b.w);
   prim = __float_as_uint(a.w);
-  if (kind != 2u) return tripar_hit(ld3(a), ld3(b), ld3(c), kind == 1u, r, t, u, v);
+  if (!EXT) {
+    if (kind != 2u) return tripar_hit(ld3(a), ld3(b), ld3(c), kind == 1u, r, t, u, v);
+  } else {
+    if (kind < 2u) return tripar_hit(ld3(a), ld3(b), ld3(c), kind == 1u, r, t, u, v);
+    if (kind == SGR_REF_ELLIPSE) return ellipse_hit(ld3(a), ld3(b), ld3(c), r, t, u, v);
+  }
   u = 0.0f;
   v = 0.0f;
+  if (EXT && kind != 2u) return local_hit(loc + SGR_LOC_STRIDE * __float_as_uint(c.x), kind, r, t);
   if (sphere_hit(ld3(a), b.x, r.o, r.d, r.t0, t)) return t < r.t1;
   return false;
 }
@@ -331,9 +416,11 @@ __device__ __forceinline__ bool partial_pass(const SceneView &S, TravState &T, u
   if (prim < S.n_tri_like) {
     const float4 a = S.tri[3 * prim + 1], b = S.tri[3 * prim + 2];
     n = unit(cross(v3(b.x, b.y, b.z), v3(a.x, a.y, a.z)));  // e2 x e1
-  } else {
+  } else if (prim < S.n_sph_end) {
     const float4 sp = S.sph[prim - S.n_tri_like];
     n = unit((T.ray.o + T.ray.d * t) - v3(sp.x, sp.y, sp.z));
+  } else {
+    n = local_normal(S.loc + SGR_LOC_STRIDE * (prim - S.n_sph_end), T.ray, t);
   }
   const float c = dot(n, -T.ray.d);
   const float refl = fresnel(T.medium, M.ior_n, 0.0f, c);
@@ -348,7 +435,7 @@ struct Nest {
   uint2 *spill;
 };
 
-template<bool ANY, bool INST>
+template<bool ANY, bool INST, bool EXT = INST>
 __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, TraceCount &cnt,
                                           LdsStack L, uint2 *spill, Nest N);
 
@@ -358,13 +445,13 @@ __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, Trac
 // model's octree is searched (Instance::Renderable::intersect /
 // intersects, surface/instance.cc:114-146); its node calls and surface
 // tests count with the scene's, as they share the render's IsecStats.
-template<bool ANY, bool INST>
+template<bool ANY, bool INST, bool EXT>
 __device__ __forceinline__ bool ref_test(const SceneView &S, uint32_t k, const RayF &r, float &t,
                                          float &u, float &v, uint32_t &prim, uint32_t &inst,
                                          TraceCount &cnt, Nest N) {
-  if (!INST) return ref_hit(S.refs, k, r, t, u, v, prim);
+  if (!INST) return ref_hit<EXT>(S.refs, S.loc, k, r, t, u, v, prim);
   const float4 *rec = S.refs + SGR_REF_STRIDE * k;
-  if (__float_as_uint(rec[1].w) != 3u) return ref_hit(S.refs, k, r, t, u, v, prim);
+  if (__float_as_uint(rec[1].w) != 3u) return ref_hit<EXT>(S.refs, S.loc, k, r, t, u, v, prim);
   const uint32_t i = __float_as_uint(rec[0].w) - S.n_prims;
   const DevInstance &I = S.insts[i];
   TravState T2;
@@ -372,7 +459,7 @@ __device__ __forceinline__ bool ref_test(const SceneView &S, uint32_t k, const R
   T2.ray.d = xform_vec(I.w2l, r.d);
   T2.ray.t0 = r.t0;
   T2.ray.t1 = r.t1;
-  trace_one<ANY, false>(S.models[I.model], T2, cnt, N.L, N.spill, N);
+  trace_one<ANY, false, EXT>(S.models[I.model], T2, cnt, N.L, N.spill, N);
   if (T2.hit == SGR_MISS) return false;
   t = T2.ray.t1;
   u = T2.hu;
@@ -382,7 +469,7 @@ __device__ __forceinline__ bool ref_test(const SceneView &S, uint32_t k, const R
   return true;
 }
 
-template<bool ANY, bool INST>
+template<bool ANY, bool INST, bool EXT = INST>
 __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, TraceCount &cnt,
                                           LdsStack L, uint2 *spill, Nest N) {
   if (T.enter) {
@@ -393,7 +480,7 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
     for (uint32_t k = nd.z, e = nd.z + nd.w; k < e; k++) {
       cnt.tests++;
       uint32_t inst = SGR_MISS;
-      if (ref_test<ANY, INST>(S, k, T.ray, t, u, v, prim, inst, cnt, N)) {
+      if (ref_test<ANY, INST, EXT>(S, k, T.ray, t, u, v, prim, inst, cnt, N)) {
         if (ANY && INST && S.has_partial && partial_pass(S, T, prim, inst, t, cnt)) continue;
         T.hit = prim;
         T.hinst = inst;
@@ -475,7 +562,7 @@ __device__ __forceinline__ bool trav_step(const SceneView &S, TravState &T, Trac
 
 // One ray from start to end: brute force over the surface list
 // (TrivSpace, space/triv-space.h:48-58) or the octree walk.
-template<bool ANY, bool INST>
+template<bool ANY, bool INST, bool EXT>
 __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, TraceCount &cnt,
                                           LdsStack L, uint2 *spill, Nest N) {
   if (ANY && INST) {  // T.medium is the caller's
@@ -492,7 +579,7 @@ __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, Trac
     for (uint32_t q = 0; q < S.n_surfs; q++) {
       cnt.tests++;
       uint32_t inst = SGR_MISS;
-      if (ref_test<ANY, INST>(S, q, T.ray, t, u, v, prim, inst, cnt, N)) {
+      if (ref_test<ANY, INST, EXT>(S, q, T.ray, t, u, v, prim, inst, cnt, N)) {
         if (ANY && INST && S.has_partial && partial_pass(S, T, prim, inst, t, cnt)) continue;
         T.hit = prim;
         T.hinst = inst;
@@ -503,7 +590,7 @@ __device__ __forceinline__ void trace_one(const SceneView &S, TravState &T, Trac
       }
     }
   } else if (trav_init(S, T, cnt)) {
-    while (!trav_step<ANY, INST>(S, T, cnt, L, spill, N)) {
+    while (!trav_step<ANY, INST, EXT>(S, T, cnt, L, spill, N)) {
     }
   }
 }
@@ -593,7 +680,7 @@ __device__ __forceinline__ void trace_refill(const SceneView &S1, uint32_t n, ui
             uint32_t prim;
             for (uint32_t q = 0; q < S.n_surfs; q++) {
               cnt.tests++;
-              if (ref_hit(S.refs, q, T.ray, t, u, v, prim)) {
+              if (ref_hit<false>(S.refs, S.loc, q, T.ray, t, u, v, prim)) {
                 T.hit = prim;
                 if (ANY) break;
                 T.ray.t1 = t;

@@ -251,6 +251,9 @@ namespace {
 struct HostScene {
   std::vector<DevMaterial> mats;
   std::vector<float4> tri, sph, env_tex;
+  // local primitives, SGR_LOC_STRIDE float4 each: the world-to-local
+  // rows, the local-to-world rows, then (record kind, 0, 0, 0)
+  std::vector<float4> loc;
   std::vector<float4> tri_vn;  // vertex normals of mesh triangles, if any mesh has them
   std::vector<uint32_t> prim_mat, prim_mesh, order;
   std::vector<DevMesh> meshes;
@@ -395,13 +398,19 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   // ---- primitives ----
   uint64_t n_tri = 0;
   for (uint32_t i = 0; i < d->num_meshes; i++) n_tri += d->meshes[i].num_triangles;
-  uint64_t n_tl = n_tri + d->num_tripars;
-  uint64_t n_prims = n_tl + d->num_spheres;
+  if ((d->num_ellipses && !d->ellipses) || (d->num_local_prims && !d->local_prims) ||
+      (d->num_tripars && !d->tripars) || (d->num_spheres && !d->spheres) ||
+      (d->num_meshes && !d->meshes) || (nm && !d->materials))
+    return fail(SGR_ERR_INVALID, "sgr_scene_create: null array with a non-zero count");
+  uint64_t n_tl = n_tri + d->num_tripars + d->num_ellipses;
+  uint64_t n_sph_end = n_tl + d->num_spheres;
+  uint64_t n_prims = n_sph_end + d->num_local_prims;
   if (n_prims >= (1ull << 28))
     return fail(SGR_ERR_LIMIT, "sgr_scene_create: more than 2^28 primitives");
   std::vector<float4> &tri = H.tri, &sph = H.sph;
   tri.resize((size_t)n_tl * 3);
   sph.resize(d->num_spheres);
+  H.loc.assign((size_t)SGR_LOC_STRIDE * d->num_local_prims, make_float4(0.f, 0.f, 0.f, 0.f));
   std::vector<uint32_t> &prim_mat = H.prim_mat, &prim_mesh = H.prim_mesh, &order = H.order;
   prim_mat.assign(n_prims, 0);
   prim_mesh.assign(n_prims, 0xffffffffu);
@@ -417,6 +426,8 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     md.meshes = m.meshes; md.num_meshes = m.num_meshes;
     md.tripars = m.tripars; md.num_tripars = m.num_tripars;
     md.spheres = m.spheres; md.num_spheres = m.num_spheres;
+    md.ellipses = m.ellipses; md.num_ellipses = m.num_ellipses;
+    md.local_prims = m.local_prims; md.num_local_prims = m.num_local_prims;
     md.materials = d->materials; md.num_materials = nm;
     md.min_trace = d->min_trace;
     md.accel = d->accel;
@@ -532,6 +543,32 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     grow(bx.lo, bx.hi);
     order.push_back(p);
   }
+  // Ellipses (surface/ellipse.h:38-45): the bounding parallelogram,
+  // corner center - radius1 - radius2 and edges 2 radius1, 2 radius2,
+  // flagged 2 in the corner's w; box of its four corners
+  // (surface/ellipse.cc:171-181).
+  for (uint32_t i = 0; i < d->num_ellipses; i++, p++) {
+    const sgr_ellipse &el = d->ellipses[i];
+    if (el.material >= nm) return fail(SGR_ERR_INVALID, "ellipse " + std::to_string(i) + ": bad material");
+    float v0[3], e1[3], e2[3];
+    for (int k = 0; k < 3; k++) {
+      v0[k] = (el.center[k] - el.radius1[k]) - el.radius2[k];
+      e1[k] = el.radius1[k] * 2.0f;
+      e2[k] = el.radius2[k] * 2.0f;
+    }
+    tri[3 * p] = make_float4(v0[0], v0[1], v0[2], 2.0f);
+    tri[3 * p + 1] = make_float4(e1[0], e1[1], e1[2], 0.0f);
+    tri[3 * p + 2] = make_float4(e2[0], e2[1], e2[2], 0.0f);
+    prim_mat[p] = el.material;
+    Box3 &bx = boxes[p];
+    for (int k = 0; k < 3; k++) {
+      const float c1 = v0[k] + e1[k], c2 = v0[k] + e2[k], c3 = (v0[k] + e1[k]) + e2[k];
+      bx.lo[k] = fmin_t(fmin_t(fmin_t(v0[k], c1), c2), c3);
+      bx.hi[k] = fmax_t(fmax_t(fmax_t(v0[k], c1), c2), c3);
+    }
+    grow(bx.lo, bx.hi);
+    order.push_back(p);
+  }
   for (uint32_t i = 0; i < d->num_spheres; i++, p++) {
     const sgr_sphere &s = d->spheres[i];
     if (s.material >= nm) return fail(SGR_ERR_INVALID, "sphere " + std::to_string(i) + ": bad material");
@@ -541,6 +578,42 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     for (int k = 0; k < 3; k++) {
       bx.lo[k] = s.center[k] - s.radius;
       bx.hi[k] = s.center[k] + s.radius;
+    }
+    grow(bx.lo, bx.hi);
+    order.push_back(p);
+  }
+  // ---- local primitives (surface/local-primitive.h): cylinder and
+  // sphere2, a canonical shape under a transform and its inverse
+  // (geometry/local-xform.cc:17-21); box: the local cube [-1,1]^3, its 8
+  // corners taken to world space (geometry/local-xform.cc:27-31) ----
+  for (uint32_t i = 0; i < d->num_local_prims; i++, p++) {
+    const sgr_local_prim &lp = d->local_prims[i];
+    if (lp.material >= nm)
+      return fail(SGR_ERR_INVALID, "local primitive " + std::to_string(i) + ": bad material");
+    if (lp.kind != SGR_LOCAL_CYLINDER && lp.kind != SGR_LOCAL_SPHERE2)
+      return fail(SGR_ERR_INVALID, "local primitive " + std::to_string(i) + ": unknown kind");
+    float w2l[12];
+    if (!invert_affine(lp.local_to_world, w2l))
+      return fail(SGR_ERR_INVALID, "local primitive " + std::to_string(i) + ": singular transform");
+    float4 *lr = &H.loc[(size_t)SGR_LOC_STRIDE * i];
+    for (int r = 0; r < 3; r++) {
+      lr[r] = make_float4(w2l[4 * r], w2l[4 * r + 1], w2l[4 * r + 2], w2l[4 * r + 3]);
+      const float *m = lp.local_to_world + 4 * r;
+      lr[3 + r] = make_float4(m[0], m[1], m[2], m[3]);
+    }
+    uint32_t rk = lp.kind == SGR_LOCAL_CYLINDER ? SGR_REF_CYLINDER : SGR_REF_SPHERE2;
+    std::memcpy(&lr[6].x, &rk, 4);
+    prim_mat[p] = lp.material;
+    Box3 &bx = boxes[p];
+    for (int c = 0; c < 8; c++) {
+      const float corner[3] = {(c & 4) ? 1.0f : -1.0f, (c & 2) ? 1.0f : -1.0f,
+                               (c & 1) ? 1.0f : -1.0f};
+      float w[3];
+      xform_point(lp.local_to_world, corner, w);
+      for (int k = 0; k < 3; k++) {
+        bx.lo[k] = c ? fmin_t(bx.lo[k], w[k]) : w[k];
+        bx.hi[k] = c ? fmax_t(bx.hi[k], w[k]) : w[k];
+      }
     }
     grow(bx.lo, bx.hi);
     order.push_back(p);
@@ -606,6 +679,7 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   }
   V.n_tri = (uint32_t)n_tri;
   V.n_tri_like = (uint32_t)n_tl;
+  V.n_sph_end = (uint32_t)n_sph_end;
   V.n_prims = (uint32_t)n_prims;
   V.min_trace = d->min_trace;
   {  // render/scene.cc:28 horizon = bbox diameter
@@ -621,7 +695,11 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     uint32_t m = prim_mat[q];
     if (mats[m].type == SGR_MAT_GLOW) {
       DevLight L;
-      L.kind = q < n_tl ? 0 : 1;
+      // tripar 0, sphere 1; surface lights of an ellipse 3, a cylinder 4,
+      // a sphere2 5
+      if (q < n_tl) L.kind = tri[3 * q].w == 2.0f ? 3 : 0;
+      else if (q < n_sph_end) L.kind = 1;
+      else L.kind = d->local_prims[q - n_sph_end].kind == SGR_LOCAL_CYLINDER ? 4 : 5;
       L.prim = q;
       for (int c = 0; c < 3; c++) L.intensity[c] = mats[m].color[c];
       H.lights.push_back(L);
@@ -651,6 +729,8 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   }
   V.n_lights = (uint32_t)H.lights.size();
   V.n_insts = n_inst;
+  V.has_ext = (d->num_ellipses || d->num_local_prims) ? 1u : 0u;
+  for (const auto &m : H.models) V.has_ext |= m->V.has_ext;
   V.has_partial = 0;
   for (uint32_t p = 0; p < n_prims; p++)
     if (mats[prim_mat[p]].partial) V.has_partial = 1;
@@ -697,7 +777,8 @@ int sgr_octree_build(const sgr_scene_desc *d, uint32_t *nodes, uint64_t *num_nod
 // Device copies of a prepared scene's (or model's) geometry: octree
 // nodes, surface references with their geometry inline in list order (a
 // node's surfaces contiguous; kind in the second record's w: 0 triangle,
-// 1 parallelogram, 2 sphere, 3 instance), primitives and meshes.
+// 1 parallelogram, 2 sphere, 3 instance, 4 ellipse, 5 cylinder, 6
+// sphere2), primitives and meshes.
 static int upload_geometry(sgr_ctx *ctx, sgr_scene *sc, hipStream_t up, const HostScene &H,
                            const FlatOctree &oct, SceneView &V) {
   std::vector<uint4> nodes4(V.n_nodes);
@@ -713,11 +794,20 @@ static int upload_geometry(sgr_ctx *ctx, sgr_scene *sc, hipStream_t up, const Ho
     uint32_t kd = 0;
     if (q < V.n_tri_like) {
       const float4 &a = H.tri[3 * q], &b = H.tri[3 * q + 1], &c = H.tri[3 * q + 2];
-      kd = a.w != 0.0f ? 1u : 0u;
+      kd = a.w == 2.0f ? SGR_REF_ELLIPSE : a.w != 0.0f ? 1u : 0u;
       std::memcpy(&kind, &kd, 4);
       refs[SGR_REF_STRIDE * k] = make_float4(a.x, a.y, a.z, pb);
       refs[SGR_REF_STRIDE * k + 1] = make_float4(b.x, b.y, b.z, kind);
       refs[SGR_REF_STRIDE * k + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+    } else if (q >= V.n_sph_end && q < V.n_prims) {  // local primitive: its table index
+      const uint32_t li = q - V.n_sph_end;
+      std::memcpy(&kd, &H.loc[(size_t)SGR_LOC_STRIDE * li + 6].x, 4);
+      std::memcpy(&kind, &kd, 4);
+      float lb;
+      std::memcpy(&lb, &li, 4);
+      refs[SGR_REF_STRIDE * k] = make_float4(0.0f, 0.0f, 0.0f, pb);
+      refs[SGR_REF_STRIDE * k + 1] = make_float4(0.0f, 0.0f, 0.0f, kind);
+      refs[SGR_REF_STRIDE * k + 2] = make_float4(lb, 0.0f, 0.0f, 0.0f);
     } else if (q < V.n_prims) {
       const float4 &sp = H.sph[q - V.n_tri_like];
       kd = 2u;
@@ -735,6 +825,7 @@ static int upload_geometry(sgr_ctx *ctx, sgr_scene *sc, hipStream_t up, const Ho
   if (!rc) rc = upload(ctx, sc, up, refs, &V.refs);
   if (!rc) rc = upload(ctx, sc, up, H.tri, &V.tri);
   if (!rc) rc = upload(ctx, sc, up, H.sph, &V.sph);
+  if (!rc && !H.loc.empty()) rc = upload(ctx, sc, up, H.loc, &V.loc);
   if (!rc && !H.tri_vn.empty()) rc = upload(ctx, sc, up, H.tri_vn, &V.tri_vn);
   if (!rc) rc = upload(ctx, sc, up, H.prim_mat, &V.prim_mat);
   if (!rc) rc = upload(ctx, sc, up, H.prim_mesh, &V.prim_mesh);
@@ -870,6 +961,35 @@ int sgr_scene_bounds(const sgr_scene *sc, float *lo, float *hi, float *horizon) 
     if (hi) hi[a] = sc->bbox_hi[a];
   }
   if (horizon) *horizon = sc->view.horizon;
+  return SGR_OK;
+}
+
+// Cylinder::xform (surface/cylinder.cc:24-36): translate (0, 0, 1), scale
+// (radius, radius, |axis| / 2), to_basis (p, p x az, az) with az =
+// unit(axis) and p = az.perpendicular() (geometry/vec.h:119-129, not
+// normalised), translate base; each step composes on the left
+// (geometry/xform.h:189-262), so the linear columns are radius p,
+// radius (p x az), (|axis| / 2) az and the translation (|axis| / 2) az
+// + base.
+int sgr_cylinder_xform(const float base[3], const float axis[3], float radius, float out[12]) {
+  if (!base || !axis || !out) return fail(SGR_ERR_INVALID, "sgr_cylinder_xform: null argument");
+  const float len = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+  if (!(len > 0.0f) || !std::isfinite(len))
+    return fail(SGR_ERR_INVALID, "sgr_cylinder_xform: the axis must be non-zero and finite");
+  const float inv = 1.0f / len;  // geometry/vec.h:113-117 unit
+  const float az[3] = {axis[0] * inv, axis[1] * inv, axis[2] * inv};
+  const float nz = (az[0] == az[2]) ? az[2] : -az[2];
+  const float ax[3] = {az[1] * az[1] + az[0] * nz, -nz * nz - az[0] * az[1],
+                       az[0] * az[0] - az[1] * nz};
+  const float zy[3] = {ax[1] * az[2] - ax[2] * az[1], ax[2] * az[0] - ax[0] * az[2],
+                       ax[0] * az[1] - ax[1] * az[0]};
+  const float h = len / 2.0f;
+  for (int r = 0; r < 3; r++) {
+    out[4 * r] = ax[r] * radius;
+    out[4 * r + 1] = zy[r] * radius;
+    out[4 * r + 2] = az[r] * h;
+    out[4 * r + 3] = az[r] * h + base[r];
+  }
   return SGR_OK;
 }
 

@@ -176,6 +176,7 @@ void enqueue_step(sgr_ctx *ctx, Engine *E, uint32_t maxp, bool refill, bool inst
   // plain walk with a stack for the models
   S0.walk_refill = refill && !inst ? 1u : 0u;
   S0.n_insts = inst ? 1u : 0u;
+  S0.has_ext = inst ? 1u : 0u;  // and the shading of ellipses, cylinders and sphere2s
   // ... and shadow rays of steps with partial occluders the walk that
   // carries their transmittance
   S0.has_partial = partial ? 1u : 0u;
@@ -370,7 +371,7 @@ static int sgr_engine_step_impl(sgr_ctx *ctx, sgr_step_info *info) {
   size_t K = 1;
   uint32_t maxp = 0;
   bool refill = false;  // any scene of the step deep enough for lane refill
-  bool inst = false;    // any scene of the step with instances
+  bool inst = false;    // any scene of the step with instances, ellipses or local primitives
   bool partial = false; // any scene of the step with partial occluders
   size_t n = E->active.size();
   size_t stop = n;
@@ -389,7 +390,7 @@ static int sgr_engine_step_impl(sgr_ctx *ctx, sgr_step_info *info) {
     K = std::max<size_t>(K, j.pl.R.K);
     maxp = std::max(maxp, j.pl.R.maxp);
     refill = refill || j.sc->view.walk_refill;
-    inst = inst || j.sc->view.n_insts > 0;
+    inst = inst || j.sc->view.n_insts > 0 || j.sc->view.has_ext;
     partial = partial || j.sc->view.has_partial;
   }
   if (run.empty()) return SGR_OK;

@@ -279,7 +279,8 @@ __device__ __forceinline__ void put_shadow(const PathBufs &P, size_t slot, C3 c,
   P.sc[0][slot] = c.r; P.sc[1][slot] = c.g; P.sc[2][slot] = c.b;
 }
 
-template<bool MULTI, bool GUIDED, typename Emit>
+// EXT: the scenes may hold ellipses and local primitives.
+template<bool MULTI, bool GUIDED, bool EXT, typename Emit>
 __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView &R1,
                                            const StepView &J, const PathBufs &P,
                                            const GuideView &G, uint32_t p, uint32_t j,
@@ -322,7 +323,7 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
     // barycentrics only matter for interpolated vertex normals
     const float hu = S.tri_vn ? P.hu[j] : 0.0f, hv = S.tri_vn ? P.hv[j] : 0.0f;
     const uint32_t hinst = S.n_insts ? P.hinst[j] : SGR_MISS;
-    make_isec(S, prim, hinst, ray, hu, hv, mtop, mencl, is);
+    make_isec<EXT>(S, prim, hinst, ray, hu, hv, mtop, mencl, is);
     const DevMaterial &HM = S.mats[is.mat];
     if (include && HM.type == 5 && !is.back) rad = rad + cload(HM.color) * T;
     if (is.bmat < 0) {
@@ -359,11 +360,11 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
             P.svis[slot_a] = 0;
             P.svis[slot_b] = 0;
             // light sample, then BSDF sample (render/direct-illum.cc:147-252)
-            direct_light_term(S, L, is, M, lu, lv, extra, [&](C3 c, const RayF &r) {
+            direct_light_term<EXT>(S, L, is, M, lu, lv, extra, [&](C3 c, const RayF &r) {
               put_shadow(P, slot_a, c, r, is.media_ior);
               emit(slot_a, dir_octant(r.d));
             });
-            direct_bsdf_term(S, L, is, M, bu, bv, [&](C3 c, const RayF &r) {
+            direct_bsdf_term<EXT>(S, L, is, M, bu, bv, [&](C3 c, const RayF &r) {
               put_shadow(P, slot_b, c, r, is.media_ior);
               emit(slot_b, dir_octant(r.d));
             });
@@ -471,7 +472,7 @@ __device__ __forceinline__ bool shade_path(const SceneView &S1, const RenderView
   return cont;
 }
 
-template<bool MULTI, bool GUIDED>
+template<bool MULTI, bool GUIDED, bool EXT>
 __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, StepView J, PathBufs P,
                                                uint32_t b, unsigned long long *stats,
                                                GuideView G) {
@@ -486,7 +487,7 @@ __global__ void __launch_bounds__(256) k_shade(SceneView S1, RenderView R1, Step
     const uint32_t p = q[j];
     uint8_t *sf = P.sflag + (size_t)j * K;
     for (size_t k = 0; k < K; k++) sf[k] = 0;
-    const bool cont = shade_path<MULTI, GUIDED>(S1, R1, J, P, G, p, j, illum,
+    const bool cont = shade_path<MULTI, GUIDED, EXT>(S1, R1, J, P, G, p, j, illum,
                                                 [&](size_t slot, uint32_t oct) {
       sf[slot - (size_t)p * K] = (uint8_t)(1u | oct << 1);
     });
@@ -778,7 +779,7 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
       P.hv[j] = T.hv;
       if (INST) P.hinst[j] = T.hinst;
       uint64_t em = 0;  // shadow slots of the path written at this vertex
-      const bool cont = shade_path<MULTI, false>(S1, R1, J, P, G, p, j, illum,
+      const bool cont = shade_path<MULTI, false, INST>(S1, R1, J, P, G, p, j, illum,
                                                  [&](size_t slot, uint32_t) {
         em |= 1ull << (uint32_t)(slot - (size_t)p * P.K);
       });
@@ -896,7 +897,7 @@ __global__ void __launch_bounds__(256) k_direct_integ(SceneView S, RenderView R,
           RayF hr = cur.ray;
           hr.t1 = T.ray.t1;
           Isec is;
-          make_isec(S, T.hit, T.hinst, hr, S.tri_vn ? T.hu : 0.0f, S.tri_vn ? T.hv : 0.0f,
+          make_isec<INST>(S, T.hit, T.hinst, hr, S.tri_vn ? T.hu : 0.0f, S.tri_vn ? T.hv : 0.0f,
                     cur.m0, cur.m1, is);
           const DevMaterial &HM = S.mats[is.mat];
           if (HM.type == 5 && !is.back) total = total + cload(HM.color) * cur.W;
@@ -921,8 +922,8 @@ __global__ void __launch_bounds__(256) k_direct_integ(SceneView S, RenderView R,
                   shadow++;
                   if (U.hit == SGR_MISS) r = r + ((INST && S.has_partial) ? c * U.trans : c);
                 };
-                direct_light_term(S, L, is, M, lu, lv, extra, visible);
-                direct_bsdf_term(S, L, is, M, bu, bv, visible);
+                direct_light_term<INST>(S, L, is, M, lu, lv, extra, visible);
+                direct_bsdf_term<INST>(S, L, is, M, bu, bv, visible);
                 lrad = lrad + r;
               }
               D = D + lrad * (1.0f / (float)nsub);  // Color / n: times 1 / n
@@ -1134,11 +1135,12 @@ static inline unsigned grid_for(uint32_t n, unsigned cap) {
   return g < cap ? g : cap;
 }
 
-// Walks whose scenes hold instances, or (any-hit walks) partial
-// occluders, take the INST kernels: the plain walk with the second stack
-// for the models and the shadow rays' transmittance.
+// Walks whose scenes hold instances, ellipses or local primitives, or
+// (any-hit walks) partial occluders, take the INST kernels: the plain
+// walk with the second stack for the models, the tests of the rarer
+// surfaces and the shadow rays' transmittance.
 static inline bool general_walk(const SceneView &S, bool any) {
-  return S.n_insts || (any && S.has_partial);
+  return S.n_insts || S.has_ext || (any && S.has_partial);
 }
 
 void launch_trace(const SceneView &S, const RayBatchDev &rb, uint32_t n, bool any,
@@ -1185,9 +1187,10 @@ void launch_extend(bool multi, const SceneView &S, const StepView &J, const Path
   // directions are random, the plain walk loses 3-9 %.
   const bool refill = S.walk_refill && b > 0;
   if (!S.walk_refill) grid = plain_walk_grid(grid);
-  if (S.n_insts && multi)  // instanced scenes take the plain walk
+  const bool gen = general_walk(S, false);  // the plain walk
+  if (gen && multi)
     hipLaunchKernelGGL((k_extend<true, false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
-  else if (S.n_insts)
+  else if (gen)
     hipLaunchKernelGGL((k_extend<false, false, true>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
   else if (multi && refill)
     hipLaunchKernelGGL((k_extend<true, true, false>), dim3(grid), dim3(256), 0, st, S, J, P, b, stats);
@@ -1223,12 +1226,18 @@ void launch_shade(bool multi, const SceneView &S, const RenderView &R, const Ste
   grid = grid < 2u * s_num_cu ? grid : 2u * s_num_cu;
   PathBufs Pk = P;
   Pk.media_ovf = stats + SGR_STAT_MEDIA_OVF;
-  if (multi)
-    hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
+  if (multi && S.has_ext)
+    hipLaunchKernelGGL((k_shade<true, false, true>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
+  else if (multi)
+    hipLaunchKernelGGL((k_shade<true, false, false>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
+  else if (G && S.has_ext)
+    hipLaunchKernelGGL((k_shade<false, true, true>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
   else if (G)
-    hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
+    hipLaunchKernelGGL((k_shade<false, true, false>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
+  else if (S.has_ext)
+    hipLaunchKernelGGL((k_shade<false, false, true>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
   else
-    hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
+    hipLaunchKernelGGL((k_shade<false, false, false>), dim3(grid), dim3(256), 0, st, S, R, J, Pk, b, stats, g);
   // the next ray queue and this bounce's shadow queue, in order
   const unsigned nblk = (P.n + SGR_COMPACT_ITEMS - 1) / SGR_COMPACT_ITEMS;
   hipLaunchKernelGGL(k_compact_count, dim3(nblk), dim3(256), 0, st, P, b);

@@ -79,6 +79,7 @@ def lib():
                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_int)]
     L.sgr_mesh_file_copy.argtypes = [vp, abi.f32p, abi.u32p, abi.f32p]
+    L.sgr_cylinder_xform.argtypes = [abi.f32p, abi.f32p, C.c_float, abi.f32p]
     L.sgr_mesh_file_close.argtypes = [vp]
     L.sgr_render_noalloc.argtypes = [vp, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams),
                                      abi.f32p, C.POINTER(abi.Stats)]
@@ -467,6 +468,17 @@ def recover_image(path, accum, fmt=None):
     check(lib().sgr_image_recover(str(path).encode(), _fmt(fmt), abi.fptr(accum), w, h,
                                   C.byref(rows)), "sgr_image_recover")
     return rows.value
+
+
+def cylinder_xform(base, axis, radius):
+    """3x4 local-to-world transform of the cylinder with BASE, AXIS and
+    RADIUS (Cylinder::xform, surface/cylinder.cc:24-36)."""
+    b = np.ascontiguousarray(base, np.float32)
+    a = np.ascontiguousarray(axis, np.float32)
+    out = np.zeros(12, np.float32)
+    check(lib().sgr_cylinder_xform(abi.fptr(b), abi.fptr(a), float(radius), abi.fptr(out)),
+          "sgr_cylinder_xform")
+    return out.reshape(3, 4)
 
 
 def load_mesh(path, fmt=None):

@@ -37,6 +37,8 @@ class SceneDesc:
         self.meshes = []
         self.tripars = []
         self.spheres = []
+        self.ellipses = []
+        self.local_prims = []  # (SGR_LOCAL_* kind, 3x4 local-to-world, material)
         self.envmaps = []
         self.models = []       # SceneDesc objects holding geometry only
         self.instances = []    # (model index, 3x4 local-to-world)
@@ -128,6 +130,29 @@ class SceneDesc:
                                  material=material))
         self._c = None
 
+    def ellipse(self, center, radius1, radius2, material):
+        """Ellipse (surface/ellipse.h:38-45) at CENTER with radius vectors
+        RADIUS1 and RADIUS2."""
+        self.ellipses.append(dict(center=center, r1=radius1, r2=radius2,
+                                  material=material))
+        self._c = None
+
+    def _local(self, kind, local_to_world, material):
+        x = np.asarray(local_to_world, np.float64)[:3, :4].astype(np.float32)
+        self.local_prims.append((kind, x, int(material)))
+        self._c = None
+
+    def cylinder(self, local_to_world, material):
+        """Cylinder (surface/cylinder.h:24-33): the canonical cylinder
+        (radius 1 about z, z in [-1, 1], no caps) placed by a 3x4 transform;
+        cylinder_xform gives the transform of a base, axis and radius."""
+        self._local(abi.SGR_LOCAL_CYLINDER, local_to_world, material)
+
+    def sphere2(self, local_to_world, material):
+        """Sphere2 (surface/sphere2.h:24-33): the unit sphere placed by a
+        3x4 transform."""
+        self._local(abi.SGR_LOCAL_SPHERE2, local_to_world, material)
+
     def envmap(self, texels, frame=Y_UP_ENV_FRAME):
         t = np.ascontiguousarray(texels, dtype=np.float32)
         assert t.ndim == 3 and t.shape[2] == 3
@@ -141,7 +166,8 @@ class SceneDesc:
 
     @property
     def num_prims(self):
-        return self.num_triangles + len(self.tripars) + len(self.spheres)
+        return (self.num_triangles + len(self.tripars) + len(self.ellipses) +
+                len(self.spheres) + len(self.local_prims))
 
     # --- C descriptor ----------------------------------------------------
     def _geometry(self):
@@ -170,7 +196,18 @@ class SceneDesc:
             sps[i].center[:] = [float(x) for x in s["center"]]
             sps[i].radius = s["radius"]
             sps[i].material = s["material"]
-        return meshes, tps, sps
+        els = (abi.Ellipse * max(1, len(self.ellipses)))()
+        for i, e in enumerate(self.ellipses):
+            els[i].center[:] = [float(x) for x in e["center"]]
+            els[i].radius1[:] = [float(x) for x in e["r1"]]
+            els[i].radius2[:] = [float(x) for x in e["r2"]]
+            els[i].material = e["material"]
+        lps = (abi.LocalPrim * max(1, len(self.local_prims)))()
+        for i, (kind, x, mat) in enumerate(self.local_prims):
+            lps[i].kind = kind
+            lps[i].local_to_world[:] = [float(v) for v in x.ravel()]
+            lps[i].material = mat
+        return meshes, tps, sps, els, lps
 
     def c(self):
         if self._c is not None:
@@ -182,7 +219,7 @@ class SceneDesc:
             mats[i].gloss_color[:] = [float(x) for x in m["gloss_color"]]
             mats[i].m, mats[i].ior_n, mats[i].ior_k = m["m"], m["ior_n"], m["ior_k"]
             mats[i].underlying = m["underlying"]
-        meshes, tps, sps = self._geometry()
+        meshes, tps, sps, els, lps = self._geometry()
         envs = (abi.Envmap * max(1, len(self.envmaps)))()
         for i, e in enumerate(self.envmaps):
             envs[i].texels = abi.fptr(e["tex"])
@@ -192,6 +229,8 @@ class SceneDesc:
         d.meshes, d.num_meshes = meshes, len(self.meshes)
         d.tripars, d.num_tripars = tps, len(self.tripars)
         d.spheres, d.num_spheres = sps, len(self.spheres)
+        d.ellipses, d.num_ellipses = els, len(self.ellipses)
+        d.local_prims, d.num_local_prims = lps, len(self.local_prims)
         d.materials, d.num_materials = mats, len(self.materials)
         d.envmaps, d.num_envmaps = envs, len(self.envmaps)
         d.min_trace, d.accel = self.min_trace, self.accel
@@ -203,6 +242,8 @@ class SceneDesc:
             models[i].meshes, models[i].num_meshes = g[0], len(m.meshes)
             models[i].tripars, models[i].num_tripars = g[1], len(m.tripars)
             models[i].spheres, models[i].num_spheres = g[2], len(m.spheres)
+            models[i].ellipses, models[i].num_ellipses = g[3], len(m.ellipses)
+            models[i].local_prims, models[i].num_local_prims = g[4], len(m.local_prims)
         insts = (abi.Instance * max(1, len(self.instances)))()
         for i, (mi, x) in enumerate(self.instances):
             insts[i].model = mi
@@ -211,7 +252,7 @@ class SceneDesc:
         d.num_models = len(self.models)
         d.instances = C.cast(insts, C.c_void_p) if self.instances else None
         d.num_instances = len(self.instances)
-        self._keep = (mats, meshes, tps, sps, envs, models, keep_models, insts)
+        self._keep = (mats, meshes, tps, sps, els, lps, envs, models, keep_models, insts)
         self._c = d
         return d
 
