@@ -62,6 +62,7 @@ extern "C" {
 
 typedef struct sgr_ctx sgr_ctx;
 typedef struct sgr_scene sgr_scene;
+typedef struct sgr_photon_map sgr_photon_map;
 
 /* One material.  Field meaning per kind:
  *   LAMBERT        color = reflectance
@@ -297,10 +298,19 @@ typedef struct sgr_render_params {
      * engine and guided renders refuse it. */
     uint32_t integrator;           /* SGR_INTEG_* */
     uint32_t light_samples;
+    /* SGR_INTEG_PHOTON: the photon maps of this scene the render reads
+     * (sgr_photon_map_create); NULL otherwise */
+    const sgr_photon_map *photon_map;
 } sgr_render_params;
 
 #define SGR_INTEG_PATH    0u
 #define SGR_INTEG_DIRECT  1u
+/* Photon mapping (render/photon-integ.cc): the direct integrator's
+ * recursion whose Lo adds, besides direct lighting, caustic photons and
+ * final gathering over the photon maps params->photon_map holds.  Runs in
+ * sgr_render / sgr_render_samples; the job engine, guided and no-alloc
+ * renders refuse it. */
+#define SGR_INTEG_PHOTON  2u
 /* recursion levels the direct integrator keeps per sample; deeper
  * specular branches (reached with probability 2^-26 or less through
  * its roulette) are dropped and counted in sgr_stats.direct_overflows */
@@ -453,6 +463,70 @@ int sgr_render_noalloc(sgr_scene *scene, const sgr_camera *cam,
  * clamp colour at 0, un-premultiply, exposure, contrast. */
 int sgr_finalize_image(const float *accum, uint32_t width, uint32_t height,
                        float exposure, float contrast, float *rgba);
+
+/* ---- photon maps (render/photon-integ.cc:36-242) -----------------------
+ * Photons are shot from the scene's lights on the device
+ * (PhotonShooter::shoot, photon/photon-shooter.cc:29-191): path n picks
+ * light floor(radical_inverse(n, 11) * lights) and its free sample at
+ * (radical_inverse(n, 2), (n, 3)) / ((n, 5), (n, 7)), scatters by BSDF
+ * samples ((n, 13), (n, 17) at the first hit, then the path's own
+ * counter stream seeded by `seed`; Russian roulette 1/2 past 3 bounces),
+ * and deposits a photon at every hit whose BSDF has a non-specular part:
+ * into the direct set before any scattering, the caustic set after
+ * specular scattering only, else the indirect set, each set until its
+ * target count (PhotonInteg::Shooter::deposit, render/photon-integ.cc:
+ * 196-226).  The photons of each set are then put in a left-balanced
+ * kd-tree (photon/photon-map.cc:22-189).  Parameter defaults are the
+ * reference's (render/photon-integ.cc:36-148); a count of
+ * SGR_PHOTON_UNSPEC means "not given". */
+#define SGR_PHOTON_UNSPEC 0xffffffffu
+typedef struct sgr_photon_params {
+    uint32_t caustic_photons;     /* "caustic_photons": 50000 */
+    uint32_t direct_photons;      /* "direct_photons": 500000; none are
+                                     shot with direct lighting on and
+                                     final gathering off */
+    uint32_t indirect_photons;    /* "indirect_photons": 500000 */
+    uint32_t use_photons;         /* photons per estimate, "use_photons": 50 */
+    float    photon_radius;       /* search radius, "photon_radius": 0.1 */
+    uint32_t final_gather;        /* "final_gather": 1 */
+    uint32_t fg_samples;          /* "final_gather_samples" (UNSPEC) */
+    uint32_t fg_bsdf_samples;     /* "final_gather_bsdf_samples" (UNSPEC) */
+    uint32_t fg_photon_samples;   /* "final_gather_photon_samples" (UNSPEC) */
+    uint32_t direct_illum;        /* "direct_illum": 1 (else the direct map) */
+    uint32_t direct_samples;      /* "direct_samples": 16 per light */
+    uint64_t seed;                /* stream of the photon paths */
+} sgr_photon_params;
+
+/* The parameters with the reference's defaults filled in. */
+void sgr_photon_params_default(sgr_photon_params *params);
+
+typedef struct sgr_photon_map_info {
+    uint64_t photons[3];          /* direct, caustic, indirect set sizes */
+    uint64_t paths[3];            /* PhotonSet::num_paths of each set */
+    float    scale[3];            /* 1 / paths (0 for an empty set) */
+    uint64_t paths_shot;          /* path numbers taken (light samples) */
+    uint64_t deposit_overflows;   /* hits past SGR_PHOTON_MAX_DEPOSITS on
+                                     one path, not deposited */
+    uint32_t fg_samples, fg_bsdf_samples, fg_photon_samples;  /* resolved */
+    double   ms_shoot;            /* device time of the shooting */
+    double   ms_build;            /* host kd-tree build and upload */
+} sgr_photon_map_info;
+#define SGR_PHOTON_MAX_DEPOSITS 32u
+/* hits a photon path takes at most (the reference has no bound; past 3
+ * bounces its roulette ends a path with probability 1/2 at each) */
+#define SGR_PHOTON_MAX_BOUNCES 256u
+
+/* Shoot the photons of a scene and build its maps
+ * (PhotonInteg::GlobalState::generate_photons). */
+int sgr_photon_map_create(sgr_scene *scene, const sgr_photon_params *params,
+                          sgr_photon_map **out);
+int sgr_photon_map_info_get(const sgr_photon_map *map, sgr_photon_map_info *info);
+/* Photons of set (0 direct, 1 caustic, 2 indirect) in kd-tree order:
+ * per photon pos xyz + split axis (4 floats), power rgb, dir xyz (the
+ * direction it came from); any output may be NULL. */
+int sgr_photon_map_export(const sgr_photon_map *map, int set, float *pos_axis,
+                          float *power, float *dir);
+int sgr_photon_map_destroy(sgr_photon_map *map);
 
 /* ---- image files (host only) ------------------------------------------
  * The formats the reference always supports (image/image-dispatch.cc:

@@ -70,6 +70,21 @@ def lib(fast=False):
         L.orc_isec_probe.argtypes = [C.c_void_p, abi.f32p, C.c_int, C.POINTER(C.c_double)]
         L.orc_light_probe.argtypes = [C.c_void_p, C.c_uint, C.c_int, C.POINTER(C.c_double),
                                       C.c_int, C.POINTER(C.c_double)]
+        L.orc_photon_map_create.restype = C.c_void_p
+        L.orc_photon_map_create.argtypes = [C.c_void_p, C.POINTER(abi.PhotonParams), C.c_char_p,
+                                            C.c_int]
+        L.orc_photon_map_import.restype = C.c_void_p
+        L.orc_photon_map_import.argtypes = [C.POINTER(abi.PhotonParams)] + \
+            [C.POINTER(abi.f32p)] * 3 + [C.POINTER(C.c_uint64)] * 2
+        L.orc_photon_map_info.argtypes = [C.c_void_p, C.POINTER(abi.PhotonMapInfo)]
+        L.orc_photon_map_export.argtypes = [C.c_void_p, C.c_int, abi.f32p, abi.f32p, abi.f32p]
+        L.orc_photon_map_destroy.argtypes = [C.c_void_p]
+        L.orc_radical_inverse.restype = C.c_double
+        L.orc_radical_inverse.argtypes = [C.c_uint, C.c_uint]
+        L.orc_kd_probe.argtypes = [abi.f32p, C.c_int, abi.f32p, C.c_int, C.c_uint, C.c_float,
+                                   abi.f32p, C.POINTER(C.c_int), C.POINTER(C.c_int), abi.f32p]
+        L.orc_dirhist_probe.argtypes = [abi.f32p, abi.f32p, C.c_int, abi.f32p, C.c_int,
+                                        abi.f32p, abi.f32p]
         L.orc_permute.restype = C.c_uint32
         L.orc_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
         _libs[fast] = L
@@ -170,6 +185,10 @@ class OracleScene:
             raise RuntimeError("light probe: no such light")
         return {"val": out[:, 0:3], "pdf": out[:, 3], "dist": out[:, 4], "dir": out[:, 5:8]}
 
+    def photon_map(self, params=None):
+        """The oracle's own photon maps of this scene."""
+        return OraclePhotonMap(self, params if params is not None else abi.photon_params())
+
     def render(self, cam, params, nthreads=8, samples=False):
         acc = np.zeros((params.height, params.width, 5), np.float64)
         x0, y0, w, h, m = abi.render_window(params)
@@ -243,3 +262,86 @@ def stream_probe(seed, px, py, chan, spp, s=0, nr=0):
 
 def permute(i, n, key):
     return lib().orc_permute(i, n, key)
+
+
+class OraclePhotonMap:
+    """Photon maps shot and built by the oracle, or (import_arrays) taken
+    from another builder's kd-ordered export; pass it to
+    abi.make_params(photon_map=...) for an oracle render."""
+
+    def __init__(self, scene=None, params=None, arrays=None, paths=None):
+        self.L = lib()
+        self.params = params
+        err = C.create_string_buffer(512)
+        if scene is not None:
+            self.h = self.L.orc_photon_map_create(scene.h, C.byref(params), err, 512)
+            if not self.h:
+                raise RuntimeError(err.value.decode())
+        else:
+            self._keep = [[np.ascontiguousarray(a, np.float32) for a in st] for st in arrays]
+            pa = (abi.f32p * 3)(*[abi.fptr(st[0]) for st in self._keep])
+            pw = (abi.f32p * 3)(*[abi.fptr(st[1]) for st in self._keep])
+            dr = (abi.f32p * 3)(*[abi.fptr(st[2]) for st in self._keep])
+            cnt = (C.c_uint64 * 3)(*[len(st[0]) for st in self._keep])
+            pth = (C.c_uint64 * 3)(*[int(x) for x in paths])
+            self.h = self.L.orc_photon_map_import(C.byref(params), pa, pw, dr, cnt, pth)
+
+    @classmethod
+    def import_arrays(cls, params, arrays, paths):
+        """ARRAYS: per set (pos_axis (n, 4), power (n, 3), dir (n, 3)) in
+        kd-tree order; PATHS: each set's path count."""
+        return cls(None, params, arrays, paths)
+
+    def handle(self):
+        return self.h
+
+    def info(self):
+        i = abi.PhotonMapInfo()
+        self.L.orc_photon_map_info(self.h, C.byref(i))
+        return i.as_dict()
+
+    def export(self, which):
+        n = self.info()["photons"][which]
+        pa = np.zeros((n, 4), np.float32)
+        pw = np.zeros((n, 3), np.float32)
+        dr = np.zeros((n, 3), np.float32)
+        self.L.orc_photon_map_export(self.h, which, abi.fptr(pa), abi.fptr(pw), abi.fptr(dr))
+        return pa, pw, dr
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_photon_map_destroy(self.h)
+            self.h = None
+
+
+def radical_inverse(num, base):
+    return lib().orc_radical_inverse(num, base)
+
+
+def kd_probe(points, queries, k, r2):
+    """The oracle's photon kd-tree over POINTS and its k-nearest search
+    from QUERIES: (heap-order positions, split axes, found kd indices
+    (m, k; -1 padded), final squared radii)."""
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    q = np.ascontiguousarray(queries, np.float32).reshape(-1, 3)
+    kp = np.zeros_like(p)
+    ax = np.zeros(len(p), np.int32)
+    found = np.zeros((len(q), k), np.int32)
+    r = np.zeros(len(q), np.float32)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    lib().orc_kd_probe(abi.fptr(p), len(p), abi.fptr(q), len(q), k, r2, abi.fptr(kp), ip(ax),
+                       ip(found), abi.fptr(r))
+    return kp, ax, found, r
+
+
+def dirhist_probe(dirs, weights, params):
+    """Direction histogram of DIRS with WEIGHTS: samples at PARAMS (n, 5:
+    direction, pdf, pdf() at it) and the 64 bin pdfs."""
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    w = np.ascontiguousarray(weights, np.float32)
+    pr = np.ascontiguousarray(params, np.float32).reshape(-1, 2)
+    out = np.zeros((len(pr), 5), np.float32)
+    bins = np.zeros(64, np.float32)
+    lib().orc_dirhist_probe(abi.fptr(d), abi.fptr(w), len(d), abi.fptr(pr), len(pr),
+                            abi.fptr(out), abi.fptr(bins))
+    return out, bins

@@ -8,6 +8,7 @@
 // C entry points for ctypes; every call catches exceptions and returns a
 // status, the message is copied into the caller's buffer.
 #include "orc_render.h"
+#include "orc_photon.h"
 #include <atomic>
 #include <thread>
 #include <memory>
@@ -83,15 +84,194 @@ struct Stream {
   static uint32_t dim(uint32_t b, uint32_t slot) { return (b << 20) | slot; }
 };
 
+// The photon maps of one scene and the photon integrator's resolved
+// parameters (PhotonInteg::GlobalState, render/photon-integ.cc:36-148).
+template<class R> struct PhotonMaps {
+  PhotonMap<R> map[3];  // 0 direct, 1 caustic, 2 indirect
+  float scale[3] = {0, 0, 0};
+  uint64_t paths[3] = {0, 0, 0};
+  uint64_t paths_shot = 0, overflows = 0;
+  unsigned k = 50;
+  R radius_sq = R(0.01f);
+  unsigned fg = 16, fg_bsdf = 8, fg_photon = 8;
+  bool direct_illum = true;
+  unsigned direct_samples = 16;
+};
+
+struct OrcPhotonMap {
+  int fp64 = 0;
+  PhotonMaps<float> f;
+  PhotonMaps<double> d;
+  PhotonMaps<float> &get(float *) { return f; }
+  PhotonMaps<double> &get(double *) { return d; }
+};
+
+// render/photon-integ.cc:36-148: the final-gathering sample split and
+// whether direct photons are needed.  TARGETS: direct, caustic,
+// indirect photon counts.
+static void resolve_photon_params(const sgr_photon_params &p, unsigned &fg, unsigned &fg_bsdf,
+                                  unsigned &fg_photon, unsigned targets[3]) {
+  const unsigned U = SGR_PHOTON_UNSPEC;
+  fg = p.fg_samples;
+  fg_bsdf = p.fg_bsdf_samples;
+  fg_photon = p.fg_photon_samples;
+  if (p.final_gather) {
+    if (fg == U && fg_bsdf == U && fg_photon == U) {
+      fg = 16;
+      fg_bsdf = 8;
+      fg_photon = 8;
+    } else if (fg == U) {
+      if (fg_photon == U) fg_photon = 0;
+      if (fg_bsdf == U) fg_bsdf = 0;
+      fg = fg_bsdf + fg_photon;
+    } else if (fg_bsdf == U || fg_photon == U) {
+      if (fg_bsdf == U && fg_photon == U) {
+        if (fg == 1) {
+          fg_bsdf = fg_photon = 0;
+        } else {
+          fg_photon = fg / 2;
+          fg_bsdf = (fg + 1) / 2;
+        }
+      } else if (fg_bsdf == U) {
+        fg_photon = fg - fg_bsdf;  // the reference's own mix-up, kept
+      } else {
+        fg_bsdf = fg - fg_photon;
+      }
+    }
+  } else {
+    fg = fg_bsdf = fg_photon = 0;
+  }
+  targets[0] = (p.direct_illum && fg == 0) ? 0u : p.direct_photons;
+  targets[1] = p.caustic_photons;
+  targets[2] = p.indirect_photons;
+}
+
+// PhotonShooter::shoot (photon/photon-shooter.cc:29-191) with
+// PhotonInteg::Shooter::deposit (render/photon-integ.cc:196-226).  Path
+// N's light, light parameters and first BSDF parameters come from radical
+// inverses of N; its later draws come from the path's own counter stream
+// (key path_key(pixel_key(seed, -2, -2), N)): two BSDF parameters then,
+// past 3 bounces, the roulette draw, per bounce.  A path deposits at
+// most SGR_PHOTON_MAX_DEPOSITS photons (later hits are counted) and ends
+// after SGR_PHOTON_MAX_BOUNCES hits, as the device's do.
+template<class R>
+static void shoot_photons(const Scene<R> &sc, const sgr_photon_params &pp,
+                          const unsigned targets[3], PhotonMaps<R> &out) {
+  std::vector<Photon<R>> sets[3];
+  uint64_t num_paths[3] = {0, 0, 0};
+  const unsigned nl = (unsigned)sc.lights.size();
+  auto complete = [&](int i) { return sets[i].size() == targets[i]; };
+  auto all_complete = [&]() { return complete(0) && complete(1) && complete(2); };
+  const uint64_t pk = pixel_key(pp.seed, -2, -2);
+  unsigned path = 0;
+  if (nl != 0) {
+    for (; !all_complete(); path++) {
+      const unsigned li = unsigned(radical_inverse(path, 11) * nl);
+      const Light<R> &L = sc.lights[li];
+      FreeSample<R> samp = sc.light_free_sample(
+          L, R(float(radical_inverse(path, 2))), R(float(radical_inverse(path, 3))),
+          R(float(radical_inverse(path, 5))), R(float(radical_inverse(path, 7))));
+      if (samp.val.is_zero() || samp.pdf == 0) continue;
+      for (int i = 0; i < 3; i++)
+        if (!complete(i)) num_paths[i]++;
+      const uint64_t qk = path_key(pk, path);
+      uint32_t nd = 0;
+      unsigned history = 0, deposits = 0;
+      R media[8];
+      int depth = 1;
+      media[0] = 1;
+      V3<R> pos = samp.pos, dir = samp.dir;
+      C3<R> power = samp.val * R(float(nl)) / samp.pdf;
+      for (unsigned path_len = 0; path_len < SGR_PHOTON_MAX_BOUNCES; path_len++) {
+        Ray<R> ray;
+        ray.origin = pos;
+        ray.dir = dir;
+        ray.t0 = sc.min_trace;
+        ray.t1 = sc.horizon;
+        R hu, hv;
+        unsigned inst = 0xffffffffu;
+        const unsigned prim = sc.accel.closest(ray, hu, hv, nullptr, nullptr, &inst);
+        if (prim == 0xffffffffu) break;
+        Isec<R> is;
+        const R enclosing = depth >= 2 ? media[depth - 2] : R(1);
+        sc.make_isec(prim, inst, ray, hu, hv, media[depth - 1], enclosing, is);
+        if (is.bsdf_mat < 0) break;
+        Bsdf<R> bsdf;
+        sc.make_bsdf(bsdf, is.bsdf_mat, is);
+        // (no medium attenuates: the materials' media only refract)
+        if (bsdf.supports(ALL & ~SPECULAR)) {
+          if (deposits++ >= SGR_PHOTON_MAX_DEPOSITS) {
+            out.overflows++;
+          } else {
+            const Photon<R> ph{is.nf.origin, power, -dir};
+            int set = history == 0 ? 0
+                      : (targets[1] != 0 && !(history & ALL_LAYERS & ~SPECULAR)) ? 1 : 2;
+            if (!complete(set)) sets[set].push_back(ph);
+          }
+        }
+        R u, v;
+        if (path_len == 0) {
+          u = R(float(radical_inverse(path, 13)));
+          v = R(float(radical_inverse(path, 17)));
+        } else {
+          u = R(draw(qk, nd++));
+          v = R(draw(qk, nd++));
+        }
+        const BsdfSample<R> bs = bsdf.sample(u, v, ALL);
+        if (bs.val.is_zero() || bs.pdf == 0) break;
+        if (path_len > 3) {
+          if (draw(qk, nd++) < 0.5f) break;
+          power /= R(0.5f);
+        }
+        pos = is.nf.origin;
+        dir = is.nf.from(bs.dir);
+        power *= bs.val * std::fabs(is.cos_n(bs.dir)) / bs.pdf;
+        if (!(bs.flags & 0x08u)) history |= bs.flags;  // TRANSLUCENT samples leave none
+        if (bs.flags & TRANSMISSIVE) {  // material/media.cc:75-118
+          bool has;
+          const R mior = sc.medium_ior(is.mat, has);
+          if (has) {
+            if (is.back) {
+              if (depth >= 2) depth--;
+            } else if (depth < 8) {
+              media[depth++] = mior;
+            }
+          }
+        }
+      }
+      if (path > 100000000u) break;
+    }
+  }
+  out.paths_shot = path;
+  for (int i = 0; i < 3; i++) {
+    out.paths[i] = num_paths[i];
+    out.scale[i] = num_paths[i] > 0 ? 1 / float(num_paths[i]) : 0.f;
+    out.map[i].set_photons(sets[i]);
+  }
+}
+
 template<class R> struct Tracer {
   const Scene<R> &sc;
   const sgr_render_params &P;
   Stream st;
   Counters cnt;
 
+  const PhotonMaps<R> *pm = nullptr;  // SGR_INTEG_PHOTON
+  std::vector<const Photon<R> *> found;
   Tracer(const Scene<R> &s, const sgr_render_params &p) : sc(s), P(p) {
+    if (p.integrator == SGR_INTEG_PHOTON && p.photon_map)
+      pm = &((OrcPhotonMap *)p.photon_map)->get((R *)nullptr);
     st.init(p.spp, p.min_path_len, (uint32_t)s.lights.size(), p.direct_samples);
-    st.init_direct(p.light_samples);
+    // the photon integrator's direct lighting takes its maps' direct_samples
+    st.init_direct(pm ? pm->direct_samples : p.light_samples);
+    if (pm) {
+      const unsigned n[2] = {pm->fg_photon, pm->fg_bsdf};
+      for (int i = 0; i < 2; i++)
+        if (n[i]) {
+          fg_grid[i] = grid_shape(p.spp * n[i]);
+          fg_nsub[i] = fg_grid[i].total / p.spp;
+        }
+    }
   }
 
   // Scene::occludes: TRANS is multiplied by the partially occluding
@@ -290,6 +470,167 @@ template<class R> struct Tracer {
     return rad;
   }
 
+  // PhotonEval::Lo (photon/photon-eval.cc:44-118): the photons near the
+  // hit, each weighted by the BSDF in its direction and a Gaussian filter
+  // of its distance, times the map's scale, over pi r^2 with r the final
+  // search radius.
+  C3<R> lo_photon(const Isec<R> &is, const Bsdf<R> &bsdf, int set, unsigned flags) {
+    const float scale = pm->scale[set];
+    if (scale == 0) return C3<R>(0);
+    if (!bsdf.supports(flags & ~SPECULAR)) return C3<R>(0);
+    found.clear();
+    const R maxd = pm->map[set].find(is.nf.origin, pm->k, pm->radius_sq, found);
+    if (found.empty()) return C3<R>(0);
+    GaussSqrt<R> g(maxd, R(2));
+    const float gscale = 2.90898f;
+    C3<R> rad(0);
+    for (const Photon<R> *ph : found) {
+      const BsdfValue<R> v = bsdf.eval(is.nf.to(ph->dir), flags);
+      if (v.pdf != 0 && v.val.gt0()) {
+        const float filt = g((ph->pos - is.nf.origin).length_squared());
+        rad += v.val * ph->power * R(filt) * R(gscale);
+      }
+    }
+    rad *= R(scale);
+    rad /= maxd * R(PIf);
+    return rad;
+  }
+
+  // PhotonEval::photon_dist (photon/photon-eval.cc:128-185): the
+  // indirect photons near the hit binned by direction, weighted by their
+  // power's intensity.
+  void photon_dist(const Isec<R> &is, DirHistDist &dd) {
+    dd.clear();
+    found.clear();
+    pm->map[2].find(is.nf.origin, pm->k, pm->radius_sq, found);
+    for (const Photon<R> *ph : found) {
+      R u, v;
+      Scene<R>::sphere_sample_inverse(ph->dir, u, v);
+      dd.add(float(u), float(v), float(ph->power.intensity()));
+    }
+    dd.calc();
+  }
+
+  // PhotonInteg::Lo_fgather_samp (render/photon-integ.cc:268-380): the
+  // radiance the photon maps give at the surface a BSDF sample reaches,
+  // through up to 3 levels of specular bounces there (sampled from the
+  // path's sequential stream); past level 0 the surface's emission or the
+  // background counts, times SCALE.  The media are the shading point's
+  // throughout.
+  C3<R> fg_samp(const Isec<R> &is, R m0, R m1, const BsdfSample<R> &bs, const C3<R> &scale,
+                unsigned depth) {
+    C3<R> rad(0);
+    if (!(bs.val.gt0() && bs.pdf != 0)) return rad;
+    const Ray<R> ray0 = sc.recursive_ray(is, bs.dir, 0, false);
+    Ray<R> ray = ray0;
+    cnt.isect++;
+    R hu, hv;
+    unsigned inst = 0xffffffffu;
+    const unsigned prim = sc.accel.closest(ray, hu, hv, &cnt.nc_c, &cnt.st_c, &inst);
+    if (prim == 0xffffffffu) {
+      if (depth != 0) rad += sc.background(ray0.dir) * scale;
+      return rad;
+    }
+    Isec<R> si;
+    sc.make_isec(prim, inst, ray, hu, hv, m0, m1, si);
+    if (si.bsdf_mat >= 0) {
+      Bsdf<R> sb;
+      sc.make_bsdf(sb, si.bsdf_mat, si);
+      const C3<R> Li = lo_photon(si, sb, 0, ALL) + lo_photon(si, sb, 2, ALL) + lo_photon(si, sb, 1, ALL);
+      const C3<R> li_to_lo = bs.val * std::fabs(is.cos_n(bs.dir)) / bs.pdf;
+      rad += Li * li_to_lo;
+      const unsigned spec = ALL_DIRECTIONS | SPECULAR;
+      if (sb.supports(spec) && depth < 3) {
+        const R u = dint_draw();
+        const R v = dint_draw();
+        const BsdfSample<R> rs = sb.sample(u, v, spec);
+        rad += fg_samp(si, m0, m1, rs, scale, depth + 1) * li_to_lo;
+      }
+    }
+    if (depth != 0) {
+      const Material<R> &hm = sc.mats[si.mat];
+      if (hm.type == SGR_MAT_GLOW && !si.back) rad += hm.color * scale;  // Intersect::Le
+    }
+    return rad;
+  }
+
+  // Color / Color with zero denominators giving 0 (color/color.h:294-300)
+  static C3<R> cdiv(const C3<R> &a, const C3<R> &b) {
+    return C3<R>(b.r == 0 ? R(0) : a.r / b.r, b.g == 0 ? R(0) : a.g / b.g,
+                 b.b == 0 ? R(0) : a.b / b.b);
+  }
+
+  // PhotonInteg::Lo_fgather (render/photon-integ.cc:397-545): photon-
+  // direction samples and BSDF samples, each through fg_samp, combined by
+  // the power heuristic with sample counts.
+  C3<R> lo_fgather(const Isec<R> &is, const Bsdf<R> &bsdf, R m0, R m1, uint64_t pk, uint32_t s,
+                   bool avoid_caustics) {
+    C3<R> rad(0);
+    const unsigned nph = pm->fg_photon, nbs = pm->fg_bsdf;
+    DirHistDist dd;
+    photon_dist(is, dd);
+    const unsigned bflags = bsdf.supports(ALL);
+    for (unsigned i = 0; i < nph; i++) {
+      float pu, pv, su, sv, spdf;
+      Stream::strat(pk, fg_chan(0), s * fg_nsub[0] + i, fg_grid[0], pu, pv);
+      dd.sample(pu, pv, su, sv, spdf);
+      spdf *= 0.25f * INV_PIf;
+      const V3<R> sdir = Scene<R>::sphere_sample(R(su), R(sv));
+      const V3<R> bdir = is.nf.to(sdir);
+      const BsdfValue<R> bv = bsdf.eval(bdir, ALL & ~SPECULAR);
+      if (bv.pdf != 0 && spdf != 0) {
+        C3<R> scale(1);
+        if (avoid_caustics) {
+          if (bflags & ALL_LAYERS & ~(SPECULAR | DIFFUSE))
+            scale = cdiv(bsdf.eval(bdir, ALL & ~(SPECULAR | DIFFUSE)).val, bv.val);
+          else
+            scale = C3<R>(0);
+        }
+        BsdfSample<R> fake;
+        fake.val = bv.val;
+        fake.pdf = R(spdf);
+        fake.dir = bdir;
+        fake.flags = 0;
+        rad += fg_samp(is, m0, m1, fake, scale, 0) *
+               R(mis_sample_weight(spdf, float(nph), float(bv.pdf), float(nbs)));
+      }
+    }
+    for (unsigned i = 0; i < nbs; i++) {
+      float bu, bv;
+      Stream::strat(pk, fg_chan(1), s * fg_nsub[1] + i, fg_grid[1], bu, bv);
+      const BsdfSample<R> bs = bsdf.sample(R(bu), R(bv), ALL & ~SPECULAR);
+      if (bs.pdf != 0) {
+        R du, dv;
+        Scene<R>::sphere_sample_inverse(is.nf.from(bs.dir), du, dv);
+        const float ph_pdf = dd.pdf(float(du), float(dv)) * 0.25f * INV_PIf;
+        const C3<R> scale = (avoid_caustics && (bs.flags & DIFFUSE)) ? C3<R>(0) : C3<R>(1);
+        rad += fg_samp(is, m0, m1, bs, scale, 0) *
+               R(mis_sample_weight(float(bs.pdf), float(nbs), ph_pdf, float(nph)));
+      }
+    }
+    return rad;
+  }
+
+  // Final-gathering channels after the direct lighting's (1 + 2 l and
+  // 2 + 2 l): photon-direction parameters 1 + 2 L, BSDF parameters 2 + 2 L,
+  // fg_photon / fg_bsdf entries per sample on jittered grids.
+  GridShape fg_grid[2];
+  uint32_t fg_nsub[2] = {0, 0};
+  uint32_t fg_chan(int which) const { return 1 + 2 * st.L + (uint32_t)which; }
+
+  // PhotonInteg::Lo (render/photon-integ.cc:558-600)
+  C3<R> pint_Lo(const Isec<R> &is, const Bsdf<R> &bsdf, uint64_t pk, uint32_t s, R m0, R m1) {
+    const bool use_fg = pm->fg > 0;
+    const bool caustics = !pm->map[1].photons.empty();
+    C3<R> rad(0);
+    if (pm->direct_illum) rad += dint_direct(is, bsdf, pk, s);
+    else rad += lo_photon(is, bsdf, 0, ALL);
+    if (caustics) rad += lo_photon(is, bsdf, 1, use_fg ? (ALL_DIRECTIONS | DIFFUSE) : ALL);
+    if (use_fg) rad += lo_fgather(is, bsdf, m0, m1, pk, s, caustics);
+    else rad += lo_photon(is, bsdf, 2, ALL);
+    return rad;
+  }
+
   void dint_Li(const Ray<R> &ray, R m0, R m1, uint32_t depth, C3<R> W, uint64_t pk,
                uint32_t s, C3<R> &total, R &alpha) {
     if (depth > 5) {  // render/recursive-integ.cc:115-138
@@ -313,7 +654,7 @@ template<class R> struct Tracer {
     if (is.bsdf_mat < 0) return;
     Bsdf<R> bsdf;
     sc.make_bsdf(bsdf, is.bsdf_mat, is);
-    total += dint_direct(is, bsdf, pk, s) * W;
+    total += (pm ? pint_Lo(is, bsdf, pk, s, m0, m1) : dint_direct(is, bsdf, pk, s)) * W;
     BsdfSample<R> rs = bsdf.sample(R(0), R(0), SPECULAR | REFLECTIVE);
     if (rs.val.gt0()) {
       if (depth + 1 >= kDintDepth) {
@@ -372,8 +713,9 @@ template<class R> struct Tracer {
     cnt.eye++;
     cnt.paths++;
     R alpha;
-    C3<R> L = P.integrator == SGR_INTEG_DIRECT ? dint(ray, pk, qk, s, alpha)
-                                               : Li(ray, pk, qk, s, alpha);
+    C3<R> L = (P.integrator == SGR_INTEG_DIRECT || P.integrator == SGR_INTEG_PHOTON)
+                  ? dint(ray, pk, qk, s, alpha)
+                  : Li(ray, pk, qk, s, alpha);
     out[0] = cx;
     out[1] = cy;
     out[2] = float(L.r * alpha);  // Tint(radiance, alpha) premultiplies
@@ -842,6 +1184,203 @@ int orc_light_probe(void *h, unsigned light, int mode, const double *q, int n, d
   OrcScene *s = (OrcScene *)h;
   return s->fp64 ? light_probe_t(*s->sd, light, mode, q, n, out)
                  : light_probe_t(*s->sf, light, mode, q, n, out);
+}
+
+}  // extern "C"
+
+namespace {
+template<class R>
+void photon_params_into(PhotonMaps<R> &m, const sgr_photon_params &p) {
+  unsigned targets[3];
+  resolve_photon_params(p, m.fg, m.fg_bsdf, m.fg_photon, targets);
+  m.k = p.use_photons;
+  m.radius_sq = R(p.photon_radius) * R(p.photon_radius);
+  m.direct_illum = p.direct_illum != 0;
+  m.direct_samples = p.direct_samples;
+}
+}  // namespace
+
+extern "C" {
+
+// Photon maps of an oracle scene (PhotonInteg::GlobalState): shot and
+// built by the oracle.
+void *orc_photon_map_create(void *h, const sgr_photon_params *p, char *err, int errlen) {
+  try {
+    OrcScene *s = (OrcScene *)h;
+    OrcPhotonMap *m = new OrcPhotonMap;
+    m->fp64 = s->fp64;
+    unsigned fg, fb, fp, targets[3];
+    resolve_photon_params(*p, fg, fb, fp, targets);
+    if (s->fp64) {
+      photon_params_into(m->d, *p);
+      shoot_photons(*s->sd, *p, targets, m->d);
+    } else {
+      photon_params_into(m->f, *p);
+      shoot_photons(*s->sf, *p, targets, m->f);
+    }
+    return m;
+  } catch (const std::exception &e) {
+    set_err(err, errlen, e.what());
+    return nullptr;
+  }
+}
+
+// Photon maps given as kd-ordered arrays (another builder's export):
+// per set COUNTS[i] photons of pos xyz + split axis, power rgb, dir xyz,
+// and PATHS[i], the set's path count.
+void *orc_photon_map_import(const sgr_photon_params *p, const float *const *pos_axis,
+                            const float *const *power, const float *const *dir,
+                            const uint64_t *counts, const uint64_t *paths) {
+  OrcPhotonMap *m = new OrcPhotonMap;
+  photon_params_into(m->f, *p);
+  photon_params_into(m->d, *p);
+  for (int i = 0; i < 3; i++) {
+    for (int w = 0; w < 2; w++) {
+      PhotonMap<float> *mf = w == 0 ? &m->f.map[i] : nullptr;
+      PhotonMap<double> *md = w == 1 ? &m->d.map[i] : nullptr;
+      const size_t n = counts[i];
+      if (mf) { mf->photons.resize(n); mf->axes.resize(n); }
+      if (md) { md->photons.resize(n); md->axes.resize(n); }
+      for (size_t k = 0; k < n; k++) {
+        const float *a = pos_axis[i] + 4 * k, *c = power[i] + 3 * k, *d = dir[i] + 3 * k;
+        unsigned ax;
+        std::memcpy(&ax, &a[3], 4);
+        if (mf) {
+          mf->photons[k] = Photon<float>{V3<float>(a[0], a[1], a[2]), C3<float>(c[0], c[1], c[2]),
+                                         V3<float>(d[0], d[1], d[2])};
+          mf->axes[k] = (unsigned char)ax;
+        } else {
+          md->photons[k] = Photon<double>{V3<double>(a[0], a[1], a[2]),
+                                          C3<double>(c[0], c[1], c[2]), V3<double>(d[0], d[1], d[2])};
+          md->axes[k] = (unsigned char)ax;
+        }
+      }
+    }
+    m->f.paths[i] = m->d.paths[i] = paths[i];
+    m->f.scale[i] = m->d.scale[i] = paths[i] > 0 ? 1 / float(paths[i]) : 0.f;
+  }
+  return m;
+}
+
+int orc_photon_map_info(void *h, sgr_photon_map_info *info) {
+  OrcPhotonMap *m = (OrcPhotonMap *)h;
+  std::memset(info, 0, sizeof *info);
+  auto fill = [&](auto &pm) {
+    for (int i = 0; i < 3; i++) {
+      info->photons[i] = pm.map[i].photons.size();
+      info->paths[i] = pm.paths[i];
+      info->scale[i] = pm.scale[i];
+    }
+    info->paths_shot = pm.paths_shot;
+    info->deposit_overflows = pm.overflows;
+    info->fg_samples = pm.fg;
+    info->fg_bsdf_samples = pm.fg_bsdf;
+    info->fg_photon_samples = pm.fg_photon;
+  };
+  if (m->fp64) fill(m->d);
+  else fill(m->f);
+  return 0;
+}
+
+int orc_photon_map_export(void *h, int set, float *pos_axis, float *power, float *dir) {
+  OrcPhotonMap *m = (OrcPhotonMap *)h;
+  if (set < 0 || set > 2) return 1;
+  auto put = [&](auto &pm) {
+    const auto &mp = pm.map[set];
+    for (size_t k = 0; k < mp.photons.size(); k++) {
+      const auto &ph = mp.photons[k];
+      const unsigned ax = mp.axes[k];
+      if (pos_axis) {
+        pos_axis[4 * k] = float(ph.pos.x);
+        pos_axis[4 * k + 1] = float(ph.pos.y);
+        pos_axis[4 * k + 2] = float(ph.pos.z);
+        std::memcpy(&pos_axis[4 * k + 3], &ax, 4);
+      }
+      if (power) {
+        power[3 * k] = float(ph.power.r);
+        power[3 * k + 1] = float(ph.power.g);
+        power[3 * k + 2] = float(ph.power.b);
+      }
+      if (dir) {
+        dir[3 * k] = float(ph.dir.x);
+        dir[3 * k + 1] = float(ph.dir.y);
+        dir[3 * k + 2] = float(ph.dir.z);
+      }
+    }
+  };
+  if (m->fp64) put(m->d);
+  else put(m->f);
+  return 0;
+}
+
+void orc_photon_map_destroy(void *h) { delete (OrcPhotonMap *)h; }
+
+}  // extern "C"
+
+extern "C" {
+
+double orc_radical_inverse(unsigned num, unsigned base) { return radical_inverse(num, base); }
+
+// kd-tree probe: builds a PhotonMap of n photons (positions xyz; power
+// and direction zero) and returns its heap-order positions, split axes
+// and, for m query points, the photons find() keeps (kd indices, -1
+// padded, k per query) and the final squared radius.
+int orc_kd_probe(const float *pts, int n, const float *queries, int m, unsigned k, float r2,
+                 float *kd_pos, int *kd_axis, int *found, float *final_r2) {
+  std::vector<Photon<float>> v(n);
+  for (int i = 0; i < n; i++) v[i] = Photon<float>{V3<float>(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), C3<float>(0), V3<float>()};
+  PhotonMap<float> pm;
+  pm.set_photons(v);
+  for (int i = 0; i < n; i++) {
+    kd_pos[3 * i] = pm.photons[i].pos.x;
+    kd_pos[3 * i + 1] = pm.photons[i].pos.y;
+    kd_pos[3 * i + 2] = pm.photons[i].pos.z;
+    kd_axis[i] = pm.axes[i];
+  }
+  std::vector<const Photon<float> *> heap;
+  for (int q = 0; q < m; q++) {
+    heap.clear();
+    const V3<float> pos(queries[3 * q], queries[3 * q + 1], queries[3 * q + 2]);
+    final_r2[q] = pm.find(pos, k, r2, heap);
+    for (unsigned j = 0; j < k; j++)
+      found[(size_t)q * k + j] = j < heap.size() ? int(heap[j] - pm.photons.data()) : -1;
+  }
+  return 0;
+}
+
+// Direction-histogram probe: bins n world directions with weights, then
+// for m parameter pairs samples the distribution (direction xyz, pdf per
+// steradian) and evaluates pdf() at the sampled direction (5 per
+// sample); out_bins gets the 64 bin pdfs (per unit parameter area).
+int orc_dirhist_probe(const float *dirs, const float *w, int n, const float *params, int m,
+                      float *out, float *out_bins) {
+  DirHistDist dd;
+  dd.clear();
+  for (int i = 0; i < n; i++) {
+    double u, v;
+    Scene<double>::sphere_sample_inverse(V3<double>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), u, v);
+    float fu, fv;
+    Scene<float>::sphere_sample_inverse(V3<float>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]), fu, fv);
+    (void)u;
+    (void)v;
+    dd.add(fu, fv, w[i]);
+  }
+  dd.calc();
+  for (int b = 0; b < 64; b++) out_bins[b] = dd.bin_pdf(b % 8, b / 8);
+  for (int i = 0; i < m; i++) {
+    float su, sv, pdf;
+    dd.sample(params[2 * i], params[2 * i + 1], su, sv, pdf);
+    pdf *= 0.25f * INV_PIf;
+    const V3<float> d = Scene<float>::sphere_sample(su, sv);
+    float du, dv;
+    Scene<float>::sphere_sample_inverse(d, du, dv);
+    out[5 * i] = d.x;
+    out[5 * i + 1] = d.y;
+    out[5 * i + 2] = d.z;
+    out[5 * i + 3] = pdf;
+    out[5 * i + 4] = dd.pdf(du, dv) * 0.25f * INV_PIf;
+  }
+  return 0;
 }
 
 }  // extern "C"

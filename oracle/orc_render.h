@@ -480,6 +480,12 @@ template<class R> struct Bsdf {
       case SGR_MAT_PHONG:
         return ((limit & REFLECTIVE) && (limit & (DIFFUSE | GLOSSY)))
                    ? ((REFLECTIVE | DIFFUSE | GLOSSY) & limit) : 0u;
+      case SGR_MAT_STENCIL: {  // material/stencil.cc:114-123 (not masked by LIMIT)
+        unsigned f = 0;
+        if (opacity.gt0() && under) f |= under->supports(limit);
+        if (opacity.sum() < R(3)) f |= TRANSMISSIVE | SPECULAR;
+        return f;
+      }
     }
     return 0;
   }
@@ -688,6 +694,9 @@ template<class R> struct Envmap {
 };
 
 template<class R> struct LightSample { C3<R> val; R pdf = 0; V3<R> dir; R dist = 0; };
+// Light::Sampler::FreeSample (light/light-sampler.h): a point and a
+// direction leaving it, for shooting photons.
+template<class R> struct FreeSample { C3<R> val; R pdf = 0; V3<R> pos, dir; };
 
 template<class R> struct Light {
   // 0 = tripar area light, 1 = sphere light, 2 = envmap; surface lights
@@ -1467,6 +1476,88 @@ template<class R> struct Scene {
     norm = xform_transpose(pr.w2l, onorm).unit();
     auto f = [&pr](R a, R b) { return xform_pos(pr.l2w, sphere_sample(a, b)); };
     apdf = approx_area_pdf(f, su, sv, pos) * R(2);
+  }
+
+  // Frame (const Vec &z) (geometry/frame.h:64-73): x from z's
+  // perpendicular, y = z x x.
+  static Frame<R> frame_of(const V3<R> &org, const V3<R> &z) {
+    const V3<R> x = z.perpendicular().unit();
+    const V3<R> y = cross(z, x).unit();
+    return Frame<R>(org, x, y, z);
+  }
+
+  // Light::Sampler::sample (param, dir_param), the "free" sample a photon
+  // starts from: SurfaceLightSampler (light/surface-light-sampler.cc:
+  // 72-100: an area sample, a cosine-distributed direction about its
+  // normal, pdf = area pdf / pi), SphereLightSampler (light/sphere-light-
+  // sampler.cc:120-155) and EnvmapLight::Sampler (light/envmap-light.cc:
+  // 189-230: a direction from the intensity distribution, a start point
+  // on the disk tangent to the scene's bounding sphere; its value is the
+  // map looked up with the world-space direction, as the reference
+  // passes it).
+  FreeSample<R> light_free_sample(const Light<R> &L, R pu, R pv, R du, R dv) const {
+    FreeSample<R> out;
+    if (L.kind == 2) {
+      const Envmap<R> &E = envs[L.env];
+      float mu, mv, pdf;
+      E.hist_sample(float(du), float(dv), mu, mv, pdf);
+      const V3<R> light_dir = Envmap<R>::latlong_to_vec(mu, mv);
+      const V3<R> world_dir = E.frame.from(light_dir);
+      R p = R(pdf) * R(0.25f) * R(INV_PIf);
+      const V3<R> cen = bbox.center();
+      const R rad = bbox.radius();
+      R px, py;
+      disk_sample(rad, pu, pv, px, py);  // geometry/tangent-disk-sample.h:31-51
+      const Frame<R> f = frame_of(cen, world_dir);
+      out.pos = cen + f.from(V3<R>(px, py, rad));
+      p = p / (R(float(rad * rad)) * R(PIf));
+      out.val = E.map(world_dir);
+      out.pdf = p;
+      out.dir = -world_dir;
+      return out;
+    }
+    V3<R> pos, norm;
+    R apdf;
+    if (L.kind == 1) {
+      const Prim<R> &pr = prims[L.prim];
+      const V3<R> spv = sphere_sample(pu, pv);
+      pos = pr.center + spv * pr.radius;
+      apdf = R(1) / (pr.radius * pr.radius * R(float(4 * PI_D)));
+      norm = spv;
+    } else if (L.kind == 0) {
+      const Prim<R> &pr = prims[L.prim];
+      V3<R> n = cross(pr.e2, pr.e1);
+      R area = n.length();
+      norm = n / area;
+      R u = pu, v = pv;
+      if (!pr.parallelogram && u + v > 1) { u = R(1) - u; v = R(1) - v; }
+      if (!pr.parallelogram) area *= R(0.5f);
+      pos = pr.v0 + pr.e1 * u + pr.e2 * v;
+      apdf = R(1) / area;
+    } else if (L.kind == 3) {
+      ellipse_area(prims[L.prim], pu, pv, pos, norm, apdf);
+    } else if (L.kind == 4) {  // Cylinder::Sampler::sample (surface/cylinder.cc:225-233)
+      const Prim<R> &pr = prims[L.prim];
+      const R theta = pu * R(2) * R(PIf);
+      norm = xform_transpose(pr.w2l, V3<R>(std::cos(theta), std::sin(theta), R(0))).unit();
+      auto f = [&pr](R a, R b) {
+        const R th = a * R(2) * R(PIf);
+        return xform_pos(pr.l2w, V3<R>(std::cos(th), std::sin(th), R(2) * b - R(1)));
+      };
+      apdf = approx_area_pdf(f, pu, pv, pos);
+    } else {  // Sphere2::Sampler::sample (surface/sphere2.cc:225-232)
+      const Prim<R> &pr = prims[L.prim];
+      norm = xform_transpose(pr.w2l, sphere_sample(pu, pv)).unit();
+      auto f = [&pr](R a, R b) { return xform_pos(pr.l2w, sphere_sample(a, b)); };
+      apdf = approx_area_pdf(f, pu, pv, pos);
+    }
+    // CosDist (material/cos-dist.h:24-44) about the normal
+    const V3<R> d = z_normal_symm_vec(std::sqrt(du), dv);
+    out.pos = pos;
+    out.dir = frame_of(V3<R>(), norm).from(d);
+    out.pdf = apdf * R(INV_PIf);
+    out.val = L.intensity;
+    return out;
   }
 
   // light/sphere-light-sampler.cc:40-48

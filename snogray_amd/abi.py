@@ -17,7 +17,9 @@ SGR_MAT_GLOW = 5
 SGR_MAT_PHONG = 6
 SGR_MAT_THIN_GLASS = 7
 SGR_MAT_STENCIL = 8
-SGR_INTEG_PATH, SGR_INTEG_DIRECT = 0, 1
+SGR_INTEG_PATH, SGR_INTEG_DIRECT, SGR_INTEG_PHOTON = 0, 1, 2
+SGR_PHOTON_UNSPEC = 0xFFFFFFFF
+SGR_PHOTON_MAX_DEPOSITS = 32
 SGR_ACCEL_OCTREE = 0
 SGR_ACCEL_TRIV = 1
 MISS = 0xFFFFFFFF
@@ -126,7 +128,43 @@ class RenderParams(C.Structure):
                 ("bg_alpha", C.c_float), ("x0", C.c_uint32),
                 ("y0", C.c_uint32), ("w", C.c_uint32), ("h", C.c_uint32),
                 ("paths_in_flight", C.c_uint32), ("flags", C.c_uint32),
-                ("integrator", C.c_uint32), ("light_samples", C.c_uint32)]
+                ("integrator", C.c_uint32), ("light_samples", C.c_uint32),
+                ("photon_map", C.c_void_p)]
+
+
+class PhotonParams(C.Structure):
+    _fields_ = [("caustic_photons", C.c_uint32), ("direct_photons", C.c_uint32),
+                ("indirect_photons", C.c_uint32), ("use_photons", C.c_uint32),
+                ("photon_radius", C.c_float), ("final_gather", C.c_uint32),
+                ("fg_samples", C.c_uint32), ("fg_bsdf_samples", C.c_uint32),
+                ("fg_photon_samples", C.c_uint32), ("direct_illum", C.c_uint32),
+                ("direct_samples", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class PhotonMapInfo(C.Structure):
+    _fields_ = [("photons", C.c_uint64 * 3), ("paths", C.c_uint64 * 3),
+                ("scale", C.c_float * 3), ("paths_shot", C.c_uint64),
+                ("deposit_overflows", C.c_uint64), ("fg_samples", C.c_uint32),
+                ("fg_bsdf_samples", C.c_uint32), ("fg_photon_samples", C.c_uint32),
+                ("ms_shoot", C.c_double), ("ms_build", C.c_double)]
+
+    def as_dict(self):
+        d = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            d[k] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
+def photon_params(caustic=50000, direct=500000, indirect=500000, use_photons=50,
+                  radius=0.1, final_gather=True, fg_samples=None, fg_bsdf_samples=None,
+                  fg_photon_samples=None, direct_illum=True, direct_samples=16, seed=1):
+    """sgr_photon_params with the reference's defaults
+    (render/photon-integ.cc:36-148); None leaves a count unspecified."""
+    u = lambda x: SGR_PHOTON_UNSPEC if x is None else int(x)
+    return PhotonParams(caustic, direct, indirect, use_photons, radius,
+                        1 if final_gather else 0, u(fg_samples), u(fg_bsdf_samples),
+                        u(fg_photon_samples), 1 if direct_illum else 0, direct_samples, seed)
 
 
 class Stats(C.Structure):
@@ -202,7 +240,8 @@ def uptr(a):
 def make_params(width, height, spp, max_path_len=8, min_path_len=3,
                 direct_samples=1, seed=1, window=None, filter_b=1.0 / 3,
                 filter_c=1.0 / 3, filter_radius=2.0, neg_clamp=-0.1,
-                bg_alpha=1.0, paths_in_flight=0, flags=0, integrator=0, light_samples=0):
+                bg_alpha=1.0, paths_in_flight=0, flags=0, integrator=0, light_samples=0,
+                photon_map=None):
     p = RenderParams()
     p.width, p.height, p.spp = width, height, spp
     p.min_path_len, p.max_path_len = min_path_len, max_path_len
@@ -214,6 +253,9 @@ def make_params(width, height, spp, max_path_len=8, min_path_len=3,
     p.paths_in_flight = paths_in_flight
     p.flags = flags
     p.integrator, p.light_samples = integrator, light_samples
+    if photon_map is not None:
+        p.integrator = SGR_INTEG_PHOTON
+        p.photon_map = photon_map.handle()
     return p
 
 

@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsnoggpu.so")
 SOURCES = ["sgr_kernels.hip", "sgr_api.cpp", "octree_build.cpp", "sgr_jobs.cpp",
-           "mlp_kernels.hip", "sgr_guide.cpp", "image_io.cpp", "mesh_io.cpp"]
+           "mlp_kernels.hip", "sgr_guide.cpp", "image_io.cpp", "mesh_io.cpp",
+           "sgr_photon.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SGR_OFFLOAD_ARCH", "gfx950")
 

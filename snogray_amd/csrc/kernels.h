@@ -58,6 +58,44 @@ static_assert(SGR_CNT_HE >= 2 * SGR_MAX_BOUNCES + 4 && SGR_CNT_HE % SGR_HEAD_STR
 #define SGR_STATS_BYTES  (SGR_STATS_HEADS_OFF + 8 * SGR_HEAD_STRIDE * 4)
 static_assert(SGR_NUM_STATS * 8 <= SGR_STATS_HEADS_OFF, "statistics layout");
 
+// One photon map in kd-tree heap order (children of node i at 2i + 1,
+// 2i + 2): position and split axis (bits of w), power, the direction the
+// photon came from.
+struct PhotonSetView {
+  const float4 *pos;
+  const float4 *pow;
+  const float4 *dir;
+  uint32_t n;
+  float scale;  // 1 / paths of the set (0: the set contributes nothing)
+};
+
+// The photon integrator's maps and parameters (render/photon-integ.cc:
+// 36-148): photons per estimate, search radius squared, final-gathering
+// sample counts and their channels' grids.
+struct PhotonView {
+  PhotonSetView set[3];  // direct, caustic, indirect
+  uint32_t k;
+  float r2;
+  uint32_t fg, fg_bsdf, fg_photon;
+  uint32_t direct_illum, caustics;
+  Grid g_fgp, g_fgb;
+  uint32_t nsub_fgp, nsub_fgb;
+};
+
+// A batch of photon paths [base, base + n): per path, rec = (first
+// deposit, deposit attempts | valid << 31); deposits are 3 float4 each
+// (position + set in w's bits, power, the direction the photon came
+// from), reserved in order of completion up to CAP.
+struct PhotonShootView {
+  uint64_t seed;
+  uint32_t base, n;
+  uint32_t caustic_on;   // a caustic set is wanted (else those photons are indirect)
+  uint2 *rec;
+  float4 *dep;
+  uint32_t cap;
+  uint32_t *counter;
+};
+
 struct RayBatchDev {
   const float *ox, *oy, *oz, *dx, *dy, *dz, *t0, *t1;
   uint32_t *prim;
@@ -74,6 +112,7 @@ struct RenderView {
   // samples per light and per sample, and their grid (spp x samples).
   uint32_t integ, nsub_dint;
   Grid g_dint;
+  PhotonView ph;  // SGR_INTEG_PHOTON
   uint64_t seed;
   float bg_alpha, fb, fc, inv_radius, neg_clamp;
   int fr;
@@ -157,6 +196,8 @@ void launch_extend(bool multi, const SceneView &S, const StepView &J, const Path
 // (single-job renders only).
 void launch_direct_integ(const SceneView &S, const RenderView &R, const PathBufs &P,
                          unsigned long long *stats, unsigned grid, hipStream_t st);
+void launch_photon_shoot(const SceneView &S, const PhotonShootView &V, unsigned long long *stats,
+                         unsigned grid, hipStream_t st);
 void launch_shade(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
                   const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,
                   hipStream_t st, const GuideView *G = nullptr);

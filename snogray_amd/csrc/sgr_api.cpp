@@ -128,6 +128,9 @@ void build_guide(const float *a, uint32_t n, uint32_t G, std::vector<uint32_t> &
 inline float fmin_t(float a, float b) { return a < b ? a : b; }
 inline float fmax_t(float a, float b) { return a > b ? a : b; }
 
+}  // namespace
+
+namespace sgr {
 // Grid shape of a channel with n samples (render/grid.cc:26-38).
 Grid grid_shape(uint32_t n) {
   uint32_t up = 1;
@@ -136,8 +139,7 @@ Grid grid_shape(uint32_t n) {
   while (down > 1 && (2 * down - 1) * (2 * down - 1) > 4 * n) down--;
   return Grid{up, down, up * down};
 }
-
-}  // namespace
+}  // namespace sgr
 
 // ---------------------------------------------------------------------------
 // Scratch of sgr_render_noalloc: the path state of its chunks lives in
@@ -917,6 +919,9 @@ int sgr_scene_destroy(sgr_scene *sc) {
   if (sc->jobs_using)
     return fail(SGR_ERR_INVALID, "sgr_scene_destroy: " + std::to_string(sc->jobs_using) +
                                      " job(s) still refer to the scene; release them first");
+  if (sc->photon_maps)
+    return fail(SGR_ERR_INVALID, "sgr_scene_destroy: " + std::to_string(sc->photon_maps) +
+                                     " photon map(s) belong to the scene; destroy them first");
   if (sc->ctx) {
     (void)hipSetDevice(sc->ctx->device);
     (void)hipDeviceSynchronize();  // no render of any thread still reads it
@@ -1172,12 +1177,19 @@ int plan_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p
   uint32_t nsub_max = std::max(R.nsub_strat, R.nsub_rand);
   R.K = 2 * R.L * nsub_max;
   R.integ = p->integrator;
-  if (R.integ != SGR_INTEG_PATH && R.integ != SGR_INTEG_DIRECT)
+  if (R.integ != SGR_INTEG_PATH && R.integ != SGR_INTEG_DIRECT && R.integ != SGR_INTEG_PHOTON)
     return fail(SGR_ERR_INVALID, "sgr_render: unknown integrator " + std::to_string(R.integ));
-  if (R.integ == SGR_INTEG_DIRECT) {
+  std::memset(&R.ph, 0, sizeof R.ph);
+  if (R.integ == SGR_INTEG_DIRECT || R.integ == SGR_INTEG_PHOTON) {
     // render/direct-integ.h:72-77: "light_samples", 16 by default; the
-    // shadow rays are traced where they are made (no shadow slots)
+    // shadow rays are traced where they are made (no shadow slots).  The
+    // photon integrator's direct lighting takes its map's direct_samples
+    // (render/photon-integ.cc:44-46).
     uint32_t ls = p->light_samples ? p->light_samples : 16u;
+    if (R.integ == SGR_INTEG_PHOTON) {
+      int rc = photon_render_view(sc, p->photon_map, R.spp, R.ph, ls);
+      if (rc) return rc;
+    }
     if ((uint64_t)R.spp * ls >= (1ull << 31))
       return fail(SGR_ERR_LIMIT, "sgr_render: spp x light_samples above 2^31");
     R.g_dint = grid_shape(R.spp * ls);
@@ -1371,7 +1383,7 @@ int run_render(sgr_scene *sc, RenderPlan &pl, float *accum_dev, float *samples_h
     t_begin();
     launch_raygen(false, R, J, P, grid, cur);
     t_end(K_OTHER);
-    if (R.integ == SGR_INTEG_DIRECT) {
+    if (R.integ == SGR_INTEG_DIRECT || R.integ == SGR_INTEG_PHOTON) {
       t_begin();
       launch_direct_integ(sc->view, R, P, ctx->stats, grid, cur);
       t_end(K_OTHER);

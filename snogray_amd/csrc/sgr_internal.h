@@ -79,6 +79,7 @@ struct sgr_scene {
   float bbox_lo[3], bbox_hi[3];
   uint32_t n_lights = 0;
   uint32_t jobs_using = 0;         // jobs of the engine referring to it
+  uint32_t photon_maps = 0;        // photon maps built for it (sgr_photon.cpp)
 };
 
 namespace sgr {
@@ -94,6 +95,12 @@ struct RenderPlan {
   uint32_t npix;   // pattern pixels including the filter margin
 };
 int plan_render(sgr_scene *sc, const sgr_camera *cam, const sgr_render_params *p, RenderPlan &pl);
+// Grid shape of a channel with n samples (render/grid.cc:26-38).
+Grid grid_shape(uint32_t n);
+// The photon integrator's view of MAP for a render of SPP samples per
+// pixel; DIRECT_SAMPLES gets the map's direct-lighting samples per light.
+int photon_render_view(const sgr_scene *sc, const sgr_photon_map *map, uint32_t spp,
+                       PhotonView &out, uint32_t &direct_samples);
 // Path buffers for CAP paths with K shadow slots, carved from BASE
 // (NULL: only the size is computed).
 size_t path_bytes(size_t cap, size_t K, PathBufs *P, char *base);

@@ -23,6 +23,7 @@
 #include <cstring>
 
 #include "dev_shade.h"
+#include "dev_photon.h"
 #include "kernels.h"
 #include "../../include/snoggpu.h"
 #include "dev_guide.h"
@@ -846,7 +847,259 @@ __device__ __forceinline__ RayF dint_ray(const SceneView &S, const Isec &is, V3 
   return r;
 }
 
+// PhotonShooter::shoot's paths (photon/photon-shooter.cc:29-191), one per
+// thread: path N takes light floor(radical_inverse(N, 11) x lights) and
+// its free sample at (radical_inverse(N, 2), (N, 3)) / ((N, 5), (N, 7)),
+// then bounces by BSDF samples at ((N, 13), (N, 17)) first and from its
+// own counter stream (two draws, then past 3 bounces the roulette draw,
+// per bounce).  At every hit whose BSDF has a non-specular part it
+// records a deposit and the set it belongs to by the scattering so far
+// (render/photon-integ.cc:196-226: none yet, direct; only specular
+// scattering with a caustic set wanted, caustic; else indirect); which
+// deposits the maps keep is decided in path order on the host.  Deposits
+// are held per thread, then written to the batch's array at a reserved
+// range.  The media stack is the path integrator's (8 levels).  A path
+// ends after SGR_PHOTON_MAX_BOUNCES hits (past 3 the roulette halves its
+// chance at each, so that is reached with probability 2^-252).
 template<bool INST>
+__global__ void __launch_bounds__(256) k_photon_shoot(SceneView S, PhotonShootView V,
+                                                      unsigned long long *stats) {
+  SGR_STACK_LDS(stk);
+  uint2 spill[SGR_MAX_DEPTH];
+  Nest N{};
+  if constexpr (INST) {
+    SGR_STACK_LDS(stk2);
+    N.L = stk2;
+  }
+  uint2 spill2[INST ? SGR_MAX_DEPTH : 1];
+  N.spill = spill2;
+  float4 dep[3 * SGR_PHOTON_MAX_DEPOSITS];
+  TraceCount cc{0, 0, 0, 0};
+  uint32_t rays = 0, movf = 0;
+  const uint64_t pk = pixel_key(V.seed, -2, -2);
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < V.n; t += gridDim.x * blockDim.x) {
+    const uint32_t path = V.base + t;
+    const uint32_t li = (uint32_t)(radical_inverse(path, 11) * (double)S.n_lights);
+    const DevLight L = S.lights[li];
+    const FreeS fs = light_free_sample<INST>(S, L, (float)radical_inverse(path, 2),
+                                             (float)radical_inverse(path, 3),
+                                             (float)radical_inverse(path, 5),
+                                             (float)radical_inverse(path, 7));
+    if (czero(fs.val) || fs.pdf == 0.0f) {
+      V.rec[t] = make_uint2(0u, 0u);
+      continue;
+    }
+    const uint64_t qk = path_key(pk, path);
+    uint32_t nd = 0, hist = 0, ndep = 0;
+    float media[SGR_MEDIA_MAX];
+    media[0] = 1.0f;
+    uint32_t depth = 1;
+    V3 pos = fs.pos, dir = fs.dir;
+    C3 power = (fs.val * (float)S.n_lights) * (1.0f / fs.pdf);  // Color * float / float
+    for (uint32_t plen = 0; plen < SGR_PHOTON_MAX_BOUNCES; plen++) {
+      TravState T;
+      T.ray.o = pos;
+      T.ray.d = dir;
+      T.ray.t0 = S.min_trace;
+      T.ray.t1 = S.horizon;
+      const RayF ray = T.ray;
+      trace_one<false, INST>(S, T, cc, stk, spill, N);
+      rays++;
+      if (T.hit == SGR_MISS) break;
+      RayF hr = ray;
+      hr.t1 = T.ray.t1;
+      Isec is;
+      float encl = 1.0f;
+#pragma unroll
+      for (uint32_t q = 0; q < SGR_MEDIA_MAX; q++)
+        if (q + 2 == depth) encl = media[q];
+      float top = 1.0f;
+#pragma unroll
+      for (uint32_t q = 0; q < SGR_MEDIA_MAX; q++)
+        if (q + 1 == depth) top = media[q];
+      make_isec<INST>(S, T.hit, T.hinst, hr, S.tri_vn ? T.hu : 0.0f, S.tri_vn ? T.hv : 0.0f, top,
+                      encl, is);
+      if (is.bmat < 0) break;
+      const DevMaterial &M = S.mats[is.bmat];
+      if (bsdf_supports_any(M, F_ALL & ~F_SPECULAR)) {
+        if (ndep < SGR_PHOTON_MAX_DEPOSITS) {
+          const uint32_t set = hist == 0 ? 0u
+                               : (V.caustic_on && !(hist & (F_GLOSSY | F_DIFFUSE))) ? 1u : 2u;
+#pragma unroll
+          for (uint32_t q = 0; q < SGR_PHOTON_MAX_DEPOSITS; q++)
+            if (q == ndep) {
+              dep[3 * q] = make_float4(is.nf.o.x, is.nf.o.y, is.nf.o.z, __uint_as_float(set));
+              dep[3 * q + 1] = make_float4(power.r, power.g, power.b, 0.0f);
+              dep[3 * q + 2] = make_float4(-dir.x, -dir.y, -dir.z, 0.0f);
+            }
+        }
+        ndep++;
+      }
+      float u, v;
+      if (plen == 0) {
+        u = (float)radical_inverse(path, 13);
+        v = (float)radical_inverse(path, 17);
+      } else {
+        u = draw(qk, nd++);
+        v = draw(qk, nd++);
+      }
+      const BSample bs = bsdf_sample(is, M, u, v, F_ALL);
+      if (czero(bs.val) || bs.pdf == 0.0f) break;
+      if (plen > 3) {
+        if (draw(qk, nd++) < 0.5f) break;
+        power = power * (1.0f / 0.5f);
+      }
+      pos = is.nf.o;
+      dir = is.nf.from(bs.dir);
+      power = power * ((bs.val * fabsf(bs.dir.z)) * (1.0f / bs.pdf));
+      if (!(bs.flags & 0x08u)) hist |= bs.flags;  // TRANSLUCENT samples leave none
+      if ((bs.flags & F_TRANSMISSIVE) && S.mats[is.mat].medium) {  // material/media.cc:75-118
+        const float mior = S.mats[is.bmat].ior_n;
+        if (is.back) {
+          if (depth >= 2) depth--;
+        } else if (depth < SGR_MEDIA_MAX) {
+#pragma unroll
+          for (uint32_t q = 0; q < SGR_MEDIA_MAX; q++)
+            if (q == depth) media[q] = mior;
+          depth++;
+        } else {
+          movf++;
+        }
+      }
+    }
+    const uint32_t nst = ndep < SGR_PHOTON_MAX_DEPOSITS ? ndep : SGR_PHOTON_MAX_DEPOSITS;
+    const uint32_t off = nst ? atomicAdd(V.counter, nst) : 0u;
+    if (off + nst <= V.cap) {
+      for (uint32_t q = 0; q < nst; q++) {
+        V.dep[3 * (size_t)(off + q)] = dep[3 * q];
+        V.dep[3 * (size_t)(off + q) + 1] = dep[3 * q + 1];
+        V.dep[3 * (size_t)(off + q) + 2] = dep[3 * q + 2];
+      }
+    }
+    V.rec[t] = make_uint2(off, ndep | 0x80000000u);
+  }
+  wave_count(&stats[SGR_STAT_NODES_C], cc.nodes);
+  wave_count(&stats[SGR_STAT_TESTS_C], cc.tests);
+  wave_count(&stats[SGR_STAT_FETCH_C], cc.fetch);
+  wave_count(&stats[SGR_STAT_ISECT], rays);
+  wave_count(&stats[SGR_STAT_MEDIA_OVF], movf);
+}
+
+// PhotonInteg::Lo_fgather_samp (render/photon-integ.cc:268-380) without
+// recursion: the photon maps' radiance at the surface the sample BS from
+// IS reaches, then up to 3 levels of specular bounces from there (their
+// parameters from the sample's sequential stream), each level's
+// radiance weighted by the Li -> Lo factors above it; past level 0 the
+// surface's emission, or the background on a miss, counts times SCALE.
+// The media are the shading point's throughout (M0, M1).
+template<bool INST>
+__device__ __forceinline__ C3 fg_samp(const SceneView &S, const RenderView &R, Isec is, BSample bs,
+                                      C3 scale, float m0, float m1, uint64_t qk, uint32_t &nd,
+                                      PhotonHeap &H, TraceCount &cc, uint32_t &rays, LdsStack stk,
+                                      uint2 *spill, Nest N) {
+  C3 rad = c3(0.0f), W = c3(1.0f);
+  for (uint32_t depth = 0;; depth++) {
+    if (!(cgt0(bs.val) && bs.pdf != 0.0f)) break;
+    TravState T;
+    T.ray = dint_ray(S, is, bs.dir);
+    const RayF ray = T.ray;
+    trace_one<false, INST>(S, T, cc, stk, spill, N);
+    rays++;
+    if (T.hit == SGR_MISS) {
+      if (depth != 0) rad = rad + (background(S, ray.d) * scale) * W;
+      break;
+    }
+    RayF hr = ray;
+    hr.t1 = T.ray.t1;
+    Isec si;
+    make_isec<INST>(S, T.hit, T.hinst, hr, S.tri_vn ? T.hu : 0.0f, S.tri_vn ? T.hv : 0.0f, m0, m1,
+                    si);
+    bool more = false;
+    BSample next;
+    C3 lil;
+    if (si.bmat >= 0) {
+      const DevMaterial &SM = S.mats[si.bmat];
+      const C3 Li = (photon_lo(si, SM, R.ph.set[0], R.ph.k, R.ph.r2, F_ALL, H) +
+                     photon_lo(si, SM, R.ph.set[2], R.ph.k, R.ph.r2, F_ALL, H)) +
+                    photon_lo(si, SM, R.ph.set[1], R.ph.k, R.ph.r2, F_ALL, H);
+      lil = (bs.val * fabsf(bs.dir.z)) * (1.0f / bs.pdf);
+      rad = rad + (Li * lil) * W;
+      const unsigned spec = F_ALL_DIRECTIONS | F_SPECULAR;
+      if (bsdf_supports_any(SM, spec) && depth < 3) {
+        const float u = draw(qk, SGR_DINT_RAND + nd++);
+        const float v = draw(qk, SGR_DINT_RAND + nd++);
+        next = bsdf_sample(si, SM, u, v, spec);
+        more = true;
+      }
+    }
+    if (depth != 0) {
+      const DevMaterial &HM = S.mats[si.mat];
+      if (HM.type == 5 && !si.back) rad = rad + (cload(HM.color) * scale) * W;  // Intersect::Le
+    }
+    if (!more) break;
+    W = W * lil;
+    is = si;
+    bs = next;
+  }
+  return rad;
+}
+
+// PhotonInteg::Lo_fgather (render/photon-integ.cc:397-545): samples of
+// the nearby indirect photons' direction histogram and of the BSDF, each
+// through fg_samp, weighted by the power heuristic with sample counts.
+template<bool INST>
+__device__ __forceinline__ C3 fg_gather(const SceneView &S, const RenderView &R, const Isec &is,
+                                        const DevMaterial &M, float m0, float m1, uint64_t pk,
+                                        uint32_t s, uint64_t qk, uint32_t &nd, PhotonHeap &H,
+                                        TraceCount &cc, uint32_t &rays, LdsStack stk, uint2 *spill,
+                                        Nest N) {
+  C3 rad = c3(0.0f);
+  const uint32_t nph = R.ph.fg_photon, nbs = R.ph.fg_bsdf;
+  const bool avoid = R.ph.caustics != 0;
+  DirHistDist D;
+  photon_find(R.ph.set[2], is.nf.o, R.ph.k, R.ph.r2, H);
+  dirhist_build(D, R.ph.set[2], H);
+  const unsigned bflags = bsdf_supports_any(M, F_ALL);
+  for (uint32_t i = 0; i < nph; i++) {
+    float pu, pv, spdf;
+    strat_uv(pk, 1 + 2 * R.L, s * R.ph.nsub_fgp + i, R.ph.g_fgp, pu, pv);
+    const V3 bdir = is.nf.to(dirhist_sample(D, pu, pv, spdf));
+    const BValue bv = bsdf_eval(is, M, bdir, F_ALL & ~F_SPECULAR);
+    if (bv.pdf != 0.0f && spdf != 0.0f) {
+      C3 scale = c3(1.0f);
+      if (avoid) {
+        if (bflags & (F_SPECULAR | F_GLOSSY | F_DIFFUSE) & ~(F_SPECULAR | F_DIFFUSE))
+          scale = cdiv(bsdf_eval(is, M, bdir, F_ALL & ~(F_SPECULAR | F_DIFFUSE)).val, bv.val);
+        else
+          scale = c3(0.0f);
+      }
+      BSample fake;
+      fake.val = bv.val;
+      fake.pdf = spdf;
+      fake.dir = bdir;
+      fake.flags = 0;
+      rad = rad + fg_samp<INST>(S, R, is, fake, scale, m0, m1, qk, nd, H, cc, rays, stk, spill, N) *
+                      mis_sample_weight(spdf, (float)nph, bv.pdf, (float)nbs);
+    }
+  }
+  for (uint32_t i = 0; i < nbs; i++) {
+    float bu, bv;
+    strat_uv(pk, 2 + 2 * R.L, s * R.ph.nsub_fgb + i, R.ph.g_fgb, bu, bv);
+    const BSample bs = bsdf_sample(is, M, bu, bv, F_ALL & ~F_SPECULAR);
+    if (bs.pdf != 0.0f) {
+      const float ph_pdf = dirhist_pdf(D, is.nf.from(bs.dir));
+      const C3 scale = (avoid && (bs.flags & F_DIFFUSE)) ? c3(0.0f) : c3(1.0f);
+      rad = rad + fg_samp<INST>(S, R, is, bs, scale, m0, m1, qk, nd, H, cc, rays, stk, spill, N) *
+                      mis_sample_weight(bs.pdf, (float)nbs, ph_pdf, (float)nph);
+    }
+  }
+  return rad;
+}
+
+// PHOTON: the photon integrator (render/photon-integ.cc:558-600), whose
+// Lo adds caustic photons and final gathering (or the indirect map) to
+// the direct lighting (or the direct map).
+template<bool INST, bool PHOTON>
 __global__ void __launch_bounds__(256) k_direct_integ(SceneView S, RenderView R, PathBufs P,
                                                       unsigned long long *stats) {
   SGR_STACK_LDS(stk);
@@ -859,6 +1112,7 @@ __global__ void __launch_bounds__(256) k_direct_integ(SceneView S, RenderView R,
   uint2 spill2[INST ? SGR_MAX_DEPTH : 1];
   N.spill = spill2;
   DintBranch pend[SGR_DIRECT_MAX_DEPTH];
+  PhotonHeap H;
   TraceCount cc{0, 0, 0, 0}, cs{0, 0, 0, 0};
   uint32_t illum = 0, rays = 0, shadow = 0, ovf = 0;
   const uint32_t nsub = R.nsub_dint;
@@ -905,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_direct_integ(SceneView S, RenderView R,
             const DevMaterial &M = S.mats[is.bmat];
             illum++;
             C3 D = c3(0.0f);
-            for (uint32_t l = 0; l < R.L; l++) {
+            for (uint32_t l = 0; l < ((!PHOTON || R.ph.direct_illum) ? R.L : 0u); l++) {
               const DevLight &L = S.lights[l];
               C3 lrad = c3(0.0f);
               for (uint32_t jj = 0; jj < nsub; jj++) {
@@ -927,6 +1181,18 @@ __global__ void __launch_bounds__(256) k_direct_integ(SceneView S, RenderView R,
                 lrad = lrad + r;
               }
               D = D + lrad * (1.0f / (float)nsub);  // Color / n: times 1 / n
+            }
+            if (PHOTON) {
+              if (!R.ph.direct_illum) D = photon_lo(is, M, R.ph.set[0], R.ph.k, R.ph.r2, F_ALL, H);
+              const bool fg = R.ph.fg > 0;
+              if (R.ph.caustics)
+                D = D + photon_lo(is, M, R.ph.set[1], R.ph.k, R.ph.r2,
+                                  fg ? (F_ALL_DIRECTIONS | F_DIFFUSE) : F_ALL, H);
+              if (fg)
+                D = D + fg_gather<INST>(S, R, is, M, cur.m0, cur.m1, pk, s, qk, nd, H, cc, rays, stk,
+                                        spill, N);
+              else
+                D = D + photon_lo(is, M, R.ph.set[2], R.ph.k, R.ph.r2, F_ALL, H);
             }
             total = total + D * cur.W;
             const BSample xs = bsdf_sample(is, M, 0.0f, 0.0f, F_SPECULAR | F_TRANSMISSIVE);
@@ -1203,10 +1469,22 @@ void launch_extend(bool multi, const SceneView &S, const StepView &J, const Path
 }
 void launch_direct_integ(const SceneView &S, const RenderView &R, const PathBufs &P,
                          unsigned long long *stats, unsigned grid, hipStream_t st) {
-  if (general_walk(S, true))
-    hipLaunchKernelGGL((k_direct_integ<true>), dim3(grid), dim3(256), 0, st, S, R, P, stats);
+  const bool gen = general_walk(S, true), ph = R.integ == SGR_INTEG_PHOTON;
+  if (gen && ph)
+    hipLaunchKernelGGL((k_direct_integ<true, true>), dim3(grid), dim3(256), 0, st, S, R, P, stats);
+  else if (ph)
+    hipLaunchKernelGGL((k_direct_integ<false, true>), dim3(grid), dim3(256), 0, st, S, R, P, stats);
+  else if (gen)
+    hipLaunchKernelGGL((k_direct_integ<true, false>), dim3(grid), dim3(256), 0, st, S, R, P, stats);
   else
-    hipLaunchKernelGGL((k_direct_integ<false>), dim3(grid), dim3(256), 0, st, S, R, P, stats);
+    hipLaunchKernelGGL((k_direct_integ<false, false>), dim3(grid), dim3(256), 0, st, S, R, P, stats);
+}
+void launch_photon_shoot(const SceneView &S, const PhotonShootView &V, unsigned long long *stats,
+                         unsigned grid, hipStream_t st) {
+  if (general_walk(S, false))
+    hipLaunchKernelGGL((k_photon_shoot<true>), dim3(grid), dim3(256), 0, st, S, V, stats);
+  else
+    hipLaunchKernelGGL((k_photon_shoot<false>), dim3(grid), dim3(256), 0, st, S, V, stats);
 }
 void launch_shade(bool multi, const SceneView &S, const RenderView &R, const StepView &J,
                   const PathBufs &P, uint32_t b, unsigned long long *stats, unsigned grid,

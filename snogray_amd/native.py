@@ -80,6 +80,12 @@ def lib():
                                      C.POINTER(C.c_int)]
     L.sgr_mesh_file_copy.argtypes = [vp, abi.f32p, abi.u32p, abi.f32p]
     L.sgr_cylinder_xform.argtypes = [abi.f32p, abi.f32p, C.c_float, abi.f32p]
+    L.sgr_photon_params_default.argtypes = [C.POINTER(abi.PhotonParams)]
+    L.sgr_photon_params_default.restype = None
+    L.sgr_photon_map_create.argtypes = [vp, C.POINTER(abi.PhotonParams), C.POINTER(vp)]
+    L.sgr_photon_map_info_get.argtypes = [vp, C.POINTER(abi.PhotonMapInfo)]
+    L.sgr_photon_map_export.argtypes = [vp, C.c_int, abi.f32p, abi.f32p, abi.f32p]
+    L.sgr_photon_map_destroy.argtypes = [vp]
     L.sgr_mesh_file_close.argtypes = [vp]
     L.sgr_render_noalloc.argtypes = [vp, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams),
                                      abi.f32p, C.POINTER(abi.Stats)]
@@ -174,8 +180,15 @@ class Scene:
 
     def close(self):
         if getattr(self, "h", None):
+            for m in list(getattr(self, "photon_maps", ())):
+                m.close()
             lib().sgr_scene_destroy(self.h)
             self.h = None
+
+    def photon_map(self, params=None):
+        """Shoot this scene's photons and build its maps
+        (sgr_photon_map_create); PARAMS: abi.photon_params(...)."""
+        return PhotonMap(self, params if params is not None else abi.photon_params())
 
     def __del__(self):
         self.close()
@@ -265,6 +278,50 @@ class Scene:
         check(lib().sgr_render_samples(self.h, C.byref(cam), C.byref(params),
                                        abi.fptr(out), C.byref(st)), "sgr_render_samples")
         return out, st.as_dict()
+
+
+class PhotonMap:
+    """Photon maps of a scene (sgr_photon_map_*); pass it to
+    abi.make_params(photon_map=...) to render with the photon integrator."""
+
+    def __init__(self, scene, params):
+        self.scene = scene
+        self.params = params
+        h = C.c_void_p()
+        check(lib().sgr_photon_map_create(scene.h, C.byref(params), C.byref(h)),
+              "sgr_photon_map_create")
+        self.h = h
+        if not hasattr(scene, "photon_maps"):
+            scene.photon_maps = weakref.WeakSet()
+        scene.photon_maps.add(self)
+
+    def handle(self):
+        return self.h.value
+
+    def info(self):
+        i = abi.PhotonMapInfo()
+        check(lib().sgr_photon_map_info_get(self.h, C.byref(i)), "sgr_photon_map_info_get")
+        return i.as_dict()
+
+    def export(self, which):
+        """Set WHICH (0 direct, 1 caustic, 2 indirect) in kd-tree order:
+        (pos_axis (n, 4) float32 with the split axis's bits in column 3,
+        power (n, 3), dir (n, 3))."""
+        n = self.info()["photons"][which]
+        pa = np.zeros((n, 4), np.float32)
+        pw = np.zeros((n, 3), np.float32)
+        dr = np.zeros((n, 3), np.float32)
+        check(lib().sgr_photon_map_export(self.h, which, abi.fptr(pa), abi.fptr(pw), abi.fptr(dr)),
+              "sgr_photon_map_export")
+        return pa, pw, dr
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sgr_photon_map_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Engine:
