@@ -308,8 +308,9 @@ typedef struct sgr_render_params {
 /* Photon mapping (render/photon-integ.cc): the direct integrator's
  * recursion whose Lo adds, besides direct lighting, caustic photons and
  * final gathering over the photon maps params->photon_map holds.  Runs in
- * sgr_render / sgr_render_samples; the job engine, guided and no-alloc
- * renders refuse it. */
+ * sgr_render / sgr_render_samples / sgr_render_noalloc; the job engine
+ * and guided renders refuse it.  Its direct lighting takes the map's
+ * direct_samples per light (light_samples is not read). */
 #define SGR_INTEG_PHOTON  2u
 /* recursion levels the direct integrator keeps per sample; deeper
  * specular branches (reached with probability 2^-26 or less through
