@@ -222,8 +222,10 @@ __global__ void __launch_bounds__(256) k_extend(SceneView S1, StepView J, PathBu
         // (by path the stores scatter over the sparse late queues)
         P.hprim[i] = T.hit;
         P.ht[i] = T.ray.t1;
-        P.hu[i] = T.hu;
-        P.hv[i] = T.hv;
+        if (MULTI || S1.tri_vn) {  // read only to interpolate vertex normals
+          P.hu[i] = T.hu;
+          P.hv[i] = T.hv;
+        }
         if (INST) P.hinst[i] = T.hinst;
       },
       cnt, stk);
@@ -776,8 +778,10 @@ __global__ void __launch_bounds__(256) k_finish(SceneView S1, RenderView R1, Ste
       trace_one<false, INST>(S, T, cc, stk, spill, N);
       P.hprim[j] = T.hit;
       P.ht[j] = T.ray.t1;
-      P.hu[j] = T.hu;
-      P.hv[j] = T.hv;
+      if (S.tri_vn) {  // read only to interpolate vertex normals
+        P.hu[j] = T.hu;
+        P.hv[j] = T.hv;
+      }
       if (INST) P.hinst[j] = T.hinst;
       uint64_t em = 0;  // shadow slots of the path written at this vertex
       const bool cont = shade_path<MULTI, false, INST>(S1, R1, J, P, G, p, j, illum,
