@@ -558,11 +558,14 @@ int sgr_image_recover(const char *path, const char *format, float *accum,
                       uint32_t width, uint32_t height, uint32_t *rows);
 
 /* ---- mesh files (host only) --------------------------------------------
- * PLY (load/mesh/load-ply.cc: vertex x y z [nx ny nz], triangle faces;
- * ASCII and binary) and Wavefront OBJ (load/mesh/obj.lua: v, vn, f with
- * i or i//i indices, polygons fanned into triangles).  format == NULL:
- * from the extension.  Meshes from both are right-handed (left_handed =
- * 0 in the sgr_mesh they fill). */
+ * The mesh loaders of load/load.lua:405-418: PLY (load/mesh/load-ply.cc:
+ * vertex x y z [nx ny nz], triangle faces; ASCII and binary), Wavefront
+ * OBJ (load/mesh/obj.lua: v, vn, f with i or i//i indices, polygons
+ * fanned into triangles), STL (load/mesh/stl.lua: the ASCII "solid ascii"
+ * grammar, equal vertex positions merged), MSH (load/mesh/load-msh.cc:
+ * vertex / triangle / texcoord / normal sections) and UG
+ * (load/mesh/ug.lua: named vertices, named lambert colours, polygons).
+ * format == NULL: from the extension, case-insensitive. */
 typedef struct sgr_mesh_file sgr_mesh_file;
 /* Parse a file; reports the vertex and triangle counts and whether it
  * carries per-vertex normals (any pointer but path and out may be NULL). */
@@ -573,6 +576,16 @@ int sgr_mesh_file_open(const char *path, const char *format, sgr_mesh_file **out
  * per vertex normal) into caller buffers; NULL skips one. */
 int sgr_mesh_file_copy(const sgr_mesh_file *mesh, float *vertices,
                        uint32_t *triangles, float *normals);
+/* Handedness and parts.  *left_handed as the reference's loader leaves
+ * the mesh: 0 for PLY, OBJ and MSH (they set it), 1 for STL and UG (the
+ * Mesh default, surface/mesh.h:55).  Parts in order of first use: a UG
+ * file has one per material (load/mesh/ug.lua:70-84), the others one (no
+ * triangles: none).  tri_parts gets one part index per triangle,
+ * part_colors 3 floats per part: the lambert colour of a UG "c" line, or
+ * -1 for the caller's material.  NULL skips an output. */
+int sgr_mesh_file_parts(const sgr_mesh_file *mesh, int *left_handed,
+                        uint32_t *num_parts, uint32_t *tri_parts,
+                        float *part_colors);
 int sgr_mesh_file_close(sgr_mesh_file *mesh);
 
 /* ---- job engine: many renders sharing one stream of launches ----------

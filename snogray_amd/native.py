@@ -22,7 +22,7 @@ EXPORTS = [
     "sgr_trace_closest", "sgr_trace_occluded", "sgr_render",
     "sgr_render_samples", "sgr_finalize_image", "sgr_render_noalloc",
     "sgr_image_write", "sgr_image_read", "sgr_image_recover",
-    "sgr_mesh_file_open", "sgr_mesh_file_copy", "sgr_mesh_file_close",
+    "sgr_mesh_file_open", "sgr_mesh_file_copy", "sgr_mesh_file_parts", "sgr_mesh_file_close",
     "sgr_engine_configure", "sgr_job_footprint", "sgr_job_submit", "sgr_engine_step",
     "sgr_job_poll", "sgr_job_result", "sgr_job_release", "sgr_engine_stats",
     "sgr_render_batch",
@@ -79,6 +79,8 @@ def lib():
                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_int)]
     L.sgr_mesh_file_copy.argtypes = [vp, abi.f32p, abi.u32p, abi.f32p]
+    L.sgr_mesh_file_parts.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_uint32),
+                                      abi.u32p, abi.f32p]
     L.sgr_cylinder_xform.argtypes = [abi.f32p, abi.f32p, C.c_float, abi.f32p]
     L.sgr_photon_params_default.argtypes = [C.POINTER(abi.PhotonParams)]
     L.sgr_photon_params_default.restype = None
@@ -540,7 +542,16 @@ def cylinder_xform(base, axis, radius):
 
 def load_mesh(path, fmt=None):
     """(vertices (V, 3) float32, triangles (T, 3) uint32, normals (V, 3)
-    float32 or None) of a PLY or OBJ file."""
+    float32 or None) of a PLY, OBJ, STL, MSH or UG file."""
+    m = load_mesh_file(path, fmt)
+    return m["v"], m["t"], m["n"]
+
+
+def load_mesh_file(path, fmt=None):
+    """A mesh file as a dict: v (V, 3) float32, t (T, 3) uint32, n (V, 3)
+    float32 or None, left_handed (as the reference's loader leaves the
+    mesh), tri_parts (T,) uint32 and part_colors (P, 3) float32 (a UG
+    material's lambert colour, or -1 for the caller's material)."""
     h = C.c_void_p()
     nv, nt, hn = C.c_uint64(), C.c_uint64(), C.c_int()
     check(lib().sgr_mesh_file_open(str(path).encode(), _fmt(fmt), C.byref(h), C.byref(nv),
@@ -552,6 +563,13 @@ def load_mesh(path, fmt=None):
         check(lib().sgr_mesh_file_copy(h, abi.fptr(v), abi.uptr(t),
                                        abi.fptr(n) if n is not None else None),
               "sgr_mesh_file_copy")
+        lh, npart = C.c_int(), C.c_uint32()
+        check(lib().sgr_mesh_file_parts(h, C.byref(lh), C.byref(npart), None, None),
+              "sgr_mesh_file_parts")
+        tp = np.zeros(nt.value, np.uint32)
+        pc = np.zeros((npart.value, 3), np.float32)
+        check(lib().sgr_mesh_file_parts(h, None, None, abi.uptr(tp), abi.fptr(pc)),
+              "sgr_mesh_file_parts")
     finally:
         lib().sgr_mesh_file_close(h)
-    return v, t, n
+    return dict(v=v, t=t, n=n, left_handed=bool(lh.value), tri_parts=tp, part_colors=pc)

@@ -99,12 +99,20 @@ class SceneDesc:
         return len(self.meshes) - 1
 
     def mesh_file(self, path, material, axis=(0.0, 0.0, 1.0), fmt=None):
-        """A PLY or OBJ mesh file (with its vertex normals, if any) as one
-        mesh; right-handed, as the reference's loaders make it
-        (load/mesh/load-ply.cc, load/mesh/obj.lua)."""
+        """A PLY, OBJ, STL, MSH or UG mesh file (with its vertex normals, if
+        any) as one mesh, handed as the reference's loader leaves it
+        (load/load.lua:405-418).  A UG file's named colours become lambert
+        materials, one per part (load/mesh/ug.lua:45-84); every other
+        triangle takes MATERIAL."""
         from . import native
-        v, t, n = native.load_mesh(path, fmt)
-        return self.mesh(v, t, material, axis=axis, left_handed=False, normals=n)
+        f = native.load_mesh_file(path, fmt)
+        tm = None
+        if len(f["part_colors"]) > 1 or (len(f["part_colors"]) and f["part_colors"][0][0] >= 0):
+            mats = np.array([material if c[0] < 0 else self.lambert(tuple(map(float, c)))
+                             for c in f["part_colors"]], np.uint32)
+            tm = mats[f["tri_parts"]]
+        return self.mesh(f["v"], f["t"], material, tri_materials=tm, axis=axis,
+                         left_handed=f["left_handed"], normals=f["n"])
 
     def model(self):
         """A new model: add geometry to the returned SceneDesc (its

@@ -7,7 +7,7 @@ import struct
 import numpy as np
 import pytest
 
-from snogray_amd import native, scenes
+from snogray_amd import abi, native, scenes
 
 V = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0.5)], np.float32)
 N = np.array([(0, 0, 1), (0.1, 0, 1), (0, 0.2, 1), (-0.3, 0, 1)], np.float32)
@@ -129,3 +129,189 @@ def test_loaded_mesh_builds_the_same_octree(tmp_path):
         assert (m["vn"] is not None) == (path == ply)
         n1, s1, o1, _ = native.octree_build(sd)
         assert np.array_equal(n0, n1) and np.array_equal(s0, s1) and np.array_equal(o0, o1)
+
+
+# ---- STL (load/mesh/stl.lua) --------------------------------------------
+
+def _first_use_dedup(pts):
+    """Unique rows of PTS in order of first occurrence, and each row's index
+    among them (numpy's unique, reordered): what a vertex group gives."""
+    u, first, inv = np.unique(pts, axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first)
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    return u[order], rank[inv.ravel()]
+
+
+def test_stl_merges_vertices_and_fans_facets(tmp_path):
+    facets = [[(0, 0, 0), (1, 0, 0), (1, 1, 0)],
+              [(0, 0, 0), (1, 1, 0), (0, 1, 0.5), (-1, 0.5, 0.25)],   # 4 vertices: a fan
+              [(1, 0, 0), (2, 0, 0), (1, 1, 0)]]
+    text = "solid ascii\n"
+    for k, f in enumerate(facets):
+        text += "  facet normal 0 0 1\n    outer loop\n"
+        text += "".join(f"      vertex {x!r} {y!r} {z!r}\n" for x, y, z in f)
+        text += "    endloop\n  endfacet\n" if k % 2 else "    end loop\n  end facet\n"
+    text += "endsolid\n"
+    # a second solid shares the file's vertex group; -0.0 equals 0.0
+    text += "solid  ascii\nfacet\nouter loop\nvertex -0.0 0 0\nvertex 2e0 0. +.0\nvertex 1 1 0\n"
+    text += "endloop\nendfacet\nend solid\n"
+    p = tmp_path / "m.stl"
+    p.write_text(text)
+    m = native.load_mesh_file(p)
+    pts = np.array([v for f in facets for v in f] + [(0, 0, 0), (2, 0, 0), (1, 1, 0)], np.float32)
+    uv, idx = _first_use_dedup(pts)
+    np.testing.assert_array_equal(m["v"], uv)
+    fi = np.cumsum([0] + [len(f) for f in facets] + [3])
+    tris = []
+    for a, b in zip(fi[:-1], fi[1:]):
+        ids = idx[a:b]
+        tris += [(ids[0], ids[k - 1], ids[k]) for k in range(2, len(ids))]
+    np.testing.assert_array_equal(m["t"], tris)
+    assert len(m["v"]) == 6 and m["n"] is None
+    assert m["left_handed"] is True      # the loader leaves the Mesh default
+    assert len(m["part_colors"]) == 1 and m["part_colors"][0][0] == -1
+
+
+def test_stl_errors(tmp_path):
+    p = tmp_path / "e.stl"
+    good_facet = "facet normal 0 0 1\nouter loop\nvertex 0 0 0\nvertex 1 0 0\nvertex 0 1 0\nendloop\nendfacet\n"
+    for body, msg in (("solid cube\n" + good_facet + "endsolid\n", "line 1: parse error"),
+                      ("solid ascii\n" + good_facet + "endsolid ascii\n", "line 9: parse error"),
+                      ("solid ascii\nfacet\nouter loop\nvertex 0 0 0\nvertex 1 0 0\nendloop\n"
+                       "endfacet\nendsolid\n", "not enough vertices in facet"),
+                      ("solid ascii\nfacet\nouter loop\nvertex 0 0 1E5\n", "line 4: parse error"),
+                      ("solid ascii\nfacet\nouter loop\nvertex 0 0\n1 endloop\n", "parse error")):
+        p.write_text(body)
+        with pytest.raises(native.SgrError, match=msg):
+            native.load_mesh(p)
+    p.write_text("")
+    v, t, n = native.load_mesh(p)
+    assert len(v) == 0 and len(t) == 0
+
+
+# ---- MSH (load/mesh/load-msh.cc) ----------------------------------------
+
+def test_msh_sections_texcoords_and_normals(tmp_path):
+    n1 = np.array([(0, 0, 2), (0.3, 0, 1), (0, 0.2, 3)], np.float32)
+    n2 = np.array([(1, 1, 1), (0, -4, 0), (0.1, 0.2, 0.3), (3, 4, 0)], np.float32)
+    text = ("3 1\nvertices\n0 0 0\n1 0 0\n0 1 0\ntriangles\n0 1 2\n"
+            "texcoords\n0 0\n1 0\n0 1\nnormals\n" + "".join(" ".join(map(repr, r.tolist())) + "\n" for r in n1) +
+            "4 2 vertices 0 0 1 1 0 1 1 1 1 0 1 1 triangles 0 1 2 0 2 3 normals\n" +
+            " ".join(map(repr, n2.ravel().tolist())))        # no final newline
+    p = tmp_path / "m.msh"
+    p.write_text(text)
+    m = native.load_mesh_file(p)
+    np.testing.assert_array_equal(m["v"], [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1),
+                                           (1, 1, 1), (0, 1, 1)])
+    # the second section's indices count from its first vertex
+    np.testing.assert_array_equal(m["t"], [(0, 1, 2), (3, 4, 5), (3, 5, 6)])
+    nn = np.concatenate([n1, n2])
+    length = np.sqrt(nn[:, 0] * nn[:, 0] + nn[:, 1] * nn[:, 1] + nn[:, 2] * nn[:, 2])
+    want = nn * (np.float32(1) / length)[:, None]
+    np.testing.assert_array_equal(m["n"], want)
+    np.testing.assert_allclose(np.linalg.norm(m["n"], axis=1), 1, atol=1e-6)
+    assert m["left_handed"] is False
+
+
+def test_msh_errors(tmp_path):
+    p = tmp_path / "e.msh"
+    for body, msg in (("3 1 verts 0 0 0 1 0 0 0 1 0 triangles 0 1 2\n", "expected vertices"),
+                      ("3 1 vertices 0 0 0 1 0 0 0 1 triangles 0 1 2\n", "bad format"),
+                      ("3 1 vertices 0 0 0 1 0 0 0 1 0 triangles 0 1 5\n", "out of range"),
+                      ("3 1 vertices 0 0 0 1 0 0 0 1 0 triangles 0 1 2 normals 0 0 1 0 0 1 0 0 1\n"
+                       "3 1 vertices 0 0 0 1 0 0 0 1 0 triangles 0 1 2\n", "some sections only"),
+                      ("", "empty")):
+        p.write_text(body)
+        with pytest.raises(native.SgrError, match=msg):
+            native.load_mesh(p)
+
+
+# ---- UG (load/mesh/ug.lua) ----------------------------------------------
+
+UG = """{ a ug file }
+c red 1 0 0;
+v a 0 0 0;
+v b 1 0 0;  { trailing comment }
+v c 1 1 0;
+v d 0 1 0;
+v e 2 1-1;
+f (a b c);
+f (a c d e) red;
+{ a comment
+  over two lines }
+c{inline}blue 0 0 1;
+f ( b e{x}c ) blue;
+c red 0.5 0.25 0.125;
+f (a b d) red;
+f (c d e) blue;
+
+"""
+
+
+def test_ug_parts_materials_and_fans(tmp_path):
+    p = tmp_path / "m.ug"
+    p.write_text(UG)
+    m = native.load_mesh_file(p)
+    np.testing.assert_array_equal(m["v"], [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (2, 1, -1)])
+    # (a c d e) red fans into (a c d) (a d e)
+    np.testing.assert_array_equal(m["t"], [(0, 1, 2), (0, 2, 3), (0, 3, 4), (1, 4, 2), (0, 1, 3),
+                                           (2, 3, 4)])
+    # parts in order of first use: default, red, blue, the redefined red
+    np.testing.assert_array_equal(m["tri_parts"], [0, 1, 1, 2, 3, 2])
+    np.testing.assert_array_equal(m["part_colors"], [(-1, -1, -1), (1, 0, 0), (0, 0, 1),
+                                                     (0.5, 0.25, 0.125)])
+    assert m["left_handed"] is True
+    sd = scenes.SceneDesc()
+    grey = sd.lambert((0.5, 0.5, 0.5))
+    sd.mesh_file(p, grey)
+    tm = sd.meshes[0]["tm"]
+    assert tm[0] == grey and len(set(tm.tolist())) == 4
+    for k, c in ((1, (1, 0, 0)), (3, (0, 0, 1)), (4, (0.5, 0.25, 0.125))):
+        mat = sd.materials[tm[k]]
+        assert mat["type"] == abi.SGR_MAT_LAMBERT and tuple(mat["color"]) == c
+
+
+def test_ug_errors(tmp_path):
+    p = tmp_path / "e.ug"
+    for body, msg in (("v a 0 0 0;\nv b 1 0 0;\nv c 0 1 0;\nv d 1 1 0;\nf (a b c d);\n",
+                       'line 5: Unknown material name "d"'),
+                      ("v a 0 0 0;\nf (a b c);\n", 'line 2: Unknown vertex name "b"'),
+                      ("v a 0 0 0;", "line 1: parse error"),            # no final newline
+                      ("v a 0 0 0; v b 1 0 0;\n", "line 1: parse error"),
+                      ("v a 0 0 0;\n  v b 1 0 0;\n", "line 2: parse error"),  # commands start a line
+                      ("{c} v a 0 0 0;\n", "line 1: parse error"),
+                      ("w 1 2 3;\n", "line 1: parse error"),
+                      ("v a 0 0;\n", "line 1: parse error"),
+                      ("f (a b);\n", "line 1: parse error")):
+        p.write_text(body)
+        with pytest.raises(native.SgrError, match=msg):
+            native.load_mesh(p)
+
+
+def test_stl_loaded_mesh_builds_the_same_octree(tmp_path):
+    """An icosphere written as ASCII STL: the merged vertices are the
+    sphere's own (each appears in a facet before any repeat is read), and
+    the octree is the in-memory mesh's, left-handed as the loader leaves
+    it."""
+    v, t = scenes.icosphere(2)
+    v = np.asarray(v, np.float32)
+    order = np.unique(t.ravel(), return_index=True)[1]
+    first = t.ravel()[np.sort(order)]          # vertices in order of first use
+    remap = np.empty(len(v), np.int64)
+    remap[first] = np.arange(len(first))
+    stl = tmp_path / "s.stl"
+    stl.write_text("solid ascii\n" + "".join(
+        "facet normal 0 0 0\nouter loop\n" +
+        "".join(f"vertex {a!r} {b!r} {c!r}\n" for a, b, c in v[tri].tolist()) +
+        "endloop\nendfacet\n" for tri in t) + "endsolid\n")
+    ref = scenes.SceneDesc()
+    ref.mesh(v[first], remap[t], ref.lambert((0.5, 0.5, 0.5)), left_handed=True)
+    sd = scenes.SceneDesc()
+    sd.mesh_file(stl, sd.lambert((0.5, 0.5, 0.5)))
+    np.testing.assert_array_equal(sd.meshes[0]["v"], v[first])
+    np.testing.assert_array_equal(sd.meshes[0]["t"], remap[t])
+    assert sd.meshes[0]["left_handed"] is True and sd.meshes[0]["tm"] is None
+    n0, s0, o0, _ = native.octree_build(ref)
+    n1, s1, o1, _ = native.octree_build(sd)
+    assert np.array_equal(n0, n1) and np.array_equal(s0, s1) and np.array_equal(o0, o1)
