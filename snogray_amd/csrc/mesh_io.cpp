@@ -18,7 +18,7 @@
 //       relative to the section's first vertex, normals made unit length.
 //   UG (load/mesh/ug.lua): "v name x y z;", "c name r g b;" (a lambert
 //       material) and "f (v1 v2 v3 ...) [material];" lines, {comments};
-//       one part per material in order of first use.
+//       one part per material in order of first use (ug.lua:75-109).
 // PLY, OBJ and MSH mark the mesh right-handed (mesh.left_handed = false);
 // the STL and UG loaders leave the Mesh default, left-handed
 // (surface/mesh.h:55).

@@ -102,7 +102,7 @@ class SceneDesc:
         """A PLY, OBJ, STL, MSH or UG mesh file (with its vertex normals, if
         any) as one mesh, handed as the reference's loader leaves it
         (load/load.lua:405-418).  A UG file's named colours become lambert
-        materials, one per part (load/mesh/ug.lua:45-84); every other
+        materials, one per part (load/mesh/ug.lua:55-90); every other
         triangle takes MATERIAL."""
         from . import native
         f = native.load_mesh_file(path, fmt)
