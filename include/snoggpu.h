@@ -579,7 +579,7 @@ int sgr_mesh_file_copy(const sgr_mesh_file *mesh, float *vertices,
 /* Handedness and parts.  *left_handed as the reference's loader leaves
  * the mesh: 0 for PLY, OBJ and MSH (they set it), 1 for STL and UG (the
  * Mesh default, surface/mesh.h:55).  Parts in order of first use: a UG
- * file has one per material (load/mesh/ug.lua:70-84), the others one (no
+ * file has one per material (load/mesh/ug.lua:75-90), the others one (no
  * triangles: none).  tri_parts gets one part index per triangle,
  * part_colors 3 floats per part: the lambert colour of a UG "c" line, or
  * -1 for the caller's material.  NULL skips an output. */
