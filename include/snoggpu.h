@@ -163,6 +163,23 @@ typedef struct sgr_local_prim {
     uint32_t material;
 } sgr_local_prim;
 
+/* Point light (light/point-light.h:25-47, light/light.lua:48-56): radiates
+ * `intensity` from `pos`.  angle <= 0: in all directions (the two-argument
+ * constructor).  angle > 0: a spotlight, a cone of apex `angle` (radians)
+ * about `dir` (used as given, not normalised: it is the frame's z axis)
+ * whose outer `fringe_angle` ramps the intensity down as d^4 toward the
+ * cone's edge.  A point light is never hit by a ray: direct lighting
+ * takes its light sample without multiple-importance weighting and no
+ * BSDF sample (render/direct-illum.cc:174, 202).  Scenes with point
+ * lights take the general kernels. */
+typedef struct sgr_point_light {
+    float pos[3];
+    float intensity[3];
+    float angle;
+    float dir[3];
+    float fringe_angle;
+} sgr_point_light;
+
 /* Lat-long environment map light (light/envmap-light.cc,
  * texture/spheremap.h).  texels: RGB float, row-major, row 0 = top
  * (north).  frame: 3x3 row-major, rows are the light frame's x, y, z
@@ -232,6 +249,11 @@ typedef struct sgr_scene_desc {
     uint32_t            num_ellipses;
     const sgr_local_prim *local_prims;
     uint32_t            num_local_prims;
+    /* lights after the surface lights and the envmap, in this order
+     * (surface/surface-group.cc:82-94 adds surfaces' lights before the
+     * group's own, those in the order added) */
+    const sgr_point_light *point_lights;
+    uint32_t            num_point_lights;
 } sgr_scene_desc;
 
 /* Rays and hits, structure-of-arrays (geometry/ray.h:84-89).  Pointers

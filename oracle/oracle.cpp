@@ -294,7 +294,7 @@ template<class R> struct Tracer {
         C3<R> trans(1);
         if (!occluded(ray, is.media_ior, trans)) {
           C3<R> r = ls.val * trans;
-          r *= mis_weight(ls.pdf, bval.pdf);
+          if (L.kind != 6) r *= mis_weight(ls.pdf, bval.pdf);  // not for a point light
           r *= bval.val;
           r *= std::fabs(is.cos_n(ls.dir));
           r /= ls.pdf;
@@ -302,6 +302,7 @@ template<class R> struct Tracer {
         }
       }
     }
+    if (L.kind == 6) return rad;  // no BSDF sample for a point light (direct-illum.cc:202)
     BsdfSample<R> bs = bsdf.sample(bu, bv, flags);
     if (bs.pdf > 0 && bs.val.gt0()) {
       LightSample<R> lval = sc.light_eval(L, is, bs.dir);

@@ -701,12 +701,27 @@ template<class R> struct FreeSample { C3<R> val; R pdf = 0; V3<R> pos, dir; };
 template<class R> struct Light {
   // 0 = tripar area light, 1 = sphere light, 2 = envmap; surface lights
   // (light/surface-light-sampler.cc) of 3 = an ellipse, 4 = a cylinder,
-  // 5 = a sphere2
+  // 5 = a sphere2; 6 = a point light (light/point-light.h)
   int kind;
   unsigned prim;
   int env;
   C3<R> intensity;
+  // point light: position, frame axes, cosines of the half cone and
+  // half core angles (-1: all directions)
+  V3<R> pos, fx, fy, fz;
+  R cos_half = -1, cos_core = -1;
 };
+
+// PointLight::intensity (light/point-light.h:64-79)
+template<class R>
+static inline C3<R> point_intensity(const Light<R> &L, R cos_dir) {
+  C3<R> intens = L.intensity;
+  if (cos_dir < L.cos_core) {
+    const R d = (cos_dir - L.cos_half) / (L.cos_core - L.cos_half);
+    intens *= d * d * d * d;
+  }
+  return intens;
+}
 
 // Inverse of an affine 3 x 4 transform: linear part by cofactors in
 // double, rounded to float (the scene's instances keep both).
@@ -1113,6 +1128,30 @@ template<class R> struct Scene {
       L.env = (int)e;
       lights.push_back(L);
     }
+    // point lights after the surface lights and the envmap
+    // (surface/surface-group.cc:82-94); PointLight (light/point-light.h:
+    // 31-47) with Frame (pos, dir) (geometry/frame.h:79-84) for a cone
+    for (unsigned i = 0; i < d.num_point_lights; i++) {
+      const sgr_point_light &pl = d.point_lights[i];
+      Light<R> L;
+      L.kind = 6;
+      L.prim = i;
+      L.env = -1;
+      L.intensity = C3<R>(pl.intensity[0], pl.intensity[1], pl.intensity[2]);
+      L.pos = V3<R>(pl.pos[0], pl.pos[1], pl.pos[2]);
+      L.fx = V3<R>(1, 0, 0);
+      L.fy = V3<R>(0, 1, 0);
+      L.fz = V3<R>(0, 0, 1);
+      if (pl.angle > 0) {
+        const Frame<R> f = frame_of(L.pos, V3<R>(pl.dir[0], pl.dir[1], pl.dir[2]));
+        L.fx = f.x;
+        L.fy = f.y;
+        L.fz = f.z;
+        L.cos_half = std::cos(R(pl.angle) / 2);
+        L.cos_core = std::cos((R(pl.angle) - R(pl.fringe_angle)) / 2);
+      }
+      lights.push_back(L);
+    }
   }
 
   // Scene::background (render/scene.cc:62-73)
@@ -1313,6 +1352,20 @@ template<class R> struct Scene {
           }
         }
       }
+    } else if (L.kind == 6) {
+      // PointLight::Sampler::sample (light/point-light.cc:79-111)
+      const V3<R> lvec = is.nf.to(L.pos - is.nf.origin);
+      if (is.cos_n(lvec) > 0 && is.cos_geom_n(lvec) > 0) {
+        const R dist = lvec.length();
+        const R inv_dist = R(1) / dist;
+        const R cos_dir = dot((is.nf.origin - L.pos) * inv_dist, L.fz);
+        if (cos_dir >= L.cos_half) {
+          out.val = point_intensity(L, cos_dir) * inv_dist * inv_dist;
+          out.pdf = 1;
+          out.dir = lvec * inv_dist;
+          out.dist = dist;
+        }
+      }
     } else if (L.kind >= 3) {
       // SurfaceLightSampler::sample (light/surface-light-sampler.cc:
       // 47-64) over the surface's own sampler
@@ -1497,6 +1550,20 @@ template<class R> struct Scene {
   // passes it).
   FreeSample<R> light_free_sample(const Light<R> &L, R pu, R pv, R du, R dv) const {
     FreeSample<R> out;
+    if (L.kind == 6) {
+      // PointLight::Sampler::sample (light/point-light.cc:115-126) with
+      // cone_sample / cone_sample_pdf (geometry/cone-sample.h:29-77)
+      const R z = L.cos_half + dv * (R(1) - L.cos_half);
+      const R r = std::sqrt(std::fabs(R(1) - z * z));
+      const R phi = du * R(2) * R(PIf);
+      V3<R> dir(r * std::cos(phi), r * std::sin(phi), z);
+      out.pdf = R(0.5f) * R(INV_PIf) / (R(1) - L.cos_half);
+      out.val = point_intensity(L, dir.z);
+      if (L.cos_half != R(-1)) dir = Frame<R>(L.pos, L.fx, L.fy, L.fz).from(dir);
+      out.pos = L.pos;
+      out.dir = dir;
+      return out;
+    }
     if (L.kind == 2) {
       const Envmap<R> &E = envs[L.env];
       float mu, mv, pdf;
@@ -1602,6 +1669,8 @@ template<class R> struct Scene {
           out.dist = dist;
         }
       }
+    } else if (L.kind == 6) {
+      // PointLight::Sampler::eval (light/point-light.cc:130-133): nothing
     } else if (L.kind >= 3) {
       // SurfaceLightSampler::eval (light/surface-light-sampler.cc:
       // 86-100) over the surface's eval_from_viewpoint

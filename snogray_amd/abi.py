@@ -72,6 +72,11 @@ class LocalPrim(C.Structure):
                 ("material", C.c_uint32)]
 
 
+class PointLight(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("intensity", C.c_float * 3), ("angle", C.c_float),
+                ("dir", C.c_float * 3), ("fringe_angle", C.c_float)]
+
+
 class Envmap(C.Structure):
     _fields_ = [("texels", f32p), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("frame", C.c_float * 9)]
@@ -88,7 +93,8 @@ class SceneDescC(C.Structure):
                 ("models", C.c_void_p), ("num_models", C.c_uint32),
                 ("instances", C.c_void_p), ("num_instances", C.c_uint32),
                 ("ellipses", C.POINTER(Ellipse)), ("num_ellipses", C.c_uint32),
-                ("local_prims", C.POINTER(LocalPrim)), ("num_local_prims", C.c_uint32)]
+                ("local_prims", C.POINTER(LocalPrim)), ("num_local_prims", C.c_uint32),
+                ("point_lights", C.POINTER(PointLight)), ("num_point_lights", C.c_uint32)]
 
 
 class Model(C.Structure):

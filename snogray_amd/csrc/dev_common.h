@@ -53,7 +53,7 @@ struct DevMaterial {
 struct DevLight {
   // 0 = tripar area light, 1 = sphere light, 2 = envmap; surface lights
   // (light/surface-light-sampler.cc) of 3 = an ellipse, 4 = a cylinder,
-  // 5 = a sphere2
+  // 5 = a sphere2; 6 = a point light (prim: its row in SceneView::plights)
   int kind;
   uint32_t prim;
   float intensity[3];
@@ -84,6 +84,9 @@ struct SceneView {
   const float4 *tri;         // 3 per triangle-like prim: v0|flags, e1, e2
   const float4 *sph;         // centre, radius
   const float4 *loc;         // local primitives (SGR_LOC_STRIDE per primitive)
+  // point lights, 4 per light: (pos, cos_half_angle), (frame x,
+  // cos_half_core_angle), (frame y, 0), (frame z, 0)
+  const float4 *plights;
   const float4 *tri_vn;      // 3 per mesh triangle: its vertex normals (or null: none)
   const uint32_t *prim_mat;  // material index per prim
   const uint32_t *prim_mesh; // mesh index per triangle (0xffffffff others)

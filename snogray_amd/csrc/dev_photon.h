@@ -58,6 +58,23 @@ __device__ __forceinline__ FreeS light_free_sample(const SceneView &S, const Dev
   FreeS o;
   o.val = c3(0.0f);
   o.pdf = 0.0f;
+  if (EXT && L.kind == 6) {
+    // PointLight::Sampler::sample (light/point-light.cc:115-126): a
+    // direction by cone_sample and its pdf (geometry/cone-sample.h:29-77),
+    // into the light's frame unless it radiates in all directions
+    const float4 *pl = S.plights + 4 * L.prim;
+    const float ch = pl[0].w;
+    const float z = ch + dv * (1.0f - ch);
+    const float r = sqrtf(fabsf(1.0f - z * z));
+    const float phi = du * 2.0f * SGR_PIf;
+    V3 d = v3(r * cosf(phi), r * sinf(phi), z);
+    o.pdf = 0.5f * SGR_INV_PIf / (1.0f - ch);
+    o.val = point_intensity(cload(L.intensity), ch, pl[1].w, d.z);
+    if (ch != -1.0f) d = Frame{ld3(pl[0]), ld3(pl[1]), ld3(pl[2]), ld3(pl[3])}.from(d);
+    o.pos = ld3(pl[0]);
+    o.dir = d;
+    return o;
+  }
   if (L.kind == 2) {
     float mu, mv, pdf;
     env_hist_sample(S, du, dv, mu, mv, pdf);

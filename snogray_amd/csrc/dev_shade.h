@@ -937,8 +937,19 @@ __device__ __forceinline__ LSample surface_light_eval(const float4 *tri,
                                                                uint32_t n_sph_end, DevLight L,
                                                                Frame nf, V3 dir);
 
-// EXT: the scene may hold ellipse, cylinder and sphere2 lights (the
-// general kernels).
+// PointLight::intensity (light/point-light.h:64-79): full inside the
+// core cone, ramped down as d^4 across the fringe.
+__device__ __forceinline__ C3 point_intensity(C3 col, float cos_half, float cos_core,
+                                              float cos_dir) {
+  if (cos_dir < cos_core) {
+    const float d = (cos_dir - cos_half) / (cos_core - cos_half);
+    col = col * (d * d * d * d);
+  }
+  return col;
+}
+
+// EXT: the scene may hold ellipse, cylinder, sphere2 and point lights
+// (the general kernels).
 template<bool EXT>
 __device__ __forceinline__ LSample light_sample(const SceneView &S, const DevLight &L,
                                                 const Isec &is, float u, float v,
@@ -983,6 +994,21 @@ __device__ __forceinline__ LSample light_sample(const SceneView &S, const DevLig
           o.dir = sd;
           o.dist = dist;
         }
+      }
+    }
+  } else if (EXT && L.kind == 6) {  // PointLight::Sampler::sample (light/point-light.cc:79-111)
+    const float4 *pl = S.plights + 4 * L.prim;
+    const V3 lp = ld3(pl[0]);
+    const V3 lvec = is.nf.to(lp - is.nf.o);
+    if (lvec.z > 0.0f && cos_geom(is, lvec) > 0.0f) {
+      const float dist = vlen(lvec);
+      const float inv = 1.0f / dist;
+      const float cos_dir = dot((is.nf.o - lp) * inv, ld3(pl[3]));
+      if (cos_dir >= pl[0].w) {
+        o.val = point_intensity(cload(L.intensity), pl[0].w, pl[1].w, cos_dir) * inv * inv;
+        o.pdf = 1.0f;
+        o.dir = lvec * inv;
+        o.dist = dist;
       }
     }
   } else if (EXT && L.kind >= 3) {
@@ -1041,6 +1067,9 @@ __device__ __forceinline__ LSample light_eval(const SceneView &S, const DevLight
         o.dist = dist;
       }
     }
+  } else if (EXT && L.kind == 6) {
+    // PointLight::Sampler::eval (light/point-light.cc:130-133): no direction
+    // meets a point
   } else if (EXT && L.kind >= 3) {
     o = surface_light_eval(S.tri, S.loc, S.n_sph_end, L, is.nf, dir);
   } else {  // light/envmap-light.cc:150-170
@@ -1129,7 +1158,8 @@ __device__ __forceinline__ float mis_weight(float pdf, float other) {
 // The two terms of DirectIllum::sample_light (render/direct-illum.cc:
 // 126-255): a sample of light L weighted against the BSDF's pdf, and a
 // sample of the BSDF weighted against the light's, with the power
-// heuristic.  A term that can contribute goes to USE(c, ray): its value
+// heuristic; for a point light the light sample unweighted and no BSDF
+// sample (render/direct-illum.cc:174, 202).  A term that can contribute goes to USE(c, ray): its value
 // if unoccluded and the shadow ray that decides it (from the hit point
 // to the light sample, or to the scene horizon for an envmap).
 __device__ __forceinline__ RayF direct_shadow_ray(const SceneView &S, const Isec &is, V3 dir,
@@ -1151,7 +1181,7 @@ __device__ __forceinline__ void direct_light_term(const SceneView &S, const DevL
     const BValue bval = bsdf_eval(is, M, ls.dir, F_ALL & ~F_SPECULAR);
     if (cgt0(bval.val)) {
       C3 c = ls.val;
-      c = c * mis_weight(ls.pdf, bval.pdf);
+      if (!(EXT && L.kind == 6)) c = c * mis_weight(ls.pdf, bval.pdf);
       c = c * bval.val;
       c = c * fabsf(ls.dir.z);
       c = c * (1.0f / ls.pdf);
@@ -1163,6 +1193,7 @@ template<bool EXT, typename Use>
 __device__ __forceinline__ void direct_bsdf_term(const SceneView &S, const DevLight &L,
                                                  const Isec &is, const DevMaterial &M, float bu,
                                                  float bv, Use use) {
+  if (EXT && L.kind == 6) return;
   const BSample bs = bsdf_sample(is, M, bu, bv, F_ALL & ~F_SPECULAR);
   if (bs.pdf > 0.0f && cgt0(bs.val)) {
     const LSample lval = light_eval<EXT>(S, L, is, bs.dir);

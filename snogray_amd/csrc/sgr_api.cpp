@@ -260,6 +260,7 @@ struct HostScene {
   std::vector<uint32_t> prim_mat, prim_mesh, order;
   std::vector<DevMesh> meshes;
   std::vector<DevLight> lights;
+  std::vector<float4> plights;  // 4 per point light (SceneView::plights)
   std::vector<float> row_cum, col_cum;
   std::vector<uint32_t> row_guide, col_guide;
   FlatOctree oct;
@@ -729,9 +730,57 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
     L.intensity[0] = L.intensity[1] = L.intensity[2] = 0;
     H.lights.push_back(L);
   }
+  // point lights (light/point-light.h:31-47): all directions, or a cone
+  // about dir in the frame Frame (pos, dir) (geometry/frame.h:79-84)
+  if (d->num_point_lights && !d->point_lights)
+    return fail(SGR_ERR_INVALID, "sgr_scene_create: null point_lights");
+  for (uint32_t i = 0; i < d->num_point_lights; i++) {
+    const sgr_point_light &pl = d->point_lights[i];
+    const std::string what = "point light " + std::to_string(i);
+    bool finite = std::isfinite(pl.angle) && std::isfinite(pl.fringe_angle);
+    for (int k = 0; k < 3; k++)
+      finite = finite && std::isfinite(pl.pos[k]) && std::isfinite(pl.intensity[k]) &&
+               std::isfinite(pl.dir[k]);
+    if (!finite) return fail(SGR_ERR_INVALID, what + ": not finite");
+    float ch = -1.0f, cc = -1.0f;
+    float fx[3] = {1, 0, 0}, fy[3] = {0, 1, 0}, fz[3] = {0, 0, 1};
+    if (pl.angle > 0.0f) {
+      const float *z = pl.dir;
+      if (!(z[0] * z[0] + z[1] * z[1] + z[2] * z[2] > 0.0f))
+        return fail(SGR_ERR_INVALID, what + ": a spotlight needs a direction");
+      ch = std::cos(pl.angle / 2);
+      cc = std::cos((pl.angle - pl.fringe_angle) / 2);
+      // Vec::perpendicular (geometry/vec.h:122-129), then unit vectors
+      const float nz = z[0] == z[2] ? z[2] : -z[2];
+      float px[3] = {z[1] * z[1] + z[0] * nz, -nz * nz - z[0] * z[1], z[0] * z[0] - z[1] * nz};
+      auto unit = [](float *v) {
+        const float inv = 1 / std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int k = 0; k < 3; k++) v[k] = v[k] * inv;
+      };
+      unit(px);
+      float py[3] = {z[1] * px[2] - z[2] * px[1], z[2] * px[0] - z[0] * px[2],
+                     z[0] * px[1] - z[1] * px[0]};
+      unit(py);
+      for (int k = 0; k < 3; k++) {
+        fx[k] = px[k];
+        fy[k] = py[k];
+        fz[k] = z[k];
+      }
+    }
+    H.plights.push_back(make_float4(pl.pos[0], pl.pos[1], pl.pos[2], ch));
+    H.plights.push_back(make_float4(fx[0], fx[1], fx[2], cc));
+    H.plights.push_back(make_float4(fy[0], fy[1], fy[2], 0.0f));
+    H.plights.push_back(make_float4(fz[0], fz[1], fz[2], 0.0f));
+    DevLight L;
+    L.kind = 6;
+    L.prim = i;
+    for (int c = 0; c < 3; c++) L.intensity[c] = pl.intensity[c];
+    H.lights.push_back(L);
+  }
   V.n_lights = (uint32_t)H.lights.size();
   V.n_insts = n_inst;
-  V.has_ext = (d->num_ellipses || d->num_local_prims) ? 1u : 0u;
+  // the general kernels also carry the point lights' sampling
+  V.has_ext = (d->num_ellipses || d->num_local_prims || d->num_point_lights) ? 1u : 0u;
   for (const auto &m : H.models) V.has_ext |= m->V.has_ext;
   V.has_partial = 0;
   for (uint32_t p = 0; p < n_prims; p++)
@@ -885,6 +934,7 @@ static int scene_create_impl(sgr_ctx *ctx, const sgr_scene_desc *d, sgr_scene **
     if (!rc) rc = upload(ctx, sc, up, H.insts, &V.insts);
   }
   if (!rc) rc = upload(ctx, sc, up, H.lights, &V.lights);
+  if (!rc && !H.plights.empty()) rc = upload(ctx, sc, up, H.plights, &V.plights);
   if (!rc) rc = upload(ctx, sc, up, H.env_tex, &V.env_tex);
   if (!rc) rc = upload(ctx, sc, up, H.row_cum, &V.env_row_cum);
   if (!rc) rc = upload(ctx, sc, up, H.col_cum, &V.env_col_cum);

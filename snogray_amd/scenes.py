@@ -40,6 +40,7 @@ class SceneDesc:
         self.ellipses = []
         self.local_prims = []  # (SGR_LOCAL_* kind, 3x4 local-to-world, material)
         self.envmaps = []
+        self.point_lights = []
         self.models = []       # SceneDesc objects holding geometry only
         self.instances = []    # (model index, 3x4 local-to-world)
         self.min_trace = min_trace
@@ -161,6 +162,15 @@ class SceneDesc:
         3x4 transform."""
         self._local(abi.SGR_LOCAL_SPHERE2, local_to_world, material)
 
+    def point_light(self, pos, intensity, angle=0.0, dir=(0.0, 0.0, 1.0), fringe_angle=0.0):
+        """Point light (light/point-light.h:31-47, light.point in
+        light/light.lua:48-56): INTENSITY from POS in all directions, or
+        with ANGLE > 0 a spotlight of that apex angle about DIR whose outer
+        FRINGE_ANGLE falls off toward the edge."""
+        self.point_lights.append(dict(pos=pos, intensity=intensity, angle=float(angle),
+                                      dir=dir, fringe_angle=float(fringe_angle)))
+        self._c = None
+
     def envmap(self, texels, frame=Y_UP_ENV_FRAME):
         t = np.ascontiguousarray(texels, dtype=np.float32)
         assert t.ndim == 3 and t.shape[2] == 3
@@ -233,6 +243,12 @@ class SceneDesc:
             envs[i].texels = abi.fptr(e["tex"])
             envs[i].height, envs[i].width = e["tex"].shape[0], e["tex"].shape[1]
             envs[i].frame[:] = [float(x) for x in e["frame"]]
+        pls = (abi.PointLight * max(1, len(self.point_lights)))()
+        for i, pl in enumerate(self.point_lights):
+            pls[i].pos[:] = [float(x) for x in pl["pos"]]
+            pls[i].intensity[:] = [float(x) for x in pl["intensity"]]
+            pls[i].dir[:] = [float(x) for x in pl["dir"]]
+            pls[i].angle, pls[i].fringe_angle = pl["angle"], pl["fringe_angle"]
         d = abi.SceneDescC()
         d.meshes, d.num_meshes = meshes, len(self.meshes)
         d.tripars, d.num_tripars = tps, len(self.tripars)
@@ -241,6 +257,7 @@ class SceneDesc:
         d.local_prims, d.num_local_prims = lps, len(self.local_prims)
         d.materials, d.num_materials = mats, len(self.materials)
         d.envmaps, d.num_envmaps = envs, len(self.envmaps)
+        d.point_lights, d.num_point_lights = pls, len(self.point_lights)
         d.min_trace, d.accel = self.min_trace, self.accel
         models = (abi.Model * max(1, len(self.models)))()
         keep_models = []
@@ -260,7 +277,7 @@ class SceneDesc:
         d.num_models = len(self.models)
         d.instances = C.cast(insts, C.c_void_p) if self.instances else None
         d.num_instances = len(self.instances)
-        self._keep = (mats, meshes, tps, sps, els, lps, envs, models, keep_models, insts)
+        self._keep = (mats, meshes, tps, sps, els, lps, envs, pls, models, keep_models, insts)
         self._c = d
         return d
 
