@@ -69,8 +69,11 @@ typedef struct sgr_photon_map sgr_photon_map;
  *   COOK_TORRANCE  color = diffuse colour, gloss_color = specular colour,
  *                  m = Ward roughness, ior_n/ior_k = complex IOR
  *   GLASS          ior_n = IOR of the enclosed medium (no absorption)
- *   MIRROR         gloss_color = reflectance, ior_n/ior_k = IOR,
- *                  color = underlying Lambert colour (all zero: none)
+ *   MIRROR         gloss_color = reflectance, ior_n/ior_k = IOR;
+ *                  underlying = index of the material under the mirror
+ *                  (a Lambert, Cook-Torrance or Phong, through glows;
+ *                  material/mirror.h:37-42), or -1: then color = the
+ *                  underlying Lambert colour (all zero: none)
  *   GLOW           color = emitted radiance; underlying = index of the
  *                  material supplying the BSDF, or -1 for none
  *   PHONG          color = diffuse colour, gloss_color = specular colour,

@@ -360,8 +360,8 @@ template<class R> struct Bsdf {
           s.pdf = 1;
           s.dir = mirror(isec->v, V3<R>(0, 0, 1));
           s.flags = SPECULAR | REFLECTIVE;
-        } else if (has_under) {
-          s = lambert_sample(u, v, flags, color);
+        } else if (has_under) {  // the underlying material's BSDF, or a Lambert
+          s = under ? under->sample(u, v, flags) : lambert_sample(u, v, flags, color);
           R fres_refl = fr.reflectance(isec->cos_n(s.dir));
           C3<R> rr = gloss * fres_refl;
           s.val *= (C3<R>(1) - rr);
@@ -474,7 +474,8 @@ template<class R> struct Bsdf {
       case SGR_MAT_MIRROR: {
         unsigned f = 0;
         if ((limit & REFLECTIVE) && (limit & SPECULAR)) f |= REFLECTIVE | SPECULAR;
-        if (has_under) f |= (limit & REFLECTIVE) ? ((REFLECTIVE | DIFFUSE) & limit) : 0u;
+        if (under) f |= under->supports(limit);
+        else if (has_under) f |= (limit & REFLECTIVE) ? ((REFLECTIVE | DIFFUSE) & limit) : 0u;
         return f;
       }
       case SGR_MAT_PHONG:
@@ -844,6 +845,18 @@ template<class R> struct Scene {
   // material/stencil.cc:140-148).
   void make_bsdf(Bsdf<R> &b, int bm, const Isec<R> &is) const {
     const Material<R> &m = mats[bm];
+    if (m.type == SGR_MAT_MIRROR && m.underlying >= 0) {
+      // Mirror (ior, reflectance, underlying material) (material/mirror.h:
+      // 37-42): the material's BSDF under the mirror, if it has one
+      b.init(m, is, m.ior_n);
+      const int ub = bsdf_material(m.underlying);
+      b.has_under = ub >= 0;
+      if (ub >= 0) {
+        b.under.reset(new Bsdf<R>);
+        make_bsdf(*b.under, ub, is);
+      }
+      return;
+    }
     if (m.type != SGR_MAT_STENCIL) {
       b.init(m, is, m.ior_n);
       return;
@@ -888,6 +901,15 @@ template<class R> struct Scene {
            (bsdf_material(mats[i].underlying) >= 0 &&
             mats[bsdf_material(mats[i].underlying)].type == SGR_MAT_STENCIL)))
         throw std::runtime_error("stencil: the underlying material must exist and not be a stencil");
+    for (unsigned i = 0; i < d.num_materials; i++) {
+      if (mats[i].type != SGR_MAT_MIRROR || mats[i].underlying < 0) continue;
+      if (mats[i].underlying >= (int)d.num_materials || mats[i].underlying == (int)i)
+        throw std::runtime_error("mirror: bad underlying");
+      const int ub = bsdf_material(mats[i].underlying);
+      if (ub >= 0 && mats[ub].type != SGR_MAT_LAMBERT && mats[ub].type != SGR_MAT_COOK_TORRANCE &&
+          mats[ub].type != SGR_MAT_PHONG)
+        throw std::runtime_error("mirror: the underlying material must be a lambert, cook-torrance or phong material");
+    }
     std::vector<BBox<R>> pb;
     for (unsigned mi = 0; mi < d.num_meshes; mi++) {
       const sgr_mesh &m = d.meshes[mi];

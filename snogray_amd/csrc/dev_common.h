@@ -40,10 +40,13 @@ struct DevMaterial {
   // Cook-Torrance constants (material/cook-torrance.cc:38-70)
   float diff_weight, inv_diff_weight, inv_gloss_weight;
   unsigned gloss_layer, have_layers;
-  int mirror_under;  // Mirror: underlying Lambert present
+  // Mirror: an underlying BSDF present: a Lambert of colour `color`, or
+  // (has_under) the Lambert, Cook-Torrance or Phong material at this
+  // entry's index + under_off
+  int mirror_under;
   int partial;       // lets shadow rays through, times its transmittance
-  // Stencil (material/stencil.cc:30-52): the underlying BSDF material at
-  // this entry's index + under_off of the table (has_under: there is
+  // Stencil (material/stencil.cc:30-52) and mirror: the underlying BSDF
+  // material at this entry's index + under_off of the table (has_under: there is
   // one), the opacity's intensity and its reciprocal (snapped to 0 / 1
   // below 0.001 / above 0.999); the opacity is in color.
   int under_off, has_under;

@@ -475,7 +475,11 @@ __device__ __forceinline__ BSample mirror_sample(const Isec &is, const DevMateri
         s.dir = v3(0.0f, 0.0f, 1.0f) * (is.v.z * 2.0f) - is.v;
         s.flags = F_SPECULAR | F_REFLECTIVE;
       } else if (M.mirror_under) {
-        s = lambert_sample(is, cload(M.color), u, v, flags);
+        const DevMaterial &U = (&M)[M.under_off];
+        if (!M.has_under) s = lambert_sample(is, cload(M.color), u, v, flags);
+        else if (U.type == 1) s = lambert_sample(is, cload(U.color), u, v, flags);
+        else if (U.type == 2) s = ct_sample(is, U, u, v, flags);
+        else s = phong_sample(is, U, u, v, flags);
         float f2 = fresnel(is.media_ior, M.ior_n, M.ior_k, s.dir.z);
         C3 rr = cload(M.gloss) * f2;
         s.val = s.val * C3{1.0f - rr.r, 1.0f - rr.g, 1.0f - rr.b};
@@ -497,9 +501,20 @@ __device__ __forceinline__ unsigned bsdf_supports(const DevMaterial &M, unsigned
                  ? ((F_REFLECTIVE | M.have_layers) & limit) : 0u;
     case 3:
     case 7: return (limit & F_SPECULAR) ? ((F_TRANSMISSIVE | F_REFLECTIVE | F_SPECULAR) & limit) : 0u;
-    case 4:
-      return (((limit & F_REFLECTIVE) && (limit & F_SPECULAR)) ? (F_REFLECTIVE | F_SPECULAR) : 0u) |
-             (M.mirror_under ? lam : 0u);
+    case 4: {
+      unsigned f = ((limit & F_REFLECTIVE) && (limit & F_SPECULAR)) ? (F_REFLECTIVE | F_SPECULAR) : 0u;
+      if (M.mirror_under) {
+        const DevMaterial &U = (&M)[M.under_off];
+        if (!M.has_under || U.type == 1) f |= lam;
+        else if (U.type == 2)
+          f |= ((limit & F_REFLECTIVE) && (limit & U.have_layers))
+                   ? ((F_REFLECTIVE | U.have_layers) & limit) : 0u;
+        else
+          f |= ((limit & F_REFLECTIVE) && (limit & (F_DIFFUSE | F_GLOSSY)))
+                   ? ((F_REFLECTIVE | F_DIFFUSE | F_GLOSSY) & limit) : 0u;
+      }
+      return f;
+    }
     case 6:
       return ((limit & F_REFLECTIVE) && (limit & (F_DIFFUSE | F_GLOSSY)))
                  ? ((F_REFLECTIVE | F_DIFFUSE | F_GLOSSY) & limit) : 0u;

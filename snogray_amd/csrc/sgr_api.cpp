@@ -371,6 +371,26 @@ int prepare_scene(const sgr_scene_desc *d, HostScene &H) {
   // material (through glows), the opacity's intensity snapped to 0 / 1
   // below 0.001 / above 0.999; fully opaque (component sum >= 3), its
   // get_bsdf is the underlying material's, which may have none.
+  // Mirror over a material (material/mirror.h:37-42): its underlying
+  // BSDF material (through glows; one that has none leaves the mirror
+  // without), a Lambert, Cook-Torrance or Phong; underlying -1 keeps the
+  // Lambert of `color` (material/mirror.cc:22-26).
+  for (uint32_t i = 0; i < nm; i++) {
+    DevMaterial &o = mats[i];
+    const int u = d->materials[i].underlying;
+    if (o.type != SGR_MAT_MIRROR || u < 0) continue;
+    if (u >= (int)nm || u == (int)i)
+      return fail(SGR_ERR_INVALID, "material " + std::to_string(i) + ": mirror: bad underlying");
+    const int ub = mats[u].bsdf;
+    if (ub >= 0 && mats[ub].type != SGR_MAT_LAMBERT && mats[ub].type != SGR_MAT_COOK_TORRANCE &&
+        mats[ub].type != SGR_MAT_PHONG)
+      return fail(SGR_ERR_INVALID, "material " + std::to_string(i) +
+                                       ": mirror: the underlying material must be a lambert, "
+                                       "cook-torrance or phong material");
+    o.mirror_under = ub >= 0 ? 1 : 0;
+    o.has_under = ub >= 0 ? 1 : 0;
+    o.under_off = ub >= 0 ? ub - (int)i : 0;
+  }
   for (uint32_t i = 0; i < nm; i++) {
     DevMaterial &o = mats[i];
     if (o.type != SGR_MAT_STENCIL) continue;

@@ -66,9 +66,11 @@ class SceneDesc:
     def glass(self, ior):
         return self._mat(type=abi.SGR_MAT_GLASS, ior_n=ior)
 
-    def mirror(self, reflectance, ior=1.5, ior_k=0.0, color=(0, 0, 0)):
+    def mirror(self, reflectance, ior=1.5, ior_k=0.0, color=(0, 0, 0), underlying=-1):
+        """Mirror (material/mirror.h): over a Lambert of COLOR, or over the
+        material UNDERLYING (a Lambert, Cook-Torrance or Phong)."""
         return self._mat(type=abi.SGR_MAT_MIRROR, gloss_color=reflectance,
-                         ior_n=ior, ior_k=ior_k, color=color)
+                         ior_n=ior, ior_k=ior_k, color=color, underlying=underlying)
 
     def phong(self, color, specular_color, exponent):
         return self._mat(type=abi.SGR_MAT_PHONG, color=color,

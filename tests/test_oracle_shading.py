@@ -435,3 +435,49 @@ def test_furnace_closed_form_on_oracle():
                                                     seed=11), samples=True)
         x = smp[:, 2].astype(float)
         assert abs(x.mean() - expect) < 4 * x.std() / math.sqrt(len(x)) + 1e-4
+
+
+def _mirror_floor_scene(floor):
+    """A floor of material FLOOR(sd), a Cook-Torrance ball and a quad
+    light, under a small sky."""
+    sd = scenes.SceneDesc()
+    sd.tripar((-20, 0, -20), (40, 0, 0), (0, 0, 40), floor(sd), parallelogram=True)
+    sd.sphere((0.0, 0.8, 0.0), 0.8, sd.cook_torrance((0.5, 0.3, 0.2), (0.4, 0.4, 0.4), 0.3, 1.6))
+    sd.tripar((-0.5, 3.0, -0.5), (0, 0, 1.0), (1.0, 0, 0), sd.glow((10.0, 9.0, 8.0)),
+              parallelogram=True)
+    sd.envmap(np.full((16, 32, 3), 0.3, np.float32))
+    return sd, scenes.look_at((0, 2.0, -5.0), (0, 0.3, 0), vfov_deg=50.0)
+
+
+@pytest.mark.parametrize("under", ["lambert", "cook_torrance", "phong"])
+def test_mirror_over_a_material(under):
+    """Mirror (ior, reflectance, material) (material/mirror.cc:54-85):
+    with reflectance 0 the specular term vanishes and every sample is the
+    underlying BSDF's (times 1 - 0 F), so with direct lighting off (then
+    emission counts at every vertex) each path equals the path over the
+    bare material, bit for bit; with reflectance 0.7 over flat geometry
+    the specular branch always wins, so each path equals the mirror's
+    without an underlying material."""
+    mk = {"lambert": lambda sd: sd.lambert((0.6, 0.5, 0.4)),
+          "cook_torrance": lambda sd: sd.cook_torrance((0.4, 0.3, 0.3), (0.5, 0.5, 0.5), 0.2, 1.7),
+          "phong": lambda sd: sd.phong((0.4, 0.3, 0.3), (0.4, 0.4, 0.4), 30.0)}[under]
+    p = abi.make_params(16, 12, 4, max_path_len=6, direct_samples=0, seed=3)
+
+    def samples(floor, params):
+        sd, cam = _mirror_floor_scene(floor)
+        return oracle.OracleScene(sd).render(cam, params, samples=True)[1]
+    bare = samples(mk, p)
+    over = samples(lambda sd: sd.mirror((0.0, 0.0, 0.0), underlying=mk(sd)), p)
+    np.testing.assert_array_equal(over, bare)
+    q = abi.make_params(16, 12, 4, max_path_len=6, seed=3)
+    plain = samples(lambda sd: sd.mirror((0.7, 0.7, 0.7)), q)
+    over = samples(lambda sd: sd.mirror((0.7, 0.7, 0.7), underlying=mk(sd)), q)
+    np.testing.assert_array_equal(over, plain)
+
+
+def test_mirror_underlying_refusals():
+    sd = scenes.SceneDesc()
+    g = sd.glass(1.5)
+    sd.tripar((0, 0, 0), (1, 0, 0), (0, 0, 1), sd.mirror((0.5, 0.5, 0.5), underlying=g))
+    with pytest.raises(RuntimeError, match="lambert, cook-torrance or phong"):
+        oracle.OracleScene(sd)
